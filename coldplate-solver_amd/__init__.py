@@ -1,0 +1,16 @@
+"""coldplate-solver MI355X backend: the SIMPLE hot path (U/V momentum Jacobi
+with SOU deferred correction and Brinkman / 2.5-D sinks, pressure-correction
+solve, velocity / pressure correction) and the 3-D conjugate-heat-transfer
+solve, as hand-written HIP kernels for gfx950 behind the C ABI declared in
+include/coldplate_hip.h (libcoldplate_hip.so, built in-tree).
+
+The shared library is loaded on first use; every entry point raises if it is
+missing or no GPU is present -- there is no CPU fallback.
+"""
+from . import _types, configs  # noqa: F401
+from ._types import default_config, SimpleConfig, StepStats, Patch, field_shape  # noqa: F401
+from . import _capi  # noqa: F401  (loads nothing until first use)
+from .simple import SIMPLE, pressure_solve_csr, momentum_jacobi_csr  # noqa: F401
+
+__all__ = ["_types", "configs", "default_config", "SimpleConfig", "StepStats", "Patch", "field_shape",
+           "SIMPLE", "pressure_solve_csr", "momentum_jacobi_csr"]

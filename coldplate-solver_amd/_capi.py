@@ -1,0 +1,122 @@
+"""ctypes binding of libcoldplate_hip.so (declarations: include/coldplate_hip.h).
+
+The library is built in-tree (csrc/Makefile) and loaded on first use.  There
+is no fallback: a missing library or a host without a HIP device raises.
+"""
+import ctypes as C
+import os
+import re
+import threading
+
+from . import _types as T
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcoldplate_hip.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "coldplate_hip.h")
+
+_lib = None
+_lock = threading.Lock()
+_ctx = {}
+
+
+class CpsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"coldplate_hip error {code}: {msg}")
+        self.code = code
+
+
+def declared_symbols(header=HEADER):
+    """Function names declared in the public header."""
+    txt = open(header).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(cps_[a-z0-9_]+)\s*\(", txt)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C coldplate-solver_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        ip, fp, dp, vp = P(C.c_int), P(C.c_float), P(C.c_double), C.c_void_p
+        L.cps_last_error.restype = C.c_char_p
+        L.cps_last_error.argtypes = []
+        L.cps_abi_version.restype = C.c_int
+        L.cps_create.argtypes = [C.c_int, P(vp)]
+        L.cps_destroy.argtypes = [vp]
+        L.cps_destroy.restype = None
+        L.cps_synchronize.argtypes = [vp]
+        for f in (L.cps_pressure_solve_csr, L.cps_momentum_jacobi_csr):
+            f.argtypes = [vp, C.c_int, ip, ip, fp, fp, fp, C.c_float, C.c_int, ip]
+        L.cps_simple_config_default.argtypes = [P(T.SimpleConfig)]
+        L.cps_simple_config_default.restype = None
+        L.cps_simple_create.argtypes = [vp, P(T.SimpleConfig), dp, dp, P(vp)]
+        L.cps_simple_destroy.argtypes = [vp]
+        L.cps_simple_destroy.restype = None
+        L.cps_simple_step.argtypes = [vp, C.c_int, P(T.StepStats)]
+        L.cps_simple_run.argtypes = [vp, P(T.StepStats)]
+        L.cps_simple_get_field.argtypes = [vp, C.c_int, dp]
+        L.cps_simple_set_field.argtypes = [vp, C.c_int, dp]
+        L.cps_simple_boundary_patch.argtypes = [vp, C.c_int, P(T.Patch)]
+        _lib = L
+        return L
+
+
+def check(status):
+    if status != T.CPS_OK:
+        msg = lib().cps_last_error().decode(errors="replace")
+        raise CpsError(status, msg)
+
+
+class Context:
+    """One device + one HIP stream (cps_create / cps_destroy)."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        h = C.c_void_p()
+        check(self.L.cps_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def synchronize(self):
+        check(self.L.cps_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cps_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def context(device=0):
+    """Process-wide default context of a device."""
+    with _lock:
+        c = _ctx.get(device)
+    if c is None:
+        c = Context(device)
+        with _lock:
+            _ctx[device] = c
+    return c
+
+
+def fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))

@@ -1,0 +1,166 @@
+// cps_common.hpp -- shared device/host helpers for the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "../../include/coldplate_hip.h"
+
+namespace cps {
+
+// ---------------------------------------------------------------------------
+// error plumbing: a failed call records its message (thread-local) and the
+// C entry point returns the status.
+void set_error(const std::string& msg);
+void clear_error();
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define CPS_HIP(expr)                                                                           \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      throw ::cps::Error{CPS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)};        \
+  } while (0)
+
+#define CPS_CHECK(cond, code, msg)                  \
+  do {                                              \
+    if (!(cond)) throw ::cps::Error{(code), (msg)}; \
+  } while (0)
+
+// Runs fn() and converts exceptions into a status code.
+template <class F>
+int guard(F&& fn) {
+  try {
+    clear_error();
+    fn();
+    return CPS_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return CPS_ERR_STATE;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// device buffer (RAII)
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    CPS_HIP(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// ---------------------------------------------------------------------------
+// 64-wide wavefront + block reductions (deterministic order)
+constexpr int kWave = 64;
+
+template <class T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_down(v, o, kWave);
+  return v;
+}
+template <class T>
+__device__ inline T wave_max(T v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    T w = __shfl_down(v, o, kWave);
+    v = v > w ? v : w;
+  }
+  return v;
+}
+
+// Sum over the block (NT threads, flattened); result valid in thread 0.
+template <int NT, class T>
+__device__ inline T block_sum(T v, T* sh) {
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+  v = wave_sum(v);
+  if ((t & (kWave - 1)) == 0) sh[t / kWave] = v;
+  __syncthreads();
+  T r = T(0);
+  if (t == 0) {
+#pragma unroll
+    for (int k = 0; k < NT / kWave; ++k) r += sh[k];
+  }
+  __syncthreads();
+  return r;
+}
+template <int NT, class T>
+__device__ inline T block_max(T v, T* sh) {
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+  v = wave_max(v);
+  if ((t & (kWave - 1)) == 0) sh[t / kWave] = v;
+  __syncthreads();
+  T r = T(0);
+  if (t == 0) {
+#pragma unroll
+    for (int k = 0; k < NT / kWave; ++k) r = r > sh[k] ? r : sh[k];
+  }
+  __syncthreads();
+  return r;
+}
+
+// Fixed-order sum / max of n block partials written by an earlier kernel on
+// the same stream (the kernel boundary makes them visible; no in-kernel
+// fences).  Every thread of the block gets the result.  sh needs
+// NT / kWave + 1 entries.
+template <int NT>
+__device__ inline double parts_sum(const double* p, int n, double* sh) {
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+  double a = 0.0;
+  for (int q = t; q < n; q += NT) a += p[q];
+  a = block_sum<NT>(a, sh);
+  if (t == 0) sh[NT / kWave] = a;
+  __syncthreads();
+  a = sh[NT / kWave];
+  __syncthreads();
+  return a;
+}
+template <int NT>
+__device__ inline double parts_max(const double* p, int n, double* sh) {
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+  double a = 0.0;
+  for (int q = t; q < n; q += NT) a = fmax(a, p[q]);
+  a = block_max<NT>(a, sh);
+  if (t == 0) sh[NT / kWave] = a;
+  __syncthreads();
+  a = sh[NT / kWave];
+  __syncthreads();
+  return a;
+}
+// Reads one int of device state once per block (uniform within the block).
+__device__ inline int block_read(const volatile int* p, int* sh) {
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+  if (t == 0) *sh = *p;
+  __syncthreads();
+  const int v = *sh;
+  __syncthreads();
+  return v;
+}
+
+inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+}  // namespace cps

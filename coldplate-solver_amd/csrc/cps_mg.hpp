@@ -1,0 +1,32 @@
+// cps_mg.hpp -- geometric-multigrid preconditioned CG for the SIMPLE
+// pressure-correction system (replaces the AMGCL solve of
+// pressure_solver.cpp:747-766 / pressure_solver.cu:81-163).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "cps_simple.hpp"
+
+namespace cps {
+
+class PressureMG {
+ public:
+  PressureMG(Ctx* ctx, const Geo& g);
+  ~PressureMG();
+  // Solves A x = b for the pressure correction on the fine grid (field
+  // layout, FP64).  r holds b on entry and the final FP64 residual on exit
+  // (mixed) ; x is zero on entry.  FP64: MG-PCG in double.  Mixed: FP64
+  // iterative refinement around an FP32 MG-PCG.
+  void solve(const double* dE, const double* dN, double* r, double* x, IterState* st, const cps_simple_config& cfg,
+             bool fp64);
+
+ private:
+  Ctx* ctx_;
+  Geo g_;
+  void* impl64_ = nullptr;  // MGImpl<double>
+  void* impl32_ = nullptr;  // MGImpl<float>
+  void* b64_ = nullptr;     // FP64 rhs kept across refinement sweeps
+};
+
+}  // namespace cps

@@ -1,0 +1,1071 @@
+// cps_simple.hip -- the SIMPLE iteration on gfx950, device resident.
+//
+// One SIMPLE step (SIMPLE::calculateStep, iterations.cpp:82-659) is a fixed
+// chain of kernels on the context stream:
+//
+//   k_bc            velocity + pressure BCs          boundaries.cpp:50-146
+//   k_assemble      U and V momentum coefficients    momentum_solver.cpp:195-314
+//                   (+ SOU, half-cell flags, Brinkman / 2.5-D sinks, dE, dN)
+//   k_jacobi  x K   fused U+V point-Jacobi sweep      jacobi_solver.cpp:44-67
+//                   with per-component sum r^2 test on the device
+//   k_finalize      clamp + scatter to uStar/vStar,  momentum_solver.cpp:335-351
+//                   BCs on uStar/vStar, RMS change   iterations.cpp:499-533
+//   k_prhs          mass residual, pressure RHS,     pressure_solver.cpp:470-496,
+//                   PCG start                        :613-623, :121-150
+//   pressure solve  Jacobi-PCG (k_pcg_a/k_pcg_b) or geometric-MG PCG
+//   k_correct       p += pAlpha p', u/v correction   iterations.cpp:556-600
+//                   with clamps, BCs
+//
+// Fields live in HBM as row-major (i = y row, j = x column) arrays with one
+// common pitch P >= N+1 (a multiple of 64 elements), so every kernel
+// reads whole 64-lane rows: one wavefront = 64 consecutive j.  Reductions are
+// wave-shuffle + block partials + a last-block finalisation in fixed order,
+// so results are bit-reproducible run to run.
+//
+// Coefficient storage C is double (CPS_PREC_FP64) or float
+// (CPS_PREC_MIXED).  In mixed mode the FP64 assembly writes the FP32
+// correction system A d = b - A u; FP32 Jacobi sweeps solve for d and the
+// FP64 state gets u* = u + d, so the fixed point is the FP64 one.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "cps_mg.hpp"
+#include "cps_simple.hpp"
+
+namespace cps {
+
+// float literals of the reference (SIMPLE.h / *.cpp), widened exactly
+constexpr double kTenthF = (double)0.1f;
+constexpr double kHalfCellTol = (double)0.999f;
+constexpr double kVelTol = (double)1e-8f;
+constexpr double kAP0Floor = (double)1e-10f;
+constexpr double kAlphaEps = (double)1e-10f;
+constexpr double kTiny20 = (double)1e-20f;
+constexpr double kKminFloor = (double)1e-20f;
+
+constexpr int BX = 64, BY = 4, NT = BX * BY;
+
+struct Fields {
+  double *u, *v, *p, *us, *vs, *ps, *dE, *dN, *gam, *alp, *inlet;
+};
+
+template <class C>
+struct MomSys {  // coefficients of one component, indexed like the field
+  C *cS, *cW, *aP, *cE, *cN, *b;
+  C *x0, *x1;  // Jacobi double buffer (x-form: velocity, delta-form: correction)
+};
+
+struct PcgBufs {
+  double *r, *z, *pa, *pb, *Ap, *diag;
+};
+
+__device__ __forceinline__ double clampv(double x, double m) { return fmin(fmax(x, -m), m); }
+
+__device__ __forceinline__ size_t at(const Geo& g, int i, int j) { return (size_t)i * g.P + j; }
+
+// ---------------------------------------------------------------------------
+// Boundary conditions (boundaries.cpp:50-146).  Segment 0 walks j = 0..N
+// (walls, p top/bottom ghosts); segment 1 walks i = 0..M (inlet, outlet,
+// p left/right ghosts).  Every element has exactly one writer.
+__global__ void k_bc(Geo g, double* u, double* v, double* p, const double* inlet, int do_p) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int M = g.M, N = g.N;
+  if (blockIdx.y == 0) {
+    const int j = t;
+    if (j > N) return;
+    if (j >= 1 && j <= N - 2) {
+      u[at(g, 0, j)] = 0.0;
+      u[at(g, M, j)] = 0.0;
+    }
+    v[at(g, 0, j)] = 0.0;
+    v[at(g, M - 1, j)] = 0.0;
+    if (do_p && j >= 1 && j <= N - 1) {
+      p[at(g, 0, j)] = p[at(g, 1, j)];
+      p[at(g, M, j)] = p[at(g, M - 1, j)];
+    }
+  } else {
+    const int i = t;
+    if (i > M) return;
+    u[at(g, i, 0)] = inlet[i];
+    u[at(g, i, N - 1)] = (i == 0 || i == M) ? 0.0 : u[at(g, i, N - 2)];
+    if (i >= 1 && i <= M - 2) {
+      v[at(g, i, 0)] = 0.0;
+      v[at(g, i, N)] = v[at(g, i, N - 1)];
+    }
+    if (do_p && i >= 1 && i <= M - 1) {
+      p[at(g, i, 0)] = p[at(g, i, 1)];
+      p[at(g, i, N)] = 0.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Momentum assembly (momentum_solver.cpp:195-314) for one U or V DOF.
+template <bool IsU>
+struct MomCoef {
+  double aS, aW, aP, aE, aN;  // positive link coefficients, aP = aP0/uvAlpha
+  double b;                   // source with non-DOF neighbours folded in
+  double aP0;
+  bool dS, dW, dE, dN;        // neighbour is a DOF of the same system
+};
+
+template <bool IsU>
+__device__ inline MomCoef<IsU> mom_coef(const Geo& g, const Fields& f, int i, int j) {
+  const int M = g.M, N = g.N;
+  auto U = [&](int a, int b) { return f.u[at(g, a, b)]; };
+  auto V = [&](int a, int b) { return f.v[at(g, a, b)]; };
+  auto A = [&](int a, int b) { return f.alp[at(g, a, b)]; };
+  auto G = [&](int a, int b) { return f.gam[at(g, a, b)]; };
+  const double mv = g.maxVel;
+  double alphaLocal, ue, uw, vn, vs;
+  if (IsU) {
+    alphaLocal = 0.5 * (A(i - 1, j - 1) + A(i - 1, j));
+    ue = clampv(0.5 * (U(i, j) + U(i, j + 1)), mv);
+    uw = clampv(0.5 * (U(i, j - 1) + U(i, j)), mv);
+    vn = clampv(0.5 * (V(i, j) + V(i, j + 1)), mv);
+    vs = clampv(0.5 * (V(i - 1, j) + V(i - 1, j + 1)), mv);
+  } else {
+    alphaLocal = 0.5 * (A(i - 1, j - 1) + A(i, j - 1));
+    ue = clampv(0.5 * (U(i, j) + U(i + 1, j)), mv);
+    uw = clampv(0.5 * (U(i, j - 1) + U(i + 1, j - 1)), mv);
+    vn = clampv(0.5 * (V(i, j) + V(i + 1, j)), mv);
+    vs = clampv(0.5 * (V(i - 1, j) + V(i, j)), mv);
+  }
+  const double Fe = g.convScale * g.rho * g.hy * ue, Fw = g.convScale * g.rho * g.hy * uw;
+  const double Fn = g.convScale * g.rho * g.hx * vn, Fs = g.convScale * g.rho * g.hx * vs;
+  const double De = g.De, Dn = g.Dn;
+  double aE = De + fmax(0.0, -Fe);
+  double aW = De + fmax(0.0, Fw);
+  double aN = Dn + fmax(0.0, -Fn);
+  double aS = Dn + fmax(0.0, Fs);
+  // half-cell wall diffusion (boundaries.cpp:148-206)
+  if (IsU) {
+    if (i == 1 && 0.5 * (G(0, j - 1) + G(0, j)) >= kHalfCellTol && fabs(U(0, j)) <= kVelTol &&
+        fabs(V(0, j)) <= kVelTol && fabs(V(0, j + 1)) <= kVelTol)
+      aS += Dn;
+    if (i == M - 1 && 0.5 * (G(M - 2, j - 1) + G(M - 2, j)) >= kHalfCellTol && fabs(U(M, j)) <= kVelTol &&
+        fabs(V(M - 1, j)) <= kVelTol && fabs(V(M - 1, j + 1)) <= kVelTol)
+      aN += Dn;
+  } else {
+    if (j == 1 && 0.5 * (G(i - 1, 0) + G(i, 0)) >= kHalfCellTol && !(fabs(g.targetVel) > kVelTol) &&
+        fabs(V(i, 0)) <= kVelTol && fabs(U(i, 0)) <= kVelTol && fabs(U(i + 1, 0)) <= kVelTol)
+      aW += De;
+    if (j == N - 1 && 0.5 * (G(i - 1, N - 2) + G(i, N - 2)) >= kHalfCellTol &&
+        !(fabs(f.p[at(g, i, N)]) <= kVelTol) && fabs(V(i, N)) <= kVelTol && fabs(U(i, N - 1)) <= kVelTol &&
+        fabs(U(i + 1, N - 1)) <= kVelTol)
+      aE += De;
+  }
+  const double sumA = aE + aW + aN + aS;
+  double aP0 = sumA + (Fe - Fw + Fn - Fs) + 0.0 + g.sinkDiag + alphaLocal * g.vol;
+  aP0 = fmax(aP0, kAP0Floor);
+  MomCoef<IsU> c;
+  c.aP0 = aP0;
+  c.aP = aP0 / g.uvAlpha;
+  double Sp, velCur;
+  if (IsU) {
+    Sp = (f.p[at(g, i, j)] - f.p[at(g, i, j + 1)]) * g.hy;
+    velCur = U(i, j);
+  } else {
+    Sp = (f.p[at(g, i, j)] - f.p[at(g, i + 1, j)]) * g.hx;
+    velCur = V(i, j);
+  }
+  Sp += (1.0 - g.uvAlpha) / g.uvAlpha * aP0 * velCur;
+  if (g.sou) {
+    // convection.cpp:53-186
+    double Sdc = 0.0;
+    if (IsU) {
+      if (Fe >= 0.0) { if (j >= 2) Sdc -= Fe * ((1.5 * U(i, j) - 0.5 * U(i, j - 1)) - U(i, j)); }
+      else { if (j + 2 < N) Sdc -= Fe * ((1.5 * U(i, j + 1) - 0.5 * U(i, j + 2)) - U(i, j + 1)); }
+      if (Fw >= 0.0) { if (j >= 2) Sdc += Fw * ((1.5 * U(i, j - 1) - 0.5 * U(i, j - 2)) - U(i, j - 1)); }
+      else { if (j + 1 < N) Sdc += Fw * ((1.5 * U(i, j) - 0.5 * U(i, j + 1)) - U(i, j)); }
+      if (Fn >= 0.0) { if (i >= 1) Sdc -= Fn * ((1.5 * U(i, j) - 0.5 * U(i - 1, j)) - U(i, j)); }
+      else { if (i + 2 <= M) Sdc -= Fn * ((1.5 * U(i + 1, j) - 0.5 * U(i + 2, j)) - U(i + 1, j)); }
+      if (Fs >= 0.0) { if (i >= 2) Sdc += Fs * ((1.5 * U(i - 1, j) - 0.5 * U(i - 2, j)) - U(i - 1, j)); }
+      else { if (i + 1 <= M) Sdc += Fs * ((1.5 * U(i, j) - 0.5 * U(i + 1, j)) - U(i, j)); }
+    } else {
+      if (Fe >= 0.0) { if (j >= 2) Sdc -= Fe * ((1.5 * V(i, j) - 0.5 * V(i, j - 1)) - V(i, j)); }
+      else { if (j + 2 <= N) Sdc -= Fe * ((1.5 * V(i, j + 1) - 0.5 * V(i, j + 2)) - V(i, j + 1)); }
+      if (Fw >= 0.0) { if (j >= 2) Sdc += Fw * ((1.5 * V(i, j - 1) - 0.5 * V(i, j - 2)) - V(i, j - 1)); }
+      else { if (j + 1 <= N) Sdc += Fw * ((1.5 * V(i, j) - 0.5 * V(i, j + 1)) - V(i, j)); }
+      if (Fn >= 0.0) { if (i >= 1) Sdc -= Fn * ((1.5 * V(i, j) - 0.5 * V(i - 1, j)) - V(i, j)); }
+      else { if (i + 2 < M) Sdc -= Fn * ((1.5 * V(i + 1, j) - 0.5 * V(i + 2, j)) - V(i + 1, j)); }
+      if (Fs >= 0.0) { if (i >= 2) Sdc += Fs * ((1.5 * V(i - 1, j) - 0.5 * V(i - 2, j)) - V(i - 1, j)); }
+      else { if (i + 1 < M) Sdc += Fs * ((1.5 * V(i, j) - 0.5 * V(i + 1, j)) - V(i, j)); }
+    }
+    Sp += Sdc;
+  }
+  // DOF neighbours (momentum_solver.cpp:98-111) and folding of the others
+  // into the source, in the reference order E, W, N, S (:299-303)
+  if (IsU) {
+    c.dE = j + 1 <= N - 2; c.dW = j - 1 >= 1; c.dN = i + 1 <= M - 1; c.dS = i - 1 >= 1;
+    if (!c.dE) Sp += aE * U(i, j + 1);
+    if (!c.dW) Sp += aW * U(i, j - 1);
+    if (!c.dN) Sp += aN * U(i + 1, j);
+    if (!c.dS) Sp += aS * U(i - 1, j);
+  } else {
+    c.dE = j + 1 <= N - 1; c.dW = j - 1 >= 1; c.dN = i + 1 <= M - 2; c.dS = i - 1 >= 1;
+    if (!c.dE) Sp += aE * V(i, j + 1);
+    if (!c.dW) Sp += aW * V(i, j - 1);
+    if (!c.dN) Sp += aN * V(i + 1, j);
+    if (!c.dS) Sp += aS * V(i - 1, j);
+  }
+  c.aE = aE; c.aW = aW; c.aN = aN; c.aS = aS;
+  c.b = Sp;
+  return c;
+}
+
+// Writes the CSR-equivalent row: c_nb = -a_nb for DOF neighbours, 0 for
+// folded ones (their value already sits in b).  Delta form: b - A x0 with
+// x0 = current velocity, evaluated in FP64 in the reference sum order.
+template <bool IsU, class C, bool Delta>
+__device__ inline void store_mom(const Geo& g, const MomCoef<IsU>& c, const double* x, MomSys<C> s, int i, int j) {
+  const size_t k = at(g, i, j);
+  const double cS = c.dS ? -c.aS : 0.0, cW = c.dW ? -c.aW : 0.0;
+  const double cE = c.dE ? -c.aE : 0.0, cN = c.dN ? -c.aN : 0.0;
+  s.cS[k] = (C)cS; s.cW[k] = (C)cW; s.aP[k] = (C)c.aP; s.cE[k] = (C)cE; s.cN[k] = (C)cN;
+  if (Delta) {
+    double Ax = 0.0;
+    if (c.dS) Ax += cS * x[k - g.P];
+    if (c.dW) Ax += cW * x[k - 1];
+    Ax += c.aP * x[k];
+    if (c.dE) Ax += cE * x[k + 1];
+    if (c.dN) Ax += cN * x[k + g.P];
+    s.b[k] = (C)(c.b - Ax);
+    s.x0[k] = (C)0;
+  } else {
+    s.b[k] = (C)c.b;
+    s.x0[k] = (C)x[k];
+  }
+}
+
+template <class C, bool Delta>
+__global__ __launch_bounds__(NT) void k_assemble(Geo g, Fields f, MomSys<C> su, MomSys<C> sv) {
+  const int j = blockIdx.x * BX + threadIdx.x;
+  const int M = g.M, N = g.N;
+  for (int i = blockIdx.y * BY + threadIdx.y; i <= M; i += g.gy * BY) {
+    if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) {
+      MomCoef<true> c = mom_coef<true>(g, f, i, j);
+      f.dE[at(g, i, j)] = g.hy / c.aP0;
+      store_mom<true, C, Delta>(g, c, f.u, su, i, j);
+    }
+    if (i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) {
+      MomCoef<false> c = mom_coef<false>(g, f, i, j);
+      f.dN[at(g, i, j)] = g.hx / c.aP0;
+      store_mom<false, C, Delta>(g, c, f.v, sv, i, j);
+    }
+  }
+}
+
+__global__ void k_jac_reset(IterState* st, int au, int av) {
+  if (threadIdx.x == 0) {
+    st->jac_active[0] = au;
+    st->jac_active[1] = av;
+    st->jac_iters[0] = 0;
+    st->jac_iters[1] = 0;
+  }
+}
+
+// Row loop of the capped field grid: thread (tx, ty) of block (bx, by)
+// visits rows by*BY + ty + k*gy*BY.
+#define FIELD_LOOP(g, i, j)                                   \
+  const int j = blockIdx.x * BX + threadIdx.x;                \
+  for (int i = blockIdx.y * BY + threadIdx.y; i <= (g).M; i += (g).gy * BY)
+
+__device__ inline int block_id() { return blockIdx.x + blockIdx.y * gridDim.x; }
+__device__ inline int thread_id() { return threadIdx.x + threadIdx.y * BX; }
+
+// Jacobi stop rule of jacobi_solver.cpp:63 applied to the partials of sweep
+// k-1 (slot parity (k-1)&1): returns the updated active flags and records
+// the sweep count of a component that just converged.
+__device__ inline void jac_detect(const Geo& g, int k, double tol2, IterState* st, const double* part, int& au,
+                                  int& av, double* sh) {
+  if (k <= 0) return;
+  const double* pin = part + (size_t)(SLOT_JAC + 2 * ((k - 1) & 1)) * g.NB;
+  if (au) {
+    const double a = parts_sum<NT>(pin, g.NB, sh);
+    if (a < tol2 * (double)g.nU) {
+      au = 0;
+      if (block_id() == 0 && thread_id() == 0) { st->jac_active[0] = 0; st->jac_iters[0] = k; }
+    }
+  }
+  if (av) {
+    const double b = parts_sum<NT>(pin + g.NB, g.NB, sh);
+    if (b < tol2 * (double)g.nV) {
+      av = 0;
+      if (block_id() == 0 && thread_id() == 0) { st->jac_active[1] = 0; st->jac_iters[1] = k; }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused U+V Jacobi sweep k (jacobi_solver.cpp:44-67): r = b - A x (sorted
+// column order S, W, P, E, N), x' = x + r / aP, sum r^2 per component into
+// the parity-k slots.  The stop test of sweep k-1 runs first.
+template <class C>
+__global__ __launch_bounds__(NT) void k_jacobi(Geo g, MomSys<C> su, MomSys<C> sv, int sweep, double tol2,
+                                               IterState* st, double* part) {
+  __shared__ double sh[NT / kWave + 1];
+  __shared__ int shi;
+  int au = block_read(&st->jac_active[0], &shi), av = block_read(&st->jac_active[1], &shi);
+  jac_detect(g, sweep, tol2, st, part, au, av, sh);
+  if (!au && !av) return;  // uniform over the grid
+  const int M = g.M, N = g.N, P = g.P;
+  double ru = 0.0, rv = 0.0;
+  auto sweep1 = [&](const MomSys<C>& s, size_t k) -> double {
+    const C* x = (sweep & 1) ? s.x1 : s.x0;
+    C* y = (sweep & 1) ? s.x0 : s.x1;
+    C Ax = (C)0;
+    Ax += s.cS[k] * x[k - P];
+    Ax += s.cW[k] * x[k - 1];
+    Ax += s.aP[k] * x[k];
+    Ax += s.cE[k] * x[k + 1];
+    Ax += s.cN[k] * x[k + P];
+    const C r = s.b[k] - Ax;
+    y[k] = x[k] + r / s.aP[k];
+    return (double)r * (double)r;
+  };
+  FIELD_LOOP(g, i, j) {
+    const size_t k = at(g, i, j);
+    if (au && i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) ru += sweep1(su, k);
+    if (av && i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) rv += sweep1(sv, k);
+  }
+  const double tu = block_sum<NT>(ru, sh);
+  const double tv = block_sum<NT>(rv, sh);
+  if (thread_id() == 0) {
+    double* pout = part + (size_t)(SLOT_JAC + 2 * (sweep & 1)) * g.NB;
+    pout[block_id()] = tu;
+    pout[g.NB + block_id()] = tv;
+  }
+}
+
+// After K launched sweeps: stop test of sweep K-1 and sweep counts of the
+// components still iterating (one block).
+__global__ __launch_bounds__(NT) void k_jac_end(Geo g, int K, double tol2, IterState* st, const double* part) {
+  __shared__ double sh[NT / kWave + 1];
+  __shared__ int shi;
+  int au = block_read(&st->jac_active[0], &shi), av = block_read(&st->jac_active[1], &shi);
+  jac_detect(g, K, tol2, st, const_cast<double*>(part), au, av, sh);
+  if (thread_id() == 0) {
+    if (au) st->jac_iters[0] = K;
+    if (av) st->jac_iters[1] = K;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// uStar / vStar = clamp(solution) (momentum_solver.cpp:335-351), their BCs
+// (iterations.cpp:169) and the change sums of iterations.cpp:499-533.
+template <class C, bool Delta>
+__global__ __launch_bounds__(NT) void k_finalize(Geo g, Fields f, MomSys<C> su, MomSys<C> sv, IterState* st,
+                                                 double* part) {
+  __shared__ double sh[NT / kWave + 1];
+  const int M = g.M, N = g.N;
+  const double mv = g.maxVel;
+  double su2 = 0.0, sv2 = 0.0, umx = 0.0, vmx = 0.0;
+  const int itU = st->jac_iters[0], itV = st->jac_iters[1];
+  FIELD_LOOP(g, i, j) {
+    if (j > N) continue;
+    const size_t k = at(g, i, j);
+    if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) {
+      const C* x = (itU & 1) ? su.x1 : su.x0;
+      const double cur = f.u[k];
+      const double nv = clampv(Delta ? cur + (double)x[k] : (double)x[k], mv);
+      f.us[k] = nv;
+      const double d = nv - cur;
+      su2 += d * d;
+      umx = fmax(umx, fabs(d));
+      if (j == N - 2) f.us[k + 1] = nv;  // outlet copy u(i,N-1) = u(i,N-2)
+    }
+    if (i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) {
+      const C* x = (itV & 1) ? sv.x1 : sv.x0;
+      const double cur = f.v[k];
+      const double nv = clampv(Delta ? cur + (double)x[k] : (double)x[k], mv);
+      f.vs[k] = nv;
+      const double d = nv - cur;
+      sv2 += d * d;
+      vmx = fmax(vmx, fabs(d));
+      if (j == N - 1) f.vs[k + 1] = nv;  // outlet copy v(i,N) = v(i,N-1)
+    }
+    // walls / inlet of uStar, vStar (boundaries.cpp:59-106)
+    if ((i == 0 || i == M) && j >= 1 && j <= N - 1) f.us[k] = 0.0;
+    if (j == 0) f.us[k] = f.inlet[i];
+    if (i <= M - 1 && (i == 0 || i == M - 1)) f.vs[k] = 0.0;
+    if (i >= 1 && i <= M - 2 && j == 0) f.vs[k] = 0.0;
+  }
+  const double a = block_sum<NT>(su2, sh), b = block_sum<NT>(sv2, sh);
+  const double c = block_max<NT>(umx, sh), d = block_max<NT>(vmx, sh);
+  if (thread_id() == 0) {
+    double* p = part + (size_t)SLOT_FIN * g.NB + block_id();
+    p[0] = a;
+    p[g.NB] = b;
+    p[2 * g.NB] = c;
+    p[3 * g.NB] = d;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pressure operator pieces from dE, dN (pressure_solver.cpp:589-594).
+struct PCoef {
+  double aS, aW, aP, aE, aN;
+};
+__device__ inline PCoef p_coef(const Geo& g, const double* dE, const double* dN, int i, int j) {
+  const int M = g.M;
+  PCoef c;
+  c.aE = g.rho * dE[at(g, i, j)] * g.hy;
+  c.aW = (j > 1) ? g.rho * dE[at(g, i, j - 1)] * g.hy : 0.0;
+  c.aN = (i < M - 1) ? g.rho * dN[at(g, i, j)] * g.hx : 0.0;
+  c.aS = (i > 1) ? g.rho * dN[at(g, i - 1, j)] * g.hx : 0.0;
+  double aP = c.aE + c.aW + c.aN + c.aS;
+  if (aP < kTiny20) aP = 1.0;
+  c.aP = aP;
+  return c;
+}
+__device__ inline double mass_b(const Geo& g, const Fields& f, int i, int j) {
+  const double mE = g.rho * f.us[at(g, i, j)] * g.hy;
+  const double mW = g.rho * f.us[at(g, i, j - 1)] * g.hy;
+  const double mN = g.rho * f.vs[at(g, i, j)] * g.hx;
+  const double mS = g.rho * f.vs[at(g, i - 1, j)] * g.hx;
+  return mW - mE + mS - mN;
+}
+
+// Mass residual (j = 1..N-1) + RHS and PCG start (x0 = 0, r = b,
+// z = r/diag, p_old = 0) for the DOFs j = 1..N-2.  pStar is cleared.
+__global__ __launch_bounds__(NT) void k_prhs(Geo g, Fields f, PcgBufs q, double* part) {
+  __shared__ double sh[NT / kWave + 1];
+  const int M = g.M, N = g.N;
+  double mx = 0, ss = 0, rz = 0, bb = 0;
+  FIELD_LOOP(g, i, j) {
+    if (j > N) continue;
+    const size_t k = at(g, i, j);
+    f.ps[k] = 0.0;
+    if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 1) {
+      const double b = mass_b(g, f, i, j);
+      const double ab = fabs(b);
+      mx = fmax(mx, ab);
+      ss += ab * ab;
+      if (j <= N - 2) {
+        PCoef c = p_coef(g, f.dE, f.dN, i, j);
+        q.diag[k] = c.aP;
+        q.r[k] = b;
+        const double z = (fabs(c.aP) > kTiny20) ? b / c.aP : b;
+        q.z[k] = z;
+        q.pa[k] = 0.0;
+        q.pb[k] = 0.0;
+        rz += b * z;
+        bb += b * b;
+      }
+    }
+  }
+  const double A = block_max<NT>(mx, sh), B = block_sum<NT>(ss, sh);
+  const double Cc = block_sum<NT>(rz, sh), D = block_sum<NT>(bb, sh);
+  if (thread_id() == 0) {
+    double* p = part + (size_t)SLOT_RHS * g.NB + block_id();
+    p[0] = A;
+    p[g.NB] = B;
+    p[2 * g.NB] = Cc;
+    p[3 * g.NB] = D;
+  }
+}
+
+// One block: residuals of the step (iterations.cpp:499-533, 628-658,
+// pressure_solver.cpp:489-495) and the PCG start scalars.
+__global__ __launch_bounds__(NT) void k_stats(Geo g, IterState* st, const double* part) {
+  __shared__ double sh[NT / kWave + 1];
+  const int NB = g.NB;
+  const double* pf = part + (size_t)SLOT_FIN * NB;
+  const double* pr = part + (size_t)SLOT_RHS * NB;
+  const double su = parts_sum<NT>(pf, NB, sh), sv = parts_sum<NT>(pf + NB, NB, sh);
+  const double um = parts_max<NT>(pf + 2 * NB, NB, sh), vm = parts_max<NT>(pf + 3 * NB, NB, sh);
+  const double mm = parts_max<NT>(pr, NB, sh), ms = parts_sum<NT>(pr + NB, NB, sh);
+  const double rz = parts_sum<NT>(pr + 2 * NB, NB, sh), bb = parts_sum<NT>(pr + 3 * NB, NB, sh);
+  if (thread_id() == 0) {
+    st->su = su;
+    st->sv = sv;
+    st->umax = um;
+    st->vmax = vm;
+    st->resid_u = g.nU > 0 ? sqrt(su / (double)g.nU) : 0.0;
+    st->resid_v = g.nV > 0 ? sqrt(sv / (double)g.nV) : 0.0;
+    const double cnt = (double)(g.M - 1) * (double)(g.N - 1);
+    st->mass_max = mm;
+    st->mass_ss = ms;
+    st->resid_mass = cnt > 0 ? sqrt(ms / cnt) : 0.0;
+    st->error = fmax(st->resid_u, fmax(st->resid_v, st->resid_mass));
+    st->rz0 = rz;
+    st->bnorm2 = bb;
+    st->pcg_stop = 0;
+    st->pcg_iters = 0;
+    st->err = bb;
+  }
+}
+
+// PCG part A (pressure_solver.cpp:153-159, 168-181 of the previous loop
+// body): stop test on ||r|| of iteration it-1, beta = rz_new / rz_old,
+// p = z + beta p_old fused into Ap = A p (CSR sorted order S, W, P, E, N).
+__global__ __launch_bounds__(NT) void k_pcg_a(Geo g, const double* dE, const double* dN, PcgBufs q, int it,
+                                             double tol, IterState* st, double* part) {
+  __shared__ double sh[NT / kWave + 1];
+  __shared__ int shi;
+  if (block_read(&st->pcg_stop, &shi)) return;
+  double* pp = part + (size_t)SLOT_PCG * g.NB;
+  double rzc, beta = 0.0;
+  if (it == 0) {
+    rzc = st->rz0;
+  } else {
+    const double err = parts_sum<NT>(pp + g.NB, g.NB, sh);
+    if (sqrt(err) < tol) {  // pressure_solver.cpp:171
+      if (block_id() == 0 && thread_id() == 0) { st->pcg_stop = 1; st->pcg_iters = it - 1; st->err = err; }
+      return;
+    }
+    rzc = parts_sum<NT>(pp + 2 * g.NB, g.NB, sh);
+    beta = rzc / st->rzs[(it - 1) & 1];
+  }
+  if (block_id() == 0 && thread_id() == 0) { st->rzs[it & 1] = rzc; st->pcg_iters = it; }
+  const int M = g.M, N = g.N, P = g.P;
+  const double* po = (it & 1) ? q.pb : q.pa;
+  double* pn = (it & 1) ? q.pa : q.pb;
+  double pap = 0.0;
+  FIELD_LOOP(g, i, j) {
+    if (!(i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2)) continue;
+    const size_t k = at(g, i, j);
+    PCoef c = p_coef(g, dE, dN, i, j);
+    const double pc = q.z[k] + beta * po[k];
+    double Ap = 0.0;
+    if (i > 1) Ap += -c.aS * (q.z[k - P] + beta * po[k - P]);
+    if (j > 1) Ap += -c.aW * (q.z[k - 1] + beta * po[k - 1]);
+    Ap += c.aP * pc;
+    if (j + 1 <= N - 2) Ap += -c.aE * (q.z[k + 1] + beta * po[k + 1]);
+    if (i < M - 1) Ap += -c.aN * (q.z[k + P] + beta * po[k + P]);
+    pn[k] = pc;
+    q.Ap[k] = Ap;
+    pap += pc * Ap;
+  }
+  const double s = block_sum<NT>(pap, sh);
+  if (thread_id() == 0) pp[block_id()] = s;
+}
+
+// PCG part B (pressure_solver.cpp:158-170): |pAp| test, alpha = rz / pAp,
+// x += alpha p, r -= alpha Ap, z = r / diag, partial ||r||^2 and r.z.
+__global__ __launch_bounds__(NT) void k_pcg_b(Geo g, double* x, PcgBufs q, int it, IterState* st, double* part) {
+  __shared__ double sh[NT / kWave + 1];
+  __shared__ int shi;
+  if (block_read(&st->pcg_stop, &shi)) return;
+  double* pp = part + (size_t)SLOT_PCG * g.NB;
+  const double pap = parts_sum<NT>(pp, g.NB, sh);
+  if (fabs(pap) < kTiny20) {  // pressure_solver.cpp:158
+    if (block_id() == 0 && thread_id() == 0) { st->pcg_stop = 1; st->pcg_iters = it; }
+    return;
+  }
+  const double alpha = st->rzs[it & 1] / pap;
+  const int M = g.M, N = g.N;
+  const double* pn = (it & 1) ? q.pa : q.pb;
+  double e = 0.0, rz = 0.0;
+  FIELD_LOOP(g, i, j) {
+    if (!(i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2)) continue;
+    const size_t k = at(g, i, j);
+    x[k] += alpha * pn[k];
+    const double r = q.r[k] - alpha * q.Ap[k];
+    q.r[k] = r;
+    const double d = q.diag[k];
+    const double z = (fabs(d) > kTiny20) ? r / d : r;
+    q.z[k] = z;
+    e += r * r;
+    rz += r * z;
+  }
+  const double A = block_sum<NT>(e, sh), B = block_sum<NT>(rz, sh);
+  if (thread_id() == 0) {
+    pp[g.NB + block_id()] = A;
+    pp[2 * g.NB + block_id()] = B;
+  }
+}
+
+// After K launched PCG iterations: stop test of iteration K-1 (one block).
+__global__ __launch_bounds__(NT) void k_pcg_end(Geo g, int K, double tol, IterState* st, const double* part) {
+  __shared__ double sh[NT / kWave + 1];
+  __shared__ int shi;
+  if (block_read(&st->pcg_stop, &shi)) return;
+  const double err = parts_sum<NT>(part + (size_t)(SLOT_PCG + 1) * g.NB, g.NB, sh);
+  if (thread_id() == 0) {
+    st->err = err;
+    if (sqrt(err) < tol) { st->pcg_stop = 1; st->pcg_iters = K - 1; }
+    else st->pcg_iters = K;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Corrections (iterations.cpp:556-600) with the BCs of u, v, p fused: the
+// thread that owns an interior value also writes the ghost copied from it.
+__global__ __launch_bounds__(NT) void k_correct(Geo g, Fields f) {
+  const int j = blockIdx.x * BX + threadIdx.x;
+  const int M = g.M, N = g.N, P = g.P;
+  if (j > N) return;
+  for (int i = blockIdx.y * BY + threadIdx.y; i <= M; i += g.gy * BY) {
+  const size_t k = at(g, i, j);
+  const double mv = g.maxVel;
+  if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 1) {
+    const double pn = f.p[k] + g.pAlpha * f.ps[k];
+    f.p[k] = pn;
+    if (i == 1) f.p[k - P] = pn;        // p(0,j) = p(1,j)
+    if (i == M - 1) f.p[k + P] = pn;    // p(M,j) = p(M-1,j)
+    if (j == 1) f.p[k - 1] = pn;        // p(i,0) = p(i,1)
+    if (j == N - 1) f.p[k + 1] = 0.0;   // p(i,N) = 0
+  }
+  if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) {
+    double du = f.dE[k] * (f.ps[k] - f.ps[k + 1]);
+    du = clampv(du, mv);
+    const double nu = clampv(f.us[k] + du, mv);
+    f.u[k] = nu;
+    if (j == N - 2) f.u[k + 1] = nu;
+  }
+  if (i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) {
+    double dv = f.dN[k] * (f.ps[k] - f.ps[k + P]);
+    dv = clampv(dv, mv);
+    const double nv = clampv(f.vs[k] + dv, mv);
+    f.v[k] = nv;
+    if (j == N - 1) f.v[k + 1] = nv;
+  }
+  if ((i == 0 || i == M) && j >= 1 && j <= N - 1) f.u[k] = 0.0;
+  if (j == 0) f.u[k] = f.inlet[i];
+  if (i <= M - 1 && (i == 0 || i == M - 1)) f.v[k] = 0.0;
+  if (i >= 1 && i <= M - 2 && j == 0) f.v[k] = 0.0;
+  }
+}
+
+// Boundary patch sums (postprocessing.cpp:245-327), one block.
+__global__ void k_patch(Geo g, Fields f, double tol, int inlet, double* out) {
+  __shared__ double sh[256 / kWave];
+  const int M = g.M, N = g.N;
+  const int uCol = inlet ? 0 : N - 1, pCol = inlet ? 0 : N, vCol = inlet ? 0 : N;
+  const int adj = inlet ? 0 : N - 2;
+  double sA = 0, dA = 0, tw = 0, area = 0, flux = 0;
+  for (int i = 1 + threadIdx.x; i < M; i += blockDim.x) {
+    const int cr = i - 1;
+    if (f.gam[at(g, cr, adj)] < 1.0 - tol) continue;
+    const double fa = g.hy;
+    area += fa;
+    const int vTop = i - 1, vBot = min(i, M - 1);
+    const double sp = f.p[at(g, i, pCol)];
+    const double un = f.u[at(g, i, uCol)];
+    const double vt = 0.5 * (f.v[at(g, vTop, vCol)] + f.v[at(g, vBot, vCol)]);
+    const double vm = sqrt(un * un + vt * vt);
+    const double dyn = 0.5 * g.rho * vm * vm;
+    sA += sp * fa;
+    dA += dyn * fa;
+    const double fl = fabs(g.rho * un * fa);
+    if (fl > 0.0) { flux += fl; tw += (sp + dyn) * fl; }
+  }
+  sA = block_sum<256>(sA, sh); dA = block_sum<256>(dA, sh); tw = block_sum<256>(tw, sh);
+  area = block_sum<256>(area, sh); flux = block_sum<256>(flux, sh);
+  if (threadIdx.x == 0) { out[0] = area; out[1] = flux; out[2] = sA; out[3] = dA; out[4] = tw; }
+}
+
+// ===========================================================================
+// host side
+// ===========================================================================
+class SimpleSolver {
+ public:
+  Ctx* ctx;
+  cps_simple_config cfg;
+  Geo g;
+  int mode_fp64 = 1;
+  size_t S = 0;  // elements per padded field
+  DevBuf<double> u, v, p, us, vs, ps, dE, dN, gam, alp, inlet;
+  DevBuf<double> momD[16];  // FP64 coefficient arrays (x-form)
+  DevBuf<float> momF[16];   // FP32 coefficient arrays (delta form)
+  DevBuf<double> pcg[6];
+  DevBuf<double> part;  // SLOT_COUNT x NB block partials
+  DevBuf<IterState> st;
+  std::unique_ptr<PressureMG> mg;
+  int iteration = 0;
+  dim3 grid, block;
+  int* hflag = nullptr;
+
+  SimpleSolver(Ctx* c, const cps_simple_config& conf, const double* gamma, const double* inlet_u)
+      : ctx(c), cfg(conf) {
+    CPS_CHECK(cfg.nx >= 2 && cfg.ny >= 2, CPS_ERR_ARG, "nx, ny must be >= 2");
+    CPS_CHECK(cfg.hx > 0 && cfg.hy > 0, CPS_ERR_ARG, "hx, hy must be > 0");
+    CPS_CHECK(cfg.uv_alpha > 0 && cfg.p_alpha > 0, CPS_ERR_ARG, "relaxation factors must be > 0");
+    CPS_CHECK(cfg.precision == CPS_PREC_FP64 || cfg.precision == CPS_PREC_MIXED, CPS_ERR_ARG,
+              "precision must be 64 or 32");
+    CPS_CHECK(cfg.pressure_solver == CPS_PRESSURE_PCG || cfg.pressure_solver == CPS_PRESSURE_GMG, CPS_ERR_ARG,
+              "pressure_solver must be 1 (PCG) or 5 (GMG)");
+    CPS_CHECK(!(cfg.precision == CPS_PREC_MIXED && cfg.pressure_solver == CPS_PRESSURE_PCG), CPS_ERR_ARG,
+              "mixed precision needs the GMG pressure solver (5)");
+    CPS_CHECK(gamma != nullptr, CPS_ERR_ARG, "gamma is required");
+    mode_fp64 = cfg.precision == CPS_PREC_FP64;
+    const int M = cfg.ny + 1, N = cfg.nx + 1;
+    g.M = M;
+    g.N = N;
+    g.P = ((N + 1 + 63) / 64) * 64;
+    g.nU = (M - 1) * (N - 2);
+    g.nV = (M - 2) * (N - 1);
+    g.nP = (M - 1) * (N - 2);
+    g.hx = cfg.hx;
+    g.hy = cfg.hy;
+    g.rho = cfg.rho;
+    g.eta = cfg.eta;
+    g.vol = g.hx * g.hy;
+    g.targetVel = cfg.u_in;
+    g.maxVel = 3.0 * std::max(std::fabs(cfg.u_in), kTenthF);
+    const bool on25 = cfg.enable_25d != 0 && cfg.ht > 0.0;  // SIMPLE.cpp:67-73
+    g.convScale = on25 ? cfg.conv_factor_25d : 1.0;
+    const double sinkCoeff = on25 ? cfg.sink_mult_25d * (12.0 * cfg.eta / (cfg.ht * cfg.ht)) : 0.0;
+    g.sinkDiag = sinkCoeff * g.vol;
+    g.De = cfg.eta * (g.hy / g.hx);
+    g.Dn = cfg.eta * (g.hx / g.hy);
+    g.uvAlpha = cfg.uv_alpha;
+    g.pAlpha = cfg.p_alpha;
+    g.sou = cfg.convection_scheme == 1;
+    S = (size_t)(M + 1) * g.P;
+
+    for (auto* b : {&u, &v, &p, &us, &vs, &ps, &dE, &dN, &gam, &alp}) b->alloc(S);
+    inlet.alloc(M + 1);
+    for (auto* b : {&u, &v, &p, &us, &vs, &ps, &dE, &dN, &gam, &alp})
+      CPS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), ctx->stream));
+    if (mode_fp64) {
+      for (auto& b : momD) { b.alloc(S); CPS_HIP(hipMemsetAsync(b.p, 0, b.bytes(), ctx->stream)); }
+    } else {
+      for (auto& b : momF) { b.alloc(S); CPS_HIP(hipMemsetAsync(b.p, 0, b.bytes(), ctx->stream)); }
+    }
+    for (auto& b : pcg) { b.alloc(S); CPS_HIP(hipMemsetAsync(b.p, 0, b.bytes(), ctx->stream)); }
+    // capped grid: about four 256-thread blocks per CU, rows strided
+    block = dim3(BX, BY);
+    g.gx = ceil_div(N + 1, BX);
+    g.gy = std::max(1, std::min(ceil_div(M + 1, BY), std::max(1, 1024 / g.gx)));
+    g.NB = g.gx * g.gy;
+    grid = dim3(g.gx, g.gy);
+    part.alloc((size_t)SLOT_COUNT * g.NB);
+    CPS_HIP(hipMemsetAsync(part.p, 0, part.bytes(), ctx->stream));
+    st.alloc(1);
+    CPS_HIP(hipMemsetAsync(st.p, 0, sizeof(IterState), ctx->stream));
+
+    // gamma (clamped, padded: SIMPLE.cpp:707-729) and alpha (SIMPLE.cpp:766-771)
+    std::vector<double> hg((size_t)M * g.P, 1.0), ha((size_t)M * g.P, 0.0);
+    const int pr = M - 1, pc = N - 1;
+    for (int i = 0; i < pr; ++i)
+      for (int j = 0; j < pc; ++j) hg[(size_t)i * g.P + j] = std::max(0.0, std::min(1.0, gamma[(size_t)i * pc + j]));
+    for (int i = 0; i < pr; ++i) hg[(size_t)i * g.P + pc] = hg[(size_t)i * g.P + pc - 1];
+    for (int j = 0; j < N; ++j) hg[(size_t)pr * g.P + j] = hg[(size_t)(pr - 1) * g.P + std::min(j, pc - 1)];
+    const double alphaMax = cfg.eta / std::max(cfg.brinkman_kmin, kKminFloor);
+    for (int i = 0; i < M; ++i)
+      for (int j = 0; j < N; ++j) {
+        const double gg = hg[(size_t)i * g.P + j];
+        ha[(size_t)i * g.P + j] = alphaMax * (1.0 - gg) / (1.0 + cfg.brinkman_q * gg + kAlphaEps);
+      }
+    CPS_HIP(hipMemcpyAsync(gam.p, hg.data(), hg.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    CPS_HIP(hipMemcpyAsync(alp.p, ha.data(), ha.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    // inlet values with the gamma > 0.5 mask (boundaries.cpp:82-100)
+    hinlet.assign(M + 1, 0.0);
+    for (int i = 0; i <= M; ++i) {
+      int cr = i <= 0 ? 0 : (i >= M ? std::max(0, M - 2) : i - 1);
+      cr = std::max(0, std::min(cr, M - 1));
+      const double speed = inlet_u ? inlet_u[i] : cfg.u_in;
+      hinlet[i] = hg[(size_t)cr * g.P] > (double)0.5f ? speed : 0.0;
+    }
+    CPS_HIP(hipMemcpyAsync(inlet.p, hinlet.data(), (M + 1) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if (cfg.pressure_solver == CPS_PRESSURE_GMG) mg_create();
+    CPS_HIP(hipStreamSynchronize(ctx->stream));
+  }
+
+  std::vector<double> hinlet;
+
+  Fields fields() {
+    return Fields{u.p, v.p, p.p, us.p, vs.p, ps.p, dE.p, dN.p, gam.p, alp.p, inlet.p};
+  }
+  template <class C>
+  MomSys<C> sys(DevBuf<C>* b, int comp) {
+    DevBuf<C>* q = b + 8 * comp;
+    return MomSys<C>{q[0].p, q[1].p, q[2].p, q[3].p, q[4].p, q[5].p, q[6].p, q[7].p};
+  }
+  PcgBufs pbufs() { return PcgBufs{pcg[0].p, pcg[1].p, pcg[2].p, pcg[3].p, pcg[4].p, pcg[5].p}; }
+
+  void apply_bc() {
+    dim3 bg(ceil_div(std::max(g.M, g.N) + 1, 256), 2);
+    k_bc<<<bg, 256, 0, ctx->stream>>>(g, u.p, v.p, p.p, inlet.p, 1);
+    CPS_HIP(hipGetLastError());
+  }
+
+  // Reads one int of device state (blocking).
+  int read_int(const int* dptr) {
+    int h = 0;
+    CPS_HIP(hipMemcpyAsync(&h, dptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    CPS_HIP(hipStreamSynchronize(ctx->stream));
+    return h;
+  }
+
+  // Launches up to `maxit` device-side-terminating iterations; when the cap is
+  // large the host checks the device stop flag between chunks.
+  static constexpr int kNoSyncCap = 32;
+
+  void momentum() {
+    hipStream_t s = ctx->stream;
+    const Fields f = fields();
+    k_jac_reset<<<1, 64, 0, s>>>(st.p, g.nU > 0, g.nV > 0);
+    const double tol2 = cfg.momentum_tol * cfg.momentum_tol;
+    const int maxit = cfg.max_momentum_iter;
+    const int chunk = maxit > kNoSyncCap ? 8 : std::max(1, maxit);
+    if (mode_fp64) k_assemble<double, false><<<grid, block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1));
+    else k_assemble<float, true><<<grid, block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1));
+    CPS_HIP(hipGetLastError());
+    int k = 0;
+    while (k < maxit) {
+      const int k1 = std::min(maxit, k + chunk);
+      for (; k < k1; ++k) {
+        if (mode_fp64)
+          k_jacobi<double><<<grid, block, 0, s>>>(g, sys(momD, 0), sys(momD, 1), k, tol2, st.p, part.p);
+        else
+          k_jacobi<float><<<grid, block, 0, s>>>(g, sys(momF, 0), sys(momF, 1), k, tol2, st.p, part.p);
+      }
+      k_jac_end<<<1, block, 0, s>>>(g, k, tol2, st.p, part.p);
+      CPS_HIP(hipGetLastError());
+      if (k >= maxit) break;
+      int act[2];
+      CPS_HIP(hipMemcpyAsync(act, st.p->jac_active, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+      CPS_HIP(hipStreamSynchronize(s));
+      if (!act[0] && !act[1]) break;
+    }
+    if (maxit == 0) k_jac_end<<<1, block, 0, s>>>(g, 0, tol2, st.p, part.p);
+    if (mode_fp64)
+      k_finalize<double, false><<<grid, block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p, part.p);
+    else
+      k_finalize<float, true><<<grid, block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p, part.p);
+    CPS_HIP(hipGetLastError());
+  }
+
+  void pressure_pcg() {
+    hipStream_t s = ctx->stream;
+    const int maxit = cfg.max_pressure_iter;
+    const int chunk = maxit > kNoSyncCap ? 16 : std::max(1, maxit);
+    const PcgBufs q = pbufs();
+    int k = 0;
+    while (k < maxit) {
+      const int k1 = std::min(maxit, k + chunk);
+      for (; k < k1; ++k) {
+        k_pcg_a<<<grid, block, 0, s>>>(g, dE.p, dN.p, q, k, cfg.p_tol, st.p, part.p);
+        k_pcg_b<<<grid, block, 0, s>>>(g, ps.p, q, k, st.p, part.p);
+      }
+      k_pcg_end<<<1, block, 0, s>>>(g, k, cfg.p_tol, st.p, part.p);
+      CPS_HIP(hipGetLastError());
+      if (k >= maxit) break;
+      if (read_int(&st.p->pcg_stop)) break;
+    }
+  }
+
+  void mg_create();
+  void pressure_mg();
+
+  void step_once() {
+    hipStream_t s = ctx->stream;
+    const Fields f = fields();
+    momentum();
+    k_prhs<<<grid, block, 0, s>>>(g, f, pbufs(), part.p);
+    k_stats<<<1, block, 0, s>>>(g, st.p, part.p);
+    CPS_HIP(hipGetLastError());
+    if (cfg.pressure_solver == CPS_PRESSURE_PCG) pressure_pcg();
+    else pressure_mg();
+    k_correct<<<grid, block, 0, s>>>(g, f);
+    CPS_HIP(hipGetLastError());
+    iteration++;
+  }
+
+  void stats(cps_step_stats* out) {
+    IterState h{};
+    CPS_HIP(hipMemcpyAsync(&h, st.p, sizeof(IterState), hipMemcpyDeviceToHost, ctx->stream));
+    CPS_HIP(hipStreamSynchronize(ctx->stream));
+    if (!out) return;
+    out->error = h.error;
+    out->resid_u = h.resid_u;
+    out->resid_v = h.resid_v;
+    out->resid_mass = h.resid_mass;
+    out->resid_u_max = h.umax;
+    out->resid_v_max = h.vmax;
+    out->resid_mass_max = h.mass_max;
+    out->u_iters = h.jac_iters[0];
+    out->v_iters = h.jac_iters[1];
+    out->p_iters = h.pcg_iters;
+    out->iteration = iteration;
+  }
+
+  void step(int n, cps_step_stats* out) {
+    apply_bc();
+    for (int k = 0; k < n; ++k) step_once();
+    cps_step_stats tmp;
+    stats(out ? out : &tmp);
+    const double e = (out ? out : &tmp)->error;
+    CPS_CHECK(std::isfinite(e), CPS_ERR_DIVERGED, "SIMPLE residual is not finite");
+  }
+
+  void run(cps_step_stats* out) {
+    apply_bc();
+    cps_step_stats sst{};
+    sst.error = 1.0;
+    int it = 0;
+    // SIMPLE.cpp:378: while (iter < maxIterations && error > epsilon)
+    while (it < cfg.max_iterations && sst.error > cfg.epsilon) {
+      step_once();
+      stats(&sst);
+      CPS_CHECK(std::isfinite(sst.error), CPS_ERR_DIVERGED, "SIMPLE residual is not finite");
+      ++it;
+    }
+    if (out) *out = sst;
+  }
+
+  DevBuf<double>* field(int which) {
+    switch (which) {
+      case CPS_FIELD_U: return &u;
+      case CPS_FIELD_V: return &v;
+      case CPS_FIELD_P: return &p;
+      case CPS_FIELD_USTAR: return &us;
+      case CPS_FIELD_VSTAR: return &vs;
+      case CPS_FIELD_PSTAR: return &ps;
+      case CPS_FIELD_DE: return &dE;
+      case CPS_FIELD_DN: return &dN;
+      case CPS_FIELD_GAMMA: return &gam;
+      case CPS_FIELD_ALPHA: return &alp;
+    }
+    return nullptr;
+  }
+  void shape(int which, int* r, int* c) {
+    const int M = g.M, N = g.N;
+    switch (which) {
+      case CPS_FIELD_U: case CPS_FIELD_USTAR: case CPS_FIELD_DE: *r = M + 1; *c = N; break;
+      case CPS_FIELD_V: case CPS_FIELD_VSTAR: case CPS_FIELD_DN: *r = M; *c = N + 1; break;
+      case CPS_FIELD_P: case CPS_FIELD_PSTAR: *r = M + 1; *c = N + 1; break;
+      default: *r = M; *c = N;
+    }
+  }
+  void get(int which, double* out) {
+    DevBuf<double>* b = field(which);
+    CPS_CHECK(b && out, CPS_ERR_ARG, "bad field id or null output");
+    int r, c;
+    shape(which, &r, &c);
+    CPS_HIP(hipMemcpy2DAsync(out, c * sizeof(double), b->p, g.P * sizeof(double), c * sizeof(double), r,
+                             hipMemcpyDeviceToHost, ctx->stream));
+    CPS_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  void set(int which, const double* in) {
+    CPS_CHECK(which >= CPS_FIELD_U && which <= CPS_FIELD_PSTAR, CPS_ERR_ARG, "only u, v, p, uStar, vStar, pStar are settable");
+    DevBuf<double>* b = field(which);
+    CPS_CHECK(in, CPS_ERR_ARG, "null input");
+    int r, c;
+    shape(which, &r, &c);
+    CPS_HIP(hipMemcpy2DAsync(b->p, g.P * sizeof(double), in, c * sizeof(double), c * sizeof(double), r,
+                             hipMemcpyHostToDevice, ctx->stream));
+    CPS_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  void patch(int inl, cps_patch* out) {
+    DevBuf<double> o;
+    o.alloc(5);
+    k_patch<<<1, 256, 0, ctx->stream>>>(g, fields(), cfg.dp_fluid_tol, inl, o.p);
+    CPS_HIP(hipGetLastError());
+    double h[5];
+    CPS_HIP(hipMemcpyAsync(h, o.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    CPS_HIP(hipStreamSynchronize(ctx->stream));
+    out->area = h[0];
+    out->mass_flux = h[1];
+    out->avg_static = h[0] > 0 ? h[2] / h[0] : 0.0;
+    out->avg_dynamic = h[0] > 0 ? h[3] / h[0] : 0.0;
+    out->valid = h[0] > 0;
+    out->avg_total = h[1] > 0 ? h[4] / h[1] : (h[0] > 0 ? out->avg_static + out->avg_dynamic : 0.0);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// GMG pressure path: defined with the multigrid kernels.
+void SimpleSolver::mg_create() { mg.reset(new PressureMG(ctx, g)); }
+void SimpleSolver::pressure_mg() {
+  const PcgBufs q = pbufs();
+  mg->solve(dE.p, dN.p, q.r, ps.p, st.p, cfg, mode_fp64 != 0);
+}
+
+}  // namespace cps
+
+struct cps_simple {
+  cps::SimpleSolver* s;
+};
+
+using cps::guard;
+
+extern "C" {
+
+void cps_simple_config_default(cps_simple_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->rho = 997.0;
+  c->eta = 0.00089;
+  c->conv_factor_25d = 1.2;
+  c->sink_mult_25d = 1.0;
+  c->brinkman_kmin = 1e-14;
+  c->brinkman_q = 1.0;
+  c->uv_alpha = 0.7;
+  c->p_alpha = 0.3;
+  c->momentum_tol = 1e-6;
+  c->max_momentum_iter = 400;
+  c->p_tol = 1e-6;
+  c->max_pressure_iter = 30;
+  c->pressure_solver = CPS_PRESSURE_PCG;
+  c->precision = CPS_PREC_FP64;
+  c->max_refine = 4;
+  c->epsilon = 1e-7;
+  c->max_iterations = 1000;
+  c->dp_fluid_tol = 1e-6;
+}
+
+int cps_simple_create(cps_ctx* ctx, const cps_simple_config* cfg, const double* gamma, const double* inlet_u,
+                      cps_simple** out) {
+  return guard([&] {
+    CPS_CHECK(ctx && cfg && out, CPS_ERR_ARG, "null argument");
+    CPS_HIP(hipSetDevice(ctx->c.device));
+    auto* h = new cps_simple;
+    try {
+      h->s = new cps::SimpleSolver(&ctx->c, *cfg, gamma, inlet_u);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+void cps_simple_destroy(cps_simple* s) {
+  if (!s) return;
+  delete s->s;
+  delete s;
+}
+
+int cps_simple_step(cps_simple* s, int n_steps, cps_step_stats* last) {
+  return guard([&] {
+    CPS_CHECK(s && n_steps >= 0, CPS_ERR_ARG, "bad arguments");
+    s->s->step(n_steps, last);
+  });
+}
+
+int cps_simple_run(cps_simple* s, cps_step_stats* last) {
+  return guard([&] {
+    CPS_CHECK(s, CPS_ERR_ARG, "null solver");
+    s->s->run(last);
+  });
+}
+
+int cps_simple_get_field(cps_simple* s, int field, double* out) {
+  return guard([&] {
+    CPS_CHECK(s, CPS_ERR_ARG, "null solver");
+    s->s->get(field, out);
+  });
+}
+
+int cps_simple_set_field(cps_simple* s, int field, const double* in) {
+  return guard([&] {
+    CPS_CHECK(s, CPS_ERR_ARG, "null solver");
+    s->s->set(field, in);
+  });
+}
+
+int cps_simple_boundary_patch(cps_simple* s, int inlet, cps_patch* out) {
+  return guard([&] {
+    CPS_CHECK(s && out, CPS_ERR_ARG, "null argument");
+    s->s->patch(inlet, out);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,68 @@
+// cps_simple.hpp -- device-resident SIMPLE solver (host side).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "cps_common.hpp"
+
+namespace cps {
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int* pinned_i = nullptr;  // small pinned host scratch for flag reads
+};
+
+// Scalars shared by kernels of one SIMPLE iteration (device memory).
+// Written by block 0 of the kernel that finalises a reduction; read by later
+// kernels on the same stream and by the host.
+struct IterState {
+  int jac_active[2];   // component still iterating
+  int jac_iters[2];    // sweeps done
+  double rz0;          // r.z of the PCG start
+  double rzs[2];       // r.z of PCG iteration it, slot it & 1
+  double err;          // ||r||^2 after the last PCG update
+  int pcg_stop, pcg_iters;
+  double su, sv, umax, vmax;  // momentum change sums / max
+  double mass_max, mass_ss;
+  double bnorm2;              // ||rhs||^2 of the pressure system
+  double resid_u, resid_v, resid_mass, error;
+  int ir_outer;               // refinement sweeps done (mixed pressure)
+  int pad_;
+};
+
+// Partial-sum slots (each NB doubles) of the SIMPLE step kernels.
+enum PartSlot {
+  SLOT_JAC = 0,     // 4 slots: U/V sum r^2, by sweep parity
+  SLOT_FIN = 4,     // 4 slots: sum du^2, sum dv^2, max |du|, max |dv|
+  SLOT_RHS = 8,     // 4 slots: max |b|, sum b^2 (j <= N-1), r.z, ||b||^2 (DOFs)
+  SLOT_PCG = 12,    // 3 slots: pAp, ||r||^2, r.z
+  SLOT_COUNT = 16
+};
+
+// Geometry + physics scalars passed to kernels by value.
+struct Geo {
+  int M, N, P;        // padded sizes and pitch (elements)
+  int gx, gy, NB;     // capped launch grid of the field kernels, NB = gx*gy
+  int nU, nV, nP;     // DOF counts
+  double hx, hy, rho, eta, vol;
+  double maxVel, convScale, sinkDiag, De, Dn;
+  double uvAlpha, pAlpha;
+  double targetVel;
+  int sou;
+};
+
+class SimpleSolver;
+class PressureMG;
+
+}  // namespace cps
+
+namespace cps {
+void csr_free(void* p);
+}
+
+struct cps_ctx {
+  cps::Ctx c;
+  void* csr = nullptr;  // CSR drop-in work buffers (cps_csr.hip)
+};

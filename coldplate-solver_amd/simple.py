@@ -1,0 +1,137 @@
+"""Host-side mirror of the reference SIMPLE solver interface on the HIP path.
+
+`SIMPLE` follows the reference class (SIMPLE.h:55-614): construction loads
+the grid, density and Brinkman field (SIMPLE::SIMPLE, SIMPLE.cpp:18-29),
+`calculateStep` performs one SIMPLE iteration (iterations.cpp:82-659) and
+returns the max RMS residual, `runIterations` iterates to `epsilon` or
+`maxIterations` (SIMPLE.cpp:378), and `sampleBoundaryPatch` /
+`full_static_drop` give the full-system pressure drop
+(postprocessing.cpp:245-327, SIMPLE.cpp:438).  Every call goes through the C
+ABI of libcoldplate_hip.so; the fields stay on the device.
+
+`pressure_solve_csr` / `momentum_jacobi_csr` are the drop-ins for the
+reference GPU backend calls amgcl_cuda::solve and
+amgcl_cuda::solveMomentumU/V (cuda_solver.hpp:24-57).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from . import _types as T
+
+
+class SIMPLE:
+    def __init__(self, cfg, gamma, inlet=None, device=0):
+        self.ctx = _capi.context(device)
+        self.L = self.ctx.L
+        self.cfg = cfg.copy()
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        if g.shape != (cfg.ny, cfg.nx):
+            raise ValueError(f"gamma must be ny x nx = {(cfg.ny, cfg.nx)}, got {g.shape}")
+        inl = None
+        if inlet is not None:
+            inl = np.ascontiguousarray(inlet, dtype=np.float64)
+            if inl.shape != (cfg.ny + 2,):
+                raise ValueError("inlet profile must have ny + 2 (= M + 1) entries")
+        h = C.c_void_p()
+        _capi.check(self.L.cps_simple_create(self.ctx.h, C.byref(self.cfg), _capi.dptr(g),
+                                             None if inl is None else _capi.dptr(inl), C.byref(h)))
+        self.h = h
+        self.M, self.N = cfg.ny + 1, cfg.nx + 1
+        self.last = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cps_simple_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- iteration -------------------------------------------------------
+    def step(self, n=1):
+        st = T.StepStats()
+        _capi.check(self.L.cps_simple_step(self.h, n, C.byref(st)))
+        self.last = st.as_dict()
+        return self.last
+
+    def calculateStep(self):
+        return self.step(1)["error"]
+
+    def runIterations(self):
+        st = T.StepStats()
+        _capi.check(self.L.cps_simple_run(self.h, C.byref(st)))
+        self.last = st.as_dict()
+        return self.last
+
+    # ---- fields ----------------------------------------------------------
+    def get(self, name):
+        out = np.empty(T.field_shape(name, self.cfg.nx, self.cfg.ny))
+        _capi.check(self.L.cps_simple_get_field(self.h, T.FIELDS[name], _capi.dptr(out)))
+        return out
+
+    def set(self, name, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != T.field_shape(name, self.cfg.nx, self.cfg.ny):
+            raise ValueError(f"{name} must have shape {T.field_shape(name, self.cfg.nx, self.cfg.ny)}")
+        _capi.check(self.L.cps_simple_set_field(self.h, T.FIELDS[name], _capi.dptr(a)))
+
+    u = property(lambda self: self.get("u"))
+    v = property(lambda self: self.get("v"))
+    p = property(lambda self: self.get("p"))
+
+    # ---- post-processing -------------------------------------------------
+    def sampleBoundaryPatch(self, inlet):
+        m = T.Patch()
+        _capi.check(self.L.cps_simple_boundary_patch(self.h, 1 if inlet else 0, C.byref(m)))
+        return m.as_dict()
+
+    def full_static_drop(self):
+        return self.sampleBoundaryPatch(True)["avg_static"] - self.sampleBoundaryPatch(False)["avg_static"]
+
+    def full_total_drop(self):
+        return self.sampleBoundaryPatch(True)["avg_total"] - self.sampleBoundaryPatch(False)["avg_total"]
+
+
+def _csr_args(row_ptr, col_idx, values, rhs, solution):
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    ci = np.ascontiguousarray(col_idx, dtype=np.int32)
+    val = np.ascontiguousarray(values, dtype=np.float32)
+    b = np.ascontiguousarray(rhs, dtype=np.float32)
+    if not (isinstance(solution, np.ndarray) and solution.dtype == np.float32 and solution.flags.c_contiguous):
+        raise TypeError("solution must be a contiguous float32 array (updated in place)")
+    n = len(b)
+    if rp.shape != (n + 1,) or solution.shape != (n,) or len(ci) != rp[-1] or len(val) != rp[-1]:
+        raise ValueError("inconsistent CSR sizes")
+    return n, rp, ci, val, b
+
+
+def pressure_solve_csr(row_ptr, col_idx, values, rhs, solution, tolerance, max_iterations, device=0):
+    """Drop-in for amgcl_cuda::solve (cuda_solver.hpp:24-33) with the
+    Jacobi-PCG semantics of solvePCG_CSR (pressure_solver.cpp:121-184);
+    `solution` (float32) is the initial guess in and the result out.
+    Returns the iteration count."""
+    ctx = _capi.context(device)
+    n, rp, ci, val, b = _csr_args(row_ptr, col_idx, values, rhs, solution)
+    it = C.c_int()
+    _capi.check(ctx.L.cps_pressure_solve_csr(ctx.h, n, _capi.iptr(rp), _capi.iptr(ci), _capi.fptr(val),
+                                             _capi.fptr(b), _capi.fptr(solution), tolerance, max_iterations,
+                                             C.byref(it)))
+    return it.value
+
+
+def momentum_jacobi_csr(row_ptr, col_idx, values, rhs, solution, tolerance, max_iterations, device=0):
+    """Drop-in for amgcl_cuda::solveMomentumU/V (cuda_solver.hpp:36-57) with
+    the stop rule of solvers::cpu::solveMomentumJacobi (jacobi_solver.cpp:
+    20-75).  Returns the sweep count."""
+    ctx = _capi.context(device)
+    n, rp, ci, val, b = _csr_args(row_ptr, col_idx, values, rhs, solution)
+    it = C.c_int()
+    _capi.check(ctx.L.cps_momentum_jacobi_csr(ctx.h, n, _capi.iptr(rp), _capi.iptr(ci), _capi.fptr(val),
+                                              _capi.fptr(b), _capi.fptr(solution), tolerance, max_iterations,
+                                              C.byref(it)))
+    return it.value

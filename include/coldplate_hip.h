@@ -1,0 +1,153 @@
+/*
+ * coldplate_hip.h -- C ABI of libcoldplate_hip.so, the MI355X (gfx950) backend
+ * for the coldplate-solver SIMPLE hot path and its 3-D thermal solve.
+ *
+ * Conventions (all entry points):
+ *   - plain C types, host pointers, sizes in elements; no exceptions cross
+ *     the ABI; every function returns an int status (CPS_OK = 0) and the
+ *     message of the last failure on this thread is in cps_last_error();
+ *   - arrays are row-major.  Grid arrays use the reference's padded staggered
+ *     shapes with M = ny + 1, N = nx + 1 (SIMPLE.cpp:53-54):
+ *       u: (M+1) x N, v: M x (N+1), p: (M+1) x (N+1), gamma/alpha: M x N;
+ *   - one context owns one device and one HIP stream; calls on one context
+ *     are serialised by the caller (the reference's solvers are not
+ *     re-entrant either: pressure_solver.cpp:502-527 keeps static state).
+ */
+#ifndef COLDPLATE_HIP_H
+#define COLDPLATE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPS_OK 0
+#define CPS_ERR_ARG 1   /* invalid argument (sizes, null pointers, options) */
+#define CPS_ERR_HIP 2   /* a HIP runtime call failed */
+#define CPS_ERR_STATE 3 /* object used in the wrong state */
+#define CPS_ERR_DIVERGED 4 /* NaN/Inf detected in a residual */
+
+typedef struct cps_ctx cps_ctx;
+typedef struct cps_simple cps_simple;
+typedef struct cps_thermal cps_thermal;
+
+/* Message of the last failed call on the calling thread ("" if none). */
+const char* cps_last_error(void);
+/* ABI version, bumped on any incompatible change. */
+int cps_abi_version(void);
+
+/* Context: one device + one stream.  Replaces the process-global state that
+ * amgcl_cuda::initialize / cleanup manage (cuda_solver.hpp:17-21). */
+int cps_create(int device, cps_ctx** out);
+void cps_destroy(cps_ctx* ctx);
+int cps_synchronize(cps_ctx* ctx);
+
+/* ---------------------------------------------------------------------------
+ * CSR drop-ins.  Same contract as the reference's GPU backend ABI
+ * (cuda_solver.hpp:24-57): host CSR in, solution in/out (initial guess in),
+ * iteration count out.
+ *
+ * cps_pressure_solve_csr replaces amgcl_cuda::solve (cuda_solver.hpp:24-33);
+ * it runs the reference CPU Jacobi-PCG semantics of solvePCG_CSR
+ * (pressure_solver.cpp:121-184): stop when ||r||_2 < tolerance (absolute) or
+ * |p.Ap| < 1e-20; *iterations = loop index at exit.
+ *
+ * cps_momentum_jacobi_csr replaces amgcl_cuda::solveMomentumU/V
+ * (cuda_solver.hpp:36-57) with the CPU convergence rule of
+ * solvers::cpu::solveMomentumJacobi (jacobi_solver.cpp:20-75): stop after the
+ * sweep whose pre-update residual satisfies sum r^2 < tol^2 n.
+ * ------------------------------------------------------------------------- */
+int cps_pressure_solve_csr(cps_ctx* ctx, int n, const int* row_ptr, const int* col_idx, const float* values,
+                           const float* rhs, float* solution, float tolerance, int max_iterations,
+                           int* iterations);
+int cps_momentum_jacobi_csr(cps_ctx* ctx, int n, const int* row_ptr, const int* col_idx, const float* values,
+                            const float* rhs, float* solution, float tolerance, int max_iterations,
+                            int* iterations);
+
+/* ---------------------------------------------------------------------------
+ * Structured SIMPLE solver (device-resident fields).  Knobs carry the
+ * reference names of SIMPLE.h; defaults come from cps_simple_config_default.
+ * ------------------------------------------------------------------------- */
+#define CPS_PRESSURE_SOR 0    /* pressureSolverType 0 (oracle only) */
+#define CPS_PRESSURE_PCG 1    /* pressureSolverType 1: Jacobi-PCG, absolute tol */
+#define CPS_PRESSURE_GMG 5    /* new type: geometric-multigrid PCG on the device */
+
+#define CPS_PREC_FP64 64      /* every kernel in FP64 */
+#define CPS_PREC_MIXED 32     /* FP32 inner Jacobi / MG-PCG, FP64 state, residuals, refinement */
+
+typedef struct cps_simple_config {
+  int nx, ny;                /* physical cells; M = ny+1, N = nx+1 */
+  double hx, hy;             /* cell size [m] */
+  double rho, eta;           /* SIMPLE.h:374-375 */
+  double u_in;               /* targetVel [m/s] (uniform inlet, or peak of inlet profile) */
+  int convection_scheme;     /* 0 FOU, 1 SOU (SIMPLE.h:193) */
+  int enable_25d;            /* SIMPLE.h:205 */
+  double ht;                 /* Ht_channel [m] */
+  double conv_factor_25d;    /* SIMPLE.h:206 */
+  double sink_mult_25d;      /* SIMPLE.h:208 */
+  double brinkman_kmin;      /* SIMPLE.h:242 */
+  double brinkman_q;         /* SIMPLE.h:244 */
+  double uv_alpha, p_alpha;  /* SIMPLE.h:397-398 */
+  double momentum_tol;       /* SIMPLE.h:176 */
+  int max_momentum_iter;     /* SIMPLE.h:177 */
+  double p_tol;              /* SIMPLE.h:153 (absolute 2-norm) */
+  int max_pressure_iter;     /* SIMPLE.h:152 */
+  int pressure_solver;       /* CPS_PRESSURE_* */
+  int precision;             /* CPS_PREC_* */
+  double p_rtol;             /* GMG-PCG: also stop when ||r|| <= p_rtol ||r0|| (0 = off) */
+  int max_refine;            /* mixed precision: max FP64 refinement sweeps per pressure solve */
+  double epsilon;            /* SIMPLE.h:68 */
+  int max_iterations;        /* SIMPLE.h:69 */
+  double dp_fluid_tol;       /* pure-fluid test of sampleBoundaryPatch (postprocessing.cpp:268) */
+} cps_simple_config;
+
+/* Per-iteration statistics (iterations.cpp:499-533, 628-658). */
+typedef struct cps_step_stats {
+  double error;              /* max(residU, residV, residMass), raw RMS */
+  double resid_u, resid_v, resid_mass;
+  double resid_u_max, resid_v_max, resid_mass_max;
+  int u_iters, v_iters, p_iters;
+  int iteration;             /* SIMPLE iterations done so far on this object */
+} cps_step_stats;
+
+/* Boundary patch metrics (postprocessing.cpp:245-327). */
+typedef struct cps_patch {
+  double area, mass_flux, avg_static, avg_dynamic, avg_total;
+  int valid;
+} cps_patch;
+
+#define CPS_FIELD_U 0
+#define CPS_FIELD_V 1
+#define CPS_FIELD_P 2
+#define CPS_FIELD_USTAR 3
+#define CPS_FIELD_VSTAR 4
+#define CPS_FIELD_PSTAR 5
+#define CPS_FIELD_DE 6
+#define CPS_FIELD_DN 7
+#define CPS_FIELD_GAMMA 8
+#define CPS_FIELD_ALPHA 9
+
+void cps_simple_config_default(cps_simple_config* cfg);
+
+/* gamma: ny x nx physical density (1 = fluid, clamped to [0,1]; padded like
+ * SIMPLE.cpp:720-729).  inlet_u: M+1 per-u-row inlet speeds (before the
+ * gamma > 0.5 inlet mask of boundaries.cpp:82-100) or NULL for the uniform
+ * reference inlet u_in.  Replaces SIMPLE::SIMPLE() (SIMPLE.cpp:18-29). */
+int cps_simple_create(cps_ctx* ctx, const cps_simple_config* cfg, const double* gamma, const double* inlet_u,
+                      cps_simple** out);
+void cps_simple_destroy(cps_simple* s);
+/* n_steps SIMPLE iterations (SIMPLE::calculateStep, iterations.cpp:82-659);
+ * stats of the last one in *last (may be NULL). */
+int cps_simple_step(cps_simple* s, int n_steps, cps_step_stats* last);
+/* Iterate until error <= epsilon or max_iterations (SIMPLE.cpp:378); the
+ * dP-window stop and pseudo-time controller are off. */
+int cps_simple_run(cps_simple* s, cps_step_stats* last);
+int cps_simple_get_field(cps_simple* s, int field, double* out);
+int cps_simple_set_field(cps_simple* s, int field, const double* in);
+/* inlet = 1: x = 0 patch, 0: x = L patch. */
+int cps_simple_boundary_patch(cps_simple* s, int inlet, cps_patch* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COLDPLATE_HIP_H */

@@ -1,0 +1,224 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and bench.py's
+cpu_baseline leg as the checker.  The product package never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib(types):
+    """Load liboracle.so; `types` is the package's _types module (shared
+    struct mirrors).  Builds it first when only the sources are present."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = os.path.join(_HERE, "liboracle.so")
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    P = C.POINTER
+    dp = P(C.c_double)
+    ip = P(C.c_int)
+    L.orc_simple_create.restype = C.c_void_p
+    L.orc_simple_create.argtypes = [C.c_int, P(types.SimpleConfig), dp, dp]
+    L.orc_simple_destroy.argtypes = [C.c_void_p]
+    L.orc_simple_step.argtypes = [C.c_void_p, C.c_int, P(types.StepStats), dp]
+    L.orc_simple_run.argtypes = [C.c_void_p, P(types.StepStats), dp, C.c_int]
+    L.orc_simple_get_field.argtypes = [C.c_void_p, C.c_int, dp]
+    L.orc_simple_set_field.argtypes = [C.c_void_p, C.c_int, dp]
+    L.orc_simple_boundary_patch.argtypes = [C.c_void_p, C.c_int, P(types.Patch)]
+    L.orc_simple_apply_bc.argtypes = [C.c_void_p]
+    L.orc_simple_assemble.argtypes = [C.c_void_p, C.c_int]
+    L.orc_simple_csr_size.argtypes = [C.c_void_p, C.c_int, ip, ip]
+    L.orc_simple_csr_get.argtypes = [C.c_void_p, C.c_int, ip, ip, dp, dp, dp, ip]
+    L.orc_simple_residuals.argtypes = [C.c_void_p, dp]
+    L.orc_simple_face.restype = C.c_double
+    L.orc_simple_face.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    L.orc_simple_sou.restype = C.c_double
+    L.orc_simple_sou.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [C.c_double] * 4
+    for f in (L.orc_jacobi_csr, L.orc_pcg_csr):
+        f.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int]
+    L.orc_thermal_create.restype = C.c_void_p
+    L.orc_thermal_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dp, ip, C.c_double,
+                                     C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                     C.c_double, dp, dp, dp, dp]
+    L.orc_thermal_destroy.argtypes = [C.c_void_p]
+    L.orc_thermal_csr_size.argtypes = [C.c_void_p, ip, ip]
+    L.orc_thermal_csr_get.argtypes = [C.c_void_p, ip, ip, dp, dp]
+    L.orc_thermal_solve.argtypes = [C.c_void_p, dp, C.c_double, C.c_int, dp, dp]
+    _LIB = L
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class OracleSimple:
+    """The reference SIMPLE solver restated on the CPU (prec 64 or 32)."""
+
+    def __init__(self, types, cfg, gamma, inlet=None, prec=64):
+        self.T = types
+        self.L = lib(types)
+        self.cfg = cfg.copy()
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        assert g.shape == (cfg.ny, cfg.nx)
+        inl = None if inlet is None else np.ascontiguousarray(inlet, dtype=np.float64)
+        if inl is not None:
+            assert inl.shape == (cfg.ny + 2,)
+        self.h = self.L.orc_simple_create(prec, C.byref(self.cfg), _d(g), None if inl is None else _d(inl))
+        self.prec = prec
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_simple_destroy(self.h)
+            self.h = None
+
+    def shape(self, name):
+        return self.T.field_shape(name, self.cfg.nx, self.cfg.ny)
+
+    def get(self, name):
+        out = np.zeros(self.shape(name))
+        self.L.orc_simple_get_field(self.h, self.T.FIELDS[name], _d(out))
+        return out
+
+    def set(self, name, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == self.shape(name)
+        self.L.orc_simple_set_field(self.h, self.T.FIELDS[name], _d(a))
+
+    def step(self, n=1):
+        st = self.T.StepStats()
+        secs = C.c_double()
+        self.L.orc_simple_step(self.h, n, C.byref(st), C.byref(secs))
+        self.last_seconds = secs.value
+        return st.as_dict()
+
+    def run(self, history_len=0):
+        st = self.T.StepStats()
+        hist = np.zeros(max(history_len, 1))
+        it = self.L.orc_simple_run(self.h, C.byref(st), _d(hist), history_len)
+        d = st.as_dict()
+        d["iterations"] = it
+        d["history"] = hist[: min(it, history_len)]
+        return d
+
+    def patch(self, inlet):
+        m = self.T.Patch()
+        self.L.orc_simple_boundary_patch(self.h, 1 if inlet else 0, C.byref(m))
+        return m.as_dict()
+
+    def dp_static(self):
+        return self.patch(True)["avg_static"] - self.patch(False)["avg_static"]
+
+    def apply_bc(self):
+        self.L.orc_simple_apply_bc(self.h)
+
+    def assemble(self, which):
+        """which: 'u', 'v' or 'p'; returns the CSR system as a dict."""
+        w = {"u": 0, "v": 1, "p": 2}[which]
+        self.L.orc_simple_assemble(self.h, w)
+        n, nnz = C.c_int(), C.c_int()
+        self.L.orc_simple_csr_size(self.h, w, C.byref(n), C.byref(nnz))
+        rp = np.zeros(n.value + 1, np.int32)
+        ci = np.zeros(nnz.value, np.int32)
+        val = np.zeros(nnz.value)
+        rhs = np.zeros(n.value)
+        sol = np.zeros(n.value)
+        ij = np.zeros(2 * n.value, np.int32)
+        self.L.orc_simple_csr_get(self.h, w, _i(rp), _i(ci), _d(val), _d(rhs), _d(sol), _i(ij))
+        return dict(row_ptr=rp, col_idx=ci, values=val, rhs=rhs, sol=sol, ij=ij.reshape(-1, 2))
+
+    def residuals(self):
+        out = np.zeros(6)
+        self.L.orc_simple_residuals(self.h, _d(out))
+        return out
+
+    def face(self, kind, i, j):
+        k = ["alphaAtU", "alphaAtV", "gammaAtU", "gammaAtV", "southU", "northU", "westV", "eastV"].index(kind)
+        return self.L.orc_simple_face(self.h, k, i, j)
+
+    def sou(self, is_u, i, j, Fe, Fw, Fn, Fs):
+        return self.L.orc_simple_sou(self.h, 1 if is_u else 0, i, j, Fe, Fw, Fn, Fs)
+
+
+def jacobi_csr(types, n, rp, ci, val, b, x0, tol, maxit, prec=64):
+    L = lib(types)
+    x = np.array(x0, dtype=np.float64)
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    it = L.orc_jacobi_csr(prec, n, _i(rp), _i(ci), _d(val), _d(b), _d(x), tol, maxit)
+    return x, it
+
+
+def pcg_csr(types, n, rp, ci, val, b, x0, tol, maxit, prec=64):
+    L = lib(types)
+    x = np.array(x0, dtype=np.float64)
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    it = L.orc_pcg_csr(prec, n, _i(rp), _i(ci), _d(val), _d(b), _d(x), tol, maxit)
+    return x, it
+
+
+class OracleThermal:
+    """Reference thermal assembly (FP64) + CPU BiCGStab(Jacobi)."""
+
+    def __init__(self, types, stack, gamma, u, v):
+        self.L = lib(types)
+        nx, ny = stack["nx"], stack["ny"]
+        self.nx, self.ny, self.nz = nx, ny, len(stack["dz"])
+        self.keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                     (stack["dz"], gamma, u, v, stack["q0"])]
+        ch = np.ascontiguousarray(stack["channel"], dtype=np.int32)
+        self.keep.append(ch)
+        dz, g, uu, vv, q0 = self.keep[:5]
+        for a in (g, uu, vv, q0):
+            assert a.shape == (ny, nx)
+        self.h = self.L.orc_thermal_create(nx, ny, self.nz, stack.get("dx", 2e-5), stack.get("dy", 2e-5), _d(dz),
+                                           _i(ch), stack["ht"], stack["k_s"], stack["k_f"], stack["rho"],
+                                           stack["cp"], stack["t_in"], stack.get("k_mode", 1), stack["qk"],
+                                           _d(g), _d(uu), _d(vv), _d(q0))
+        self.t_in = stack["t_in"]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_thermal_destroy(self.h)
+            self.h = None
+
+    def csr(self):
+        n, nnz = C.c_int(), C.c_int()
+        self.L.orc_thermal_csr_size(self.h, C.byref(n), C.byref(nnz))
+        rp = np.zeros(n.value + 1, np.int32)
+        ci = np.zeros(nnz.value, np.int32)
+        val = np.zeros(nnz.value)
+        b = np.zeros(n.value)
+        self.L.orc_thermal_csr_get(self.h, _i(rp), _i(ci), _d(val), _d(b))
+        return rp, ci, val, b
+
+    def solve(self, rtol=1e-10, maxit=20000, T0=None):
+        T = np.full(self.nx * self.ny * self.nz, self.t_in) if T0 is None else np.array(T0, dtype=np.float64)
+        rr, secs = C.c_double(), C.c_double()
+        it = self.L.orc_thermal_solve(self.h, _d(T), rtol, maxit, C.byref(rr), C.byref(secs))
+        self.last_seconds = secs.value
+        self.last_relres = rr.value
+        self.last_iters = it
+        return T.reshape(self.nz, self.ny, self.nx)

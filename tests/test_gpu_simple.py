@@ -1,0 +1,192 @@
+"""GPU parity of the HIP SIMPLE path against the FP64 CPU oracle.
+
+All calls go through the C ABI (libcoldplate_hip.so via ctypes).  Tolerance:
+the FP64 device path follows the oracle's arithmetic step by step, so after a
+few SIMPLE iterations fields agree to round-off (rel. L2 <= 1e-10); at the
+converged state the bar is the north-star 1e-8.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(pkg, orc, cfg, gamma, inlet):
+    print(f"case nx={cfg.nx} ny={cfg.ny} sou={cfg.convection_scheme} 25d={cfg.enable_25d}", flush=True)
+    return pkg.SIMPLE(cfg, gamma, inlet), orc.OracleSimple(pkg._types, cfg, gamma, inlet, prec=64)
+
+
+@pytest.mark.parametrize("seed,nx,ny", [(1, 40, 20), (2, 64, 33), (5, 97, 41)])
+def test_density_field_steps_match_oracle(pkg, orc, seed, nx, ny):
+    cfg, gamma, inlet = pkg.configs.density_field(nx=nx, ny=ny, seed=seed, sigma=3.0)
+    cfg.pressure_solver = pkg._types.CPS_PRESSURE_PCG
+    cfg.max_pressure_iter = 300
+    cfg.p_tol = 1e-10
+    gpu, ref = _pair(pkg, orc, cfg, gamma, inlet)
+    for k in range(3):
+        sg, sr = gpu.step(2), ref.step(2)
+        assert sg["u_iters"] == sr["u_iters"] and sg["v_iters"] == sr["v_iters"]
+        assert abs(sg["p_iters"] - sr["p_iters"]) <= 1
+        for key in ("error", "resid_u", "resid_v", "resid_mass"):
+            assert sg[key] == pytest.approx(sr[key], rel=1e-9, abs=1e-300)
+        for name in ("u", "v", "p", "uStar", "vStar", "dE", "dN"):
+            assert rel_l2(gpu.get(name), ref.get(name)) < 1e-10, name
+    # boundary / inlet values are set by the same masks: bit-exact
+    assert np.array_equal(gpu.get("u")[:, 0], ref.get("u")[:, 0])
+    assert np.array_equal(gpu.get("gamma"), ref.get("gamma"))
+
+
+def test_fou_uniform_inlet_short_run(pkg, orc):
+    # p_tol far below the round-off of the solve, so the PCG stop iteration
+    # cannot flip between the two reduction orders
+    cfg, gamma, inlet = pkg.configs.channel_200x50(nx=60, ny=20, max_pressure_iter=600, p_tol=1e-14)
+    gpu, ref = _pair(pkg, orc, cfg, gamma, inlet)
+    sg, sr = gpu.step(10), ref.step(10)
+    assert sg["error"] == pytest.approx(sr["error"], rel=1e-9)
+    for name in ("u", "v", "p"):
+        assert rel_l2(gpu.get(name), ref.get(name)) < 1e-10, name
+
+
+def test_channel_200x50_run_to_epsilon(pkg, orc):
+    """Config: straight channel 200x50, FOU, Jacobi-PCG pTol 1e-8, stop at
+    residual 1e-6 (2000 cap).  Same stop iteration, dP and fields as the
+    oracle run (both stop at a state converged to ~1e-6)."""
+    cfg, gamma, inlet = pkg.configs.channel_200x50()
+    gpu, ref = _pair(pkg, orc, cfg, gamma, inlet)
+    sg = gpu.runIterations()
+    sr = ref.run()
+    assert sr["error"] <= cfg.epsilon and sg["error"] <= cfg.epsilon
+    assert abs(sg["iteration"] - sr["iterations"]) <= 2
+    for name in ("u", "v", "p"):
+        assert rel_l2(gpu.get(name), ref.get(name)) <= 1e-6, name
+    assert gpu.full_static_drop() == pytest.approx(ref.dp_static(), rel=1e-6)
+    # global mass balance: inflow == outflow
+    pin, pout = gpu.sampleBoundaryPatch(True), gpu.sampleBoundaryPatch(False)
+    assert pin["mass_flux"] == pytest.approx(pout["mass_flux"], rel=1e-4)
+
+
+def test_channel_200x50_fixed_point(pkg, orc):
+    """FP64 channel run driven towards its fixed point (1200 iterations, PCG
+    to round-off: pTol 0 so only the |pAp| < 1e-20 test stops it, momentum
+    tol 1e-13): the device path reproduces the oracle's u, v, p and dP to
+    rel. L2 1e-8.  (An absolute pTol above round-off lets the two reduction
+    orders stop PCG one iteration apart, which SIMPLE's slow pressure mode
+    then carries for hundreds of iterations.)"""
+    cfg, gamma, inlet = pkg.configs.channel_200x50(p_tol=0.0, max_pressure_iter=600, momentum_tol=1e-13)
+    gpu, ref = _pair(pkg, orc, cfg, gamma, inlet)
+    ref.step(1200)
+    gpu.step(1200)
+    for name in ("u", "v", "p"):
+        assert rel_l2(gpu.get(name), ref.get(name)) <= 1e-8, name
+    assert gpu.full_static_drop() == pytest.approx(ref.dp_static(), rel=1e-8)
+
+
+def test_channel_200x50_gmg_close_to_oracle(pkg, orc):
+    """Same run with the GMG-PCG pressure solver: the SIMPLE iteration has a
+    slow pressure-level mode on this geometry (relative change ~1e-9 per
+    100 iterations after 1200), so the two solvers' iterates agree to 1e-6,
+    not to round-off."""
+    cfg, gamma, inlet = pkg.configs.channel_200x50(p_tol=1e-12, max_pressure_iter=600, momentum_tol=1e-13)
+    ref = orc.OracleSimple(pkg._types, cfg, gamma, inlet, prec=64)
+    ref.step(1200)
+    for precision in (64, 32):
+        print(f"case channel gmg precision={precision}", flush=True)
+        gpu = pkg.SIMPLE(_with(cfg, pressure_solver=5, precision=precision, p_tol=0.0, p_rtol=1e-10,
+                               max_pressure_iter=40), gamma, inlet)
+        gpu.step(1200)
+        for name in ("u", "v", "p"):
+            assert rel_l2(gpu.get(name), ref.get(name)) <= 1e-6, name
+
+
+def test_csr_pressure_dropin_matches_oracle(pkg, orc):
+    cfg, gamma, inlet = pkg.configs.density_field(nx=50, ny=30, seed=7, sigma=3.0)
+    ref = orc.OracleSimple(pkg._types, cfg, gamma, inlet, prec=64)
+    ref.step(3)
+    A = ref.assemble("p")
+    n = len(A["rhs"])
+    x_ref, it_ref = orc.pcg_csr(pkg._types, n, A["row_ptr"], A["col_idx"], A["values"], A["rhs"], np.zeros(n),
+                                1e-9, 500, prec=32)
+    x = np.zeros(n, np.float32)
+    it = pkg.pressure_solve_csr(A["row_ptr"], A["col_idx"], A["values"], A["rhs"], x, 1e-9, 500)
+    assert abs(it - it_ref) <= max(3, 0.05 * it_ref)
+    # both solve the same system: compare with the FP64 dense solution
+    Ad = np.zeros((n, n))
+    for r in range(n):
+        for k in range(A["row_ptr"][r], A["row_ptr"][r + 1]):
+            Ad[r, A["col_idx"][k]] = A["values"][k]
+    xd = np.linalg.solve(Ad, A["rhs"])
+    assert rel_l2(x.astype(np.float64), xd) < 1e-4
+    assert rel_l2(x_ref, xd) < 1e-4
+
+
+def test_csr_jacobi_dropin_matches_oracle(pkg, orc):
+    cfg, gamma, inlet = pkg.configs.density_field(nx=50, ny=30, seed=8, sigma=3.0)
+    ref = orc.OracleSimple(pkg._types, cfg, gamma, inlet, prec=64)
+    ref.step(2)
+    A = ref.assemble("u")
+    n = len(A["rhs"])
+    x0 = A["sol"].astype(np.float32)
+    x_ref, it_ref = orc.jacobi_csr(pkg._types, n, A["row_ptr"], A["col_idx"], A["values"], A["rhs"], x0, 1e-7, 400,
+                                   prec=32)
+    x = x0.copy()
+    it = pkg.momentum_jacobi_csr(A["row_ptr"], A["col_idx"], A["values"], A["rhs"], x, 1e-7, 400)
+    assert abs(it - it_ref) <= 1
+    assert rel_l2(x.astype(np.float64), x_ref) < 1e-5
+
+
+def test_errors_are_reported(pkg):
+    cfg, gamma, inlet = pkg.configs.channel_200x50(nx=10, ny=5)
+    cfg.precision = 17
+    with pytest.raises(pkg._capi.CpsError):
+        pkg.SIMPLE(cfg, gamma, inlet)
+
+
+def _fou_density(pkg, nx=64, ny=32, seed=0):
+    # FOU keeps a true fixed point (the SOU upwind switch leaves a ~1e-8
+    # limit cycle); Brinkman, 2.5-D sink and the parabolic inlet stay on.
+    cfg, gamma, inlet = pkg.configs.density_field(nx=nx, ny=ny, seed=seed, sigma=3.0)
+    cfg.convection_scheme = 0
+    return cfg, gamma, inlet
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_gmg_converged_state_matches_oracle(pkg, orc, precision):
+    """GMG-PCG pressure (FP64, or FP32 inner + FP64 refinement) converges to
+    the oracle's fixed point (Jacobi-PCG) to rel. L2 1e-8: the converged
+    SIMPLE state does not depend on the inner solver or its precision."""
+    cfg, gamma, inlet = _fou_density(pkg)
+    ref = orc.OracleSimple(pkg._types, _with(cfg, max_pressure_iter=500, p_tol=1e-14), gamma, inlet, prec=64)
+    ref.step(700)
+    gc = _with(cfg, pressure_solver=5, precision=precision, p_tol=0.0, p_rtol=1e-9, max_pressure_iter=60)
+    print(f"case gmg precision={precision}", flush=True)
+    gpu = pkg.SIMPLE(gc, gamma, inlet)
+    st = gpu.step(700)
+    assert st["resid_u"] < 1e-11 and st["resid_v"] < 1e-11
+    for name in ("u", "v", "p"):
+        assert rel_l2(gpu.get(name), ref.get(name)) <= 1e-8, name
+    assert gpu.full_static_drop() == pytest.approx(ref.dp_static(), rel=1e-8)
+
+
+def _with(cfg, **kw):
+    c = cfg.copy()
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def test_gmg_sou_density_field_dp(pkg, orc):
+    """SOU density field: GMG path and oracle settle on the same dP to the
+    limit-cycle amplitude of the scheme (~1e-8 relative)."""
+    cfg, gamma, inlet = pkg.configs.density_field(nx=64, ny=32, seed=0, sigma=3.0)
+    ref = orc.OracleSimple(pkg._types, _with(cfg, max_pressure_iter=500, p_tol=1e-14), gamma, inlet, prec=64)
+    ref.step(400)
+    for precision in (64, 32):
+        print(f"case sou gmg precision={precision}", flush=True)
+        gpu = pkg.SIMPLE(_with(cfg, pressure_solver=5, precision=precision, p_tol=0.0, p_rtol=1e-9,
+                               max_pressure_iter=60), gamma, inlet)
+        gpu.step(400)
+        assert gpu.full_static_drop() == pytest.approx(ref.dp_static(), rel=1e-6)
+        for name in ("u", "v", "p"):
+            assert rel_l2(gpu.get(name), ref.get(name)) <= 1e-4, name
