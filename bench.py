@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""SIMPLE iteration throughput on one MI355X.
+
+Workload: the 2-D density-field solve at 1024x512 (dx = dy = 2e-5 m,
+projected random gamma with seed 0, Brinkman q = 0.1 / alpha_max = 1e8,
+2.5-D sink with Ht = 1e-4 m, SOU, parabolic inlet 0.5 m/s).  A step is one
+SIMPLE iteration (momentum assembly + Jacobi, pressure-correction solve,
+corrections) with every field resident in HBM.
+
+Two forms run the same algorithm and caps (GMG-PCG pressure, p_rtol 1e-3,
+12 PCG iterations, 20 Jacobi sweeps): FP32 inner iterations inside FP64
+state / residuals / refinement ("value"), and plain FP64.  The pressure drop
+of the two after warmup + steps iterations is compared.
+
+The kernel-level roofline line times the fine-grid multigrid smoother (the
+pressure stencil) with HIP events on the solver stream over the timed
+region.  The CPU baseline is the oracle's FP32 restatement of the
+reference CPU path (Jacobi-PCG pressure, 30 iterations, pTol 1e-6; Jacobi
+momentum) on a bounded sample of the same workload.
+
+usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SIMPLE iter/s @1024x512 FP32-refined vs FP64 at dP err<=1e-6; HBM %; grads/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def perf_config(pkg, nx, ny, precision):
+    cfg, gamma, inlet = pkg.configs.density_field(nx=nx, ny=ny, seed=0)
+    cfg.pressure_solver = pkg._types.CPS_PRESSURE_GMG
+    cfg.precision = precision
+    cfg.p_tol = 0.0
+    cfg.p_rtol = 1e-3
+    cfg.max_pressure_iter = 12
+    cfg.max_refine = 1
+    cfg.max_momentum_iter = 20
+    return cfg, gamma, inlet
+
+
+def run_form(pkg, nx, ny, precision, steps, warmup, kernel=None):
+    cfg, gamma, inlet = perf_config(pkg, nx, ny, precision)
+    s = pkg.SIMPLE(cfg, gamma, inlet)
+    if warmup:
+        s.step(warmup)
+    s.synchronize()
+    if kernel:
+        s.time_kernel(kernel)
+    t0 = time.perf_counter()
+    st = s.step(steps)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    kt = s.kernel_timing() if kernel else None
+    out = dict(it_s=steps / dt, seconds=dt, stats=st, dp=s.full_static_drop(), kernel=kt)
+    s.close()
+    return out
+
+
+def cpu_baseline(pkg, nx, ny, seconds):
+    """Oracle FP32 restatement of the reference CPU path, timed for about
+    `seconds` of host work (at least 3 SIMPLE iterations)."""
+    import __graft_entry__ as entry
+
+    orc = entry.load_oracle()
+    cfg, gamma, inlet = pkg.configs.density_field(nx=nx, ny=ny, seed=0)  # reference knobs: PCG, 30 its
+    o = orc.OracleSimple(pkg._types, cfg, gamma, inlet, prec=32)
+    o.step(1)  # first step builds maps / touches memory
+    n, spent = 0, 0.0
+    while spent < seconds or n < 3:
+        o.step(1)
+        spent += o.last_seconds
+        n += 1
+    return dict(value=n / spent, unit="SIMPLE it/s", cores=int(os.environ.get("OMP_NUM_THREADS", "0")) or None,
+                kind="port",
+                sample=f"{n} SIMPLE iterations of the {nx}x{ny} density field after 1 warm-up "
+                       f"(oracle FP32, Jacobi-PCG 30 its, Jacobi momentum), {spent:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--ny", type=int, default=512)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: the SIMPLE path runs one density field per device and "
+              "this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    if "OMP_NUM_THREADS" not in os.environ:
+        os.environ["OMP_NUM_THREADS"] = str(min(16, os.cpu_count() or 1))
+
+    import __graft_entry__ as entry
+
+    pkg = entry.load_package()
+    nx, ny = args.nx, args.ny
+    mixed = run_form(pkg, nx, ny, 32, args.steps, args.warmup, kernel="mg_smooth0")
+    fp64 = None if args.no_fp64 else run_form(pkg, nx, ny, 64, args.steps, args.warmup)
+
+    kt = mixed["kernel"]
+    roof = None
+    if kt and kt["launches"] > 0 and kt["avg_us"] > 0:
+        gbs = kt["bytes_per_launch"] / (kt["avg_us"] * 1e-6) / 1e9
+        roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_mg_jac<float> (fine-level pressure smoother)",
+                "avg_us": round(kt["avg_us"], 3), "launches": kt["launches"],
+                "bytes_per_launch": kt["bytes_per_launch"]}
+
+    cpu = None if args.no_cpu else cpu_baseline(pkg, nx, ny, args.cpu_seconds)
+
+    dp_err = None
+    if fp64 is not None and fp64["dp"] != 0:
+        dp_err = abs(mixed["dp"] - fp64["dp"]) / abs(fp64["dp"])
+    line = {
+        "metric": METRIC,
+        "value": round(mixed["it_s"], 2),
+        "unit": "SIMPLE it/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 / mixed["it_s"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-inner/f64-refined",
+        "data": "synthetic",
+        "config": {
+            "workload": f"2-D density field {nx}x{ny}: seeded projected random gamma, Brinkman q=0.1 "
+                        f"alpha_max=1e8, 2.5-D Ht=1e-4 m, SOU, parabolic inlet 0.5 m/s; GMG-PCG pressure "
+                        f"(p_rtol 1e-3, <=12 its), <=20 Jacobi sweeps",
+            "nx": nx, "ny": ny, "cells": nx * ny,
+        },
+        "fp64_it_s": None if fp64 is None else round(fp64["it_s"], 2),
+        "speedup_vs_fp64": None if fp64 is None else round(mixed["it_s"] / fp64["it_s"], 3),
+        "dp_pa": mixed["dp"],
+        "dp_fp64_pa": None if fp64 is None else fp64["dp"],
+        "dp_rel_err": dp_err,
+        "cell_updates_per_s": round(mixed["it_s"] * nx * ny, 1),
+        "grads_per_s": None,
+        "p_iters_last": mixed["stats"]["p_iters"],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -64,6 +64,8 @@ def lib():
         L.cps_simple_get_field.argtypes = [vp, C.c_int, dp]
         L.cps_simple_set_field.argtypes = [vp, C.c_int, dp]
         L.cps_simple_boundary_patch.argtypes = [vp, C.c_int, P(T.Patch)]
+        L.cps_simple_time_kernel.argtypes = [vp, C.c_int]
+        L.cps_simple_kernel_timing.argtypes = [vp, ip, dp, dp]
         _lib = L
         return L
 

@@ -36,18 +36,39 @@ constexpr double kTiny20 = (double)1e-20f;
 template <class T>
 struct Lv {  // device view of one level; cell (r,c) at off + r*pitch + c
   int nx, ny, pitch, off;
-  T *x, *y, *b, *ex, *ey, *dg;
+  T *x, *y, *b, *ex, *ey;
   __device__ size_t at(int r, int c) const { return (size_t)off + (size_t)r * pitch + c; }
 };
+
+// Face coefficients around cell k: ex(k) is the east face (the Dirichlet
+// outlet face in the last column), ey(k) the north face (0 in the last row).
+// The diagonal is their sum in the reference order aE + aW + aN + aS, so it
+// is never stored (pressure_solver.cpp:589-594; floor of :594).
+template <class T>
+struct Faces {
+  T e, w, n, s, d;
+};
+template <class T>
+__device__ inline Faces<T> faces(const Lv<T>& L, size_t k, int r, int c) {
+  Faces<T> f;
+  f.e = L.ex[k];
+  f.w = c > 0 ? L.ex[k - 1] : (T)0;
+  f.n = L.ey[k];
+  f.s = r > 0 ? L.ey[k - L.pitch] : (T)0;
+  T d = f.e + f.w + f.n + f.s;
+  f.d = d < (T)kTiny20 ? (T)1 : d;
+  return f;
+}
 
 template <class T>
 __device__ inline T apply_at(const Lv<T>& L, const T* x, int r, int c) {
   const size_t k = L.at(r, c);
-  T ax = L.dg[k] * x[k];
-  if (c + 1 < L.nx) ax -= L.ex[k] * x[k + 1];
-  if (c > 0) ax -= L.ex[k - 1] * x[k - 1];
-  if (r + 1 < L.ny) ax -= L.ey[k] * x[k + L.pitch];
-  if (r > 0) ax -= L.ey[k - L.pitch] * x[k - L.pitch];
+  const Faces<T> f = faces(L, k, r, c);
+  T ax = f.d * x[k];
+  if (c + 1 < L.nx) ax -= f.e * x[k + 1];
+  if (c > 0) ax -= f.w * x[k - 1];
+  if (r + 1 < L.ny) ax -= f.n * x[k + L.pitch];
+  if (r > 0) ax -= f.s * x[k - L.pitch];
   return ax;
 }
 
@@ -60,15 +81,10 @@ __global__ __launch_bounds__(MNT) void k_mg_build0(Geo g, const double* dE, cons
   const int i = r + 1, j = c + 1, M = g.M;
   const size_t f = (size_t)i * g.P + j;
   const double aE = g.rho * dE[f] * g.hy;
-  const double aW = (j > 1) ? g.rho * dE[f - 1] * g.hy : 0.0;
   const double aN = (i < M - 1) ? g.rho * dN[f] * g.hx : 0.0;
-  const double aS = (i > 1) ? g.rho * dN[f - g.P] * g.hx : 0.0;
-  double aP = aE + aW + aN + aS;
-  if (aP < kTiny20) aP = 1.0;
   const size_t k = L.at(r, c);
   L.ex[k] = (T)aE;
   L.ey[k] = (T)aN;
-  L.dg[k] = (T)aP;
 }
 
 // coarse faces: half the sum of the covered fine faces
@@ -92,14 +108,9 @@ template <class T>
 __global__ __launch_bounds__(MNT) void k_mg_coarsen(Lv<T> F, Lv<T> Cl) {
   const int C = blockIdx.x * MBX + threadIdx.x, R = blockIdx.y * MBY + threadIdx.y;
   if (R >= Cl.ny || C >= Cl.nx) return;
-  const T e = coarse_ex(F, R, C), n = coarse_ey(F, R, C);
-  T d = e + n;
-  if (C > 0) d += coarse_ex(F, R, C - 1);
-  if (R > 0) d += coarse_ey(F, R - 1, C);
   const size_t k = Cl.at(R, C);
-  Cl.ex[k] = e;
-  Cl.ey[k] = n;
-  Cl.dg[k] = d > (T)0 ? d : (T)1;
+  Cl.ex[k] = coarse_ex(F, R, C);
+  Cl.ey[k] = coarse_ey(F, R, C);
 }
 
 // first pre-smoothing sweep from a zero guess: y = omega b / diag
@@ -109,7 +120,7 @@ __global__ __launch_bounds__(MNT) void k_mg_jac0(Lv<T> L, const int* stop) {
   const int c = blockIdx.x * MBX + threadIdx.x, r = blockIdx.y * MBY + threadIdx.y;
   if (r >= L.ny || c >= L.nx) return;
   const size_t k = L.at(r, c);
-  L.x[k] = (T)kOmega * L.b[k] / L.dg[k];
+  L.x[k] = (T)kOmega * L.b[k] / faces(L, k, r, c).d;
 }
 
 // dst = xs + omega (b - A xs) / diag, xs = src (+ coarse correction of the
@@ -126,13 +137,14 @@ __global__ __launch_bounds__(MNT) void k_mg_jac(Lv<T> L, const T* src, T* dst, L
     return v;
   };
   const size_t k = L.at(r, c);
+  const Faces<T> f = faces(L, k, r, c);
   const T xc = xs(r, c);
-  T ax = L.dg[k] * xc;
-  if (c + 1 < L.nx) ax -= L.ex[k] * xs(r, c + 1);
-  if (c > 0) ax -= L.ex[k - 1] * xs(r, c - 1);
-  if (r + 1 < L.ny) ax -= L.ey[k] * xs(r + 1, c);
-  if (r > 0) ax -= L.ey[k - L.pitch] * xs(r - 1, c);
-  dst[k] = xc + (T)kOmega * (L.b[k] - ax) / L.dg[k];
+  T ax = f.d * xc;
+  if (c + 1 < L.nx) ax -= f.e * xs(r, c + 1);
+  if (c > 0) ax -= f.w * xs(r, c - 1);
+  if (r + 1 < L.ny) ax -= f.n * xs(r + 1, c);
+  if (r > 0) ax -= f.s * xs(r - 1, c);
+  dst[k] = xc + (T)kOmega * (L.b[k] - ax) / f.d;
 }
 
 // coarse rhs = sum of the fine residuals b - A x over the 2x2 aggregate
@@ -179,7 +191,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
       sb[q] = top.b[k];
       se[q] = top.ex[k];
       sn[q] = top.ey[k];
-      sd[q] = top.dg[k];
+      sd[q] = faces(top, k, r, c).d;
     }
   }
   __syncthreads();
@@ -334,12 +346,13 @@ __global__ __launch_bounds__(MNT) void k_mg_jac_dot(Grid0 G, Lv<T> L, const T* s
   L0_LOOP(G, L, r, c) {
     if (c >= L.nx) continue;
     const size_t k = L.at(r, c);
-    T ax = L.dg[k] * src[k];
-    if (c + 1 < L.nx) ax -= L.ex[k] * src[k + 1];
-    if (c > 0) ax -= L.ex[k - 1] * src[k - 1];
-    if (r + 1 < L.ny) ax -= L.ey[k] * src[k + L.pitch];
-    if (r > 0) ax -= L.ey[k - L.pitch] * src[k - L.pitch];
-    const T z = src[k] + (T)kOmega * (L.b[k] - ax) / L.dg[k];
+    const Faces<T> f = faces(L, k, r, c);
+    T ax = f.d * src[k];
+    if (c + 1 < L.nx) ax -= f.e * src[k + 1];
+    if (c > 0) ax -= f.w * src[k - 1];
+    if (r + 1 < L.ny) ax -= f.n * src[k + L.pitch];
+    if (r > 0) ax -= f.s * src[k - L.pitch];
+    const T z = src[k] + (T)kOmega * (L.b[k] - ax) / f.d;
     dst[k] = z;
     e += (double)L.b[k] * (double)z;
   }
@@ -381,12 +394,13 @@ __global__ __launch_bounds__(MNT) void k_pcg_ap(Grid0 G, Lv<T> L, const T* z, co
     if (c >= L.nx) continue;
     const size_t k = L.at(r, c);
     auto pv = [&](size_t q) -> T { return z[q] + beta * po[q]; };
+    const Faces<T> f = faces(L, k, r, c);
     const T pc = pv(k);
-    T ax = L.dg[k] * pc;
-    if (c + 1 < L.nx) ax -= L.ex[k] * pv(k + 1);
-    if (c > 0) ax -= L.ex[k - 1] * pv(k - 1);
-    if (r + 1 < L.ny) ax -= L.ey[k] * pv(k + L.pitch);
-    if (r > 0) ax -= L.ey[k - L.pitch] * pv(k - L.pitch);
+    T ax = f.d * pc;
+    if (c + 1 < L.nx) ax -= f.e * pv(k + 1);
+    if (c > 0) ax -= f.w * pv(k - 1);
+    if (r + 1 < L.ny) ax -= f.n * pv(k + L.pitch);
+    if (r > 0) ax -= f.s * pv(k - L.pitch);
     pn[k] = pc;
     Ap[k] = ax;
     e += (double)pc * (double)ax;
@@ -521,8 +535,8 @@ struct MGImpl {
   Geo g;
   struct Level {
     int nx, ny, pitch, off;
-    DevBuf<T> x, y, b, ex, ey, dg;
-    Lv<T> view() { return Lv<T>{nx, ny, pitch, off, x.p, y.p, b.p, ex.p, ey.p, dg.p}; }
+    DevBuf<T> x, y, b, ex, ey;
+    Lv<T> view() { return Lv<T>{nx, ny, pitch, off, x.p, y.p, b.p, ex.p, ey.p}; }
     dim3 grid() const { return dim3(ceil_div(nx, MBX), ceil_div(ny, MBY)); }
   };
   std::vector<std::unique_ptr<Level>> lv;  // grid-kernel levels 0 .. nGrid-1, lv[nGrid] = coarse top
@@ -534,6 +548,7 @@ struct MGImpl {
   DevBuf<PcgScal> scal;          // [0] inner, [1] outer
   Grid0 G0{};
   int coarseCells;
+  KernelTimer* timer = nullptr;
 
   MGImpl(Ctx* c, const Geo& geo) : ctx(c), g(geo) {
     coarseCells = sizeof(T) == 8 ? 1024 : 2048;
@@ -544,7 +559,7 @@ struct MGImpl {
     L0->pitch = g.P;
     L0->off = g.P + 1;
     const size_t S0 = (size_t)(g.M + 1) * g.P;
-    for (auto* b : {&L0->x, &L0->y, &L0->ex, &L0->ey, &L0->dg}) b->alloc(S0);
+    for (auto* b : {&L0->x, &L0->y, &L0->ex, &L0->ey}) b->alloc(S0);
     for (auto* b : {&r0, &xs, &pa, &pb, &Ap}) b->alloc(S0);
     lv.push_back(std::move(L0));
     // at least one grid level: its last post-smoothing sweep carries the
@@ -556,7 +571,7 @@ struct MGImpl {
       L->pitch = ((L->nx + 63) / 64) * 64;
       L->off = 0;
       const size_t S = (size_t)L->ny * L->pitch;
-      for (auto* b : {&L->x, &L->y, &L->b, &L->ex, &L->ey, &L->dg}) b->alloc(S);
+      for (auto* b : {&L->x, &L->y, &L->b, &L->ex, &L->ey}) b->alloc(S);
       lv.push_back(std::move(L));
     }
     nGrid = (int)lv.size() - 1;
@@ -575,7 +590,7 @@ struct MGImpl {
     cd.total = off;
     coarseSmem = (size_t)5 * off * sizeof(T);
     CPS_CHECK(coarseSmem <= 64 * 1024, CPS_ERR_STATE, "coarse multigrid levels exceed LDS");
-    for (auto* b : {&lv[0]->x, &lv[0]->y, &lv[0]->ex, &lv[0]->ey, &lv[0]->dg, &r0, &xs, &pa, &pb, &Ap})
+    for (auto* b : {&lv[0]->x, &lv[0]->y, &lv[0]->ex, &lv[0]->ey, &r0, &xs, &pa, &pb, &Ap})
       CPS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), ctx->stream));
     G0.gx = ceil_div(lv[0]->nx, MBX);
     G0.gy = std::max(1, std::min(ceil_div(lv[0]->ny, MBY), std::max(1, 1024 / G0.gx)));
@@ -610,7 +625,9 @@ struct MGImpl {
     for (int l = 0; l < nGrid; ++l) {
       Lv<T> L = view(l);
       k_mg_jac0<T><<<lv[l]->grid(), blk(), 0, s>>>(L, stop);
+      if (l == 0) timer->begin(KT_MG_SMOOTH0);
       k_mg_jac<T><<<lv[l]->grid(), blk(), 0, s>>>(L, L.x, L.y, none, 0, stop);
+      if (l == 0) timer->end(KT_MG_SMOOTH0);
       k_mg_restrict<T><<<lv[l + 1]->grid(), blk(), 0, s>>>(L, L.y, view(l + 1), stop);
     }
     k_mg_coarse<T><<<1, kCoarseThreads, coarseSmem, s>>>(view(nGrid), cd, stop);
@@ -628,6 +645,11 @@ struct MGImpl {
   // flag every sync_every iterations.
   void pcg(const double* rhs, double abs_tol, double rtol, int maxit, int sync_every, const int* outer_stop) {
     hipStream_t s = ctx->stream;
+    const double nP = (double)lv[0]->nx * (double)lv[0]->ny;
+    // smoother: x, b, east and north faces read, x' written; PCG stencil:
+    // z, p_old, faces read, p and Ap written
+    if (timer->which == KT_MG_SMOOTH0) timer->bytes_per_launch = 5.0 * sizeof(T) * nP;
+    if (timer->which == KT_PCG_STENCIL) timer->bytes_per_launch = 6.0 * sizeof(T) * nP;
     const Lv<T> L = view(0);
     PcgScal* ps = scal.p;
     k_pcg_start<T, double><<<grid0(), blk(), 0, s>>>(G0, L, rhs, xs.p, r0.p, ps, outer_stop, part.p);
@@ -637,7 +659,9 @@ struct MGImpl {
       vcycle(&ps->stop);
       T* po = (it & 1) ? pb.p : pa.p;
       T* pn = (it & 1) ? pa.p : pb.p;
+      timer->begin(KT_PCG_STENCIL);
       k_pcg_ap<T><<<grid0(), blk(), 0, s>>>(G0, L, lv[0]->y.p, po, pn, Ap.p, it, abs_tol, rtol, ps, part.p);
+      timer->end(KT_PCG_STENCIL);
       k_pcg_xr<T><<<grid0(), blk(), 0, s>>>(G0, L, xs.p, r0.p, pn, Ap.p, it, ps, part.p);
       CPS_HIP(hipGetLastError());
       if (sync_every > 0 && (it + 1) % sync_every == 0 && it + 1 < maxit) {
@@ -666,13 +690,14 @@ PressureMG::~PressureMG() {
 static constexpr int kNoSyncCap = 32;
 
 void PressureMG::solve(const double* dE, const double* dN, double* r, double* x, IterState* st,
-                       const cps_simple_config& cfg, bool fp64) {
+                       const cps_simple_config& cfg, bool fp64, KernelTimer* timer) {
   hipStream_t s = ctx_->stream;
   const int maxit = std::max(0, cfg.max_pressure_iter);
   const int sync_every = maxit > kNoSyncCap ? 4 : 0;
   if (fp64) {
     if (!impl64_) impl64_ = new MGImpl<double>(ctx_, g_);
     auto* m = static_cast<MGImpl<double>*>(impl64_);
+    m->timer = timer;
     m->setup(dE, dN);
     m->pcg(r, cfg.p_tol, cfg.p_rtol, maxit, sync_every, nullptr);
     k_copy_out<double><<<m->lv[0]->grid(), dim3(MBX, MBY), 0, s>>>(m->view(0), m->xs.p, x, g_.P);
@@ -687,6 +712,7 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
       b64_ = b;
     }
     double* b64 = static_cast<DevBuf<double>*>(b64_)->p;
+    m->timer = timer;
     m->setup(dE, dN);
     CPS_HIP(hipMemcpyAsync(b64, r, S * sizeof(double), hipMemcpyDeviceToDevice, s));
     PcgScal* inner = m->scal.p;

@@ -19,7 +19,7 @@ class PressureMG {
   // (mixed) ; x is zero on entry.  FP64: MG-PCG in double.  Mixed: FP64
   // iterative refinement around an FP32 MG-PCG.
   void solve(const double* dE, const double* dN, double* r, double* x, IterState* st, const cps_simple_config& cfg,
-             bool fp64);
+             bool fp64, KernelTimer* timer);
 
  private:
   Ctx* ctx_;

@@ -676,6 +676,7 @@ class SimpleSolver {
   DevBuf<double> part;  // SLOT_COUNT x NB block partials
   DevBuf<IterState> st;
   std::unique_ptr<PressureMG> mg;
+  KernelTimer timer;
   int iteration = 0;
   dim3 grid, block;
   int* hflag = nullptr;
@@ -720,6 +721,7 @@ class SimpleSolver {
 
     for (auto* b : {&u, &v, &p, &us, &vs, &ps, &dE, &dN, &gam, &alp}) b->alloc(S);
     inlet.alloc(M + 1);
+    timer.stream = ctx->stream;
     for (auto* b : {&u, &v, &p, &us, &vs, &ps, &dE, &dN, &gam, &alp})
       CPS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), ctx->stream));
     if (mode_fp64) {
@@ -811,11 +813,16 @@ class SimpleSolver {
     while (k < maxit) {
       const int k1 = std::min(maxit, k + chunk);
       for (; k < k1; ++k) {
+        timer.begin(KT_MOM_JACOBI);
         if (mode_fp64)
           k_jacobi<double><<<grid, block, 0, s>>>(g, sys(momD, 0), sys(momD, 1), k, tol2, st.p, part.p);
         else
           k_jacobi<float><<<grid, block, 0, s>>>(g, sys(momF, 0), sys(momF, 1), k, tol2, st.p, part.p);
+        timer.end(KT_MOM_JACOBI);
       }
+      // x, cS, cW, aP, cE, cN, b read + x' written, both components
+      if (timer.which == KT_MOM_JACOBI)
+        timer.bytes_per_launch = 8.0 * (mode_fp64 ? 8 : 4) * (double)(g.nU + g.nV);
       k_jac_end<<<1, block, 0, s>>>(g, k, tol2, st.p, part.p);
       CPS_HIP(hipGetLastError());
       if (k >= maxit) break;
@@ -834,6 +841,8 @@ class SimpleSolver {
 
   void pressure_pcg() {
     hipStream_t s = ctx->stream;
+    // z, p_old, dE, dN read; p, Ap written
+    if (timer.which == KT_PCG_STENCIL) timer.bytes_per_launch = 6.0 * 8.0 * (double)g.nP;
     const int maxit = cfg.max_pressure_iter;
     const int chunk = maxit > kNoSyncCap ? 16 : std::max(1, maxit);
     const PcgBufs q = pbufs();
@@ -841,7 +850,9 @@ class SimpleSolver {
     while (k < maxit) {
       const int k1 = std::min(maxit, k + chunk);
       for (; k < k1; ++k) {
+        timer.begin(KT_PCG_STENCIL);
         k_pcg_a<<<grid, block, 0, s>>>(g, dE.p, dN.p, q, k, cfg.p_tol, st.p, part.p);
+        timer.end(KT_PCG_STENCIL);
         k_pcg_b<<<grid, block, 0, s>>>(g, ps.p, q, k, st.p, part.p);
       }
       k_pcg_end<<<1, block, 0, s>>>(g, k, cfg.p_tol, st.p, part.p);
@@ -975,7 +986,7 @@ class SimpleSolver {
 void SimpleSolver::mg_create() { mg.reset(new PressureMG(ctx, g)); }
 void SimpleSolver::pressure_mg() {
   const PcgBufs q = pbufs();
-  mg->solve(dE.p, dN.p, q.r, ps.p, st.p, cfg, mode_fp64 != 0);
+  mg->solve(dE.p, dN.p, q.r, ps.p, st.p, cfg, mode_fp64 != 0, &timer);
 }
 
 }  // namespace cps
@@ -1058,6 +1069,25 @@ int cps_simple_set_field(cps_simple* s, int field, const double* in) {
   return guard([&] {
     CPS_CHECK(s, CPS_ERR_ARG, "null solver");
     s->s->set(field, in);
+  });
+}
+
+int cps_simple_time_kernel(cps_simple* s, int kernel) {
+  return guard([&] {
+    CPS_CHECK(s, CPS_ERR_ARG, "null solver");
+    CPS_CHECK(kernel >= -1 && kernel <= 2, CPS_ERR_ARG, "unknown kernel id");
+    s->s->timer.reset(kernel);
+  });
+}
+
+int cps_simple_kernel_timing(cps_simple* s, int* launches, double* avg_us, double* bytes_per_launch) {
+  return guard([&] {
+    CPS_CHECK(s, CPS_ERR_ARG, "null solver");
+    cps::KernelTimer& t = s->s->timer;
+    t.drain();
+    if (launches) *launches = (int)t.launches;
+    if (avg_us) *avg_us = t.launches ? 1e3 * t.total_ms / (double)t.launches : 0.0;
+    if (bytes_per_launch) *bytes_per_launch = t.bytes_per_launch;
   });
 }
 

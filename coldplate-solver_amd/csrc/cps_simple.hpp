@@ -4,6 +4,8 @@
 #include <memory>
 #include <vector>
 
+#include <hip/hip_runtime.h>
+
 #include "cps_common.hpp"
 
 namespace cps {
@@ -55,6 +57,58 @@ struct Geo {
 
 class SimpleSolver;
 class PressureMG;
+
+// Kernels whose launches can be timed with HIP events on the solver stream.
+enum TimedKernel {
+  KT_NONE = -1,
+  KT_MG_SMOOTH0 = 0,   // fine-level multigrid Jacobi smoother sweep (pressure)
+  KT_MOM_JACOBI = 1,   // fused U+V momentum Jacobi sweep
+  KT_PCG_STENCIL = 2,  // fine-level PCG p-update + 5-point apply
+};
+
+// Brackets the launches of one kernel with events; drain() sums them.
+struct KernelTimer {
+  int which = KT_NONE;
+  hipStream_t stream = nullptr;
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+  size_t launches = 0;
+  double total_ms = 0.0;
+  double bytes_per_launch = 0.0;
+  void mark(int id) {
+    if (which != id) return;
+    if (used == ev.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      ev.push_back(e);
+    }
+    (void)hipEventRecord(ev[used++], stream);
+  }
+  void begin(int id) { mark(id); }
+  void end(int id) {
+    if (which != id) return;
+    mark(id);
+    launches++;
+  }
+  // adds the elapsed time of the recorded (begin, end) pairs to total_ms
+  void drain() {
+    if (used >= 2) (void)hipEventSynchronize(ev[used - 1]);
+    for (size_t k = 0; k + 1 < used; k += 2) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) == hipSuccess) total_ms += ms;
+    }
+    used = 0;
+  }
+  void reset(int id) {
+    drain();
+    which = id;
+    launches = 0;
+    total_ms = 0.0;
+  }
+  ~KernelTimer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+};
 
 }  // namespace cps
 

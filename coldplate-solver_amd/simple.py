@@ -90,6 +90,22 @@ class SIMPLE:
         _capi.check(self.L.cps_simple_boundary_patch(self.h, 1 if inlet else 0, C.byref(m)))
         return m.as_dict()
 
+    # ---- measurement -----------------------------------------------------
+    KERNELS = {"mg_smooth0": 0, "mom_jacobi": 1, "pcg_stencil": 2}
+
+    def time_kernel(self, name):
+        """Bracket every launch of one kernel with HIP events on the solver
+        stream (None stops timing); clears the counters."""
+        _capi.check(self.L.cps_simple_time_kernel(self.h, -1 if name is None else self.KERNELS[name]))
+
+    def kernel_timing(self):
+        n, us, by = C.c_int(), C.c_double(), C.c_double()
+        _capi.check(self.L.cps_simple_kernel_timing(self.h, C.byref(n), C.byref(us), C.byref(by)))
+        return {"launches": n.value, "avg_us": us.value, "bytes_per_launch": by.value}
+
+    def synchronize(self):
+        self.ctx.synchronize()
+
     def full_static_drop(self):
         return self.sampleBoundaryPatch(True)["avg_static"] - self.sampleBoundaryPatch(False)["avg_static"]
 

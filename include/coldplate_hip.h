@@ -146,6 +146,17 @@ int cps_simple_set_field(cps_simple* s, int field, const double* in);
 /* inlet = 1: x = 0 patch, 0: x = L patch. */
 int cps_simple_boundary_patch(cps_simple* s, int inlet, cps_patch* out);
 
+/* Measurement hook (no reference counterpart): brackets every launch of one
+ * kernel of the step with HIP events on the solver stream.  kernel:
+ * CPS_KT_*; -1 stops timing.  Selecting a kernel clears the counters. */
+#define CPS_KT_MG_SMOOTH0 0   /* fine-level multigrid Jacobi smoother (pressure) */
+#define CPS_KT_MOM_JACOBI 1   /* fused U+V momentum Jacobi sweep */
+#define CPS_KT_PCG_STENCIL 2  /* fine-level PCG p-update + 5-point apply */
+int cps_simple_time_kernel(cps_simple* s, int kernel);
+/* launches timed so far, their mean duration [us] and the algorithmic bytes
+ * one launch moves (every operand read once, every result written once). */
+int cps_simple_kernel_timing(cps_simple* s, int* launches, double* avg_us, double* bytes_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -106,10 +106,13 @@ def test_csr_pressure_dropin_matches_oracle(pkg, orc):
     ref.step(3)
     A = ref.assemble("p")
     n = len(A["rhs"])
+    # absolute tolerance well above FP32 round-off of this system, so both
+    # FP32 solvers stop on the same rule rather than on stagnation
+    tol = 1e-5 * float(np.linalg.norm(A["rhs"]))
     x_ref, it_ref = orc.pcg_csr(pkg._types, n, A["row_ptr"], A["col_idx"], A["values"], A["rhs"], np.zeros(n),
-                                1e-9, 500, prec=32)
+                                tol, 2000, prec=32)
     x = np.zeros(n, np.float32)
-    it = pkg.pressure_solve_csr(A["row_ptr"], A["col_idx"], A["values"], A["rhs"], x, 1e-9, 500)
+    it = pkg.pressure_solve_csr(A["row_ptr"], A["col_idx"], A["values"], A["rhs"], x, tol, 2000)
     assert abs(it - it_ref) <= max(3, 0.05 * it_ref)
     # both solve the same system: compare with the FP64 dense solution
     Ad = np.zeros((n, n))
