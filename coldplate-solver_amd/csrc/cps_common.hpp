@@ -151,6 +151,28 @@ __device__ inline double parts_max(const double* p, int n, double* sh) {
   __syncthreads();
   return a;
 }
+// Two-phase form of parts_sum for latency hiding: parts_load issues the
+// loads of this thread's partials (n <= 2 * NT) early; parts_total (after
+// any other loads have been issued) reduces them.  Same fixed order as
+// parts_sum: thread t holds p[t] + p[t + NT].
+template <int NT>
+__device__ inline double parts_load(const double* p, int n) {
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+  double a = t < n ? p[t] : 0.0;
+  if (t + NT < n) a += p[t + NT];
+  return a;
+}
+template <int NT>
+__device__ inline double parts_total(double a, double* sh) {
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+  a = block_sum<NT>(a, sh);
+  if (t == 0) sh[NT / kWave] = a;
+  __syncthreads();
+  a = sh[NT / kWave];
+  __syncthreads();
+  return a;
+}
+
 // Reads one int of device state once per block (uniform within the block).
 __device__ inline int block_read(const volatile int* p, int* sh) {
   const int t = threadIdx.x + threadIdx.y * blockDim.x;

@@ -113,53 +113,194 @@ __global__ __launch_bounds__(MNT) void k_mg_coarsen(Lv<T> F, Lv<T> Cl) {
   Cl.ey[k] = coarse_ey(F, R, C);
 }
 
-// first pre-smoothing sweep from a zero guess: y = omega b / diag
-template <class T>
-__global__ __launch_bounds__(MNT) void k_mg_jac0(Lv<T> L, const int* stop) {
-  if (*stop) return;
-  const int c = blockIdx.x * MBX + threadIdx.x, r = blockIdx.y * MBY + threadIdx.y;
-  if (r >= L.ny || c >= L.nx) return;
-  const size_t k = L.at(r, c);
-  L.x[k] = (T)kOmega * L.b[k] / faces(L, k, r, c).d;
-}
+// ---------------------------------------------------------------------------
+// Fused V-cycle level kernels.  A 256-thread block owns a TX x TY tile and
+// stages b and the faces of the tile plus a halo of HL cells in LDS, so the
+// two Jacobi sweeps, the residual and the restriction (down) or the
+// prolongation and the two sweeps (up) of one level run in one launch.
+struct PcgScal {
+  double rzs[2], rr0, rr, tol2;
+  int stop, iters, pad_[2];
+};
 
-// dst = xs + omega (b - A xs) / diag, xs = src (+ coarse correction of the
-// parent aggregate when Cl.y != nullptr: fused prolongation)
-template <class T>
-__global__ __launch_bounds__(MNT) void k_mg_jac(Lv<T> L, const T* src, T* dst, Lv<T> Cl, int add_coarse,
-                                               const int* stop) {
-  if (*stop) return;
-  const int c = blockIdx.x * MBX + threadIdx.x, r = blockIdx.y * MBY + threadIdx.y;
-  if (r >= L.ny || c >= L.nx) return;
-  auto xs = [&](int rr, int cc) -> T {
-    T v = src[L.at(rr, cc)];
-    if (add_coarse) v += Cl.y[Cl.at(rr >> 1, cc >> 1)];
-    return v;
-  };
-  const size_t k = L.at(r, c);
-  const Faces<T> f = faces(L, k, r, c);
-  const T xc = xs(r, c);
-  T ax = f.d * xc;
-  if (c + 1 < L.nx) ax -= f.e * xs(r, c + 1);
-  if (c > 0) ax -= f.w * xs(r, c - 1);
-  if (r + 1 < L.ny) ax -= f.n * xs(r + 1, c);
-  if (r > 0) ax -= f.s * xs(r - 1, c);
-  dst[k] = xc + (T)kOmega * (L.b[k] - ax) / f.d;
-}
+// Convergence test carried by the first kernel of a V-cycle (fine level,
+// iteration it > 0): partial ||r||^2 of the previous update.
+struct PcgTest {
+  const double* part;  // nullptr: no test
+  int nb, it;
+  PcgScal* ps;
+  double abs_tol, rtol;  // it == 0: the target max(abs_tol, rtol ||r0||) is set here
+};
 
-// coarse rhs = sum of the fine residuals b - A x over the 2x2 aggregate
+// One thread per tile cell: blocks of TX x TY = 64 x 16 threads.
+constexpr int TX = 64, TY = 16, HL = 3, RW = TX + 2 * HL, RH = TY + 2 * HL, RN = RW * RH;
+constexpr int FT = TX * TY;
+static_assert(TX == MBX && (TY / 2) * (TX / 2) <= FT, "tile shape");
+
 template <class T>
-__global__ __launch_bounds__(MNT) void k_mg_restrict(Lv<T> L, const T* x, Lv<T> Cl, const int* stop) {
-  if (*stop) return;
-  const int C = blockIdx.x * MBX + threadIdx.x, R = blockIdx.y * MBY + threadIdx.y;
-  if (R >= Cl.ny || C >= Cl.nx) return;
-  T s = (T)0;
-  for (int dr = 0; dr < 2; ++dr)
-    for (int dc = 0; dc < 2; ++dc) {
-      const int r = 2 * R + dr, c = 2 * C + dc;
-      if (r < L.ny && c < L.nx) s += L.b[L.at(r, c)] - apply_at(L, x, r, c);
+struct Tile {
+  T* sb;
+  T* se;
+  T* sn;
+  int r0, c0, nx, ny;
+  __device__ int gr(int lr) const { return r0 - HL + lr; }
+  __device__ int gc(int lc) const { return c0 - HL + lc; }
+  __device__ bool inside(int r, int c) const { return r >= 0 && r < ny && c >= 0 && c < nx; }
+  // diagonal = sum of the four faces (needs lr, lc >= 1)
+  __device__ T diag(int q, int r, int c) const {
+    T d = se[q] + (c > 0 ? se[q - 1] : (T)0) + sn[q] + (r > 0 ? sn[q - RW] : (T)0);
+    return d < (T)kTiny20 ? (T)1 : d;
+  }
+  __device__ T ax(const T* x, int q, int r, int c, T d) const {
+    T a = d * x[q];
+    if (c + 1 < nx) a -= se[q] * x[q + 1];
+    if (c > 0) a -= se[q - 1] * x[q - 1];
+    if (r + 1 < ny) a -= sn[q] * x[q + RW];
+    if (r > 0) a -= sn[q - RW] * x[q - RW];
+    return a;
+  }
+  // one damped-Jacobi sweep src -> dst on the region ring [h, RH-h) x [h, RW-h)
+  __device__ void sweep(const T* src, T* dst, int h, int t) const {
+    for (int q = t; q < RN; q += FT) {
+      const int lr = q / RW, lc = q - (q / RW) * RW;
+      if (lr < h || lr >= RH - h || lc < h || lc >= RW - h) continue;
+      const int r = gr(lr), c = gc(lc);
+      if (inside(r, c)) {
+        const T d = diag(q, r, c);
+        dst[q] = src[q] + (T)kOmega * (sb[q] - ax(src, q, r, c, d)) / d;
+      } else {
+        dst[q] = (T)0;
+      }
     }
-  Cl.b[Cl.at(R, C)] = s;
+  }
+};
+
+// Stage b, the faces and (up kernels) x1 + P e of the tile + halo in LDS.
+// All global loads of a thread are issued before the first LDS store.
+template <class T>
+__device__ inline void tile_load(const Lv<T>& L, const Tile<T>& tl, int t, T* sx, const Lv<T>* Cl) {
+  constexpr int NI = (RN + FT - 1) / FT;
+  T vb[NI], ve[NI], vn[NI], vx[NI];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int q = t + i * FT;
+    const int lr = q / RW, lc = q - (q / RW) * RW;
+    const int r = tl.gr(lr), c = tl.gc(lc);
+    vb[i] = ve[i] = vn[i] = vx[i] = (T)0;
+    if (q < RN && tl.inside(r, c)) {
+      const size_t k = L.at(r, c);
+      vb[i] = L.b[k];
+      ve[i] = L.ex[k];
+      vn[i] = L.ey[k];
+      if (sx) vx[i] = L.x[k] + Cl->y[Cl->at(r >> 1, c >> 1)];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int q = t + i * FT;
+    if (q < RN) {
+      tl.sb[q] = vb[i];
+      tl.se[q] = ve[i];
+      tl.sn[q] = vn[i];
+      if (sx) sx[q] = vx[i];
+    }
+  }
+}
+
+// down: x1 = two Jacobi sweeps from zero (stored in L.x), coarse b = sum of
+// the residuals b - A x1 over each 2x2 aggregate
+template <class T>
+__global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* stop, PcgTest test) {
+  __shared__ T sb[RN], se[RN], sn[RN], s0[RN], s1[RN];
+  __shared__ double sh[FT / kWave + 1];
+  __shared__ int sflag;
+  const int t = threadIdx.x + threadIdx.y * TX;
+  const int stop0 = t == 0 ? *stop : 0;  // thread 0's view, published below
+  const double perr = test.part ? parts_load<FT>(test.part, test.nb) : 0.0;
+  Tile<T> tl{sb, se, sn, (int)blockIdx.y * TY, (int)blockIdx.x * TX, L.nx, L.ny};
+  tile_load<T>(L, tl, t, nullptr, nullptr);
+  if (t == 0) sflag = stop0;
+  __syncthreads();
+  if (sflag) return;  // block-uniform
+  if (test.part) {    // fine level: stop test on ||r|| of the previous PCG update
+    const double err = parts_total<FT>(perr, sh);
+    const bool b0 = blockIdx.x == 0 && blockIdx.y == 0 && t == 0;
+    double tol2;
+    if (test.it == 0) {  // err = ||r0||^2
+      const double tgt = fmax(test.abs_tol, test.rtol * sqrt(err));
+      tol2 = tgt * tgt;
+      if (b0) { test.ps->rr0 = err; test.ps->tol2 = tol2; }
+    } else {
+      tol2 = test.ps->tol2;
+    }
+    if (b0) { test.ps->rr = err; test.ps->iters = test.it; }
+    if (err <= tol2) {
+      if (b0) test.ps->stop = 1;
+      return;
+    }
+  }
+  __syncthreads();
+  // first sweep from a zero guess on the halo-2 ring: x0 = omega b / diag
+  for (int q = t; q < RN; q += FT) {
+    const int lr = q / RW, lc = q - (q / RW) * RW;
+    if (lr < 1 || lr >= RH - 1 || lc < 1 || lc >= RW - 1) continue;
+    const int r = tl.gr(lr), c = tl.gc(lc);
+    s0[q] = tl.inside(r, c) ? (T)kOmega * sb[q] / tl.diag(q, r, c) : (T)0;
+  }
+  __syncthreads();
+  tl.sweep(s0, s1, 2, t);  // x1 on the halo-1 ring
+  __syncthreads();
+  {  // residual of the tile cell, x1 to memory
+    const int lr = HL + threadIdx.y, lc = HL + threadIdx.x, q = lr * RW + lc;
+    const int r = tl.gr(lr), c = tl.gc(lc);
+    if (tl.inside(r, c)) {
+      s0[q] = sb[q] - tl.ax(s1, q, r, c, tl.diag(q, r, c));
+      L.x[L.at(r, c)] = s1[q];
+    } else {
+      s0[q] = (T)0;
+    }
+  }
+  __syncthreads();
+  const int cr = t / (TX / 2), cc = t - (t / (TX / 2)) * (TX / 2);
+  const int R = tl.r0 / 2 + cr, C = tl.c0 / 2 + cc;
+  if (t < (TY / 2) * (TX / 2) && R < Cl.ny && C < Cl.nx) {
+    const int q = (HL + 2 * cr) * RW + HL + 2 * cc;
+    Cl.b[Cl.at(R, C)] = (s0[q] + s0[q + 1]) + (s0[q + RW] + s0[q + RW + 1]);
+  }
+}
+
+// up: x2 = x1 + P e (e = the coarse level's result in Cl.y), two Jacobi
+// sweeps, result in L.y; with part != nullptr also the partial b . z of the
+// block (the PCG's r . z on the fine level)
+template <class T>
+__global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, const int* stop) {
+  __shared__ T sb[RN], se[RN], sn[RN], s0[RN], s1[RN];
+  __shared__ int sflag;
+  const int t = threadIdx.x + threadIdx.y * TX;
+  const int stop0 = t == 0 ? *stop : 0;
+  Tile<T> tl{sb, se, sn, (int)blockIdx.y * TY, (int)blockIdx.x * TX, L.nx, L.ny};
+  tile_load<T>(L, tl, t, s0, &Cl);
+  if (t == 0) sflag = stop0;
+  __syncthreads();
+  if (sflag) return;
+  tl.sweep(s0, s1, 2, t);  // halo-1 ring
+  __syncthreads();
+  double e = 0.0;
+  {
+    const int lr = HL + threadIdx.y, lc = HL + threadIdx.x, q = lr * RW + lc;
+    const int r = tl.gr(lr), c = tl.gc(lc);
+    if (tl.inside(r, c)) {
+      const T d = tl.diag(q, r, c);
+      const T z = s1[q] + (T)kOmega * (sb[q] - tl.ax(s1, q, r, c, d)) / d;
+      L.y[L.at(r, c)] = z;
+      e += (double)sb[q] * (double)z;
+    }
+  }
+  if (part) {
+    __shared__ double sh[FT / kWave + 1];
+    const double s = block_sum<FT>(e, sh);
+    if (t == 0) part[blockIdx.x + blockIdx.y * gridDim.x] = s;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -172,152 +313,189 @@ struct CoarseDesc {
   int total;
 };
 
+#define CELLS(l)                                              \
+  for (int r = ty; r < d.ny[l]; r += kCoarseThreads / 64)     \
+    for (int c = tx; c < d.nx[l]; c += 64)
+
+// Once per pressure solve: faces and inverse diagonals of every level of the
+// coarse solver, built from the top level's faces (half the sum of the
+// covered fine faces), stored as cf[0 .. 3 total) = east, north, 1/diag.
 template <class T>
-__global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseDesc d, const int* stop) {
+__global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, CoarseDesc d, T* cf) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  T* se = reinterpret_cast<T*>(smem_raw);
+  T* sn = se + d.total;
+  const int t = threadIdx.x, tx = t & 63, ty = t >> 6;
+  CELLS(0) {
+    const size_t k = top.at(r, c);
+    se[r * d.nx[0] + c] = top.ex[k];
+    sn[r * d.nx[0] + c] = top.ey[k];
+  }
+  __syncthreads();
+  for (int l = 1; l < d.nlev; ++l) {
+    const int fx = d.nx[l - 1], fy = d.ny[l - 1], fo = d.off[l - 1], co = d.off[l], cx = d.nx[l];
+    CELLS(l) {
+      const int col = min(2 * c + 1, fx - 1), row = 2 * r + 1;
+      T e = se[fo + 2 * r * fx + col];
+      if (2 * r + 1 < fy) e += se[fo + (2 * r + 1) * fx + col];
+      T n = (T)0;
+      if (row <= fy - 1) {
+        n = sn[fo + row * fx + 2 * c];
+        if (2 * c + 1 < fx) n += sn[fo + row * fx + 2 * c + 1];
+      }
+      se[co + r * cx + c] = (T)kCoarseScale * e;
+      sn[co + r * cx + c] = (T)kCoarseScale * n;
+    }
+    __syncthreads();
+  }
+  for (int l = 0; l < d.nlev; ++l) {
+    const int o = d.off[l], nx = d.nx[l];
+    CELLS(l) {
+      const int k = o + r * nx + c;
+      T dd = se[k] + (c > 0 ? se[k - 1] : (T)0) + sn[k] + (r > 0 ? sn[k - nx] : (T)0);
+      dd = dd < (T)kTiny20 ? (T)1 : dd;
+      cf[k] = se[k];
+      cf[d.total + k] = sn[k];
+      cf[2 * d.total + k] = (T)1 / dd;
+    }
+  }
+}
+
+// All levels from `top` down inside one workgroup, in LDS: per level one
+// pre-smoothing launch-free phase (both sweeps from zero fused: x1 =
+// x0 + w (b - A x0)/d with x0 = w b/d formed on the fly), one restriction
+// phase, and two post-smoothing phases (the first with the prolongation
+// fused).  The coarsest level (one cell) is solved exactly in FP64.  The top
+// level's correction goes to top.y.
+template <class T>
+__global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseDesc d, const T* cf, const int* stop) {
   if (*stop) return;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   T* sx = reinterpret_cast<T*>(smem_raw);
-  T* sb = sx + d.total;
+  T* sy = sx + d.total;
+  T* sb = sy + d.total;
   T* se = sb + d.total;
   T* sn = se + d.total;
-  T* sd = sn + d.total;
-  const int t = threadIdx.x;
-  // load top level
-  {
-    const int n = d.nx[0] * d.ny[0];
-    for (int q = t; q < n; q += kCoarseThreads) {
-      const int r = q / d.nx[0], c = q % d.nx[0];
-      const size_t k = top.at(r, c);
-      sb[q] = top.b[k];
-      se[q] = top.ex[k];
-      sn[q] = top.ey[k];
-      sd[q] = faces(top, k, r, c).d;
+  T* si = sn + d.total;  // 1 / diagonal
+  const int t = threadIdx.x, tx = t & 63, ty = t >> 6;
+  {  // every global load of the thread is issued before the LDS stores
+    constexpr int NQ = 3;   // d.total <= 3 * kCoarseThreads
+    constexpr int NB0 = 2;  // top level <= 2 * kCoarseThreads cells
+    T ve[NQ], vn[NQ], vi[NQ], vb[NB0];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      const int q = t + i * kCoarseThreads;
+      ve[i] = q < d.total ? cf[q] : (T)0;
+      vn[i] = q < d.total ? cf[d.total + q] : (T)0;
+      vi[i] = q < d.total ? cf[2 * d.total + q] : (T)0;
+    }
+    const int n0 = d.nx[0] * d.ny[0];
+#pragma unroll
+    for (int i = 0; i < NB0; ++i) {
+      const int q = t + i * kCoarseThreads;
+      vb[i] = q < n0 ? top.b[top.at(q / d.nx[0], q % d.nx[0])] : (T)0;
+    }
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      const int q = t + i * kCoarseThreads;
+      if (q < d.total) { se[q] = ve[i]; sn[q] = vn[i]; si[q] = vi[i]; }
+    }
+#pragma unroll
+    for (int i = 0; i < NB0; ++i) {
+      const int q = t + i * kCoarseThreads;
+      if (q < n0) sb[q] = vb[i];
     }
   }
   __syncthreads();
-  // build coarser faces in LDS
-  for (int l = 1; l < d.nlev; ++l) {
-    const int fx = d.nx[l - 1], fy = d.ny[l - 1], fo = d.off[l - 1];
-    const int cx = d.nx[l], cy = d.ny[l], co = d.off[l];
-    auto cex = [&](int R, int C) -> T {
-      const int col = min(2 * C + 1, fx - 1);
-      T s = se[fo + 2 * R * fx + col];
-      if (2 * R + 1 < fy) s += se[fo + (2 * R + 1) * fx + col];
-      return (T)kCoarseScale * s;
-    };
-    auto cey = [&](int R, int C) -> T {
-      const int row = 2 * R + 1;
-      if (row > fy - 1) return (T)0;
-      T s = sn[fo + row * fx + 2 * C];
-      if (2 * C + 1 < fx) s += sn[fo + row * fx + 2 * C + 1];
-      return (T)kCoarseScale * s;
-    };
-    for (int q = t; q < cx * cy; q += kCoarseThreads) {
-      const int R = q / cx, C = q % cx;
-      const T e = cex(R, C), nn = cey(R, C);
-      T dd = e + nn;
-      if (C > 0) dd += cex(R, C - 1);
-      if (R > 0) dd += cey(R - 1, C);
-      se[co + q] = e;
-      sn[co + q] = nn;
-      sd[co + q] = dd > (T)0 ? dd : (T)1;
-    }
-    __syncthreads();
-  }
-  auto Ax = [&](int l, int r, int c) -> T {
-    const int nx = d.nx[l], ny = d.ny[l], o = d.off[l];
-    const int k = o + r * nx + c;
-    T a = sd[k] * sx[k];
-    if (c + 1 < nx) a -= se[k] * sx[k + 1];
-    if (c > 0) a -= se[k - 1] * sx[k - 1];
-    if (r + 1 < ny) a -= sn[k] * sx[k + nx];
-    if (r > 0) a -= sn[k - nx] * sx[k - nx];
+  const T om = (T)kOmega;
+  // A x at cell k of level l, x given by the functor xv(k)
+  auto ax = [&](auto xv, int l, int k, int r, int c) -> T {
+    const int nx = d.nx[l], ny = d.ny[l];
+    T a = xv(k) / si[k];
+    if (c + 1 < nx) a -= se[k] * xv(k + 1);
+    if (c > 0) a -= se[k - 1] * xv(k - 1);
+    if (r + 1 < ny) a -= sn[k] * xv(k + nx);
+    if (r > 0) a -= sn[k - nx] * xv(k - nx);
     return a;
   };
-  constexpr int KMAX = 4;  // cells per thread held in registers during a sweep
-  auto jacobi = [&](int l, bool from_zero) {
-    const int nx = d.nx[l], n = nx * d.ny[l], o = d.off[l];
-    T nv[KMAX];
-    int cnt = 0;
-    for (int q = t; q < n && cnt < KMAX; q += kCoarseThreads, ++cnt) {
-      const int k = o + q;
-      nv[cnt] = from_zero ? (T)kOmega * sb[k] / sd[k]
-                          : sx[k] + (T)kOmega * (sb[k] - Ax(l, q / nx, q % nx)) / sd[k];
+  const int last = d.nlev - 1;
+  for (int l = 0; l < last; ++l) {
+    const int o = d.off[l], nx = d.nx[l], ny = d.ny[l];
+    // x1 = x0 + w (b - A x0) / d, x0 = w b / d
+    auto x0 = [&](int k) -> T { return om * sb[k] * si[k]; };
+    CELLS(l) {
+      const int k = o + r * nx + c;
+      sy[k] = x0(k) + om * (sb[k] - ax(x0, l, k, r, c)) * si[k];
     }
     __syncthreads();
-    cnt = 0;
-    for (int q = t; q < n && cnt < KMAX; q += kCoarseThreads, ++cnt) sx[o + q] = nv[cnt];
-    __syncthreads();
-  };
-  const int last = d.nlev - 1;
-  // down
-  for (int l = 0; l < last; ++l) {
-    jacobi(l, true);
-    jacobi(l, false);
-    const int fx = d.nx[l], fy = d.ny[l], cx = d.nx[l + 1], cy = d.ny[l + 1], co = d.off[l + 1];
-    for (int q = t; q < cx * cy; q += kCoarseThreads) {
-      const int R = q / cx, C = q % cx;
-      T s = (T)0;
+    const int co = d.off[l + 1], cx = d.nx[l + 1];
+    auto x1 = [&](int k) -> T { return sy[k]; };
+    CELLS(l + 1) {
+      T acc = (T)0;
       for (int dr = 0; dr < 2; ++dr)
         for (int dc = 0; dc < 2; ++dc) {
-          const int r = 2 * R + dr, c = 2 * C + dc;
-          if (r < fy && c < fx) s += sb[d.off[l] + r * fx + c] - Ax(l, r, c);
+          const int rr = 2 * r + dr, cc = 2 * c + dc;
+          if (rr < ny && cc < nx) {
+            const int k = o + rr * nx + cc;
+            acc += sb[k] - ax(x1, l, k, rr, cc);
+          }
         }
-      sb[co + q] = s;
+      sb[co + r * cx + c] = acc;
     }
     __syncthreads();
   }
-  // coarsest level is a single cell: exact solve in FP64
-  if (t == 0) {
-    const int o = d.off[last];
-    sx[o] = (T)((double)sb[o] / (double)sd[o]);
+  if (t == 0) {  // one cell: exact, in FP64
+    const int k = d.off[last];
+    sy[k] = (T)((double)sb[k] * (double)si[k]);
   }
   __syncthreads();
-  // up
   for (int l = last - 1; l >= 0; --l) {
-    const int fx = d.nx[l], n = fx * d.ny[l], o = d.off[l];
-    const int cx = d.nx[l + 1], co = d.off[l + 1];
-    for (int q = t; q < n; q += kCoarseThreads) {
-      const int r = q / fx, c = q % fx;
-      sx[o + q] += sx[co + (r >> 1) * cx + (c >> 1)];
+    const int o = d.off[l], nx = d.nx[l], co = d.off[l + 1], cx = d.nx[l + 1];
+    // x2 = x1 + P e formed on the fly, first sweep into sx
+    auto x2 = [&](int k) -> T {
+      const int q = k - o, r = q / nx, c = q - (q / nx) * nx;
+      return sy[k] + sy[co + (r >> 1) * cx + (c >> 1)];
+    };
+    CELLS(l) {
+      const int k = o + r * nx + c;
+      sx[k] = x2(k) + om * (sb[k] - ax(x2, l, k, r, c)) * si[k];
     }
     __syncthreads();
-    jacobi(l, false);
-    jacobi(l, false);
+    auto x3 = [&](int k) -> T { return sx[k]; };
+    CELLS(l) {
+      const int k = o + r * nx + c;
+      sy[k] = x3(k) + om * (sb[k] - ax(x3, l, k, r, c)) * si[k];
+    }
+    __syncthreads();
   }
-  {
-    const int n = d.nx[0] * d.ny[0];
-    for (int q = t; q < n; q += kCoarseThreads) top.y[top.at(q / d.nx[0], q % d.nx[0])] = sx[q];
-  }
+  CELLS(0) { top.y[top.at(r, c)] = sy[r * d.nx[0] + c]; }
 }
+#undef CELLS
 
 // ---------------------------------------------------------------------------
 // PCG pieces on level 0.  Reductions: each block stores its partial; the next
 // kernel on the stream sums them in fixed order (no in-kernel fences).
-struct PcgScal {
-  double rzs[2], rr0, rr, tol2;
-  int stop, iters, pad_[2];
-};
 
 enum MgSlot { S_RR0 = 0, S_RZ = 1, S_PAP = 2, S_ERR = 3, S_IR = 4, S_COUNT = 5 };
 
-struct Grid0 {  // capped grid of the level-0 vector kernels
+struct Grid0 {  // tile grid of the level-0 kernels (one TX x TY tile per block)
   int gx, gy, NB;
 };
 
-#define L0_LOOP(G, L, r, c)                                  \
-  const int c = blockIdx.x * MBX + threadIdx.x;              \
-  for (int r = blockIdx.y * MBY + threadIdx.y; r < (L).ny; r += (G).gy * MBY)
+// one cell per thread: blocks of TX x TY threads over the tile grid
+#define L0_LOOP(G, L, r, c)                                    \
+  const int c = blockIdx.x * TX + threadIdx.x;                 \
+  for (int r = blockIdx.y * TY + threadIdx.y; r < (L).ny; r = (L).ny)
 
 __device__ inline int bid() { return blockIdx.x + blockIdx.y * gridDim.x; }
-__device__ inline int tid() { return threadIdx.x + threadIdx.y * MBX; }
+__device__ inline int tid() { return threadIdx.x + threadIdx.y * TX; }
 
 // r = src (cast), x = 0, partial ||r||^2.  An outer refinement loop that has
 // already converged (outer_stop) stops the inner solve at once.
 template <class T, class S>
-__global__ __launch_bounds__(MNT) void k_pcg_start(Grid0 G, Lv<T> L, const S* src, T* xv, T* rv, PcgScal* ps,
+__global__ __launch_bounds__(FT) void k_pcg_start(Grid0 G, Lv<T> L, const S* src, T* xv, T* rv, PcgScal* ps,
                                                   const int* outer_stop, double* part) {
-  __shared__ double sh[MNT / kWave + 1];
+  __shared__ double sh[FT / kWave + 1];
   if (outer_stop && *outer_stop) {
     if (bid() == 0 && tid() == 0) { ps->stop = 1; ps->iters = 0; }
     return;
@@ -332,115 +510,98 @@ __global__ __launch_bounds__(MNT) void k_pcg_start(Grid0 G, Lv<T> L, const S* sr
     xv[k] = (T)0;
     e += (double)v * (double)v;
   }
-  const double s = block_sum<MNT>(e, sh);
+  const double s = block_sum<FT>(e, sh);
   if (tid() == 0) part[(size_t)S_RR0 * G.NB + bid()] = s;
 }
 
-// last post-smoothing sweep of level 0, fused with the partial r.z
+// beta = rz / rz_old, p = z + beta p_old fused into Ap, partial pAp.  The
+// stop tests run in the first kernel of each V-cycle; this kernel only
+// reads the flag.  Every global load is issued before the first use.
 template <class T>
-__global__ __launch_bounds__(MNT) void k_mg_jac_dot(Grid0 G, Lv<T> L, const T* src, T* dst, const int* stop,
-                                                   double* part) {
-  __shared__ double sh[MNT / kWave + 1];
-  if (*stop) return;
-  double e = 0.0;
-  L0_LOOP(G, L, r, c) {
-    if (c >= L.nx) continue;
-    const size_t k = L.at(r, c);
-    const Faces<T> f = faces(L, k, r, c);
-    T ax = f.d * src[k];
-    if (c + 1 < L.nx) ax -= f.e * src[k + 1];
-    if (c > 0) ax -= f.w * src[k - 1];
-    if (r + 1 < L.ny) ax -= f.n * src[k + L.pitch];
-    if (r > 0) ax -= f.s * src[k - L.pitch];
-    const T z = src[k] + (T)kOmega * (L.b[k] - ax) / f.d;
-    dst[k] = z;
-    e += (double)L.b[k] * (double)z;
+__global__ __launch_bounds__(FT) void k_pcg_ap(Grid0 G, Lv<T> L, const T* z, const T* po, T* pn, T* Ap, int it,
+                                               PcgScal* ps, double* part) {
+  __shared__ double sh[FT / kWave + 1];
+  __shared__ int sflag;
+  const int t = tid();
+  const int stop0 = t == 0 ? ps->stop : 0;
+  const double prz = parts_load<FT>(part + (size_t)S_RZ * G.NB, G.NB);
+  const double rzold = it > 0 ? ps->rzs[(it - 1) & 1] : 1.0;
+  const int c = blockIdx.x * TX + threadIdx.x, r = blockIdx.y * TY + threadIdx.y;
+  const bool in = r < L.ny && c < L.nx;
+  const size_t k = in ? L.at(r, c) : 0;
+  T zc = 0, zE = 0, zW = 0, zN = 0, zS = 0, pc = 0, pE = 0, pW = 0, pN = 0, pS = 0, fe = 0, fw = 0, fn = 0, fs = 0;
+  if (in) {
+    zc = z[k]; pc = po[k]; fe = L.ex[k]; fn = L.ey[k];
+    if (c + 1 < L.nx) { zE = z[k + 1]; pE = po[k + 1]; }
+    if (c > 0) { zW = z[k - 1]; pW = po[k - 1]; fw = L.ex[k - 1]; }
+    if (r + 1 < L.ny) { zN = z[k + L.pitch]; pN = po[k + L.pitch]; }
+    if (r > 0) { zS = z[k - L.pitch]; pS = po[k - L.pitch]; fs = L.ey[k - L.pitch]; }
   }
-  const double s = block_sum<MNT>(e, sh);
-  if (tid() == 0) part[(size_t)S_RZ * G.NB + bid()] = s;
-}
-
-// stop test (start: ||r0|| against max(abs, rtol ||r0||); later: ||r|| of
-// the previous update), beta = rz / rz_old, p = z + beta p_old fused into
-// Ap, partial pAp
-template <class T>
-__global__ __launch_bounds__(MNT) void k_pcg_ap(Grid0 G, Lv<T> L, const T* z, const T* po, T* pn, T* Ap, int it,
-                                               double abs_tol, double rtol, PcgScal* ps, double* part) {
-  __shared__ double sh[MNT / kWave + 1];
-  __shared__ int shi;
-  if (block_read(&ps->stop, &shi)) return;
-  const bool b0 = bid() == 0 && tid() == 0;
-  if (it == 0) {
-    const double rr0 = parts_sum<MNT>(part + (size_t)S_RR0 * G.NB, G.NB, sh);
-    const double tgt = fmax(abs_tol, rtol * sqrt(rr0));
-    if (b0) { ps->rr0 = rr0; ps->rr = rr0; ps->tol2 = tgt * tgt; }
-    if (rr0 <= tgt * tgt) {
-      if (b0) { ps->stop = 1; ps->iters = 0; }
-      return;
-    }
-  } else {
-    const double err = parts_sum<MNT>(part + (size_t)S_ERR * G.NB, G.NB, sh);
-    if (b0) { ps->rr = err; ps->iters = it; }
-    if (err <= ps->tol2) {
-      if (b0) ps->stop = 1;
-      return;
-    }
-  }
-  const double rz = parts_sum<MNT>(part + (size_t)S_RZ * G.NB, G.NB, sh);
-  const T beta = it == 0 ? (T)0 : (T)(rz / ps->rzs[(it - 1) & 1]);
-  if (b0) ps->rzs[it & 1] = rz;
+  if (t == 0) sflag = stop0;
+  __syncthreads();
+  if (sflag) return;
+  const double rz = parts_total<FT>(prz, sh);
+  const T beta = it == 0 ? (T)0 : (T)(rz / rzold);
+  if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) ps->rzs[it & 1] = rz;
   double e = 0.0;
-  L0_LOOP(G, L, r, c) {
-    if (c >= L.nx) continue;
-    const size_t k = L.at(r, c);
-    auto pv = [&](size_t q) -> T { return z[q] + beta * po[q]; };
-    const Faces<T> f = faces(L, k, r, c);
-    const T pc = pv(k);
-    T ax = f.d * pc;
-    if (c + 1 < L.nx) ax -= f.e * pv(k + 1);
-    if (c > 0) ax -= f.w * pv(k - 1);
-    if (r + 1 < L.ny) ax -= f.n * pv(k + L.pitch);
-    if (r > 0) ax -= f.s * pv(k - L.pitch);
-    pn[k] = pc;
+  if (in) {
+    T d = fe + fw + fn + fs;
+    d = d < (T)kTiny20 ? (T)1 : d;
+    const T p0 = zc + beta * pc;
+    T ax = d * p0;
+    if (c + 1 < L.nx) ax -= fe * (zE + beta * pE);
+    if (c > 0) ax -= fw * (zW + beta * pW);
+    if (r + 1 < L.ny) ax -= fn * (zN + beta * pN);
+    if (r > 0) ax -= fs * (zS + beta * pS);
+    pn[k] = p0;
     Ap[k] = ax;
-    e += (double)pc * (double)ax;
+    e = (double)p0 * (double)ax;
   }
-  const double s = block_sum<MNT>(e, sh);
-  if (tid() == 0) part[(size_t)S_PAP * G.NB + bid()] = s;
+  const double sblk = block_sum<FT>(e, sh);
+  if (t == 0) part[(size_t)S_PAP * G.NB + bid()] = sblk;
 }
 
 // alpha = rz / pAp, x += alpha p, r -= alpha Ap, partial ||r||^2
 template <class T>
-__global__ __launch_bounds__(MNT) void k_pcg_xr(Grid0 G, Lv<T> L, T* xv, T* rv, const T* pn, const T* Ap, int it,
+__global__ __launch_bounds__(FT) void k_pcg_xr(Grid0 G, Lv<T> L, T* xv, T* rv, const T* pn, const T* Ap, int it,
                                                PcgScal* ps, double* part) {
-  __shared__ double sh[MNT / kWave + 1];
-  __shared__ int shi;
-  if (block_read(&ps->stop, &shi)) return;
-  const double pap = parts_sum<MNT>(part + (size_t)S_PAP * G.NB, G.NB, sh);
+  __shared__ double sh[FT / kWave + 1];
+  __shared__ int sflag;
+  const int t = tid();
+  const int stop0 = t == 0 ? ps->stop : 0;
+  const double ppap = parts_load<FT>(part + (size_t)S_PAP * G.NB, G.NB);
+  const double rz = ps->rzs[it & 1];
+  const int c = blockIdx.x * TX + threadIdx.x, r = blockIdx.y * TY + threadIdx.y;
+  const bool in = r < L.ny && c < L.nx;
+  const size_t k = in ? L.at(r, c) : 0;
+  T x = 0, rr = 0, p = 0, ap = 0;
+  if (in) { x = xv[k]; rr = rv[k]; p = pn[k]; ap = Ap[k]; }
+  if (t == 0) sflag = stop0;
+  __syncthreads();
+  if (sflag) return;
+  const double pap = parts_total<FT>(ppap, sh);
   if (fabs(pap) < kTiny20) {
-    if (bid() == 0 && tid() == 0) { ps->stop = 1; ps->iters = it; }
+    if (bid() == 0 && t == 0) { ps->stop = 1; ps->iters = it; }
     return;
   }
-  const T alpha = (T)(ps->rzs[it & 1] / pap);
+  const T alpha = (T)(rz / pap);
   double e = 0.0;
-  L0_LOOP(G, L, r, c) {
-    if (c >= L.nx) continue;
-    const size_t k = L.at(r, c);
-    xv[k] += alpha * pn[k];
-    const T rn = rv[k] - alpha * Ap[k];
+  if (in) {
+    xv[k] = x + alpha * p;
+    const T rn = rr - alpha * ap;
     rv[k] = rn;
-    e += (double)rn * (double)rn;
+    e = (double)rn * (double)rn;
   }
-  const double s = block_sum<MNT>(e, sh);
-  if (tid() == 0) part[(size_t)S_ERR * G.NB + bid()] = s;
+  const double sblk = block_sum<FT>(e, sh);
+  if (t == 0) part[(size_t)S_ERR * G.NB + bid()] = sblk;
 }
 
 // after K launched iterations: stop test of iteration K-1 (one block)
-__global__ __launch_bounds__(MNT) void k_pcg_end(Grid0 G, int K, PcgScal* ps, const double* part) {
-  __shared__ double sh[MNT / kWave + 1];
+__global__ __launch_bounds__(FT) void k_pcg_end(Grid0 G, int K, PcgScal* ps, const double* part) {
+  __shared__ double sh[FT / kWave + 1];
   __shared__ int shi;
   if (block_read(&ps->stop, &shi)) return;
-  const double err = parts_sum<MNT>(part + (size_t)S_ERR * G.NB, G.NB, sh);
+  const double err = parts_sum<FT>(part + (size_t)S_ERR * G.NB, G.NB, sh);
   if (tid() == 0) {
     ps->rr = err;
     ps->iters = K;
@@ -457,10 +618,10 @@ __global__ __launch_bounds__(MNT) void k_ir_add(Lv<float> L, const float* e, dou
   x64[(size_t)(r + 1) * P + c + 1] += (double)e[L.at(r, c)];
 }
 
-__global__ __launch_bounds__(MNT) void k_ir_resid(Grid0 G, Geo g, Lv<float> L, const double* dE, const double* dN,
+__global__ __launch_bounds__(FT) void k_ir_resid(Grid0 G, Geo g, Lv<float> L, const double* dE, const double* dN,
                                                  const double* x64, const double* b64, double* r64, const int* stop,
                                                  double* part) {
-  __shared__ double sh[MNT / kWave + 1];
+  __shared__ double sh[FT / kWave + 1];
   if (*stop) return;
   const int M = g.M, N = g.N, P = g.P;
   double e2 = 0.0;
@@ -483,18 +644,18 @@ __global__ __launch_bounds__(MNT) void k_ir_resid(Grid0 G, Geo g, Lv<float> L, c
     r64[f] = rn;
     e2 += rn * rn;
   }
-  const double s = block_sum<MNT>(e2, sh);
+  const double s = block_sum<FT>(e2, sh);
   if (tid() == 0) part[(size_t)S_IR * G.NB + bid()] = s;
 }
 
 // outer test: ||r64||^2 <= max(abs, rtol ||b||)^2 (||b||^2 = rr0 of the first
 // inner solve); one block
-__global__ __launch_bounds__(MNT) void k_ir_check(Grid0 G, PcgScal* outer, const PcgScal* first, int o,
+__global__ __launch_bounds__(FT) void k_ir_check(Grid0 G, PcgScal* outer, const PcgScal* first, int o,
                                                  double abs_tol, double rtol, const double* part) {
-  __shared__ double sh[MNT / kWave + 1];
+  __shared__ double sh[FT / kWave + 1];
   __shared__ int shi;
   if (block_read(&outer->stop, &shi)) return;
-  const double rr = parts_sum<MNT>(part + (size_t)S_IR * G.NB, G.NB, sh);
+  const double rr = parts_sum<FT>(part + (size_t)S_IR * G.NB, G.NB, sh);
   if (tid() == 0) {
     if (o == 0) {
       const double t = fmax(abs_tol, rtol * sqrt(first->rr0));
@@ -542,7 +703,8 @@ struct MGImpl {
   std::vector<std::unique_ptr<Level>> lv;  // grid-kernel levels 0 .. nGrid-1, lv[nGrid] = coarse top
   int nGrid = 0;
   CoarseDesc cd{};
-  size_t coarseSmem = 0;
+  size_t coarseSmem = 0, coarseSetupSmem = 0;
+  DevBuf<T> cf;  // coarse-solver faces and inverse diagonals
   DevBuf<T> r0, xs, pa, pb, Ap;  // PCG vectors, level-0 layout (xs = solution)
   DevBuf<double> part;
   DevBuf<PcgScal> scal;          // [0] inner, [1] outer
@@ -588,12 +750,18 @@ struct MGImpl {
     }
     cd.nlev = n;
     cd.total = off;
-    coarseSmem = (size_t)5 * off * sizeof(T);
-    CPS_CHECK(coarseSmem <= 64 * 1024, CPS_ERR_STATE, "coarse multigrid levels exceed LDS");
+    CPS_CHECK(off <= 3 * kCoarseThreads && lv.back()->nx * lv.back()->ny <= 2 * kCoarseThreads, CPS_ERR_STATE,
+              "coarse multigrid plan exceeds the coarse solver's staging");
+    coarseSmem = (size_t)6 * off * sizeof(T);
+    coarseSetupSmem = (size_t)2 * off * sizeof(T);
+    int lds = 0;
+    CPS_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+    CPS_CHECK(coarseSmem <= (size_t)lds, CPS_ERR_STATE, "coarse multigrid levels exceed LDS");
+    cf.alloc((size_t)3 * off);
     for (auto* b : {&lv[0]->x, &lv[0]->y, &lv[0]->ex, &lv[0]->ey, &r0, &xs, &pa, &pb, &Ap})
       CPS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), ctx->stream));
-    G0.gx = ceil_div(lv[0]->nx, MBX);
-    G0.gy = std::max(1, std::min(ceil_div(lv[0]->ny, MBY), std::max(1, 1024 / G0.gx)));
+    G0.gx = ceil_div(lv[0]->nx, TX);
+    G0.gy = ceil_div(lv[0]->ny, TY);
     G0.NB = G0.gx * G0.gy;
     part.alloc((size_t)S_COUNT * G0.NB);
     CPS_HIP(hipMemsetAsync(part.p, 0, part.bytes(), ctx->stream));
@@ -603,6 +771,7 @@ struct MGImpl {
 
   dim3 blk() const { return dim3(MBX, MBY); }
   dim3 grid0() const { return dim3(G0.gx, G0.gy); }
+  dim3 blk0() const { return dim3(TX, TY); }
   // level views; the level-0 rhs of the V-cycle is the PCG residual r0
   Lv<T> view(int l) {
     Lv<T> v = lv[l]->view();
@@ -615,28 +784,27 @@ struct MGImpl {
     k_mg_build0<T><<<lv[0]->grid(), blk(), 0, s>>>(g, dE, dN, lv[0]->view());
     for (size_t l = 1; l < lv.size(); ++l)
       k_mg_coarsen<T><<<lv[l]->grid(), blk(), 0, s>>>(lv[l - 1]->view(), lv[l]->view());
+    k_mg_coarse_setup<T><<<1, kCoarseThreads, coarseSetupSmem, s>>>(view(nGrid), cd, cf.p);
     CPS_HIP(hipGetLastError());
   }
 
   // z = V(r0) into level-0 y, with the partial r.z of the last sweep
-  void vcycle(const int* stop) {
+  dim3 tgrid(int l) const { return dim3(ceil_div(lv[l]->nx, TX), ceil_div(lv[l]->ny, TY)); }
+  double abs_tol_ = 0.0, rtol_ = 0.0;  // target of the running PCG solve
+  void vcycle(const int* stop, int it) {
     hipStream_t s = ctx->stream;
-    Lv<T> none{};
     for (int l = 0; l < nGrid; ++l) {
-      Lv<T> L = view(l);
-      k_mg_jac0<T><<<lv[l]->grid(), blk(), 0, s>>>(L, stop);
+      PcgTest test{nullptr, 0, 0, nullptr, 0.0, 0.0};
+      if (l == 0)
+        test = PcgTest{part.p + (size_t)(it == 0 ? S_RR0 : S_ERR) * G0.NB, G0.NB, it, scal.p, abs_tol_, rtol_};
       if (l == 0) timer->begin(KT_MG_SMOOTH0);
-      k_mg_jac<T><<<lv[l]->grid(), blk(), 0, s>>>(L, L.x, L.y, none, 0, stop);
+      k_mg_down<T><<<tgrid(l), dim3(TX, TY), 0, s>>>(view(l), view(l + 1), stop, test);
       if (l == 0) timer->end(KT_MG_SMOOTH0);
-      k_mg_restrict<T><<<lv[l + 1]->grid(), blk(), 0, s>>>(L, L.y, view(l + 1), stop);
     }
-    k_mg_coarse<T><<<1, kCoarseThreads, coarseSmem, s>>>(view(nGrid), cd, stop);
-    for (int l = nGrid - 1; l >= 0; --l) {
-      Lv<T> L = view(l);
-      k_mg_jac<T><<<lv[l]->grid(), blk(), 0, s>>>(L, L.y, L.x, view(l + 1), 1, stop);
-      if (l > 0) k_mg_jac<T><<<lv[l]->grid(), blk(), 0, s>>>(L, L.x, L.y, none, 0, stop);
-      else k_mg_jac_dot<T><<<grid0(), blk(), 0, s>>>(G0, L, L.x, L.y, stop, part.p);
-    }
+    k_mg_coarse<T><<<1, kCoarseThreads, coarseSmem, s>>>(view(nGrid), cd, cf.p, stop);
+    for (int l = nGrid - 1; l >= 0; --l)
+      k_mg_up<T><<<tgrid(l), dim3(TX, TY), 0, s>>>(view(l), view(l + 1),
+                                                  l == 0 ? part.p + (size_t)S_RZ * G0.NB : nullptr, stop);
     CPS_HIP(hipGetLastError());
   }
 
@@ -646,33 +814,36 @@ struct MGImpl {
   void pcg(const double* rhs, double abs_tol, double rtol, int maxit, int sync_every, const int* outer_stop) {
     hipStream_t s = ctx->stream;
     const double nP = (double)lv[0]->nx * (double)lv[0]->ny;
-    // smoother: x, b, east and north faces read, x' written; PCG stencil:
-    // z, p_old, faces read, p and Ap written
-    if (timer->which == KT_MG_SMOOTH0) timer->bytes_per_launch = 5.0 * sizeof(T) * nP;
+    // fine-level down kernel: b, east and north faces read, x1 and the
+    // coarse rhs (1/4) written; PCG stencil: z, p_old, faces read, p and Ap
+    // written
+    if (timer->which == KT_MG_SMOOTH0) timer->bytes_per_launch = 4.25 * sizeof(T) * nP;
     if (timer->which == KT_PCG_STENCIL) timer->bytes_per_launch = 6.0 * sizeof(T) * nP;
     const Lv<T> L = view(0);
     PcgScal* ps = scal.p;
-    k_pcg_start<T, double><<<grid0(), blk(), 0, s>>>(G0, L, rhs, xs.p, r0.p, ps, outer_stop, part.p);
+    abs_tol_ = abs_tol;
+    rtol_ = rtol;
+    k_pcg_start<T, double><<<grid0(), blk0(), 0, s>>>(G0, L, rhs, xs.p, r0.p, ps, outer_stop, part.p);
     // p_old of iteration 0 is pa: zero since creation, finite afterwards, and
     // multiplied by beta = 0
     for (int it = 0; it < maxit; ++it) {
-      vcycle(&ps->stop);
+      vcycle(&ps->stop, it);
       T* po = (it & 1) ? pb.p : pa.p;
       T* pn = (it & 1) ? pa.p : pb.p;
       timer->begin(KT_PCG_STENCIL);
-      k_pcg_ap<T><<<grid0(), blk(), 0, s>>>(G0, L, lv[0]->y.p, po, pn, Ap.p, it, abs_tol, rtol, ps, part.p);
+      k_pcg_ap<T><<<grid0(), blk0(), 0, s>>>(G0, L, lv[0]->y.p, po, pn, Ap.p, it, ps, part.p);
       timer->end(KT_PCG_STENCIL);
-      k_pcg_xr<T><<<grid0(), blk(), 0, s>>>(G0, L, xs.p, r0.p, pn, Ap.p, it, ps, part.p);
+      k_pcg_xr<T><<<grid0(), blk0(), 0, s>>>(G0, L, xs.p, r0.p, pn, Ap.p, it, ps, part.p);
       CPS_HIP(hipGetLastError());
       if (sync_every > 0 && (it + 1) % sync_every == 0 && it + 1 < maxit) {
-        k_pcg_end<<<1, blk(), 0, s>>>(G0, it + 1, ps, part.p);
+        k_pcg_end<<<1, blk0(), 0, s>>>(G0, it + 1, ps, part.p);
         int stop = 0;
         CPS_HIP(hipMemcpyAsync(&stop, &ps->stop, sizeof(int), hipMemcpyDeviceToHost, s));
         CPS_HIP(hipStreamSynchronize(s));
         if (stop) return;
       }
     }
-    k_pcg_end<<<1, blk(), 0, s>>>(G0, maxit, ps, part.p);
+    k_pcg_end<<<1, blk0(), 0, s>>>(G0, maxit, ps, part.p);
   }
 };
 
@@ -725,8 +896,8 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
     for (int o = 0; o < max_outer; ++o) {
       m->pcg(r, 0.0, inner_rtol, maxit, sync_every, &outer->stop);
       k_ir_add<<<m->lv[0]->grid(), dim3(MBX, MBY), 0, s>>>(L, m->xs.p, x, g_.P, &outer->stop);
-      k_ir_resid<<<m->grid0(), dim3(MBX, MBY), 0, s>>>(m->G0, g_, L, dE, dN, x, b64, r, &outer->stop, m->part.p);
-      k_ir_check<<<1, dim3(MBX, MBY), 0, s>>>(m->G0, outer, inner, o, cfg.p_tol, cfg.p_rtol, m->part.p);
+      k_ir_resid<<<m->grid0(), m->blk0(), 0, s>>>(m->G0, g_, L, dE, dN, x, b64, r, &outer->stop, m->part.p);
+      k_ir_check<<<1, m->blk0(), 0, s>>>(m->G0, outer, inner, o, cfg.p_tol, cfg.p_rtol, m->part.p);
       CPS_HIP(hipGetLastError());
       if (poll && o + 1 < max_outer) {
         int stop = 0;

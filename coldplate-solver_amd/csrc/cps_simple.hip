@@ -45,7 +45,7 @@ constexpr double kAlphaEps = (double)1e-10f;
 constexpr double kTiny20 = (double)1e-20f;
 constexpr double kKminFloor = (double)1e-20f;
 
-constexpr int BX = 64, BY = 4, NT = BX * BY;
+constexpr int BX = 64, BY = 16, NT = BX * BY;  // 1024-thread blocks, one row per thread
 
 struct Fields {
   double *u, *v, *p, *us, *vs, *ps, *dE, *dN, *gam, *alp, *inlet;
@@ -307,9 +307,27 @@ template <class C>
 __global__ __launch_bounds__(NT) void k_jacobi(Geo g, MomSys<C> su, MomSys<C> sv, int sweep, double tol2,
                                                IterState* st, double* part) {
   __shared__ double sh[NT / kWave + 1];
-  __shared__ int shi;
-  int au = block_read(&st->jac_active[0], &shi), av = block_read(&st->jac_active[1], &shi);
-  jac_detect(g, sweep, tol2, st, part, au, av, sh);
+  __shared__ int sfl[2];
+  const int t = thread_id();
+  // flags (thread 0) and the partials of sweep-1 are loaded together
+  const int f0 = t == 0 ? st->jac_active[0] : 0, f1 = t == 0 ? st->jac_active[1] : 0;
+  const double* pin = part + (size_t)(SLOT_JAC + 2 * ((sweep + 1) & 1)) * g.NB;
+  const double qu = sweep > 0 ? parts_load<NT>(pin, g.NB) : 0.0;
+  const double qv = sweep > 0 ? parts_load<NT>(pin + g.NB, g.NB) : 0.0;
+  if (t == 0) { sfl[0] = f0; sfl[1] = f1; }
+  __syncthreads();
+  int au = sfl[0], av = sfl[1];
+  if (sweep > 0) {  // stop rule of jacobi_solver.cpp:63 on sweep-1
+    const bool b0 = block_id() == 0 && t == 0;
+    if (au) {
+      const double a = parts_total<NT>(qu, sh);
+      if (a < tol2 * (double)g.nU) { au = 0; if (b0) { st->jac_active[0] = 0; st->jac_iters[0] = sweep; } }
+    }
+    if (av) {
+      const double b = parts_total<NT>(qv, sh);
+      if (b < tol2 * (double)g.nV) { av = 0; if (b0) { st->jac_active[1] = 0; st->jac_iters[1] = sweep; } }
+    }
+  }
   if (!au && !av) return;  // uniform over the grid
   const int M = g.M, N = g.N, P = g.P;
   double ru = 0.0, rv = 0.0;
@@ -730,10 +748,11 @@ class SimpleSolver {
       for (auto& b : momF) { b.alloc(S); CPS_HIP(hipMemsetAsync(b.p, 0, b.bytes(), ctx->stream)); }
     }
     for (auto& b : pcg) { b.alloc(S); CPS_HIP(hipMemsetAsync(b.p, 0, b.bytes(), ctx->stream)); }
-    // capped grid: about four 256-thread blocks per CU, rows strided
+    // 64 x 16 tiles; rows are strided only on grids large enough to need
+    // more than 2048 blocks (bounds the partial sums every block re-reads)
     block = dim3(BX, BY);
     g.gx = ceil_div(N + 1, BX);
-    g.gy = std::max(1, std::min(ceil_div(M + 1, BY), std::max(1, 1024 / g.gx)));
+    g.gy = std::max(1, std::min(ceil_div(M + 1, BY), std::max(1, 2048 / g.gx)));
     g.NB = g.gx * g.gy;
     grid = dim3(g.gx, g.gy);
     part.alloc((size_t)SLOT_COUNT * g.NB);
