@@ -66,6 +66,18 @@ def lib():
         L.cps_simple_boundary_patch.argtypes = [vp, C.c_int, P(T.Patch)]
         L.cps_simple_time_kernel.argtypes = [vp, C.c_int]
         L.cps_simple_kernel_timing.argtypes = [vp, ip, dp, dp]
+        L.cps_thermal_config_default.argtypes = [P(T.ThermalConfig)]
+        L.cps_thermal_config_default.restype = None
+        L.cps_thermal_create.argtypes = [vp, P(T.ThermalConfig), dp, ip, dp, dp, dp, dp, P(vp)]
+        L.cps_thermal_destroy.argtypes = [vp]
+        L.cps_thermal_destroy.restype = None
+        L.cps_thermal_update.argtypes = [vp, dp, dp, dp, dp]
+        L.cps_thermal_solve.argtypes = [vp, dp, P(T.SolveStats)]
+        L.cps_thermal_get_temperature.argtypes = [vp, dp]
+        L.cps_thermal_get_metrics.argtypes = [vp, C.c_double, P(T.ThermalMetrics)]
+        L.cps_thermal_solve_adjoint.argtypes = [vp, dp, dp, P(T.SolveStats)]
+        L.cps_thermal_gradient.argtypes = [vp, C.c_int, C.c_double, dp, dp, P(T.SolveStats)]
+        L.cps_thermal_get_operator.argtypes = [vp, dp, dp]
         _lib = L
         return L
 

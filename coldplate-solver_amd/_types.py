@@ -129,3 +129,73 @@ def field_shape(name, nx, ny):
         "p": (M + 1, N + 1), "pStar": (M + 1, N + 1),
         "gamma": (M, N), "alpha": (M, N),
     }[name]
+
+
+# ---------------------------------------------------------------------------
+# 3-D thermal solve
+
+CPS_THERMAL_OBJ_T_AVG_BASE = 0
+CPS_THERMAL_OBJ_PNORM_L0 = 1
+THERMAL_OBJECTIVES = {"t_avg_base": CPS_THERMAL_OBJ_T_AVG_BASE, "pnorm": CPS_THERMAL_OBJ_PNORM_L0}
+
+
+class ThermalConfig(C.Structure):
+    """cps_thermal_config; property names follow Params (thermal_solver.cpp:148-192)."""
+
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double),
+        ("ht", C.c_double),
+        ("k_s", C.c_double), ("k_f", C.c_double), ("rho", C.c_double), ("cp", C.c_double),
+        ("t_in", C.c_double),
+        ("k_mode", C.c_int),
+        ("qk", C.c_double),
+        ("precision", C.c_int),
+        ("rtol", C.c_double),
+        ("max_iter", C.c_int),
+        ("max_refine", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("refinements", C.c_int), ("rel_residual", C.c_double),
+                ("converged", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ThermalMetrics(C.Structure):
+    """cps_thermal_metrics (ThermalMetrics of thermal_output.cpp:16-44 without
+    the gradient fields, plus Tb_max / Tb_pnorm and the energy balance)."""
+
+    _fields_ = [(k, C.c_double) for k in (
+        "t_avg_base", "t_avg_fluid", "t_avg_solid", "t_avg_global",
+        "t_std_base", "t_std_fluid", "t_std_global",
+        "t_range_base", "t_range_fluid", "t_range_global",
+        "t_uniformity_base", "t_uniformity_global",
+        "tb_max", "tb_pnorm", "q_in", "q_out", "t_out", "imbalance_pct")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def thermal_config(stack, precision=CPS_PREC_FP64, rtol=1e-6, max_iter=1000, max_refine=8, k_mode=1):
+    """cps_thermal_config from a layer-stack description (configs.thermal_stack);
+    solver knobs default to Params (thermal_solver.cpp:167-168)."""
+    c = ThermalConfig()
+    c.nx, c.ny, c.nz = int(stack["nx"]), int(stack["ny"]), len(stack["dz"])
+    c.dx, c.dy = float(stack["dx"]), float(stack["dy"])
+    c.ht = float(stack["ht"])
+    c.k_s, c.k_f = float(stack["k_s"]), float(stack["k_f"])
+    c.rho, c.cp, c.t_in = float(stack["rho"]), float(stack["cp"]), float(stack["t_in"])
+    c.k_mode = int(stack.get("k_mode", k_mode))
+    c.qk = float(stack["qk"])
+    c.precision = precision
+    c.rtol = rtol
+    c.max_iter = max_iter
+    c.max_refine = max_refine
+    return c

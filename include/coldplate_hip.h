@@ -157,6 +157,77 @@ int cps_simple_time_kernel(cps_simple* s, int kernel);
  * one launch moves (every operand read once, every result written once). */
 int cps_simple_kernel_timing(cps_simple* s, int* launches, double* avg_us, double* bytes_per_launch);
 
+/* ---------------------------------------------------------------------------
+ * 3-D conjugate heat transfer (ThermalSolver/thermal_solver.cpp).  Replaces
+ * the assembly + AMGCL solve of ThermalSolver3D::solve (thermal_solver.cpp:
+ * 422-553) and the post-processing of thermal_output.cpp:135-270.  Cells are
+ * (i, j, k) with k the layer from the bottom; 3-D host arrays are
+ * [k][j][i] with i fastest (thermal_solver.cpp:235), 2-D ones [j][i].
+ * Layers are a list: channel[k] = 0 is solid (k_s), 1 is a channel layer
+ * (gamma-interpolated k, advected with u_xy * pois(z)); nz_solid base layers
+ * followed by nz_fluid channel layers is the reference stack.
+ * ------------------------------------------------------------------------- */
+typedef struct cps_thermal_config {
+  int nx, ny, nz;
+  double dx, dy;             /* cell size [m] (Params::dx, dy) */
+  double ht;                 /* channel height H_t for pois(z) (thermal_solver.cpp:323-327) */
+  double k_s, k_f, rho, cp, t_in;  /* thermal_solver.cpp:160-164 */
+  int k_mode;                /* 0 linear, 1 RAMP, 2 harmonic (:182) */
+  double qk;                 /* RAMP q_k (:183) */
+  int precision;             /* CPS_PREC_FP64 or CPS_PREC_MIXED (FP32 Krylov + FP64 refinement) */
+  double rtol;               /* stop at ||b - A T|| <= rtol ||b|| (:167) */
+  int max_iter;              /* Krylov iterations per solve, all refinements together (:168) */
+  int max_refine;            /* restarts from the true FP64 residual */
+} cps_thermal_config;
+
+typedef struct cps_solve_stats {
+  int iterations;            /* Krylov iterations */
+  int refinements;           /* restarts from the true residual */
+  double rel_residual;       /* ||b - A x|| / ||b|| in FP64 at exit */
+  int converged;             /* rel_residual <= rtol */
+} cps_solve_stats;
+
+/* ThermalMetrics (thermal_output.cpp:16-44, without the gradient fields),
+ * the objective quantities of compute_optimization_results
+ * (thermal_solver.cpp:361-381) and the energy balance (:592-620). */
+typedef struct cps_thermal_metrics {
+  double t_avg_base, t_avg_fluid, t_avg_solid, t_avg_global;
+  double t_std_base, t_std_fluid, t_std_global;
+  double t_range_base, t_range_fluid, t_range_global;
+  double t_uniformity_base, t_uniformity_global;
+  double tb_max, tb_pnorm;   /* over layer k = 0 */
+  double q_in, q_out, t_out, imbalance_pct;
+} cps_thermal_metrics;
+
+#define CPS_THERMAL_OBJ_T_AVG_BASE 0  /* volume mean of T over the layers below the channel */
+#define CPS_THERMAL_OBJ_PNORM_L0 1    /* (mean over layer 0 of T^p)^(1/p) */
+
+void cps_thermal_config_default(cps_thermal_config* cfg);
+/* dz, channel: nz entries; gamma, u, v (cell-centred velocity), q0 (bottom
+ * heat flux [W/m^2]): ny x nx.  Assembles the operator. */
+int cps_thermal_create(cps_ctx* ctx, const cps_thermal_config* cfg, const double* dz, const int* channel,
+                       const double* gamma, const double* u, const double* v, const double* q0,
+                       cps_thermal** out);
+void cps_thermal_destroy(cps_thermal* t);
+/* New gamma / u / v / q0 (NULL keeps the current array); reassembles. */
+int cps_thermal_update(cps_thermal* t, const double* gamma, const double* u, const double* v, const double* q0);
+/* Solves A T = b from T0 (NULL: T = t_in everywhere, thermal_solver.cpp:428). */
+int cps_thermal_solve(cps_thermal* t, const double* T0, cps_solve_stats* stats);
+int cps_thermal_get_temperature(cps_thermal* t, double* T);
+int cps_thermal_get_metrics(cps_thermal* t, double pnorm_exp, cps_thermal_metrics* out);
+/* A^T lambda = rhs (3-D arrays). */
+int cps_thermal_solve_adjoint(cps_thermal* t, const double* rhs, double* lambda, cps_solve_stats* stats);
+/* Objective J (CPS_THERMAL_OBJ_*) of the last solve and dJ/dgamma (ny x nx)
+ * by one adjoint solve.  The reference's thermal adjoint is a stub
+ * (thermal_adjoint.cpp:16-60); the derivative follows its dk_dgamma
+ * (thermal_solver.cpp:317-321) through every conduction term of the
+ * assembly; the region masks (gamma > 0.01) are held fixed. */
+int cps_thermal_gradient(cps_thermal* t, int objective, double pnorm_exp, double* dJ_dgamma, double* J,
+                         cps_solve_stats* stats);
+/* Assembled operator as 7 stencil arrays [d, w, e, s, n, b, t] of nz*ny*nx
+ * each (entry of A for the neighbour; 0 where none) and the right side b. */
+int cps_thermal_get_operator(cps_thermal* t, double* coef7, double* rhs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -222,3 +222,88 @@ class OracleThermal:
         self.last_relres = rr.value
         self.last_iters = it
         return T.reshape(self.nz, self.ny, self.nx)
+
+    def matrix(self):
+        """Assembled system as a scipy CSR matrix and b."""
+        import scipy.sparse as sp
+        rp, ci, val, b = self.csr()
+        n = len(b)
+        return sp.csr_matrix((val, ci, rp), shape=(n, n)), b
+
+    def direct(self, transpose=False, rhs=None):
+        """Sparse LU solve of A T = b (or A^T x = rhs): the exact solution of
+        the oracle-assembled system, shape (nz, ny, nx)."""
+        import scipy.sparse.linalg as sla
+        A, b = self.matrix()
+        if transpose:
+            A = A.T.tocsc()
+        r = b if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64).ravel()
+        return sla.spsolve(A.tocsc(), r).reshape(self.nz, self.ny, self.nx)
+
+    def rel_residual(self, T):
+        A, b = self.matrix()
+        return np.linalg.norm(b - A @ np.ravel(T)) / np.linalg.norm(b)
+
+
+def thermal_metrics(stack, gamma, u, T, pnorm_exp=10.0):
+    """numpy restatement of compute_thermal_metrics (thermal_output.cpp:135-
+    270, without the gradient pass), Tb_max / Tb_pnorm of
+    compute_optimization_results (thermal_solver.cpp:361-381) and the energy
+    balance of ThermalSolver3D::solve (thermal_solver.cpp:592-620).  Base
+    layers are those below the first channel layer (k < nz_solid)."""
+    dz = np.asarray(stack["dz"], dtype=np.float64)
+    ch = np.asarray(stack["channel"]) != 0
+    nz, ny, nx = T.shape
+    dx, dy = stack["dx"], stack["dy"]
+    k0 = int(np.argmax(ch)) if ch.any() else nz
+    V = (dx * dy * dz)[:, None, None] * np.ones((nz, ny, nx))
+    base = np.zeros((nz, ny, nx), bool)
+    base[:k0] = True
+    fl = ch[:, None, None] & (np.asarray(gamma)[None] > 0.01)
+    solid = ~base & ~fl
+
+    def avg(mask):
+        return (T[mask] * V[mask]).sum() / V[mask].sum() if mask.any() else 0.0
+
+    def std(mask, a):
+        return np.sqrt((V[mask] * (T[mask] - a) ** 2).sum() / V[mask].sum()) if mask.any() else 0.0
+
+    def rng(mask):
+        return T[mask].max() - T[mask].min() if mask.any() else 0.0
+
+    allm = np.ones_like(base)
+    m = dict(t_avg_base=avg(base), t_avg_fluid=avg(fl), t_avg_solid=avg(solid), t_avg_global=avg(allm))
+    m.update(t_std_base=std(base, m["t_avg_base"]), t_std_fluid=std(fl, m["t_avg_fluid"]),
+             t_std_global=std(allm, m["t_avg_global"]))
+    m.update(t_range_base=rng(base), t_range_fluid=rng(fl), t_range_global=rng(allm))
+
+    def unif(sd, a):
+        return 0.0 if a <= 0 else max(0.0, min(1.0, 1.0 - sd / a))
+
+    m["t_uniformity_base"] = unif(m["t_std_base"], m["t_avg_base"])
+    m["t_uniformity_global"] = unif(m["t_std_global"], m["t_avg_global"])
+    m["tb_max"] = T[0].max()
+    m["tb_pnorm"] = np.mean(T[0] ** pnorm_exp) ** (1.0 / pnorm_exp)
+    m["q_in"] = (np.asarray(stack["q0"]) * dx * dy).sum()
+    # outlet: i = nx - 1, fluid channel cells with u_xy * pois > 0
+    mdot = mdotT = 0.0
+    for k in range(nz):
+        if not ch[k]:
+            continue
+        zb = dz[k0:k].sum()
+        z = (zb + 0.5 * dz[k]) / stack["ht"]
+        po = 4 * z * (1 - z)
+        uo = np.asarray(u)[:, nx - 1] * po
+        sel = (np.asarray(gamma)[:, nx - 1] > 0.01) & (uo > 0)
+        mm = stack["rho"] * uo[sel] * dy * dz[k]
+        mdot += mm.sum()
+        mdotT += (mm * T[k, sel, nx - 1]).sum()
+    m["t_out"] = mdotT / mdot if mdot > 0 else stack["t_in"]
+    m["q_out"] = mdot * stack["cp"] * (m["t_out"] - stack["t_in"])
+    m["imbalance_pct"] = 100 * (m["q_in"] - m["q_out"]) / (m["q_in"] + 1e-30)
+    return m
+
+
+def thermal_objective(stack, gamma, u, T, objective, pnorm_exp=8.0):
+    m = thermal_metrics(stack, gamma, u, T, pnorm_exp)
+    return m["t_avg_base"] if objective == "t_avg_base" else m["tb_pnorm"]

@@ -1,0 +1,157 @@
+"""3-D conjugate-heat-transfer solve on the HIP path against the oracle.
+
+The oracle assembles the reference operator (thermal_solver.cpp:437-530) on
+the CPU; scipy's sparse LU of that matrix is the exact solution the
+reference's AMGCL solve approximates to rtol 1e-6.  Parity gates:
+  - operator: the device stencil equals the oracle CSR entry by entry
+    (rel 1e-13: both are FP64 evaluations of the same expressions; the
+    compilers may contract different multiply-adds);
+  - solve: ||b - A T|| / ||b|| measured with the ORACLE's matrix is below
+    rtol, and T matches the direct solve;
+  - metrics: the numpy restatement of thermal_output.cpp on the same T;
+  - gradient: brute-force central differences of the objective through the
+    oracle's direct solve (h = 1e-5).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def synthetic_case(pkg, seed, nx, ny, n_base=2, n_channel=3, n_lid=1, h=5e-5, qk=1.0, k_mode=1, solid_frac=0.3):
+    rng = np.random.default_rng(seed)
+    stack = pkg.configs.thermal_stack(nx=nx, ny=ny, n_base=n_base, n_channel=n_channel, n_lid=n_lid, h=h, qk=qk,
+                                      seed=seed)
+    stack["k_mode"] = k_mode
+    gamma = rng.uniform(0.05, 1.0, (ny, nx))
+    gamma[rng.uniform(size=(ny, nx)) < solid_frac] = 0.0
+    u = 0.05 + 0.3 * rng.uniform(size=(ny, nx))
+    v = 0.05 * rng.standard_normal((ny, nx))
+    return stack, gamma, u, v
+
+
+def csr_to_stencil(rp, ci, val, nx, ny, nz):
+    n = nx * ny * nz
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    off = ci - rows
+    out = np.zeros((7, n))
+    for slot, o in enumerate((0, -1, 1, -nx, nx, -nx * ny, nx * ny)):  # d, w, e, s, n, b, t
+        sel = off == o
+        out[slot, rows[sel]] = val[sel]
+    return out.reshape(7, nz, ny, nx)
+
+
+@pytest.mark.parametrize("seed,nx,ny,k_mode", [(0, 24, 12, 1), (1, 37, 21, 2), (2, 70, 9, 0)])
+def test_operator_matches_oracle(pkg, orc, seed, nx, ny, k_mode):
+    stack, gamma, u, v = synthetic_case(pkg, seed, nx, ny, k_mode=k_mode)
+    s = pkg.ThermalSolver3D(stack, gamma, u, v)
+    o = orc.OracleThermal(pkg._types, stack, gamma, u, v)
+    rp, ci, val, b = o.csr()
+    ref = csr_to_stencil(rp, ci, val, nx, ny, s.nz)
+    coef, bg = s.operator()
+    scale = np.abs(ref[0]).max()
+    np.testing.assert_allclose(coef, ref, rtol=1e-13, atol=1e-13 * scale)
+    np.testing.assert_allclose(bg.ravel(), b, rtol=1e-13, atol=1e-13 * np.abs(b).max())
+    s.close()
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("seed,nx,ny", [(3, 32, 16), (4, 45, 27)])
+def test_solve_matches_direct(pkg, orc, precision, seed, nx, ny):
+    stack, gamma, u, v = synthetic_case(pkg, seed, nx, ny)
+    s = pkg.ThermalSolver3D(stack, gamma, u, v, precision=precision, rtol=1e-12, max_iter=2000, max_refine=12)
+    st = s.solve()
+    o = orc.OracleThermal(pkg._types, stack, gamma, u, v)
+    Tg = s.T
+    Td = o.direct()
+    assert st["converged"] == 1, st
+    assert o.rel_residual(Tg) < 2e-12, (o.rel_residual(Tg), st)
+    rise = np.abs(Td - stack["t_in"]).max()
+    assert np.abs(Tg - Td).max() / rise < 1e-8
+    s.close()
+
+
+def test_reference_tolerance_full_stack(pkg, orc):
+    """512 x 256 x 12 stack (4 Cu base, 4 channel, 4 Cu lid) at the
+    reference stop rule rtol = 1e-6: the oracle's matrix confirms the
+    residual for both precisions, and the metrics match the restatement."""
+    nx, ny = 512, 256
+    stack, gamma, u, v = synthetic_case(pkg, 7, nx, ny, n_base=4, n_channel=4, n_lid=4, h=2e-5)
+    o = orc.OracleThermal(pkg._types, stack, gamma, u, v)
+    out = {}
+    for prec in (64, 32):
+        s = pkg.ThermalSolver3D(stack, gamma, u, v, precision=prec)
+        st = s.solve()
+        print(f"precision {prec}: {st}", flush=True)
+        assert st["converged"] == 1
+        T = s.T
+        assert o.rel_residual(T) < 1.0001e-6
+        m = s.metrics(8.0)
+        ref = orc.thermal_metrics(stack, gamma, u, T, 8.0)
+        for k, val in ref.items():
+            assert m[k] == pytest.approx(val, rel=1e-11, abs=1e-9), k
+        out[prec] = T
+        s.close()
+    rise = np.abs(out[64] - stack["t_in"]).max()
+    assert np.abs(out[32] - out[64]).max() / rise < 1e-3
+
+
+def test_adjoint_solve_matches_direct(pkg, orc):
+    stack, gamma, u, v = synthetic_case(pkg, 5, 30, 14)
+    s = pkg.ThermalSolver3D(stack, gamma, u, v, rtol=1e-12, max_iter=2000)
+    o = orc.OracleThermal(pkg._types, stack, gamma, u, v)
+    rhs = np.random.default_rng(0).standard_normal((s.nz, s.ny, s.nx))
+    lam = s.solve_adjoint(rhs)
+    ref = o.direct(transpose=True, rhs=rhs)
+    assert np.abs(lam - ref).max() / np.abs(ref).max() < 1e-8
+    s.close()
+
+
+@pytest.mark.parametrize("objective,k_mode", [("t_avg_base", 1), ("pnorm", 1), ("t_avg_base", 2), ("pnorm", 0)])
+def test_gradient_matches_central_differences(pkg, orc, objective, k_mode):
+    nx, ny = 12, 8
+    stack, gamma, u, v = synthetic_case(pkg, 11, nx, ny, k_mode=k_mode, qk=3.0)
+    s = pkg.ThermalSolver3D(stack, gamma, u, v, rtol=1e-13, max_iter=2000, max_refine=12)
+    s.solve()
+    J, grad = s.gradient(objective, 8.0)
+    o0 = orc.OracleThermal(pkg._types, stack, gamma, u, v)
+    assert J == pytest.approx(orc.thermal_objective(stack, gamma, u, o0.direct(), objective, 8.0), rel=1e-10)
+    rng = np.random.default_rng(1)
+    cand = np.argwhere((gamma > 0.06) & (gamma < 0.95))
+    cells = cand[rng.choice(len(cand), 8, replace=False)]
+    h = 1e-5
+    gscale = np.abs(grad).max()
+    for (j, i) in cells:
+        Js = []
+        for sgn in (1, -1):
+            g2 = gamma.copy()
+            g2[j, i] += sgn * h
+            o = orc.OracleThermal(pkg._types, stack, g2, u, v)
+            Js.append(orc.thermal_objective(stack, g2, u, o.direct(), objective, 8.0))
+        fd = (Js[0] - Js[1]) / (2 * h)
+        assert grad[j, i] == pytest.approx(fd, rel=1e-5, abs=1e-6 * gscale), (j, i, grad[j, i], fd)
+    s.close()
+
+
+def test_torch_autograd_objective(pkg):
+    import torch
+
+    stack, gamma, u, v = synthetic_case(pkg, 13, 20, 10)
+    s = pkg.ThermalSolver3D(stack, gamma, u, v, rtol=1e-12, max_iter=2000)
+    g = torch.tensor(gamma, dtype=torch.float64, requires_grad=True)
+    J = pkg.thermal_objective(g, s, "pnorm", 8.0)
+    J.backward()
+    J2, grad = s.gradient("pnorm", 8.0)
+    assert float(J) == pytest.approx(J2, rel=1e-14)
+    np.testing.assert_allclose(g.grad.numpy(), grad, rtol=1e-12, atol=0)
+    s.close()
+
+
+def test_thermal_errors_are_reported(pkg):
+    stack, gamma, u, v = synthetic_case(pkg, 0, 8, 4)
+    with pytest.raises(ValueError):
+        pkg.ThermalSolver3D(stack, gamma[:, :3], u, v)
+    s = pkg.ThermalSolver3D(stack, gamma, u, v)
+    with pytest.raises(pkg._capi.CpsError):
+        s.metrics()  # no solve yet
+    s.close()
