@@ -66,6 +66,9 @@ def lib():
         L.cps_simple_boundary_patch.argtypes = [vp, C.c_int, P(T.Patch)]
         L.cps_simple_time_kernel.argtypes = [vp, C.c_int]
         L.cps_simple_kernel_timing.argtypes = [vp, ip, dp, dp]
+        L.cps_simple_gradient_dp.argtypes = [vp, C.c_double, C.c_int, dp, dp, P(T.SolveStats)]
+        L.cps_simple_state_residual.argtypes = [vp, dp]
+        L.cps_simple_jacobian_apply.argtypes = [vp, C.c_int, dp, dp]
         L.cps_thermal_config_default.argtypes = [P(T.ThermalConfig)]
         L.cps_thermal_config_default.restype = None
         L.cps_thermal_create.argtypes = [vp, P(T.ThermalConfig), dp, ip, dp, dp, dp, dp, P(vp)]

@@ -694,6 +694,7 @@ class SimpleSolver {
   DevBuf<double> part;  // SLOT_COUNT x NB block partials
   DevBuf<IterState> st;
   std::unique_ptr<PressureMG> mg;
+  FlowAdjoint* adjoint = nullptr;
   KernelTimer timer;
   int iteration = 0;
   dim3 grid, block;
@@ -789,6 +790,16 @@ class SimpleSolver {
   }
 
   std::vector<double> hinlet;
+
+  ~SimpleSolver() { flow_adjoint_free(adjoint); }
+
+  SimpleView view() const {
+    return SimpleView{g, S, cfg, u.p, v.p, p.p, gam.p, alp.p, inlet.p, dE.p, dN.p};
+  }
+  FlowAdjoint* adj() {
+    if (!adjoint) adjoint = flow_adjoint_new(ctx, g, S);
+    return adjoint;
+  }
 
   Fields fields() {
     return Fields{u.p, v.p, p.p, us.p, vs.p, ps.p, dE.p, dN.p, gam.p, alp.p, inlet.p};
@@ -1107,6 +1118,32 @@ int cps_simple_kernel_timing(cps_simple* s, int* launches, double* avg_us, doubl
     if (launches) *launches = (int)t.launches;
     if (avg_us) *avg_us = t.launches ? 1e3 * t.total_ms / (double)t.launches : 0.0;
     if (bytes_per_launch) *bytes_per_launch = t.bytes_per_launch;
+  });
+}
+
+int cps_simple_gradient_dp(cps_simple* s, double rtol, int max_iter, double* dp, double* dJ_dgamma,
+                           cps_solve_stats* stats) {
+  return guard([&] {
+    CPS_CHECK(s, CPS_ERR_ARG, "null solver");
+    CPS_CHECK(rtol > 0 && max_iter >= 1, CPS_ERR_ARG, "rtol must be > 0 and max_iter >= 1");
+    s->s->apply_bc();
+    cps::flow_adjoint_run(s->s->adj(), s->s->view(), rtol, max_iter, dp, dJ_dgamma, stats);
+  });
+}
+
+int cps_simple_state_residual(cps_simple* s, double* out3) {
+  return guard([&] {
+    CPS_CHECK(s && out3, CPS_ERR_ARG, "null argument");
+    s->s->apply_bc();
+    cps::flow_adjoint_residual(s->s->adj(), s->s->view(), out3);
+  });
+}
+
+int cps_simple_jacobian_apply(cps_simple* s, int transpose, const double* x, double* y) {
+  return guard([&] {
+    CPS_CHECK(s && x && y, CPS_ERR_ARG, "null argument");
+    s->s->apply_bc();
+    cps::flow_adjoint_apply(s->s->adj(), s->s->view(), transpose, x, y);
   });
 }
 

@@ -110,6 +110,22 @@ struct KernelTimer {
   }
 };
 
+// Device state of a SIMPLE solver as seen by the adjoint (cps_adjoint.hip).
+struct SimpleView {
+  Geo g;
+  size_t S;  // elements per padded field
+  cps_simple_config cfg;
+  const double *u, *v, *p, *gam, *alp, *inlet, *dE, *dN;
+};
+
+class FlowAdjoint;
+FlowAdjoint* flow_adjoint_new(Ctx* ctx, const Geo& g, size_t S);
+void flow_adjoint_free(FlowAdjoint* a);
+void flow_adjoint_run(FlowAdjoint* a, const SimpleView& sv, double rtol, int max_iter, double* dp, double* grad,
+                      cps_solve_stats* stats);
+void flow_adjoint_residual(FlowAdjoint* a, const SimpleView& sv, double* out3);
+void flow_adjoint_apply(FlowAdjoint* a, const SimpleView& sv, int transpose, const double* x, double* y);
+
 }  // namespace cps
 
 namespace cps {

@@ -90,6 +90,39 @@ class SIMPLE:
         _capi.check(self.L.cps_simple_boundary_patch(self.h, 1 if inlet else 0, C.byref(m)))
         return m.as_dict()
 
+    # ---- adjoint ---------------------------------------------------------
+    def gradient_dp(self, rtol=1e-10, max_iter=400):
+        """(dP, d(dP)/dgamma (ny, nx)) at the current state by the discrete
+        adjoint of the converged SIMPLE equations."""
+        grad = np.empty((self.cfg.ny, self.cfg.nx))
+        dp = C.c_double()
+        st = T.SolveStats()
+        _capi.check(self.L.cps_simple_gradient_dp(self.h, rtol, max_iter, C.byref(dp), _capi.dptr(grad),
+                                                  C.byref(st)))
+        self.last_adjoint = st.as_dict()
+        return dp.value, grad
+
+    def state_residual(self):
+        """max |R| of the U-momentum, V-momentum and continuity equations."""
+        out = np.empty(3)
+        _capi.check(self.L.cps_simple_state_residual(self.h, _capi.dptr(out)))
+        return out
+
+    def jacobian_apply(self, x, transpose=False):
+        """J x or J^T x; x, y are dicts (or tuples) of u, v, p arrays."""
+        shapes = [T.field_shape(k, self.cfg.nx, self.cfg.ny) for k in ("u", "v", "p")]
+        parts = [x[k] for k in ("u", "v", "p")] if isinstance(x, dict) else list(x)
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float64).ravel() for a in parts]))
+        y = np.empty_like(flat)
+        _capi.check(self.L.cps_simple_jacobian_apply(self.h, 1 if transpose else 0, _capi.dptr(flat),
+                                                     _capi.dptr(y)))
+        out, off = {}, 0
+        for k, sh in zip(("u", "v", "p"), shapes):
+            m = sh[0] * sh[1]
+            out[k] = y[off:off + m].reshape(sh)
+            off += m
+        return out
+
     # ---- measurement -----------------------------------------------------
     KERNELS = {"mg_smooth0": 0, "mom_jacobi": 1, "pcg_stencil": 2}
 

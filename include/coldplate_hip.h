@@ -30,6 +30,14 @@ typedef struct cps_ctx cps_ctx;
 typedef struct cps_simple cps_simple;
 typedef struct cps_thermal cps_thermal;
 
+/* Outcome of an iterative solve (thermal, adjoints). */
+typedef struct cps_solve_stats {
+  int iterations;            /* Krylov iterations */
+  int refinements;           /* restarts from the true residual */
+  double rel_residual;       /* ||b - A x|| / ||b|| in FP64 at exit */
+  int converged;             /* rel_residual <= rtol */
+} cps_solve_stats;
+
 /* Message of the last failed call on the calling thread ("" if none). */
 const char* cps_last_error(void);
 /* ABI version, bumped on any incompatible change. */
@@ -146,6 +154,28 @@ int cps_simple_set_field(cps_simple* s, int field, const double* in);
 /* inlet = 1: x = 0 patch, 0: x = L patch. */
 int cps_simple_boundary_patch(cps_simple* s, int inlet, cps_patch* out);
 
+/* Discrete adjoint of the converged state (no reference counterpart: the
+ * reference optimiser's sensitivities ignore the flow, mma_optimizer.py:
+ * 442-567).  Linearises the momentum and continuity equations the SIMPLE
+ * iteration converges to (momentum_solver.cpp:195-314, convection.cpp:53-186,
+ * pressure_solver.cpp:470-496, boundaries.cpp:50-146) at the current fields
+ * and solves the transposed system (FGMRES, SIMPLE block preconditioner) to
+ * ||g - J^T lambda|| <= rtol ||g||.  dp: the full static pressure drop of
+ * sampleBoundaryPatch (postprocessing.cpp:245-327, dp_fluid_tol rows);
+ * dJ_dgamma (ny x nx, may be NULL): its derivative through the Brinkman term
+ * alpha(gamma) (SIMPLE.cpp:751-777), evaluated at gamma clamped to [0, 1].
+ * The result is exact for the discrete equations when the state is a fixed
+ * point (cps_simple_state_residual). */
+int cps_simple_gradient_dp(cps_simple* s, double rtol, int max_iter, double* dp, double* dJ_dgamma,
+                           cps_solve_stats* stats);
+/* max |R| over the U-momentum, V-momentum and continuity equations at the
+ * current fields (out3[3]). */
+int cps_simple_state_residual(cps_simple* s, double* out3);
+/* y = J x (transpose = 0) or J^T x (1) with J the Jacobian of those
+ * equations at the current fields; x, y are [u | v | p] in the field shapes
+ * above, concatenated; entries that are not DOFs pass through (identity). */
+int cps_simple_jacobian_apply(cps_simple* s, int transpose, const double* x, double* y);
+
 /* Measurement hook (no reference counterpart): brackets every launch of one
  * kernel of the step with HIP events on the solver stream.  kernel:
  * CPS_KT_*; -1 stops timing.  Selecting a kernel clears the counters. */
@@ -179,13 +209,6 @@ typedef struct cps_thermal_config {
   int max_iter;              /* Krylov iterations per solve, all refinements together (:168) */
   int max_refine;            /* restarts from the true FP64 residual */
 } cps_thermal_config;
-
-typedef struct cps_solve_stats {
-  int iterations;            /* Krylov iterations */
-  int refinements;           /* restarts from the true residual */
-  double rel_residual;       /* ||b - A x|| / ||b|| in FP64 at exit */
-  int converged;             /* rel_residual <= rtol */
-} cps_solve_stats;
 
 /* ThermalMetrics (thermal_output.cpp:16-44, without the gradient fields),
  * the objective quantities of compute_optimization_results

@@ -144,6 +144,33 @@ class OracleSimple:
         self.L.orc_simple_csr_get(self.h, w, _i(rp), _i(ci), _d(val), _d(rhs), _d(sol), _i(ij))
         return dict(row_ptr=rp, col_idx=ci, values=val, rhs=rhs, sol=sol, ij=ij.reshape(-1, 2))
 
+    def residual_fields(self, u, v, p):
+        """Residuals of the discrete equations a converged SIMPLE state
+        satisfies, at the given fields: U / V momentum A x - b at x = the
+        field (relaxation cancels) and the mass residual of the pressure
+        DOFs, scattered into arrays of the u, v, p shapes (zero elsewhere).
+        The fields of this object are overwritten."""
+        import scipy.sparse as sp
+        self.set("u", u)
+        self.set("v", v)
+        self.set("p", p)
+        self.apply_bc()
+        self.set("uStar", self.get("u"))
+        self.set("vStar", self.get("v"))
+        out = {}
+        for w in ("u", "v"):
+            sysd = self.assemble(w)
+            m = len(sysd["rhs"])
+            A = sp.csr_matrix((sysd["values"], sysd["col_idx"], sysd["row_ptr"]), shape=(m, m))
+            r = np.zeros(self.shape(w))
+            r[sysd["ij"][:, 0], sysd["ij"][:, 1]] = A @ sysd["sol"] - sysd["rhs"]
+            out[w] = r
+        sysd = self.assemble("p")
+        r = np.zeros(self.shape("p"))
+        r[sysd["ij"][:, 0], sysd["ij"][:, 1]] = sysd["rhs"]
+        out["p"] = r
+        return out
+
     def residuals(self):
         out = np.zeros(6)
         self.L.orc_simple_residuals(self.h, _d(out))
