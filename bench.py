@@ -18,6 +18,14 @@ region.  The CPU baseline is the oracle's FP32 restatement of the
 reference CPU path (Jacobi-PCG pressure, 30 iterations, pTol 1e-6; Jacobi
 momentum) on a bounded sample of the same workload.
 
+Two more legs share the line: grads/s (d(dP)/d(gamma) of a 512x256 density
+field at its SIMPLE fixed point by the discrete adjoint) and the 512x256x12
+conjugate-heat-transfer solve.
+
+--copy-calibration runs only a device-to-device copy of 1 GiB (4 times),
+the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters
+to bytes.
+
 usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20]
 """
 import argparse
@@ -63,6 +71,74 @@ def run_form(pkg, nx, ny, precision, steps, warmup, kernel=None):
     return out
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_mg_down_1024x512_mixed_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the fine-grid k_mg_down<float> measured by the
+    two rocprofv3 --pmc runs recorded in the profile summary (FETCH_SIZE and
+    WRITE_SIZE, scaled by a 1 GiB copy), or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def grads_leg(pkg, nx=512, ny=256, forward_steps=4000, reps=3, rtol=1e-8):
+    """d(dP)/d(gamma) of a 512 x 256 density field: SIMPLE (FP64 GMG) driven
+    to its fixed point, then `reps` full gradients (Jacobian build, adjoint
+    FGMRES, gradient assembly) timed end to end."""
+    T = pkg._types
+    cfg, gamma, inlet = pkg.configs.density_field(nx=nx, ny=ny, seed=0)
+    cfg.pressure_solver = T.CPS_PRESSURE_GMG
+    cfg.p_tol, cfg.p_rtol, cfg.max_pressure_iter = 1e-13, 1e-8, 60
+    cfg.momentum_tol, cfg.max_momentum_iter = 1e-13, 60
+    s = pkg.SIMPLE(cfg, gamma, inlet)
+    t0 = time.perf_counter()
+    st = s.step(forward_steps)
+    s.synchronize()
+    t_fwd = time.perf_counter() - t0
+    s.gradient_dp(rtol=rtol, max_iter=4000)  # warm-up (allocations)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dp, grad = s.gradient_dp(rtol=rtol, max_iter=4000)
+    dt = (time.perf_counter() - t0) / reps
+    out = dict(grads_per_s=1.0 / dt, seconds_per_grad=dt, adjoint=s.last_adjoint, dp=dp,
+               forward_steps=forward_steps, forward_seconds=t_fwd, state_residual=list(s.state_residual()),
+               resid_u=st["resid_u"])
+    s.close()
+    return out
+
+
+def thermal_leg(pkg, reps=3):
+    """512 x 256 x 12 conjugate-heat-transfer solve (4 Cu base, 4 channel,
+    4 Cu lid layers) to the reference's rtol 1e-6, FP64 and FP32-refined;
+    synthetic seeded gamma / velocity / power map."""
+    import numpy as np
+
+    T = pkg._types
+    st = pkg.configs.thermal_stack()
+    rng = np.random.default_rng(0)
+    gamma = pkg.configs.projected_random_field(st["ny"], st["nx"], seed=5, sigma=6.0)
+    u = np.where(gamma > 0.01, 0.1 + 0.4 * rng.uniform(size=gamma.shape), 0.0)
+    v = np.where(gamma > 0.01, 0.05 * rng.standard_normal(gamma.shape), 0.0)
+    out = {}
+    for name, prec in (("fp64", T.CPS_PREC_FP64), ("mixed", T.CPS_PREC_MIXED)):
+        s = pkg.ThermalSolver3D(st, gamma, u, v, precision=prec)
+        s.solve()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = s.solve()
+        dt = (time.perf_counter() - t0) / reps
+        out[name] = dict(seconds=dt, iterations=r["iterations"], refinements=r["refinements"],
+                         rel_residual=r["rel_residual"])
+        s.close()
+    return out
+
+
 def cpu_baseline(pkg, nx, ny, seconds):
     """Oracle FP32 restatement of the reference CPU path, timed for about
     `seconds` of host work (at least 3 SIMPLE iterations)."""
@@ -93,7 +169,21 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
+    ap.add_argument("--no-grads", action="store_true", help="skip the d(dP)/d(gamma) leg")
+    ap.add_argument("--no-thermal", action="store_true", help="skip the 3-D thermal leg")
+    ap.add_argument("--copy-calibration", action="store_true", help="only copy 1 GiB on the device (4x) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        n = (1 << 30) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n}), flush=True)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -115,11 +205,13 @@ def main():
     if kt and kt["launches"] > 0 and kt["avg_us"] > 0:
         gbs = kt["bytes_per_launch"] / (kt["avg_us"] * 1e-6) / 1e9
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                 "kernel": "k_mg_down<float> on the fine grid (2 pressure smoother sweeps + residual + restriction)",
                 "avg_us": round(kt["avg_us"], 3), "launches": kt["launches"],
                 "bytes_per_launch": kt["bytes_per_launch"]}
 
+    grads = None if args.no_grads else grads_leg(pkg)
+    thermal = None if args.no_thermal else thermal_leg(pkg)
     cpu = None if args.no_cpu else cpu_baseline(pkg, nx, ny, args.cpu_seconds)
 
     dp_err = None
@@ -150,7 +242,21 @@ def main():
         "dp_fp64_pa": None if fp64 is None else fp64["dp"],
         "dp_rel_err": dp_err,
         "cell_updates_per_s": round(mixed["it_s"] * nx * ny, 1),
-        "grads_per_s": None,
+        "grads_per_s": None if grads is None else round(grads["grads_per_s"], 3),
+        "grads": None if grads is None else {
+            "workload": "d(dP)/d(gamma), 512x256 density field (seed 0, SOU) at its SIMPLE fixed point; "
+                        "Jacobian build + adjoint FGMRES (rtol 1e-8) + gradient",
+            "seconds_per_grad": round(grads["seconds_per_grad"], 4),
+            "fgmres_iterations": grads["adjoint"]["iterations"],
+            "adjoint_rel_residual": grads["adjoint"]["rel_residual"],
+            "forward_steps": grads["forward_steps"], "forward_seconds": round(grads["forward_seconds"], 3),
+            "state_residual_max": max(grads["state_residual"]),
+        },
+        "thermal": None if thermal is None else {
+            "workload": "512x256x12 CHT stack (4 Cu base, 4 channel, 4 Cu lid), rtol 1e-6",
+            **{k: {kk: (float(f"{vv:.4g}") if isinstance(vv, float) else vv) for kk, vv in d.items()}
+               for k, d in thermal.items()},
+        },
         "p_iters_last": mixed["stats"]["p_iters"],
         "roofline": roof,
         "cpu_baseline": cpu,

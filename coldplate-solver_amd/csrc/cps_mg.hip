@@ -797,9 +797,10 @@ struct MGImpl {
       PcgTest test{nullptr, 0, 0, nullptr, 0.0, 0.0};
       if (l == 0)
         test = PcgTest{part.p + (size_t)(it == 0 ? S_RR0 : S_ERR) * G0.NB, G0.NB, it, scal.p, abs_tol_, rtol_};
-      if (l == 0) timer->begin(KT_MG_SMOOTH0);
-      k_mg_down<T><<<tgrid(l), dim3(TX, TY), 0, s>>>(view(l), view(l + 1), stop, test);
-      if (l == 0) timer->end(KT_MG_SMOOTH0);
+      if (l == 0 && timer->active(KT_MG_SMOOTH0))
+        timer->ext_launch(k_mg_down<T>, tgrid(l), dim3(TX, TY), 0, s, view(l), view(l + 1), stop, test);
+      else
+        k_mg_down<T><<<tgrid(l), dim3(TX, TY), 0, s>>>(view(l), view(l + 1), stop, test);
     }
     k_mg_coarse<T><<<1, kCoarseThreads, coarseSmem, s>>>(view(nGrid), cd, cf.p, stop);
     for (int l = nGrid - 1; l >= 0; --l)
@@ -830,9 +831,11 @@ struct MGImpl {
       vcycle(&ps->stop, it);
       T* po = (it & 1) ? pb.p : pa.p;
       T* pn = (it & 1) ? pa.p : pb.p;
-      timer->begin(KT_PCG_STENCIL);
-      k_pcg_ap<T><<<grid0(), blk0(), 0, s>>>(G0, L, lv[0]->y.p, po, pn, Ap.p, it, ps, part.p);
-      timer->end(KT_PCG_STENCIL);
+      if (timer->active(KT_PCG_STENCIL))
+        timer->ext_launch(k_pcg_ap<T>, grid0(), blk0(), 0, s, G0, L, (const T*)lv[0]->y.p, (const T*)po, pn, Ap.p,
+                          it, ps, part.p);
+      else
+        k_pcg_ap<T><<<grid0(), blk0(), 0, s>>>(G0, L, lv[0]->y.p, po, pn, Ap.p, it, ps, part.p);
       k_pcg_xr<T><<<grid0(), blk0(), 0, s>>>(G0, L, xs.p, r0.p, pn, Ap.p, it, ps, part.p);
       CPS_HIP(hipGetLastError());
       if (sync_every > 0 && (it + 1) % sync_every == 0 && it + 1 < maxit) {

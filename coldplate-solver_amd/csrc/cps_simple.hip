@@ -843,12 +843,15 @@ class SimpleSolver {
     while (k < maxit) {
       const int k1 = std::min(maxit, k + chunk);
       for (; k < k1; ++k) {
-        timer.begin(KT_MOM_JACOBI);
-        if (mode_fp64)
+        const bool timed = timer.active(KT_MOM_JACOBI);
+        if (mode_fp64 && timed)
+          timer.ext_launch(k_jacobi<double>, grid, block, 0, s, g, sys(momD, 0), sys(momD, 1), k, tol2, st.p, part.p);
+        else if (mode_fp64)
           k_jacobi<double><<<grid, block, 0, s>>>(g, sys(momD, 0), sys(momD, 1), k, tol2, st.p, part.p);
+        else if (timed)
+          timer.ext_launch(k_jacobi<float>, grid, block, 0, s, g, sys(momF, 0), sys(momF, 1), k, tol2, st.p, part.p);
         else
           k_jacobi<float><<<grid, block, 0, s>>>(g, sys(momF, 0), sys(momF, 1), k, tol2, st.p, part.p);
-        timer.end(KT_MOM_JACOBI);
       }
       // x, cS, cW, aP, cE, cN, b read + x' written, both components
       if (timer.which == KT_MOM_JACOBI)
@@ -880,9 +883,11 @@ class SimpleSolver {
     while (k < maxit) {
       const int k1 = std::min(maxit, k + chunk);
       for (; k < k1; ++k) {
-        timer.begin(KT_PCG_STENCIL);
-        k_pcg_a<<<grid, block, 0, s>>>(g, dE.p, dN.p, q, k, cfg.p_tol, st.p, part.p);
-        timer.end(KT_PCG_STENCIL);
+        if (timer.active(KT_PCG_STENCIL))
+          timer.ext_launch(k_pcg_a, grid, block, 0, s, g, (const double*)dE.p, (const double*)dN.p, q, k, cfg.p_tol,
+                           st.p, part.p);
+        else
+          k_pcg_a<<<grid, block, 0, s>>>(g, dE.p, dN.p, q, k, cfg.p_tol, st.p, part.p);
         k_pcg_b<<<grid, block, 0, s>>>(g, ps.p, q, k, st.p, part.p);
       }
       k_pcg_end<<<1, block, 0, s>>>(g, k, cfg.p_tol, st.p, part.p);

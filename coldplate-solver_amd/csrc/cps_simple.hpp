@@ -4,6 +4,7 @@
 #include <memory>
 #include <vector>
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "cps_common.hpp"
@@ -66,7 +67,10 @@ enum TimedKernel {
   KT_PCG_STENCIL = 2,  // fine-level PCG p-update + 5-point apply
 };
 
-// Brackets the launches of one kernel with events; drain() sums them.
+// Times the launches of one kernel: the selected kernel is dispatched with
+// hipExtLaunchKernel, whose start / stop events are written by the dispatch
+// itself (kernel execution only, like a profiler's kernel trace); drain()
+// sums the pairs.
 struct KernelTimer {
   int which = KT_NONE;
   hipStream_t stream = nullptr;
@@ -75,19 +79,19 @@ struct KernelTimer {
   size_t launches = 0;
   double total_ms = 0.0;
   double bytes_per_launch = 0.0;
-  void mark(int id) {
-    if (which != id) return;
+  bool active(int id) const { return which == id; }
+  hipEvent_t next_event() {
     if (used == ev.size()) {
       hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
       ev.push_back(e);
     }
-    (void)hipEventRecord(ev[used++], stream);
+    return ev[used++];
   }
-  void begin(int id) { mark(id); }
-  void end(int id) {
-    if (which != id) return;
-    mark(id);
+  template <class F, class... Args>
+  void ext_launch(F kernel, dim3 grid, dim3 block, unsigned shm, hipStream_t s, Args... args) {
+    hipEvent_t a = next_event(), b = next_event();
+    hipExtLaunchKernelGGL(kernel, grid, block, shm, s, a, b, 0, args...);
     launches++;
   }
   // adds the elapsed time of the recorded (begin, end) pairs to total_ms

@@ -184,3 +184,44 @@ def momentum_jacobi_csr(row_ptr, col_idx, values, rhs, solution, tolerance, max_
                                               _capi.fptr(b), _capi.fptr(solution), tolerance, max_iterations,
                                               C.byref(it)))
     return it.value
+
+
+def _pressure_drop_function():
+    import torch
+
+    class PressureDrop(torch.autograd.Function):
+        """dP(gamma): forward iterates SIMPLE to a fixed point, backward is
+        one discrete-adjoint solve at that state."""
+
+        @staticmethod
+        def forward(ctx, gamma, cfg, inlet, steps, rtol, device):
+            g = gamma.detach().to("cpu", torch.float64).numpy()
+            s = SIMPLE(cfg, g, inlet, device=device)
+            s.step(steps)
+            ctx.solver, ctx.rtol = s, rtol
+            ctx.gdev, ctx.gdtype = gamma.device, gamma.dtype
+            return gamma.new_tensor(s.full_static_drop())
+
+        @staticmethod
+        def backward(ctx, gout):
+            import torch as _t
+
+            _, grad = ctx.solver.gradient_dp(rtol=ctx.rtol, max_iter=4000)
+            ctx.solver.close()
+            g = _t.from_numpy(grad).to(ctx.gdev, ctx.gdtype)
+            return gout * g, None, None, None, None, None
+
+    return PressureDrop
+
+
+_PD = None
+
+
+def pressure_drop(gamma, cfg, inlet=None, steps=4000, rtol=1e-8, device=0):
+    """Differentiable full static pressure drop (0-d tensor) of the density
+    field `gamma` (ny x nx tensor): `steps` SIMPLE iterations from rest, then
+    d(dP)/d(gamma) by the discrete adjoint on backward."""
+    global _PD
+    if _PD is None:
+        _PD = _pressure_drop_function()
+    return _PD.apply(gamma, cfg, inlet, steps, rtol, device)

@@ -199,3 +199,18 @@ def test_jacobian_matches_oracle_residual_differences(pkg, orc, T, scheme):
     rhs = sum((x[k][masks[k]] * Jd[k][masks[k]]).sum() for k in masks)
     assert lhs == pytest.approx(rhs, rel=1e-12)
     s.close()
+
+
+def test_torch_autograd_pressure_drop(pkg):
+    import torch
+
+    cfg, g, inlet = pkg.configs.density_field(nx=48, ny=24, seed=2, sigma=3.0)
+    gt = torch.tensor(g, requires_grad=True)
+    dp = pkg.pressure_drop(gt, cfg, inlet, steps=3000, rtol=1e-10)
+    dp.backward()
+    s = pkg.SIMPLE(cfg, g, inlet)
+    s.step(3000)
+    dp2, grad = s.gradient_dp(rtol=1e-10)
+    assert float(dp) == pytest.approx(dp2, rel=1e-12)
+    np.testing.assert_allclose(gt.grad.numpy(), grad, rtol=1e-8, atol=1e-10 * np.abs(grad).max())
+    s.close()
