@@ -18,9 +18,10 @@ region.  The CPU baseline is the oracle's FP32 restatement of the
 reference CPU path (Jacobi-PCG pressure, 30 iterations, pTol 1e-6; Jacobi
 momentum) on a bounded sample of the same workload.
 
-Two more legs share the line: grads/s (d(dP)/d(gamma) of a 512x256 density
-field at its SIMPLE fixed point by the discrete adjoint) and the 512x256x12
-conjugate-heat-transfer solve.
+More legs share the line: grads/s (d(dP)/d(gamma) of a 512x256 density
+field at its SIMPLE fixed point by the discrete adjoint), a 48-field batch
+of mixed sizes stepped by per-stream hipGraph launches, and the
+512x256x12 conjugate-heat-transfer solve.
 
 --copy-calibration runs only a device-to-device copy of 1 GiB (4 times),
 the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters
@@ -30,6 +31,7 @@ usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20]
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -113,6 +115,34 @@ def grads_leg(pkg, nx=512, ny=256, forward_steps=4000, reps=3, rtol=1e-8):
     return out
 
 
+def batch_leg(pkg, steps=500, n_fields=48):
+    """48 density fields, sizes drawn from {256x128, 512x256, 768x384,
+    1024x512}, sigma ~ U[4, 12], per-field seeds, the headline knobs
+    (FP32-inner GMG-PCG, p_rtol 1e-3, <= 6 PCG iterations, <= 8 Jacobi
+    sweeps): one hipGraph launch per stream and batch iteration, stream count
+    picked by the autotuner, `steps` iterations timed."""
+    T = pkg._types
+    fl = []
+    for cfg, gamma, inlet in pkg.configs.batch_sweep(n_fields=n_fields):
+        cfg.pressure_solver, cfg.precision = T.CPS_PRESSURE_GMG, T.CPS_PREC_MIXED
+        cfg.p_tol, cfg.p_rtol, cfg.max_pressure_iter, cfg.max_refine = 0.0, 1e-3, 6, 1
+        cfg.max_momentum_iter = 8
+        fl.append((cfg, gamma, inlet))
+    b = pkg.Batch(fl)
+    ns, us_tune = b.tune(streams=0, reps=3)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.synchronize()
+    dt = time.perf_counter() - t0
+    cells = sum(c.nx * c.ny for c, _, _ in fl)
+    finite = all(math.isfinite(b.stats(k)["error"]) for k in range(b.n))
+    b.close()
+    return dict(fields=n_fields, steps=steps, seconds=dt, streams=ns, batch_it_s=steps / dt,
+                field_it_s=n_fields * steps / dt, cell_updates_per_s=cells * steps / dt, total_cells=cells,
+                all_finite=finite)
+
+
 def thermal_leg(pkg, reps=3):
     """512 x 256 x 12 conjugate-heat-transfer solve (4 Cu base, 4 channel,
     4 Cu lid layers) to the reference's rtol 1e-6, FP64 and FP32-refined;
@@ -171,6 +201,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
     ap.add_argument("--no-grads", action="store_true", help="skip the d(dP)/d(gamma) leg")
     ap.add_argument("--no-thermal", action="store_true", help="skip the 3-D thermal leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 48-field batch leg")
     ap.add_argument("--copy-calibration", action="store_true", help="only copy 1 GiB on the device (4x) and exit")
     args = ap.parse_args()
     if args.copy_calibration:
@@ -190,6 +221,9 @@ def main():
         print("bench.py measures one GPU only: the SIMPLE path runs one density field per device and "
               "this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    # hardware queues per process for the concurrent streams of the batch
+    # leg (HIP reads it at initialisation; the caller's setting wins)
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
     if "OMP_NUM_THREADS" not in os.environ:
         os.environ["OMP_NUM_THREADS"] = str(min(16, os.cpu_count() or 1))
 
@@ -212,6 +246,7 @@ def main():
 
     grads = None if args.no_grads else grads_leg(pkg)
     thermal = None if args.no_thermal else thermal_leg(pkg)
+    batch = None if args.no_batch else batch_leg(pkg)
     cpu = None if args.no_cpu else cpu_baseline(pkg, nx, ny, args.cpu_seconds)
 
     dp_err = None
@@ -251,6 +286,11 @@ def main():
             "adjoint_rel_residual": grads["adjoint"]["rel_residual"],
             "forward_steps": grads["forward_steps"], "forward_seconds": round(grads["forward_seconds"], 3),
             "state_residual_max": max(grads["state_residual"]),
+        },
+        "batch": None if batch is None else {
+            "workload": "48 density fields, sizes from {256x128, 512x256, 768x384, 1024x512}, sigma U[4,12], "
+                        "headline knobs; one hipGraph launch per stream per batch iteration",
+            **{k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in batch.items()},
         },
         "thermal": None if thermal is None else {
             "workload": "512x256x12 CHT stack (4 Cu base, 4 channel, 4 Cu lid), rtol 1e-6",

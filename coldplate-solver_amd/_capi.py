@@ -69,6 +69,15 @@ def lib():
         L.cps_simple_gradient_dp.argtypes = [vp, C.c_double, C.c_int, dp, dp, P(T.SolveStats)]
         L.cps_simple_state_residual.argtypes = [vp, dp]
         L.cps_simple_jacobian_apply.argtypes = [vp, C.c_int, dp, dp]
+        L.cps_batch_create.argtypes = [vp, C.c_int, P(T.SimpleConfig), P(dp), P(dp), P(vp)]
+        L.cps_batch_destroy.argtypes = [vp]
+        L.cps_batch_destroy.restype = None
+        L.cps_batch_tune.argtypes = [vp, C.c_int, C.c_int, ip, dp]
+        L.cps_batch_step.argtypes = [vp, C.c_int]
+        L.cps_batch_synchronize.argtypes = [vp]
+        L.cps_batch_field_stats.argtypes = [vp, C.c_int, P(T.StepStats)]
+        L.cps_batch_get_field.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.cps_batch_boundary_patch.argtypes = [vp, C.c_int, C.c_int, P(T.Patch)]
         L.cps_thermal_config_default.argtypes = [P(T.ThermalConfig)]
         L.cps_thermal_config_default.restype = None
         L.cps_thermal_create.argtypes = [vp, P(T.ThermalConfig), dp, ip, dp, dp, dp, dp, P(vp)]

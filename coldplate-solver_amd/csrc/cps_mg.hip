@@ -311,6 +311,7 @@ struct CoarseDesc {
   int nlev;
   int nx[16], ny[16], off[16];  // compact LDS layout, pitch = nx
   int total;
+  int wl;  // first level of at most 64 cells: it and the coarser ones run in wave 0
 };
 
 #define CELLS(l)                                              \
@@ -362,11 +363,13 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, C
 }
 
 // All levels from `top` down inside one workgroup, in LDS: per level one
-// pre-smoothing launch-free phase (both sweeps from zero fused: x1 =
-// x0 + w (b - A x0)/d with x0 = w b/d formed on the fly), one restriction
-// phase, and two post-smoothing phases (the first with the prolongation
-// fused).  The coarsest level (one cell) is solved exactly in FP64.  The top
-// level's correction goes to top.y.
+// pre-smoothing phase (both sweeps from zero fused: x1 = x0 + w (b - A x0)/d
+// with x0 = w b/d formed on the fly), one restriction phase, and two
+// post-smoothing phases (the first with the prolongation fused).  Levels of
+// more than 64 cells separate their phases with workgroup barriers; the
+// smaller ones run in wave 0 alone, one cell per lane, with wavefront-scope
+// ordering only.  The coarsest level (one cell) is solved exactly in FP64.
+// The top level's correction goes to top.y.
 template <class T>
 __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseDesc d, const T* cf, const int* stop) {
   if (*stop) return;
@@ -418,54 +421,86 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
     if (r > 0) a -= sn[k - nx] * xv(k - nx);
     return a;
   };
-  const int last = d.nlev - 1;
-  for (int l = 0; l < last; ++l) {
-    const int o = d.off[l], nx = d.nx[l], ny = d.ny[l];
-    // x1 = x0 + w (b - A x0) / d, x0 = w b / d
+  const int last = d.nlev - 1, wl = d.wl;
+  // one smoothing sweep from x0 = w b / d into sy, then the restricted
+  // residual into the next level's right side; cell (r, c) of level l
+  auto down_sweep = [&](int l, int r, int c) {
+    const int o = d.off[l], nx = d.nx[l];
     auto x0 = [&](int k) -> T { return om * sb[k] * si[k]; };
-    CELLS(l) {
-      const int k = o + r * nx + c;
-      sy[k] = x0(k) + om * (sb[k] - ax(x0, l, k, r, c)) * si[k];
-    }
-    __syncthreads();
-    const int co = d.off[l + 1], cx = d.nx[l + 1];
+    const int k = o + r * nx + c;
+    sy[k] = x0(k) + om * (sb[k] - ax(x0, l, k, r, c)) * si[k];
+  };
+  auto restrict_cell = [&](int l, int r, int c) {
+    const int o = d.off[l], nx = d.nx[l], ny = d.ny[l], co = d.off[l + 1], cx = d.nx[l + 1];
     auto x1 = [&](int k) -> T { return sy[k]; };
-    CELLS(l + 1) {
-      T acc = (T)0;
-      for (int dr = 0; dr < 2; ++dr)
-        for (int dc = 0; dc < 2; ++dc) {
-          const int rr = 2 * r + dr, cc = 2 * c + dc;
-          if (rr < ny && cc < nx) {
-            const int k = o + rr * nx + cc;
-            acc += sb[k] - ax(x1, l, k, rr, cc);
-          }
+    T acc = (T)0;
+    for (int dr = 0; dr < 2; ++dr)
+      for (int dc = 0; dc < 2; ++dc) {
+        const int rr = 2 * r + dr, cc = 2 * c + dc;
+        if (rr < ny && cc < nx) {
+          const int k = o + rr * nx + cc;
+          acc += sb[k] - ax(x1, l, k, rr, cc);
         }
-      sb[co + r * cx + c] = acc;
-    }
+      }
+    sb[co + r * cx + c] = acc;
+  };
+  // first up sweep: x2 = x1 + P e formed on the fly, into sx
+  auto up_sweep1 = [&](int l, int r, int c) {
+    const int o = d.off[l], nx = d.nx[l], co = d.off[l + 1], cx = d.nx[l + 1];
+    auto x2 = [&](int k) -> T {
+      const int q = k - o, rr = q / nx, cc = q - rr * nx;
+      return sy[k] + sy[co + (rr >> 1) * cx + (cc >> 1)];
+    };
+    const int k = o + r * nx + c;
+    sx[k] = x2(k) + om * (sb[k] - ax(x2, l, k, r, c)) * si[k];
+  };
+  auto up_sweep2 = [&](int l, int r, int c) {
+    const int o = d.off[l], nx = d.nx[l];
+    auto x3 = [&](int k) -> T { return sx[k]; };
+    const int k = o + r * nx + c;
+    sy[k] = x3(k) + om * (sb[k] - ax(x3, l, k, r, c)) * si[k];
+  };
+  for (int l = 0; l < wl; ++l) {
+    CELLS(l) { down_sweep(l, r, c); }
+    __syncthreads();
+    CELLS(l + 1) { restrict_cell(l, r, c); }
     __syncthreads();
   }
-  if (t == 0) {  // one cell: exact, in FP64
-    const int k = d.off[last];
-    sy[k] = (T)((double)sb[k] * (double)si[k]);
+  // Levels of at most 64 cells: one cell per lane of wave 0.  A wavefront's
+  // LDS accesses complete in order, so wavefront-scope fences (which keep
+  // the compiler from moving LDS accesses across them) order the phases.
+  if (t < 64) {
+    auto wave_sync = [] {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    };
+    for (int l = wl; l < last; ++l) {
+      const int nx = d.nx[l], n = nx * d.ny[l];
+      if (t < n) down_sweep(l, t / nx, t - (t / nx) * nx);
+      wave_sync();
+      const int cx = d.nx[l + 1], cn = cx * d.ny[l + 1];
+      if (t < cn) restrict_cell(l, t / cx, t - (t / cx) * cx);
+      wave_sync();
+    }
+    if (t == 0) {  // one cell: exact, in FP64
+      const int k = d.off[last];
+      sy[k] = (T)((double)sb[k] * (double)si[k]);
+    }
+    wave_sync();
+    for (int l = last - 1; l >= wl; --l) {
+      const int nx = d.nx[l], n = nx * d.ny[l];
+      if (t < n) up_sweep1(l, t / nx, t - (t / nx) * nx);
+      wave_sync();
+      if (t < n) up_sweep2(l, t / nx, t - (t / nx) * nx);
+      wave_sync();
+    }
   }
   __syncthreads();
-  for (int l = last - 1; l >= 0; --l) {
-    const int o = d.off[l], nx = d.nx[l], co = d.off[l + 1], cx = d.nx[l + 1];
-    // x2 = x1 + P e formed on the fly, first sweep into sx
-    auto x2 = [&](int k) -> T {
-      const int q = k - o, r = q / nx, c = q - (q / nx) * nx;
-      return sy[k] + sy[co + (r >> 1) * cx + (c >> 1)];
-    };
-    CELLS(l) {
-      const int k = o + r * nx + c;
-      sx[k] = x2(k) + om * (sb[k] - ax(x2, l, k, r, c)) * si[k];
-    }
+  for (int l = wl - 1; l >= 0; --l) {
+    CELLS(l) { up_sweep1(l, r, c); }
     __syncthreads();
-    auto x3 = [&](int k) -> T { return sx[k]; };
-    CELLS(l) {
-      const int k = o + r * nx + c;
-      sy[k] = x3(k) + om * (sb[k] - ax(x3, l, k, r, c)) * si[k];
-    }
+    CELLS(l) { up_sweep2(l, r, c); }
     __syncthreads();
   }
   CELLS(0) { top.y[top.at(r, c)] = sy[r * d.nx[0] + c]; }
@@ -680,6 +715,10 @@ __global__ void k_copy_state(IterState* st, const PcgScal* inner, const PcgScal*
   }
 }
 
+__global__ __launch_bounds__(256) void k_copy_f64(const double* src, double* dst, size_t n) {
+  for (size_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) dst[q] = src[q];
+}
+
 template <class T>
 __global__ __launch_bounds__(MNT) void k_copy_out(Lv<T> L, const T* src, double* x64, int P) {
   const int c = blockIdx.x * MBX + threadIdx.x, r = blockIdx.y * MBY + threadIdx.y;
@@ -750,6 +789,12 @@ struct MGImpl {
     }
     cd.nlev = n;
     cd.total = off;
+    cd.wl = n - 1;
+    for (int l = 0; l < n; ++l)
+      if (cd.nx[l] * cd.ny[l] <= 64) {
+        cd.wl = l;
+        break;
+      }
     CPS_CHECK(off <= 3 * kCoarseThreads && lv.back()->nx * lv.back()->ny <= 2 * kCoarseThreads, CPS_ERR_STATE,
               "coarse multigrid plan exceeds the coarse solver's staging");
     coarseSmem = (size_t)6 * off * sizeof(T);
@@ -888,7 +933,8 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
     double* b64 = static_cast<DevBuf<double>*>(b64_)->p;
     m->timer = timer;
     m->setup(dE, dN);
-    CPS_HIP(hipMemcpyAsync(b64, r, S * sizeof(double), hipMemcpyDeviceToDevice, s));
+    // a kernel, not a memcpy node, so the step stays graph-capturable
+    k_copy_f64<<<std::min(ceil_div((int)S, 256), 2048), 256, 0, s>>>(r, b64, S);
     PcgScal* inner = m->scal.p;
     PcgScal* outer = m->scal.p + 1;
     k_outer_reset<<<1, 64, 0, s>>>(outer);

@@ -31,6 +31,8 @@
 #include <cstring>
 #include <vector>
 
+#include <chrono>
+
 #include "cps_mg.hpp"
 #include "cps_simple.hpp"
 
@@ -1024,6 +1026,151 @@ void SimpleSolver::pressure_mg() {
   mg->solve(dE.p, dN.p, q.r, ps.p, st.p, cfg, mode_fp64 != 0, &timer);
 }
 
+// ===========================================================================
+// Batch of fields: every field keeps its own solver state.  Fields are
+// assigned largest-first to the least loaded of `nstreams` streams, and one
+// SIMPLE iteration of each stream's fields is captured once into a hipGraph
+// of its own; a batch iteration launches one graph per stream, so the
+// fields' small kernels run concurrently on the otherwise idle CUs (the
+// parallel branches of a single graph are executed one after another by
+// this runtime, which is why the graph is split per stream).
+class BatchSolver {
+ public:
+  Ctx* base;
+  std::vector<std::unique_ptr<Ctx>> fctx;
+  std::vector<std::unique_ptr<SimpleSolver>> f;
+  std::vector<hipStream_t> streams;
+  std::vector<hipGraph_t> graphs;
+  std::vector<hipGraphExec_t> execs;
+  int nstreams = 0;
+  double us_per_step = 0.0;
+  static constexpr int kMaxStreams = 16;
+
+  BatchSolver(Ctx* c, int n, const cps_simple_config* cfgs, const double* const* gammas,
+              const double* const* inlets)
+      : base(c) {
+    CPS_CHECK(n >= 1 && cfgs && gammas, CPS_ERR_ARG, "batch needs n >= 1 fields with configs and gammas");
+    for (int i = 0; i < n; ++i) {
+      const cps_simple_config& cf = cfgs[i];
+      // capture-safe caps: no host synchronisation inside one iteration
+      CPS_CHECK(cf.max_momentum_iter <= SimpleSolver::kNoSyncCap && cf.max_pressure_iter <= SimpleSolver::kNoSyncCap &&
+                    (cf.precision != CPS_PREC_MIXED || cf.max_refine <= 4),
+                CPS_ERR_ARG, "batched fields need max_momentum_iter, max_pressure_iter <= 32 and max_refine <= 4");
+    }
+    for (int k = 0; k < kMaxStreams; ++k) {
+      hipStream_t st;
+      CPS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));  // default priority
+      streams.push_back(st);
+    }
+    for (int i = 0; i < n; ++i) {
+      fctx.emplace_back(new Ctx(*c));
+      fctx.back()->stream = streams[0];
+      f.emplace_back(new SimpleSolver(fctx.back().get(), cfgs[i], gammas[i], inlets ? inlets[i] : nullptr));
+    }
+    // one eager iteration per field: creates the lazily built multigrid
+    // state (allocations and clears stay out of the captured graphs)
+    for (auto& s : f) {
+      s->apply_bc();
+      s->step_once();
+    }
+    CPS_HIP(hipStreamSynchronize(streams[0]));
+  }
+
+  ~BatchSolver() {
+    (void)sync_all_nothrow();
+    release_graphs();
+    f.clear();
+    for (auto st : streams) (void)hipStreamDestroy(st);
+  }
+
+  bool sync_all_nothrow() {
+    bool ok = true;
+    for (auto st : streams) ok = hipStreamSynchronize(st) == hipSuccess && ok;
+    return ok;
+  }
+  void sync_all() {
+    for (auto st : streams) CPS_HIP(hipStreamSynchronize(st));
+  }
+
+  void release_graphs() {
+    for (auto e : execs) (void)hipGraphExecDestroy(e);
+    for (auto g : graphs) (void)hipGraphDestroy(g);
+    execs.clear();
+    graphs.clear();
+  }
+
+  // greedy longest-processing-time assignment (cost ~ cells)
+  std::vector<int> assign(int ns) {
+    std::vector<int> order(f.size()), owner(f.size(), 0);
+    for (size_t i = 0; i < f.size(); ++i) order[i] = (int)i;
+    auto cells = [&](int i) { return (double)f[i]->cfg.nx * f[i]->cfg.ny; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells(a) > cells(b); });
+    std::vector<double> load(ns, 0.0);
+    for (int i : order) {
+      const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      owner[i] = k;
+      load[k] += cells(i);
+    }
+    return owner;
+  }
+
+  void capture(int ns) {
+    sync_all();
+    release_graphs();
+    ns = std::max(1, std::min(ns, std::min((int)f.size(), kMaxStreams)));
+    const std::vector<int> owner = assign(ns);
+    for (size_t i = 0; i < f.size(); ++i) fctx[i]->stream = streams[owner[i]];
+    for (int k = 0; k < ns; ++k) {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t e = nullptr;
+      CPS_HIP(hipStreamBeginCapture(streams[k], hipStreamCaptureModeThreadLocal));
+      for (size_t i = 0; i < f.size(); ++i)
+        if (owner[i] == k) {
+          f[i]->step_once();
+          f[i]->iteration--;  // capture does not run an iteration
+        }
+      CPS_HIP(hipStreamEndCapture(streams[k], &g));
+      graphs.push_back(g);
+      CPS_HIP(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+      execs.push_back(e);
+    }
+    nstreams = ns;
+  }
+
+  void launch(int n) {
+    CPS_CHECK(!execs.empty(), CPS_ERR_STATE, "batch graphs not built: call cps_batch_tune first");
+    for (int it = 0; it < n; ++it)
+      for (int k = 0; k < nstreams; ++k) CPS_HIP(hipGraphLaunch(execs[k], streams[k]));
+    for (auto& s : f) s->iteration += n;
+  }
+
+  // times `reps` batch iterations per candidate stream count, keeps the best
+  void tune(int fixed, int reps) {
+    std::vector<int> cand;
+    if (fixed > 0) cand.push_back(fixed);
+    else
+      for (int ns : {1, 2, 4, 8, 16})
+        if (ns <= (int)f.size()) cand.push_back(ns);
+    int best = cand[0];
+    double best_us = 1e300;
+    for (int ns : cand) {
+      capture(ns);
+      launch(1);  // warm-up
+      sync_all();
+      const auto t0 = std::chrono::steady_clock::now();
+      launch(reps);
+      sync_all();
+      const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() / reps;
+      if (us < best_us) {
+        best_us = us;
+        best = ns;
+      }
+    }
+    if (best != nstreams) capture(best);
+    us_per_step = best_us;
+  }
+};
+
 }  // namespace cps
 
 struct cps_simple {
@@ -1156,6 +1303,83 @@ int cps_simple_boundary_patch(cps_simple* s, int inlet, cps_patch* out) {
   return guard([&] {
     CPS_CHECK(s && out, CPS_ERR_ARG, "null argument");
     s->s->patch(inlet, out);
+  });
+}
+
+}  // extern "C"
+
+struct cps_batch {
+  cps::BatchSolver* b;
+};
+
+extern "C" {
+
+int cps_batch_create(cps_ctx* ctx, int n_fields, const cps_simple_config* cfgs, const double* const* gammas,
+                     const double* const* inlets, cps_batch** out) {
+  return guard([&] {
+    CPS_CHECK(ctx && out, CPS_ERR_ARG, "null argument");
+    CPS_HIP(hipSetDevice(ctx->c.device));
+    auto* h = new cps_batch{nullptr};
+    try {
+      h->b = new cps::BatchSolver(&ctx->c, n_fields, cfgs, gammas, inlets);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+void cps_batch_destroy(cps_batch* b) {
+  if (!b) return;
+  delete b->b;
+  delete b;
+}
+
+int cps_batch_tune(cps_batch* b, int streams, int reps, int* chosen, double* us_per_step) {
+  return guard([&] {
+    CPS_CHECK(b && streams >= 0 && reps >= 1, CPS_ERR_ARG, "bad arguments");
+    b->b->tune(streams, reps);
+    if (chosen) *chosen = b->b->nstreams;
+    if (us_per_step) *us_per_step = b->b->us_per_step;
+  });
+}
+
+int cps_batch_step(cps_batch* b, int n_steps) {
+  return guard([&] {
+    CPS_CHECK(b && n_steps >= 0, CPS_ERR_ARG, "bad arguments");
+    b->b->launch(n_steps);
+  });
+}
+
+int cps_batch_synchronize(cps_batch* b) {
+  return guard([&] {
+    CPS_CHECK(b, CPS_ERR_ARG, "null batch");
+    b->b->sync_all();
+  });
+}
+
+int cps_batch_field_stats(cps_batch* b, int field, cps_step_stats* out) {
+  return guard([&] {
+    CPS_CHECK(b && out && field >= 0 && field < (int)b->b->f.size(), CPS_ERR_ARG, "bad field index");
+    b->b->sync_all();
+    b->b->f[field]->stats(out);
+  });
+}
+
+int cps_batch_get_field(cps_batch* b, int field, int which, double* out) {
+  return guard([&] {
+    CPS_CHECK(b && field >= 0 && field < (int)b->b->f.size(), CPS_ERR_ARG, "bad field index");
+    b->b->sync_all();
+    b->b->f[field]->get(which, out);
+  });
+}
+
+int cps_batch_boundary_patch(cps_batch* b, int field, int inlet, cps_patch* out) {
+  return guard([&] {
+    CPS_CHECK(b && out && field >= 0 && field < (int)b->b->f.size(), CPS_ERR_ARG, "bad field index");
+    b->b->sync_all();
+    b->b->f[field]->patch(inlet, out);
   });
 }
 

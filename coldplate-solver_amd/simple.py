@@ -146,6 +146,82 @@ class SIMPLE:
         return self.sampleBoundaryPatch(True)["avg_total"] - self.sampleBoundaryPatch(False)["avg_total"]
 
 
+class Batch:
+    """Many density fields stepped together: fields are spread over several
+    streams (count picked by `tune`) and one SIMPLE iteration of each
+    stream's fields is one hipGraph launch, the streams running concurrently.  `fields` is a list of
+    (cfg, gamma, inlet) tuples; iteration caps must allow graph capture
+    (momentum and pressure iterations <= 32)."""
+
+    def __init__(self, fields, device=0):
+        self.ctx = _capi.context(device)
+        self.L = self.ctx.L
+        n = len(fields)
+        self.cfgs = (T.SimpleConfig * n)(*[f[0] for f in fields])
+        self._keep = []
+        gptr = (C.POINTER(C.c_double) * n)()
+        iptr = (C.POINTER(C.c_double) * n)()
+        for k, (cfg, gamma, inlet) in enumerate(fields):
+            g = np.ascontiguousarray(gamma, dtype=np.float64)
+            if g.shape != (cfg.ny, cfg.nx):
+                raise ValueError(f"field {k}: gamma must be ny x nx")
+            self._keep.append(g)
+            gptr[k] = _capi.dptr(g)
+            if inlet is not None:
+                a = np.ascontiguousarray(inlet, dtype=np.float64)
+                self._keep.append(a)
+                iptr[k] = _capi.dptr(a)
+        h = C.c_void_p()
+        _capi.check(self.L.cps_batch_create(self.ctx.h, n, self.cfgs, gptr, iptr, C.byref(h)))
+        self.h = h
+        self.n = n
+        self.streams = None
+        self.us_per_step = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cps_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tune(self, streams=0, reps=3):
+        """Captures the batch iteration; streams = 0 times 1, 2, 4, 8, 16
+        streams (reps launches each, which advance the fields) and keeps the
+        fastest."""
+        ch, us = C.c_int(), C.c_double()
+        _capi.check(self.L.cps_batch_tune(self.h, streams, reps, C.byref(ch), C.byref(us)))
+        self.streams, self.us_per_step = ch.value, us.value
+        return self.streams, self.us_per_step
+
+    def step(self, n=1):
+        _capi.check(self.L.cps_batch_step(self.h, n))
+
+    def synchronize(self):
+        _capi.check(self.L.cps_batch_synchronize(self.h))
+
+    def stats(self, k):
+        st = T.StepStats()
+        _capi.check(self.L.cps_batch_field_stats(self.h, k, C.byref(st)))
+        return st.as_dict()
+
+    def get(self, k, name):
+        cfg = self.cfgs[k]
+        out = np.empty(T.field_shape(name, cfg.nx, cfg.ny))
+        _capi.check(self.L.cps_batch_get_field(self.h, k, T.FIELDS[name], _capi.dptr(out)))
+        return out
+
+    def pressure_drop(self, k):
+        a, b = T.Patch(), T.Patch()
+        _capi.check(self.L.cps_batch_boundary_patch(self.h, k, 1, C.byref(a)))
+        _capi.check(self.L.cps_batch_boundary_patch(self.h, k, 0, C.byref(b)))
+        return a.avg_static - b.avg_static
+
+
 def _csr_args(row_ptr, col_idx, values, rhs, solution):
     rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
     ci = np.ascontiguousarray(col_idx, dtype=np.int32)

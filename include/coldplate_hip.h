@@ -176,6 +176,33 @@ int cps_simple_state_residual(cps_simple* s, double* out3);
  * above, concatenated; entries that are not DOFs pass through (identity). */
 int cps_simple_jacobian_apply(cps_simple* s, int transpose, const double* x, double* y);
 
+/* ---------------------------------------------------------------------------
+ * Batches of fields (no reference counterpart: the reference solves one
+ * design per process, run_solvers.py).  Each field keeps its own solver
+ * state; fields are spread over several streams and one SIMPLE iteration of
+ * each stream's fields is captured once into a hipGraph, so a batch
+ * iteration is one graph launch per stream, the streams running
+ * concurrently.  Fields need capture-safe caps:
+ * max_momentum_iter and max_pressure_iter <= 32, max_refine <= 4.  Creation
+ * runs one eager iteration per field.
+ * ------------------------------------------------------------------------- */
+typedef struct cps_batch cps_batch;
+/* cfgs[n_fields]; gammas[k]: ny x nx of field k; inlets: NULL or n_fields
+ * pointers (NULL entries = uniform inlet). */
+int cps_batch_create(cps_ctx* ctx, int n_fields, const cps_simple_config* cfgs, const double* const* gammas,
+                     const double* const* inlets, cps_batch** out);
+void cps_batch_destroy(cps_batch* b);
+/* Captures the batch iteration.  streams > 0 fixes the stream count; 0 lets
+ * the autotuner time 1, 2, 4, 8, 16 streams (reps launches each; those
+ * launches advance the fields) and keep the fastest. */
+int cps_batch_tune(cps_batch* b, int streams, int reps, int* chosen, double* us_per_step);
+/* n_steps graph launches (asynchronous). */
+int cps_batch_step(cps_batch* b, int n_steps);
+int cps_batch_synchronize(cps_batch* b);
+int cps_batch_field_stats(cps_batch* b, int field, cps_step_stats* out);
+int cps_batch_get_field(cps_batch* b, int field, int which, double* out);
+int cps_batch_boundary_patch(cps_batch* b, int field, int inlet, cps_patch* out);
+
 /* Measurement hook (no reference counterpart): brackets every launch of one
  * kernel of the step with HIP events on the solver stream.  kernel:
  * CPS_KT_*; -1 stops timing.  Selecting a kernel clears the counters. */
