@@ -11,7 +11,7 @@ from . import _types, configs  # noqa: F401
 from ._types import default_config, SimpleConfig, StepStats, Patch, field_shape  # noqa: F401
 from . import _capi  # noqa: F401  (loads nothing until first use)
 from .simple import SIMPLE, Batch, pressure_solve_csr, momentum_jacobi_csr, pressure_drop  # noqa: F401
-from .thermal import ThermalSolver3D, thermal_objective  # noqa: F401
+from .thermal import ThermalSolver3D, thermal_objective, surrogate_adjoint  # noqa: F401
 
 __all__ = ["_types", "configs", "default_config", "SimpleConfig", "StepStats", "Patch", "field_shape",
-           "SIMPLE", "Batch", "pressure_solve_csr", "momentum_jacobi_csr", "pressure_drop", "ThermalSolver3D", "thermal_objective"]
+           "SIMPLE", "Batch", "pressure_solve_csr", "momentum_jacobi_csr", "pressure_drop", "ThermalSolver3D", "thermal_objective", "surrogate_adjoint"]

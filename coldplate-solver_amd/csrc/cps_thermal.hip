@@ -570,6 +570,56 @@ __global__ __launch_bounds__(TNT) void k_sens(Phys P, Op<double> A, const double
   grad[j * P.nx + i] = -dkdg(P, i, j) * acc;
 }
 
+// ---------------------------------------------------------------------------
+// 2-D conduction surrogate of AdjointMethod/thermal_adjoint.py:8-159 (nz = 1
+// operator): harmonic-mean faces k_eff / dx^2 (:56-62) and 2 k / dx^2 per
+// boundary side on the diagonal (:64-71); row = y (i of the script), column
+// = x (j).  k, P, sens, gamma are ny x nx, row pitch nx.
+__global__ __launch_bounds__(TNT) void k_surr_assemble(int nx, int ny, double dx2inv, const double* k, Op<double> A) {
+  const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  const double kP = k[j * nx + i];
+  auto coef = [&](double kN) { return 2 * kP * kN / (kP + kN) * dx2inv; };
+  double d = 0, w = 0, e = 0, s = 0, n = 0;
+  if (i > 0) { const double c = coef(k[j * nx + i - 1]); w = -c; d += c; }
+  if (i + 1 < nx) { const double c = coef(k[j * nx + i + 1]); e = -c; d += c; }
+  if (j > 0) { const double c = coef(k[(j - 1) * nx + i]); s = -c; d += c; }
+  if (j + 1 < ny) { const double c = coef(k[(j + 1) * nx + i]); n = -c; d += c; }
+  const int bc = (i == 0) + (i == nx - 1) + (j == 0) + (j == ny - 1);
+  if (bc > 0) d += 2 * bc * kP * dx2inv;
+  const size_t q = A.at(i, j, 0);
+  A.d[q] = d; A.w[q] = w; A.e[q] = e; A.s[q] = s; A.n[q] = n; A.b[q] = 0.0; A.t[q] = 0.0;
+}
+
+// sens = - sum_nb (l_P - l_nb) dk_eff/dk_P / dx^2 (T_P - T_nb) - l_P 2 bc / dx^2 T_P
+// (:101-123); with gamma, dJ/dgamma = sens dk/dgamma of the RAMP law at the
+// clipped gamma (run_coldplate_adjoint.py:51-70, 136-150)
+__global__ __launch_bounds__(TNT) void k_surr_sens(int nx, int ny, double dx2inv, const double* k, Op<double> A,
+                                                  const double* T, const double* lam, double* sens,
+                                                  const double* gamma, double ks, double kf, double qk, double* dJ) {
+  const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  const size_t q = A.at(i, j, 0);
+  const double kP = k[j * nx + i], tP = T[q], lP = lam[q];
+  double g = 0.0;
+  auto face = [&](double kN, size_t qn) {
+    const double dkeff = 2 * kN * kN / ((kP + kN) * (kP + kN));
+    g += -((lP - lam[qn]) * (dkeff * dx2inv) * (tP - T[qn]));
+  };
+  if (i > 0) face(k[j * nx + i - 1], q - 1);
+  if (i + 1 < nx) face(k[j * nx + i + 1], q + 1);
+  if (j > 0) face(k[(j - 1) * nx + i], q - A.pitch);
+  if (j + 1 < ny) face(k[(j + 1) * nx + i], q + A.pitch);
+  const int bc = (i == 0) + (i == nx - 1) + (j == 0) + (j == ny - 1);
+  if (bc > 0) g += -lP * (2.0 * bc * dx2inv) * tP;
+  sens[j * nx + i] = g;
+  if (gamma) {
+    const double gc = fmin(1.0, fmax(0.0, gamma[j * nx + i]));
+    const double den = 1.0 + qk * gc;
+    dJ[j * nx + i] = g * (-(ks - kf) * (1.0 + qk) / (den * den));
+  }
+}
+
 }  // namespace th
 
 // ===========================================================================
@@ -1012,6 +1062,110 @@ class ThermalSolver {
   }
 };
 
+// ===========================================================================
+// 2-D conduction surrogate adjoint (one-shot; state lives for one call)
+struct SurrogateAdjoint {
+  Ctx* ctx;
+  int nx, ny;
+  DevBuf<double> c[7], kk, gam, rhs, T, lam, sens, dJ, part, sums, r64;
+  std::unique_ptr<ThermalKrylov<double>> kr;
+
+  SurrogateAdjoint(Ctx* cx, int nx_, int ny_) : ctx(cx), nx(nx_), ny(ny_) {
+    hipStream_t s = ctx->stream;
+    kr.reset(new ThermalKrylov<double>(s, nx, ny, 1, true));
+    const size_t S = kr->lv[0]->size(), m = (size_t)nx * ny;
+    double* cp[7];
+    for (int q = 0; q < 7; ++q) {
+      c[q].alloc(S);
+      CPS_HIP(hipMemsetAsync(c[q].p, 0, c[q].bytes(), s));
+      cp[q] = c[q].p;
+    }
+    kr->set_level0(cp);
+    for (auto* b : {&rhs, &T, &lam, &r64}) {
+      b->alloc(S);
+      CPS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), s));
+    }
+    kk.alloc(m);
+    gam.alloc(m);
+    sens.alloc(m);
+    dJ.alloc(m);
+    part.alloc((size_t)2 * kr->nb0);
+    sums.alloc(2);
+  }
+  th::Op<double> op() { return kr->lv[0]->op(); }
+  dim3 grid() const { return dim3(ceil_div(nx, th::TBX), ceil_div(ny, th::TBY), 1); }
+
+  void upload(const double* h, double* d) {
+    CPS_HIP(hipMemcpy2DAsync(d, kr->lv[0]->pitch * sizeof(double), h, nx * sizeof(double), nx * sizeof(double), ny,
+                             hipMemcpyHostToDevice, ctx->stream));
+  }
+  void download(const double* d, double* h) {
+    CPS_HIP(hipMemcpy2DAsync(h, nx * sizeof(double), d, kr->lv[0]->pitch * sizeof(double), nx * sizeof(double), ny,
+                             hipMemcpyDeviceToHost, ctx->stream));
+  }
+  void norms(const double* x, const double* b, double* rr, double* bb) {
+    th::k_resid64<double><<<grid(), dim3(th::TBX, th::TBY), 0, ctx->stream>>>(op(), x, b, r64.p, part.p);
+    th::k_sum<<<1, dim3(th::TBX, th::TBY), 0, ctx->stream>>>(part.p, kr->nb0, 2, sums.p);
+    double h[2];
+    CPS_HIP(hipMemcpyAsync(h, sums.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    CPS_HIP(hipStreamSynchronize(ctx->stream));
+    *rr = h[0];
+    *bb = h[1];
+  }
+  // K x = b from x = 0, BiCGStab-MG restarted from the true residual
+  void solve(const double* b, double* x, double rtol, cps_solve_stats* st) {
+    CPS_HIP(hipMemsetAsync(x, 0, kr->lv[0]->size() * sizeof(double), ctx->stream));
+    double rr, bb;
+    norms(x, b, &rr, &bb);
+    const double target2 = rtol * rtol * bb;
+    int its = 0, outer = 0;
+    while (outer < 8 && rr > target2 && its < 4000) {
+      CPS_HIP(hipMemcpyAsync(kr->r.p, r64.p, r64.bytes(), hipMemcpyDeviceToDevice, ctx->stream));
+      its += kr->bicgstab(rr, 0.5 * std::sqrt(target2), 4000 - its);
+      th::k_xupd<double, double><<<grid(), dim3(th::TBX, th::TBY), 0, ctx->stream>>>(op(), x, kr->e.p, 1.0, nullptr,
+                                                                                      0.0);
+      norms(x, b, &rr, &bb);
+      ++outer;
+    }
+    if (st) {
+      st->iterations += its;
+      st->refinements += outer;
+      st->rel_residual = std::max(st->rel_residual, bb > 0 ? std::sqrt(rr / bb) : 0.0);
+      st->converged = st->converged && rr <= target2;
+    }
+  }
+
+  void run(const double* hk, const double* hP, const double* hT, double rtol, const double* hg, double ks, double kf,
+           double qk, double* oT, double* olam, double* osens, double* odJ, cps_solve_stats* st) {
+    hipStream_t s = ctx->stream;
+    const size_t m = (size_t)nx * ny;
+    const double dx2inv = (double)nx * (double)nx;  // dx = 1 / nx (thermal_adjoint.py:40)
+    CPS_HIP(hipMemcpyAsync(kk.p, hk, m * sizeof(double), hipMemcpyHostToDevice, s));
+    th::k_surr_assemble<<<grid(), dim3(th::TBX, th::TBY), 0, s>>>(nx, ny, dx2inv, kk.p, op());
+    kr->setup();
+    cps_solve_stats local{0, 0, 0.0, 1};
+    if (hT) {
+      upload(hT, T.p);
+    } else {
+      upload(hP, rhs.p);
+      solve(rhs.p, T.p, rtol, &local);
+    }
+    // lambda = K^-T (1/n); K is symmetric (the same harmonic mean on both sides of a face)
+    th::k_fill<double><<<grid(), dim3(th::TBX, th::TBY), 0, s>>>(op(), rhs.p, 1.0 / (double)m);
+    solve(rhs.p, lam.p, rtol, &local);
+    if (hg) CPS_HIP(hipMemcpyAsync(gam.p, hg, m * sizeof(double), hipMemcpyHostToDevice, s));
+    th::k_surr_sens<<<grid(), dim3(th::TBX, th::TBY), 0, s>>>(nx, ny, dx2inv, kk.p, op(), T.p, lam.p, sens.p,
+                                                              hg ? gam.p : nullptr, ks, kf, qk, dJ.p);
+    CPS_HIP(hipGetLastError());
+    if (oT) download(T.p, oT);
+    if (olam) download(lam.p, olam);
+    if (osens) CPS_HIP(hipMemcpyAsync(osens, sens.p, m * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (odJ && hg) CPS_HIP(hipMemcpyAsync(odJ, dJ.p, m * sizeof(double), hipMemcpyDeviceToHost, s));
+    CPS_HIP(hipStreamSynchronize(s));
+    if (st) *st = local;
+  }
+};
+
 }  // namespace cps
 
 struct cps_thermal {
@@ -1109,6 +1263,23 @@ int cps_thermal_get_operator(cps_thermal* t, double* coef7, double* rhs) {
   return guard([&] {
     CPS_CHECK(t && coef7, CPS_ERR_ARG, "null argument");
     t->s->get_operator(coef7, rhs);
+  });
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int cps_surrogate_adjoint(cps_ctx* ctx, int nx, int ny, const double* k, const double* P, const double* T_in,
+                          double rtol, const double* gamma, double k_s, double k_f, double q_k, double* T_out,
+                          double* lambda, double* sens, double* dJ_dgamma, cps_solve_stats* stats) {
+  return guard([&] {
+    CPS_CHECK(ctx && k && (P || T_in), CPS_ERR_ARG, "need k and P or T");
+    CPS_CHECK(nx >= 1 && ny >= 1 && rtol > 0, CPS_ERR_ARG, "bad sizes or rtol");
+    CPS_CHECK(!dJ_dgamma || gamma, CPS_ERR_ARG, "dJ_dgamma needs gamma");
+    CPS_HIP(hipSetDevice(ctx->c.device));
+    cps::SurrogateAdjoint sa(&ctx->c, nx, ny);
+    sa.run(k, P, T_in, rtol, gamma, k_s, k_f, q_k, T_out, lambda, sens, dJ_dgamma, stats);
   });
 }
 

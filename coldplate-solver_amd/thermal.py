@@ -19,6 +19,7 @@ import ctypes as C
 import numpy as np
 
 from . import _capi
+from . import _types
 from . import _types as T
 
 
@@ -128,6 +129,32 @@ class ThermalSolver3D:
                                                 _capi.dptr(grad), C.byref(J), C.byref(st)))
         self.last_adjoint = st.as_dict()
         return J.value, grad
+
+
+def surrogate_adjoint(k, P=None, T=None, gamma=None, k_s=400.0, k_f=0.6, q_k=1.0, rtol=1e-12, device=0):
+    """The 2-D conduction surrogate adjoint of AdjointMethod/thermal_adjoint.py
+    (solve_thermal_adjoint) on the device: T (solved from P unless given),
+    lambda, sens = dQ/dk, and with gamma dJ/dgamma = sens dk/dgamma (RAMP),
+    as run_coldplate_adjoint.py applies it.  Arrays are ny x nx."""
+    ctx = _capi.context(device)
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    ny, nx = k.shape
+    arr = {}
+    for name, a in (("P", P), ("T", T), ("gamma", gamma)):
+        arr[name] = None if a is None else _f64(a, (ny, nx), name)
+    if arr["P"] is None and arr["T"] is None:
+        raise ValueError("need P or T")
+    outT, lam, sens = np.empty((ny, nx)), np.empty((ny, nx)), np.empty((ny, nx))
+    dJ = np.empty((ny, nx)) if gamma is not None else None
+    st = _types.SolveStats()  # (the argument T shadows the module alias)
+    p = lambda a: None if a is None else _capi.dptr(a)  # noqa: E731
+    _capi.check(ctx.L.cps_surrogate_adjoint(ctx.h, nx, ny, _capi.dptr(k), p(arr["P"]), p(arr["T"]), rtol,
+                                            p(arr["gamma"]), k_s, k_f, q_k, _capi.dptr(outT), _capi.dptr(lam),
+                                            _capi.dptr(sens), p(dJ), C.byref(st)))
+    out = {"T": outT, "lambda": lam, "sens": sens, "stats": st.as_dict()}
+    if dJ is not None:
+        out["dJ_dgamma"] = dJ
+    return out
 
 
 def _torch():

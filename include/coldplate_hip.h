@@ -274,6 +274,18 @@ int cps_thermal_solve_adjoint(cps_thermal* t, const double* rhs, double* lambda,
  * assembly; the region masks (gamma > 0.01) are held fixed. */
 int cps_thermal_gradient(cps_thermal* t, int objective, double pnorm_exp, double* dJ_dgamma, double* J,
                          cps_solve_stats* stats);
+/* 2-D conduction surrogate adjoint of AdjointMethod/thermal_adjoint.py:8-159
+ * (solve_thermal_adjoint) on the device.  k, P, T_in, gamma, T_out, lambda,
+ * sens, dJ_dgamma: ny x nx, row = y.  K: harmonic-mean faces k_eff / dx^2 with
+ * dx = 1 / nx and 2 k / dx^2 per boundary side; T = K^-1 P, or T_in when
+ * given (the external-T mode of run_coldplate_adjoint.py:112-117); lambda =
+ * K^-T (1/n); sens = dQ/dk as in the script (:101-123).  With gamma,
+ * dJ_dgamma = sens * dk/dgamma of the RAMP law (k_s, k_f, q_k) at the
+ * clipped gamma (run_coldplate_adjoint.py:51-70, 136-150).  Solves run to
+ * ||b - K x|| <= rtol ||b||.  Output pointers may be NULL. */
+int cps_surrogate_adjoint(cps_ctx* ctx, int nx, int ny, const double* k, const double* P, const double* T_in,
+                          double rtol, const double* gamma, double k_s, double k_f, double q_k, double* T_out,
+                          double* lambda, double* sens, double* dJ_dgamma, cps_solve_stats* stats);
 /* Assembled operator as 7 stencil arrays [d, w, e, s, n, b, t] of nz*ny*nx
  * each (entry of A for the neighbour; 0 where none) and the right side b. */
 int cps_thermal_get_operator(cps_thermal* t, double* coef7, double* rhs);

@@ -334,3 +334,56 @@ def thermal_metrics(stack, gamma, u, T, pnorm_exp=10.0):
 def thermal_objective(stack, gamma, u, T, objective, pnorm_exp=8.0):
     m = thermal_metrics(stack, gamma, u, T, pnorm_exp)
     return m["t_avg_base"] if objective == "t_avg_base" else m["tb_pnorm"]
+
+
+def surrogate_adjoint(k, P, T=None):
+    """2-D conduction surrogate adjoint of AdjointMethod/thermal_adjoint.py:
+    8-159 restated with scipy sparse: K with harmonic-mean faces k_eff / dx^2
+    (dx = 1 / nx) and 2 k / dx^2 per boundary side on the diagonal (:45-76);
+    T = K^-1 P unless given (:81-88); lambda = K^-T 1/n (:93-96); sens =
+    -sum_nb (l_i - l_nb) 2 k_nb^2 / (k_i + k_nb)^2 / dx^2 (T_i - T_nb)
+    - l_i 2 (boundary sides) / dx^2 T_i (:101-123).  Returns T, lambda, sens."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    k = np.asarray(k, dtype=np.float64)
+    ny, nx = k.shape
+    n = nx * ny
+    dx = 1.0 / nx
+    idx = np.arange(n).reshape(ny, nx)
+    rows, cols, vals = [], [], []
+    diag = np.zeros((ny, nx))
+    for di, dj in ((0, -1), (0, 1), (-1, 0), (1, 0)):
+        i0, i1 = max(0, -di), ny - max(0, di)
+        j0, j1 = max(0, -dj), nx - max(0, dj)
+        ki = k[i0:i1, j0:j1]
+        kn = k[i0 + di:i1 + di, j0 + dj:j1 + dj]
+        c = 2 * ki * kn / (ki + kn) / dx ** 2
+        rows.append(idx[i0:i1, j0:j1].ravel())
+        cols.append(idx[i0 + di:i1 + di, j0 + dj:j1 + dj].ravel())
+        vals.append(-c.ravel())
+        diag[i0:i1, j0:j1] += c
+    bc = np.zeros((ny, nx))
+    bc[:, 0] += 1
+    bc[:, -1] += 1
+    bc[0, :] += 1
+    bc[-1, :] += 1
+    diag += 2 * bc * k / dx ** 2
+    rows.append(idx.ravel())
+    cols.append(idx.ravel())
+    vals.append(diag.ravel())
+    K = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    if T is None:
+        T = sla.spsolve(K.tocsc(), np.asarray(P, dtype=np.float64).ravel()).reshape(ny, nx)
+    T = np.asarray(T, dtype=np.float64)
+    lam = sla.spsolve(K.T.tocsc(), np.ones(n) / n).reshape(ny, nx)
+    sens = np.zeros((ny, nx))
+    for di, dj in ((0, -1), (0, 1), (-1, 0), (1, 0)):
+        i0, i1 = max(0, -di), ny - max(0, di)
+        j0, j1 = max(0, -dj), nx - max(0, dj)
+        ki = k[i0:i1, j0:j1]
+        kn = k[i0 + di:i1 + di, j0 + dj:j1 + dj]
+        dkeff = 2 * kn ** 2 / (ki + kn) ** 2
+        sens[i0:i1, j0:j1] += -(lam[i0:i1, j0:j1] - lam[i0 + di:i1 + di, j0 + dj:j1 + dj]) * (dkeff / dx ** 2) * \
+            (T[i0:i1, j0:j1] - T[i0 + di:i1 + di, j0 + dj:j1 + dj])
+    sens += -lam * (2 * bc / dx ** 2) * T
+    return T, lam, sens

@@ -155,3 +155,20 @@ def test_thermal_errors_are_reported(pkg):
     with pytest.raises(pkg._capi.CpsError):
         s.metrics()  # no solve yet
     s.close()
+
+
+def test_surrogate_adjoint_matches_reference_vectors(pkg):
+    """Device version of AdjointMethod/thermal_adjoint.py against the
+    reference's own outputs (tests/golden/surrogate_adjoint.npz)."""
+    import os
+
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "surrogate_adjoint.npz"))
+    out = pkg.surrogate_adjoint(z["ext_k"], P=z["ext_P"], T=z["ext_T"], gamma=z["ext_gamma"], rtol=1e-13)
+    assert out["stats"]["converged"] == 1
+    np.testing.assert_allclose(out["sens"], z["ext_sens"], rtol=1e-8, atol=1e-12 * np.abs(z["ext_sens"]).max())
+    g = np.clip(z["ext_gamma"], 0, 1)
+    dk = -(400.0 - 0.6) * 2.0 / (1.0 + g) ** 2
+    np.testing.assert_allclose(out["dJ_dgamma"], z["ext_sens"] * dk, rtol=1e-8,
+                               atol=1e-12 * np.abs(z["ext_sens"] * dk).max())
+    out = pkg.surrogate_adjoint(z["def_k"], P=z["def_P"], rtol=1e-13)
+    np.testing.assert_allclose(out["sens"], z["def_sens"], rtol=1e-8, atol=1e-12 * np.abs(z["def_sens"]).max())
