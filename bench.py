@@ -143,6 +143,36 @@ def batch_leg(pkg, steps=500, n_fields=48):
                 all_finite=finite)
 
 
+def grads_batch_leg(pkg, n_fields=32, nx=512, ny=256, forward_steps=1500, rtol=1e-8, workers=8):
+    """32 seeded 512 x 256 density fields brought near their fixed points as
+    one batch (per-stream graphs), then all 32 d(dP)/d(gamma) computed at
+    once: one host thread per field in flight, each adjoint on its field's
+    stream.  Timed: the 32 gradients."""
+    T = pkg._types
+    fl = []
+    for k in range(n_fields):
+        cfg, gamma, inlet = pkg.configs.density_field(nx=nx, ny=ny, seed=k)
+        cfg.pressure_solver, cfg.precision = T.CPS_PRESSURE_GMG, T.CPS_PREC_FP64
+        cfg.p_tol, cfg.p_rtol, cfg.max_pressure_iter, cfg.max_refine = 0.0, 1e-6, 30, 1
+        cfg.momentum_tol, cfg.max_momentum_iter = 1e-13, 30
+        fl.append((cfg, gamma, inlet))
+    b = pkg.Batch(fl)
+    b.tune(streams=8, reps=2)
+    t0 = time.perf_counter()
+    b.step(forward_steps)
+    b.synchronize()
+    t_fwd = time.perf_counter() - t0
+    b.gradient_dp(0, rtol=rtol)  # warm-up (one field's adjoint allocations)
+    t0 = time.perf_counter()
+    res = b.gradients_dp(rtol=rtol, workers=workers)
+    dt = time.perf_counter() - t0
+    its = [r[2]["iterations"] for r in res]
+    ok = all(r[2]["converged"] for r in res)
+    b.close()
+    return dict(fields=n_fields, workers=workers, seconds=dt, grads_per_s=n_fields / dt, forward_steps=forward_steps,
+                forward_seconds=t_fwd, fgmres_iterations_mean=sum(its) / len(its), all_converged=ok)
+
+
 def thermal_leg(pkg, reps=3):
     """512 x 256 x 12 conjugate-heat-transfer solve (4 Cu base, 4 channel,
     4 Cu lid layers) to the reference's rtol 1e-6, FP64 and FP32-refined;
@@ -247,6 +277,7 @@ def main():
     grads = None if args.no_grads else grads_leg(pkg)
     thermal = None if args.no_thermal else thermal_leg(pkg)
     batch = None if args.no_batch else batch_leg(pkg)
+    grads_batch = None if args.no_grads else grads_batch_leg(pkg)
     cpu = None if args.no_cpu else cpu_baseline(pkg, nx, ny, args.cpu_seconds)
 
     dp_err = None
@@ -286,6 +317,11 @@ def main():
             "adjoint_rel_residual": grads["adjoint"]["rel_residual"],
             "forward_steps": grads["forward_steps"], "forward_seconds": round(grads["forward_seconds"], 3),
             "state_residual_max": max(grads["state_residual"]),
+        },
+        "grads_batch": None if grads_batch is None else {
+            "workload": "32 seeded 512x256 density fields (SOU) after 1500 batched SIMPLE iterations; all 32 "
+                        "d(dP)/d(gamma) at once (host threads, per-field streams), rtol 1e-8",
+            **{k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in grads_batch.items()},
         },
         "batch": None if batch is None else {
             "workload": "48 density fields, sizes from {256x128, 512x256, 768x384, 1024x512}, sigma U[4,12], "

@@ -78,6 +78,7 @@ def lib():
         L.cps_batch_field_stats.argtypes = [vp, C.c_int, P(T.StepStats)]
         L.cps_batch_get_field.argtypes = [vp, C.c_int, C.c_int, dp]
         L.cps_batch_boundary_patch.argtypes = [vp, C.c_int, C.c_int, P(T.Patch)]
+        L.cps_batch_gradient_dp.argtypes = [vp, C.c_int, C.c_double, C.c_int, dp, dp, P(T.SolveStats)]
         L.cps_surrogate_adjoint.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, C.c_double, dp, C.c_double, C.c_double,
                                             C.c_double, dp, dp, dp, dp, P(T.SolveStats)]
         L.cps_thermal_config_default.argtypes = [P(T.ThermalConfig)]

@@ -1375,6 +1375,18 @@ int cps_batch_get_field(cps_batch* b, int field, int which, double* out) {
   });
 }
 
+int cps_batch_gradient_dp(cps_batch* b, int field, double rtol, int max_iter, double* dp, double* dJ_dgamma,
+                          cps_solve_stats* stats) {
+  return guard([&] {
+    CPS_CHECK(b && field >= 0 && field < (int)b->b->f.size(), CPS_ERR_ARG, "bad field index");
+    CPS_CHECK(rtol > 0 && max_iter >= 1, CPS_ERR_ARG, "rtol must be > 0 and max_iter >= 1");
+    cps::SimpleSolver& s = *b->b->f[field];
+    CPS_HIP(hipStreamSynchronize(s.ctx->stream));
+    s.apply_bc();
+    cps::flow_adjoint_run(s.adj(), s.view(), rtol, max_iter, dp, dJ_dgamma, stats);
+  });
+}
+
 int cps_batch_boundary_patch(cps_batch* b, int field, int inlet, cps_patch* out) {
   return guard([&] {
     CPS_CHECK(b && out && field >= 0 && field < (int)b->b->f.size(), CPS_ERR_ARG, "bad field index");

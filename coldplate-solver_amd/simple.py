@@ -215,6 +215,26 @@ class Batch:
         _capi.check(self.L.cps_batch_get_field(self.h, k, T.FIELDS[name], _capi.dptr(out)))
         return out
 
+    def gradient_dp(self, k, rtol=1e-10, max_iter=4000):
+        """(dP, d(dP)/dgamma) of field k at its current state (discrete adjoint)."""
+        cfg = self.cfgs[k]
+        grad = np.empty((cfg.ny, cfg.nx))
+        dp = C.c_double()
+        st = T.SolveStats()
+        _capi.check(self.L.cps_batch_gradient_dp(self.h, k, rtol, max_iter, C.byref(dp), _capi.dptr(grad),
+                                                 C.byref(st)))
+        return dp.value, grad, st.as_dict()
+
+    def gradients_dp(self, fields=None, rtol=1e-10, max_iter=4000, workers=8):
+        """Gradients of several fields at once: one host thread per field in
+        flight, each field's adjoint on its own stream."""
+        from concurrent.futures import ThreadPoolExecutor
+
+        fields = list(range(self.n)) if fields is None else list(fields)
+        self.synchronize()
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            return list(ex.map(lambda k: self.gradient_dp(k, rtol, max_iter), fields))
+
     def pressure_drop(self, k):
         a, b = T.Patch(), T.Patch()
         _capi.check(self.L.cps_batch_boundary_patch(self.h, k, 1, C.byref(a)))

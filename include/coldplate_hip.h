@@ -202,6 +202,11 @@ int cps_batch_synchronize(cps_batch* b);
 int cps_batch_field_stats(cps_batch* b, int field, cps_step_stats* out);
 int cps_batch_get_field(cps_batch* b, int field, int which, double* out);
 int cps_batch_boundary_patch(cps_batch* b, int field, int inlet, cps_patch* out);
+/* cps_simple_gradient_dp for one field of the batch, on that field's stream.
+ * Calls for different fields may run concurrently from different host
+ * threads (while no batch iteration is in flight). */
+int cps_batch_gradient_dp(cps_batch* b, int field, double rtol, int max_iter, double* dp, double* dJ_dgamma,
+                          cps_solve_stats* stats);
 
 /* Measurement hook (no reference counterpart): brackets every launch of one
  * kernel of the step with HIP events on the solver stream.  kernel:
