@@ -21,12 +21,26 @@
 // J is built exactly, one thread per equation, by evaluating the equation
 // with 16-slot forward dual numbers seeded on the DOFs it reads; it is stored
 // row-wise (ELL) and transposed on the device with a counting pass and a
-// per-row sort (deterministic).  The adjoint system is solved by FGMRES with a
-// SIMPLE block preconditioner: momentum blocks by 1/aP0 (taken from the
-// Jacobian kernel), the Schur complement C diag(aP0)^-1 B, which is exactly
-// the SIMPLE pressure-correction operator with dE = hy / aP0, dN = hx / aP0,
-// by a few geometric-multigrid PCG iterations.  Vectors are three padded fields [u | v | p]; rows that are not
-// DOFs are identity rows.
+// per-row sort (deterministic).  J^T = [[A, B], [C, 0]] (momentum rows, then
+// continuity rows) is solved by right-preconditioned FGMRES with a block
+// factorisation preconditioner
+//   x* = A~^-1 f_m,  y = S~^-1 (C x* - f_p),  x = x* - A~^-1 B y.
+// Two forms:
+//   SIMPLE (default)  A~ = diag(aP0) (aP0 taken from the Jacobian kernel),
+//                     S~ = C diag(aP0)^-1 B, which is exactly the SIMPLE
+//                     pressure-correction operator with dE = hy / aP0,
+//                     dN = hx / aP0, solved by a few geometric-multigrid PCG
+//                     iterations;
+//   LSC_LINE          A~^-1 = a few x-line Jacobi sweeps on A (pentadiagonal
+//                     lines along the grid rows), S~^-1 the least-squares-
+//                     commutator approximation L^-1 (C D^-1 A D^-1 B) L^-1 of
+//                     S = C A^-1 B, D = diag(A), L = C D^-1 B (the same
+//                     multigrid operator with D in place of aP0).  Several
+//                     times fewer iterations where it converges, but the line
+//                     sweeps do not converge on every second-order-upwind
+//                     state, so it is opt-in.
+// Vectors are three padded fields [u | v | p]; rows that are not DOFs are
+// identity rows.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -414,6 +428,10 @@ __global__ __launch_bounds__(BS) void k_mcomb(int n, double a0, double* w, const
   }
 }
 
+__global__ __launch_bounds__(BS) void k_fill(int n, double a, double* y) {
+  const int r = blockIdx.x * BS + threadIdx.x;
+  if (r < n) y[r] = a;
+}
 __global__ __launch_bounds__(BS) void k_scale(int n, const double* x, double a, double* y) {
   for (int r = blockIdx.x * BS + threadIdx.x; r < n; r += gridDim.x * BS) y[r] = a * x[r];
 }
@@ -443,14 +461,14 @@ __global__ __launch_bounds__(BS) void k_pre1(View V, int n, const double* dinv, 
   const int comp = r / (int)V.S;
   t[r] = comp < 2 ? dinv[r] * f[r] : 0.0;
 }
-// stage 2: pressure rhs = (J^T t)_p - f_p on pressure DOFs (field layout), x_p = 0
+// stage 2: pressure rhs = (J^T t)_p - f_p (f may be null) on pressure DOFs (field layout), x_p = 0
 __global__ __launch_bounds__(BS) void k_pre2(View V, const double* Jt, const double* f, double* rp, double* xp) {
   const int q = blockIdx.x * BS + threadIdx.x;
   if (q >= (int)V.S) return;
   const Geo& g = V.g;
   const int i = q / g.P, j = q % g.P;
   const size_t r = 2 * V.S + q;
-  rp[q] = p_dof(g, i, j) ? Jt[r] - f[r] : 0.0;
+  rp[q] = p_dof(g, i, j) ? Jt[r] - (f ? f[r] : 0.0) : 0.0;
   xp[q] = 0.0;
 }
 // stage 3: s = [0 ; y] for the second product
@@ -471,6 +489,175 @@ __global__ __launch_bounds__(BS) void k_pre4(View V, int n, const double* dinv, 
   double out;
   if (comp == 0) out = u_dof(g, i, j) ? t[r] - dinv[r] * Js[r] : f[r];
   else if (comp == 1) out = v_dof(g, i, j) ? t[r] - dinv[r] * Js[r] : f[r];
+  else out = p_dof(g, i, j) ? y[q] : f[r];
+  z[r] = out;
+}
+
+// ---------------------------------------------------------------------------
+// x-line preconditioner of the momentum block A of J^T.  The momentum
+// equations couple a DOF to its neighbours j-2..j+2 along the grid row
+// (second-order upwinding reaches two cells), so each grid row of u (i =
+// 1..M-1, j = 1..N-2) and of v (i = 1..M-2, j = 1..N-1) is a pentadiagonal
+// system; the couplings across rows and between u and v stay explicit.  One
+// line-Jacobi sweep: x += Dl^-1 (f - A x).  Dropping the j +- 2 entries
+// (tridiagonal lines) makes the sweeps diverge on SOU states.
+constexpr int kLineChunk = 16;
+
+__device__ inline int n_lines(const Geo& g) { return (g.M - 1) + (g.M - 2); }
+// first row (in the [u | v | p] numbering) and length of line l
+__device__ inline void line_span(const View& V, int l, size_t* base, int* len) {
+  const Geo& g = V.g;
+  if (l < g.M - 1) {
+    *base = (size_t)(l + 1) * g.P + 1;
+    *len = g.N - 2;
+  } else {
+    *base = V.S + (size_t)(l - (g.M - 1) + 1) * g.P + 1;
+    *len = g.N - 1;
+  }
+}
+
+// band[k * 2S + r] = entry (r, r + k - 2) of J^T on momentum DOF rows, the
+// pressure part and cross-row entries excluded; dj = 1 / diagonal
+__global__ __launch_bounds__(BS) void k_band(View V, int n, int WT, const int* cnt, const int* colT,
+                                            const double* valT, double* band, double* dj) {
+  const int r = blockIdx.x * BS + threadIdx.x;
+  const size_t S2 = 2 * V.S;
+  if (r >= (int)S2) return;
+  const Geo& g = V.g;
+  const int comp = r / (int)V.S, q = r % (int)V.S, i = q / g.P, j = q % g.P;
+  double b[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const bool dof = comp == 0 ? u_dof(g, i, j) : v_dof(g, i, j);
+  if (dof) {
+    const int m = min(cnt[r], WT);
+    for (int s = 0; s < m; ++s) {
+      const int c = colT[(size_t)s * n + r];
+      const int off = c - r;
+      if (off >= -2 && off <= 2 && c / (int)V.S == comp && c / g.P == r / g.P) b[off + 2] += valT[(size_t)s * n + r];
+    }
+  } else {
+    b[2] = 1.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) band[k * S2 + r] = b[k];
+  dj[r] = b[2] != 0.0 ? 1.0 / b[2] : 1.0;
+}
+
+// in-place banded LU without pivoting, one thread per line:
+// band slots become [l2, l1, 1/u0, u1, u2]
+__global__ __launch_bounds__(64) void k_line_factor(View V, double* band, int* bad) {
+  const int l = blockIdx.x * 64 + threadIdx.x;
+  if (l >= n_lines(V.g)) return;
+  size_t base;
+  int len;
+  line_span(V, l, &base, &len);
+  const size_t S2 = 2 * V.S;
+  double* A2 = band;
+  double* A1 = band + S2;
+  double* D = band + 2 * S2;
+  double* C1 = band + 3 * S2;
+  double* C2 = band + 4 * S2;
+  // U rows j-1 and j-2: (u0, u1, u2)
+  double p0 = 1.0, p1 = 0.0, p2 = 0.0, q0 = 1.0, q1 = 0.0, q2 = 0.0;
+  int fail = 0;
+  for (int j = 0; j < len; ++j) {
+    const size_t r = base + j;
+    const double a2 = j >= 2 ? A2[r] : 0.0, a1 = j >= 1 ? A1[r] : 0.0;
+    const double c1 = j + 1 < len ? C1[r] : 0.0, c2 = j + 2 < len ? C2[r] : 0.0;
+    const double l2 = a2 / q0;
+    const double l1 = (a1 - l2 * q1) / p0;
+    const double u0 = D[r] - l2 * q2 - l1 * p1;
+    const double u1 = c1 - l1 * p2;
+    if (!(std::fabs(u0) > 0.0) || !std::isfinite(u0)) fail = 1;
+    A2[r] = l2;
+    A1[r] = l1;
+    D[r] = 1.0 / u0;
+    C1[r] = u1;
+    C2[r] = c2;
+    q0 = p0; q1 = p1; q2 = p2;
+    p0 = u0; p1 = u1; p2 = c2;
+  }
+  if (fail) atomicOr(bad, 1);
+}
+
+// x = Dl^-1 f (accumulate = 0) or x += Dl^-1 (f - Ax) on momentum DOF rows;
+// tmp holds the forward sweep.  Loads go in chunks so each chunk waits once.
+__global__ __launch_bounds__(64) void k_line_solve(View V, const double* band, const double* f, const double* Ax,
+                                                  double* x, double* tmp, int accumulate) {
+  const int l = blockIdx.x * 64 + threadIdx.x;
+  if (l >= n_lines(V.g)) return;
+  size_t base;
+  int len;
+  line_span(V, l, &base, &len);
+  const size_t S2 = 2 * V.S;
+  const double* L2 = band;
+  const double* L1 = band + S2;
+  const double* U0 = band + 2 * S2;
+  const double* U1 = band + 3 * S2;
+  const double* U2 = band + 4 * S2;
+  double y1 = 0.0, y2 = 0.0;
+  for (int j0 = 0; j0 < len; j0 += kLineChunk) {
+    double bb[kLineChunk], a1[kLineChunk], a2[kLineChunk];
+#pragma unroll
+    for (int k = 0; k < kLineChunk; ++k) {
+      const size_t r = base + min(j0 + k, len - 1);
+      bb[k] = accumulate ? f[r] - Ax[r] : f[r];
+      a1[k] = L1[r];
+      a2[k] = L2[r];
+    }
+#pragma unroll
+    for (int k = 0; k < kLineChunk; ++k) {
+      if (j0 + k < len) {
+        const double yj = bb[k] - a1[k] * y1 - a2[k] * y2;
+        tmp[base + j0 + k] = yj;
+        y2 = y1;
+        y1 = yj;
+      }
+    }
+  }
+  double x1 = 0.0, x2 = 0.0;
+  for (int j1 = len - 1; j1 >= 0; j1 -= kLineChunk) {
+    double yy[kLineChunk], d[kLineChunk], c1[kLineChunk], c2[kLineChunk], xo[kLineChunk];
+#pragma unroll
+    for (int k = 0; k < kLineChunk; ++k) {
+      const size_t r = base + max(j1 - k, 0);
+      yy[k] = tmp[r];
+      d[k] = U0[r];
+      c1[k] = U1[r];
+      c2[k] = U2[r];
+      xo[k] = accumulate ? x[r] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < kLineChunk; ++k) {
+      if (j1 - k >= 0) {
+        const double xj = (yy[k] - c1[k] * x1 - c2[k] * x2) * d[k];
+        x[base + j1 - k] = xo[k] + xj;
+        x2 = x1;
+        x1 = xj;
+      }
+    }
+  }
+}
+
+// y = [D^-1 x_m ; 0] on momentum DOF rows, 0 elsewhere
+__global__ __launch_bounds__(BS) void k_dscale(View V, int n, const double* dj, const double* x, double* y) {
+  const int r = blockIdx.x * BS + threadIdx.x;
+  if (r >= n) return;
+  const Geo& g = V.g;
+  const int comp = r / (int)V.S, q = r % (int)V.S, i = q / g.P, j = q % g.P;
+  const bool dof = comp == 0 ? u_dof(g, i, j) : (comp == 1 ? v_dof(g, i, j) : false);
+  y[r] = dof ? dj[r] * x[r] : 0.0;
+}
+
+// z = [xs - xb ; y] on DOFs, f on the identity rows
+__global__ __launch_bounds__(BS) void k_lsc_out(View V, int n, const double* f, const double* xs, const double* xb,
+                                               const double* y, double* z) {
+  const int r = blockIdx.x * BS + threadIdx.x;
+  if (r >= n) return;
+  const Geo& g = V.g;
+  const int comp = r / (int)V.S, q = r % (int)V.S, i = q / g.P, j = q % g.P;
+  double out;
+  if (comp == 0) out = u_dof(g, i, j) ? xs[r] - xb[r] : f[r];
+  else if (comp == 1) out = v_dof(g, i, j) ? xs[r] - xb[r] : f[r];
   else out = p_dof(g, i, j) ? y[q] : f[r];
   z[r] = out;
 }
@@ -541,8 +728,12 @@ class FlowAdjoint {
   Geo g;
   size_t S;
   int n = 0, WT = 0, nb = 0;
-  DevBuf<int> col, colT, cnt, cur, flag, iout;
-  DevBuf<double> val, valT, res, dinv, gvec, lam, r, Vb, Zb, w, t1, t2, t3, y, rp, xp, dEa, dNa, part, sums, grad, obj;
+  DevBuf<int> col, colT, cnt, cur, flag, iout, bad;
+  DevBuf<double> val, valT, res, dinv, gvec, lam, r, Vb, Zb, w, t1, t2, t3, t4, ta, lt, y, rp, xp, dEa, dNa, part,
+      sums, grad, obj, band, dj;
+  // preconditioner: CPS_ADJ_PRECOND_SIMPLE (default) or CPS_ADJ_PRECOND_LSC_LINE
+  int kind = CPS_ADJ_PRECOND_SIMPLE;
+  int sweeps = 4;  // x-line Jacobi sweeps per momentum-block solve
   DevBuf<IterState> st;
   std::unique_ptr<PressureMG> mg;
   KernelTimer timer;
@@ -558,7 +749,10 @@ class FlowAdjoint {
     val.alloc((size_t)adj::K * n);
     for (auto* b : {&cnt, &cur, &flag}) b->alloc(n);
     iout.alloc(2);
-    for (auto* b : {&res, &dinv, &gvec, &lam, &r, &w, &t1, &t2, &t3}) b->alloc(n);
+    for (auto* b : {&res, &dinv, &gvec, &lam, &r, &w, &t1, &t2, &t3, &t4, &ta, &lt}) b->alloc(n);
+    band.alloc(5 * 2 * S);
+    dj.alloc(2 * S);
+    bad.alloc(1);
     Vb.alloc((size_t)(kRestart + 1) * n);
     Zb.alloc((size_t)kRestart * n);
     y.alloc(S);
@@ -599,7 +793,17 @@ class FlowAdjoint {
     CPS_HIP(hipMemsetAsync(cur.p, 0, cur.bytes(), s));
     adj::k_scatter<<<blocks(n), adj::BS, 0, s>>>(n, col.p, val.p, cur.p, WT, colT.p, valT.p);
     adj::k_sort<<<blocks(n), adj::BS, 0, s>>>(n, WT, cnt.p, colT.p, valT.p);
-    adj::k_faces<<<blocks((int)S), adj::BS, 0, s>>>(V, dinv.p, dEa.p, dNa.p);
+    if (kind == CPS_ADJ_PRECOND_LSC_LINE) {
+      adj::k_band<<<blocks((int)(2 * S)), adj::BS, 0, s>>>(V, n, WT, cnt.p, colT.p, valT.p, band.p, dj.p);
+      CPS_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+      adj::k_line_factor<<<ceil_div(g.M - 1 + g.M - 2, 64), 64, 0, s>>>(V, band.p, bad.p);
+      int hb = 0;
+      CPS_HIP(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      CPS_HIP(hipStreamSynchronize(s));
+      CPS_CHECK(hb == 0, CPS_ERR_DIVERGED, "adjoint: zero pivot in an x-line factorisation of the momentum block");
+    }
+    adj::k_faces<<<blocks((int)S), adj::BS, 0, s>>>(V, kind == CPS_ADJ_PRECOND_LSC_LINE ? dj.p : dinv.p, dEa.p,
+                                                   dNa.p);
     CPS_HIP(hipGetLastError());
   }
 
@@ -607,8 +811,49 @@ class FlowAdjoint {
     adj::k_spmv<<<blocks(n), adj::BS, 0, ctx->stream>>>(n, WT, cnt.p, colT.p, valT.p, x, out);
   }
 
-  // z = M^-1 f (SIMPLE block preconditioner)
+  // x = A~^-1 f_m by x-line Jacobi (x: momentum DOF rows, 0 elsewhere)
+  void line_solve(const adj::View& V, const double* f, double* x) {
+    hipStream_t s = ctx->stream;
+    const int lb = ceil_div(g.M - 1 + g.M - 2, 64);
+    adj::k_fill<<<blocks(n), adj::BS, 0, s>>>(n, 0.0, x);
+    adj::k_line_solve<<<lb, 64, 0, s>>>(V, band.p, f, nullptr, x, lt.p, 0);
+    for (int k = 0; k < sweeps; ++k) {
+      spmv(x, ta.p);
+      adj::k_line_solve<<<lb, 64, 0, s>>>(V, band.p, f, ta.p, x, lt.p, 1);
+    }
+  }
+
   void precond(const adj::View& V, const double* f, double* z, const cps_simple_config& pcfg) {
+    if (kind == CPS_ADJ_PRECOND_LSC_LINE) precond_lsc(V, f, z, pcfg);
+    else precond_simple(V, f, z, pcfg);
+  }
+
+  // z = M^-1 f, block factorisation with the x-line momentum solve and the
+  // least-squares-commutator Schur approximation
+  void precond_lsc(const adj::View& V, const double* f, double* z, const cps_simple_config& pcfg) {
+    hipStream_t s = ctx->stream;
+    const int sb = blocks((int)S);
+    line_solve(V, f, t1.p);  // x* = A~^-1 f_m
+    spmv(t1.p, t2.p);
+    adj::k_pre2<<<sb, adj::BS, 0, s>>>(V, t2.p, f, rp.p, xp.p);  // C x* - f_p
+    mg->solve(dEa.p, dNa.p, rp.p, xp.p, st.p, pcfg, true, &timer);  // L^-1
+    adj::k_pre3<<<blocks(n), adj::BS, 0, s>>>(V, n, xp.p, t3.p);
+    spmv(t3.p, t2.p);  // B y1
+    adj::k_dscale<<<blocks(n), adj::BS, 0, s>>>(V, n, dj.p, t2.p, t3.p);
+    spmv(t3.p, t2.p);  // A D^-1 B y1
+    adj::k_dscale<<<blocks(n), adj::BS, 0, s>>>(V, n, dj.p, t2.p, t3.p);
+    spmv(t3.p, t2.p);  // C D^-1 A D^-1 B y1
+    adj::k_pre2<<<sb, adj::BS, 0, s>>>(V, t2.p, nullptr, rp.p, xp.p);
+    mg->solve(dEa.p, dNa.p, rp.p, xp.p, st.p, pcfg, true, &timer);  // y = L^-1 (...)
+    adj::k_pre3<<<blocks(n), adj::BS, 0, s>>>(V, n, xp.p, t3.p);
+    spmv(t3.p, t2.p);  // B y
+    line_solve(V, t2.p, t4.p);
+    adj::k_lsc_out<<<blocks(n), adj::BS, 0, s>>>(V, n, f, t1.p, t4.p, xp.p, z);
+    CPS_HIP(hipGetLastError());
+  }
+
+  // z = M^-1 f (SIMPLE block preconditioner)
+  void precond_simple(const adj::View& V, const double* f, double* z, const cps_simple_config& pcfg) {
     hipStream_t s = ctx->stream;
     adj::k_pre1<<<blocks(n), adj::BS, 0, s>>>(V, n, dinv.p, f, t1.p);
     spmv(t1.p, t2.p);
@@ -799,6 +1044,10 @@ void flow_adjoint_run(FlowAdjoint* a, const SimpleView& sv, double rtol, int max
 void flow_adjoint_apply(FlowAdjoint* a, const SimpleView& sv, int transpose, const double* x, double* y) {
   adj::View V{sv.g, sv.S, sv.u, sv.v, sv.p, sv.gam, sv.alp, sv.inlet, sv.dE, sv.dN};
   a->apply(V, transpose != 0, x, y);
+}
+void flow_adjoint_options(FlowAdjoint* a, int kind, int sweeps) {
+  a->kind = kind;
+  a->sweeps = sweeps;
 }
 void flow_adjoint_residual(FlowAdjoint* a, const SimpleView& sv, double* out3) {
   adj::View V{sv.g, sv.S, sv.u, sv.v, sv.p, sv.gam, sv.alp, sv.inlet, sv.dE, sv.dN};

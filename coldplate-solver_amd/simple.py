@@ -102,6 +102,13 @@ class SIMPLE:
         self.last_adjoint = st.as_dict()
         return dp.value, grad
 
+    def adjoint_options(self, preconditioner="simple", line_sweeps=4):
+        """Preconditioner of the adjoint solve: "lsc_line" (x-line Jacobi
+        momentum solves, least-squares-commutator Schur complement) or
+        "simple" (1/aP0 momentum, pressure-correction Schur complement)."""
+        kinds = {"simple": T.CPS_ADJ_PRECOND_SIMPLE, "lsc_line": T.CPS_ADJ_PRECOND_LSC_LINE}
+        _capi.check(self.L.cps_simple_adjoint_options(self.h, kinds[preconditioner], int(line_sweeps)))
+
     def state_residual(self):
         """max |R| of the U-momentum, V-momentum and continuity equations."""
         out = np.empty(3)

@@ -46,6 +46,14 @@
 
 namespace oracle {
 
+#ifndef ORACLE_OMP_MIN_LOOP_DEFINED
+#define ORACLE_OMP_MIN_LOOP_DEFINED
+// loops shorter than this run on one thread: on small grids the fork/join
+// and barrier cost of a parallel region exceeds the loop's work, badly so
+// when other processes hold the cores
+constexpr int kOmpMinLoop = 16384;
+#endif
+
 template <class R>
 struct Csr {
   int n = 0;
@@ -393,7 +401,7 @@ class Simple {
     const R DeScale = hy / hx, DnScale = hx / hy;
     const R convScale = enable25D ? convFactor25D : R(1.0f);
     const int n = csr.n;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinLoop)
     for (int row = 0; row < n; ++row) {
       const int i = map[row].first, j = map[row].second;
       R alphaLocal = IsU ? alphaAtU(i, j) : alphaAtV(i, j);
@@ -494,7 +502,7 @@ class Simple {
     int iter = 0;
     for (; iter < maxIter; ++iter) {
       R total = 0;
-#pragma omp parallel for reduction(+ : total) schedule(static)
+#pragma omp parallel for reduction(+ : total) schedule(static) if(n >= kOmpMinLoop)
       for (int i = 0; i < n; ++i) {
         R Ax = 0;
         for (int k = rp[i]; k < rp[i + 1]; ++k) Ax += val[k] * (*xc)[ci[k]];
@@ -521,7 +529,7 @@ class Simple {
       diag[i] = d;
     }
     auto spmv = [&](const std::vector<R>& xx, std::vector<R>& y) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinLoop)
       for (int i = 0; i < n; ++i) {
         R s = 0;
         for (int k = rp[i]; k < rp[i + 1]; ++k) s += val[k] * xx[ci[k]];
@@ -536,32 +544,32 @@ class Simple {
     jac(r, z);
     pv = z;
     R rz = 0;
-#pragma omp parallel for reduction(+ : rz)
+#pragma omp parallel for reduction(+ : rz) if(n >= kOmpMinLoop)
     for (int i = 0; i < n; ++i) rz += r[i] * z[i];
     int iter = 0;
     for (; iter < maxIter; ++iter) {
       spmv(pv, Ap);
       R pAp = 0;
-#pragma omp parallel for reduction(+ : pAp)
+#pragma omp parallel for reduction(+ : pAp) if(n >= kOmpMinLoop)
       for (int i = 0; i < n; ++i) pAp += pv[i] * Ap[i];
       if (std::abs(pAp) < R(1e-20f)) break;
       R a = rz / pAp;
-#pragma omp parallel for
+#pragma omp parallel for if(n >= kOmpMinLoop)
       for (int i = 0; i < n; ++i) {
         x[i] += a * pv[i];
         r[i] -= a * Ap[i];
       }
       R err = 0;
-#pragma omp parallel for reduction(+ : err)
+#pragma omp parallel for reduction(+ : err) if(n >= kOmpMinLoop)
       for (int i = 0; i < n; ++i) err += r[i] * r[i];
       if (std::sqrt(err) < tol) break;
       jac(r, z);
       R rz_prev = rz;
       rz = 0;
-#pragma omp parallel for reduction(+ : rz)
+#pragma omp parallel for reduction(+ : rz) if(n >= kOmpMinLoop)
       for (int i = 0; i < n; ++i) rz += r[i] * z[i];
       R beta = rz / rz_prev;
-#pragma omp parallel for
+#pragma omp parallel for if(n >= kOmpMinLoop)
       for (int i = 0; i < n; ++i) pv[i] = z[i] + beta * pv[i];
     }
     return iter;
@@ -591,7 +599,7 @@ class Simple {
   // pressure_solver.cpp:703-742 : values + RHS of the pressure system
   void assemblePressure() {
     Csr<R>& c = csrP;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(c.n >= kOmpMinLoop)
     for (int row = 0; row < c.n; ++row) {
       int i = mapP[row].first, j = mapP[row].second;
       R aE = (j < N - 1) ? rho * dE[(size_t)i * N + j] * hy : R(0);

@@ -33,6 +33,14 @@
 
 namespace oracle {
 
+#ifndef ORACLE_OMP_MIN_LOOP_DEFINED
+#define ORACLE_OMP_MIN_LOOP_DEFINED
+// loops shorter than this run on one thread: on small grids the fork/join
+// and barrier cost of a parallel region exceeds the loop's work, badly so
+// when other processes hold the cores
+constexpr int kOmpMinLoop = 16384;
+#endif
+
 struct ThermalProblem {
   int nx = 0, ny = 0, nz = 0;
   double dx = 0, dy = 0;
@@ -97,7 +105,7 @@ inline ThermalCsr assembleThermal(const ThermalProblem& P) {
   A.row_ptr.assign(n + 1, 0);
   A.b.assign(n, 0.0);
   std::vector<std::vector<std::pair<int, double>>> rows(n);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if((long)nz * nx * ny >= kOmpMinLoop)
   for (int k = 0; k < nz; ++k) {
     const double dzk = P.dz[k], Ax = dy * dzk, Ay = dx * dzk, Az = dx * dy;
     for (int j = 0; j < ny; ++j)
@@ -191,7 +199,7 @@ inline int bicgstabCsr(const ThermalCsr& A, std::vector<double>& x, double rtol,
     dinv[i] = 1.0 / d;
   }
   auto spmv = [&](const std::vector<double>& xx, std::vector<double>& y) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinLoop)
     for (int i = 0; i < n; ++i) {
       double acc = 0;
       for (int q = A.row_ptr[i]; q < A.row_ptr[i + 1]; ++q) acc += A.values[q] * xx[A.col_idx[q]];
@@ -200,7 +208,7 @@ inline int bicgstabCsr(const ThermalCsr& A, std::vector<double>& x, double rtol,
   };
   auto dot = [&](const std::vector<double>& a, const std::vector<double>& b) {
     double acc = 0;
-#pragma omp parallel for reduction(+ : acc) schedule(static)
+#pragma omp parallel for reduction(+ : acc) schedule(static) if(n >= kOmpMinLoop)
     for (int i = 0; i < n; ++i) acc += a[i] * b[i];
     return acc;
   };
@@ -216,21 +224,21 @@ inline int bicgstabCsr(const ThermalCsr& A, std::vector<double>& x, double rtol,
     double rho1 = dot(rh, r);
     double beta = (rho1 / rho) * (alpha / omega);
     rho = rho1;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinLoop)
     for (int i = 0; i < n; ++i) {
       p[i] = r[i] + beta * (p[i] - omega * vv[i]);
       ph[i] = dinv[i] * p[i];
     }
     spmv(ph, vv);
     alpha = rho / dot(rh, vv);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinLoop)
     for (int i = 0; i < n; ++i) {
       s[i] = r[i] - alpha * vv[i];
       sh[i] = dinv[i] * s[i];
     }
     spmv(sh, t);
     omega = dot(t, s) / dot(t, t);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinLoop)
     for (int i = 0; i < n; ++i) {
       x[i] += alpha * ph[i] + omega * sh[i];
       r[i] = s[i] - omega * t[i];

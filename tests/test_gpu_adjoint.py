@@ -214,3 +214,32 @@ def test_torch_autograd_pressure_drop(pkg):
     assert float(dp) == pytest.approx(dp2, rel=1e-12)
     np.testing.assert_allclose(gt.grad.numpy(), grad, rtol=1e-8, atol=1e-10 * np.abs(grad).max())
     s.close()
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_preconditioners_agree_and_lsc_line_needs_fewer_iterations(pkg, scheme):
+    """The x-line / least-squares-commutator preconditioner and the SIMPLE
+    block preconditioner solve the same adjoint system: same gradient to the
+    solve tolerance, the former in fewer FGMRES iterations."""
+    cfg, g, inlet = pkg.configs.density_field(nx=96, ny=48, seed=2, sigma=9.0)
+    cfg.convection_scheme = scheme
+    cfg.p_tol, cfg.max_pressure_iter = 1e-13, 200
+    cfg.momentum_tol, cfg.max_momentum_iter = 1e-13, 200
+    s = pkg.SIMPLE(cfg, g, inlet)
+    s.step(4000)
+    out = {}
+    for kind, sweeps in (("simple", 0), ("lsc_line", 4), ("lsc_line", 0)):
+        s.adjoint_options(kind, sweeps)
+        dp, grad = s.gradient_dp(rtol=1e-9, max_iter=3000)
+        print(kind, sweeps, s.last_adjoint, flush=True)
+        assert s.last_adjoint["converged"] == 1
+        out[(kind, sweeps)] = (dp, grad, s.last_adjoint["iterations"])
+    ref = out[("simple", 0)]
+    for key, (dp, grad, its) in out.items():
+        assert dp == ref[0]
+        err = np.abs(grad - ref[1]).max() / np.abs(ref[1]).max()
+        assert err < 1e-6, (key, err)
+    assert out[("lsc_line", 4)][2] < out[("simple", 0)][2]
+    with pytest.raises(pkg._capi.CpsError):
+        s.adjoint_options("lsc_line", -1)
+    s.close()
