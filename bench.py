@@ -168,9 +168,11 @@ def grads_batch_leg(pkg, n_fields=32, nx=512, ny=256, forward_steps=1500, rtol=1
     dt = time.perf_counter() - t0
     its = [r[2]["iterations"] for r in res]
     ok = all(r[2]["converged"] for r in res)
+    its_max = max(its)
     b.close()
     return dict(fields=n_fields, workers=workers, seconds=dt, grads_per_s=n_fields / dt, forward_steps=forward_steps,
-                forward_seconds=t_fwd, fgmres_iterations_mean=sum(its) / len(its), all_converged=ok)
+                forward_seconds=t_fwd, fgmres_iterations_mean=sum(its) / len(its), fgmres_iterations_max=its_max,
+                all_converged=ok)
 
 
 def thermal_leg(pkg, reps=3):
@@ -311,7 +313,8 @@ def main():
         "grads_per_s": None if grads is None else round(grads["grads_per_s"], 3),
         "grads": None if grads is None else {
             "workload": "d(dP)/d(gamma), 512x256 density field (seed 0, SOU) at its SIMPLE fixed point; "
-                        "Jacobian build + adjoint FGMRES (rtol 1e-8) + gradient",
+                        "Jacobian build + adjoint FGMRES (rtol 1e-8; LSC block preconditioner, x/y line "
+                        "momentum solves) + gradient",
             "seconds_per_grad": round(grads["seconds_per_grad"], 4),
             "fgmres_iterations": grads["adjoint"]["iterations"],
             "adjoint_rel_residual": grads["adjoint"]["rel_residual"],
@@ -320,7 +323,8 @@ def main():
         },
         "grads_batch": None if grads_batch is None else {
             "workload": "32 seeded 512x256 density fields (SOU) after 1500 batched SIMPLE iterations; all 32 "
-                        "d(dP)/d(gamma) at once (host threads, per-field streams), rtol 1e-8",
+                        "d(dP)/d(gamma) at once (host threads, per-field streams), rtol 1e-8, LSC x/y line "
+                        "preconditioner",
             **{k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in grads_batch.items()},
         },
         "batch": None if batch is None else {

@@ -17,6 +17,7 @@ CPS_PRESSURE_GMG = 5
 
 CPS_ADJ_PRECOND_SIMPLE = 0
 CPS_ADJ_PRECOND_LSC_LINE = 1
+CPS_ADJ_PRECOND_LSC_ADI = 2
 
 CPS_PREC_FP64 = 64
 CPS_PREC_MIXED = 32

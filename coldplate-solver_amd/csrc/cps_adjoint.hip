@@ -25,20 +25,21 @@
 // continuity rows) is solved by right-preconditioned FGMRES with a block
 // factorisation preconditioner
 //   x* = A~^-1 f_m,  y = S~^-1 (C x* - f_p),  x = x* - A~^-1 B y.
-// Two forms:
-//   SIMPLE (default)  A~ = diag(aP0) (aP0 taken from the Jacobian kernel),
-//                     S~ = C diag(aP0)^-1 B, which is exactly the SIMPLE
-//                     pressure-correction operator with dE = hy / aP0,
-//                     dN = hx / aP0, solved by a few geometric-multigrid PCG
+// Three forms:
+//   LSC_ADI (default) A~^-1 = line Jacobi on A: pentadiagonal solves along
+//                     grid rows (x-lines) and columns (y-lines), an x pass,
+//                     a y pass, then damped rounds of both; S~^-1 the
+//                     least-squares-commutator approximation
+//                     L^-1 (C D^-1 A D^-1 B) L^-1 of S = C A^-1 B, D = diag(A),
+//                     L = C D^-1 B, which is exactly the SIMPLE
+//                     pressure-correction operator with dE = hy / D_u,
+//                     dN = hx / D_v, solved by a few geometric-multigrid PCG
 //                     iterations;
-//   LSC_LINE          A~^-1 = a few x-line Jacobi sweeps on A (pentadiagonal
-//                     lines along the grid rows), S~^-1 the least-squares-
-//                     commutator approximation L^-1 (C D^-1 A D^-1 B) L^-1 of
-//                     S = C A^-1 B, D = diag(A), L = C D^-1 B (the same
-//                     multigrid operator with D in place of aP0).  Several
-//                     times fewer iterations where it converges, but the line
-//                     sweeps do not converge on every second-order-upwind
-//                     state, so it is opt-in.
+//   LSC_LINE          the same with x-lines only (fewer passes, but its
+//                     sweeps diverge where the flow turns across the rows);
+//   SIMPLE            A~ = diag(aP0) (aP0 taken from the Jacobian kernel),
+//                     S~ = C diag(aP0)^-1 B (the same multigrid operator with
+//                     aP0 in place of D): 5-20x more iterations.
 // Vectors are three padded fields [u | v | p]; rows that are not DOFs are
 // identity rows.
 #include <algorithm>
@@ -494,30 +495,51 @@ __global__ __launch_bounds__(BS) void k_pre4(View V, int n, const double* dinv, 
 }
 
 // ---------------------------------------------------------------------------
-// x-line preconditioner of the momentum block A of J^T.  The momentum
-// equations couple a DOF to its neighbours j-2..j+2 along the grid row
-// (second-order upwinding reaches two cells), so each grid row of u (i =
-// 1..M-1, j = 1..N-2) and of v (i = 1..M-2, j = 1..N-1) is a pentadiagonal
-// system; the couplings across rows and between u and v stay explicit.  One
-// line-Jacobi sweep: x += Dl^-1 (f - A x).  Dropping the j +- 2 entries
-// (tridiagonal lines) makes the sweeps diverge on SOU states.
+// Line preconditioners of the momentum block A of J^T.  The momentum
+// equations couple a DOF to its neighbours two cells away in each direction
+// (second-order upwinding), so every grid row (x-line) and every grid column
+// (y-line) of u (i = 1..M-1, j = 1..N-2) and of v (i = 1..M-2, j = 1..N-1) is
+// a pentadiagonal system; couplings off the line and between u and v stay
+// explicit.  A sweep is x += omega Dl^-1 (f - A x).  Dropping the +-2 entries
+// (tridiagonal lines) makes the sweeps diverge on SOU states; undamped x-line
+// sweeps alone diverge where the flow turns across the lines (the v DOFs of
+// an inlet bend carry an eigenvalue -0.53 +- 1.0i of I - Dl^-1 A), which
+// damping and alternating x / y lines cure.
 constexpr int kLineChunk = 16;
 
-__device__ inline int n_lines(const Geo& g) { return (g.M - 1) + (g.M - 2); }
-// first row (in the [u | v | p] numbering) and length of line l
-__device__ inline void line_span(const View& V, int l, size_t* base, int* len) {
+// Lines of one direction: Y = false rows (x-lines), Y = true columns.
+template <bool Y>
+__device__ inline int n_lines(const Geo& g) {
+  return Y ? (g.N - 2) + (g.N - 1) : (g.M - 1) + (g.M - 2);
+}
+// first row (in the [u | v | p] numbering), length and stride of line l
+template <bool Y>
+__device__ inline void line_span(const View& V, int l, size_t* base, int* len, int* step) {
   const Geo& g = V.g;
-  if (l < g.M - 1) {
-    *base = (size_t)(l + 1) * g.P + 1;
-    *len = g.N - 2;
+  if (!Y) {
+    *step = 1;
+    if (l < g.M - 1) {
+      *base = (size_t)(l + 1) * g.P + 1;
+      *len = g.N - 2;
+    } else {
+      *base = V.S + (size_t)(l - (g.M - 1) + 1) * g.P + 1;
+      *len = g.N - 1;
+    }
   } else {
-    *base = V.S + (size_t)(l - (g.M - 1) + 1) * g.P + 1;
-    *len = g.N - 1;
+    *step = g.P;
+    if (l < g.N - 2) {
+      *base = (size_t)g.P + (l + 1);
+      *len = g.M - 1;
+    } else {
+      *base = V.S + (size_t)g.P + (l - (g.N - 2) + 1);
+      *len = g.M - 2;
+    }
   }
 }
 
-// band[k * 2S + r] = entry (r, r + k - 2) of J^T on momentum DOF rows, the
-// pressure part and cross-row entries excluded; dj = 1 / diagonal
+// band[k * 2S + r] = entry (r, r + (k - 2) step) of J^T on momentum DOF rows
+// (same component, same line); dj = 1 / diagonal
+template <bool Y>
 __global__ __launch_bounds__(BS) void k_band(View V, int n, int WT, const int* cnt, const int* colT,
                                             const double* valT, double* band, double* dj) {
   const int r = blockIdx.x * BS + threadIdx.x;
@@ -531,36 +553,40 @@ __global__ __launch_bounds__(BS) void k_band(View V, int n, int WT, const int* c
     const int m = min(cnt[r], WT);
     for (int s = 0; s < m; ++s) {
       const int c = colT[(size_t)s * n + r];
-      const int off = c - r;
-      if (off >= -2 && off <= 2 && c / (int)V.S == comp && c / g.P == r / g.P) b[off + 2] += valT[(size_t)s * n + r];
+      if (c / (int)V.S != comp) continue;
+      const int qc = c % (int)V.S, ic = qc / g.P, jc = qc % g.P;
+      const int off = Y ? ic - i : jc - j;
+      const bool same_line = Y ? jc == j : ic == i;
+      if (same_line && off >= -2 && off <= 2) b[off + 2] += valT[(size_t)s * n + r];
     }
   } else {
     b[2] = 1.0;
   }
 #pragma unroll
   for (int k = 0; k < 5; ++k) band[k * S2 + r] = b[k];
-  dj[r] = b[2] != 0.0 ? 1.0 / b[2] : 1.0;
+  if (dj) dj[r] = b[2] != 0.0 ? 1.0 / b[2] : 1.0;
 }
 
 // in-place banded LU without pivoting, one thread per line:
 // band slots become [l2, l1, 1/u0, u1, u2]
+template <bool Y>
 __global__ __launch_bounds__(64) void k_line_factor(View V, double* band, int* bad) {
   const int l = blockIdx.x * 64 + threadIdx.x;
-  if (l >= n_lines(V.g)) return;
+  if (l >= n_lines<Y>(V.g)) return;
   size_t base;
-  int len;
-  line_span(V, l, &base, &len);
+  int len, step;
+  line_span<Y>(V, l, &base, &len, &step);
   const size_t S2 = 2 * V.S;
   double* A2 = band;
   double* A1 = band + S2;
   double* D = band + 2 * S2;
   double* C1 = band + 3 * S2;
   double* C2 = band + 4 * S2;
-  // U rows j-1 and j-2: (u0, u1, u2)
+  // U rows j-1 (p) and j-2 (q): (u0, u1, u2)
   double p0 = 1.0, p1 = 0.0, p2 = 0.0, q0 = 1.0, q1 = 0.0, q2 = 0.0;
   int fail = 0;
   for (int j = 0; j < len; ++j) {
-    const size_t r = base + j;
+    const size_t r = base + (size_t)j * step;
     const double a2 = j >= 2 ? A2[r] : 0.0, a1 = j >= 1 ? A1[r] : 0.0;
     const double c1 = j + 1 < len ? C1[r] : 0.0, c2 = j + 2 < len ? C2[r] : 0.0;
     const double l2 = a2 / q0;
@@ -579,15 +605,20 @@ __global__ __launch_bounds__(64) void k_line_factor(View V, double* band, int* b
   if (fail) atomicOr(bad, 1);
 }
 
-// x = Dl^-1 f (accumulate = 0) or x += Dl^-1 (f - Ax) on momentum DOF rows;
+// x = Dl^-1 f (Ax null) or x += omega Dl^-1 (f - Ax) on momentum DOF rows;
 // tmp holds the forward sweep.  Loads go in chunks so each chunk waits once.
+// x-lines: consecutive threads walk rows P apart (each thread streams its own
+// cache lines); y-lines: consecutive threads walk adjacent columns
+// (coalesced).
+template <bool Y>
 __global__ __launch_bounds__(64) void k_line_solve(View V, const double* band, const double* f, const double* Ax,
-                                                  double* x, double* tmp, int accumulate) {
+                                                  double omega, double* x, double* tmp) {
   const int l = blockIdx.x * 64 + threadIdx.x;
-  if (l >= n_lines(V.g)) return;
+  if (l >= n_lines<Y>(V.g)) return;
   size_t base;
-  int len;
-  line_span(V, l, &base, &len);
+  int len, step;
+  line_span<Y>(V, l, &base, &len, &step);
+  const bool acc = Ax != nullptr;
   const size_t S2 = 2 * V.S;
   const double* L2 = band;
   const double* L1 = band + S2;
@@ -599,8 +630,8 @@ __global__ __launch_bounds__(64) void k_line_solve(View V, const double* band, c
     double bb[kLineChunk], a1[kLineChunk], a2[kLineChunk];
 #pragma unroll
     for (int k = 0; k < kLineChunk; ++k) {
-      const size_t r = base + min(j0 + k, len - 1);
-      bb[k] = accumulate ? f[r] - Ax[r] : f[r];
+      const size_t r = base + (size_t)min(j0 + k, len - 1) * step;
+      bb[k] = acc ? f[r] - Ax[r] : f[r];
       a1[k] = L1[r];
       a2[k] = L2[r];
     }
@@ -608,7 +639,7 @@ __global__ __launch_bounds__(64) void k_line_solve(View V, const double* band, c
     for (int k = 0; k < kLineChunk; ++k) {
       if (j0 + k < len) {
         const double yj = bb[k] - a1[k] * y1 - a2[k] * y2;
-        tmp[base + j0 + k] = yj;
+        tmp[base + (size_t)(j0 + k) * step] = yj;
         y2 = y1;
         y1 = yj;
       }
@@ -619,18 +650,18 @@ __global__ __launch_bounds__(64) void k_line_solve(View V, const double* band, c
     double yy[kLineChunk], d[kLineChunk], c1[kLineChunk], c2[kLineChunk], xo[kLineChunk];
 #pragma unroll
     for (int k = 0; k < kLineChunk; ++k) {
-      const size_t r = base + max(j1 - k, 0);
+      const size_t r = base + (size_t)max(j1 - k, 0) * step;
       yy[k] = tmp[r];
       d[k] = U0[r];
       c1[k] = U1[r];
       c2[k] = U2[r];
-      xo[k] = accumulate ? x[r] : 0.0;
+      xo[k] = acc ? x[r] : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < kLineChunk; ++k) {
       if (j1 - k >= 0) {
         const double xj = (yy[k] - c1[k] * x1 - c2[k] * x2) * d[k];
-        x[base + j1 - k] = xo[k] + xj;
+        x[base + (size_t)(j1 - k) * step] = acc ? xo[k] + omega * xj : xj;
         x2 = x1;
         x1 = xj;
       }
@@ -730,17 +761,27 @@ class FlowAdjoint {
   int n = 0, WT = 0, nb = 0;
   DevBuf<int> col, colT, cnt, cur, flag, iout, bad;
   DevBuf<double> val, valT, res, dinv, gvec, lam, r, Vb, Zb, w, t1, t2, t3, t4, ta, lt, y, rp, xp, dEa, dNa, part,
-      sums, grad, obj, band, dj;
-  // preconditioner: CPS_ADJ_PRECOND_SIMPLE (default) or CPS_ADJ_PRECOND_LSC_LINE
-  int kind = CPS_ADJ_PRECOND_SIMPLE;
-  int sweeps = 4;  // x-line Jacobi sweeps per momentum-block solve
+      sums, grad, obj, bandx, bandy, dj;
+  // preconditioner: CPS_ADJ_PRECOND_SIMPLE (default), _LSC_LINE or _LSC_ADI
+  int kind = CPS_ADJ_PRECOND_LSC_ADI;
+  int sweeps = 2;       // line sweeps per momentum-block solve after the first
+  double omega = 0.8;   // damping of the sweeps
   DevBuf<IterState> st;
   std::unique_ptr<PressureMG> mg;
   KernelTimer timer;
   // Krylov basis per cycle: the SIMPLE-preconditioned adjoint of SOU states
   // stagnates under short restarts (right-preconditioned GMRES(40) makes no
-  // progress on 32 x 16 fields; 160-200 converges)
-  static constexpr int kRestart = 200;
+  // progress on 32 x 16 fields; 160-200 converges); the LSC forms converge
+  // in 20-60 iterations on 512 x 256 fields.  The basis (restart + 1 vectors
+  // of 3 fields) is allocated for the form in use.
+  int restart() const { return kind == CPS_ADJ_PRECOND_SIMPLE ? 200 : 80; }
+  void ensure_basis(int m) {
+    if (Vb.n >= (size_t)(m + 1) * n) return;
+    Vb.alloc((size_t)(m + 1) * n);
+    Zb.alloc((size_t)m * n);
+    part.alloc((size_t)(m + 1) * nb);
+    sums.alloc(m + 1);
+  }
 
   FlowAdjoint(Ctx* c, const Geo& geo, size_t fieldS) : ctx(c), g(geo), S(fieldS) {
     n = (int)(3 * S);
@@ -750,19 +791,16 @@ class FlowAdjoint {
     for (auto* b : {&cnt, &cur, &flag}) b->alloc(n);
     iout.alloc(2);
     for (auto* b : {&res, &dinv, &gvec, &lam, &r, &w, &t1, &t2, &t3, &t4, &ta, &lt}) b->alloc(n);
-    band.alloc(5 * 2 * S);
+    bandx.alloc(5 * 2 * S);
+    bandy.alloc(5 * 2 * S);
     dj.alloc(2 * S);
     bad.alloc(1);
-    Vb.alloc((size_t)(kRestart + 1) * n);
-    Zb.alloc((size_t)kRestart * n);
     y.alloc(S);
     dEa.alloc(S);
     dNa.alloc(S);
     rp.alloc(S);
     xp.alloc(S);
     nb = std::min(ceil_div(n, adj::BS), 1024);
-    part.alloc((size_t)(kRestart + 1) * nb);
-    sums.alloc(kRestart + 1);
     grad.alloc((size_t)(g.M - 1) * (g.N - 1));
     obj.alloc(2);
     st.alloc(1);
@@ -793,16 +831,21 @@ class FlowAdjoint {
     CPS_HIP(hipMemsetAsync(cur.p, 0, cur.bytes(), s));
     adj::k_scatter<<<blocks(n), adj::BS, 0, s>>>(n, col.p, val.p, cur.p, WT, colT.p, valT.p);
     adj::k_sort<<<blocks(n), adj::BS, 0, s>>>(n, WT, cnt.p, colT.p, valT.p);
-    if (kind == CPS_ADJ_PRECOND_LSC_LINE) {
-      adj::k_band<<<blocks((int)(2 * S)), adj::BS, 0, s>>>(V, n, WT, cnt.p, colT.p, valT.p, band.p, dj.p);
+    if (kind != CPS_ADJ_PRECOND_SIMPLE) {
+      const int b2 = blocks((int)(2 * S));
       CPS_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
-      adj::k_line_factor<<<ceil_div(g.M - 1 + g.M - 2, 64), 64, 0, s>>>(V, band.p, bad.p);
+      adj::k_band<false><<<b2, adj::BS, 0, s>>>(V, n, WT, cnt.p, colT.p, valT.p, bandx.p, dj.p);
+      adj::k_line_factor<false><<<ceil_div(nlines(false), 64), 64, 0, s>>>(V, bandx.p, bad.p);
+      if (kind == CPS_ADJ_PRECOND_LSC_ADI) {
+        adj::k_band<true><<<b2, adj::BS, 0, s>>>(V, n, WT, cnt.p, colT.p, valT.p, bandy.p, nullptr);
+        adj::k_line_factor<true><<<ceil_div(nlines(true), 64), 64, 0, s>>>(V, bandy.p, bad.p);
+      }
       int hb = 0;
       CPS_HIP(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
       CPS_HIP(hipStreamSynchronize(s));
-      CPS_CHECK(hb == 0, CPS_ERR_DIVERGED, "adjoint: zero pivot in an x-line factorisation of the momentum block");
+      CPS_CHECK(hb == 0, CPS_ERR_DIVERGED, "adjoint: zero pivot in a line factorisation of the momentum block");
     }
-    adj::k_faces<<<blocks((int)S), adj::BS, 0, s>>>(V, kind == CPS_ADJ_PRECOND_LSC_LINE ? dj.p : dinv.p, dEa.p,
+    adj::k_faces<<<blocks((int)S), adj::BS, 0, s>>>(V, kind == CPS_ADJ_PRECOND_SIMPLE ? dinv.p : dj.p, dEa.p,
                                                    dNa.p);
     CPS_HIP(hipGetLastError());
   }
@@ -811,21 +854,37 @@ class FlowAdjoint {
     adj::k_spmv<<<blocks(n), adj::BS, 0, ctx->stream>>>(n, WT, cnt.p, colT.p, valT.p, x, out);
   }
 
-  // x = A~^-1 f_m by x-line Jacobi (x: momentum DOF rows, 0 elsewhere)
-  void line_solve(const adj::View& V, const double* f, double* x) {
+  int nlines(bool y) const { return y ? (g.N - 2) + (g.N - 1) : (g.M - 1) + (g.M - 2); }
+
+  // one line pass: x = Dl^-1 f (first) or x += omega Dl^-1 (f - A x)
+  void line_pass(const adj::View& V, bool y, const double* f, double* x, bool first) {
     hipStream_t s = ctx->stream;
-    const int lb = ceil_div(g.M - 1 + g.M - 2, 64);
-    adj::k_fill<<<blocks(n), adj::BS, 0, s>>>(n, 0.0, x);
-    adj::k_line_solve<<<lb, 64, 0, s>>>(V, band.p, f, nullptr, x, lt.p, 0);
-    for (int k = 0; k < sweeps; ++k) {
+    const double* Ax = nullptr;
+    if (!first) {
       spmv(x, ta.p);
-      adj::k_line_solve<<<lb, 64, 0, s>>>(V, band.p, f, ta.p, x, lt.p, 1);
+      Ax = ta.p;
+    }
+    if (y) adj::k_line_solve<true><<<ceil_div(nlines(true), 64), 64, 0, s>>>(V, bandy.p, f, Ax, omega, x, lt.p);
+    else adj::k_line_solve<false><<<ceil_div(nlines(false), 64), 64, 0, s>>>(V, bandx.p, f, Ax, omega, x, lt.p);
+  }
+
+  // x = A~^-1 f_m by line Jacobi (x: momentum DOF rows, 0 elsewhere): an
+  // x-line solve, then (ADI) a y-line pass, then `sweeps` rounds of x (and y)
+  // passes
+  void line_solve(const adj::View& V, const double* f, double* x) {
+    const bool adi = kind == CPS_ADJ_PRECOND_LSC_ADI;
+    adj::k_fill<<<blocks(n), adj::BS, 0, ctx->stream>>>(n, 0.0, x);
+    line_pass(V, false, f, x, true);
+    if (adi) line_pass(V, true, f, x, false);
+    for (int k = 0; k < sweeps; ++k) {
+      line_pass(V, false, f, x, false);
+      if (adi) line_pass(V, true, f, x, false);
     }
   }
 
   void precond(const adj::View& V, const double* f, double* z, const cps_simple_config& pcfg) {
-    if (kind == CPS_ADJ_PRECOND_LSC_LINE) precond_lsc(V, f, z, pcfg);
-    else precond_simple(V, f, z, pcfg);
+    if (kind == CPS_ADJ_PRECOND_SIMPLE) precond_simple(V, f, z, pcfg);
+    else precond_lsc(V, f, z, pcfg);
   }
 
   // z = M^-1 f, block factorisation with the x-line momentum solve and the
@@ -889,10 +948,11 @@ class FlowAdjoint {
     }
   }
 
-  // FGMRES(kRestart) on J^T lam = g, from lam = 0; returns iterations
+  // FGMRES(restart()) on J^T lam = g, from lam = 0; returns iterations
   int fgmres(const adj::View& V, double rtol, int max_iter, const cps_simple_config& pcfg, double* relres) {
     hipStream_t s = ctx->stream;
-    const int m = kRestart;
+    const int m = restart();
+    ensure_basis(m);
     CPS_HIP(hipMemsetAsync(lam.p, 0, lam.bytes(), s));
     const double gnorm = norm(gvec.p);
     if (gnorm == 0.0) {
@@ -1045,9 +1105,10 @@ void flow_adjoint_apply(FlowAdjoint* a, const SimpleView& sv, int transpose, con
   adj::View V{sv.g, sv.S, sv.u, sv.v, sv.p, sv.gam, sv.alp, sv.inlet, sv.dE, sv.dN};
   a->apply(V, transpose != 0, x, y);
 }
-void flow_adjoint_options(FlowAdjoint* a, int kind, int sweeps) {
+void flow_adjoint_options(FlowAdjoint* a, int kind, int sweeps, double omega) {
   a->kind = kind;
   a->sweeps = sweeps;
+  a->omega = omega;
 }
 void flow_adjoint_residual(FlowAdjoint* a, const SimpleView& sv, double* out3) {
   adj::View V{sv.g, sv.S, sv.u, sv.v, sv.p, sv.gam, sv.alp, sv.inlet, sv.dE, sv.dN};

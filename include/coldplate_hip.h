@@ -175,16 +175,22 @@ int cps_simple_state_residual(cps_simple* s, double* out3);
  * equations at the current fields; x, y are [u | v | p] in the field shapes
  * above, concatenated; entries that are not DOFs pass through (identity). */
 int cps_simple_jacobian_apply(cps_simple* s, int transpose, const double* x, double* y);
-/* Preconditioner of the adjoint solve behind cps_simple_gradient_dp:
- * CPS_ADJ_PRECOND_SIMPLE (default) = the SIMPLE block form (momentum by
- * 1/aP0, Schur complement by the pressure-correction operator);
- * CPS_ADJ_PRECOND_LSC_LINE = block factorisation with x-line Jacobi momentum
- * solves (line_sweeps >= 0 sweeps after the first line solve) and a
- * least-squares-commutator Schur complement.  Both solve the same system and
- * give the same gradient; they differ in iteration count. */
+/* Preconditioner of the adjoint solve behind cps_simple_gradient_dp
+ * (no reference counterpart).  All forms solve the same system and give the
+ * same gradient; they differ in iteration count and cost per iteration:
+ *   CPS_ADJ_PRECOND_SIMPLE    momentum by 1/aP0, Schur complement by the
+ *                             pressure-correction operator;
+ *   CPS_ADJ_PRECOND_LSC_LINE  momentum by pentadiagonal x-line solves plus
+ *                             line_sweeps damped x-line Jacobi sweeps, Schur
+ *                             complement by the least-squares commutator;
+ *   CPS_ADJ_PRECOND_LSC_ADI   as LSC_LINE with each sweep an x-line pass then
+ *                             a y-line pass (default: line_sweeps 2,
+ *                             omega 0.8).
+ * line_sweeps in [0, 64], omega (sweep damping) in (0, 1]. */
 #define CPS_ADJ_PRECOND_SIMPLE 0
 #define CPS_ADJ_PRECOND_LSC_LINE 1
-int cps_simple_adjoint_options(cps_simple* s, int preconditioner, int line_sweeps);
+#define CPS_ADJ_PRECOND_LSC_ADI 2
+int cps_simple_adjoint_options(cps_simple* s, int preconditioner, int line_sweeps, double omega);
 
 /* ---------------------------------------------------------------------------
  * Batches of fields (no reference counterpart: the reference solves one

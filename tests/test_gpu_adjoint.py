@@ -218,7 +218,7 @@ def test_torch_autograd_pressure_drop(pkg):
 
 @pytest.mark.parametrize("scheme", [0, 1])
 def test_preconditioners_agree_and_lsc_line_needs_fewer_iterations(pkg, scheme):
-    """The x-line / least-squares-commutator preconditioner and the SIMPLE
+    """The line / least-squares-commutator preconditioners and the SIMPLE
     block preconditioner solve the same adjoint system: same gradient to the
     solve tolerance, the former in fewer FGMRES iterations."""
     cfg, g, inlet = pkg.configs.density_field(nx=96, ny=48, seed=2, sigma=9.0)
@@ -228,7 +228,7 @@ def test_preconditioners_agree_and_lsc_line_needs_fewer_iterations(pkg, scheme):
     s = pkg.SIMPLE(cfg, g, inlet)
     s.step(4000)
     out = {}
-    for kind, sweeps in (("simple", 0), ("lsc_line", 4), ("lsc_line", 0)):
+    for kind, sweeps in (("simple", 0), ("lsc_line", 4), ("lsc_line", 0), ("lsc_adi", 2)):
         s.adjoint_options(kind, sweeps)
         dp, grad = s.gradient_dp(rtol=1e-9, max_iter=3000)
         print(kind, sweeps, s.last_adjoint, flush=True)
@@ -240,6 +240,9 @@ def test_preconditioners_agree_and_lsc_line_needs_fewer_iterations(pkg, scheme):
         err = np.abs(grad - ref[1]).max() / np.abs(ref[1]).max()
         assert err < 1e-6, (key, err)
     assert out[("lsc_line", 4)][2] < out[("simple", 0)][2]
+    assert out[("lsc_adi", 2)][2] < out[("simple", 0)][2]
     with pytest.raises(pkg._capi.CpsError):
         s.adjoint_options("lsc_line", -1)
+    with pytest.raises(pkg._capi.CpsError):
+        s.adjoint_options("lsc_adi", 2, omega=1.5)
     s.close()
