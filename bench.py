@@ -25,7 +25,7 @@ of mixed sizes stepped by per-stream hipGraph launches, and the
 
 --copy-calibration runs only a device-to-device copy of 1 GiB (4 times),
 the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters
-to bytes.
+to bytes; --leg NAME runs one leg alone (for profiling it).
 
 usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20]
 """
@@ -235,6 +235,8 @@ def main():
     ap.add_argument("--no-thermal", action="store_true", help="skip the 3-D thermal leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-field batch leg")
     ap.add_argument("--copy-calibration", action="store_true", help="only copy 1 GiB on the device (4x) and exit")
+    ap.add_argument("--leg", choices=["grads", "grads_batch", "thermal", "batch"],
+                    help="run only this leg and print its results (for profiling)")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -262,6 +264,11 @@ def main():
     import __graft_entry__ as entry
 
     pkg = entry.load_package()
+    if args.leg:
+        legs = {"grads": grads_leg, "grads_batch": grads_batch_leg, "thermal": thermal_leg, "batch": batch_leg}
+        res = legs[args.leg](pkg)
+        print(json.dumps({"leg": args.leg, **res}, default=str), flush=True)
+        return
     nx, ny = args.nx, args.ny
     mixed = run_form(pkg, nx, ny, 32, args.steps, args.warmup, kernel="mg_smooth0")
     fp64 = None if args.no_fp64 else run_form(pkg, nx, ny, 64, args.steps, args.warmup)

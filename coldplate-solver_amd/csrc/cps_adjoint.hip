@@ -505,8 +505,6 @@ __global__ __launch_bounds__(BS) void k_pre4(View V, int n, const double* dinv, 
 // sweeps alone diverge where the flow turns across the lines (the v DOFs of
 // an inlet bend carry an eigenvalue -0.53 +- 1.0i of I - Dl^-1 A), which
 // damping and alternating x / y lines cure.
-constexpr int kLineChunk = 16;
-
 // Lines of one direction: Y = false rows (x-lines), Y = true columns.
 template <bool Y>
 __device__ inline int n_lines(const Geo& g) {
@@ -605,16 +603,46 @@ __global__ __launch_bounds__(64) void k_line_factor(View V, double* band, int* b
   if (fail) atomicOr(bad, 1);
 }
 
-// x = Dl^-1 f (Ax null) or x += omega Dl^-1 (f - Ax) on momentum DOF rows;
-// tmp holds the forward sweep.  Loads go in chunks so each chunk waits once.
-// x-lines: consecutive threads walk rows P apart (each thread streams its own
-// cache lines); y-lines: consecutive threads walk adjacent columns
-// (coalesced).
+// 2 x 2 affine map s -> M s + c of the substitution state s = (x_j, x_j-1)
+struct Aff {
+  double m00, m01, m10, m11, c0, c1;
+};
+__device__ inline Aff aff_id() { return {1.0, 0.0, 0.0, 1.0, 0.0, 0.0}; }
+// one recurrence step x' = beta - p x1 - q x2 applied after map a
+__device__ inline Aff aff_step(const Aff& a, double p, double q, double beta) {
+  return {-p * a.m00 - q * a.m10, -p * a.m01 - q * a.m11, a.m00, a.m01, beta - p * a.c0 - q * a.c1, a.c0};
+}
+// k o j (j first)
+__device__ inline Aff aff_compose(const Aff& k, const Aff& j) {
+  return {k.m00 * j.m00 + k.m01 * j.m10, k.m00 * j.m01 + k.m01 * j.m11,
+          k.m10 * j.m00 + k.m11 * j.m10, k.m10 * j.m01 + k.m11 * j.m11,
+          k.m00 * j.c0 + k.m01 * j.c1 + k.c0, k.m10 * j.c0 + k.m11 * j.c1 + k.c1};
+}
+template <bool Up>
+__device__ inline Aff aff_shfl(const Aff& a, int off) {
+  if (Up)
+    return {__shfl_up(a.m00, off), __shfl_up(a.m01, off), __shfl_up(a.m10, off), __shfl_up(a.m11, off),
+            __shfl_up(a.c0, off), __shfl_up(a.c1, off)};
+  return {__shfl_down(a.m00, off), __shfl_down(a.m01, off), __shfl_down(a.m10, off), __shfl_down(a.m11, off),
+          __shfl_down(a.c0, off), __shfl_down(a.c1, off)};
+}
+
+// x = Dl^-1 f (Ax null) or x += omega Dl^-1 (f - Ax) on momentum DOF rows,
+// one wavefront per line.  Both substitutions of the factored line are
+// second-order linear recurrences; a line is cut into tiles of 64 x kSeg
+// elements, lane k owning kSeg consecutive ones.  Per tile each lane folds
+// its segment into one affine map of the state (x_j, x_j-1), a 6-step
+// shuffle scan composes the maps of the lanes before it (after it for the
+// backward substitution), and each lane then replays its segment from the
+// state entering it.  The forward sweep goes to tmp; every element is written
+// and read back by the same lane.  4 lines per 256-thread block.
+constexpr int kSeg = 8;
 template <bool Y>
-__global__ __launch_bounds__(64) void k_line_solve(View V, const double* band, const double* f, const double* Ax,
-                                                  double omega, double* x, double* tmp) {
-  const int l = blockIdx.x * 64 + threadIdx.x;
-  if (l >= n_lines<Y>(V.g)) return;
+__global__ __launch_bounds__(256) void k_line_solve(View V, const double* band, const double* f, const double* Ax,
+                                                   double omega, double* x, double* tmp) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int l = blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+  if (l >= n_lines<Y>(V.g)) return;  // uniform per wavefront
   size_t base;
   int len, step;
   line_span<Y>(V, l, &base, &len, &step);
@@ -625,47 +653,98 @@ __global__ __launch_bounds__(64) void k_line_solve(View V, const double* band, c
   const double* U0 = band + 2 * S2;
   const double* U1 = band + 3 * S2;
   const double* U2 = band + 4 * S2;
-  double y1 = 0.0, y2 = 0.0;
-  for (int j0 = 0; j0 < len; j0 += kLineChunk) {
-    double bb[kLineChunk], a1[kLineChunk], a2[kLineChunk];
+  constexpr int TILE = kWave * kSeg;
+  // forward substitution y_j = b_j - l1_j y_j-1 - l2_j y_j-2
+  double s0 = 0.0, s1 = 0.0;  // state carried into the tile
+  for (int t0 = 0; t0 < len; t0 += TILE) {
+    const int j0 = t0 + lane * kSeg;
+    double bb[kSeg], p[kSeg], q[kSeg];
 #pragma unroll
-    for (int k = 0; k < kLineChunk; ++k) {
-      const size_t r = base + (size_t)min(j0 + k, len - 1) * step;
-      bb[k] = acc ? f[r] - Ax[r] : f[r];
-      a1[k] = L1[r];
-      a2[k] = L2[r];
+    for (int k = 0; k < kSeg; ++k) {
+      const int j = j0 + k;
+      if (j < len) {
+        const size_t r = base + (size_t)j * step;
+        bb[k] = acc ? f[r] - Ax[r] : f[r];
+        p[k] = L1[r];
+        q[k] = L2[r];
+      } else {
+        bb[k] = p[k] = q[k] = 0.0;
+      }
+    }
+    Aff a = aff_id();
+#pragma unroll
+    for (int k = 0; k < kSeg; ++k)
+      if (j0 + k < len) a = aff_step(a, p[k], q[k], bb[k]);
+    for (int off = 1; off < kWave; off <<= 1) {
+      const Aff o = aff_shfl<true>(a, off);
+      if (lane >= off) a = aff_compose(a, o);
+    }
+    const Aff before = aff_shfl<true>(a, 1);  // lanes 0..k-1
+    double y1 = s0, y2 = s1;
+    if (lane > 0) {
+      y1 = before.m00 * s0 + before.m01 * s1 + before.c0;
+      y2 = before.m10 * s0 + before.m11 * s1 + before.c1;
     }
 #pragma unroll
-    for (int k = 0; k < kLineChunk; ++k) {
-      if (j0 + k < len) {
-        const double yj = bb[k] - a1[k] * y1 - a2[k] * y2;
-        tmp[base + (size_t)(j0 + k) * step] = yj;
+    for (int k = 0; k < kSeg; ++k) {
+      const int j = j0 + k;
+      if (j < len) {
+        const double yj = bb[k] - p[k] * y1 - q[k] * y2;
+        tmp[base + (size_t)j * step] = yj;
         y2 = y1;
         y1 = yj;
       }
     }
+    s0 = __shfl(y1, kWave - 1);
+    s1 = __shfl(y2, kWave - 1);
   }
-  double x1 = 0.0, x2 = 0.0;
-  for (int j1 = len - 1; j1 >= 0; j1 -= kLineChunk) {
-    double yy[kLineChunk], d[kLineChunk], c1[kLineChunk], c2[kLineChunk], xo[kLineChunk];
+  // backward substitution x_j = (y_j - u1_j x_j+1 - u2_j x_j+2) / u0_j
+  s0 = 0.0;
+  s1 = 0.0;
+  const int ntiles = (len + TILE - 1) / TILE;
+  for (int t = ntiles - 1; t >= 0; --t) {
+    const int j0 = t * TILE + lane * kSeg;
+    double yy[kSeg], d[kSeg], p[kSeg], q[kSeg], xo[kSeg];
 #pragma unroll
-    for (int k = 0; k < kLineChunk; ++k) {
-      const size_t r = base + (size_t)max(j1 - k, 0) * step;
-      yy[k] = tmp[r];
-      d[k] = U0[r];
-      c1[k] = U1[r];
-      c2[k] = U2[r];
-      xo[k] = acc ? x[r] : 0.0;
+    for (int k = 0; k < kSeg; ++k) {
+      const int j = j0 + k;
+      if (j < len) {
+        const size_t r = base + (size_t)j * step;
+        yy[k] = tmp[r];
+        d[k] = U0[r];
+        p[k] = U1[r] * d[k];
+        q[k] = U2[r] * d[k];
+        xo[k] = acc ? x[r] : 0.0;
+      } else {
+        yy[k] = d[k] = p[k] = q[k] = xo[k] = 0.0;
+      }
+    }
+    Aff a = aff_id();
+#pragma unroll
+    for (int k = kSeg - 1; k >= 0; --k)
+      if (j0 + k < len) a = aff_step(a, p[k], q[k], yy[k] * d[k]);
+    for (int off = 1; off < kWave; off <<= 1) {
+      const Aff o = aff_shfl<false>(a, off);
+      if (lane + off < kWave) a = aff_compose(a, o);
+    }
+    const Aff after = aff_shfl<false>(a, 1);  // lanes k+1..63
+    double x1 = s0, x2 = s1;
+    if (lane < kWave - 1) {
+      x1 = after.m00 * s0 + after.m01 * s1 + after.c0;
+      x2 = after.m10 * s0 + after.m11 * s1 + after.c1;
     }
 #pragma unroll
-    for (int k = 0; k < kLineChunk; ++k) {
-      if (j1 - k >= 0) {
-        const double xj = (yy[k] - c1[k] * x1 - c2[k] * x2) * d[k];
-        x[base + (size_t)(j1 - k) * step] = acc ? xo[k] + omega * xj : xj;
+    for (int k = kSeg - 1; k >= 0; --k) {
+      const int j = j0 + k;
+      if (j < len) {
+        const double xj = yy[k] * d[k] - p[k] * x1 - q[k] * x2;
+        x[base + (size_t)j * step] = acc ? xo[k] + omega * xj : xj;
         x2 = x1;
         x1 = xj;
       }
     }
+    s0 = __shfl(x1, 0);
+    s1 = __shfl(x2, 0);
   }
 }
 
@@ -864,8 +943,9 @@ class FlowAdjoint {
       spmv(x, ta.p);
       Ax = ta.p;
     }
-    if (y) adj::k_line_solve<true><<<ceil_div(nlines(true), 64), 64, 0, s>>>(V, bandy.p, f, Ax, omega, x, lt.p);
-    else adj::k_line_solve<false><<<ceil_div(nlines(false), 64), 64, 0, s>>>(V, bandx.p, f, Ax, omega, x, lt.p);
+    const int lpb = 256 / kWave;  // lines per block (one wavefront each)
+    if (y) adj::k_line_solve<true><<<ceil_div(nlines(true), lpb), 256, 0, s>>>(V, bandy.p, f, Ax, omega, x, lt.p);
+    else adj::k_line_solve<false><<<ceil_div(nlines(false), lpb), 256, 0, s>>>(V, bandx.p, f, Ax, omega, x, lt.p);
   }
 
   // x = A~^-1 f_m by line Jacobi (x: momentum DOF rows, 0 elsewhere): an
