@@ -69,7 +69,7 @@ def lib():
         L.cps_simple_gradient_dp.argtypes = [vp, C.c_double, C.c_int, dp, dp, P(T.SolveStats)]
         L.cps_simple_state_residual.argtypes = [vp, dp]
         L.cps_simple_jacobian_apply.argtypes = [vp, C.c_int, dp, dp]
-        L.cps_simple_adjoint_options.argtypes = [vp, C.c_int, C.c_int, C.c_double]
+        L.cps_simple_adjoint_options.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int]
         L.cps_batch_create.argtypes = [vp, C.c_int, P(T.SimpleConfig), P(dp), P(dp), P(vp)]
         L.cps_batch_destroy.argtypes = [vp]
         L.cps_batch_destroy.restype = None

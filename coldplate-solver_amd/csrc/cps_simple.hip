@@ -1299,7 +1299,8 @@ int cps_simple_jacobian_apply(cps_simple* s, int transpose, const double* x, dou
   });
 }
 
-int cps_simple_adjoint_options(cps_simple* s, int preconditioner, int line_sweeps, double omega) {
+int cps_simple_adjoint_options(cps_simple* s, int preconditioner, int line_sweeps, double omega,
+                               int schur_iters) {
   return guard([&] {
     CPS_CHECK(s, CPS_ERR_ARG, "null argument");
     CPS_CHECK(preconditioner == CPS_ADJ_PRECOND_SIMPLE || preconditioner == CPS_ADJ_PRECOND_LSC_LINE ||
@@ -1307,7 +1308,8 @@ int cps_simple_adjoint_options(cps_simple* s, int preconditioner, int line_sweep
               CPS_ERR_ARG, "unknown adjoint preconditioner");
     CPS_CHECK(line_sweeps >= 0 && line_sweeps <= 64, CPS_ERR_ARG, "line_sweeps must be in [0, 64]");
     CPS_CHECK(omega > 0.0 && omega <= 1.0, CPS_ERR_ARG, "omega must be in (0, 1]");
-    cps::flow_adjoint_options(s->s->adj(), preconditioner, line_sweeps, omega);
+    CPS_CHECK(schur_iters >= 1 && schur_iters <= 1000, CPS_ERR_ARG, "schur_iters must be in [1, 1000]");
+    cps::flow_adjoint_options(s->s->adj(), preconditioner, line_sweeps, omega, schur_iters);
   });
 }
 

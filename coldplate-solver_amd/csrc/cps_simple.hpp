@@ -129,7 +129,7 @@ void flow_adjoint_run(FlowAdjoint* a, const SimpleView& sv, double rtol, int max
                       cps_solve_stats* stats);
 void flow_adjoint_residual(FlowAdjoint* a, const SimpleView& sv, double* out3);
 void flow_adjoint_apply(FlowAdjoint* a, const SimpleView& sv, int transpose, const double* x, double* y);
-void flow_adjoint_options(FlowAdjoint* a, int kind, int sweeps, double omega);
+void flow_adjoint_options(FlowAdjoint* a, int kind, int sweeps, double omega, int schur_iters);
 
 }  // namespace cps
 

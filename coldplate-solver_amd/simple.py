@@ -102,14 +102,16 @@ class SIMPLE:
         self.last_adjoint = st.as_dict()
         return dp.value, grad
 
-    def adjoint_options(self, preconditioner="lsc_adi", line_sweeps=2, omega=0.8):
+    def adjoint_options(self, preconditioner="lsc_adi", line_sweeps=2, omega=0.8, schur_iters=4):
         """Preconditioner of the adjoint solve: "lsc_adi" (default: x- and
         y-line momentum solves with damped alternating sweeps, least-squares-
         commutator Schur complement), "lsc_line" (x-lines only) or "simple"
-        (1/aP0 momentum, pressure-correction Schur complement)."""
+        (1/aP0 momentum, pressure-correction Schur complement); schur_iters
+        caps the multigrid-PCG iterations of each Schur-complement solve."""
         kinds = {"simple": T.CPS_ADJ_PRECOND_SIMPLE, "lsc_line": T.CPS_ADJ_PRECOND_LSC_LINE,
                  "lsc_adi": T.CPS_ADJ_PRECOND_LSC_ADI}
-        _capi.check(self.L.cps_simple_adjoint_options(self.h, kinds[preconditioner], int(line_sweeps), float(omega)))
+        _capi.check(self.L.cps_simple_adjoint_options(self.h, kinds[preconditioner], int(line_sweeps), float(omega),
+                                                      int(schur_iters)))
 
     def state_residual(self):
         """max |R| of the U-momentum, V-momentum and continuity equations."""

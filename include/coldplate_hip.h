@@ -185,12 +185,15 @@ int cps_simple_jacobian_apply(cps_simple* s, int transpose, const double* x, dou
  *                             complement by the least-squares commutator;
  *   CPS_ADJ_PRECOND_LSC_ADI   as LSC_LINE with each sweep an x-line pass then
  *                             a y-line pass (default: line_sweeps 2,
- *                             omega 0.8).
- * line_sweeps in [0, 64], omega (sweep damping) in (0, 1]. */
+ *                             omega 0.8, schur_iters 4).
+ * line_sweeps in [0, 64], omega (sweep damping) in (0, 1]; schur_iters
+ * (multigrid-PCG iterations per Schur-complement solve, rtol 1e-3) in
+ * [1, 1000]. */
 #define CPS_ADJ_PRECOND_SIMPLE 0
 #define CPS_ADJ_PRECOND_LSC_LINE 1
 #define CPS_ADJ_PRECOND_LSC_ADI 2
-int cps_simple_adjoint_options(cps_simple* s, int preconditioner, int line_sweeps, double omega);
+int cps_simple_adjoint_options(cps_simple* s, int preconditioner, int line_sweeps, double omega,
+                               int schur_iters);
 
 /* ---------------------------------------------------------------------------
  * Batches of fields (no reference counterpart: the reference solves one
