@@ -613,6 +613,76 @@ __global__ __launch_bounds__(NT) void k_pcg_end(Geo g, int K, double tol, IterSt
 }
 
 // ---------------------------------------------------------------------------
+// Red-black SOR pressure solve (pressureSolverType 0, pressure_solver.cpp:
+// 819-919): cells of one colour ((i + j) % 2) over i = 1..M-1, j = 1..N-1
+// read only the other colour, so a colour is one launch.  Column N-1 and
+// cells with aP < 1e-20 are pinned to 0; p' is clamped to +-1e4; the stop is
+// max |change| < pTol over both colours of an iteration.  Each launch writes
+// its per-block max |change| into the slots of parity it & 1; colour 0 of
+// iteration it first tests the slots of it - 1 (the other parity, so no
+// block reads a slot this launch writes).  The right side b is the mass
+// residual k_prhs left in q.r; pStar starts at 0 (cleared by k_prhs).
+__global__ __launch_bounds__(NT) void k_sor(Geo g, Fields f, const double* b, int it, int colour, double omega,
+                                           double tol, IterState* st, double* part) {
+  __shared__ double sh[NT / kWave + 1];
+  __shared__ int shi;
+  if (block_read(&st->pcg_stop, &shi)) return;
+  if (colour == 0 && it > 0) {
+    const double* pv = part + (size_t)(SLOT_SOR + 2 * ((it - 1) & 1)) * g.NB;
+    const double mc = fmax(parts_max<NT>(pv, g.NB, sh), parts_max<NT>(pv + g.NB, g.NB, sh));
+    if (mc < tol) {
+      if (block_id() == 0 && thread_id() == 0) {
+        st->pcg_stop = 1;
+        st->pcg_iters = it;
+      }
+      return;
+    }
+  }
+  if (colour == 0 && block_id() == 0 && thread_id() == 0) st->pcg_iters = it + 1;
+  const int M = g.M, N = g.N, P = g.P;
+  double* ps = f.ps;
+  double mx = 0.0;
+  FIELD_LOOP(g, i, j) {
+    if (!(i >= 1 && i <= M - 1 && j >= 1 && j <= N - 1) || ((i + j) & 1) != colour) continue;
+    const size_t k = at(g, i, j);
+    if (j == N - 1) {
+      ps[k] = 0.0;
+      continue;
+    }
+    const PCoef c = p_coef(g, f.dE, f.dN, i, j);
+    const double aP = c.aE + c.aW + c.aN + c.aS;
+    if (aP < kTiny20) {
+      ps[k] = 0.0;
+      continue;
+    }
+    const double pE = ps[k + 1];
+    const double pW = (j > 1) ? ps[k - 1] : 0.0;
+    const double pN = (i < M - 1) ? ps[k + P] : 0.0;
+    const double pS = (i > 1) ? ps[k - P] : 0.0;
+    const double pGS = (c.aE * pE + c.aW * pW + c.aN * pN + c.aS * pS + b[k]) / aP;
+    const double pOld = ps[k];
+    const double pNew = fmax(-1e4, fmin(1e4, pOld + omega * (pGS - pOld)));
+    mx = fmax(mx, fabs(pNew - pOld));
+    ps[k] = pNew;
+  }
+  mx = block_max<NT>(mx, sh);
+  if (thread_id() == 0) part[(size_t)(SLOT_SOR + 2 * (it & 1) + colour) * g.NB + block_id()] = mx;
+}
+
+// After K launched SOR iterations: stop test of iteration K-1 (one block).
+__global__ __launch_bounds__(NT) void k_sor_end(Geo g, int K, double tol, IterState* st, const double* part) {
+  __shared__ double sh[NT / kWave + 1];
+  __shared__ int shi;
+  if (block_read(&st->pcg_stop, &shi)) return;
+  const double* pv = part + (size_t)(SLOT_SOR + 2 * ((K - 1) & 1)) * g.NB;
+  const double mc = fmax(parts_max<NT>(pv, g.NB, sh), parts_max<NT>(pv + g.NB, g.NB, sh));
+  if (thread_id() == 0) {
+    if (mc < tol) st->pcg_stop = 1;
+    st->pcg_iters = K;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Corrections (iterations.cpp:556-600) with the BCs of u, v, p fused: the
 // thread that owns an interior value also writes the ghost copied from it.
 __global__ __launch_bounds__(NT) void k_correct(Geo g, Fields f) {
@@ -709,9 +779,10 @@ class SimpleSolver {
     CPS_CHECK(cfg.uv_alpha > 0 && cfg.p_alpha > 0, CPS_ERR_ARG, "relaxation factors must be > 0");
     CPS_CHECK(cfg.precision == CPS_PREC_FP64 || cfg.precision == CPS_PREC_MIXED, CPS_ERR_ARG,
               "precision must be 64 or 32");
-    CPS_CHECK(cfg.pressure_solver == CPS_PRESSURE_PCG || cfg.pressure_solver == CPS_PRESSURE_GMG, CPS_ERR_ARG,
-              "pressure_solver must be 1 (PCG) or 5 (GMG)");
-    CPS_CHECK(!(cfg.precision == CPS_PREC_MIXED && cfg.pressure_solver == CPS_PRESSURE_PCG), CPS_ERR_ARG,
+    CPS_CHECK(cfg.pressure_solver == CPS_PRESSURE_SOR || cfg.pressure_solver == CPS_PRESSURE_PCG ||
+                  cfg.pressure_solver == CPS_PRESSURE_GMG,
+              CPS_ERR_ARG, "pressure_solver must be 0 (SOR), 1 (PCG) or 5 (GMG)");
+    CPS_CHECK(!(cfg.precision == CPS_PREC_MIXED && cfg.pressure_solver != CPS_PRESSURE_GMG), CPS_ERR_ARG,
               "mixed precision needs the GMG pressure solver (5)");
     CPS_CHECK(gamma != nullptr, CPS_ERR_ARG, "gamma is required");
     mode_fp64 = cfg.precision == CPS_PREC_FP64;
@@ -899,6 +970,29 @@ class SimpleSolver {
     }
   }
 
+  // red-black SOR with the reference's automatic omega (pressure_solver.cpp:
+  // 820-826): 2 / (1 + sqrt(1 - cos(pi / max(M, N))^2)), clamped to [1, 1.95]
+  void pressure_sor() {
+    hipStream_t s = ctx->stream;
+    const double sr = std::cos(M_PI / (double)std::max(g.M, g.N));
+    const double omega = std::max(1.0, std::min(1.95, 2.0 / (1.0 + std::sqrt(1.0 - sr * sr))));
+    const int maxit = cfg.max_pressure_iter;
+    const int chunk = maxit > kNoSyncCap ? 16 : std::max(1, maxit);
+    const Fields f = fields();
+    const PcgBufs q = pbufs();
+    int k = 0;
+    while (k < maxit) {
+      const int k1 = std::min(maxit, k + chunk);
+      for (; k < k1; ++k)
+        for (int colour = 0; colour < 2; ++colour)
+          k_sor<<<grid, block, 0, s>>>(g, f, q.r, k, colour, omega, cfg.p_tol, st.p, part.p);
+      k_sor_end<<<1, block, 0, s>>>(g, k, cfg.p_tol, st.p, part.p);
+      CPS_HIP(hipGetLastError());
+      if (k >= maxit) break;
+      if (read_int(&st.p->pcg_stop)) break;
+    }
+  }
+
   void mg_create();
   void pressure_mg();
 
@@ -910,6 +1004,7 @@ class SimpleSolver {
     k_stats<<<1, block, 0, s>>>(g, st.p, part.p);
     CPS_HIP(hipGetLastError());
     if (cfg.pressure_solver == CPS_PRESSURE_PCG) pressure_pcg();
+    else if (cfg.pressure_solver == CPS_PRESSURE_SOR) pressure_sor();
     else pressure_mg();
     k_correct<<<grid, block, 0, s>>>(g, f);
     CPS_HIP(hipGetLastError());

@@ -41,7 +41,8 @@ enum PartSlot {
   SLOT_FIN = 4,     // 4 slots: sum du^2, sum dv^2, max |du|, max |dv|
   SLOT_RHS = 8,     // 4 slots: max |b|, sum b^2 (j <= N-1), r.z, ||b||^2 (DOFs)
   SLOT_PCG = 12,    // 3 slots: pAp, ||r||^2, r.z
-  SLOT_COUNT = 16
+  SLOT_SOR = 16,    // 4 slots: max |change| per colour, by iteration parity
+  SLOT_COUNT = 20
 };
 
 // Geometry + physics scalars passed to kernels by value.

@@ -75,7 +75,7 @@ int cps_momentum_jacobi_csr(cps_ctx* ctx, int n, const int* row_ptr, const int* 
  * Structured SIMPLE solver (device-resident fields).  Knobs carry the
  * reference names of SIMPLE.h; defaults come from cps_simple_config_default.
  * ------------------------------------------------------------------------- */
-#define CPS_PRESSURE_SOR 0    /* pressureSolverType 0 (oracle only) */
+#define CPS_PRESSURE_SOR 0    /* pressureSolverType 0: red-black SOR, max-change stop */
 #define CPS_PRESSURE_PCG 1    /* pressureSolverType 1: Jacobi-PCG, absolute tol */
 #define CPS_PRESSURE_GMG 5    /* new type: geometric-multigrid PCG on the device */
 

@@ -198,3 +198,35 @@ def test_gmg_sou_density_field_dp(pkg, orc):
         assert gpu.full_static_drop() == pytest.approx(ref.dp_static(), rel=1e-6)
         for name in ("u", "v", "p"):
             assert rel_l2(gpu.get(name), ref.get(name)) <= 1e-4, name
+
+
+@pytest.mark.parametrize("seed,nx,ny,scheme,maxit,ptol", [(1, 40, 20, 0, 30, 1e-6), (2, 64, 33, 1, 30, 1e-6),
+                                                            (3, 50, 24, 1, 400, 1e-9)])
+def test_sor_pressure_matches_oracle(pkg, orc, seed, nx, ny, scheme, maxit, ptol):
+    """pressureSolverType 0 (red-black SOR, pressure_solver.cpp:819-919) on the
+    device against the oracle's FP64 restatement: same sweep order per colour,
+    same automatic omega, clamp and max-change stop, so fields agree to
+    round-off and the SOR iteration counts match."""
+    cfg, gamma, inlet = pkg.configs.density_field(nx=nx, ny=ny, seed=seed, sigma=3.0)
+    cfg.convection_scheme = scheme
+    cfg.pressure_solver = pkg._types.CPS_PRESSURE_SOR
+    cfg.max_pressure_iter = maxit
+    cfg.p_tol = ptol
+    gpu, ref = _pair(pkg, orc, cfg, gamma, inlet)
+    for k in range(3):
+        sg, sr = gpu.step(2), ref.step(2)
+        print(k, "p_iters", sg["p_iters"], sr["p_iters"], flush=True)
+        assert abs(sg["p_iters"] - sr["p_iters"]) <= 1
+        for key in ("error", "resid_u", "resid_v", "resid_mass"):
+            assert sg[key] == pytest.approx(sr[key], rel=1e-9, abs=1e-300)
+        for name in ("u", "v", "p", "pStar"):
+            assert rel_l2(gpu.get(name), ref.get(name)) < 1e-10, name
+    assert np.all(gpu.get("pStar")[1:-1, cfg.nx] == 0.0)  # column N-1 pinned
+
+
+def test_sor_rejected_in_mixed_precision(pkg):
+    cfg, gamma, inlet = pkg.configs.density_field(nx=16, ny=8, seed=0)
+    cfg.pressure_solver = pkg._types.CPS_PRESSURE_SOR
+    cfg.precision = pkg._types.CPS_PREC_MIXED
+    with pytest.raises(pkg._capi.CpsError):
+        pkg.SIMPLE(cfg, gamma, inlet)

@@ -339,3 +339,95 @@ def test_boundary_patch_known_answer(pkg, orc, T):
     assert inlet["area"] == pytest.approx(2 * cfg.hy)
     outlet = o.patch(False)
     assert outlet["avg_static"] == 0.0
+
+
+# ---------------------------------------------------------------- a15 ------
+def _sor_state(pkg, orc, T, maxit, tol, fluid=False):
+    """A small density field (or an all-fluid channel) one SIMPLE step in,
+    with the SOR pressure backend (pressureSolverType 0) capped at maxit
+    iterations."""
+    rng = np.random.default_rng(7)
+    nx, ny = 14, 9
+    g = np.clip(rng.uniform(-0.3, 1.3, (ny, nx)), 0.0, 1.0)
+    g[:, 0] = 1.0
+    if fluid:
+        g[:] = 1.0
+    cfg = small_cfg(pkg, nx=nx, ny=ny, pressure_solver=0, max_pressure_iter=maxit, p_tol=tol)
+    o = orc.OracleSimple(T, cfg, g)
+    st = o.step(1)
+    sysd = o.assemble("p")  # the system of that step (uStar, vStar, dE, dN are unchanged by the corrections)
+    return cfg, o, st, csr(sysd), sysd["rhs"], sysd["ij"]
+
+
+def _red_black_sor(A, b, ij, omega, x, sweeps=1):
+    """Textbook SOR on A x = b with red-black ordering by (i + j) % 2:
+    x_k += omega ((b_k - sum_{m != k} A_km x_m) / A_kk - x_k); cells of one
+    colour do not couple in a 5-point operator, so each colour is one Jacobi
+    half-step on the latest values.  Returns x and the last max |change|."""
+    d = A.diagonal()
+    colour = (ij[:, 0] + ij[:, 1]) % 2
+    for _ in range(sweeps):
+        mx = 0.0
+        for c in (0, 1):
+            sel = colour == c
+            gs = (b - (A @ x - d * x)) / d
+            new = x + omega * (gs - x)
+            mx = max(mx, float(np.abs(new - x)[sel].max()))
+            x = np.where(sel, new, x)
+    return x, mx
+
+
+def test_oracle_sor_is_red_black_sor_of_the_pressure_matrix(pkg, orc, T):
+    """The SOR backend (pressure_solver.cpp:819-919) is red-black SOR on the
+    same pressure matrix as the PCG path (pinned above by the Laplacian and
+    symmetry tests), from p' = 0, with omega = 2 / (1 + sqrt(1 - cos^2(pi /
+    max(M, N)))) clamped to [1, 1.95]; column N-1 stays 0."""
+    cfg, o, st, A, b, ij = _sor_state(pkg, orc, T, maxit=1, tol=0.0)
+    M, N = cfg.ny + 1, cfg.nx + 1
+    sr = np.cos(np.pi / max(M, N))
+    omega = min(1.95, max(1.0, 2.0 / (1.0 + np.sqrt(1.0 - sr * sr))))
+    x, _ = _red_black_sor(A, b, ij, omega, np.zeros(len(b)))
+    ps = o.get("pStar")
+    np.testing.assert_allclose(ps[ij[:, 0], ij[:, 1]], x, rtol=1e-12, atol=1e-12 * np.abs(x).max())
+    assert st["p_iters"] == 1
+    assert np.all(ps[1:M, N - 1] == 0.0)
+    # a wrong omega (plain Gauss-Seidel) would not match
+    x1, _ = _red_black_sor(A, b, ij, 1.0, np.zeros(len(b)))
+    assert np.abs(x1 - x).max() > 1e-3 * np.abs(x).max()
+
+
+def test_oracle_sor_converges_to_the_direct_solve_and_counts_sweeps(pkg, orc, T):
+    """All-fluid channel (p' well inside the +-1e4 clamp): the converged
+    sweep is the direct solve, and the count is that of the first sweep whose
+    max |change| is below pTol."""
+    import scipy.sparse.linalg as sla
+
+    cfg, o, st, A, b, ij = _sor_state(pkg, orc, T, maxit=20000, tol=1e-13, fluid=True)
+    x = sla.spsolve(A.tocsc(), b)
+    assert np.abs(x).max() < 1e3
+    ps = o.get("pStar")
+    assert np.abs(ps[ij[:, 0], ij[:, 1]] - x).max() < 1e-9 * np.abs(x).max()
+    tol = 1e-4 * np.abs(x).max()
+    cfg, o, st, A, b, ij = _sor_state(pkg, orc, T, maxit=20000, tol=tol, fluid=True)
+    M, N = cfg.ny + 1, cfg.nx + 1
+    sr = np.cos(np.pi / max(M, N))
+    omega = min(1.95, max(1.0, 2.0 / (1.0 + np.sqrt(1.0 - sr * sr))))
+    xx, k = np.zeros(len(b)), 0
+    while True:
+        xx, mx = _red_black_sor(A, b, ij, omega, xx)
+        k += 1
+        if mx < tol:
+            break
+    assert abs(st["p_iters"] - k) <= 1, (st["p_iters"], k)
+
+
+def test_oracle_sor_clamps_pressure_correction(pkg, orc, T):
+    """Brinkman field whose direct solve exceeds 1e4: the SOR iterate is
+    clamped to +-1e4 (pressure_solver.cpp:853) and the sweep stalls there."""
+    import scipy.sparse.linalg as sla
+
+    cfg, o, st, A, b, ij = _sor_state(pkg, orc, T, maxit=3000, tol=1e-12)
+    x = sla.spsolve(A.tocsc(), b)
+    assert np.abs(x).max() > 1e5
+    ps = o.get("pStar")[ij[:, 0], ij[:, 1]]
+    assert np.abs(ps).max() == 1e4
