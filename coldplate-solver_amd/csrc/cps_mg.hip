@@ -30,7 +30,19 @@ namespace {
 constexpr int MBX = 64, MBY = 4, MNT = MBX * MBY;
 constexpr double kOmega = 0.8;
 constexpr double kCoarseScale = 0.5;
-constexpr int kCoarseThreads = 1024;
+#ifndef CPS_COARSE_THREADS
+#define CPS_COARSE_THREADS 1024
+#endif
+constexpr int kCoarseThreads = CPS_COARSE_THREADS;
+// staging limits of the coarse solver: all its levels together, and its top
+constexpr int kCoarseTotalMax = 3072, kCoarseTopMax = 2048;
+static_assert(kCoarseTotalMax % kCoarseThreads == 0 && kCoarseTopMax % kCoarseThreads == 0, "coarse staging");
+#ifndef CPS_WAVE_CELLS
+#define CPS_WAVE_CELLS 64
+#endif
+// coarse levels of at most this many cells are smoothed by one wavefront
+// (no workgroup barriers)
+constexpr int kWaveCells = CPS_WAVE_CELLS;
 constexpr double kTiny20 = (double)1e-20f;
 
 template <class T>
@@ -311,7 +323,7 @@ struct CoarseDesc {
   int nlev;
   int nx[16], ny[16], off[16];  // compact LDS layout, pitch = nx
   int total;
-  int wl;  // first level of at most 64 cells: it and the coarser ones run in wave 0
+  int wl;  // first level of at most kWaveCells cells: it and the coarser ones run in wave 0
 };
 
 #define CELLS(l)                                              \
@@ -366,9 +378,8 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, C
 // pre-smoothing phase (both sweeps from zero fused: x1 = x0 + w (b - A x0)/d
 // with x0 = w b/d formed on the fly), one restriction phase, and two
 // post-smoothing phases (the first with the prolongation fused).  Levels of
-// more than 64 cells separate their phases with workgroup barriers; the
-// smaller ones run in wave 0 alone, one cell per lane, with wavefront-scope
-// ordering only.  The coarsest level (one cell) is solved exactly in FP64.
+// more than kWaveCells cells separate their phases with workgroup barriers;
+// the smaller ones run in wave 0 alone with wavefront-scope ordering only.  The coarsest level (one cell) is solved exactly in FP64.
 // The top level's correction goes to top.y.
 template <class T>
 __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseDesc d, const T* cf, const int* stop) {
@@ -382,8 +393,8 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
   T* si = sn + d.total;  // 1 / diagonal
   const int t = threadIdx.x, tx = t & 63, ty = t >> 6;
   {  // every global load of the thread is issued before the LDS stores
-    constexpr int NQ = 3;   // d.total <= 3 * kCoarseThreads
-    constexpr int NB0 = 2;  // top level <= 2 * kCoarseThreads cells
+    constexpr int NQ = kCoarseTotalMax / kCoarseThreads;  // d.total <= kCoarseTotalMax
+    constexpr int NB0 = kCoarseTopMax / kCoarseThreads;   // top level <= kCoarseTopMax cells
     T ve[NQ], vn[NQ], vi[NQ], vb[NB0];
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
@@ -466,9 +477,11 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
     CELLS(l + 1) { restrict_cell(l, r, c); }
     __syncthreads();
   }
-  // Levels of at most 64 cells: one cell per lane of wave 0.  A wavefront's
-  // LDS accesses complete in order, so wavefront-scope fences (which keep
-  // the compiler from moving LDS accesses across them) order the phases.
+  // Levels of at most kWaveCells cells run in wave 0 alone, each lane taking
+  // cells t, t + 64, ...  Within a phase the arrays read and written are
+  // disjoint; a wavefront's LDS accesses complete in order, so
+  // wavefront-scope fences (which keep the compiler from moving LDS accesses
+  // across them) order the phases.
   if (t < 64) {
     auto wave_sync = [] {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -477,10 +490,10 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
     };
     for (int l = wl; l < last; ++l) {
       const int nx = d.nx[l], n = nx * d.ny[l];
-      if (t < n) down_sweep(l, t / nx, t - (t / nx) * nx);
+      for (int q = t; q < n; q += 64) down_sweep(l, q / nx, q - (q / nx) * nx);
       wave_sync();
       const int cx = d.nx[l + 1], cn = cx * d.ny[l + 1];
-      if (t < cn) restrict_cell(l, t / cx, t - (t / cx) * cx);
+      for (int q = t; q < cn; q += 64) restrict_cell(l, q / cx, q - (q / cx) * cx);
       wave_sync();
     }
     if (t == 0) {  // one cell: exact, in FP64
@@ -490,9 +503,9 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
     wave_sync();
     for (int l = last - 1; l >= wl; --l) {
       const int nx = d.nx[l], n = nx * d.ny[l];
-      if (t < n) up_sweep1(l, t / nx, t - (t / nx) * nx);
+      for (int q = t; q < n; q += 64) up_sweep1(l, q / nx, q - (q / nx) * nx);
       wave_sync();
-      if (t < n) up_sweep2(l, t / nx, t - (t / nx) * nx);
+      for (int q = t; q < n; q += 64) up_sweep2(l, q / nx, q - (q / nx) * nx);
       wave_sync();
     }
   }
@@ -791,11 +804,11 @@ struct MGImpl {
     cd.total = off;
     cd.wl = n - 1;
     for (int l = 0; l < n; ++l)
-      if (cd.nx[l] * cd.ny[l] <= 64) {
+      if (cd.nx[l] * cd.ny[l] <= kWaveCells) {
         cd.wl = l;
         break;
       }
-    CPS_CHECK(off <= 3 * kCoarseThreads && lv.back()->nx * lv.back()->ny <= 2 * kCoarseThreads, CPS_ERR_STATE,
+    CPS_CHECK(off <= kCoarseTotalMax && lv.back()->nx * lv.back()->ny <= kCoarseTopMax, CPS_ERR_STATE,
               "coarse multigrid plan exceeds the coarse solver's staging");
     coarseSmem = (size_t)6 * off * sizeof(T);
     coarseSetupSmem = (size_t)2 * off * sizeof(T);
