@@ -12,9 +12,10 @@ at most 6 PCG iterations, at most 8 Jacobi sweeps): FP32 inner iterations inside
 state / residuals / refinement ("value"), and plain FP64.  The pressure drop
 of the two after warmup + steps iterations is compared.
 
-The kernel-level roofline line times the fine-grid multigrid smoother (the
-pressure stencil) with HIP events on the solver stream over the timed
-region.  The CPU baseline is the oracle's FP32 restatement of the
+The kernel-level roofline line times the fine-grid multigrid down kernel
+(the previous PCG update fused with the pressure smoother, residual and
+restriction) with HIP events carried by one launch in 8 on the solver
+stream, over the timed region.  The CPU baseline is the oracle's FP32 restatement of the
 reference CPU path (Jacobi-PCG pressure, 30 iterations, pTol 1e-6; Jacobi
 momentum) on a bounded sample of the same workload.
 
@@ -279,7 +280,8 @@ def main():
         gbs = kt["bytes_per_launch"] / (kt["avg_us"] * 1e-6) / 1e9
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
-                "kernel": "k_mg_down<float> on the fine grid (2 pressure smoother sweeps + residual + restriction)",
+                "kernel": "k_mg_down<float> on the fine grid with the PCG update fused (x += alpha p, "
+                          "r -= alpha Ap, 2 pressure smoother sweeps, residual, restriction)",
                 "avg_us": round(kt["avg_us"], 3), "launches": kt["launches"],
                 "bytes_per_launch": kt["bytes_per_launch"]}
 

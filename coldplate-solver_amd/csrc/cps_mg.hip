@@ -144,6 +144,24 @@ struct PcgTest {
   double abs_tol, rtol;  // it == 0: the target max(abs_tol, rtol ||r0||) is set here
 };
 
+// PCG update of the previous iteration carried by the fine-level down kernel
+// of the next V-cycle: alpha = rz / pAp, x += alpha p, r_new = r - alpha Ap.
+// r_new (tile + halo, recomputed by every block from r and Ap) is the
+// V-cycle's right side; the owner block also stores x and r_new (into the
+// other residual buffer: neighbours still read r for their halos) and the
+// partial ||r_new||^2 of the stop test.
+template <class T>
+struct XrFuse {
+  const T* rold;  // nullptr: no fused update
+  const T* p;
+  const T* Ap;
+  T* xs;
+  const double* pap;  // pAp partials of iteration it - 1
+  double* err;        // ||r_new||^2 partials
+  int nb, it;         // it: this V-cycle; the update is iteration it - 1's
+  PcgScal* ps;
+};
+
 // One thread per tile cell: blocks of TX x TY = 64 x 16 threads.
 constexpr int TX = 64, TY = 16, HL = 3, RW = TX + 2 * HL, RH = TY + 2 * HL, RN = RW * RH;
 constexpr int FT = TX * TY;
@@ -222,14 +240,71 @@ __device__ inline void tile_load(const Lv<T>& L, const Tile<T>& tl, int t, T* sx
 // down: x1 = two Jacobi sweeps from zero (stored in L.x), coarse b = sum of
 // the residuals b - A x1 over each 2x2 aggregate
 template <class T>
-__global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* stop, PcgTest test) {
+__global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* stop, PcgTest test, XrFuse<T> xr) {
   __shared__ T sb[RN], se[RN], sn[RN], s0[RN], s1[RN];
   __shared__ double sh[FT / kWave + 1];
   __shared__ int sflag;
   const int t = threadIdx.x + threadIdx.y * TX;
   const int stop0 = t == 0 ? *stop : 0;  // thread 0's view, published below
-  const double perr = test.part ? parts_load<FT>(test.part, test.nb) : 0.0;
   Tile<T> tl{sb, se, sn, (int)blockIdx.y * TY, (int)blockIdx.x * TX, L.nx, L.ny};
+  if (xr.rold) {
+    const double ppap = parts_load<FT>(xr.pap, xr.nb);
+    const double rz = xr.ps->rzs[(xr.it - 1) & 1];
+    constexpr int NI = (RN + FT - 1) / FT;
+    T vr[NI], va[NI], ve[NI], vn[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int q = t + i * FT;
+      const int lr = q / RW, lc = q - (q / RW) * RW;
+      const int r = tl.gr(lr), c = tl.gc(lc);
+      vr[i] = va[i] = ve[i] = vn[i] = (T)0;
+      if (q < RN && tl.inside(r, c)) {
+        const size_t k = L.at(r, c);
+        vr[i] = xr.rold[k];
+        va[i] = xr.Ap[k];
+        ve[i] = L.ex[k];
+        vn[i] = L.ey[k];
+      }
+    }
+    const int lr0 = HL + threadIdx.y, lc0 = HL + threadIdx.x, q0 = lr0 * RW + lc0;
+    const int r0 = tl.gr(lr0), c0 = tl.gc(lc0);
+    const bool own = tl.inside(r0, c0);
+    const size_t k0 = own ? L.at(r0, c0) : 0;
+    T xo = (T)0, pc = (T)0;
+    if (own) {
+      xo = xr.xs[k0];
+      pc = xr.p[k0];
+    }
+    if (t == 0) sflag = stop0;
+    __syncthreads();
+    if (sflag) return;  // block-uniform
+    const double pap = parts_total<FT>(ppap, sh);
+    if (fabs(pap) < kTiny20) {  // pressure_solver.cpp:161
+      if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) { xr.ps->stop = 1; xr.ps->iters = xr.it - 1; }
+      return;
+    }
+    const T alpha = (T)(rz / pap);
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int q = t + i * FT;
+      if (q < RN) {
+        sb[q] = vr[i] - alpha * va[i];
+        se[q] = ve[i];
+        sn[q] = vn[i];
+      }
+    }
+    __syncthreads();
+    double e = 0.0;
+    if (own) {
+      const T rn = sb[q0];
+      xr.xs[k0] = xo + alpha * pc;
+      L.b[k0] = rn;
+      e = (double)rn * (double)rn;
+    }
+    const double sblk = block_sum<FT>(e, sh);
+    if (t == 0) xr.err[blockIdx.x + blockIdx.y * gridDim.x] = sblk;
+  } else {
+  const double perr = test.part ? parts_load<FT>(test.part, test.nb) : 0.0;
   tile_load<T>(L, tl, t, nullptr, nullptr);
   if (t == 0) sflag = stop0;
   __syncthreads();
@@ -250,6 +325,7 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
       if (b0) test.ps->stop = 1;
       return;
     }
+  }
   }
   __syncthreads();
   // first sweep from a zero guess on the halo-2 ring: x0 = omega b / diag
@@ -382,8 +458,27 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, C
 // the smaller ones run in wave 0 alone with wavefront-scope ordering only.  The coarsest level (one cell) is solved exactly in FP64.
 // The top level's correction goes to top.y.
 template <class T>
-__global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseDesc d, const T* cf, const int* stop) {
+__global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseDesc d, const T* cf, const int* stop,
+                                                              PcgTest test) {
   if (*stop) return;
+  if (test.part) {  // the PCG stop test when the fine down kernel carries the update
+    __shared__ double shr[kCoarseThreads / kWave + 1];
+    const double err = parts_total<kCoarseThreads>(parts_load<kCoarseThreads>(test.part, test.nb), shr);
+    const bool b0 = threadIdx.x == 0;
+    double tol2;
+    if (test.it == 0) {
+      const double tgt = fmax(test.abs_tol, test.rtol * sqrt(err));
+      tol2 = tgt * tgt;
+      if (b0) { test.ps->rr0 = err; test.ps->tol2 = tol2; }
+    } else {
+      tol2 = test.ps->tol2;
+    }
+    if (b0) { test.ps->rr = err; test.ps->iters = test.it; }
+    if (err <= tol2) {
+      if (b0) test.ps->stop = 1;
+      return;
+    }
+  }
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   T* sx = reinterpret_cast<T*>(smem_raw);
   T* sy = sx + d.total;
@@ -757,7 +852,8 @@ struct MGImpl {
   CoarseDesc cd{};
   size_t coarseSmem = 0, coarseSetupSmem = 0;
   DevBuf<T> cf;  // coarse-solver faces and inverse diagonals
-  DevBuf<T> r0, xs, pa, pb, Ap;  // PCG vectors, level-0 layout (xs = solution)
+  DevBuf<T> r0, r1, xs, pa, pb, Ap;  // PCG vectors, level-0 layout (xs = solution; r0 / r1 residual by parity)
+  T* b0cur = nullptr;                   // the level-0 right side of the running V-cycle
   DevBuf<double> part;
   DevBuf<PcgScal> scal;          // [0] inner, [1] outer
   Grid0 G0{};
@@ -774,7 +870,8 @@ struct MGImpl {
     L0->off = g.P + 1;
     const size_t S0 = (size_t)(g.M + 1) * g.P;
     for (auto* b : {&L0->x, &L0->y, &L0->ex, &L0->ey}) b->alloc(S0);
-    for (auto* b : {&r0, &xs, &pa, &pb, &Ap}) b->alloc(S0);
+    for (auto* b : {&r0, &r1, &xs, &pa, &pb, &Ap}) b->alloc(S0);
+    b0cur = r0.p;
     lv.push_back(std::move(L0));
     // at least one grid level: its last post-smoothing sweep carries the
     // fused r.z partials of the PCG
@@ -816,7 +913,7 @@ struct MGImpl {
     CPS_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
     CPS_CHECK(coarseSmem <= (size_t)lds, CPS_ERR_STATE, "coarse multigrid levels exceed LDS");
     cf.alloc((size_t)3 * off);
-    for (auto* b : {&lv[0]->x, &lv[0]->y, &lv[0]->ex, &lv[0]->ey, &r0, &xs, &pa, &pb, &Ap})
+    for (auto* b : {&lv[0]->x, &lv[0]->y, &lv[0]->ex, &lv[0]->ey, &r0, &r1, &xs, &pa, &pb, &Ap})
       CPS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), ctx->stream));
     G0.gx = ceil_div(lv[0]->nx, TX);
     G0.gy = ceil_div(lv[0]->ny, TY);
@@ -830,12 +927,14 @@ struct MGImpl {
   dim3 blk() const { return dim3(MBX, MBY); }
   dim3 grid0() const { return dim3(G0.gx, G0.gy); }
   dim3 blk0() const { return dim3(TX, TY); }
-  // level views; the level-0 rhs of the V-cycle is the PCG residual r0
+  // level views; the level-0 rhs of the V-cycle is the PCG residual
   Lv<T> view(int l) {
     Lv<T> v = lv[l]->view();
-    if (l == 0) v.b = r0.p;
+    if (l == 0) v.b = b0cur;
     return v;
   }
+  T* rbuf(int it) { return (it & 1) ? r1.p : r0.p; }
+  T* pnew(int it) { return (it & 1) ? pa.p : pb.p; }  // p written by iteration it
 
   void setup(const double* dE, const double* dN) {
     hipStream_t s = ctx->stream;
@@ -849,18 +948,25 @@ struct MGImpl {
   // z = V(r0) into level-0 y, with the partial r.z of the last sweep
   dim3 tgrid(int l) const { return dim3(ceil_div(lv[l]->nx, TX), ceil_div(lv[l]->ny, TY)); }
   double abs_tol_ = 0.0, rtol_ = 0.0;  // target of the running PCG solve
-  void vcycle(const int* stop, int it) {
+  // fused: the fine down kernel carries the PCG update of iteration it - 1
+  // and the coarse kernel the stop test; otherwise the fine down kernel tests
+  void vcycle(const int* stop, int it, bool fused) {
     hipStream_t s = ctx->stream;
+    const PcgTest none{nullptr, 0, 0, nullptr, 0.0, 0.0};
+    const PcgTest test{part.p + (size_t)(it == 0 ? S_RR0 : S_ERR) * G0.NB, G0.NB, it, scal.p, abs_tol_, rtol_};
     for (int l = 0; l < nGrid; ++l) {
-      PcgTest test{nullptr, 0, 0, nullptr, 0.0, 0.0};
-      if (l == 0)
-        test = PcgTest{part.p + (size_t)(it == 0 ? S_RR0 : S_ERR) * G0.NB, G0.NB, it, scal.p, abs_tol_, rtol_};
-      if (l == 0 && timer->active(KT_MG_SMOOTH0))
-        timer->ext_launch(k_mg_down<T>, tgrid(l), dim3(TX, TY), 0, s, view(l), view(l + 1), stop, test);
+      XrFuse<T> xr{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr};
+      if (l == 0 && fused && it > 0)
+        xr = XrFuse<T>{rbuf(it - 1), pnew(it - 1), Ap.p, xs.p, part.p + (size_t)S_PAP * G0.NB,
+                       part.p + (size_t)S_ERR * G0.NB, G0.NB, it, scal.p};
+      const PcgTest dt = (l == 0 && !fused) ? test : none;
+      // the timed launches are the fused ones when fusing (the byte count is theirs)
+      if (l == 0 && (!fused || it > 0) && timer->active(KT_MG_SMOOTH0))
+        timer->ext_launch(k_mg_down<T>, tgrid(l), dim3(TX, TY), 0, s, view(l), view(l + 1), stop, dt, xr);
       else
-        k_mg_down<T><<<tgrid(l), dim3(TX, TY), 0, s>>>(view(l), view(l + 1), stop, test);
+        k_mg_down<T><<<tgrid(l), dim3(TX, TY), 0, s>>>(view(l), view(l + 1), stop, dt, xr);
     }
-    k_mg_coarse<T><<<1, kCoarseThreads, coarseSmem, s>>>(view(nGrid), cd, cf.p, stop);
+    k_mg_coarse<T><<<1, kCoarseThreads, coarseSmem, s>>>(view(nGrid), cd, cf.p, stop, fused ? test : none);
     for (int l = nGrid - 1; l >= 0; --l)
       k_mg_up<T><<<tgrid(l), dim3(TX, TY), 0, s>>>(view(l), view(l + 1),
                                                   l == 0 ? part.p + (size_t)S_RZ * G0.NB : nullptr, stop);
@@ -873,28 +979,37 @@ struct MGImpl {
   void pcg(const double* rhs, double abs_tol, double rtol, int maxit, int sync_every, const int* outer_stop) {
     hipStream_t s = ctx->stream;
     const double nP = (double)lv[0]->nx * (double)lv[0]->ny;
+    // Without host polling the PCG update rides on the next V-cycle's fine
+    // down kernel (one launch fewer per iteration); with polling every
+    // iteration ends with its own update kernel, so the host can read the
+    // stop test.
+    const bool fused = sync_every == 0;
     // fine-level down kernel: b, east and north faces read, x1 and the
-    // coarse rhs (1/4) written; PCG stencil: z, p_old, faces read, p and Ap
-    // written
-    if (timer->which == KT_MG_SMOOTH0) timer->bytes_per_launch = 4.25 * sizeof(T) * nP;
+    // coarse rhs (1/4) written; fused, also r, Ap, p, x read and x, r written
+    // (b is formed from r and Ap); PCG stencil: z, p_old, faces read, p and
+    // Ap written
+    if (timer->which == KT_MG_SMOOTH0) timer->bytes_per_launch = (fused ? 9.25 : 4.25) * sizeof(T) * nP;
     if (timer->which == KT_PCG_STENCIL) timer->bytes_per_launch = 6.0 * sizeof(T) * nP;
     const Lv<T> L = view(0);
     PcgScal* ps = scal.p;
     abs_tol_ = abs_tol;
     rtol_ = rtol;
+    b0cur = r0.p;
     k_pcg_start<T, double><<<grid0(), blk0(), 0, s>>>(G0, L, rhs, xs.p, r0.p, ps, outer_stop, part.p);
     // p_old of iteration 0 is pa: zero since creation, finite afterwards, and
     // multiplied by beta = 0
     for (int it = 0; it < maxit; ++it) {
-      vcycle(&ps->stop, it);
+      b0cur = fused ? rbuf(it) : r0.p;
+      vcycle(&ps->stop, it, fused);
+      const Lv<T> Lc = view(0);
       T* po = (it & 1) ? pb.p : pa.p;
-      T* pn = (it & 1) ? pa.p : pb.p;
+      T* pn = pnew(it);
       if (timer->active(KT_PCG_STENCIL))
-        timer->ext_launch(k_pcg_ap<T>, grid0(), blk0(), 0, s, G0, L, (const T*)lv[0]->y.p, (const T*)po, pn, Ap.p,
+        timer->ext_launch(k_pcg_ap<T>, grid0(), blk0(), 0, s, G0, Lc, (const T*)lv[0]->y.p, (const T*)po, pn, Ap.p,
                           it, ps, part.p);
       else
-        k_pcg_ap<T><<<grid0(), blk0(), 0, s>>>(G0, L, lv[0]->y.p, po, pn, Ap.p, it, ps, part.p);
-      k_pcg_xr<T><<<grid0(), blk0(), 0, s>>>(G0, L, xs.p, r0.p, pn, Ap.p, it, ps, part.p);
+        k_pcg_ap<T><<<grid0(), blk0(), 0, s>>>(G0, Lc, lv[0]->y.p, po, pn, Ap.p, it, ps, part.p);
+      if (!fused) k_pcg_xr<T><<<grid0(), blk0(), 0, s>>>(G0, Lc, xs.p, r0.p, pn, Ap.p, it, ps, part.p);
       CPS_HIP(hipGetLastError());
       if (sync_every > 0 && (it + 1) % sync_every == 0 && it + 1 < maxit) {
         k_pcg_end<<<1, blk0(), 0, s>>>(G0, it + 1, ps, part.p);
@@ -904,6 +1019,10 @@ struct MGImpl {
         if (stop) return;
       }
     }
+    // the update of the last iteration (in place in its residual buffer)
+    if (fused && maxit > 0)
+      k_pcg_xr<T><<<grid0(), blk0(), 0, s>>>(G0, view(0), xs.p, rbuf(maxit - 1), pnew(maxit - 1), Ap.p, maxit - 1,
+                                             ps, part.p);
     k_pcg_end<<<1, blk0(), 0, s>>>(G0, maxit, ps, part.p);
   }
 };

@@ -227,9 +227,11 @@ int cps_batch_boundary_patch(cps_batch* b, int field, int inlet, cps_patch* out)
 int cps_batch_gradient_dp(cps_batch* b, int field, double rtol, int max_iter, double* dp, double* dJ_dgamma,
                           cps_solve_stats* stats);
 
-/* Measurement hook (no reference counterpart): brackets every launch of one
- * kernel of the step with HIP events on the solver stream.  kernel:
- * CPS_KT_*; -1 stops timing.  Selecting a kernel clears the counters. */
+/* Measurement hook (no reference counterpart): brackets one launch in 8 of
+ * one kernel of the step with HIP events carried by the launch itself, on
+ * the solver stream (sampling keeps the timing from slowing the run it
+ * measures).  kernel: CPS_KT_*; -1 stops timing.  Selecting a kernel clears
+ * the counters. */
 #define CPS_KT_MG_SMOOTH0 0   /* fine-level multigrid Jacobi smoother (pressure) */
 #define CPS_KT_MOM_JACOBI 1   /* fused U+V momentum Jacobi sweep */
 #define CPS_KT_PCG_STENCIL 2  /* fine-level PCG p-update + 5-point apply */
