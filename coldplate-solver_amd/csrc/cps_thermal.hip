@@ -38,7 +38,9 @@ namespace cps {
 namespace th {
 
 constexpr int TBX = 64, TBY = 4, TNT = TBX * TBY;
-constexpr double kOm = 0.7;  // smoother damping
+constexpr double kOm = 0.7;   // point-Jacobi smoother damping
+constexpr double kOmZ = 0.8;  // z-line Jacobi smoother damping
+constexpr int kZMax = 64;     // layers a z-line smoother handles
 
 // 7-point operator in stencil form: (A x)_P = d x_P + w x_W + e x_E + s x_S +
 // n x_N + b x_B + t x_T; the off-diagonal coefficients are the matrix entries
@@ -267,13 +269,15 @@ struct Lvl {
   Op<T> A;
   T *x, *y;
   const T* rhs;
+  const int* stop;  // nullable: the running solve has stopped, skip the work
+  __device__ bool stopped() const { return stop && *stop; }
 };
 
 // x = om rhs / d  (first sweep from a zero guess)
 template <class T>
 __global__ __launch_bounds__(TNT) void k_jac0(Lvl<T> L) {
   CELL_IJK(L.A.nx, L.A.ny);
-  if (!inside) return;
+  if (!inside || L.stopped()) return;
   const size_t q = L.A.at(i, j, k);
   L.x[q] = (T)kOm * L.rhs[q] / L.A.d[q];
 }
@@ -283,7 +287,7 @@ __global__ __launch_bounds__(TNT) void k_jac0(Lvl<T> L) {
 template <class T>
 __global__ __launch_bounds__(TNT) void k_jac(Lvl<T> L, const T* src, T* dst, const T* cy, Op<T> C, int add) {
   CELL_IJK(L.A.nx, L.A.ny);
-  if (!inside) return;
+  if (!inside || L.stopped()) return;
   const Op<T>& A = L.A;
   auto xs = [&](int ii, int jj, int kk) -> T {
     T v = src[A.at(ii, jj, kk)];
@@ -302,11 +306,90 @@ __global__ __launch_bounds__(TNT) void k_jac(Lvl<T> L, const T* src, T* dst, con
   dst[q] = xc + (T)kOm * (L.rhs[q] - a) / A.d[q];
 }
 
+// z-line Jacobi: one thread per x-y column solves the column's tridiagonal
+// z system exactly (Thomas; the operator is an M-matrix) with the x-y
+// neighbours taken from xs:  x_line = Dz^-1 (rhs - Axy xs),
+// dst = xs + om (x_line - xs).  The hierarchy coarsens in x-y only and keeps
+// every layer, so the z coupling (thin, highly conductive layers) is left to
+// the smoother; point Jacobi handles it poorly.  mode 0: xs = 0; 1: xs =
+// src; 2: xs = src + the coarse result of the parent aggregate.
+// Thomas factors of every column's z system, once per operator:
+// zc = t / den, zdi = 1 / den with den = d - b zc(k-1).
+template <class T>
+__global__ __launch_bounds__(TNT) void k_zfactor(Op<T> A, T* zc, T* zdi) {
+  const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+  if (i >= A.nx || j >= A.ny) return;
+  double cp = 0.0;
+  for (int k = 0; k < A.nz; ++k) {
+    const size_t q = A.at(i, j, k);
+    const double lo = k > 0 ? (double)A.b[q] : 0.0, up = k + 1 < A.nz ? (double)A.t[q] : 0.0;
+    const double di = 1.0 / ((double)A.d[q] - lo * cp);
+    cp = up * di;
+    zc[q] = (T)cp;
+    zdi[q] = (T)di;
+  }
+}
+
+// NZ: compile-time bound on the layers (the host picks the smallest of 4, 8,
+// 12, 16, 24, 32, 64 that holds nz), so the per-column arrays live in
+// registers.  Layer indices are clamped to nz - 1 so every load is
+// unconditional and can be issued ahead of the recurrences (FMAs); layers
+// past nz are masked out of the recurrences and stores.
+template <class T, int NZ>
+__global__ __launch_bounds__(TNT) void k_zjac(Lvl<T> L, const T* zc, const T* zdi, const T* src, T* dst,
+                                             const T* cy, Op<T> C, int mode) {
+  const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+  const Op<T>& A = L.A;
+  if (i >= A.nx || j >= A.ny || L.stopped()) return;
+  const int nz = A.nz;
+  const bool hw = i > 0, he = i + 1 < A.nx, hs = j > 0, hn = j + 1 < A.ny;
+  const int I2 = i >> 1, J2 = j >> 1;
+  T g[NZ], cz[NZ], x0[NZ];
+  T gp = (T)0;
+#pragma unroll
+  for (int k = 0; k < NZ; ++k) {
+    const int kk = k < nz ? k : nz - 1;
+    const size_t q = A.at(i, j, kk);
+    T xc = (T)0, xw = (T)0, xe = (T)0, xs_ = (T)0, xn = (T)0;
+    if (mode != 0) {
+      xc = src[q];
+      xw = hw ? src[q - 1] : (T)0;
+      xe = he ? src[q + 1] : (T)0;
+      xs_ = hs ? src[q - A.pitch] : (T)0;
+      xn = hn ? src[q + A.pitch] : (T)0;
+      if (mode == 2) {
+        const size_t cq = C.at(I2, J2, kk);
+        const T cc = cy[cq];
+        xc += cc;
+        if (hw) xw += cy[C.at((i - 1) >> 1, J2, kk)];
+        if (he) xe += cy[C.at((i + 1) >> 1, J2, kk)];
+        if (hs) xs_ += cy[C.at(I2, (j - 1) >> 1, kk)];
+        if (hn) xn += cy[C.at(I2, (j + 1) >> 1, kk)];
+      }
+    }
+    const T rk = L.rhs[q] - (A.w[q] * xw + A.e[q] * xe + A.s[q] * xs_ + A.n[q] * xn);
+    const T lo = k > 0 ? A.b[q] : (T)0;
+    const T gn = (rk - lo * gp) * zdi[q];
+    cz[k] = zc[q];
+    x0[k] = xc;
+    gp = k < nz ? gn : gp;
+    g[k] = gp;
+  }
+  T xk = (T)0;
+#pragma unroll
+  for (int k = NZ - 1; k >= 0; --k) {
+    if (k < nz) {
+      xk = k == nz - 1 ? g[k] : g[k] - cz[k] * xk;
+      dst[A.at(i, j, k)] = x0[k] + (T)kOmZ * (xk - x0[k]);
+    }
+  }
+}
+
 // coarse right side = sum of the fine residuals over the aggregate
 template <class T>
 __global__ __launch_bounds__(TNT) void k_restrict(Lvl<T> L, const T* x, Op<T> C, T* crhs) {
   const int I = blockIdx.x * TBX + threadIdx.x, J = blockIdx.y * TBY + threadIdx.y, k = blockIdx.z;
-  if (I >= C.nx || J >= C.ny) return;
+  if (I >= C.nx || J >= C.ny || L.stopped()) return;
   T acc = 0;
   for (int dj = 0; dj < 2; ++dj)
     for (int di = 0; di < 2; ++di) {
@@ -321,7 +404,7 @@ __global__ __launch_bounds__(TNT) void k_restrict(Lvl<T> L, const T* x, Op<T> C,
 constexpr int kMaxNz = 64;
 template <class T>
 __global__ void k_coarsest(Lvl<T> L) {
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x != 0 || L.stopped()) return;
   const int nz = L.A.nz;
   double c[kMaxNz], g[kMaxNz];
   for (int k = 0; k < nz; ++k) {
@@ -342,48 +425,6 @@ __global__ void k_coarsest(Lvl<T> L) {
 // ---------------------------------------------------------------------------
 // Krylov vector kernels.  Reductions leave fixed-order block partials that
 // k_sum adds up in one block.
-
-// y = A x; partials (a . y) and (b2 . y) (null vectors skipped)
-template <class T>
-__global__ __launch_bounds__(TNT) void k_spmv_dot(Op<T> A, const T* x, T* y, const T* a, const T* b2, double* part) {
-  __shared__ double sh[TNT / kWave + 1];
-  CELL_IJK(A.nx, A.ny);
-  double s1 = 0, s2 = 0;
-  if (inside) {
-    const size_t q = A.at(i, j, k);
-    const T v = A.apply(x, i, j, k);
-    y[q] = v;
-    if (a) s1 = (double)a[q] * (double)v;
-    if (b2) s2 = (double)b2[q] * (double)v;
-  }
-  put_parts2(s1, s2, part, sh);
-}
-
-// out = a0 v0 + a1 v1 + a2 v2 (null terms skipped); partials (out . out) and
-// (dotw . out) when part is set
-template <class T>
-struct Axpy {
-  T* out;
-  const T *v0, *v1, *v2;
-  double a0, a1, a2;
-  const T* dotw;
-};
-template <class T>
-__global__ __launch_bounds__(TNT) void k_axpy(Op<T> A, Axpy<T> P, double* part) {
-  __shared__ double sh[TNT / kWave + 1];
-  CELL_IJK(A.nx, A.ny);
-  double s1 = 0, s2 = 0;
-  if (inside) {
-    const size_t q = A.at(i, j, k);
-    T v = (T)P.a0 * P.v0[q];
-    if (P.v1) v += (T)P.a1 * P.v1[q];
-    if (P.v2) v += (T)P.a2 * P.v2[q];
-    P.out[q] = v;
-    s1 = (double)v * (double)v;
-    if (P.dotw) s2 = (double)P.dotw[q] * (double)v;
-  }
-  if (part) put_parts2(s1, s2, part, sh);
-}
 
 // x += a p (+ w s)
 template <class X, class T>
@@ -424,6 +465,132 @@ __global__ __launch_bounds__(TNT) void k_sum(const double* part, int nb, int nsl
   for (int sl = 0; sl < nslots; ++sl) {
     const double a = parts_sum<TNT>(part + (size_t)sl * nb, nb, sh);
     if (flat_tid() == 0) out[sl] = a;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BiCGStab with its scalars on the device.  Iteration it keeps rho, alpha and
+// omega in the slots of parity it & 1 (rho[it & 1] = rho_it, the slot of
+// rho_it+1 is the one of rho_it-1, whose last reader is iteration it's
+// beta).  One-block kernels turn the reduction partials into the scalars and
+// the stop decision; every later kernel of the solve reads `stop` first and
+// skips (block-uniform), so iterations can be launched ahead of the host.
+struct BiState {
+  double rho[2], alpha[2], omega[2];
+  double rr, s2, tol2;
+  int stop;   // converged (recursive residual), half-step converged, or breakdown
+  int half;   // the last iteration converged at its half step
+  int iters;  // iterations completed
+  int pad_;
+};
+
+__global__ void k_bi_init(BiState* bs, double rr0, double tol2) {
+  if (threadIdx.x != 0) return;
+  bs->rho[0] = rr0;
+  bs->rho[1] = 1.0;
+  bs->alpha[0] = bs->alpha[1] = 1.0;
+  bs->omega[0] = bs->omega[1] = 1.0;
+  bs->rr = rr0;
+  bs->s2 = 0.0;
+  bs->tol2 = tol2;
+  bs->half = 0;
+  bs->iters = 0;
+  bs->stop = rr0 <= tol2 ? 1 : 0;
+}
+
+// p = r (it = 0) or r + beta (p - omega v), beta = (rho_it / rho_it-1)(alpha / omega)_it-1
+template <class T>
+__global__ __launch_bounds__(TNT) void k_bi_p(Op<T> A, T* p, const T* r, const T* v, const BiState* bs, int it) {
+  CELL_IJK(A.nx, A.ny);
+  if (!inside || bs->stop) return;
+  const size_t q = A.at(i, j, k);
+  if (it == 0) {
+    p[q] = (T)1.0 * r[q];
+    return;
+  }
+  const int P = it & 1, Q = P ^ 1;
+  const double beta = (bs->rho[P] / bs->rho[Q]) * (bs->alpha[Q] / bs->omega[Q]);
+  p[q] = (T)1.0 * r[q] + (T)beta * p[q] + (T)(-beta * bs->omega[Q]) * v[q];
+}
+
+// y = A x with partials (a . y), (b2 . y); skipped once stopped
+template <class T>
+__global__ __launch_bounds__(TNT) void k_bi_spmv(Op<T> A, const T* x, T* y, const T* a, const T* b2, double* part,
+                                                const BiState* bs) {
+  __shared__ double sh[TNT / kWave + 1];
+  if (bs->stop) return;  // block-uniform
+  CELL_IJK(A.nx, A.ny);
+  double s1 = 0, s2 = 0;
+  if (inside) {
+    const size_t q = A.at(i, j, k);
+    const T vv = A.apply(x, i, j, k);
+    y[q] = vv;
+    if (a) s1 = (double)a[q] * (double)vv;
+    if (b2) s2 = (double)b2[q] * (double)vv;
+  }
+  put_parts2(s1, s2, part, sh);
+}
+
+// s = r - alpha v (partial ||s||^2), or (second form, after the smoothing
+// of s) r = s - omega t, or r = s after a half-step convergence (partials
+// ||r||^2 and (rh . r))
+template <class T>
+__global__ __launch_bounds__(TNT) void k_bi_axpy(Op<T> A, int second, T* out, const T* a0v, const T* a1v,
+                                                const T* rh, double* part, const BiState* bs, int it) {
+  __shared__ double sh[TNT / kWave + 1];
+  if (bs->stop) return;  // block-uniform
+  CELL_IJK(A.nx, A.ny);
+  const int P = it & 1;
+  const double c = second ? (bs->half ? 0.0 : -bs->omega[P]) : -bs->alpha[P];
+  double s1 = 0, s2 = 0;
+  if (inside) {
+    const size_t q = A.at(i, j, k);
+    const T vv = (T)1.0 * a0v[q] + (T)c * a1v[q];
+    out[q] = vv;
+    s1 = (double)vv * (double)vv;
+    if (rh) s2 = (double)rh[q] * (double)vv;
+  }
+  put_parts2(s1, s2, part, sh);
+}
+
+// x += alpha ph (+ omega shat unless the half step converged)
+template <class T>
+__global__ __launch_bounds__(TNT) void k_bi_xupd(Op<T> A, T* x, const T* ph, const T* sh_, const BiState* bs, int it) {
+  CELL_IJK(A.nx, A.ny);
+  if (!inside || bs->stop) return;
+  const int P = it & 1;
+  const size_t q = A.at(i, j, k);
+  T vv = x[q] + (T)bs->alpha[P] * ph[q];
+  if (!bs->half) vv += (T)bs->omega[P] * sh_[q];
+  x[q] = vv;
+}
+
+// One block: the scalar a stage produces, from the partials of the kernel
+// before it.  stage 0: alpha = rho / (rh . v) (breakdown if 0 or not
+// finite); 1: ||s||^2 and the half-step test; 2: omega = (s . t) / (t . t);
+// 3: rr, rho_it+1 and the stop tests of the iteration's end.
+__global__ __launch_bounds__(TNT) void k_bi_scalar(const double* part, int nb, int stage, BiState* bs, int it) {
+  __shared__ double sh[TNT / kWave + 1];
+  __shared__ int shi;
+  if (block_read(&bs->stop, &shi)) return;
+  const double a = parts_sum<TNT>(part, nb, sh);
+  const double b = (stage >= 2) ? parts_sum<TNT>(part + nb, nb, sh) : 0.0;
+  if (flat_tid() != 0) return;
+  const int P = it & 1;
+  if (stage == 0) {
+    if (a == 0.0 || !isfinite(a)) bs->stop = 1;  // iteration it does not count
+    else bs->alpha[P] = bs->rho[P] / a;
+  } else if (stage == 1) {
+    bs->s2 = a;
+    bs->half = a <= bs->tol2 ? 1 : 0;
+  } else if (stage == 2) {
+    bs->omega[P] = bs->half ? 1.0 : (b != 0.0 ? a / b : 0.0);
+  } else {
+    bs->rr = a;
+    bs->rho[P ^ 1] = b;
+    bs->iters = it + 1;
+    const bool brk = !bs->half && (bs->omega[P] == 0.0 || b == 0.0);
+    if (bs->half || a <= bs->tol2 || brk || !isfinite(a)) bs->stop = 1;
   }
 }
 
@@ -629,7 +796,7 @@ struct ThermalKrylov {
   hipStream_t st;
   struct Level {
     int nx, ny, nz, pitch;
-    DevBuf<T> own[7], x, y, rhs;
+    DevBuf<T> own[7], x, y, rhs, zc, zdi;
     T* c[7] = {};
     th::Op<T> op() const { return th::Op<T>{nx, ny, nz, pitch, c[0], c[1], c[2], c[3], c[4], c[5], c[6]}; }
     dim3 grid() const { return dim3(ceil_div(nx, th::TBX), ceil_div(ny, th::TBY), nz); }
@@ -638,6 +805,8 @@ struct ThermalKrylov {
   std::vector<std::unique_ptr<Level>> lv;
   DevBuf<T> r, rh, p, ph, v, s, shat, t, e;
   DevBuf<double> part, sums;
+  DevBuf<th::BiState> bs;
+  const int* stop_flag = nullptr;  // skip flag of the level kernels during a solve
   int nb0 = 0;
 
   // borrow_level0: the finest operator's arrays belong to the caller (set_level0)
@@ -657,6 +826,8 @@ struct ThermalKrylov {
           L->c[q] = L->own[q].p;
         }
       L->x.alloc(S);
+      L->zc.alloc(S);
+      L->zdi.alloc(S);
       if (l > 0) {
         L->y.alloc(S);
         L->rhs.alloc(S);
@@ -675,6 +846,7 @@ struct ThermalKrylov {
     nb0 = (int)(g.x * g.y * g.z);
     part.alloc((size_t)2 * nb0);
     sums.alloc(4);
+    bs.alloc(1);
   }
 
   void set_level0(T* const c[7]) {
@@ -685,7 +857,31 @@ struct ThermalKrylov {
   void setup() {
     for (size_t l = 1; l < lv.size(); ++l)
       th::k_coarsen<T><<<lv[l]->grid(), dim3(th::TBX, th::TBY), 0, st>>>(lv[l - 1]->op(), lv[l]->op());
+    if (kZLine)
+      for (size_t l = 0; l < lv.size(); ++l)
+        th::k_zfactor<T><<<col_grid((int)l), dim3(th::TBX, th::TBY), 0, st>>>(lv[l]->op(), lv[l]->zc.p, lv[l]->zdi.p);
     CPS_HIP(hipGetLastError());
+  }
+
+#ifndef CPS_THERMAL_POINT_JACOBI
+  static constexpr bool kZLine = true;
+#else
+  static constexpr bool kZLine = false;
+#endif
+  dim3 col_grid(int l) const { return dim3(ceil_div(lv[l]->nx, th::TBX), ceil_div(lv[l]->ny, th::TBY), 1); }
+  void zjac(int l, const th::Lvl<T>& L, const T* src, T* dst, const T* cy, const th::Op<T>& C, int mode) {
+    const dim3 G = col_grid(l), B(th::TBX, th::TBY);
+    const int nz = lv[l]->nz;
+    const T *zc = lv[l]->zc.p, *zdi = lv[l]->zdi.p;
+#define CPS_ZJAC(NZ) th::k_zjac<T, NZ><<<G, B, 0, st>>>(L, zc, zdi, src, dst, cy, C, mode)
+    if (nz <= 4) CPS_ZJAC(4);
+    else if (nz <= 8) CPS_ZJAC(8);
+    else if (nz <= 12) CPS_ZJAC(12);
+    else if (nz <= 16) CPS_ZJAC(16);
+    else if (nz <= 24) CPS_ZJAC(24);
+    else if (nz <= 32) CPS_ZJAC(32);
+    else CPS_ZJAC(64);
+#undef CPS_ZJAC
   }
 
   // out = V-cycle(in) on level 0 (in and out distinct)
@@ -694,78 +890,75 @@ struct ThermalKrylov {
     const dim3 B(th::TBX, th::TBY);
     auto lvl = [&](int l) {
       Level& L = *lv[l];
-      if (l == 0) return th::Lvl<T>{L.op(), L.x.p, out, in};
-      return th::Lvl<T>{L.op(), L.x.p, L.y.p, L.rhs.p};
+      if (l == 0) return th::Lvl<T>{L.op(), L.x.p, out, in, stop_flag};
+      return th::Lvl<T>{L.op(), L.x.p, L.y.p, L.rhs.p, stop_flag};
     };
     const th::Op<T> none{};
     for (int l = 0; l < last; ++l) {
       const th::Lvl<T> L = lvl(l);
-      th::k_jac0<T><<<lv[l]->grid(), B, 0, st>>>(L);
-      th::k_jac<T><<<lv[l]->grid(), B, 0, st>>>(L, L.x, L.y, nullptr, none, 0);
+      if (kZLine) {
+        zjac(l, L, nullptr, L.x, nullptr, none, 0);
+        zjac(l, L, L.x, L.y, nullptr, none, 1);
+      } else {
+        th::k_jac0<T><<<lv[l]->grid(), B, 0, st>>>(L);
+        th::k_jac<T><<<lv[l]->grid(), B, 0, st>>>(L, L.x, L.y, nullptr, none, 0);
+      }
       th::k_restrict<T><<<lv[l + 1]->grid(), B, 0, st>>>(L, L.y, lv[l + 1]->op(), lv[l + 1]->rhs.p);
     }
     th::k_coarsest<T><<<1, 64, 0, st>>>(lvl(last));
     for (int l = last - 1; l >= 0; --l) {
       const th::Lvl<T> L = lvl(l);
-      th::k_jac<T><<<lv[l]->grid(), B, 0, st>>>(L, L.y, L.x, lv[l + 1]->y.p, lv[l + 1]->op(), 1);
-      th::k_jac<T><<<lv[l]->grid(), B, 0, st>>>(L, L.x, L.y, nullptr, none, 0);
+      if (kZLine) {
+        zjac(l, L, L.y, L.x, lv[l + 1]->y.p, lv[l + 1]->op(), 2);
+        zjac(l, L, L.x, L.y, nullptr, none, 1);
+      } else {
+        th::k_jac<T><<<lv[l]->grid(), B, 0, st>>>(L, L.y, L.x, lv[l + 1]->y.p, lv[l + 1]->op(), 1);
+        th::k_jac<T><<<lv[l]->grid(), B, 0, st>>>(L, L.x, L.y, nullptr, none, 0);
+      }
     }
     CPS_HIP(hipGetLastError());
-  }
-
-  void read_sums(int nslots, double* h) {
-    th::k_sum<<<1, dim3(th::TBX, th::TBY), 0, st>>>(part.p, nb0, nslots, sums.p);
-    CPS_HIP(hipMemcpyAsync(h, sums.p, nslots * sizeof(double), hipMemcpyDeviceToHost, st));
-    CPS_HIP(hipStreamSynchronize(st));
   }
 
   // BiCGStab, right preconditioned, on A e = r (r holds the right side, with
   // ||r||^2 = rr0); e starts at zero.  Stops when the recursive residual
-  // satisfies ||r|| <= tol_abs, after maxit iterations, or on breakdown.
+  // satisfies ||r|| <= tol_abs, after maxit iterations, or on breakdown.  The
+  // scalars stay on the device; the host reads the stop flag once per
+  // kPoll launched iterations (kernels of iterations past the stop skip).
+  static constexpr int kPoll = 4;
   int bicgstab(double rr0, double tol_abs, int maxit) {
     const th::Op<T> A = lv[0]->op();
     const dim3 G = lv[0]->grid(), B(th::TBX, th::TBY);
+    const double tol2 = tol_abs * tol_abs;
+    th::BiState* d = bs.p;
     CPS_HIP(hipMemcpyAsync(rh.p, r.p, r.bytes(), hipMemcpyDeviceToDevice, st));
     CPS_HIP(hipMemsetAsync(e.p, 0, e.bytes(), st));
-    double rho = 1.0, alpha = 1.0, omega = 1.0, rr = rr0, rho_new = rr0;
-    const double tol2 = tol_abs * tol_abs;
+    th::k_bi_init<<<1, 64, 0, st>>>(d, rr0, tol2);
+    stop_flag = &d->stop;
     int it = 0;
-    double h[4];
-    while (it < maxit && rr > tol2) {
-      const double beta = (rho_new / rho) * (alpha / omega);
-      rho = rho_new;
-      if (it == 0)
-        th::k_axpy<T><<<G, B, 0, st>>>(A, th::Axpy<T>{p.p, r.p, nullptr, nullptr, 1.0, 0.0, 0.0, nullptr}, nullptr);
-      else  // p = r + beta (p - omega v)
-        th::k_axpy<T><<<G, B, 0, st>>>(A, th::Axpy<T>{p.p, r.p, p.p, v.p, 1.0, beta, -beta * omega, nullptr},
-                                       nullptr);
-      precond(p.p, ph.p);
-      th::k_spmv_dot<T><<<G, B, 0, st>>>(A, ph.p, v.p, rh.p, nullptr, part.p);  // v = A ph, (rh, v)
-      read_sums(1, h);
-      if (h[0] == 0.0 || !std::isfinite(h[0])) break;
-      alpha = rho / h[0];
-      th::k_axpy<T><<<G, B, 0, st>>>(A, th::Axpy<T>{s.p, r.p, v.p, nullptr, 1.0, -alpha, 0.0, nullptr}, part.p);
-      read_sums(1, h);  // ||s||^2
-      if (h[0] <= tol2) {  // converged at the half step
-        th::k_xupd<T, T><<<G, B, 0, st>>>(A, e.p, ph.p, alpha, nullptr, 0.0);
-        CPS_HIP(hipMemcpyAsync(r.p, s.p, r.bytes(), hipMemcpyDeviceToDevice, st));
-        ++it;
-        break;
+    th::BiState h{};
+    while (it < maxit) {
+      const int k1 = std::min(maxit, it + kPoll);
+      for (; it < k1; ++it) {
+        th::k_bi_p<T><<<G, B, 0, st>>>(A, p.p, r.p, v.p, d, it);
+        precond(p.p, ph.p);
+        th::k_bi_spmv<T><<<G, B, 0, st>>>(A, ph.p, v.p, rh.p, nullptr, part.p, d);  // v = A ph, (rh, v)
+        th::k_bi_scalar<<<1, B, 0, st>>>(part.p, nb0, 0, d, it);                     // alpha
+        th::k_bi_axpy<T><<<G, B, 0, st>>>(A, 0, s.p, r.p, v.p, nullptr, part.p, d, it);  // s, ||s||^2
+        th::k_bi_scalar<<<1, B, 0, st>>>(part.p, nb0, 1, d, it);                     // half-step test
+        precond(s.p, shat.p);
+        th::k_bi_spmv<T><<<G, B, 0, st>>>(A, shat.p, t.p, s.p, t.p, part.p, d);  // t = A shat, (s, t), (t, t)
+        th::k_bi_scalar<<<1, B, 0, st>>>(part.p, nb0, 2, d, it);                 // omega
+        th::k_bi_xupd<T><<<G, B, 0, st>>>(A, e.p, ph.p, shat.p, d, it);
+        th::k_bi_axpy<T><<<G, B, 0, st>>>(A, 1, r.p, s.p, t.p, rh.p, part.p, d, it);  // r, ||r||^2, (rh, r)
+        th::k_bi_scalar<<<1, B, 0, st>>>(part.p, nb0, 3, d, it);                      // rr, rho, stop
       }
-      precond(s.p, shat.p);
-      th::k_spmv_dot<T><<<G, B, 0, st>>>(A, shat.p, t.p, s.p, t.p, part.p);  // t = A shat, (s, t), (t, t)
-      read_sums(2, h);
-      omega = h[1] != 0.0 ? h[0] / h[1] : 0.0;
-      th::k_xupd<T, T><<<G, B, 0, st>>>(A, e.p, ph.p, alpha, shat.p, omega);
-      th::k_axpy<T><<<G, B, 0, st>>>(A, th::Axpy<T>{r.p, s.p, t.p, nullptr, 1.0, -omega, 0.0, rh.p}, part.p);
-      read_sums(2, h);
-      rr = h[0];
-      rho_new = h[1];
-      ++it;
-      if (omega == 0.0 || rho_new == 0.0 || !std::isfinite(rr)) break;
+      CPS_HIP(hipGetLastError());
+      CPS_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, st));
+      CPS_HIP(hipStreamSynchronize(st));
+      if (h.stop) break;
     }
-    CPS_HIP(hipGetLastError());
-    return it;
+    stop_flag = nullptr;
+    return h.iters;
   }
 };
 
