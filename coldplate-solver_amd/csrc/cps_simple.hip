@@ -1139,7 +1139,7 @@ class BatchSolver {
   std::vector<hipGraphExec_t> execs;
   int nstreams = 0;
   double us_per_step = 0.0;
-  static constexpr int kMaxStreams = 16;
+  static constexpr int kMaxStreams = 32;
 
   BatchSolver(Ctx* c, int n, const cps_simple_config* cfgs, const double* const* gammas,
               const double* const* inlets)
@@ -1244,7 +1244,7 @@ class BatchSolver {
     std::vector<int> cand;
     if (fixed > 0) cand.push_back(fixed);
     else
-      for (int ns : {1, 2, 4, 8, 16})
+      for (int ns : {1, 2, 4, 8, 16, 24, 32})
         if (ns <= (int)f.size()) cand.push_back(ns);
     int best = cand[0];
     double best_us = 1e300;
