@@ -228,6 +228,15 @@ __global__ __launch_bounds__(TNT) void k_cast_op(Op<double> A, Op<T> B) {
   B.n[q] = (T)A.n[q]; B.b[q] = (T)A.b[q]; B.t[q] = (T)A.t[q];
 }
 
+// out = (To) in over the level's cells; skipped once the running solve stopped
+template <class Ti, class To>
+__global__ __launch_bounds__(TNT) void k_cast_vec(Op<To> A, const Ti* in, To* out, const int* stop) {
+  CELL_IJK(A.nx, A.ny);
+  if (!inside || (stop && *stop)) return;
+  const size_t q = A.at(i, j, k);
+  out[q] = (To)in[q];
+}
+
 // Galerkin P0 coarsening over 2 x 2 x 1 aggregates: a coarse coupling is the
 // sum of the fine couplings between the two aggregates; the coarse diagonal
 // is the sum of the fine diagonals plus the couplings inside the aggregate.
@@ -790,10 +799,15 @@ __global__ __launch_bounds__(TNT) void k_surr_sens(int nx, int ny, double dx2inv
 }  // namespace th
 
 // ===========================================================================
-// multigrid hierarchy + BiCGStab in precision T
-template <class T>
+// BiCGStab with vectors in precision TK, preconditioned by a multigrid
+// hierarchy in precision T (TK = double, T = float: FP64 Krylov, FP32
+// V-cycles; the casts sit at the preconditioner's boundary)
+template <class T, class TK = T>
 struct ThermalKrylov {
+  static constexpr bool kSplit = !std::is_same<T, TK>::value;
+  using Vec = TK;
   hipStream_t st;
+  th::Op<TK> A0{};  // the Krylov operator (level 0's own when not split)
   struct Level {
     int nx, ny, nz, pitch;
     DevBuf<T> own[7], x, y, rhs, zc, zdi;
@@ -803,7 +817,7 @@ struct ThermalKrylov {
     size_t size() const { return (size_t)nz * ny * pitch; }
   };
   std::vector<std::unique_ptr<Level>> lv;
-  DevBuf<T> r, rh, p, ph, v, s, shat, t, e;
+  DevBuf<TK> r, rh, p, ph, v, s, shat, t, e;
   DevBuf<double> part, sums;
   DevBuf<th::BiState> bs;
   const int* stop_flag = nullptr;  // skip flag of the level kernels during a solve
@@ -828,7 +842,7 @@ struct ThermalKrylov {
       L->x.alloc(S);
       L->zc.alloc(S);
       L->zdi.alloc(S);
-      if (l > 0) {
+      if (l > 0 || kSplit) {
         L->y.alloc(S);
         L->rhs.alloc(S);
       }
@@ -847,11 +861,15 @@ struct ThermalKrylov {
     part.alloc((size_t)2 * nb0);
     sums.alloc(4);
     bs.alloc(1);
+    if constexpr (!kSplit) A0 = lv[0]->op();
   }
 
   void set_level0(T* const c[7]) {
     for (int q = 0; q < 7; ++q) lv[0]->c[q] = c[q];
+    if constexpr (!kSplit) A0 = lv[0]->op();
   }
+  // the operator of the Krylov products (FP64 when split; level 0's otherwise)
+  void set_krylov_op(const th::Op<TK>& A) { A0 = A; }
 
   // coarse operators from the finest one
   void setup() {
@@ -885,12 +903,22 @@ struct ThermalKrylov {
   }
 
   // out = V-cycle(in) on level 0 (in and out distinct)
-  void precond(const T* in, T* out) {
+  void precond(const TK* in, TK* out) {
     const int last = (int)lv.size() - 1;
     const dim3 B(th::TBX, th::TBY);
+    const T* in0;
+    T* out0;
+    if constexpr (kSplit) {
+      th::k_cast_vec<TK, T><<<lv[0]->grid(), B, 0, st>>>(lv[0]->op(), in, lv[0]->rhs.p, stop_flag);
+      in0 = lv[0]->rhs.p;
+      out0 = lv[0]->y.p;
+    } else {
+      in0 = in;
+      out0 = out;
+    }
     auto lvl = [&](int l) {
       Level& L = *lv[l];
-      if (l == 0) return th::Lvl<T>{L.op(), L.x.p, out, in, stop_flag};
+      if (l == 0) return th::Lvl<T>{L.op(), L.x.p, out0, in0, stop_flag};
       return th::Lvl<T>{L.op(), L.x.p, L.y.p, L.rhs.p, stop_flag};
     };
     const th::Op<T> none{};
@@ -916,6 +944,7 @@ struct ThermalKrylov {
         th::k_jac<T><<<lv[l]->grid(), B, 0, st>>>(L, L.x, L.y, nullptr, none, 0);
       }
     }
+    if constexpr (kSplit) th::k_cast_vec<T, TK><<<lv[0]->grid(), B, 0, st>>>(A0, out0, out, stop_flag);
     CPS_HIP(hipGetLastError());
   }
 
@@ -926,7 +955,7 @@ struct ThermalKrylov {
   // kPoll launched iterations (kernels of iterations past the stop skip).
   static constexpr int kPoll = 4;
   int bicgstab(double rr0, double tol_abs, int maxit) {
-    const th::Op<T> A = lv[0]->op();
+    const th::Op<TK> A = A0;
     const dim3 G = lv[0]->grid(), B(th::TBX, th::TBY);
     const double tol2 = tol_abs * tol_abs;
     th::BiState* d = bs.p;
@@ -939,17 +968,17 @@ struct ThermalKrylov {
     while (it < maxit) {
       const int k1 = std::min(maxit, it + kPoll);
       for (; it < k1; ++it) {
-        th::k_bi_p<T><<<G, B, 0, st>>>(A, p.p, r.p, v.p, d, it);
+        th::k_bi_p<TK><<<G, B, 0, st>>>(A, p.p, r.p, v.p, d, it);
         precond(p.p, ph.p);
-        th::k_bi_spmv<T><<<G, B, 0, st>>>(A, ph.p, v.p, rh.p, nullptr, part.p, d);  // v = A ph, (rh, v)
+        th::k_bi_spmv<TK><<<G, B, 0, st>>>(A, ph.p, v.p, rh.p, nullptr, part.p, d);  // v = A ph, (rh, v)
         th::k_bi_scalar<<<1, B, 0, st>>>(part.p, nb0, 0, d, it);                     // alpha
-        th::k_bi_axpy<T><<<G, B, 0, st>>>(A, 0, s.p, r.p, v.p, nullptr, part.p, d, it);  // s, ||s||^2
+        th::k_bi_axpy<TK><<<G, B, 0, st>>>(A, 0, s.p, r.p, v.p, nullptr, part.p, d, it);  // s, ||s||^2
         th::k_bi_scalar<<<1, B, 0, st>>>(part.p, nb0, 1, d, it);                     // half-step test
         precond(s.p, shat.p);
-        th::k_bi_spmv<T><<<G, B, 0, st>>>(A, shat.p, t.p, s.p, t.p, part.p, d);  // t = A shat, (s, t), (t, t)
+        th::k_bi_spmv<TK><<<G, B, 0, st>>>(A, shat.p, t.p, s.p, t.p, part.p, d);  // t = A shat, (s, t), (t, t)
         th::k_bi_scalar<<<1, B, 0, st>>>(part.p, nb0, 2, d, it);                 // omega
-        th::k_bi_xupd<T><<<G, B, 0, st>>>(A, e.p, ph.p, shat.p, d, it);
-        th::k_bi_axpy<T><<<G, B, 0, st>>>(A, 1, r.p, s.p, t.p, rh.p, part.p, d, it);  // r, ||r||^2, (rh, r)
+        th::k_bi_xupd<TK><<<G, B, 0, st>>>(A, e.p, ph.p, shat.p, d, it);
+        th::k_bi_axpy<TK><<<G, B, 0, st>>>(A, 1, r.p, s.p, t.p, rh.p, part.p, d, it);  // r, ||r||^2, (rh, r)
         th::k_bi_scalar<<<1, B, 0, st>>>(part.p, nb0, 3, d, it);                      // rr, rho, stop
       }
       CPS_HIP(hipGetLastError());
@@ -972,7 +1001,7 @@ class ThermalSolver {
   DevBuf<int> channel;
   DevBuf<double> A64[7], At64[7], rhs, T, lam, g, part, ext, sums, dgrad;
   std::unique_ptr<ThermalKrylov<double>> k64;
-  std::unique_ptr<ThermalKrylov<float>> k32;
+  std::unique_ptr<ThermalKrylov<float, double>> k32;  // FP32 V-cycles inside FP64 BiCGStab
   bool solved = false, transposed_ready = false;
 
   ThermalSolver(Ctx* c, const cps_thermal_config& conf, const double* hdz, const int* hch, const double* hg,
@@ -1037,7 +1066,7 @@ class ThermalSolver {
     if (cfg.precision == CPS_PREC_FP64)
       k64.reset(new ThermalKrylov<double>(st, nx, ny, nz, true));
     else
-      k32.reset(new ThermalKrylov<float>(st, nx, ny, nz, false));
+      k32.reset(new ThermalKrylov<float, double>(st, nx, ny, nz, false));
     update(hg, hu, hv, hq);
   }
 
@@ -1095,8 +1124,9 @@ class ThermalSolver {
   // Solves op x = b to ||b - op x|| <= rtol ||b|| (FP64 residual), starting
   // from x: BiCGStab-MG in the Krylov precision, restarted from the true
   // FP64 residual (iterative refinement when that precision is FP32).
-  template <class Tk>
-  void refine_solve(ThermalKrylov<Tk>& kr, bool transposed, const double* b, double* x, cps_solve_stats* stats) {
+  template <class K>
+  void refine_solve(K& kr, bool transposed, const double* b, double* x, cps_solve_stats* stats) {
+    using Tk = typename K::Vec;
     const th::Op<double> A = op64(transposed);
     const dim3 G = grid(), B(th::TBX, th::TBY);
     double rr, bb;
@@ -1111,7 +1141,7 @@ class ThermalSolver {
       const double tol_abs =
           fp64 ? 0.5 * std::sqrt(target2) : std::max(1e-4 * std::sqrt(rr), 0.5 * std::sqrt(target2));
       inner_total += kr.bicgstab(rr, tol_abs, cfg.max_iter - inner_total);
-      th::k_xupd<double, Tk><<<G, B, 0, ctx->stream>>>(kr.lv[0]->op(), x, kr.e.p, 1.0, nullptr, 0.0);
+      th::k_xupd<double, Tk><<<G, B, 0, ctx->stream>>>(kr.A0, x, kr.e.p, 1.0, nullptr, 0.0);
       residual<Tk>(A, x, b, kr.r.p, &rr, &bb);
       ++outer;
       if (!std::isfinite(rr)) throw Error{CPS_ERR_DIVERGED, "thermal solve produced a non-finite residual"};
@@ -1134,6 +1164,7 @@ class ThermalSolver {
       refine_solve(*k64, transposed, b, x, stats);
     } else {
       th::k_cast_op<float><<<grid(), dim3(th::TBX, th::TBY), 0, ctx->stream>>>(op64(transposed), k32->lv[0]->op());
+      k32->set_krylov_op(op64(transposed));
       k32->setup();
       refine_solve(*k32, transposed, b, x, stats);
     }
