@@ -319,19 +319,20 @@ __global__ __launch_bounds__(BS) void k_count(int n, const int* col, int* cnt) {
   }
 }
 
-// max of cnt[0..n) and of flag[0..n) in one block
+// max of cnt[0..n) and of flag[0..n): block maxima folded into out[0..1]
+// (zeroed beforehand) with integer atomics
 __global__ __launch_bounds__(BS) void k_imax(int n, const int* cnt, const int* flag, int* out) {
   __shared__ int sh[BS / kWave + 1];
   int a = 0, b = 0;
-  for (int r = threadIdx.x; r < n; r += BS) {
+  for (int r = blockIdx.x * BS + threadIdx.x; r < n; r += gridDim.x * BS) {
     a = max(a, cnt[r]);
     b = max(b, flag[r]);
   }
   a = block_max<BS>(a, sh);
   b = block_max<BS>(b, sh);
   if (threadIdx.x == 0) {
-    out[0] = a;
-    out[1] = b;
+    atomicMax(&out[0], a);
+    atomicMax(&out[1], b);
   }
 }
 
@@ -901,7 +902,8 @@ class FlowAdjoint {
     adj::k_rows<<<blocks(n), adj::BS, 0, s>>>(V, n, col.p, val.p, res.p, flag.p, dinv.p);
     CPS_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), s));
     adj::k_count<<<blocks(n), adj::BS, 0, s>>>(n, col.p, cnt.p);
-    adj::k_imax<<<1, adj::BS, 0, s>>>(n, cnt.p, flag.p, iout.p);
+    CPS_HIP(hipMemsetAsync(iout.p, 0, 2 * sizeof(int), s));
+    adj::k_imax<<<std::min(blocks(n), 1024), adj::BS, 0, s>>>(n, cnt.p, flag.p, iout.p);
     int h[2];
     CPS_HIP(hipMemcpyAsync(h, iout.p, sizeof(h), hipMemcpyDeviceToHost, s));
     CPS_HIP(hipStreamSynchronize(s));
