@@ -12,8 +12,9 @@
 // 1e8 coefficient contrast because faces are summed, not averaged).
 // Smoother: damped Jacobi (omega 0.8), 2 pre + 2 post sweeps, so the V-cycle
 // is symmetric and PCG-safe.  Levels down to 1024 (FP64) / 2048 (FP32)
-// cells run as grid kernels; all smaller levels, down to a single cell
-// solved exactly in FP64, run inside one workgroup in LDS (k_mg_coarse).
+// cells run as grid kernels; the smaller ones run inside one workgroup in LDS
+// (k_mg_coarse) down to the first level of at most 32 cells, which is solved
+// exactly with its inverse (Gauss-Jordan in FP64 once per solve).
 //
 // Mixed precision: r64 = b - A x64 (FP64), FP32 MG-PCG on A e = r64, x64 += e
 // (FP64), repeated until ||r64|| meets the tolerance.
@@ -37,12 +38,10 @@ constexpr int kCoarseThreads = CPS_COARSE_THREADS;
 // staging limits of the coarse solver: all its levels together, and its top
 constexpr int kCoarseTotalMax = 3072, kCoarseTopMax = 2048;
 static_assert(kCoarseTotalMax % kCoarseThreads == 0 && kCoarseTopMax % kCoarseThreads == 0, "coarse staging");
-#ifndef CPS_WAVE_CELLS
-#define CPS_WAVE_CELLS 64
-#endif
-// coarse levels of at most this many cells are smoothed by one wavefront
-// (no workgroup barriers)
-constexpr int kWaveCells = CPS_WAVE_CELLS;
+// the coarse hierarchy ends at its first level of at most kDirectCells cells,
+// which is solved with its inverse (one column per lane of a wavefront)
+constexpr int kDirectCells = 32;
+static_assert(kDirectCells * kDirectCells <= kCoarseThreads && kDirectCells <= 64, "direct level staging");
 constexpr double kTiny20 = (double)1e-20f;
 
 template <class T>
@@ -399,7 +398,8 @@ struct CoarseDesc {
   int nlev;
   int nx[16], ny[16], off[16];  // compact LDS layout, pitch = nx
   int total;
-  int wl;  // first level of at most kWaveCells cells: it and the coarser ones run in wave 0
+  int wl;  // the last level (at most kDirectCells cells): solved with its inverse
+  int dn;  // its cell count; the inverse sits at cf[3 total ..), column-major
 };
 
 #define CELLS(l)                                              \
@@ -448,15 +448,65 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, C
       cf[2 * d.total + k] = (T)1 / dd;
     }
   }
+  // Inverse of the last level's operator (the faces and floored diagonal the
+  // V-cycle applies) by Gauss-Jordan in FP64 in wave 0: lane t holds column
+  // t in registers, pivot entries travel by shuffles.  The operator is a
+  // symmetric M-matrix, so no pivoting is needed.
+  __syncthreads();
+  if (t < 64) {
+    const int l = d.wl, o = d.off[l], nx = d.nx[l], ny = d.ny[l], n = d.dn;
+    double col[kDirectCells];
+    const int r = t / max(nx, 1), c = t - r * nx, k = o + t;
+    const bool own = t < n;
+    double dg = 1.0, ee = 0.0, ww = 0.0, nn = 0.0, ss = 0.0;
+    if (own) {
+      const T dT = se[k] + (c > 0 ? se[k - 1] : (T)0) + sn[k] + (r > 0 ? sn[k - nx] : (T)0);
+      dg = dT < (T)kTiny20 ? 1.0 : (double)dT;
+      ee = c + 1 < nx ? -(double)se[k] : 0.0;
+      ww = c > 0 ? -(double)se[k - 1] : 0.0;
+      nn = r + 1 < ny ? -(double)sn[k] : 0.0;
+      ss = r > 0 ? -(double)sn[k - nx] : 0.0;
+    }
+    // column t of a symmetric matrix = row t
+#pragma unroll
+    for (int i = 0; i < kDirectCells; ++i) {
+      double v = 0.0;
+      if (own) {
+        if (i == t) v = dg;
+        else if (i == t + 1) v = ee;
+        else if (i == t - 1) v = ww;
+        else if (i == t + nx) v = nn;
+        else if (i == t - nx) v = ss;
+      }
+      col[i] = v;
+    }
+    for (int p = 0; p < n; ++p) {
+      double cp = 0.0;  // this column's row-p entry
+#pragma unroll
+      for (int i = 0; i < kDirectCells; ++i) cp = i == p ? col[i] : cp;
+      const double inv = 1.0 / __shfl(cp, p);
+      const double mp = (t == p ? 1.0 : cp) * inv;
+#pragma unroll
+      for (int i = 0; i < kDirectCells; ++i) {
+        const double f = __shfl(col[i], p);  // A[i][p], before lane p updates it
+        const double v = (t == p ? 0.0 : col[i]) - f * mp;
+        col[i] = i == p ? mp : v;
+      }
+    }
+    if (own)
+#pragma unroll
+      for (int i = 0; i < kDirectCells; ++i)
+        if (i < n) cf[3 * d.total + t * n + i] = (T)col[i];
+  }
 }
 
 // All levels from `top` down inside one workgroup, in LDS: per level one
 // pre-smoothing phase (both sweeps from zero fused: x1 = x0 + w (b - A x0)/d
 // with x0 = w b/d formed on the fly), one restriction phase, and two
-// post-smoothing phases (the first with the prolongation fused).  Levels of
-// more than kWaveCells cells separate their phases with workgroup barriers;
-// the smaller ones run in wave 0 alone with wavefront-scope ordering only.  The coarsest level (one cell) is solved exactly in FP64.
-// The top level's correction goes to top.y.
+// post-smoothing phases (the first with the prolongation fused), separated by
+// workgroup barriers.  The last level (at most kDirectCells cells) is one
+// product with its inverse, from the setup kernel.  The top level's
+// correction goes to top.y.
 template <class T>
 __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseDesc d, const T* cf, const int* stop,
                                                               PcgTest test) {
@@ -486,6 +536,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
   T* se = sb + d.total;
   T* sn = se + d.total;
   T* si = sn + d.total;  // 1 / diagonal
+  T* sA = si + d.total;  // inverse of the last level's operator, column-major
   const int t = threadIdx.x, tx = t & 63, ty = t >> 6;
   {  // every global load of the thread is issued before the LDS stores
     constexpr int NQ = kCoarseTotalMax / kCoarseThreads;  // d.total <= kCoarseTotalMax
@@ -514,6 +565,9 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
       const int q = t + i * kCoarseThreads;
       if (q < n0) sb[q] = vb[i];
     }
+    const int na = d.dn * d.dn;  // <= kDirectCells^2 = kCoarseThreads
+    const T va = t < na ? cf[3 * d.total + t] : (T)0;
+    if (t < na) sA[t] = va;
   }
   __syncthreads();
   const T om = (T)kOmega;
@@ -527,7 +581,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
     if (r > 0) a -= sn[k - nx] * xv(k - nx);
     return a;
   };
-  const int last = d.nlev - 1, wl = d.wl;
+  const int wl = d.wl;
   // one smoothing sweep from x0 = w b / d into sy, then the restricted
   // residual into the next level's right side; cell (r, c) of level l
   auto down_sweep = [&](int l, int r, int c) {
@@ -572,37 +626,12 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
     CELLS(l + 1) { restrict_cell(l, r, c); }
     __syncthreads();
   }
-  // Levels of at most kWaveCells cells run in wave 0 alone, each lane taking
-  // cells t, t + 64, ...  Within a phase the arrays read and written are
-  // disjoint; a wavefront's LDS accesses complete in order, so
-  // wavefront-scope fences (which keep the compiler from moving LDS accesses
-  // across them) order the phases.
-  if (t < 64) {
-    auto wave_sync = [] {
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    };
-    for (int l = wl; l < last; ++l) {
-      const int nx = d.nx[l], n = nx * d.ny[l];
-      for (int q = t; q < n; q += 64) down_sweep(l, q / nx, q - (q / nx) * nx);
-      wave_sync();
-      const int cx = d.nx[l + 1], cn = cx * d.ny[l + 1];
-      for (int q = t; q < cn; q += 64) restrict_cell(l, q / cx, q - (q / cx) * cx);
-      wave_sync();
-    }
-    if (t == 0) {  // one cell: exact, in FP64
-      const int k = d.off[last];
-      sy[k] = (T)((double)sb[k] * (double)si[k]);
-    }
-    wave_sync();
-    for (int l = last - 1; l >= wl; --l) {
-      const int nx = d.nx[l], n = nx * d.ny[l];
-      for (int q = t; q < n; q += 64) up_sweep1(l, q / nx, q - (q / nx) * nx);
-      wave_sync();
-      for (int q = t; q < n; q += 64) up_sweep2(l, q / nx, q - (q / nx) * nx);
-      wave_sync();
-    }
+  // the last level: y = A^-1 b, one row per lane of wave 0
+  if (t < d.dn) {
+    const int o = d.off[wl], n = d.dn;
+    T acc = (T)0;
+    for (int j = 0; j < n; ++j) acc += sA[j * n + t] * sb[o + j];
+    sy[o + t] = acc;
   }
   __syncthreads();
   for (int l = wl - 1; l >= 0; --l) {
@@ -887,32 +916,31 @@ struct MGImpl {
     }
     nGrid = (int)lv.size() - 1;
     int nx = lv.back()->nx, ny = lv.back()->ny, off = 0, n = 0;
+    // coarse levels down to the first of at most kDirectCells cells, which
+    // is solved directly
     while (true) {
       cd.nx[n] = nx;
       cd.ny[n] = ny;
       cd.off[n] = off;
       off += nx * ny;
       ++n;
-      if ((nx == 1 && ny == 1) || n == 16) break;
+      if (nx * ny <= kDirectCells) break;
+      CPS_CHECK(n < 16, CPS_ERR_STATE, "coarse multigrid plan too deep");
       nx = (nx + 1) / 2;
       ny = (ny + 1) / 2;
     }
     cd.nlev = n;
     cd.total = off;
     cd.wl = n - 1;
-    for (int l = 0; l < n; ++l)
-      if (cd.nx[l] * cd.ny[l] <= kWaveCells) {
-        cd.wl = l;
-        break;
-      }
+    cd.dn = nx * ny;
     CPS_CHECK(off <= kCoarseTotalMax && lv.back()->nx * lv.back()->ny <= kCoarseTopMax, CPS_ERR_STATE,
               "coarse multigrid plan exceeds the coarse solver's staging");
-    coarseSmem = (size_t)6 * off * sizeof(T);
+    coarseSmem = ((size_t)6 * off + (size_t)cd.dn * cd.dn) * sizeof(T);
     coarseSetupSmem = (size_t)2 * off * sizeof(T);
     int lds = 0;
     CPS_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
     CPS_CHECK(coarseSmem <= (size_t)lds, CPS_ERR_STATE, "coarse multigrid levels exceed LDS");
-    cf.alloc((size_t)3 * off);
+    cf.alloc((size_t)3 * off + (size_t)cd.dn * cd.dn);
     for (auto* b : {&lv[0]->x, &lv[0]->y, &lv[0]->ex, &lv[0]->ey, &r0, &r1, &xs, &pa, &pb, &Ap})
       CPS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), ctx->stream));
     G0.gx = ceil_div(lv[0]->nx, TX);
