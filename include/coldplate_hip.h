@@ -257,7 +257,7 @@ typedef struct cps_thermal_config {
   double k_s, k_f, rho, cp, t_in;  /* thermal_solver.cpp:160-164 */
   int k_mode;                /* 0 linear, 1 RAMP, 2 harmonic (:182) */
   double qk;                 /* RAMP q_k (:183) */
-  int precision;             /* CPS_PREC_FP64 or CPS_PREC_MIXED (FP32 Krylov + FP64 refinement) */
+  int precision;             /* CPS_PREC_FP64 or CPS_PREC_MIXED (FP32 multigrid inside FP64 BiCGStab) */
   double rtol;               /* stop at ||b - A T|| <= rtol ||b|| (:167) */
   int max_iter;              /* Krylov iterations per solve, all refinements together (:168) */
   int max_refine;            /* restarts from the true FP64 residual */
