@@ -96,9 +96,12 @@ def test_reference_tolerance_full_stack(pkg, orc):
     assert np.abs(out[32] - out[64]).max() / rise < 1e-3
 
 
-def test_adjoint_solve_matches_direct(pkg, orc):
+@pytest.mark.parametrize("precision", [64, 32])
+def test_adjoint_solve_matches_direct(pkg, orc, precision):
+    """A^T lambda = rhs against the direct transposed solve; the mixed form
+    runs FP32 V-cycles inside FP64 BiCGStab and FP64 refinement."""
     stack, gamma, u, v = synthetic_case(pkg, 5, 30, 14)
-    s = pkg.ThermalSolver3D(stack, gamma, u, v, rtol=1e-12, max_iter=2000)
+    s = pkg.ThermalSolver3D(stack, gamma, u, v, precision=precision, rtol=1e-12, max_iter=2000, max_refine=12)
     o = orc.OracleThermal(pkg._types, stack, gamma, u, v)
     rhs = np.random.default_rng(0).standard_normal((s.nz, s.ny, s.nx))
     lam = s.solve_adjoint(rhs)
@@ -107,11 +110,12 @@ def test_adjoint_solve_matches_direct(pkg, orc):
     s.close()
 
 
-@pytest.mark.parametrize("objective,k_mode", [("t_avg_base", 1), ("pnorm", 1), ("t_avg_base", 2), ("pnorm", 0)])
-def test_gradient_matches_central_differences(pkg, orc, objective, k_mode):
+@pytest.mark.parametrize("objective,k_mode,precision", [("t_avg_base", 1, 64), ("pnorm", 1, 64), ("t_avg_base", 2, 64),
+                                                        ("pnorm", 0, 64), ("pnorm", 1, 32)])
+def test_gradient_matches_central_differences(pkg, orc, objective, k_mode, precision):
     nx, ny = 12, 8
     stack, gamma, u, v = synthetic_case(pkg, 11, nx, ny, k_mode=k_mode, qk=3.0)
-    s = pkg.ThermalSolver3D(stack, gamma, u, v, rtol=1e-13, max_iter=2000, max_refine=12)
+    s = pkg.ThermalSolver3D(stack, gamma, u, v, precision=precision, rtol=1e-13, max_iter=2000, max_refine=12)
     s.solve()
     J, grad = s.gradient(objective, 8.0)
     o0 = orc.OracleThermal(pkg._types, stack, gamma, u, v)
