@@ -181,7 +181,8 @@ def thermal_leg(pkg, reps=3, flow_steps=3000):
     4 Cu lid layers) to the reference's rtol 1e-6, FP64 and FP32-refined.
     The channel velocity is the cell-centred field of a 512 x 256 density-
     field SIMPLE solve (seed 0, FP64 GMG, flow_steps iterations) on the same
-    gamma; seeded power map (20 W/cm^2 + 8 hotspots)."""
+    gamma; seeded power map (20 W/cm^2 + 8 hotspots).  Each form also times
+    the gradient of the layer-0 p-norm (p = 8): adjoint solve + sensitivity."""
     T = pkg._types
     st = pkg.configs.thermal_stack()
     cfg, gamma, inlet = pkg.configs.density_field(nx=st["nx"], ny=st["ny"], seed=0)
@@ -202,6 +203,13 @@ def thermal_leg(pkg, reps=3, flow_steps=3000):
         dt = (time.perf_counter() - t0) / reps
         out[name] = dict(seconds=dt, iterations=r["iterations"], refinements=r["refinements"],
                          rel_residual=r["rel_residual"])
+        # dJ/dgamma of the layer-0 p-norm (p = 8): adjoint solve + sensitivity
+        s.gradient("pnorm", 8.0)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            s.gradient("pnorm", 8.0)
+        out[name]["grad_seconds"] = (time.perf_counter() - t0) / reps
+        out[name]["grad_iterations"] = s.last_adjoint["iterations"]
         if name == "fp64":
             out["tb_max"] = s.metrics(8.0)["tb_max"]
         s.close()
@@ -349,7 +357,7 @@ def main():
         },
         "thermal": None if thermal is None else {
             "workload": "512x256x12 CHT stack (4 Cu base, 4 channel, 4 Cu lid), rtol 1e-6; channel velocity from "
-                        "the 512x256 density-field SIMPLE solve on the same gamma",
+                        "the 512x256 density-field SIMPLE solve on the same gamma; grad = d(layer-0 8-norm)/d(gamma)",
             **{k: ({kk: (float(f"{vv:.4g}") if isinstance(vv, float) else vv) for kk, vv in d.items()}
                    if isinstance(d, dict) else (float(f"{d:.6g}") if isinstance(d, float) else d))
                for k, d in thermal.items()},
