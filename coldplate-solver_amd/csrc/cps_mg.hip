@@ -823,7 +823,7 @@ __global__ __launch_bounds__(FT) void k_ir_resid(Grid0 G, Geo g, Lv<float> L, co
 // outer test: ||r64||^2 <= max(abs, rtol ||b||)^2 (||b||^2 = rr0 of the first
 // inner solve); one block
 __global__ __launch_bounds__(FT) void k_ir_check(Grid0 G, PcgScal* outer, const PcgScal* first, int o,
-                                                 double abs_tol, double rtol, const double* part) {
+                                                 double abs_tol, double rtol, const double* part, IterState* st) {
   __shared__ double sh[FT / kWave + 1];
   __shared__ int shi;
   if (block_read(&outer->stop, &shi)) return;
@@ -837,11 +837,12 @@ __global__ __launch_bounds__(FT) void k_ir_check(Grid0 G, PcgScal* outer, const 
     outer->rr = rr;
     outer->iters = o + 1;
     if (rr <= outer->tol2) outer->stop = 1;
+    // the step's pressure statistics, as of this sweep (a later sweep that
+    // runs overwrites them; one that is skipped leaves them)
+    st->pcg_iters = first->iters;
+    st->err = rr;
+    st->ir_outer = o + 1;
   }
-}
-
-__global__ void k_outer_reset(PcgScal* outer) {
-  if (threadIdx.x == 0) { outer->stop = 0; outer->iters = 0; }
 }
 
 __global__ void k_copy_state(IterState* st, const PcgScal* inner, const PcgScal* outer, int mixed) {
@@ -852,7 +853,12 @@ __global__ void k_copy_state(IterState* st, const PcgScal* inner, const PcgScal*
   }
 }
 
-__global__ __launch_bounds__(256) void k_copy_f64(const double* src, double* dst, size_t n) {
+// dst = src; block 0 also resets the refinement loop's state
+__global__ __launch_bounds__(256) void k_copy_f64(const double* src, double* dst, size_t n, PcgScal* outer) {
+  if (outer && blockIdx.x == 0 && threadIdx.x == 0) {
+    outer->stop = 0;
+    outer->iters = 0;
+  }
   for (size_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) dst[q] = src[q];
 }
 
@@ -1094,10 +1100,9 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
     m->timer = timer;
     m->setup(dE, dN);
     // a kernel, not a memcpy node, so the step stays graph-capturable
-    k_copy_f64<<<std::min(ceil_div((int)S, 256), 2048), 256, 0, s>>>(r, b64, S);
     PcgScal* inner = m->scal.p;
     PcgScal* outer = m->scal.p + 1;
-    k_outer_reset<<<1, 64, 0, s>>>(outer);
+    k_copy_f64<<<std::min(ceil_div((int)S, 256), 2048), 256, 0, s>>>(r, b64, S, outer);
     const Lv<float> L = m->view(0);
     const double inner_rtol = std::max(cfg.p_rtol, 1e-5);
     const int max_outer = std::max(1, cfg.max_refine);
@@ -1106,7 +1111,7 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
       m->pcg(r, 0.0, inner_rtol, maxit, sync_every, &outer->stop);
       k_ir_add<<<m->lv[0]->grid(), dim3(MBX, MBY), 0, s>>>(L, m->xs.p, x, g_.P, &outer->stop);
       k_ir_resid<<<m->grid0(), m->blk0(), 0, s>>>(m->G0, g_, L, dE, dN, x, b64, r, &outer->stop, m->part.p);
-      k_ir_check<<<1, m->blk0(), 0, s>>>(m->G0, outer, inner, o, cfg.p_tol, cfg.p_rtol, m->part.p);
+      k_ir_check<<<1, m->blk0(), 0, s>>>(m->G0, outer, inner, o, cfg.p_tol, cfg.p_rtol, m->part.p, st);
       CPS_HIP(hipGetLastError());
       if (poll && o + 1 < max_outer) {
         int stop = 0;
@@ -1115,7 +1120,6 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
         if (stop) break;
       }
     }
-    k_copy_state<<<1, 64, 0, s>>>(st, inner, outer, 1);
   }
   CPS_HIP(hipGetLastError());
 }

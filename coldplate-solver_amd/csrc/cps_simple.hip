@@ -243,9 +243,16 @@ __device__ inline void store_mom(const Geo& g, const MomCoef<IsU>& c, const doub
 }
 
 template <class C, bool Delta>
-__global__ __launch_bounds__(NT) void k_assemble(Geo g, Fields f, MomSys<C> su, MomSys<C> sv) {
+__global__ __launch_bounds__(NT) void k_assemble(Geo g, Fields f, MomSys<C> su, MomSys<C> sv, IterState* st) {
   const int j = blockIdx.x * BX + threadIdx.x;
   const int M = g.M, N = g.N;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    // the Jacobi loop's state for this step
+    st->jac_active[0] = g.nU > 0;
+    st->jac_active[1] = g.nV > 0;
+    st->jac_iters[0] = 0;
+    st->jac_iters[1] = 0;
+  }
   for (int i = blockIdx.y * BY + threadIdx.y; i <= M; i += g.gy * BY) {
     if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) {
       MomCoef<true> c = mom_coef<true>(g, f, i, j);
@@ -257,15 +264,6 @@ __global__ __launch_bounds__(NT) void k_assemble(Geo g, Fields f, MomSys<C> su, 
       f.dN[at(g, i, j)] = g.hx / c.aP0;
       store_mom<false, C, Delta>(g, c, f.v, sv, i, j);
     }
-  }
-}
-
-__global__ void k_jac_reset(IterState* st, int au, int av) {
-  if (threadIdx.x == 0) {
-    st->jac_active[0] = au;
-    st->jac_active[1] = av;
-    st->jac_iters[0] = 0;
-    st->jac_iters[1] = 0;
   }
 }
 
@@ -905,12 +903,11 @@ class SimpleSolver {
   void momentum() {
     hipStream_t s = ctx->stream;
     const Fields f = fields();
-    k_jac_reset<<<1, 64, 0, s>>>(st.p, g.nU > 0, g.nV > 0);
     const double tol2 = cfg.momentum_tol * cfg.momentum_tol;
     const int maxit = cfg.max_momentum_iter;
     const int chunk = maxit > kNoSyncCap ? 8 : std::max(1, maxit);
-    if (mode_fp64) k_assemble<double, false><<<grid, block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1));
-    else k_assemble<float, true><<<grid, block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1));
+    if (mode_fp64) k_assemble<double, false><<<grid, block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p);
+    else k_assemble<float, true><<<grid, block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p);
     CPS_HIP(hipGetLastError());
     int k = 0;
     while (k < maxit) {
