@@ -487,17 +487,47 @@ __global__ __launch_bounds__(NT) void k_prhs(Geo g, Fields f, PcgBufs q, double*
 }
 
 // One block: residuals of the step (iterations.cpp:499-533, 628-658,
-// pressure_solver.cpp:489-495) and the PCG start scalars.
+// pressure_solver.cpp:489-495) and the PCG start scalars.  All eight
+// reductions share one pass over the block partials and one barrier; per
+// reduction the order is that of parts_sum / parts_max (thread t adds
+// p[t], p[t + NT], ..., then the wavefronts in order), so the results are
+// the same.
 __global__ __launch_bounds__(NT) void k_stats(Geo g, IterState* st, const double* part) {
-  __shared__ double sh[NT / kWave + 1];
-  const int NB = g.NB;
+  constexpr int NW = NT / kWave;
+  __shared__ double sh[8][NW];
+  const int NB = g.NB, t = thread_id();
   const double* pf = part + (size_t)SLOT_FIN * NB;
   const double* pr = part + (size_t)SLOT_RHS * NB;
-  const double su = parts_sum<NT>(pf, NB, sh), sv = parts_sum<NT>(pf + NB, NB, sh);
-  const double um = parts_max<NT>(pf + 2 * NB, NB, sh), vm = parts_max<NT>(pf + 3 * NB, NB, sh);
-  const double mm = parts_max<NT>(pr, NB, sh), ms = parts_sum<NT>(pr + NB, NB, sh);
-  const double rz = parts_sum<NT>(pr + 2 * NB, NB, sh), bb = parts_sum<NT>(pr + 3 * NB, NB, sh);
-  if (thread_id() == 0) {
+  // su, sv, umax, vmax, mass max, mass sum of squares, r.z, ||b||^2
+  double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int q = t; q < NB; q += NT) {
+    v[0] += pf[q];
+    v[1] += pf[NB + q];
+    v[2] = fmax(v[2], pf[2 * NB + q]);
+    v[3] = fmax(v[3], pf[3 * NB + q]);
+    v[4] = fmax(v[4], pr[q]);
+    v[5] += pr[NB + q];
+    v[6] += pr[2 * NB + q];
+    v[7] += pr[3 * NB + q];
+  }
+  constexpr bool kMax[8] = {false, false, true, true, true, false, false, false};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const double w = kMax[k] ? wave_max(v[k]) : wave_sum(v[k]);
+    if ((t & (kWave - 1)) == 0) sh[k][t / kWave] = w;
+  }
+  __syncthreads();
+  if (t != 0) return;
+  double r[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    double a = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) a = kMax[k] ? (a > sh[k][w] ? a : sh[k][w]) : a + sh[k][w];
+    r[k] = a;
+  }
+  const double su = r[0], sv = r[1], um = r[2], vm = r[3], mm = r[4], ms = r[5], rz = r[6], bb = r[7];
+  {
     st->su = su;
     st->sv = sv;
     st->umax = um;
