@@ -55,6 +55,15 @@ class SimpleConfig(C.Structure):
         ("epsilon", C.c_double),
         ("max_iterations", C.c_int),
         ("dp_fluid_tol", C.c_double),
+        ("x_core_in", C.c_double), ("x_core_out", C.c_double),
+        ("use_dp_convergence", C.c_int),
+        ("dp_window", C.c_int),
+        ("dp_percent", C.c_double),
+        ("use_dp_slope_gate", C.c_int),
+        ("dp_slope_window", C.c_int),
+        ("dp_slope_max_delta", C.c_double),
+        ("inlet_ramp", C.c_int),
+        ("ramp_steps", C.c_int),
     ]
 
     def copy(self):
@@ -94,6 +103,15 @@ def default_config(**overrides):
     c.epsilon = 1e-7
     c.max_iterations = 1000
     c.dp_fluid_tol = 1e-6
+    c.x_core_in, c.x_core_out = 0.01, 0.05
+    c.use_dp_convergence = 1
+    c.dp_window = 500
+    c.dp_percent = 1.0
+    c.use_dp_slope_gate = 0
+    c.dp_slope_window = 1000
+    c.dp_slope_max_delta = 10.0
+    c.inlet_ramp = 0
+    c.ramp_steps = 1000
     for k, v in overrides.items():
         if not hasattr(c, k):
             raise KeyError(k)
@@ -108,6 +126,10 @@ class StepStats(C.Structure):
         ("resid_u_max", C.c_double), ("resid_v_max", C.c_double), ("resid_mass_max", C.c_double),
         ("u_iters", C.c_int), ("v_iters", C.c_int), ("p_iters", C.c_int),
         ("iteration", C.c_int),
+        ("dp_core_total", C.c_double), ("dp_core_static", C.c_double),
+        ("dp_full_total", C.c_double), ("dp_full_static", C.c_double),
+        ("dp_converged", C.c_int),
+        ("pad_", C.c_int),
     ]
 
     def as_dict(self):

@@ -25,11 +25,12 @@ def channel_200x50(**overrides):
     """Plain 2-D SIMPLE, straight channel 200x50 cells, dx = dy = 1e-4 m,
     all fluid, uniform inlet 0.1 m/s, FOU, Jacobi momentum (tol 1e-6, 400
     sweeps), Jacobi-PCG pressure (pTol 1e-8, cap raised to 1000), uvAlpha
-    0.7, pAlpha 0.3, 2.5-D off, 2000 iterations or residual 1e-6."""
+    0.7, pAlpha 0.3, 2.5-D off, 2000 iterations or residual 1e-6 (the
+    dP-window stop is off)."""
     cfg = default_config(
         nx=200, ny=50, hx=1e-4, hy=1e-4, u_in=0.1, convection_scheme=0,
         p_tol=1e-8, max_pressure_iter=1000, pressure_solver=CPS_PRESSURE_PCG,
-        epsilon=1e-6, max_iterations=2000,
+        epsilon=1e-6, max_iterations=2000, use_dp_convergence=0,
     )
     for k, v in overrides.items():
         setattr(cfg, k, v)
@@ -71,15 +72,16 @@ def density_field(nx=1024, ny=512, seed=0, sigma=6.0, h=2e-5, **overrides):
     (the reference RAMP alpha_max (1-g)/(1+q'g) with q' = 1/q = 10 and
     K_min = mu/alpha_max), 2.5-D on with Ht = 1e-4 m (sink 12 mu/Ht^2,
     reference convection factor 1.2), SOU, parabolic inlet 0.5 m/s on rows
-    ny/8..7ny/8, pressure outlet.  The boundary-patch dP averages rows whose
-    adjacent cell has gamma >= 0.5 (the reference's 1 - 1e-6 test selects
-    almost no rows of a projected random field)."""
+    ny/8..7ny/8, pressure outlet, a fixed iteration count (dP-window stop
+    off).  The boundary-patch dP averages rows whose adjacent cell has
+    gamma >= 0.5 (the reference's 1 - 1e-6 test selects almost no rows of a
+    projected random field)."""
     mu = 0.00089
     cfg = default_config(
         nx=nx, ny=ny, hx=h, hy=h, u_in=0.5, convection_scheme=1,
         enable_25d=1, ht=1e-4, conv_factor_25d=1.2, sink_mult_25d=1.0,
         brinkman_q=10.0, brinkman_kmin=mu / 1e8,
-        max_iterations=1000, dp_fluid_tol=0.5,
+        max_iterations=1000, dp_fluid_tol=0.5, use_dp_convergence=0,
     )
     for k, v in overrides.items():
         setattr(cfg, k, v)

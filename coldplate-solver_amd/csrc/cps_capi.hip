@@ -17,7 +17,7 @@ extern "C" {
 
 const char* cps_last_error(void) { return cps::g_last_error.c_str(); }
 
-int cps_abi_version(void) { return 1; }
+int cps_abi_version(void) { return 2; }
 
 int cps_create(int device, cps_ctx** out) {
   return cps::guard([&] {

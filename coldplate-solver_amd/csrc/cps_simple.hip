@@ -16,6 +16,10 @@
 //   k_correct       p += pAlpha p', u/v correction   iterations.cpp:556-600
 //                   with clamps, BCs
 //
+// Outside the step: k_sample reduces the core-plane and boundary-patch
+// pressure drops (postprocessing.cpp:102-327) that the outer loop's dP
+// window (SIMPLE.cpp:310-491) reads back with the residuals.
+//
 // Fields live in HBM as row-major (i = y row, j = x column) arrays with one
 // common pitch P >= N+1 (a multiple of 64 elements), so every kernel
 // reads whole 64-lane rows: one wavefront = 64 consecutive j.  Reductions are
@@ -29,6 +33,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <vector>
 
 #include <chrono>
@@ -749,23 +754,41 @@ __global__ __launch_bounds__(NT) void k_correct(Geo g, Fields f) {
   }
 }
 
-// Boundary patch sums (postprocessing.cpp:245-327), one block.
-__global__ void k_patch(Geo g, Fields f, double tol, int inlet, double* out) {
+// Columns of one pressure-drop sample.  A boundary patch
+// (postprocessing.cpp:245-327) reads one column of each array (left = right,
+// t = 0); a plane (samplePlaneAtX, postprocessing.cpp:102-243) lerps between
+// the two cell columns that bracket x with weight t on the right one.
+struct SampleCols {
+  int gl, gr;  // gamma columns of the pure-fluid row test
+  int pl, pr;  // p columns
+  int ul, ur;  // u columns
+  int vl, vr;  // v columns (rows i - 1 and i)
+  double t;
+};
+struct SampleSet {
+  SampleCols c[4];
+};
+
+// Sums (area, |mass flux|, static * area, dynamic * area, total * |flux|) of
+// one sample per block, over rows i = 1..M-1 whose cells are pure fluid.
+__global__ void k_sample(Geo g, Fields f, double tol, SampleSet set, double* out) {
   __shared__ double sh[256 / kWave];
-  const int M = g.M, N = g.N;
-  const int uCol = inlet ? 0 : N - 1, pCol = inlet ? 0 : N, vCol = inlet ? 0 : N;
-  const int adj = inlet ? 0 : N - 2;
+  const SampleCols c = set.c[blockIdx.x];
+  const int M = g.M;
+  const double w = 1.0 - c.t;
   double sA = 0, dA = 0, tw = 0, area = 0, flux = 0;
   for (int i = 1 + threadIdx.x; i < M; i += blockDim.x) {
-    const int cr = i - 1;
-    if (f.gam[at(g, cr, adj)] < 1.0 - tol) continue;
+    const int row = i - 1;
+    if (f.gam[at(g, row, c.gl)] < 1.0 - tol || f.gam[at(g, row, c.gr)] < 1.0 - tol) continue;
     const double fa = g.hy;
     area += fa;
     const int vTop = i - 1, vBot = min(i, M - 1);
-    const double sp = f.p[at(g, i, pCol)];
-    const double un = f.u[at(g, i, uCol)];
-    const double vt = 0.5 * (f.v[at(g, vTop, vCol)] + f.v[at(g, vBot, vCol)]);
-    const double vm = sqrt(un * un + vt * vt);
+    const double sp = w * f.p[at(g, i, c.pl)] + c.t * f.p[at(g, i, c.pr)];
+    const double un = w * f.u[at(g, i, c.ul)] + c.t * f.u[at(g, i, c.ur)];
+    const double vt = w * f.v[at(g, vTop, c.vl)] + c.t * f.v[at(g, vTop, c.vr)];
+    const double vb = w * f.v[at(g, vBot, c.vl)] + c.t * f.v[at(g, vBot, c.vr)];
+    const double vtan = 0.5 * (vt + vb);
+    const double vm = sqrt(un * un + vtan * vtan);
     const double dyn = 0.5 * g.rho * vm * vm;
     sA += sp * fa;
     dA += dyn * fa;
@@ -774,7 +797,24 @@ __global__ void k_patch(Geo g, Fields f, double tol, int inlet, double* out) {
   }
   sA = block_sum<256>(sA, sh); dA = block_sum<256>(dA, sh); tw = block_sum<256>(tw, sh);
   area = block_sum<256>(area, sh); flux = block_sum<256>(flux, sh);
-  if (threadIdx.x == 0) { out[0] = area; out[1] = flux; out[2] = sA; out[3] = dA; out[4] = tw; }
+  double* o = out + 5 * blockIdx.x;
+  if (threadIdx.x == 0) { o[0] = area; o[1] = flux; o[2] = sA; o[3] = dA; o[4] = tw; }
+}
+
+// inlet = s * inlet0 (the ramped inlet of SIMPLE.cpp:381-386)
+__global__ void k_inlet_scale(int n, double s, const double* __restrict__ inlet0, double* __restrict__ inlet) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) inlet[i] = s * inlet0[i];
+}
+
+// Averages of one sample from its k_sample sums (postprocessing.cpp:226-240).
+static void finish_sample(const double* h, cps_patch* out) {
+  out->area = h[0];
+  out->mass_flux = h[1];
+  out->avg_static = h[0] > 0 ? h[2] / h[0] : 0.0;
+  out->avg_dynamic = h[0] > 0 ? h[3] / h[0] : 0.0;
+  out->valid = h[0] > 0;
+  out->avg_total = h[1] > 0 ? h[4] / h[1] : (h[0] > 0 ? out->avg_static + out->avg_dynamic : 0.0);
 }
 
 // ===========================================================================
@@ -787,7 +827,7 @@ class SimpleSolver {
   Geo g;
   int mode_fp64 = 1;
   size_t S = 0;  // elements per padded field
-  DevBuf<double> u, v, p, us, vs, ps, dE, dN, gam, alp, inlet;
+  DevBuf<double> u, v, p, us, vs, ps, dE, dN, gam, alp, inlet, inlet0;
   DevBuf<double> momD[16];  // FP64 coefficient arrays (x-form)
   DevBuf<float> momF[16];   // FP32 coefficient arrays (delta form)
   DevBuf<double> pcg[6];
@@ -841,6 +881,7 @@ class SimpleSolver {
 
     for (auto* b : {&u, &v, &p, &us, &vs, &ps, &dE, &dN, &gam, &alp}) b->alloc(S);
     inlet.alloc(M + 1);
+    inlet0.alloc(M + 1);
     timer.stream = ctx->stream;
     for (auto* b : {&u, &v, &p, &us, &vs, &ps, &dE, &dN, &gam, &alp})
       CPS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), ctx->stream));
@@ -886,13 +927,78 @@ class SimpleSolver {
       hinlet[i] = hg[(size_t)cr * g.P] > (double)0.5f ? speed : 0.0;
     }
     CPS_HIP(hipMemcpyAsync(inlet.p, hinlet.data(), (M + 1) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    CPS_HIP(hipMemcpyAsync(inlet0.p, hinlet.data(), (M + 1) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    CPS_CHECK(!cfg.use_dp_convergence || cfg.dp_window >= 1, CPS_ERR_ARG, "dp_window must be >= 1");
+    CPS_CHECK(!cfg.use_dp_slope_gate || cfg.dp_slope_window >= 0, CPS_ERR_ARG, "dp_slope_window must be >= 0");
+    monitor.c[0] = plane_cols(cfg.x_core_in);
+    monitor.c[1] = plane_cols(cfg.x_core_out);
+    monitor.c[2] = patch_cols(1);
+    monitor.c[3] = patch_cols(0);
     if (cfg.pressure_solver == CPS_PRESSURE_GMG) mg_create();
     CPS_HIP(hipStreamSynchronize(ctx->stream));
   }
 
   std::vector<double> hinlet;
+  SampleSet monitor{};       // core in / out planes, inlet / outlet patches
+  double inlet_scale = 1.0;  // current inlet ramp factor
 
   ~SimpleSolver() { flow_adjoint_free(adjoint); }
+
+  // sampleBoundaryPatch columns (postprocessing.cpp:258-262)
+  SampleCols patch_cols(int inl) const {
+    const int N = g.N;
+    SampleCols c{};
+    c.gl = c.gr = inl ? 0 : N - 2;
+    c.pl = c.pr = inl ? 0 : N;
+    c.ul = c.ur = inl ? 0 : N - 1;
+    c.vl = c.vr = inl ? 0 : N;
+    c.t = 0.0;
+    return c;
+  }
+  // samplePlaneAtX columns: cell columns cl, cl + 1 whose centres bracket x,
+  // clamped to the first / last centre (postprocessing.cpp:62-96); p sits
+  // one column right of its cell, u and v at the cell's own column index.
+  SampleCols plane_cols(double x) const {
+    const int NCells = g.N - 1;
+    int cl = 0;
+    double t = 0.0;
+    if (NCells > 1) {
+      const double xc = std::max(0.5 * g.hx, std::min(x, (NCells - 0.5) * g.hx));
+      cl = (int)std::floor(xc / g.hx - 0.5);
+      cl = std::max(0, std::min(cl, NCells - 2));
+      const double xl = (cl + 0.5) * g.hx, xr = (cl + 1 + 0.5) * g.hx;
+      t = std::fabs(xr - xl) < (double)1e-12f ? 0.0 : (xc - xl) / (xr - xl);
+      t = std::max(0.0, std::min(1.0, t));
+    }
+    const int cr = NCells > 1 ? cl + 1 : cl;
+    SampleCols c{};
+    c.gl = cl;
+    c.gr = cr;
+    c.pl = cl + 1;
+    c.pr = cr + 1;
+    c.ul = cl;
+    c.ur = cr;
+    c.vl = cl;
+    c.vr = cr;
+    c.t = t;
+    return c;
+  }
+  // the four samples into st->mon
+  void sample() {
+    k_sample<<<4, 256, 0, ctx->stream>>>(g, fields(), cfg.dp_fluid_tol, monitor, &st.p->mon[0][0]);
+    CPS_HIP(hipGetLastError());
+  }
+  // ramped inlet (SIMPLE.cpp:381-386) and the velocity clamp that follows it
+  // (iterations.cpp:104, momentum_solver.cpp:186)
+  void set_inlet_scale(double sc) {
+    if (sc == inlet_scale) return;
+    inlet_scale = sc;
+    g.maxVel = 3.0 * std::max(std::fabs(sc * cfg.u_in), kTenthF);
+    const int n = g.M + 1;
+    k_inlet_scale<<<ceil_div(n, 256), 256, 0, ctx->stream>>>(n, sc, inlet0.p, inlet.p);
+    CPS_HIP(hipGetLastError());
+    apply_bc();
+  }
 
   SimpleView view() const {
     return SimpleView{g, S, cfg, u.p, v.p, p.p, gam.p, alp.p, inlet.p, dE.p, dN.p};
@@ -1054,30 +1160,92 @@ class SimpleSolver {
     out->v_iters = h.jac_iters[1];
     out->p_iters = h.pcg_iters;
     out->iteration = iteration;
+    cps_patch m[4];
+    for (int k = 0; k < 4; ++k) finish_sample(h.mon[k], &m[k]);
+    out->dp_core_total = m[0].avg_total - m[1].avg_total;
+    out->dp_core_static = m[0].avg_static - m[1].avg_static;
+    out->dp_full_total = m[2].avg_total - m[3].avg_total;
+    out->dp_full_static = m[2].avg_static - m[3].avg_static;
+    out->dp_converged = 0;
   }
 
   void step(int n, cps_step_stats* out) {
     apply_bc();
     for (int k = 0; k < n; ++k) step_once();
+    sample();
     cps_step_stats tmp;
     stats(out ? out : &tmp);
     const double e = (out ? out : &tmp)->error;
     CPS_CHECK(std::isfinite(e), CPS_ERR_DIVERGED, "SIMPLE residual is not finite");
   }
 
+  // SIMPLE.cpp:310-491 without logging, checkpoints and the pseudo-time
+  // controller.  The four drops are reduced on the device after every
+  // iteration and read back with the residuals; the window is host state.
   void run(cps_step_stats* out) {
     apply_bc();
     cps_step_stats sst{};
     sst.error = 1.0;
+    const int win = std::max(1, cfg.dp_window);
+    std::vector<double> hist(win, 0.0);
+    int idx = 0;
+    bool filled = false, conv = false;
+    std::deque<std::pair<int, double>> slope;
     int it = 0;
-    // SIMPLE.cpp:378: while (iter < maxIterations && error > epsilon)
-    while (it < cfg.max_iterations && sst.error > cfg.epsilon) {
+    while (it < cfg.max_iterations && sst.error > cfg.epsilon && !conv) {
+      set_inlet_scale(cfg.inlet_ramp && cfg.ramp_steps > 0 && it < cfg.ramp_steps
+                          ? double(it + 1) / double(cfg.ramp_steps)
+                          : 1.0);
       step_once();
+      sample();
       stats(&sst);
       CPS_CHECK(std::isfinite(sst.error), CPS_ERR_DIVERGED, "SIMPLE residual is not finite");
+      const double core = sst.dp_core_total;
+      if (cfg.use_dp_slope_gate && cfg.dp_slope_window > 0) {
+        slope.emplace_back(it, core);
+        if (slope.size() > (size_t)cfg.dp_slope_window) slope.pop_front();
+      }
+      if (cfg.use_dp_convergence && it >= cfg.ramp_steps) {
+        hist[idx] = core;
+        idx = (idx + 1) % win;
+        if (idx == 0) filled = true;
+        if (filled) conv = window_settled(hist, idx, slope);
+      }
       ++it;
     }
+    sst.dp_converged = conv;
     if (out) *out = sst;
+  }
+
+  // SIMPLE.cpp:456-490: standard deviation of the window and distance of the
+  // newest entry to its mean, both in % of max(|mean|, 1); optional
+  // least-squares slope gate (SIMPLE.cpp:347-366).
+  bool window_settled(const std::vector<double>& hist, int idx, const std::deque<std::pair<int, double>>& slope) const {
+    const int win = (int)hist.size();
+    double sum = 0.0;
+    for (double x : hist) sum += x;
+    const double mean = sum / win;
+    double var = 0.0;
+    for (double x : hist) var += (x - mean) * (x - mean);
+    const double den = std::max(std::fabs(mean), 1.0);
+    const double sdPct = std::sqrt(var / win) / den * 100.0;
+    const double lastPct = std::fabs(hist[(idx - 1 + win) % win] - mean) / den * 100.0;
+    bool slopeOk = true;
+    if (cfg.use_dp_slope_gate && cfg.dp_slope_window > 0 && slope.size() >= (size_t)cfg.dp_slope_window) {
+      double sx = 0, sy = 0, sxx = 0, sxy = 0;
+      const double x0 = slope.front().first, n = (double)slope.size();
+      for (const auto& e : slope) {
+        const double x = e.first - x0;
+        sx += x;
+        sy += e.second;
+        sxx += x * x;
+        sxy += x * e.second;
+      }
+      const double d = n * sxx - sx * sx;
+      const double s = std::fabs(d) < (double)1e-12f ? 0.0 : (n * sxy - sx * sy) / d;
+      slopeOk = std::fabs(s) <= cfg.dp_slope_max_delta / cfg.dp_slope_window;
+    }
+    return sdPct < cfg.dp_percent && lastPct < cfg.dp_percent && slopeOk;
   }
 
   DevBuf<double>* field(int which) {
@@ -1123,21 +1291,20 @@ class SimpleSolver {
                              hipMemcpyHostToDevice, ctx->stream));
     CPS_HIP(hipStreamSynchronize(ctx->stream));
   }
-  void patch(int inl, cps_patch* out) {
+  void sample_one(const SampleCols& c, cps_patch* out) {
     DevBuf<double> o;
     o.alloc(5);
-    k_patch<<<1, 256, 0, ctx->stream>>>(g, fields(), cfg.dp_fluid_tol, inl, o.p);
+    SampleSet set{};
+    set.c[0] = c;
+    k_sample<<<1, 256, 0, ctx->stream>>>(g, fields(), cfg.dp_fluid_tol, set, o.p);
     CPS_HIP(hipGetLastError());
     double h[5];
     CPS_HIP(hipMemcpyAsync(h, o.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
     CPS_HIP(hipStreamSynchronize(ctx->stream));
-    out->area = h[0];
-    out->mass_flux = h[1];
-    out->avg_static = h[0] > 0 ? h[2] / h[0] : 0.0;
-    out->avg_dynamic = h[0] > 0 ? h[3] / h[0] : 0.0;
-    out->valid = h[0] > 0;
-    out->avg_total = h[1] > 0 ? h[4] / h[1] : (h[0] > 0 ? out->avg_static + out->avg_dynamic : 0.0);
+    finish_sample(h, out);
   }
+  void patch(int inl, cps_patch* out) { sample_one(patch_cols(inl), out); }
+  void plane(double x, cps_patch* out) { sample_one(plane_cols(x), out); }
 };
 
 // ---------------------------------------------------------------------------
@@ -1324,6 +1491,16 @@ void cps_simple_config_default(cps_simple_config* c) {
   c->epsilon = 1e-7;
   c->max_iterations = 1000;
   c->dp_fluid_tol = 1e-6;
+  c->x_core_in = 0.01;
+  c->x_core_out = 0.05;
+  c->use_dp_convergence = 1;
+  c->dp_window = 500;
+  c->dp_percent = 1.0;
+  c->use_dp_slope_gate = 0;
+  c->dp_slope_window = 1000;
+  c->dp_slope_max_delta = 10.0;
+  c->inlet_ramp = 0;
+  c->ramp_steps = 1000;
 }
 
 int cps_simple_create(cps_ctx* ctx, const cps_simple_config* cfg, const double* gamma, const double* inlet_u,
@@ -1373,6 +1550,14 @@ int cps_simple_set_field(cps_simple* s, int field, const double* in) {
   return guard([&] {
     CPS_CHECK(s, CPS_ERR_ARG, "null solver");
     s->s->set(field, in);
+  });
+}
+
+int cps_simple_plane(cps_simple* s, double x, cps_patch* out) {
+  return guard([&] {
+    CPS_CHECK(s && out, CPS_ERR_ARG, "null argument");
+    CPS_CHECK(std::isfinite(x), CPS_ERR_ARG, "x must be finite");
+    s->s->plane(x, out);
   });
 }
 

@@ -33,6 +33,7 @@ struct IterState {
   double resid_u, resid_v, resid_mass, error;
   int ir_outer;               // refinement sweeps done (mixed pressure)
   int pad_;
+  double mon[4][5];           // k_sample sums: core in / out planes, inlet / outlet patches
 };
 
 // Partial-sum slots (each NB doubles) of the SIMPLE step kernels.

@@ -63,6 +63,9 @@ class SIMPLE:
         return self.step(1)["error"]
 
     def runIterations(self):
+        """SIMPLE::runIterations (SIMPLE.cpp:310-491): stop on error <=
+        epsilon, max_iterations or the settled core-dP window; the returned
+        stats carry the four pressure drops and `dp_converged`."""
         st = T.StepStats()
         _capi.check(self.L.cps_simple_run(self.h, C.byref(st)))
         self.last = st.as_dict()
@@ -88,6 +91,12 @@ class SIMPLE:
     def sampleBoundaryPatch(self, inlet):
         m = T.Patch()
         _capi.check(self.L.cps_simple_boundary_patch(self.h, 1 if inlet else 0, C.byref(m)))
+        return m.as_dict()
+
+    def samplePlaneAtX(self, x):
+        """Plane metrics at physical x [m] (postprocessing.cpp:102-243)."""
+        m = T.Patch()
+        _capi.check(self.L.cps_simple_plane(self.h, float(x), C.byref(m)))
         return m.as_dict()
 
     # ---- adjoint ---------------------------------------------------------

@@ -106,6 +106,16 @@ typedef struct cps_simple_config {
   double epsilon;            /* SIMPLE.h:68 */
   int max_iterations;        /* SIMPLE.h:69 */
   double dp_fluid_tol;       /* pure-fluid test of sampleBoundaryPatch (postprocessing.cpp:268) */
+  /* outer loop of cps_simple_run (SIMPLE.cpp:310-491) */
+  double x_core_in, x_core_out;  /* xPlaneCoreInlet / Outlet [m] (SIMPLE.h:88-89) */
+  int use_dp_convergence;    /* usePressureDropConvergence (SIMPLE.h:98) */
+  int dp_window;             /* dpConvergenceWindow (SIMPLE.h:99), >= 1 */
+  double dp_percent;         /* dpConvergencePercent (SIMPLE.h:100) */
+  int use_dp_slope_gate;     /* usePressureDropSlopeGate (SIMPLE.h:101) */
+  int dp_slope_window;       /* dpSlopeWindowIters (SIMPLE.h:102) */
+  double dp_slope_max_delta; /* dpSlopeMaxDelta [Pa] (SIMPLE.h:103) */
+  int inlet_ramp;            /* enableInletRamp (SIMPLE.h:385) */
+  int ramp_steps;            /* rampSteps (SIMPLE.h:386); also gates the dP window */
 } cps_simple_config;
 
 /* Per-iteration statistics (iterations.cpp:499-533, 628-658). */
@@ -115,6 +125,12 @@ typedef struct cps_step_stats {
   double resid_u_max, resid_v_max, resid_mass_max;
   int u_iters, v_iters, p_iters;
   int iteration;             /* SIMPLE iterations done so far on this object */
+  /* pressure drops after the last iteration (SIMPLE.cpp:421-439): core =
+   * samplePlaneAtX(x_core_in) - samplePlaneAtX(x_core_out), full = inlet -
+   * outlet sampleBoundaryPatch; total = mass-flux-weighted total pressure */
+  double dp_core_total, dp_core_static, dp_full_total, dp_full_static;
+  int dp_converged;          /* cps_simple_run stopped on the dP window */
+  int pad_;
 } cps_step_stats;
 
 /* Boundary patch metrics (postprocessing.cpp:245-327). */
@@ -146,13 +162,27 @@ void cps_simple_destroy(cps_simple* s);
 /* n_steps SIMPLE iterations (SIMPLE::calculateStep, iterations.cpp:82-659);
  * stats of the last one in *last (may be NULL). */
 int cps_simple_step(cps_simple* s, int n_steps, cps_step_stats* last);
-/* Iterate until error <= epsilon or max_iterations (SIMPLE.cpp:378); the
- * dP-window stop and pseudo-time controller are off. */
+/* SIMPLE::runIterations (SIMPLE.cpp:310-491): iterate while iteration <
+ * max_iterations, error > epsilon and the core dP window has not settled.
+ * With inlet_ramp the inlet speed (and the velocity clamp) is scaled by
+ * (k+1)/ramp_steps on iteration k < ramp_steps.  From iteration ramp_steps
+ * on, each core dP enters a window of dp_window values; the run stops once
+ * the window is full, its standard deviation and the newest value's distance
+ * to its mean are both below dp_percent % of max(|mean|, 1) and, with
+ * use_dp_slope_gate, the least-squares slope of the last dp_slope_window
+ * core dPs is at most dp_slope_max_delta / dp_slope_window.  The
+ * pseudo-time controller (enablePseudoTimeStepping, off by default) is not
+ * built.  Iteration counts restart at 0 on every call, as in the reference. */
 int cps_simple_run(cps_simple* s, cps_step_stats* last);
 int cps_simple_get_field(cps_simple* s, int field, double* out);
 int cps_simple_set_field(cps_simple* s, int field, const double* in);
 /* inlet = 1: x = 0 patch, 0: x = L patch. */
 int cps_simple_boundary_patch(cps_simple* s, int inlet, cps_patch* out);
+/* samplePlaneAtX (postprocessing.cpp:102-243): p, u and v interpolated
+ * linearly between the two cell columns whose centres bracket x (clamped to
+ * the first / last centre), over rows where both cells are pure fluid
+ * (gamma >= 1 - dp_fluid_tol). */
+int cps_simple_plane(cps_simple* s, double x, cps_patch* out);
 
 /* Discrete adjoint of the converged state (no reference counterpart: the
  * reference optimiser's sensitivities ignore the flow, mma_optimizer.py:

@@ -60,6 +60,16 @@ void toStats(const oracle::StepStats& st, int iteration, cps_step_stats* out) {
   out->iteration = iteration;
 }
 
+template <class D>
+void toDrops(const D& d, int converged, cps_step_stats* out) {
+  if (!out) return;
+  out->dp_core_total = d.core_total;
+  out->dp_core_static = d.core_static;
+  out->dp_full_total = d.full_total;
+  out->dp_full_static = d.full_static;
+  out->dp_converged = converged;
+}
+
 // Stand-alone CSR solvers on double inputs (prec 32 runs them in float).
 template <class R>
 int jacobiT(int n, const int* rp, const int* ci, const double* val, const double* b, double* x, double tol,
@@ -111,26 +121,44 @@ int orc_simple_step(void* hp, int n, cps_step_stats* out, double* seconds) {
   auto t1 = std::chrono::steady_clock::now();
   if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
   toStats(st, n, out);
+  with(h, [&](auto& s) { toDrops(s.drops(), 0, out); return 0; });
   return n;
 }
 
-// Iterate until error <= epsilon or max_iterations (SIMPLE.cpp:378).
-int orc_simple_run(void* hp, cps_step_stats* out, double* history, int history_len) {
+// runIterations (SIMPLE.cpp:310-491); history / dp_history receive the
+// error and the core total-pressure drop of the first history_len iterations.
+int orc_simple_run(void* hp, cps_step_stats* out, double* history, double* dp_history, int history_len) {
   auto* h = static_cast<Handle*>(hp);
   return with(h, [&](auto& s) {
-    using R = std::remove_reference_t<decltype(s.u[0])>;
-    int iter = 0;
-    R error = R(1);
     oracle::StepStats st;
-    while (iter < s.maxIterations && error > s.epsilon) {
-      st = s.calculateStep();
-      error = R(st.error);
-      if (history && iter < history_len) history[iter] = st.error;
-      iter++;
-    }
-    toStats(st, iter, out);
-    return iter;
+    typename std::remove_reference_t<decltype(s)>::DropStats dl;
+    bool conv = false;
+    const int iters = s.runIterations(st, dl, conv, [&](int it, const oracle::StepStats& x, const auto& d) {
+      if (it < history_len) {
+        if (history) history[it] = x.error;
+        if (dp_history) dp_history[it] = d.core_total;
+      }
+    });
+    toStats(st, iters, out);
+    toDrops(dl, conv ? 1 : 0, out);
+    return iters;
   });
+}
+
+// samplePlaneAtX (postprocessing.cpp:102-243) at the current fields.
+int orc_simple_plane(void* hp, double x, cps_patch* out) {
+  auto* h = static_cast<Handle*>(hp);
+  oracle::PatchMetrics m = with(h, [&](auto& s) {
+    using R = std::remove_reference_t<decltype(s.u[0])>;
+    return s.planeAtX(R(x));
+  });
+  out->area = m.area;
+  out->mass_flux = m.mass_flux;
+  out->avg_static = m.avg_static;
+  out->avg_dynamic = m.avg_dynamic;
+  out->avg_total = m.avg_total;
+  out->valid = m.valid;
+  return 0;
 }
 
 int orc_simple_get_field(void* hp, int field, double* out) {

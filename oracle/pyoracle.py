@@ -34,7 +34,8 @@ def lib(types):
     L.orc_simple_create.argtypes = [C.c_int, P(types.SimpleConfig), dp, dp]
     L.orc_simple_destroy.argtypes = [C.c_void_p]
     L.orc_simple_step.argtypes = [C.c_void_p, C.c_int, P(types.StepStats), dp]
-    L.orc_simple_run.argtypes = [C.c_void_p, P(types.StepStats), dp, C.c_int]
+    L.orc_simple_run.argtypes = [C.c_void_p, P(types.StepStats), dp, dp, C.c_int]
+    L.orc_simple_plane.argtypes = [C.c_void_p, C.c_double, P(types.Patch)]
     L.orc_simple_get_field.argtypes = [C.c_void_p, C.c_int, dp]
     L.orc_simple_set_field.argtypes = [C.c_void_p, C.c_int, dp]
     L.orc_simple_boundary_patch.argtypes = [C.c_void_p, C.c_int, P(types.Patch)]
@@ -112,11 +113,18 @@ class OracleSimple:
     def run(self, history_len=0):
         st = self.T.StepStats()
         hist = np.zeros(max(history_len, 1))
-        it = self.L.orc_simple_run(self.h, C.byref(st), _d(hist), history_len)
+        dph = np.zeros(max(history_len, 1))
+        it = self.L.orc_simple_run(self.h, C.byref(st), _d(hist), _d(dph), history_len)
         d = st.as_dict()
         d["iterations"] = it
         d["history"] = hist[: min(it, history_len)]
+        d["dp_history"] = dph[: min(it, history_len)]
         return d
+
+    def plane(self, x):
+        m = self.T.Patch()
+        self.L.orc_simple_plane(self.h, float(x), C.byref(m))
+        return m.as_dict()
 
     def patch(self, inlet):
         m = self.T.Patch()

@@ -23,8 +23,10 @@
 //   pressure_solver.cpp:121-184   Jacobi-PCG with absolute 2-norm tolerance
 //   pressure_solver.cpp:819-919   red-black SOR fallback
 //   iterations.cpp:82-659         one SIMPLE step (implicit momentum branch)
+//   postprocessing.cpp:62-243     samplePlaneAtX (core-plane dP)
 //   postprocessing.cpp:245-327    sampleBoundaryPatch (full-system dP)
-//   SIMPLE.cpp:378                outer stop rule error <= epsilon
+//   SIMPLE.cpp:310-491            outer loop: inlet ramp, stop on error <=
+//                                 epsilon or on the core-dP window (+ slope)
 //
 // Extensions the reference does not have (documented conventions):
 //   * an optional per-u-row inlet profile (parabolic inlet); the reference's
@@ -94,6 +96,17 @@ class Simple {
   int maxIterations = 1000;
   R dpFluidTol = R(1e-6f);
   bool hasInletProfile = false;
+  // outer loop (SIMPLE.h:88-103, 385-386)
+  R xCoreIn = R(0.01f), xCoreOut = R(0.05f);
+  bool usePressureDropConvergence = true;
+  int dpConvergenceWindow = 500;
+  R dpConvergencePercent = R(1.0f);
+  bool usePressureDropSlopeGate = false;
+  int dpSlopeWindowIters = 1000;
+  R dpSlopeMaxDelta = R(10.0f);
+  bool enableInletRamp = false;
+  int rampSteps = 1000;
+  R inletScale = R(1);  // ramp factor of the inlet profile
 
   // ---- fields ----
   std::vector<R> u, v, p, uStar, vStar, pStar, uOld, vOld, dE, dN;
@@ -146,6 +159,16 @@ class Simple {
     epsilon = R(c.epsilon);
     maxIterations = c.max_iterations;
     dpFluidTol = R(c.dp_fluid_tol);
+    xCoreIn = R(c.x_core_in);
+    xCoreOut = R(c.x_core_out);
+    usePressureDropConvergence = c.use_dp_convergence != 0;
+    dpConvergenceWindow = c.dp_window;
+    dpConvergencePercent = R(c.dp_percent);
+    usePressureDropSlopeGate = c.use_dp_slope_gate != 0;
+    dpSlopeWindowIters = c.dp_slope_window;
+    dpSlopeMaxDelta = R(c.dp_slope_max_delta);
+    enableInletRamp = c.inlet_ramp != 0;
+    rampSteps = c.ramp_steps;
 
     // SIMPLE.cpp:78-79
     R K_min = std::max(brinkmanKMin, R(1e-20f));
@@ -230,7 +253,7 @@ class Simple {
   }
   R inletSpeed(int uRow) const {
     // uniform reference inlet uses the (possibly ramped) inletVelocity
-    return hasInletProfile ? inletProfile[uRow] : inletVelocity;
+    return hasInletProfile ? inletScale * inletProfile[uRow] : inletVelocity;
   }
   // boundaries.cpp:123-146
   void setPressureBC(std::vector<R>& pp) {
@@ -785,6 +808,150 @@ class Simple {
     if (flux > R(0)) m.avg_total = tw / flux;
     else if (area > R(0)) m.avg_total = m.avg_static + m.avg_dynamic;
     return m;
+  }
+
+  // postprocessing.cpp:62-96: the cell columns cl, cl + 1 whose centres
+  // (c + 1/2) hx bracket x (clamped to the first / last centre), t in [0, 1].
+  void bracket(R x, int& cl, R& t) const {
+    const int NCells = N - 1;
+    cl = 0;
+    t = R(0);
+    if (NCells <= 1) return;
+    const R xc = std::max(R(0.5f) * hx, std::min(x, (R(NCells) - R(0.5f)) * hx));
+    cl = (int)std::floor(xc / hx - R(0.5f));
+    cl = std::max(0, std::min(cl, NCells - 2));
+    const R xl = (R(cl) + R(0.5f)) * hx, xr = (R(cl + 1) + R(0.5f)) * hx;
+    t = std::abs(xr - xl) < R(1e-12f) ? R(0) : (xc - xl) / (xr - xl);
+    t = std::max(R(0), std::min(R(1), t));
+  }
+
+  // postprocessing.cpp:102-243 (samplePlaneAtX): p at columns cl+1, cl+2,
+  // u at cl, cl+1, v at cl, cl+1 (rows i-1, i), each lerped by t; rows where
+  // both bracketing cells are pure fluid.
+  PatchMetrics planeAtX(R x) {
+    PatchMetrics m;
+    const int MCells = M - 1, NCells = N - 1;
+    if (MCells <= 0 || NCells <= 0 || !(hx > R(0)) || !(hy > R(0))) return m;
+    int cl;
+    R t;
+    bracket(x, cl, t);
+    const int cr = NCells > 1 ? cl + 1 : cl;
+    const R w = R(1.0f) - t;
+    R sA = 0, dA = 0, tw = 0, area = 0, flux = 0;
+    for (int i = 1; i < M; ++i) {
+      const int row = i - 1;
+      if (G(row, cl) < R(1.0f) - dpFluidTol || G(row, cr) < R(1.0f) - dpFluidTol) continue;
+      const R fa = hy;
+      area += fa;
+      const R sp = w * P(i, cl + 1) + t * P(i, cr + 1);
+      const R un = w * U(i, cl) + t * U(i, cr);
+      const int vTop = std::max(0, std::min(i - 1, M - 1));
+      const int vBot = std::max(0, std::min(i, M - 1));
+      const R vt = w * V(vTop, cl) + t * V(vTop, cr);
+      const R vb = w * V(vBot, cl) + t * V(vBot, cr);
+      const R vtan = R(0.5f) * (vt + vb);
+      const R vm = std::sqrt(un * un + vtan * vtan);
+      const R dyn = R(0.5f) * rho * vm * vm;
+      sA += sp * fa;
+      dA += dyn * fa;
+      const R fl = std::abs(rho * un * fa);
+      if (fl > R(0)) { flux += fl; tw += (sp + dyn) * fl; }
+    }
+    m.area = area;
+    m.mass_flux = flux;
+    if (area > R(0)) { m.avg_static = sA / area; m.avg_dynamic = dA / area; m.valid = 1; }
+    if (flux > R(0)) m.avg_total = tw / flux;
+    else if (area > R(0)) m.avg_total = m.avg_static + m.avg_dynamic;
+    return m;
+  }
+
+  struct DropStats {
+    double core_total = 0, core_static = 0, full_total = 0, full_static = 0;
+  };
+  // SIMPLE.cpp:421-439
+  DropStats drops() {
+    const PatchMetrics ci = planeAtX(xCoreIn), co = planeAtX(xCoreOut);
+    const PatchMetrics fi = boundaryPatch(true), fo = boundaryPatch(false);
+    DropStats d;
+    d.core_total = (double)(R(ci.avg_total) - R(co.avg_total));
+    d.core_static = (double)(R(ci.avg_static) - R(co.avg_static));
+    d.full_total = (double)(R(fi.avg_total) - R(fo.avg_total));
+    d.full_static = (double)(R(fi.avg_static) - R(fo.avg_static));
+    return d;
+  }
+
+  // SIMPLE.cpp:347-366: least-squares slope of (iteration, core dP) pairs,
+  // iterations taken relative to the first.
+  static R dpSlope(const std::vector<std::pair<int, R>>& buf) {
+    if (buf.size() < 2) return R(0);
+    const R x0 = R(buf.front().first), n = R(buf.size());
+    R sx = 0, sy = 0, sxx = 0, sxy = 0;
+    for (const auto& e : buf) {
+      const R x = R(e.first) - x0, y = e.second;
+      sx += x;
+      sy += y;
+      sxx += x * x;
+      sxy += x * y;
+    }
+    const R den = n * sxx - sx * sx;
+    if (std::abs(den) < R(1e-12f)) return R(0);
+    return (n * sxy - sx * sy) / den;
+  }
+
+  // SIMPLE.cpp:310-491 (runIterations) without logging, checkpoints and the
+  // pseudo-time controller (enablePseudoTimeStepping off): inlet ramp, one
+  // calculateStep, the four dPs, the core-dP window and slope gate.
+  // trace(iter, stats, drops) is called after every iteration.
+  template <class F>
+  int runIterations(StepStats& st, DropStats& dl, bool& dpConverged, F&& trace) {
+    int iter = 0;
+    R error = R(1.0f);
+    const int win = std::max(1, dpConvergenceWindow);
+    std::vector<R> hist(win, R(0));
+    int idx = 0;
+    bool filled = false;
+    dpConverged = false;
+    std::vector<std::pair<int, R>> slopeBuf;
+    while (iter < maxIterations && error > epsilon && !dpConverged) {
+      inletScale = (enableInletRamp && rampSteps > 0 && iter < rampSteps) ? R(iter + 1) / R(rampSteps) : R(1);
+      inletVelocity = inletScale * targetVel;
+      st = calculateStep();
+      error = R(st.error);
+      dl = drops();
+      const R core = R(dl.core_total);
+      if (usePressureDropSlopeGate && dpSlopeWindowIters > 0) {
+        slopeBuf.emplace_back(iter, core);
+        if (slopeBuf.size() > (size_t)dpSlopeWindowIters) slopeBuf.erase(slopeBuf.begin());
+      }
+      if (usePressureDropConvergence && iter >= rampSteps) {
+        hist[idx] = core;
+        idx = (idx + 1) % win;
+        if (idx == 0) filled = true;
+        if (filled) {
+          double sum = 0.0;
+          for (int k = 0; k < win; ++k) sum += (double)hist[k];
+          const double mean = sum / double(win);
+          double var = 0.0;
+          for (int k = 0; k < win; ++k) {
+            const double d = (double)hist[k] - mean;
+            var += d * d;
+          }
+          const double sd = std::sqrt(var / double(win));
+          const double den = std::max(std::abs(mean), 1.0);
+          const double stdPct = sd / den * 100.0;
+          const double latest = (double)hist[(idx - 1 + win) % win];
+          const double lastPct = std::abs(latest - mean) / den * 100.0;
+          bool slopeOk = true;
+          if (usePressureDropSlopeGate && dpSlopeWindowIters > 0 && slopeBuf.size() >= (size_t)dpSlopeWindowIters)
+            slopeOk = std::abs(dpSlope(slopeBuf)) <= dpSlopeMaxDelta / R(dpSlopeWindowIters);
+          if (stdPct < (double)dpConvergencePercent && lastPct < (double)dpConvergencePercent && slopeOk)
+            dpConverged = true;
+        }
+      }
+      trace(iter, st, dl);
+      iter++;
+    }
+    return iter;
   }
 };
 

@@ -38,6 +38,58 @@ def test_density_field_steps_match_oracle(pkg, orc, seed, nx, ny):
     assert np.array_equal(gpu.get("gamma"), ref.get("gamma"))
 
 
+def test_plane_and_patch_drops_match_oracle(pkg, orc):
+    """samplePlaneAtX at interior, cell-centre and out-of-domain x, and the
+    four drops the step reports, against the oracle after a few steps of a
+    density field (rows tested against gamma >= 0.5)."""
+    cfg, gamma, inlet = pkg.configs.density_field(nx=64, ny=33, seed=3, sigma=3.0)
+    cfg.pressure_solver = pkg._types.CPS_PRESSURE_PCG
+    cfg.max_pressure_iter, cfg.p_tol = 300, 1e-10
+    cfg.x_core_in, cfg.x_core_out = 0.3e-3, 1.0e-3
+    gpu, ref = _pair(pkg, orc, cfg, gamma, inlet)
+    sg, sr = gpu.step(4), ref.step(4)
+    for key in ("dp_core_total", "dp_core_static", "dp_full_total", "dp_full_static"):
+        assert sg[key] == pytest.approx(sr[key], rel=1e-9), key
+    # the lerp weight can differ in its last bits (the host may fuse the
+    # multiply-subtract of (x - x_left) / h), so values that cancel to ~0
+    # (x = 1.27e-3 lands on column N-1, where p is pinned to 0) are compared
+    # on the scale of the field
+    pscale = np.abs(ref.get("p")).max()
+    for x in (-1.0, 0.0, 0.3e-3, 0.71e-3, 1.0e-3 + 1e-5, 1.27e-3, 5.0):
+        pg, pr = gpu.samplePlaneAtX(x), ref.plane(x)
+        assert pg["valid"] == pr["valid"], x
+        for key in ("area", "mass_flux", "avg_dynamic"):
+            assert pg[key] == pytest.approx(pr[key], rel=1e-9, abs=1e-300), (x, key)
+        for key in ("avg_static", "avg_total"):
+            assert pg[key] == pytest.approx(pr[key], rel=1e-9, abs=1e-12 * pscale), (x, key)
+
+
+def test_run_dp_window_and_inlet_ramp_match_oracle(pkg, orc):
+    """runIterations with the inlet ramp and the core-dP window on: the run
+    stops at the same iteration as the oracle's, on the window, with the same
+    drops and fields."""
+    cfg, gamma, inlet = pkg.configs.channel_200x50(nx=60, ny=20, max_pressure_iter=600, p_tol=1e-14)
+    for k, v in dict(use_dp_convergence=1, inlet_ramp=1, ramp_steps=20, dp_window=15, dp_percent=1.0,
+                     x_core_in=0.002, x_core_out=0.004, epsilon=1e-12, max_iterations=150).items():
+        setattr(cfg, k, v)
+    gpu, ref = _pair(pkg, orc, cfg, gamma, inlet)
+    sg, sr = gpu.runIterations(), ref.run()
+    assert sr["dp_converged"] == 1 and sg["dp_converged"] == 1
+    assert sg["iteration"] == sr["iterations"] < cfg.max_iterations
+    for key in ("dp_core_total", "dp_full_total", "error"):
+        assert sg[key] == pytest.approx(sr[key], rel=1e-9), key
+    for name in ("u", "v", "p"):
+        assert rel_l2(gpu.get(name), ref.get(name)) < 1e-10, name
+    # a run that stops inside the ramp leaves the ramped inlet in place
+    cfg.max_iterations, cfg.use_dp_convergence = 3, 0
+    cfg.ramp_steps = 10
+    gpu, ref = _pair(pkg, orc, cfg, gamma, inlet)
+    gpu.runIterations()
+    ref.run()
+    assert np.array_equal(gpu.get("u")[:, 0], ref.get("u")[:, 0])
+    assert rel_l2(gpu.get("u"), ref.get("u")) < 1e-10
+
+
 def test_fou_uniform_inlet_short_run(pkg, orc):
     # p_tol far below the round-off of the solve, so the PCG stop iteration
     # cannot flip between the two reduction orders

@@ -307,14 +307,158 @@ def test_developed_channel_poiseuille_band_and_mass_balance(pkg, orc, T):
     few per cent), and inflow equals outflow."""
     nx, ny, h, U = 120, 12, 1e-4, 0.01
     cfg = small_cfg(pkg, nx=nx, ny=ny, u_in=U, epsilon=1e-9, max_iterations=20000, p_tol=1e-12,
-                    max_pressure_iter=1000, momentum_tol=1e-10, max_momentum_iter=2000)
+                    max_pressure_iter=1000, momentum_tol=1e-10, max_momentum_iter=2000, use_dp_convergence=0)
     o = orc.OracleSimple(T, cfg, np.ones((ny, nx)))
     r = o.run()
-    assert r["error"] <= 1e-9
+    assert r["error"] <= 1e-9 and r["dp_converged"] == 0
     dp = o.dp_static()
     ana = 12 * cfg.eta * U * (nx * h) / (ny * h) ** 2
     assert 1.0 < dp / ana < 1.10, dp / ana
-    assert o.patch(True)["mass_flux"] == pytest.approx(o.patch(False)["mass_flux"], rel=1e-8)
+    flux = o.patch(True)["mass_flux"]
+    assert o.patch(False)["mass_flux"] == pytest.approx(flux, rel=1e-8)
+    # every x-plane carries the same mass flux: sum_i u(i, j) is the same for
+    # each column at a converged state, so any lerp of two columns is too
+    for x in (-1.0, 0.3e-3, 5.55e-3, 11.9e-3, 1.0):
+        assert o.plane(x)["mass_flux"] == pytest.approx(flux, rel=1e-8), x
+    assert r["dp_full_static"] == pytest.approx(dp, rel=1e-12)
+
+
+def _plane_field(pkg, orc, T):
+    """4 x 3 cells, h = 1e-4, cell (1, 2) not pure fluid; p(i, j) = 10 j + i,
+    u(i, j) = 0.01 (j + 1) + 0.001 i, v(i, j) = 0.002 j - 0.001 i."""
+    cfg = small_cfg(pkg, nx=4, ny=3)
+    g = np.ones((3, 4))
+    g[1, 2] = 0.5
+    o = orc.OracleSimple(T, cfg, g)
+    for name, f in (("p", lambda i, j: 10.0 * j + i), ("u", lambda i, j: 0.01 * (j + 1) + 0.001 * i),
+                    ("v", lambda i, j: 0.002 * j - 0.001 * i)):
+        i, j = np.indices(o.shape(name))
+        o.set(name, f(i, j).astype(np.float64))
+    return cfg, o
+
+
+def _expected_plane(cfg, rows):
+    """Metrics from hand-interpolated (p, u, v_tangential) per sampled row."""
+    area = cfg.hy * len(rows)
+    dyn = [0.5 * cfg.rho * (u * u + vt * vt) for _, u, vt in rows]
+    fl = [abs(cfg.rho * u * cfg.hy) for _, u, _ in rows]
+    tot = sum((p + d) * f for (p, _, _), d, f in zip(rows, dyn, fl)) / sum(fl)
+    return dict(area=area, mass_flux=sum(fl), avg_static=sum(p for p, _, _ in rows) / len(rows),
+                avg_dynamic=sum(dyn) / len(rows), avg_total=tot, valid=1)
+
+
+def test_plane_at_x_known_answer(pkg, orc, T):
+    """samplePlaneAtX (postprocessing.cpp:62-243) on a hand-set field.
+    x = 1.75e-4 lies a quarter of the way from cell centre 1 (1.5e-4) to
+    cell centre 2: p lerps columns 2, 3, u and v columns 1, 2 with t = 0.25,
+    and row i = 2 is skipped (its cell in column 2 has gamma 0.5).  x beyond
+    either end clamps to the first / last cell centre."""
+    cfg, o = _plane_field(pkg, orc, T)
+    # rows i = 1, 3: p = 22.5 + i, u = 0.0225 + 0.001 i,
+    # v_t = mean over rows i-1, min(i, 3) of 0.0025 - 0.001 r
+    want = _expected_plane(cfg, [(23.5, 0.0235, 0.002), (25.5, 0.0255, 0.0)])
+    got = o.plane(1.75e-4)
+    for k, v in want.items():
+        assert got[k] == pytest.approx(v, rel=1e-12, abs=1e-18), k
+    # x = 1 m -> last centre: cells 2, 3 (row 2 skipped), t = 1: p col 4, u / v col 3
+    want = _expected_plane(cfg, [(41.0, 0.041, 0.5 * (0.006 + 0.005)), (43.0, 0.043, 0.5 * (0.004 + 0.003))])
+    got = o.plane(1.0)
+    for k, v in want.items():
+        assert got[k] == pytest.approx(v, rel=1e-12, abs=1e-18), k
+    # x = -1 m -> first centre: cells 0, 1 (no row skipped), t = 0: p col 1, u / v col 0
+    want = _expected_plane(cfg, [(11.0, 0.011, -0.0005), (12.0, 0.012, -0.0015), (13.0, 0.013, -0.0025)])
+    got = o.plane(-1.0)
+    for k, v in want.items():
+        assert got[k] == pytest.approx(v, rel=1e-12, abs=1e-18), k
+
+
+def _window_cfg(pkg, **kw):
+    """60 x 20 channel whose core dP settles within a 15-iteration window a
+    few dozen iterations after a 20-iteration inlet ramp."""
+    cfg, g, inl = pkg.configs.channel_200x50(nx=60, ny=20, max_pressure_iter=600, p_tol=1e-14)
+    opts = dict(use_dp_convergence=1, inlet_ramp=1, ramp_steps=20, dp_window=15, dp_percent=1.0,
+                x_core_in=0.002, x_core_out=0.004, epsilon=1e-12, max_iterations=150)
+    opts.update(kw)
+    for k, v in opts.items():
+        setattr(cfg, k, v)
+    return cfg, g, inl
+
+
+def _first_settled(h, start, win, pct, slope_win=0, slope_tol=None):
+    """First iteration k >= start + win - 1 at which the last win core dPs
+    have std and |newest - mean| below pct % of max(|mean|, 1) and (with a
+    slope window) the least-squares slope of the last slope_win dPs is at
+    most slope_tol in magnitude."""
+    for k in range(start + win - 1, len(h)):
+        w = h[k - win + 1:k + 1]
+        m = w.mean()
+        den = max(abs(m), 1.0)
+        ok = w.std() / den * 100 < pct and abs(w[-1] - m) / den * 100 < pct
+        if ok and slope_win and k + 1 >= slope_win:
+            s = np.polyfit(np.arange(slope_win), h[k - slope_win + 1:k + 1], 1)[0]
+            ok = abs(s) <= slope_tol
+        if ok:
+            return k
+    return None
+
+
+def test_dp_window_stop_rule(pkg, orc, T):
+    """runIterations' dP-window stop (SIMPLE.cpp:441-491): the run ends on
+    the first iteration whose window (entries from iteration rampSteps on)
+    has settled, checked against the recorded core-dP history."""
+    cfg, g, inl = _window_cfg(pkg)
+    r = orc.OracleSimple(T, cfg, g, inl).run(history_len=200)
+    h = r["dp_history"]
+    k = _first_settled(h, cfg.ramp_steps, cfg.dp_window, cfg.dp_percent)
+    assert r["dp_converged"] == 1 and r["iterations"] == k + 1 < cfg.max_iterations
+    assert r["dp_core_total"] == pytest.approx(h[-1], rel=1e-15)
+    # the window ignores everything before rampSteps: a later ramp end
+    # delays the stop to the first settled window after it
+    cfg2, _, _ = _window_cfg(pkg, inlet_ramp=0, ramp_steps=k + 5)
+    r2 = orc.OracleSimple(T, cfg2, g, inl).run(history_len=200)
+    k2 = _first_settled(r2["dp_history"], cfg2.ramp_steps, cfg2.dp_window, cfg2.dp_percent)
+    assert k2 is not None and r2["iterations"] == k2 + 1 and k2 >= cfg2.ramp_steps + cfg2.dp_window - 1
+    # window off: the run goes to max_iterations
+    cfg3, _, _ = _window_cfg(pkg, use_dp_convergence=0, max_iterations=k + 10)
+    r3 = orc.OracleSimple(T, cfg3, g, inl).run()
+    assert r3["iterations"] == k + 10 and r3["dp_converged"] == 0
+
+
+def test_dp_slope_gate(pkg, orc, T):
+    """usePressureDropSlopeGate (SIMPLE.cpp:347-366, 478-483): the window
+    also needs |least-squares slope| of the last dpSlopeWindowIters core dPs
+    <= dpSlopeMaxDelta / dpSlopeWindowIters; the slope is checked against
+    numpy.polyfit on the recorded history."""
+    cfg0, g, inl = _window_cfg(pkg)
+    k0 = orc.OracleSimple(T, cfg0, g, inl).run()["iterations"] - 1
+    # a loose gate changes nothing
+    cfg, _, _ = _window_cfg(pkg, use_dp_slope_gate=1, dp_slope_window=10, dp_slope_max_delta=1e6)
+    assert orc.OracleSimple(T, cfg, g, inl).run()["iterations"] == k0 + 1
+    # a tight gate delays the stop to where the slope has flattened
+    cfg, _, _ = _window_cfg(pkg, use_dp_slope_gate=1, dp_slope_window=10, dp_slope_max_delta=2e-3,
+                            max_iterations=400)
+    r = orc.OracleSimple(T, cfg, g, inl).run(history_len=400)
+    k = _first_settled(r["dp_history"], cfg.ramp_steps, cfg.dp_window, cfg.dp_percent, 10, 2e-4)
+    assert r["dp_converged"] == 1 and k > k0 and r["iterations"] == k + 1
+    # a zero tolerance never passes
+    cfg, _, _ = _window_cfg(pkg, use_dp_slope_gate=1, dp_slope_window=10, dp_slope_max_delta=0.0, max_iterations=k0 + 20)
+    r = orc.OracleSimple(T, cfg, g, inl).run()
+    assert r["dp_converged"] == 0 and r["iterations"] == k0 + 20
+
+
+def test_inlet_ramp(pkg, orc, T):
+    """enableInletRamp (SIMPLE.cpp:381-386): iteration k < rampSteps uses
+    inlet speed (k + 1) / rampSteps * targetVel."""
+    cfg, g, inl = _window_cfg(pkg, use_dp_convergence=0, max_iterations=3, ramp_steps=10)
+    o = orc.OracleSimple(T, cfg, g, inl)
+    o.run()
+    u = o.get("u")
+    np.testing.assert_allclose(u[:, 0], 0.3 * cfg.u_in, rtol=1e-15)
+    # with the ramp off the inlet is at full speed from the first iteration
+    cfg, g, inl = _window_cfg(pkg, use_dp_convergence=0, max_iterations=3, ramp_steps=10, inlet_ramp=0)
+    o = orc.OracleSimple(T, cfg, g, inl)
+    o.run()
+    np.testing.assert_allclose(o.get("u")[:, 0], cfg.u_in, rtol=1e-15)
 
 
 def test_boundary_patch_known_answer(pkg, orc, T):
