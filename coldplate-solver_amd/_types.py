@@ -195,15 +195,17 @@ class SolveStats(C.Structure):
 
 
 class ThermalMetrics(C.Structure):
-    """cps_thermal_metrics (ThermalMetrics of thermal_output.cpp:16-44 without
-    the gradient fields, plus Tb_max / Tb_pnorm and the energy balance)."""
+    """cps_thermal_metrics (ThermalMetrics of thermal_output.cpp:16-44, plus
+    Tb_max / Tb_pnorm and the energy balance)."""
 
     _fields_ = [(k, C.c_double) for k in (
         "t_avg_base", "t_avg_fluid", "t_avg_solid", "t_avg_global",
         "t_std_base", "t_std_fluid", "t_std_global",
         "t_range_base", "t_range_fluid", "t_range_global",
         "t_uniformity_base", "t_uniformity_global",
-        "tb_max", "tb_pnorm", "q_in", "q_out", "t_out", "imbalance_pct")]
+        "tb_max", "tb_pnorm", "q_in", "q_out", "t_out", "imbalance_pct",
+        "grad_max", "grad_avg", "grad_max_base", "grad_avg_base")] + [
+        (k, C.c_int) for k in ("grad_max_i", "grad_max_j", "grad_max_k", "pad_")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

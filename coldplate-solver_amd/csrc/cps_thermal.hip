@@ -604,14 +604,17 @@ __global__ __launch_bounds__(TNT) void k_bi_scalar(const double* part, int nb, i
 }
 
 // ---------------------------------------------------------------------------
-// metrics (thermal_output.cpp:135-270; thermal_solver.cpp:345-381, 592-620)
+// metrics (thermal_output.cpp:135-321; thermal_solver.cpp:345-381, 592-620)
 enum MetricSum {
   MS_TV_BASE, MS_V_BASE, MS_TV_FLUID, MS_V_FLUID, MS_TV_SOLID, MS_V_SOLID, MS_TV_ALL, MS_V_ALL,
   MS_TP_L0, MS_MDOT, MS_MDOTT, MS_QIN, MS_COUNT
 };
 // maxima of T (MX_*) and of -T (MN_*)
 enum MetricExt { MX_BASE, MX_FLUID, MX_ALL, MX_L0, MN_BASE, MN_FLUID, MN_ALL, MX_COUNT };
-enum MetricVar { MV_BASE, MV_FLUID, MV_ALL, MV_COUNT };
+enum MetricVar { MV_BASE, MV_FLUID, MV_ALL, MV_GRAD_ALL, MV_GRAD_BASE, MV_COUNT };
+// per-block gradient extremes of pass 2: max |grad T| and its linear cell
+// index (as a double), max |grad T| over the base layers
+enum MetricGrad { MG_MAX, MG_ARG, MG_MAX_BASE, MG_COUNT };
 
 struct MetricIn {
   double pexp;
@@ -673,21 +676,107 @@ __global__ __launch_bounds__(TNT) void k_metrics1(Phys P, Op<double> A, const do
   }
 }
 
-// pass 2: volume-weighted squared deviations from the pass-1 means
-__global__ __launch_bounds__(TNT) void k_metrics2(Phys P, Op<double> A, const double* T, MetricIn in, double* part) {
-  __shared__ double sh[TNT / kWave + 1];
+// |grad T| of one cell (thermal_output.cpp:63-120): central differences
+// inside, one-sided first differences at the faces; in z the central
+// difference spans 2 * (dz[k-1] + dz[k]) / 2 and the one-sided ones dz[0] /
+// dz[nz-1].  An axis of one cell contributes 0.
+__device__ inline double grad_mag(const Phys& P, const Op<double>& A, const double* T, int i, int j, int k) {
+  const double t = T[A.at(i, j, k)];
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  if (P.nx > 1) {
+    if (i == 0) gx = (T[A.at(1, j, k)] - t) / P.dx;
+    else if (i == P.nx - 1) gx = (t - T[A.at(i - 1, j, k)]) / P.dx;
+    else gx = (T[A.at(i + 1, j, k)] - T[A.at(i - 1, j, k)]) / (2.0 * P.dx);
+  }
+  if (P.ny > 1) {
+    if (j == 0) gy = (T[A.at(i, 1, k)] - t) / P.dy;
+    else if (j == P.ny - 1) gy = (t - T[A.at(i, j - 1, k)]) / P.dy;
+    else gy = (T[A.at(i, j + 1, k)] - T[A.at(i, j - 1, k)]) / (2.0 * P.dy);
+  }
+  if (P.nz > 1) {
+    if (k == 0) gz = (T[A.at(i, j, 1)] - t) / P.dz[0];
+    else if (k == P.nz - 1) gz = (t - T[A.at(i, j, k - 1)]) / P.dz[P.nz - 1];
+    else gz = (T[A.at(i, j, k + 1)] - T[A.at(i, j, k - 1)]) / (2.0 * (0.5 * (P.dz[k - 1] + P.dz[k])));
+  }
+  return sqrt(gx * gx + gy * gy + gz * gz);
+}
+
+// (value, index) pairs: the larger value wins, the smaller index on ties
+__device__ inline void argmax_merge(double& v, double& x, double ov, double ox) {
+  if (ov > v || (ov == v && ox < x)) {
+    v = ov;
+    x = ox;
+  }
+}
+// block argmax; result valid in thread 0.  sh: 2 * TNT / kWave doubles.
+__device__ inline void block_argmax(double& v, double& x, double* sh) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    const double ov = __shfl_down(v, o, kWave), ox = __shfl_down(x, o, kWave);
+    argmax_merge(v, x, ov, ox);
+  }
+  const int t = flat_tid();
+  if ((t & (kWave - 1)) == 0) {
+    sh[t / kWave] = v;
+    sh[TNT / kWave + t / kWave] = x;
+  }
+  __syncthreads();
+  if (t == 0)
+    for (int q = 1; q < TNT / kWave; ++q) argmax_merge(v, x, sh[q], sh[TNT / kWave + q]);
+  __syncthreads();
+}
+
+// pass 2: volume-weighted squared deviations from the pass-1 means, and the
+// gradient pass (thermal_output.cpp:270-321): sums of |grad T| V, per-block
+// extremes into ext[MG_* x nb]
+__global__ __launch_bounds__(TNT) void k_metrics2(Phys P, Op<double> A, const double* T, MetricIn in, double* part,
+                                                  double* ext) {
+  __shared__ double sh[2 * (TNT / kWave) + 1];
   CELL_IJK(P.nx, P.ny);
-  double s[MV_COUNT] = {0, 0, 0};
+  double s[MV_COUNT] = {0, 0, 0, 0, 0};
+  double gmax = -HUGE_VAL, garg = HUGE_VAL, gbase = -HUGE_VAL;
   if (inside) {
     const double t = T[A.at(i, j, k)], V = P.dx * P.dy * P.dz[k];
     s[MV_ALL] = V * (t - in.avg[2]) * (t - in.avg[2]);
     if (k < P.first_channel) s[MV_BASE] = V * (t - in.avg[0]) * (t - in.avg[0]);
     else if (fluid(P, i, j, k)) s[MV_FLUID] = V * (t - in.avg[1]) * (t - in.avg[1]);
+    const double gm = grad_mag(P, A, T, i, j, k);
+    s[MV_GRAD_ALL] = gm * V;
+    gmax = gm;
+    garg = (double)(((size_t)k * P.ny + j) * P.nx + i);
+    if (k < P.first_channel) {
+      s[MV_GRAD_BASE] = gm * V;
+      gbase = gm;
+    }
   }
   const int nb = num_blocks(), bid = flat_bid();
   for (int q = 0; q < MV_COUNT; ++q) {
     const double r = block_sum<TNT>(s[q], sh);
     if (flat_tid() == 0) part[q * nb + bid] = r;
+  }
+  block_argmax(gmax, garg, sh);
+  const double rb = block_maxd(gbase, sh);
+  if (flat_tid() == 0) {
+    ext[MG_MAX * nb + bid] = gmax;
+    ext[MG_ARG * nb + bid] = garg;
+    ext[MG_MAX_BASE * nb + bid] = rb;
+  }
+}
+
+// over the blocks of k_metrics2: out = [max |grad T|, its index, base max]
+__global__ __launch_bounds__(TNT) void k_grad_ext(const double* ext, int nb, double* out) {
+  __shared__ double sh[2 * (TNT / kWave) + 1];
+  double v = -HUGE_VAL, x = HUGE_VAL, b = -HUGE_VAL;
+  for (int q = flat_tid(); q < nb; q += TNT) {
+    argmax_merge(v, x, ext[MG_MAX * nb + q], ext[MG_ARG * nb + q]);
+    b = fmax(b, ext[MG_MAX_BASE * nb + q]);
+  }
+  block_argmax(v, x, sh);
+  b = block_maxd(b, sh);
+  if (flat_tid() == 0) {
+    out[0] = v;
+    out[1] = x;
+    out[2] = b;
   }
 }
 
@@ -1227,7 +1316,10 @@ class ThermalSolver {
     in.avg[0] = avg(s[th::MS_TV_BASE], s[th::MS_V_BASE]);
     in.avg[1] = avg(s[th::MS_TV_FLUID], s[th::MS_V_FLUID]);
     in.avg[2] = avg(s[th::MS_TV_ALL], s[th::MS_V_ALL]);
-    th::k_metrics2<<<G, B, 0, st>>>(phys(), op64(false), T.p, in, part.p);
+    th::k_metrics2<<<G, B, 0, st>>>(phys(), op64(false), T.p, in, part.p, ext.p);
+    th::k_grad_ext<<<1, B, 0, st>>>(ext.p, nblocks(), sums.p + th::MS_COUNT);
+    double gx[th::MG_COUNT];
+    CPS_HIP(hipMemcpyAsync(gx, sums.p + th::MS_COUNT, sizeof(gx), hipMemcpyDeviceToHost, st));
     double var[th::MV_COUNT];
     sum_parts(part.p, th::MV_COUNT, var);
     auto range = [&](int mx, int mn, double vol) { return vol > 0 ? m[mx] + m[mn] : 0.0; };  // max T - min T
@@ -1253,6 +1345,17 @@ class ThermalSolver {
     r.t_out = mdot > 0 ? s[th::MS_MDOTT] / mdot : cfg.t_in;
     r.q_out = mdot * cfg.cp * (r.t_out - cfg.t_in);
     r.imbalance_pct = 100.0 * (r.q_in - r.q_out) / (r.q_in + 1e-30);
+    // thermal_output.cpp:274-320: maxima start at -1e30, means over volume
+    r.grad_max = std::max(gx[0], -1e30);
+    r.grad_max_base = std::max(gx[2], -1e30);
+    r.grad_avg = avg(var[th::MV_GRAD_ALL], s[th::MS_V_ALL]);
+    r.grad_avg_base = avg(var[th::MV_GRAD_BASE], s[th::MS_V_BASE]);
+    if (gx[0] > -1e30) {
+      const long long q = (long long)gx[1];
+      r.grad_max_i = (int)(q % nx);
+      r.grad_max_j = (int)((q / nx) % ny);
+      r.grad_max_k = (int)(q / ((long long)nx * ny));
+    }
     *out = r;
   }
 

@@ -303,6 +303,12 @@ typedef struct cps_thermal_metrics {
   double t_uniformity_base, t_uniformity_global;
   double tb_max, tb_pnorm;   /* over layer k = 0 */
   double q_in, q_out, t_out, imbalance_pct;
+  /* |grad T| statistics (thermal_output.cpp:270-321): central differences
+   * inside, one-sided at the faces (:63-108); volume-weighted means; the
+   * maximum's cell is the first in (k, j, i) order; -1e30 when a region has
+   * no cells; an axis of one cell contributes 0 */
+  double grad_max, grad_avg, grad_max_base, grad_avg_base;
+  int grad_max_i, grad_max_j, grad_max_k, pad_;
 } cps_thermal_metrics;
 
 #define CPS_THERMAL_OBJ_T_AVG_BASE 0  /* volume mean of T over the layers below the channel */

@@ -280,9 +280,30 @@ class OracleThermal:
         return np.linalg.norm(b - A @ np.ravel(T)) / np.linalg.norm(b)
 
 
+def grad_magnitude(T, dx, dy, dz):
+    """|grad T| per cell (compute_gradient_components, thermal_output.cpp:
+    63-120): (T[i+1] - T[i-1]) / (2 h) inside, first differences at the
+    faces; in z the central difference divides by 2 * (dz[k-1] + dz[k]) / 2
+    and the face differences by dz[0] / dz[nz-1].  An axis of one cell
+    contributes 0."""
+    nz, ny, nx = T.shape
+    g2 = np.zeros_like(T)
+    for ax, h in ((2, dx), (1, dy)):
+        if T.shape[ax] > 1:
+            g2 += np.gradient(T, h, axis=ax) ** 2
+    if nz > 1:
+        gz = np.empty_like(T)
+        gz[0] = (T[1] - T[0]) / dz[0]
+        gz[-1] = (T[-1] - T[-2]) / dz[-1]
+        for k in range(1, nz - 1):
+            gz[k] = (T[k + 1] - T[k - 1]) / (2.0 * (0.5 * (dz[k - 1] + dz[k])))
+        g2 += gz ** 2
+    return np.sqrt(g2)
+
+
 def thermal_metrics(stack, gamma, u, T, pnorm_exp=10.0):
     """numpy restatement of compute_thermal_metrics (thermal_output.cpp:135-
-    270, without the gradient pass), Tb_max / Tb_pnorm of
+    321, including the gradient pass), Tb_max / Tb_pnorm of
     compute_optimization_results (thermal_solver.cpp:361-381) and the energy
     balance of ThermalSolver3D::solve (thermal_solver.cpp:592-620).  Base
     layers are those below the first channel layer (k < nz_solid)."""
@@ -336,6 +357,14 @@ def thermal_metrics(stack, gamma, u, T, pnorm_exp=10.0):
     m["t_out"] = mdotT / mdot if mdot > 0 else stack["t_in"]
     m["q_out"] = mdot * stack["cp"] * (m["t_out"] - stack["t_in"])
     m["imbalance_pct"] = 100 * (m["q_in"] - m["q_out"]) / (m["q_in"] + 1e-30)
+    # gradient pass (thermal_output.cpp:270-321)
+    G = grad_magnitude(T, dx, dy, dz)
+    q = int(np.argmax(G))  # first maximum in (k, j, i) order
+    m["grad_max"] = G.ravel()[q]
+    m["grad_max_k"], m["grad_max_j"], m["grad_max_i"] = np.unravel_index(q, G.shape)
+    m["grad_avg"] = (G * V).sum() / V.sum()
+    m["grad_max_base"] = G[base].max() if base.any() else -1e30
+    m["grad_avg_base"] = (G[base] * V[base]).sum() / V[base].sum() if base.any() else 0.0
     return m
 
 

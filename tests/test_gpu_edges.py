@@ -52,7 +52,9 @@ def _stack_case(pkg, nx, ny, n_base, n_channel, n_lid, seed):
                                                           (9, 1, 1, 1, 1)])
 def test_thermal_degenerate_stacks_match_direct_solve(pkg, orc, nx, ny, n_base, n_channel, n_lid):
     """One layer (the z lines are single cells), one column (the multigrid
-    hierarchy is the coarsest level alone), one-row and one-column grids."""
+    hierarchy is the coarsest level alone), one-row and one-column grids.
+    The metrics (an axis of one cell has no gradient; no base layers leave
+    the base maxima at -1e30) match the restatement on the solved field."""
     stack, gamma, u, v = _stack_case(pkg, nx, ny, n_base, n_channel, n_lid, seed=nx + 7 * ny)
     print(f"thermal case nx={nx} ny={ny} nz={len(stack['dz'])}", flush=True)
     for prec in (64, 32):
@@ -64,6 +66,13 @@ def test_thermal_degenerate_stacks_match_direct_solve(pkg, orc, nx, ny, n_base, 
         assert o.rel_residual(s.T) < 2e-12
         rise = max(np.abs(Td - stack["t_in"]).max(), 1e-12)
         assert np.abs(s.T - Td).max() / rise < 1e-8
+        T = s.T
+        m, ref = s.metrics(8.0), orc.thermal_metrics(stack, gamma, u, T, 8.0)
+        for k, val in ref.items():
+            if not k.startswith("grad_max_") or k == "grad_max_base":
+                assert m[k] == pytest.approx(val, rel=1e-11, abs=1e-9), k
+        G = orc.grad_magnitude(T, stack["dx"], stack["dy"], np.asarray(stack["dz"]))
+        assert G[m["grad_max_k"], m["grad_max_j"], m["grad_max_i"]] == pytest.approx(ref["grad_max"], rel=1e-13)
         s.close()
 
 

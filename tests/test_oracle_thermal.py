@@ -127,6 +127,37 @@ def test_small_stack_dense_solve(pkg, orc, T):
     np.testing.assert_allclose(Tb, np.linalg.solve(D, b), rtol=1e-10)
 
 
+def test_gradient_metrics_closed_forms(pkg, orc):
+    """Gradient pass of compute_thermal_metrics (thermal_output.cpp:63-120,
+    270-321).  A linear field 3x + 4y + 12z on a uniform grid: |grad T| = 13
+    at every cell, faces included, so the maximum sits at the first cell.
+    x^2: central differences are exact inside (2x), the faces are first
+    differences (dx at i = 0, (2 nx - 3) dx at i = nx - 1, the maximum).  In
+    z, T = k over dz = (1, 2, 4): 1 / dz0 at k = 0, 2 / (dz0 + dz1) at k = 1,
+    1 / dz2 at k = 2."""
+    nx, ny = 6, 5
+    st = stack_of(nx, ny, [0.25] * 4, [0, 0, 1, 1], h=0.5)
+    g, u = np.ones((ny, nx)), np.zeros((ny, nx))
+    k, j, i = np.indices((4, ny, nx)).astype(np.float64)
+    m = orc.thermal_metrics(st, g, u, 3.0 * i * 0.5 + 4.0 * j * 0.5 + 12.0 * k * 0.25)
+    for key in ("grad_max", "grad_avg", "grad_max_base", "grad_avg_base"):
+        assert m[key] == pytest.approx(13.0, rel=1e-15), key
+    assert (m["grad_max_i"], m["grad_max_j"], m["grad_max_k"]) == (0, 0, 0)
+    m = orc.thermal_metrics(st, g, u, (i * 0.5) ** 2)
+    assert m["grad_max"] == pytest.approx((2 * nx - 3) * 0.5, rel=1e-15)
+    assert (m["grad_max_i"], m["grad_max_j"], m["grad_max_k"]) == (nx - 1, 0, 0)
+    x = np.arange(nx) * 0.5
+    per_col = np.r_[0.5, 2 * x[1:-1], (2 * nx - 3) * 0.5]
+    assert m["grad_avg"] == pytest.approx(per_col.mean(), rel=1e-14)
+    st = stack_of(2, 2, [1.0, 2.0, 4.0], [0, 1, 1], h=1.0)
+    k3 = np.indices((3, 2, 2))[0].astype(np.float64)
+    G = orc.grad_magnitude(k3, 1.0, 1.0, st["dz"])
+    np.testing.assert_allclose(G[:, 0, 0], [1.0, 2.0 / 3.0, 0.25], rtol=1e-15)
+    m = orc.thermal_metrics(st, np.ones((2, 2)), np.zeros((2, 2)), k3)
+    assert m["grad_max_base"] == 1.0 and m["grad_avg_base"] == 1.0
+    assert m["grad_avg"] == pytest.approx((1.0 * 1 + 2.0 / 3.0 * 2 + 0.25 * 4) / 7, rel=1e-15)
+
+
 def test_metrics_closed_forms(pkg, orc):
     """Uniform T: every mean is T, std 0, uniformity 1, p-norm T.  Layer 0
     split in two values a, b: p-norm = ((a^p + b^p) / 2)^(1/p)."""
