@@ -7,11 +7,11 @@ include/coldplate_hip.h (libcoldplate_hip.so, built in-tree).
 The shared library is loaded on first use; every entry point raises if it is
 missing or no GPU is present -- there is no CPU fallback.
 """
-from . import _types, configs  # noqa: F401
+from . import _types, configs, exportfiles  # noqa: F401
 from ._types import default_config, SimpleConfig, StepStats, Patch, field_shape  # noqa: F401
 from . import _capi  # noqa: F401  (loads nothing until first use)
 from .simple import SIMPLE, Batch, pressure_solve_csr, momentum_jacobi_csr, pressure_drop  # noqa: F401
 from .thermal import ThermalSolver3D, thermal_objective, surrogate_adjoint  # noqa: F401
 
-__all__ = ["_types", "configs", "default_config", "SimpleConfig", "StepStats", "Patch", "field_shape",
+__all__ = ["_types", "configs", "exportfiles", "default_config", "SimpleConfig", "StepStats", "Patch", "field_shape",
            "SIMPLE", "Batch", "pressure_solve_csr", "momentum_jacobi_csr", "pressure_drop", "ThermalSolver3D", "thermal_objective", "surrogate_adjoint"]

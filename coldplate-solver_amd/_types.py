@@ -136,6 +136,12 @@ class StepStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class IterLog(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("resid_mass", "resid_u", "resid_v", "dp_core_total", "dp_full_total",
+                                          "dp_core_static", "dp_full_static")] + [
+        ("p_iters", C.c_int), ("pad_", C.c_int)]
+
+
 class Patch(C.Structure):
     _fields_ = [
         ("area", C.c_double), ("mass_flux", C.c_double), ("avg_static", C.c_double),

@@ -1182,7 +1182,7 @@ class SimpleSolver {
   // SIMPLE.cpp:310-491 without logging, checkpoints and the pseudo-time
   // controller.  The four drops are reduced on the device after every
   // iteration and read back with the residuals; the window is host state.
-  void run(cps_step_stats* out) {
+  int run(cps_step_stats* out, cps_iter_log* log = nullptr, int log_cap = 0) {
     apply_bc();
     cps_step_stats sst{};
     sst.error = 1.0;
@@ -1201,6 +1201,18 @@ class SimpleSolver {
       stats(&sst);
       CPS_CHECK(std::isfinite(sst.error), CPS_ERR_DIVERGED, "SIMPLE residual is not finite");
       const double core = sst.dp_core_total;
+      if (log && it < log_cap) {
+        cps_iter_log& r = log[it];
+        r.resid_mass = sst.resid_mass;
+        r.resid_u = sst.resid_u;
+        r.resid_v = sst.resid_v;
+        r.dp_core_total = sst.dp_core_total;
+        r.dp_full_total = sst.dp_full_total;
+        r.dp_core_static = sst.dp_core_static;
+        r.dp_full_static = sst.dp_full_static;
+        r.p_iters = sst.p_iters;
+        r.pad_ = 0;
+      }
       if (cfg.use_dp_slope_gate && cfg.dp_slope_window > 0) {
         slope.emplace_back(it, core);
         if (slope.size() > (size_t)cfg.dp_slope_window) slope.pop_front();
@@ -1215,6 +1227,7 @@ class SimpleSolver {
     }
     sst.dp_converged = conv;
     if (out) *out = sst;
+    return it;
   }
 
   // SIMPLE.cpp:456-490: standard deviation of the window and distance of the
@@ -1536,6 +1549,15 @@ int cps_simple_run(cps_simple* s, cps_step_stats* last) {
   return guard([&] {
     CPS_CHECK(s, CPS_ERR_ARG, "null solver");
     s->s->run(last);
+  });
+}
+
+int cps_simple_run_logged(cps_simple* s, cps_step_stats* last, cps_iter_log* log, int log_cap, int* n_iters) {
+  return guard([&] {
+    CPS_CHECK(s, CPS_ERR_ARG, "null solver");
+    CPS_CHECK(log_cap >= 0 && (log || log_cap == 0), CPS_ERR_ARG, "log_cap needs a log of that many rows");
+    const int n = s->s->run(last, log, log_cap);
+    if (n_iters) *n_iters = n;
   });
 }
 

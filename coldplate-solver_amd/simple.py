@@ -71,6 +71,20 @@ class SIMPLE:
         self.last = st.as_dict()
         return self.last
 
+    def runIterationsLogged(self):
+        """runIterations plus the per-iteration log rows (residuals and the
+        four pressure drops, writeIterationLogs output.cpp:330-360) as a
+        dict of numpy arrays."""
+        st = T.StepStats()
+        cap = max(int(self.cfg.max_iterations), 0)
+        log = (T.IterLog * max(cap, 1))()
+        n = C.c_int()
+        _capi.check(self.L.cps_simple_run_logged(self.h, C.byref(st), log, cap, C.byref(n)))
+        self.last = st.as_dict()
+        rows = log[: n.value]
+        hist = {k: np.array([getattr(r, k) for r in rows]) for k, _ in T.IterLog._fields_ if k != "pad_"}
+        return self.last, hist
+
     # ---- fields ----------------------------------------------------------
     def get(self, name):
         out = np.empty(T.field_shape(name, self.cfg.nx, self.cfg.ny))

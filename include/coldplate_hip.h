@@ -174,6 +174,16 @@ int cps_simple_step(cps_simple* s, int n_steps, cps_step_stats* last);
  * pseudo-time controller (enablePseudoTimeStepping, off by default) is not
  * built.  Iteration counts restart at 0 on every call, as in the reference. */
 int cps_simple_run(cps_simple* s, cps_step_stats* last);
+/* One row of the reference's per-iteration logs (writeIterationLogs,
+ * output.cpp:330-360): raw RMS residuals and the four pressure drops. */
+typedef struct cps_iter_log {
+  double resid_mass, resid_u, resid_v;
+  double dp_core_total, dp_full_total, dp_core_static, dp_full_static;
+  int p_iters, pad_;
+} cps_iter_log;
+/* cps_simple_run that also fills log[k] for iterations k < log_cap (log may
+ * be NULL); *n_iters (may be NULL) receives the iterations done. */
+int cps_simple_run_logged(cps_simple* s, cps_step_stats* last, cps_iter_log* log, int log_cap, int* n_iters);
 int cps_simple_get_field(cps_simple* s, int field, double* out);
 int cps_simple_set_field(cps_simple* s, int field, const double* in);
 /* inlet = 1: x = 0 patch, 0: x = L patch. */

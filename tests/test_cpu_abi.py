@@ -30,7 +30,8 @@ def test_header_struct_fields_match_ctypes_mirrors(pkg):
     T = pkg._types
     for cname, mirror in (("cps_simple_config", T.SimpleConfig), ("cps_step_stats", T.StepStats),
                           ("cps_patch", T.Patch), ("cps_thermal_config", T.ThermalConfig),
-                          ("cps_solve_stats", T.SolveStats), ("cps_thermal_metrics", T.ThermalMetrics)):
+                          ("cps_solve_stats", T.SolveStats), ("cps_thermal_metrics", T.ThermalMetrics),
+                          ("cps_iter_log", T.IterLog)):
         body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (cname, cname), txt, flags=re.S).group(1)
         names = []
         for decl in body.split(";"):
