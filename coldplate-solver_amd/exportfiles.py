@@ -22,6 +22,8 @@ Outputs (same names, layouts and number formats as the reference):
   thermal_metrics.txt, k_thermal.txt, T_bottom.txt
                         (thermal_output.cpp:323-370, 524-560, 619-650)
   objective.txt, dJ_dgamma.txt (thermal_solver.cpp:383-421, on request)
+  sensitivity_field.txt, dJ_dgamma.txt
+                        (run_coldplate_adjoint.py:110-150, by run_adjoint)
 
 The solvers are the device ones (SIMPLE, ThermalSolver3D); this module only
 reads and writes text.
@@ -323,6 +325,45 @@ def save_optimization_outputs(export_dir, metrics, dJ_dgamma):
     with open(os.path.join(export_dir, "objective.txt"), "w") as f:
         f.write("%.10f\n%.10f\n10\n%.10f\n" % (metrics["tb_max"], metrics["tb_pnorm"], metrics["imbalance_pct"]))
     save_matrix(os.path.join(export_dir, "dJ_dgamma.txt"), dJ_dgamma, fmt=_fixed(10))
+
+
+# ---------------------------------------------------------------------------
+# the optimiser's adjoint step and objective
+
+def run_adjoint(export_dir, q_k=1.0, k_s=400.0, k_f=0.6, device=0):
+    """run_coldplate_adjoint.main (run_coldplate_adjoint.py:73-161) on the
+    device: the 2-D surrogate adjoint with T = T_bottom.txt, P =
+    pressure_thermal.txt and k = k_thermal.txt (external-T mode), dQ/dk to
+    sensitivity_field.txt and, when geometry_thermal.txt has the same shape,
+    dJ/dgamma = dQ/dk * dk/dgamma (RAMP at gamma = 1 - clip(value)) to
+    dJ_dgamma.txt; both tab-separated in %.10e.  Returns (sens, dJ or None)."""
+    from .thermal import surrogate_adjoint
+
+    load = lambda n: np.atleast_2d(np.loadtxt(os.path.join(export_dir, n), delimiter="\t"))  # noqa: E731
+    Tb, P, k = load("T_bottom.txt"), load("pressure_thermal.txt"), load("k_thermal.txt")
+    if P.shape != Tb.shape or k.shape != Tb.shape:
+        raise ValueError(f"shape mismatch T {Tb.shape}, P {P.shape}, k {k.shape}")
+    gamma = None
+    geo_path = os.path.join(export_dir, "geometry_thermal.txt")
+    if os.path.exists(geo_path):
+        geo = np.atleast_2d(np.loadtxt(geo_path, delimiter="\t"))
+        if geo.shape == Tb.shape:
+            gamma = 1.0 - np.clip(geo, 0.0, 1.0)
+    out = surrogate_adjoint(k, P=P, T=Tb, gamma=gamma, k_s=k_s, k_f=k_f, q_k=q_k, device=device)
+    np.savetxt(os.path.join(export_dir, "sensitivity_field.txt"), out["sens"], delimiter="\t", fmt="%.10e")
+    dJ = out.get("dJ_dgamma")
+    if dJ is not None:
+        np.savetxt(os.path.join(export_dir, "dJ_dgamma.txt"), dJ, delimiter="\t", fmt="%.10e")
+    return out["sens"], dJ
+
+
+def objective(metrics, w_peak=1.0, w_uniformity=0.0):
+    """evaluate_objective's weighted objective (mma_optimizer.py:475-489)
+    from the parsed thermal_metrics.txt dict: w_peak (T_avg_base +
+    T_range_base / 2) + w_uniformity (1 - T_uniformity_base), with the same
+    fallbacks (100, 0, 1) for missing entries."""
+    t_peak = metrics.get("T_avg_base", 100.0) + 0.5 * metrics.get("T_range_base", 0.0)
+    return w_peak * t_peak + w_uniformity * (1.0 - metrics.get("T_uniformity_base", 1.0))
 
 
 # ---------------------------------------------------------------------------

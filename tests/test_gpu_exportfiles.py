@@ -53,3 +53,16 @@ def test_run_solvers_on_generator_files(pkg, orc, tmp_path):
     assert metrics["t_avg_base"] == pytest.approx(want["t_avg_base"], abs=1e-5 * rise)
     k = np.array(xf.read_matrix(os.path.join(d, "k_thermal.txt")))
     np.testing.assert_allclose(k, xf.k_field(tg, 400.0, 0.6, 1.0), atol=1e-6)
+
+    # the optimiser's adjoint step on the written files, against the
+    # oracle's surrogate adjoint of the same (rounded) inputs
+    sens, dJ = xf.run_adjoint(d, q_k=3.0)
+    Tb, P = np.loadtxt(os.path.join(d, "T_bottom.txt")), np.loadtxt(os.path.join(d, "pressure_thermal.txt"))
+    _, _, sref = orc.surrogate_adjoint(k, P, Tb)
+    np.testing.assert_allclose(sens, sref, rtol=1e-8, atol=1e-10 * np.abs(sref).max())
+    g = 1.0 - np.clip(np.loadtxt(os.path.join(d, "geometry_thermal.txt")), 0.0, 1.0)
+    dk = -(400.0 - 0.6) * 4.0 / (1.0 + 3.0 * g) ** 2
+    np.testing.assert_allclose(np.loadtxt(os.path.join(d, "dJ_dgamma.txt")), sref * dk, rtol=1e-8,
+                               atol=1e-10 * np.abs(sref * dk).max())
+    assert xf.objective(parsed, 1.0, 2.0) == pytest.approx(
+        parsed["T_avg_base"] + 0.5 * parsed["T_range_base"] + 2.0 * (1 - parsed["T_uniformity_base"]), rel=1e-15)
