@@ -480,15 +480,22 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, C
       }
       col[i] = v;
     }
+    // p is uniform across the wave, so lane p's values are read into scalar
+    // registers (v_readlane) instead of going through LDS permutes
+    auto lane = [](double v, int src) {
+      const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+      const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+      return __hiloint2double(hi, lo);
+    };
     for (int p = 0; p < n; ++p) {
       double cp = 0.0;  // this column's row-p entry
 #pragma unroll
       for (int i = 0; i < kDirectCells; ++i) cp = i == p ? col[i] : cp;
-      const double inv = 1.0 / __shfl(cp, p);
+      const double inv = 1.0 / lane(cp, p);
       const double mp = (t == p ? 1.0 : cp) * inv;
 #pragma unroll
       for (int i = 0; i < kDirectCells; ++i) {
-        const double f = __shfl(col[i], p);  // A[i][p], before lane p updates it
+        const double f = lane(col[i], p);  // A[i][p], before lane p updates it
         const double v = (t == p ? 0.0 : col[i]) - f * mp;
         col[i] = i == p ? mp : v;
       }
