@@ -933,6 +933,7 @@ class FlowAdjoint {
     adj::k_faces<<<blocks((int)S), adj::BS, 0, s>>>(V, kind == CPS_ADJ_PRECOND_SIMPLE ? dinv.p : dj.p, dEa.p,
                                                    dNa.p);
     CPS_HIP(hipGetLastError());
+    schur_ready = false;  // new L: the next Schur solve rebuilds its hierarchy
   }
 
   void spmv(const double* x, double* out) {
@@ -968,6 +969,14 @@ class FlowAdjoint {
     }
   }
 
+  // xp = L^-1 rp by multigrid-PCG; the hierarchy of L (fixed from build()
+  // on) is set up by the first solve after each build only
+  bool schur_ready = false;
+  void schur_solve(const cps_simple_config& pcfg) {
+    mg->solve(dEa.p, dNa.p, rp.p, xp.p, st.p, pcfg, true, &timer, schur_ready);
+    schur_ready = true;
+  }
+
   void precond(const adj::View& V, const double* f, double* z, const cps_simple_config& pcfg) {
     if (kind == CPS_ADJ_PRECOND_SIMPLE) precond_simple(V, f, z, pcfg);
     else precond_lsc(V, f, z, pcfg);
@@ -981,7 +990,7 @@ class FlowAdjoint {
     line_solve(V, f, t1.p);  // x* = A~^-1 f_m
     spmv(t1.p, t2.p);
     adj::k_pre2<<<sb, adj::BS, 0, s>>>(V, t2.p, f, rp.p, xp.p);  // C x* - f_p
-    mg->solve(dEa.p, dNa.p, rp.p, xp.p, st.p, pcfg, true, &timer);  // L^-1
+    schur_solve(pcfg);  // L^-1
     adj::k_pre3<<<blocks(n), adj::BS, 0, s>>>(V, n, xp.p, t3.p);
     spmv(t3.p, t2.p);  // B y1
     adj::k_dscale<<<blocks(n), adj::BS, 0, s>>>(V, n, dj.p, t2.p, t3.p);
@@ -989,7 +998,7 @@ class FlowAdjoint {
     adj::k_dscale<<<blocks(n), adj::BS, 0, s>>>(V, n, dj.p, t2.p, t3.p);
     spmv(t3.p, t2.p);  // C D^-1 A D^-1 B y1
     adj::k_pre2<<<sb, adj::BS, 0, s>>>(V, t2.p, nullptr, rp.p, xp.p);
-    mg->solve(dEa.p, dNa.p, rp.p, xp.p, st.p, pcfg, true, &timer);  // y = L^-1 (...)
+    schur_solve(pcfg);  // y = L^-1 (...)
     adj::k_pre3<<<blocks(n), adj::BS, 0, s>>>(V, n, xp.p, t3.p);
     spmv(t3.p, t2.p);  // B y
     line_solve(V, t2.p, t4.p);
@@ -1003,7 +1012,7 @@ class FlowAdjoint {
     adj::k_pre1<<<blocks(n), adj::BS, 0, s>>>(V, n, dinv.p, f, t1.p);
     spmv(t1.p, t2.p);
     adj::k_pre2<<<blocks((int)S), adj::BS, 0, s>>>(V, t2.p, f, rp.p, xp.p);
-    mg->solve(dEa.p, dNa.p, rp.p, xp.p, st.p, pcfg, true, &timer);
+    schur_solve(pcfg);
     adj::k_pre3<<<blocks(n), adj::BS, 0, s>>>(V, n, xp.p, t3.p);
     spmv(t3.p, t2.p);
     adj::k_pre4<<<blocks(n), adj::BS, 0, s>>>(V, n, dinv.p, f, t1.p, t2.p, xp.p, z);

@@ -977,8 +977,10 @@ struct MGImpl {
   T* rbuf(int it) { return (it & 1) ? r1.p : r0.p; }
   T* pnew(int it) { return (it & 1) ? pa.p : pb.p; }  // p written by iteration it
 
+  bool built = false;  // a hierarchy has been set up
   void setup(const double* dE, const double* dN) {
     hipStream_t s = ctx->stream;
+    built = true;
     k_mg_build0<T><<<lv[0]->grid(), blk(), 0, s>>>(g, dE, dN, lv[0]->view());
     for (size_t l = 1; l < lv.size(); ++l)
       k_mg_coarsen<T><<<lv[l]->grid(), blk(), 0, s>>>(lv[l - 1]->view(), lv[l]->view());
@@ -1082,7 +1084,7 @@ PressureMG::~PressureMG() {
 static constexpr int kNoSyncCap = 32;
 
 void PressureMG::solve(const double* dE, const double* dN, double* r, double* x, IterState* st,
-                       const cps_simple_config& cfg, bool fp64, KernelTimer* timer) {
+                       const cps_simple_config& cfg, bool fp64, KernelTimer* timer, bool same_operator) {
   hipStream_t s = ctx_->stream;
   const int maxit = std::max(0, cfg.max_pressure_iter);
   const int sync_every = maxit > kNoSyncCap ? 4 : 0;
@@ -1090,7 +1092,7 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
     if (!impl64_) impl64_ = new MGImpl<double>(ctx_, g_);
     auto* m = static_cast<MGImpl<double>*>(impl64_);
     m->timer = timer;
-    m->setup(dE, dN);
+    if (!same_operator || !m->built) m->setup(dE, dN);
     m->pcg(r, cfg.p_tol, cfg.p_rtol, maxit, sync_every, nullptr);
     k_copy_out<double><<<m->lv[0]->grid(), dim3(MBX, MBY), 0, s>>>(m->view(0), m->xs.p, x, g_.P);
     k_copy_state<<<1, 64, 0, s>>>(st, m->scal.p, m->scal.p + 1, 0);
@@ -1105,7 +1107,7 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
     }
     double* b64 = static_cast<DevBuf<double>*>(b64_)->p;
     m->timer = timer;
-    m->setup(dE, dN);
+    if (!same_operator || !m->built) m->setup(dE, dN);
     // a kernel, not a memcpy node, so the step stays graph-capturable
     PcgScal* inner = m->scal.p;
     PcgScal* outer = m->scal.p + 1;

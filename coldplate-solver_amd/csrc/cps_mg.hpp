@@ -17,9 +17,11 @@ class PressureMG {
   // Solves A x = b for the pressure correction on the fine grid (field
   // layout, FP64).  r holds b on entry and the final FP64 residual on exit
   // (mixed) ; x is zero on entry.  FP64: MG-PCG in double.  Mixed: FP64
-  // iterative refinement around an FP32 MG-PCG.
+  // iterative refinement around an FP32 MG-PCG.  same_operator: dE, dN are
+  // unchanged since the previous solve in this precision, so the hierarchy
+  // built then is reused.
   void solve(const double* dE, const double* dN, double* r, double* x, IterState* st, const cps_simple_config& cfg,
-             bool fp64, KernelTimer* timer);
+             bool fp64, KernelTimer* timer, bool same_operator = false);
 
  private:
   Ctx* ctx_;
