@@ -430,6 +430,23 @@ __global__ __launch_bounds__(BS) void k_mcomb(int n, double a0, double* w, const
   }
 }
 
+// w -= sum_j c_j V_j with the coefficients on the device (same arithmetic
+// as k_mcomb with a0 = 1 and c = -coef)
+__global__ __launch_bounds__(BS) void k_mcomb_dev(int n, double* w, const double* Vb, int nv, const double* coef) {
+  for (int r = blockIdx.x * BS + threadIdx.x; r < n; r += gridDim.x * BS) {
+    double a = w[r];
+    for (int jv = 0; jv < nv; ++jv) a += (-coef[jv]) * Vb[(size_t)jv * n + r];
+    w[r] = a;
+  }
+}
+// y = x / sqrt(*ss) when *ss > 0 (the next Arnoldi vector)
+__global__ __launch_bounds__(BS) void k_scale_rn(int n, const double* x, const double* ss, double* y) {
+  const double q = *ss;
+  if (!(q > 0.0)) return;
+  const double a = 1.0 / sqrt(q);
+  for (int r = blockIdx.x * BS + threadIdx.x; r < n; r += gridDim.x * BS) y[r] = a * x[r];
+}
+
 __global__ __launch_bounds__(BS) void k_fill(int n, double a, double* y) {
   const int r = blockIdx.x * BS + threadIdx.x;
   if (r < n) y[r] = a;
@@ -864,7 +881,7 @@ class FlowAdjoint {
     Vb.alloc((size_t)(m + 1) * n);
     Zb.alloc((size_t)m * n);
     part.alloc((size_t)(m + 1) * nb);
-    sums.alloc(m + 1);
+    sums.alloc(2 * (m + 1) + 1);
   }
 
   FlowAdjoint(Ctx* c, const Geo& geo, size_t fieldS) : ctx(c), g(geo), S(fieldS) {
@@ -1020,10 +1037,15 @@ class FlowAdjoint {
   }
 
   // nv dot products V_j . x (j < nv), returned on the host
-  void mdot(const double* Vbase, int nv, const double* x, double* h) {
+  // out[j] = V_j . x on the device, j < nv
+  void mdot_dev(const double* Vbase, int nv, const double* x, double* out) {
     hipStream_t s = ctx->stream;
     adj::k_mdot<<<nb, adj::BS, 0, s>>>(n, Vbase, nv, x, part.p);
-    adj::k_psum<<<1, adj::BS, 0, s>>>(part.p, nb, nv, sums.p);
+    adj::k_psum<<<1, adj::BS, 0, s>>>(part.p, nb, nv, out);
+  }
+  void mdot(const double* Vbase, int nv, const double* x, double* h) {
+    hipStream_t s = ctx->stream;
+    mdot_dev(Vbase, nv, x, sums.p);
     CPS_HIP(hipMemcpyAsync(h, sums.p, nv * sizeof(double), hipMemcpyDeviceToHost, s));
     CPS_HIP(hipStreamSynchronize(s));
   }
@@ -1058,7 +1080,7 @@ class FlowAdjoint {
     CPS_HIP(hipMemcpyAsync(r.p, gvec.p, r.bytes(), hipMemcpyDeviceToDevice, s));
     double beta = gnorm;
     int it = 0;
-    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), e(m + 1), h(m + 1), h2(m + 1);
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), e(m + 1), h(m + 1), hh(2 * (m + 1) + 1);
     while (it < max_iter && beta > tol) {
       adj::k_scale<<<nb, adj::BS, 0, s>>>(n, r.p, 1.0 / beta, Vb.p);
       std::fill(e.begin(), e.end(), 0.0);
@@ -1069,21 +1091,23 @@ class FlowAdjoint {
         double* zk = Zb.p + (size_t)k * n;
         precond(V, vk, zk, pcfg);
         spmv(zk, w.p);
-        // classical Gram-Schmidt, twice
-        mdot(Vb.p, k + 1, w.p, h.data());
-        std::vector<double> c(h.begin(), h.begin() + k + 1);
-        for (auto& x : c) x = -x;
-        comb(1.0, w.p, Vb.p, c);
-        mdot(Vb.p, k + 1, w.p, h2.data());
-        for (int q = 0; q <= k; ++q) {
-          c[q] = -h2[q];
-          h[q] += h2[q];
-        }
-        comb(1.0, w.p, Vb.p, c);
-        const double hn = norm(w.p);
+        // classical Gram-Schmidt, twice, and the next basis vector, with the
+        // coefficients kept on the device: one read-back per step
+        double* hd = sums.p;               // h: first pass
+        double* h2d = sums.p + (m + 1);    // h2: second pass
+        double* nd = sums.p + 2 * (m + 1); // ||w||^2
+        mdot_dev(Vb.p, k + 1, w.p, hd);
+        adj::k_mcomb_dev<<<nb, adj::BS, 0, s>>>(n, w.p, Vb.p, k + 1, hd);
+        mdot_dev(Vb.p, k + 1, w.p, h2d);
+        adj::k_mcomb_dev<<<nb, adj::BS, 0, s>>>(n, w.p, Vb.p, k + 1, h2d);
+        mdot_dev(w.p, 1, w.p, nd);
+        adj::k_scale_rn<<<nb, adj::BS, 0, s>>>(n, w.p, nd, Vb.p + (size_t)(k + 1) * n);
+        CPS_HIP(hipMemcpyAsync(hh.data(), sums.p, (2 * (m + 1) + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+        CPS_HIP(hipStreamSynchronize(s));
+        for (int q = 0; q <= k; ++q) h[q] = hh[q] + hh[m + 1 + q];
+        const double hn = std::sqrt(hh[2 * (m + 1)]);
         for (int q = 0; q <= k; ++q) H[(size_t)q * m + k] = h[q];
         H[(size_t)(k + 1) * m + k] = hn;
-        if (hn > 0.0) adj::k_scale<<<nb, adj::BS, 0, s>>>(n, w.p, 1.0 / hn, Vb.p + (size_t)(k + 1) * n);
         // Givens rotations
         for (int q = 0; q < k; ++q) {
           const double a = H[(size_t)q * m + k], b = H[(size_t)(q + 1) * m + k];
