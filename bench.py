@@ -294,8 +294,9 @@ def main():
         gbs = kt["bytes_per_launch"] / (kt["avg_us"] * 1e-6) / 1e9
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
-                "kernel": "k_mg_down<float> on the fine grid with the PCG update fused (x += alpha p, "
-                          "r -= alpha Ap, 2 pressure smoother sweeps, residual, restriction)",
+                "kernel": "k_mg_down<float> on the fine grid with the single-reduction PCG update fused "
+                          "(p = z + beta p, s = w + beta s, x += alpha p, r -= alpha s, 2 pressure smoother "
+                          "sweeps, residual, restriction)",
                 "avg_us": round(kt["avg_us"], 3), "launches": kt["launches"],
                 "bytes_per_launch": kt["bytes_per_launch"]}
 

@@ -132,7 +132,29 @@ __global__ __launch_bounds__(MNT) void k_mg_coarsen(Lv<T> F, Lv<T> Cl) {
 struct PcgScal {
   double rzs[2], rr0, rr, tol2;
   int stop, iters, pad_[2];
+  double alphas[2];  // step lengths of the single-reduction form, by iteration parity
 };
+
+// Single-reduction (Chronopoulos-Gear) PCG step of iteration j from
+// gamma = r_j . z_j and delta = z_j . A z_j: beta_j = gamma / gamma_(j-1),
+// alpha_j = gamma / (delta - beta_j gamma / alpha_(j-1)); the denominator
+// equals p_j . A p_j of the classical form, and p_j = z_j + beta_j p_(j-1),
+// s_j = A z_j + beta_j s_(j-1) (= A p_j), x += alpha_j p_j, r -= alpha_j s_j.
+struct CgStep {
+  double alpha, beta, den;
+};
+__device__ inline CgStep cg_step(const PcgScal* ps, int j, double gamma, double delta) {
+  CgStep c;
+  if (j == 0) {
+    c.beta = 0.0;
+    c.den = delta;
+  } else {
+    c.beta = gamma / ps->rzs[(j - 1) & 1];
+    c.den = delta - c.beta * gamma / ps->alphas[(j - 1) & 1];
+  }
+  c.alpha = gamma / c.den;
+  return c;
+}
 
 // Convergence test carried by the first kernel of a V-cycle (fine level,
 // iteration it > 0): partial ||r||^2 of the previous update.
@@ -143,21 +165,27 @@ struct PcgTest {
   double abs_tol, rtol;  // it == 0: the target max(abs_tol, rtol ||r0||) is set here
 };
 
-// PCG update of the previous iteration carried by the fine-level down kernel
-// of the next V-cycle: alpha = rz / pAp, x += alpha p, r_new = r - alpha Ap.
-// r_new (tile + halo, recomputed by every block from r and Ap) is the
-// V-cycle's right side; the owner block also stores x and r_new (into the
-// other residual buffer: neighbours still read r for their halos) and the
-// partial ||r_new||^2 of the stop test.
+// PCG update of the previous iteration j = it - 1 carried by the fine-level
+// down kernel of the next V-cycle, in the single-reduction form (cg_step):
+// s = w + beta s_old and r_new = r - alpha s on the tile and its halo
+// (recomputed by every block), the V-cycle's right side; the owner block
+// stores p = z + beta p_old, s, x += alpha p and r_new (into the other
+// residual buffer: neighbours still read r for their halos) and the partial
+// ||r_new||^2 of the stop test.
 template <class T>
 struct XrFuse {
-  const T* rold;  // nullptr: no fused update
-  const T* p;
-  const T* Ap;
+  const T* rold;  // nullptr: no fused update; r_j (tile + halo)
+  const T* w;     // A z_j (tile + halo)
+  const T* sold;  // s_(j-1) (tile + halo; finite, times beta = 0 when j = 0)
+  T* snew;        // s_j
+  const T* z;     // z_j
+  const T* pold;  // p_(j-1) (finite, times beta = 0 when j = 0)
+  T* pnew;        // p_j
   T* xs;
-  const double* pap;  // pAp partials of iteration it - 1
-  double* err;        // ||r_new||^2 partials
-  int nb, it;         // it: this V-cycle; the update is iteration it - 1's
+  const double* rz;  // r_j . z_j partials
+  const double* zw;  // z_j . A z_j partials
+  double* err;       // ||r_new||^2 partials
+  int nb, it;        // it: this V-cycle; the update is iteration it - 1's
   PcgScal* ps;
 };
 
@@ -247,20 +275,21 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
   const int stop0 = t == 0 ? *stop : 0;  // thread 0's view, published below
   Tile<T> tl{sb, se, sn, (int)blockIdx.y * TY, (int)blockIdx.x * TX, L.nx, L.ny};
   if (xr.rold) {
-    const double ppap = parts_load<FT>(xr.pap, xr.nb);
-    const double rz = xr.ps->rzs[(xr.it - 1) & 1];
+    const double prz = parts_load<FT>(xr.rz, xr.nb);
+    const double pzw = parts_load<FT>(xr.zw, xr.nb);
     constexpr int NI = (RN + FT - 1) / FT;
-    T vr[NI], va[NI], ve[NI], vn[NI];
+    T vr[NI], vw[NI], vs[NI], ve[NI], vn[NI];
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
       const int q = t + i * FT;
       const int lr = q / RW, lc = q - (q / RW) * RW;
       const int r = tl.gr(lr), c = tl.gc(lc);
-      vr[i] = va[i] = ve[i] = vn[i] = (T)0;
+      vr[i] = vw[i] = vs[i] = ve[i] = vn[i] = (T)0;
       if (q < RN && tl.inside(r, c)) {
         const size_t k = L.at(r, c);
         vr[i] = xr.rold[k];
-        va[i] = xr.Ap[k];
+        vw[i] = xr.w[k];
+        vs[i] = xr.sold[k];
         ve[i] = L.ex[k];
         vn[i] = L.ey[k];
       }
@@ -269,25 +298,34 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
     const int r0 = tl.gr(lr0), c0 = tl.gc(lc0);
     const bool own = tl.inside(r0, c0);
     const size_t k0 = own ? L.at(r0, c0) : 0;
-    T xo = (T)0, pc = (T)0;
+    T xo = (T)0, zc = (T)0, po = (T)0, wo = (T)0, so = (T)0;
     if (own) {
       xo = xr.xs[k0];
-      pc = xr.p[k0];
+      zc = xr.z[k0];
+      po = xr.pold[k0];
+      wo = xr.w[k0];
+      so = xr.sold[k0];
     }
     if (t == 0) sflag = stop0;
     __syncthreads();
     if (sflag) return;  // block-uniform
-    const double pap = parts_total<FT>(ppap, sh);
-    if (fabs(pap) < kTiny20) {  // pressure_solver.cpp:161
+    const double gam = parts_total<FT>(prz, sh);
+    const double del = parts_total<FT>(pzw, sh);
+    const CgStep cs = cg_step(xr.ps, xr.it - 1, gam, del);
+    if (fabs(cs.den) < kTiny20) {  // p.Ap < 1e-20 (pressure_solver.cpp:161)
       if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) { xr.ps->stop = 1; xr.ps->iters = xr.it - 1; }
       return;
     }
-    const T alpha = (T)(rz / pap);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) {
+      xr.ps->rzs[(xr.it - 1) & 1] = gam;
+      xr.ps->alphas[(xr.it - 1) & 1] = cs.alpha;
+    }
+    const T alpha = (T)cs.alpha, beta = (T)cs.beta;
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
       const int q = t + i * FT;
       if (q < RN) {
-        sb[q] = vr[i] - alpha * va[i];
+        sb[q] = vr[i] - alpha * (vw[i] + beta * vs[i]);
         se[q] = ve[i];
         sn[q] = vn[i];
       }
@@ -296,7 +334,10 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
     double e = 0.0;
     if (own) {
       const T rn = sb[q0];
-      xr.xs[k0] = xo + alpha * pc;
+      const T pn = zc + beta * po;
+      xr.xs[k0] = xo + alpha * pn;
+      xr.pnew[k0] = pn;
+      xr.snew[k0] = wo + beta * so;
       L.b[k0] = rn;
       e = (double)rn * (double)rn;
     }
@@ -360,7 +401,8 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
 // sweeps, result in L.y; with part != nullptr also the partial b . z of the
 // block (the PCG's r . z on the fine level)
 template <class T>
-__global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, const int* stop) {
+__global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, const int* stop, T* w,
+                                             double* part2) {
   __shared__ T sb[RN], se[RN], sn[RN], s0[RN], s1[RN];
   __shared__ int sflag;
   const int t = threadIdx.x + threadIdx.y * TX;
@@ -370,6 +412,35 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
   if (t == 0) sflag = stop0;
   __syncthreads();
   if (sflag) return;
+  if (w) {
+    // single-reduction PCG: z on the tile and its halo-1 ring (the same
+    // arithmetic as the owner blocks'), w = A z on the tile, partials r.z
+    // and z.w
+    tl.sweep(s0, s1, 1, t);  // halo-2 ring
+    __syncthreads();
+    tl.sweep(s1, s0, 2, t);  // z on the halo-1 ring
+    __syncthreads();
+    double e = 0.0, e2 = 0.0;
+    const int lr = HL + threadIdx.y, lc = HL + threadIdx.x, q = lr * RW + lc;
+    const int r = tl.gr(lr), c = tl.gc(lc);
+    if (tl.inside(r, c)) {
+      const T z = s0[q];
+      const T wz = tl.ax(s0, q, r, c, tl.diag(q, r, c));
+      const size_t k = L.at(r, c);
+      L.y[k] = z;
+      w[k] = wz;
+      e = (double)sb[q] * (double)z;
+      e2 = (double)z * (double)wz;
+    }
+    __shared__ double sh2[FT / kWave + 1];
+    const double a = block_sum<FT>(e, sh2);
+    const double b2 = block_sum<FT>(e2, sh2);
+    if (t == 0) {
+      part[blockIdx.x + blockIdx.y * gridDim.x] = a;
+      part2[blockIdx.x + blockIdx.y * gridDim.x] = b2;
+    }
+    return;
+  }
   tl.sweep(s0, s1, 2, t);  // halo-1 ring
   __syncthreads();
   double e = 0.0;
@@ -655,7 +726,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
 // PCG pieces on level 0.  Reductions: each block stores its partial; the next
 // kernel on the stream sums them in fixed order (no in-kernel fences).
 
-enum MgSlot { S_RR0 = 0, S_RZ = 1, S_PAP = 2, S_ERR = 3, S_IR = 4, S_COUNT = 5 };
+enum MgSlot { S_RR0 = 0, S_RZ = 1, S_PAP = 2, S_ERR = 3, S_IR = 4, S_ZW = 5, S_COUNT = 6 };
 
 struct Grid0 {  // tile grid of the level-0 kernels (one TX x TY tile per block)
   int gx, gy, NB;
@@ -773,6 +844,43 @@ __global__ __launch_bounds__(FT) void k_pcg_xr(Grid0 G, Lv<T> L, T* xv, T* rv, c
   }
   const double sblk = block_sum<FT>(e, sh);
   if (t == 0) part[(size_t)S_ERR * G.NB + bid()] = sblk;
+}
+
+// the last iteration's update in the single-reduction form (pointwise: x +=
+// alpha p, r -= alpha s in place in r_j's buffer), partial ||r||^2
+template <class T>
+__global__ __launch_bounds__(FT) void k_pcg_cg_last(Grid0 G, Lv<T> L, XrFuse<T> xr, T* rv) {
+  __shared__ double sh[FT / kWave + 1];
+  __shared__ int sflag;
+  const int t = tid();
+  const int stop0 = t == 0 ? xr.ps->stop : 0;
+  const double prz = parts_load<FT>(xr.rz, G.NB);
+  const double pzw = parts_load<FT>(xr.zw, G.NB);
+  const int c = blockIdx.x * TX + threadIdx.x, r = blockIdx.y * TY + threadIdx.y;
+  const bool in = r < L.ny && c < L.nx;
+  const size_t k = in ? L.at(r, c) : 0;
+  T xo = 0, rr = 0, zc = 0, po = 0, wo = 0, so = 0;
+  if (in) { xo = xr.xs[k]; rr = rv[k]; zc = xr.z[k]; po = xr.pold[k]; wo = xr.w[k]; so = xr.sold[k]; }
+  if (t == 0) sflag = stop0;
+  __syncthreads();
+  if (sflag) return;
+  const double gam = parts_total<FT>(prz, sh);
+  const double del = parts_total<FT>(pzw, sh);
+  const CgStep cs = cg_step(xr.ps, xr.it - 1, gam, del);
+  if (fabs(cs.den) < kTiny20) {
+    if (bid() == 0 && t == 0) { xr.ps->stop = 1; xr.ps->iters = xr.it - 1; }
+    return;
+  }
+  const T alpha = (T)cs.alpha, beta = (T)cs.beta;
+  double e = 0.0;
+  if (in) {
+    xr.xs[k] = xo + alpha * (zc + beta * po);
+    const T rn = rr - alpha * (wo + beta * so);
+    rv[k] = rn;
+    e = (double)rn * (double)rn;
+  }
+  const double sblk = block_sum<FT>(e, sh);
+  if (t == 0) xr.err[bid()] = sblk;
 }
 
 // after K launched iterations: stop test of iteration K-1 (one block)
@@ -895,6 +1003,7 @@ struct MGImpl {
   size_t coarseSmem = 0, coarseSetupSmem = 0;
   DevBuf<T> cf;  // coarse-solver faces and inverse diagonals
   DevBuf<T> r0, r1, xs, pa, pb, Ap;  // PCG vectors, level-0 layout (xs = solution; r0 / r1 residual by parity)
+  DevBuf<T> sa, sb, wz;              // single-reduction form: s = A p by parity, w = A z
   T* b0cur = nullptr;                   // the level-0 right side of the running V-cycle
   DevBuf<double> part;
   DevBuf<PcgScal> scal;          // [0] inner, [1] outer
@@ -912,7 +1021,7 @@ struct MGImpl {
     L0->off = g.P + 1;
     const size_t S0 = (size_t)(g.M + 1) * g.P;
     for (auto* b : {&L0->x, &L0->y, &L0->ex, &L0->ey}) b->alloc(S0);
-    for (auto* b : {&r0, &r1, &xs, &pa, &pb, &Ap}) b->alloc(S0);
+    for (auto* b : {&r0, &r1, &xs, &pa, &pb, &Ap, &sa, &sb, &wz}) b->alloc(S0);
     b0cur = r0.p;
     lv.push_back(std::move(L0));
     // at least one grid level: its last post-smoothing sweep carries the
@@ -954,7 +1063,7 @@ struct MGImpl {
     CPS_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
     CPS_CHECK(coarseSmem <= (size_t)lds, CPS_ERR_STATE, "coarse multigrid levels exceed LDS");
     cf.alloc((size_t)3 * off + (size_t)cd.dn * cd.dn);
-    for (auto* b : {&lv[0]->x, &lv[0]->y, &lv[0]->ex, &lv[0]->ey, &r0, &r1, &xs, &pa, &pb, &Ap})
+    for (auto* b : {&lv[0]->x, &lv[0]->y, &lv[0]->ex, &lv[0]->ey, &r0, &r1, &xs, &pa, &pb, &Ap, &sa, &sb, &wz})
       CPS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), ctx->stream));
     G0.gx = ceil_div(lv[0]->nx, TX);
     G0.gy = ceil_div(lv[0]->ny, TY);
@@ -976,6 +1085,14 @@ struct MGImpl {
   }
   T* rbuf(int it) { return (it & 1) ? r1.p : r0.p; }
   T* pnew(int it) { return (it & 1) ? pa.p : pb.p; }  // p written by iteration it
+  T* snew(int it) { return (it & 1) ? sa.p : sb.p; }  // s written by iteration it (single-reduction form)
+  // the update of iteration j (single-reduction form): reads r_j, z_j, w_j,
+  // p_(j-1), s_(j-1); writes p_j, s_j
+  XrFuse<T> cg_update(int j) {
+    return XrFuse<T>{rbuf(j), wz.p, snew(j - 1), snew(j), lv[0]->y.p, pnew(j - 1), pnew(j), xs.p,
+                     part.p + (size_t)S_RZ * G0.NB, part.p + (size_t)S_ZW * G0.NB,
+                     part.p + (size_t)S_ERR * G0.NB, G0.NB, j + 1, scal.p};
+  }
 
   bool built = false;  // a hierarchy has been set up
   void setup(const double* dE, const double* dN) {
@@ -998,10 +1115,8 @@ struct MGImpl {
     const PcgTest none{nullptr, 0, 0, nullptr, 0.0, 0.0};
     const PcgTest test{part.p + (size_t)(it == 0 ? S_RR0 : S_ERR) * G0.NB, G0.NB, it, scal.p, abs_tol_, rtol_};
     for (int l = 0; l < nGrid; ++l) {
-      XrFuse<T> xr{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr};
-      if (l == 0 && fused && it > 0)
-        xr = XrFuse<T>{rbuf(it - 1), pnew(it - 1), Ap.p, xs.p, part.p + (size_t)S_PAP * G0.NB,
-                       part.p + (size_t)S_ERR * G0.NB, G0.NB, it, scal.p};
+      XrFuse<T> xr{};
+      if (l == 0 && fused && it > 0) xr = cg_update(it - 1);
       const PcgTest dt = (l == 0 && !fused) ? test : none;
       // the timed launches are the fused ones when fusing (the byte count is theirs)
       if (l == 0 && (!fused || it > 0) && timer->active(KT_MG_SMOOTH0))
@@ -1012,7 +1127,9 @@ struct MGImpl {
     k_mg_coarse<T><<<1, kCoarseThreads, coarseSmem, s>>>(view(nGrid), cd, cf.p, stop, fused ? test : none);
     for (int l = nGrid - 1; l >= 0; --l)
       k_mg_up<T><<<tgrid(l), dim3(TX, TY), 0, s>>>(view(l), view(l + 1),
-                                                  l == 0 ? part.p + (size_t)S_RZ * G0.NB : nullptr, stop);
+                                                  l == 0 ? part.p + (size_t)S_RZ * G0.NB : nullptr, stop,
+                                                  l == 0 && fused ? wz.p : nullptr,
+                                                  part.p + (size_t)S_ZW * G0.NB);
     CPS_HIP(hipGetLastError());
   }
 
@@ -1022,16 +1139,18 @@ struct MGImpl {
   void pcg(const double* rhs, double abs_tol, double rtol, int maxit, int sync_every, const int* outer_stop) {
     hipStream_t s = ctx->stream;
     const double nP = (double)lv[0]->nx * (double)lv[0]->ny;
-    // Without host polling the PCG update rides on the next V-cycle's fine
-    // down kernel (one launch fewer per iteration); with polling every
-    // iteration ends with its own update kernel, so the host can read the
-    // stop test.
+    // Without host polling the PCG runs in its single-reduction form: the
+    // fine up kernel also forms w = A z and z . w, and the update rides on
+    // the next V-cycle's fine down kernel (two launches fewer per
+    // iteration); with polling every iteration is the classical form with
+    // its own stencil and update kernels, so the host can read the stop test.
     const bool fused = sync_every == 0;
     // fine-level down kernel: b, east and north faces read, x1 and the
-    // coarse rhs (1/4) written; fused, also r, Ap, p, x read and x, r written
-    // (b is formed from r and Ap); PCG stencil: z, p_old, faces read, p and
-    // Ap written
-    if (timer->which == KT_MG_SMOOTH0) timer->bytes_per_launch = (fused ? 9.25 : 4.25) * sizeof(T) * nP;
+    // coarse rhs (1/4) written; fused, the faces, r, w, s_old (tile + halo)
+    // and z, p_old, x read, x, p, s, r, x1 and the coarse rhs (1/4) written
+    // (b is formed from r, w and s_old); PCG stencil: z, p_old, faces read,
+    // p and Ap written
+    if (timer->which == KT_MG_SMOOTH0) timer->bytes_per_launch = (fused ? 13.25 : 4.25) * sizeof(T) * nP;
     if (timer->which == KT_PCG_STENCIL) timer->bytes_per_launch = 6.0 * sizeof(T) * nP;
     const Lv<T> L = view(0);
     PcgScal* ps = scal.p;
@@ -1044,15 +1163,17 @@ struct MGImpl {
     for (int it = 0; it < maxit; ++it) {
       b0cur = fused ? rbuf(it) : r0.p;
       vcycle(&ps->stop, it, fused);
-      const Lv<T> Lc = view(0);
-      T* po = (it & 1) ? pb.p : pa.p;
-      T* pn = pnew(it);
-      if (timer->active(KT_PCG_STENCIL))
-        timer->ext_launch(k_pcg_ap<T>, grid0(), blk0(), 0, s, G0, Lc, (const T*)lv[0]->y.p, (const T*)po, pn, Ap.p,
-                          it, ps, part.p);
-      else
-        k_pcg_ap<T><<<grid0(), blk0(), 0, s>>>(G0, Lc, lv[0]->y.p, po, pn, Ap.p, it, ps, part.p);
-      if (!fused) k_pcg_xr<T><<<grid0(), blk0(), 0, s>>>(G0, Lc, xs.p, r0.p, pn, Ap.p, it, ps, part.p);
+      if (!fused) {
+        const Lv<T> Lc = view(0);
+        T* po = (it & 1) ? pb.p : pa.p;
+        T* pn = pnew(it);
+        if (timer->active(KT_PCG_STENCIL))
+          timer->ext_launch(k_pcg_ap<T>, grid0(), blk0(), 0, s, G0, Lc, (const T*)lv[0]->y.p, (const T*)po, pn,
+                            Ap.p, it, ps, part.p);
+        else
+          k_pcg_ap<T><<<grid0(), blk0(), 0, s>>>(G0, Lc, lv[0]->y.p, po, pn, Ap.p, it, ps, part.p);
+        k_pcg_xr<T><<<grid0(), blk0(), 0, s>>>(G0, Lc, xs.p, r0.p, pn, Ap.p, it, ps, part.p);
+      }
       CPS_HIP(hipGetLastError());
       if (sync_every > 0 && (it + 1) % sync_every == 0 && it + 1 < maxit) {
         k_pcg_end<<<1, blk0(), 0, s>>>(G0, it + 1, ps, part.p);
@@ -1064,8 +1185,7 @@ struct MGImpl {
     }
     // the update of the last iteration (in place in its residual buffer)
     if (fused && maxit > 0)
-      k_pcg_xr<T><<<grid0(), blk0(), 0, s>>>(G0, view(0), xs.p, rbuf(maxit - 1), pnew(maxit - 1), Ap.p, maxit - 1,
-                                             ps, part.p);
+      k_pcg_cg_last<T><<<grid0(), blk0(), 0, s>>>(G0, view(0), cg_update(maxit - 1), rbuf(maxit - 1));
     k_pcg_end<<<1, blk0(), 0, s>>>(G0, maxit, ps, part.p);
   }
 };

@@ -274,7 +274,7 @@ int cps_batch_gradient_dp(cps_batch* b, int field, double rtol, int max_iter, do
  * the counters. */
 #define CPS_KT_MG_SMOOTH0 0   /* fine-level multigrid Jacobi smoother (pressure) */
 #define CPS_KT_MOM_JACOBI 1   /* fused U+V momentum Jacobi sweep */
-#define CPS_KT_PCG_STENCIL 2  /* fine-level PCG p-update + 5-point apply */
+#define CPS_KT_PCG_STENCIL 2  /* fine-level PCG p-update + 5-point apply (classical form: caps above 32) */
 int cps_simple_time_kernel(cps_simple* s, int kernel);
 /* launches timed so far, their mean duration [us] and the algorithmic bytes
  * one launch moves (every operand read once, every result written once). */

@@ -276,6 +276,28 @@ def test_sor_pressure_matches_oracle(pkg, orc, seed, nx, ny, scheme, maxit, ptol
     assert np.all(gpu.get("pStar")[1:-1, cfg.nx] == 0.0)  # column N-1 pinned
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_single_reduction_pcg_matches_classical(pkg, precision):
+    """The multigrid PCG runs in its single-reduction form when the cap is
+    small (<= 32 iterations, no host polling) and in the classical form
+    otherwise; both solve the pressure system of the same SIMPLE step to
+    ||r|| <= 1e-11 ||r0||, so the corrected fields agree to that accuracy."""
+    T = pkg._types
+    fields = []
+    for cap in (30, 40):
+        cfg, gamma, inlet = pkg.configs.density_field(nx=97, ny=41, seed=5, sigma=3.0)
+        cfg.pressure_solver, cfg.precision = T.CPS_PRESSURE_GMG, precision
+        cfg.p_tol, cfg.p_rtol, cfg.max_pressure_iter, cfg.max_refine = 0.0, 1e-11, cap, 8
+        print(f"case cap={cap} precision={precision}", flush=True)
+        s = pkg.SIMPLE(cfg, gamma, inlet)
+        st = s.step(3)
+        assert st["p_iters"] < cap
+        fields.append({n: s.get(n) for n in ("u", "v", "p", "pStar")})
+        s.close()
+    for n in ("u", "v", "p", "pStar"):
+        assert rel_l2(fields[0][n], fields[1][n]) < 1e-8, n
+
+
 def test_sor_rejected_in_mixed_precision(pkg):
     cfg, gamma, inlet = pkg.configs.density_field(nx=16, ny=8, seed=0)
     cfg.pressure_solver = pkg._types.CPS_PRESSURE_SOR
