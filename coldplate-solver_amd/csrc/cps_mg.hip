@@ -742,20 +742,31 @@ __device__ inline int tid() { return threadIdx.x + threadIdx.y * TX; }
 
 // r = src (cast), x = 0, partial ||r||^2.  An outer refinement loop that has
 // already converged (outer_stop) stops the inner solve at once.
+// With keep != nullptr (the first refinement sweep) the FP64 right side is
+// also kept in keep (DOF cells) and the refinement state `reset` is cleared.
 template <class T, class S>
 __global__ __launch_bounds__(FT) void k_pcg_start(Grid0 G, Lv<T> L, const S* src, T* xv, T* rv, PcgScal* ps,
-                                                  const int* outer_stop, double* part) {
+                                                  const int* outer_stop, double* part, S* keep, PcgScal* reset) {
   __shared__ double sh[FT / kWave + 1];
   if (outer_stop && *outer_stop) {
     if (bid() == 0 && tid() == 0) { ps->stop = 1; ps->iters = 0; }
     return;
   }
-  if (bid() == 0 && tid() == 0) { ps->stop = 0; ps->iters = 0; }
+  if (bid() == 0 && tid() == 0) {
+    ps->stop = 0;
+    ps->iters = 0;
+    if (reset) {
+      reset->stop = 0;
+      reset->iters = 0;
+    }
+  }
   double e = 0.0;
   L0_LOOP(G, L, r, c) {
     if (c >= L.nx) continue;
     const size_t k = L.at(r, c);
-    const T v = (T)src[k];
+    const S sv = src[k];
+    if (keep) keep[k] = sv;
+    const T v = (T)sv;
     rv[k] = v;
     xv[k] = (T)0;
     e += (double)v * (double)v;
@@ -905,9 +916,13 @@ __global__ __launch_bounds__(MNT) void k_ir_add(Lv<float> L, const float* e, dou
   x64[(size_t)(r + 1) * P + c + 1] += (double)e[L.at(r, c)];
 }
 
+// r64 = b64 - A x64 with the FP64 operator; partial ||r64||^2.  first: the
+// first refinement sweep, whose x64 is zero on entry, so x64 + e = e: the
+// sweep's k_ir_add is folded in (x64 = e stored by the owner, neighbours
+// read e, which nobody writes here).
 __global__ __launch_bounds__(FT) void k_ir_resid(Grid0 G, Geo g, Lv<float> L, const double* dE, const double* dN,
-                                                 const double* x64, const double* b64, double* r64, const int* stop,
-                                                 double* part) {
+                                                 double* x64, const double* b64, double* r64, const int* stop,
+                                                 double* part, const float* e, int first) {
   __shared__ double sh[FT / kWave + 1];
   if (*stop) return;
   const int M = g.M, N = g.N, P = g.P;
@@ -922,11 +937,14 @@ __global__ __launch_bounds__(FT) void k_ir_resid(Grid0 G, Geo g, Lv<float> L, co
     const double aS = (i > 1) ? g.rho * dN[f - P] * g.hx : 0.0;
     double aP = aE + aW + aN + aS;
     if (aP < kTiny20) aP = 1.0;
-    double ax = aP * x64[f];
-    if (i > 1) ax -= aS * x64[f - P];
-    if (j > 1) ax -= aW * x64[f - 1];
-    if (j + 1 <= N - 2) ax -= aE * x64[f + 1];
-    if (i < M - 1) ax -= aN * x64[f + P];
+    auto xv = [&](int rr, int cc, size_t ff) -> double { return first ? (double)e[L.at(rr, cc)] : x64[ff]; };
+    const double xc = xv(r, c, f);
+    if (first) x64[f] = xc;
+    double ax = aP * xc;
+    if (i > 1) ax -= aS * xv(r - 1, c, f - P);
+    if (j > 1) ax -= aW * xv(r, c - 1, f - 1);
+    if (j + 1 <= N - 2) ax -= aE * xv(r, c + 1, f + 1);
+    if (i < M - 1) ax -= aN * xv(r + 1, c, f + P);
     const double rn = b64[f] - ax;
     r64[f] = rn;
     e2 += rn * rn;
@@ -937,10 +955,21 @@ __global__ __launch_bounds__(FT) void k_ir_resid(Grid0 G, Geo g, Lv<float> L, co
 
 // outer test: ||r64||^2 <= max(abs, rtol ||b||)^2 (||b||^2 = rr0 of the first
 // inner solve); one block
-__global__ __launch_bounds__(FT) void k_ir_check(Grid0 G, PcgScal* outer, const PcgScal* first, int o,
-                                                 double abs_tol, double rtol, const double* part, IterState* st) {
+// end_K > 0: first runs the inner solve's end-of-loop stop test (that of
+// k_pcg_end after end_K launched iterations), which this kernel then reads.
+__global__ __launch_bounds__(FT) void k_ir_check(Grid0 G, PcgScal* outer, PcgScal* first, int o,
+                                                 double abs_tol, double rtol, const double* part, IterState* st,
+                                                 int end_K) {
   __shared__ double sh[FT / kWave + 1];
   __shared__ int shi;
+  if (end_K > 0 && !block_read(&first->stop, &shi)) {
+    const double err = parts_sum<FT>(part + (size_t)S_ERR * G.NB, G.NB, sh);
+    if (tid() == 0) {
+      first->rr = err;
+      first->iters = end_K;
+      if (err <= first->tol2) first->stop = 1;
+    }
+  }
   if (block_read(&outer->stop, &shi)) return;
   const double rr = parts_sum<FT>(part + (size_t)S_IR * G.NB, G.NB, sh);
   if (tid() == 0) {
@@ -966,15 +995,6 @@ __global__ void k_copy_state(IterState* st, const PcgScal* inner, const PcgScal*
     st->err = mixed ? outer->rr : inner->rr;
     st->ir_outer = mixed ? outer->iters : 0;
   }
-}
-
-// dst = src; block 0 also resets the refinement loop's state
-__global__ __launch_bounds__(256) void k_copy_f64(const double* src, double* dst, size_t n, PcgScal* outer) {
-  if (outer && blockIdx.x == 0 && threadIdx.x == 0) {
-    outer->stop = 0;
-    outer->iters = 0;
-  }
-  for (size_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) dst[q] = src[q];
 }
 
 template <class T>
@@ -1136,7 +1156,10 @@ struct MGImpl {
   // MG-PCG: A xs = rhs (rhs FP64 in the field layout).  Iterations run with
   // device-side termination; with sync_every > 0 the host polls the stop
   // flag every sync_every iterations.
-  void pcg(const double* rhs, double abs_tol, double rtol, int maxit, int sync_every, const int* outer_stop) {
+  // keep / reset: see k_pcg_start; end = false leaves the end-of-loop stop
+  // test to the caller (k_ir_check)
+  void pcg(const double* rhs, double abs_tol, double rtol, int maxit, int sync_every, const int* outer_stop,
+           double* keep = nullptr, PcgScal* reset = nullptr, bool end = true) {
     hipStream_t s = ctx->stream;
     const double nP = (double)lv[0]->nx * (double)lv[0]->ny;
     // Without host polling the PCG runs in its single-reduction form: the
@@ -1157,7 +1180,7 @@ struct MGImpl {
     abs_tol_ = abs_tol;
     rtol_ = rtol;
     b0cur = r0.p;
-    k_pcg_start<T, double><<<grid0(), blk0(), 0, s>>>(G0, L, rhs, xs.p, r0.p, ps, outer_stop, part.p);
+    k_pcg_start<T, double><<<grid0(), blk0(), 0, s>>>(G0, L, rhs, xs.p, r0.p, ps, outer_stop, part.p, keep, reset);
     // p_old of iteration 0 is pa: zero since creation, finite afterwards, and
     // multiplied by beta = 0
     for (int it = 0; it < maxit; ++it) {
@@ -1186,7 +1209,7 @@ struct MGImpl {
     // the update of the last iteration (in place in its residual buffer)
     if (fused && maxit > 0)
       k_pcg_cg_last<T><<<grid0(), blk0(), 0, s>>>(G0, view(0), cg_update(maxit - 1), rbuf(maxit - 1));
-    k_pcg_end<<<1, blk0(), 0, s>>>(G0, maxit, ps, part.p);
+    if (end) k_pcg_end<<<1, blk0(), 0, s>>>(G0, maxit, ps, part.p);
   }
 };
 
@@ -1228,19 +1251,25 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
     double* b64 = static_cast<DevBuf<double>*>(b64_)->p;
     m->timer = timer;
     if (!same_operator || !m->built) m->setup(dE, dN);
-    // a kernel, not a memcpy node, so the step stays graph-capturable
+    // The first sweep's PCG start also keeps the FP64 right side in b64 and
+    // clears the refinement state; its x64 update (x is zero on entry) is
+    // folded into the residual kernel; without host polling the inner
+    // solve's end test runs in k_ir_check.  All kernels, no memcpy or
+    // memset nodes, so the step stays graph-capturable.
     PcgScal* inner = m->scal.p;
     PcgScal* outer = m->scal.p + 1;
-    k_copy_f64<<<std::min(ceil_div((int)S, 256), 2048), 256, 0, s>>>(r, b64, S, outer);
     const Lv<float> L = m->view(0);
     const double inner_rtol = std::max(cfg.p_rtol, 1e-5);
     const int max_outer = std::max(1, cfg.max_refine);
     const bool poll = sync_every > 0 || max_outer > 4;
+    const int endK = sync_every == 0 ? maxit : 0;
     for (int o = 0; o < max_outer; ++o) {
-      m->pcg(r, 0.0, inner_rtol, maxit, sync_every, &outer->stop);
-      k_ir_add<<<m->lv[0]->grid(), dim3(MBX, MBY), 0, s>>>(L, m->xs.p, x, g_.P, &outer->stop);
-      k_ir_resid<<<m->grid0(), m->blk0(), 0, s>>>(m->G0, g_, L, dE, dN, x, b64, r, &outer->stop, m->part.p);
-      k_ir_check<<<1, m->blk0(), 0, s>>>(m->G0, outer, inner, o, cfg.p_tol, cfg.p_rtol, m->part.p, st);
+      if (o == 0) m->pcg(r, 0.0, inner_rtol, maxit, sync_every, nullptr, b64, outer, endK == 0);
+      else m->pcg(r, 0.0, inner_rtol, maxit, sync_every, &outer->stop, nullptr, nullptr, endK == 0);
+      if (o > 0) k_ir_add<<<m->lv[0]->grid(), dim3(MBX, MBY), 0, s>>>(L, m->xs.p, x, g_.P, &outer->stop);
+      k_ir_resid<<<m->grid0(), m->blk0(), 0, s>>>(m->G0, g_, L, dE, dN, x, b64, r, &outer->stop, m->part.p, m->xs.p,
+                                                o == 0 ? 1 : 0);
+      k_ir_check<<<1, m->blk0(), 0, s>>>(m->G0, outer, inner, o, cfg.p_tol, cfg.p_rtol, m->part.p, st, endK);
       CPS_HIP(hipGetLastError());
       if (poll && o + 1 < max_outer) {
         int stop = 0;
