@@ -497,7 +497,7 @@ __global__ __launch_bounds__(NT) void k_prhs(Geo g, Fields f, PcgBufs q, double*
 // reduction the order is that of parts_sum / parts_max (thread t adds
 // p[t], p[t + NT], ..., then the wavefronts in order), so the results are
 // the same.
-__global__ __launch_bounds__(NT) void k_stats(Geo g, IterState* st, const double* part) {
+__global__ __launch_bounds__(NT) void k_stats(Geo g, IterState* st, const double* part, int reset_pcg) {
   constexpr int NW = NT / kWave;
   __shared__ double sh[8][NW];
   const int NB = g.NB, t = thread_id();
@@ -546,9 +546,11 @@ __global__ __launch_bounds__(NT) void k_stats(Geo g, IterState* st, const double
     st->error = fmax(st->resid_u, fmax(st->resid_v, st->resid_mass));
     st->rz0 = rz;
     st->bnorm2 = bb;
-    st->pcg_stop = 0;
-    st->pcg_iters = 0;
-    st->err = bb;
+    if (reset_pcg) {
+      st->pcg_stop = 0;
+      st->pcg_iters = 0;
+      st->err = bb;
+    }
   }
 }
 
@@ -1134,7 +1136,10 @@ class SimpleSolver {
     const Fields f = fields();
     momentum();
     k_prhs<<<grid, block, 0, s>>>(g, f, pbufs(), part.p);
-    k_stats<<<1, block, 0, s>>>(g, st.p, part.p);
+    // the Jacobi-PCG and SOR solves start from the statistics' r.z and
+    // ||b||^2; the multigrid solve reads none of them, so there the
+    // reductions wait until the host asks (stats())
+    if (cfg.pressure_solver != CPS_PRESSURE_GMG) k_stats<<<1, block, 0, s>>>(g, st.p, part.p, 1);
     CPS_HIP(hipGetLastError());
     if (cfg.pressure_solver == CPS_PRESSURE_PCG) pressure_pcg();
     else if (cfg.pressure_solver == CPS_PRESSURE_SOR) pressure_sor();
@@ -1145,6 +1150,12 @@ class SimpleSolver {
   }
 
   void stats(cps_step_stats* out) {
+    // multigrid steps leave their statistics' partials in place (SLOT_FIN,
+    // SLOT_RHS are written once per step and read by nothing else)
+    if (cfg.pressure_solver == CPS_PRESSURE_GMG) {
+      k_stats<<<1, block, 0, ctx->stream>>>(g, st.p, part.p, 0);
+      CPS_HIP(hipGetLastError());
+    }
     IterState h{};
     CPS_HIP(hipMemcpyAsync(&h, st.p, sizeof(IterState), hipMemcpyDeviceToHost, ctx->stream));
     CPS_HIP(hipStreamSynchronize(ctx->stream));
