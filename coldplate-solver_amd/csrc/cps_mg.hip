@@ -83,21 +83,6 @@ __device__ inline T apply_at(const Lv<T>& L, const T* x, int r, int c) {
   return ax;
 }
 
-// level-0 faces from dE, dN (pressure_solver.cpp:589-594) in type T; the
-// diagonal keeps the reference floor.
-template <class T>
-__global__ __launch_bounds__(MNT) void k_mg_build0(Geo g, const double* dE, const double* dN, Lv<T> L) {
-  const int c = blockIdx.x * MBX + threadIdx.x, r = blockIdx.y * MBY + threadIdx.y;
-  if (r >= L.ny || c >= L.nx) return;
-  const int i = r + 1, j = c + 1, M = g.M;
-  const size_t f = (size_t)i * g.P + j;
-  const double aE = g.rho * dE[f] * g.hy;
-  const double aN = (i < M - 1) ? g.rho * dN[f] * g.hx : 0.0;
-  const size_t k = L.at(r, c);
-  L.ex[k] = (T)aE;
-  L.ey[k] = (T)aN;
-}
-
 // coarse faces: half the sum of the covered fine faces
 template <class T>
 __device__ inline T coarse_ex(const Lv<T>& F, int R, int C) {
@@ -113,6 +98,80 @@ __device__ inline T coarse_ey(const Lv<T>& F, int R, int C) {
   T s = F.ey[F.at(row, 2 * C)];
   if (2 * C + 1 < F.nx) s += F.ey[F.at(row, 2 * C + 1)];
   return (T)kCoarseScale * s;
+}
+
+// Level 0 and up to kHierLevels coarser levels in one launch: coarse faces
+// are half the sums of the faces they cover (coarse_ex / coarse_ey), a purely
+// local 2x2 aggregation, so a block that owns a HX x HY level-0 tile owns the
+// nested tiles of every coarser level.  Level 0: aE = rho dE hy, aN = rho dN
+// hx (0 on the last row), the pressure operator's faces
+// (pressure_solver.cpp:589-594); coarser levels: the arithmetic of
+// k_mg_coarsen.
+constexpr int kHierLevels = 4, HX = 128, HY = 32, HT = 1024;
+template <class T>
+struct HierLevels {
+  Lv<T> l[kHierLevels + 1];
+  int n;  // coarser levels built here (<= kHierLevels)
+};
+template <class T>
+__global__ __launch_bounds__(HT) void k_mg_hier(Geo g, const double* dE, const double* dN, HierLevels<T> H) {
+  // even levels in the level-0-sized buffers, odd levels in the quarter-sized ones
+  __shared__ T sEa[HY * HX], sNa[HY * HX], sEb[(HY / 2) * (HX / 2)], sNb[(HY / 2) * (HX / 2)];
+  const int t = threadIdx.x;
+  const int M = g.M;
+  int cx0 = blockIdx.x * HX, cy0 = blockIdx.y * HY;  // tile origin at the current level
+  int tw = HX, th = HY;                              // tile size at the current level
+  {
+    const Lv<T>& L = H.l[0];
+    for (int q = t; q < HX * HY; q += HT) {
+      const int lr = q / HX, lc = q - lr * HX, r = cy0 + lr, c = cx0 + lc;
+      if (r >= L.ny || c >= L.nx) continue;
+      const int i = r + 1, j = c + 1;
+      const size_t f = (size_t)i * g.P + j;
+      const double aE = g.rho * dE[f] * g.hy;
+      const double aN = (i < M - 1) ? g.rho * dN[f] * g.hx : 0.0;
+      const size_t k = L.at(r, c);
+      L.ex[k] = (T)aE;
+      L.ey[k] = (T)aN;
+      sEa[q] = (T)aE;
+      sNa[q] = (T)aN;
+    }
+  }
+  for (int l = 1; l <= H.n; ++l) {
+    __syncthreads();
+    const Lv<T>& F = H.l[l - 1];
+    const Lv<T>& Cl = H.l[l];
+    const T* fe = (l & 1) ? sEa : sEb;  // the finer level's faces
+    const T* fn = (l & 1) ? sNa : sNb;
+    T* ce = (l & 1) ? sEb : sEa;
+    T* cn = (l & 1) ? sNb : sNa;
+    const int fw = tw;  // row length of the finer tile in LDS
+    cx0 >>= 1;
+    cy0 >>= 1;
+    tw >>= 1;
+    th >>= 1;
+    for (int q = t; q < tw * th; q += HT) {
+      const int lr = q / tw, lc = q - lr * tw, R = cy0 + lr, C = cx0 + lc;
+      if (R >= Cl.ny || C >= Cl.nx) continue;
+      // fine local (2 lr + dr, 2 lc + dc); the east column clamps to the
+      // last fine column, the second row exists when 2R + 1 < F.ny
+      const int lcol = 2 * C + 1 <= F.nx - 1 ? 2 * lc + 1 : 2 * lc;
+      T e = fe[(2 * lr) * fw + lcol];
+      if (2 * R + 1 < F.ny) e += fe[(2 * lr + 1) * fw + lcol];
+      T n = (T)0;
+      if (2 * R + 1 <= F.ny - 1) {
+        n = fn[(2 * lr + 1) * fw + 2 * lc];
+        if (2 * C + 1 < F.nx) n += fn[(2 * lr + 1) * fw + 2 * lc + 1];
+      }
+      e = (T)kCoarseScale * e;
+      n = (T)kCoarseScale * n;
+      const size_t k = Cl.at(R, C);
+      Cl.ex[k] = e;
+      Cl.ey[k] = n;
+      ce[lr * tw + lc] = e;
+      cn[lr * tw + lc] = n;
+    }
+  }
 }
 
 template <class T>
@@ -1118,8 +1177,13 @@ struct MGImpl {
   void setup(const double* dE, const double* dN) {
     hipStream_t s = ctx->stream;
     built = true;
-    k_mg_build0<T><<<lv[0]->grid(), blk(), 0, s>>>(g, dE, dN, lv[0]->view());
-    for (size_t l = 1; l < lv.size(); ++l)
+    // level 0 and the next kHierLevels levels in one launch, any deeper
+    // levels one launch each
+    HierLevels<T> H{};
+    H.n = std::min((int)lv.size() - 1, kHierLevels);
+    for (int l = 0; l <= H.n; ++l) H.l[l] = lv[l]->view();
+    k_mg_hier<T><<<dim3(ceil_div(lv[0]->nx, HX), ceil_div(lv[0]->ny, HY)), HT, 0, s>>>(g, dE, dN, H);
+    for (size_t l = H.n + 1; l < lv.size(); ++l)
       k_mg_coarsen<T><<<lv[l]->grid(), blk(), 0, s>>>(lv[l - 1]->view(), lv[l]->view());
     k_mg_coarse_setup<T><<<1, kCoarseThreads, coarseSetupSmem, s>>>(view(nGrid), cd, cf.p);
     CPS_HIP(hipGetLastError());
