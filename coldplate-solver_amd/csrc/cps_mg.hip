@@ -1293,6 +1293,7 @@ static constexpr int kNoSyncCap = 32;
 void PressureMG::solve(const double* dE, const double* dN, double* r, double* x, IterState* st,
                        const cps_simple_config& cfg, bool fp64, KernelTimer* timer, bool same_operator) {
   hipStream_t s = ctx_->stream;
+  pending_.on = false;  // a new solve supersedes an unread check
   const int maxit = std::max(0, cfg.max_pressure_iter);
   const int sync_every = maxit > kNoSyncCap ? 4 : 0;
   if (fp64) {
@@ -1333,6 +1334,10 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
       if (o > 0) k_ir_add<<<m->lv[0]->grid(), dim3(MBX, MBY), 0, s>>>(L, m->xs.p, x, g_.P, &outer->stop);
       k_ir_resid<<<m->grid0(), m->blk0(), 0, s>>>(m->G0, g_, L, dE, dN, x, b64, r, &outer->stop, m->part.p, m->xs.p,
                                                 o == 0 ? 1 : 0);
+      if (max_outer == 1 && !poll) {  // the only sweep: its check waits for finish()
+        pending_ = PendingCheck{true, cfg.p_tol, cfg.p_rtol, endK};
+        continue;
+      }
       k_ir_check<<<1, m->blk0(), 0, s>>>(m->G0, outer, inner, o, cfg.p_tol, cfg.p_rtol, m->part.p, st, endK);
       CPS_HIP(hipGetLastError());
       if (poll && o + 1 < max_outer) {
@@ -1343,6 +1348,17 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
       }
     }
   }
+  CPS_HIP(hipGetLastError());
+}
+
+void PressureMG::finish(IterState* st) {
+  if (!pending_.on) return;
+  pending_.on = false;
+  auto* m = static_cast<MGImpl<float>*>(impl32_);
+  PcgScal* inner = m->scal.p;
+  PcgScal* outer = m->scal.p + 1;
+  k_ir_check<<<1, m->blk0(), 0, ctx_->stream>>>(m->G0, outer, inner, 0, pending_.abs_tol, pending_.rtol, m->part.p,
+                                                st, pending_.endK);
   CPS_HIP(hipGetLastError());
 }
 

@@ -22,10 +22,19 @@ class PressureMG {
   // built then is reused.
   void solve(const double* dE, const double* dN, double* r, double* x, IterState* st, const cps_simple_config& cfg,
              bool fp64, KernelTimer* timer, bool same_operator = false);
+  // Mixed solves with one refinement sweep leave their closing check (the
+  // pressure statistics in IterState) until the host reads the statistics:
+  // no later kernel of the step reads them.  Launches it if one is pending.
+  void finish(IterState* st);
 
  private:
   Ctx* ctx_;
   Geo g_;
+  struct PendingCheck {
+    bool on = false;
+    double abs_tol = 0.0, rtol = 0.0;
+    int endK = 0;
+  } pending_;
   void* impl64_ = nullptr;  // MGImpl<double>
   void* impl32_ = nullptr;  // MGImpl<float>
   void* b64_ = nullptr;     // FP64 rhs kept across refinement sweeps

@@ -1155,6 +1155,7 @@ class SimpleSolver {
     if (cfg.pressure_solver == CPS_PRESSURE_GMG) {
       k_stats<<<1, block, 0, ctx->stream>>>(g, st.p, part.p, 0);
       CPS_HIP(hipGetLastError());
+      if (mg) mg->finish(st.p);
     }
     IterState h{};
     CPS_HIP(hipMemcpyAsync(&h, st.p, sizeof(IterState), hipMemcpyDeviceToHost, ctx->stream));
