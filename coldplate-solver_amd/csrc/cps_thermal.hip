@@ -345,11 +345,9 @@ __global__ __launch_bounds__(TNT) void k_zfactor(Op<T> A, T* zc, T* zdi) {
 // unconditional and can be issued ahead of the recurrences (FMAs); layers
 // past nz are masked out of the recurrences and stores.
 template <class T, int NZ>
-__global__ __launch_bounds__(TNT) void k_zjac(Lvl<T> L, const T* zc, const T* zdi, const T* src, T* dst,
-                                             const T* cy, Op<T> C, int mode) {
-  const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+__device__ inline void zjac_col(const Lvl<T>& L, const T* zc, const T* zdi, const T* src, T* dst, const T* cy,
+                                const Op<T>& C, int mode, int i, int j) {
   const Op<T>& A = L.A;
-  if (i >= A.nx || j >= A.ny || L.stopped()) return;
   const int nz = A.nz;
   const bool hw = i > 0, he = i + 1 < A.nx, hs = j > 0, hn = j + 1 < A.ny;
   const int I2 = i >> 1, J2 = j >> 1;
@@ -393,12 +391,17 @@ __global__ __launch_bounds__(TNT) void k_zjac(Lvl<T> L, const T* zc, const T* zd
     }
   }
 }
+template <class T, int NZ>
+__global__ __launch_bounds__(TNT) void k_zjac(Lvl<T> L, const T* zc, const T* zdi, const T* src, T* dst,
+                                             const T* cy, Op<T> C, int mode) {
+  const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+  if (i >= L.A.nx || j >= L.A.ny || L.stopped()) return;
+  zjac_col<T, NZ>(L, zc, zdi, src, dst, cy, C, mode, i, j);
+}
 
 // coarse right side = sum of the fine residuals over the aggregate
 template <class T>
-__global__ __launch_bounds__(TNT) void k_restrict(Lvl<T> L, const T* x, Op<T> C, T* crhs) {
-  const int I = blockIdx.x * TBX + threadIdx.x, J = blockIdx.y * TBY + threadIdx.y, k = blockIdx.z;
-  if (I >= C.nx || J >= C.ny || L.stopped()) return;
+__device__ inline void restrict_cell(const Lvl<T>& L, const T* x, const Op<T>& C, T* crhs, int I, int J, int k) {
   T acc = 0;
   for (int dj = 0; dj < 2; ++dj)
     for (int di = 0; di < 2; ++di) {
@@ -407,13 +410,18 @@ __global__ __launch_bounds__(TNT) void k_restrict(Lvl<T> L, const T* x, Op<T> C,
     }
   crhs[C.at(I, J, k)] = acc;
 }
+template <class T>
+__global__ __launch_bounds__(TNT) void k_restrict(Lvl<T> L, const T* x, Op<T> C, T* crhs) {
+  const int I = blockIdx.x * TBX + threadIdx.x, J = blockIdx.y * TBY + threadIdx.y, k = blockIdx.z;
+  if (I >= C.nx || J >= C.ny || L.stopped()) return;
+  restrict_cell(L, x, C, crhs, I, J, k);
+}
 
 // coarsest level: one x-y column, tridiagonal in z (Thomas algorithm in
 // FP64; the operator is an M-matrix, so no pivoting is needed)
 constexpr int kMaxNz = 64;
 template <class T>
-__global__ void k_coarsest(Lvl<T> L) {
-  if (threadIdx.x != 0 || L.stopped()) return;
+__device__ inline void coarsest_col(const Lvl<T>& L) {
   const int nz = L.A.nz;
   double c[kMaxNz], g[kMaxNz];
   for (int k = 0; k < nz; ++k) {
@@ -428,6 +436,61 @@ __global__ void k_coarsest(Lvl<T> L) {
   for (int k = nz - 2; k >= 0; --k) {
     xk = g[k] - c[k] * xk;
     L.y[L.A.at(0, 0, k)] = (T)xk;
+  }
+}
+template <class T>
+__global__ void k_coarsest(Lvl<T> L) {
+  if (threadIdx.x != 0 || L.stopped()) return;
+  coarsest_col(L);
+}
+
+// The coarse end of the V-cycle (levels lc .. the 1x1 column) in one
+// workgroup: the per-column and per-cell work of the level kernels above,
+// in the same order, phase by phase with workgroup barriers.  The levels'
+// arrays stay in global memory; a barrier makes one phase's stores visible
+// to the next phase's loads within the workgroup.  One launch replaces five
+// per level.
+constexpr int kThCoarseMaxLv = 12;
+constexpr int kThCoarseCols = TNT;  // the first level with at most this many columns starts it
+template <class T>
+struct ThCoarse {
+  Lvl<T> L[kThCoarseMaxLv];
+  const T* zc[kThCoarseMaxLv];
+  const T* zdi[kThCoarseMaxLv];
+  T* rhs[kThCoarseMaxLv];  // writable right sides (L[l].rhs), l >= 1
+  int n;                   // levels; L[0] = level lc (right side given), L[n-1] = 1 x 1
+};
+template <class T, int NZ>
+__global__ __launch_bounds__(TNT) void k_th_coarse(ThCoarse<T> H) {
+  if (H.L[0].stopped()) return;  // uniform
+  const int t = threadIdx.x;
+  const Op<T> none{};
+  const int last = H.n - 1;
+  for (int l = 0; l < last; ++l) {
+    const Lvl<T>& L = H.L[l];
+    const int ncol = L.A.nx * L.A.ny;
+    for (int q = t; q < ncol; q += TNT) zjac_col<T, NZ>(L, H.zc[l], H.zdi[l], nullptr, L.x, nullptr, none, 0, q % L.A.nx, q / L.A.nx);
+    __syncthreads();
+    for (int q = t; q < ncol; q += TNT) zjac_col<T, NZ>(L, H.zc[l], H.zdi[l], L.x, L.y, nullptr, none, 1, q % L.A.nx, q / L.A.nx);
+    __syncthreads();
+    const Op<T>& C = H.L[l + 1].A;
+    const int ncc = C.nx * C.ny;
+    for (int q = t; q < ncc * C.nz; q += TNT) {
+      const int k = q / ncc, rest = q - k * ncc;
+      restrict_cell(L, L.y, C, H.rhs[l + 1], rest % C.nx, rest / C.nx, k);
+    }
+    __syncthreads();
+  }
+  if (t == 0) coarsest_col(H.L[last]);
+  __syncthreads();
+  for (int l = last - 1; l >= 0; --l) {
+    const Lvl<T>& L = H.L[l];
+    const int ncol = L.A.nx * L.A.ny;
+    for (int q = t; q < ncol; q += TNT)
+      zjac_col<T, NZ>(L, H.zc[l], H.zdi[l], L.y, L.x, H.L[l + 1].y, H.L[l + 1].A, 2, q % L.A.nx, q / L.A.nx);
+    __syncthreads();
+    for (int q = t; q < ncol; q += TNT) zjac_col<T, NZ>(L, H.zc[l], H.zdi[l], L.x, L.y, nullptr, none, 1, q % L.A.nx, q / L.A.nx);
+    __syncthreads();
   }
 }
 
@@ -990,6 +1053,18 @@ struct ThermalKrylov {
     else CPS_ZJAC(64);
 #undef CPS_ZJAC
   }
+  void coarse(const th::ThCoarse<T>& H) {
+    const int nz = lv[0]->nz;
+#define CPS_THC(NZ) th::k_th_coarse<T, NZ><<<1, th::TNT, 0, st>>>(H)
+    if (nz <= 4) CPS_THC(4);
+    else if (nz <= 8) CPS_THC(8);
+    else if (nz <= 12) CPS_THC(12);
+    else if (nz <= 16) CPS_THC(16);
+    else if (nz <= 24) CPS_THC(24);
+    else if (nz <= 32) CPS_THC(32);
+    else CPS_THC(64);
+#undef CPS_THC
+  }
 
   // out = V-cycle(in) on level 0 (in and out distinct)
   void precond(const TK* in, TK* out) {
@@ -1011,7 +1086,13 @@ struct ThermalKrylov {
       return th::Lvl<T>{L.op(), L.x.p, L.y.p, L.rhs.p, stop_flag};
     };
     const th::Op<T> none{};
-    for (int l = 0; l < last; ++l) {
+    // levels lc .. last run in one workgroup (z-line smoother only)
+    int lc = last;
+    if (kZLine) {
+      lc = 0;
+      while (lc < last && lv[lc]->nx * lv[lc]->ny > th::kThCoarseCols) ++lc;
+    }
+    for (int l = 0; l < lc; ++l) {
       const th::Lvl<T> L = lvl(l);
       if (kZLine) {
         zjac(l, L, nullptr, L.x, nullptr, none, 0);
@@ -1022,8 +1103,21 @@ struct ThermalKrylov {
       }
       th::k_restrict<T><<<lv[l + 1]->grid(), B, 0, st>>>(L, L.y, lv[l + 1]->op(), lv[l + 1]->rhs.p);
     }
-    th::k_coarsest<T><<<1, 64, 0, st>>>(lvl(last));
-    for (int l = last - 1; l >= 0; --l) {
+    if (kZLine) {
+      th::ThCoarse<T> H{};
+      H.n = last - lc + 1;
+      CPS_CHECK(H.n <= th::kThCoarseMaxLv, CPS_ERR_STATE, "thermal coarse levels exceed the coarse kernel's table");
+      for (int q = 0; q < H.n; ++q) {
+        H.L[q] = lvl(lc + q);
+        H.zc[q] = lv[lc + q]->zc.p;
+        H.zdi[q] = lv[lc + q]->zdi.p;
+        H.rhs[q] = q == 0 ? nullptr : lv[lc + q]->rhs.p;
+      }
+      coarse(H);
+    } else {
+      th::k_coarsest<T><<<1, 64, 0, st>>>(lvl(last));
+    }
+    for (int l = lc - 1; l >= 0; --l) {
       const th::Lvl<T> L = lvl(l);
       if (kZLine) {
         zjac(l, L, L.y, L.x, lv[l + 1]->y.p, lv[l + 1]->op(), 2);
