@@ -979,12 +979,43 @@ __global__ __launch_bounds__(MNT) void k_ir_add(Lv<float> L, const float* e, dou
 // first refinement sweep, whose x64 is zero on entry, so x64 + e = e: the
 // sweep's k_ir_add is folded in (x64 = e stored by the owner, neighbours
 // read e, which nobody writes here).
+// The last single-reduction PCG update, folded into the first sweep's
+// residual kernel: e = xs + alpha (z + beta p_old) at each cell read (the
+// arithmetic of k_pcg_cg_last), unless the inner solve stopped.
+struct CgFinal {
+  const float* z;
+  const float* pold;
+  const double* rz;  // r.z and z.w partials of iteration j
+  const double* zw;
+  const PcgScal* ps;
+  int j;  // -1: no update
+};
 __global__ __launch_bounds__(FT) void k_ir_resid(Grid0 G, Geo g, Lv<float> L, const double* dE, const double* dN,
                                                  double* x64, const double* b64, double* r64, const int* stop,
-                                                 double* part, const float* e, int first) {
+                                                 double* part, const float* e, int first, CgFinal cf) {
   __shared__ double sh[FT / kWave + 1];
+  __shared__ int sfl;
   if (*stop) return;
   const int M = g.M, N = g.N, P = g.P;
+  float alpha = 0.0f, beta = 0.0f;
+  bool upd = false;
+  if (first && cf.j >= 0) {
+    const double prz = parts_load<FT>(cf.rz, G.NB);
+    const double pzw = parts_load<FT>(cf.zw, G.NB);
+    if (tid() == 0) sfl = cf.ps->stop;
+    __syncthreads();
+    const bool stopped = sfl != 0;  // block-uniform
+    const double gam = parts_total<FT>(prz, sh);
+    const double del = parts_total<FT>(pzw, sh);
+    if (!stopped) {
+      const CgStep cs = cg_step(cf.ps, cf.j, gam, del);
+      if (!(fabs(cs.den) < kTiny20)) {
+        upd = true;
+        alpha = (float)cs.alpha;
+        beta = (float)cs.beta;
+      }
+    }
+  }
   double e2 = 0.0;
   L0_LOOP(G, L, r, c) {
     if (c >= L.nx) continue;
@@ -996,7 +1027,11 @@ __global__ __launch_bounds__(FT) void k_ir_resid(Grid0 G, Geo g, Lv<float> L, co
     const double aS = (i > 1) ? g.rho * dN[f - P] * g.hx : 0.0;
     double aP = aE + aW + aN + aS;
     if (aP < kTiny20) aP = 1.0;
-    auto xv = [&](int rr, int cc, size_t ff) -> double { return first ? (double)e[L.at(rr, cc)] : x64[ff]; };
+    auto xv = [&](int rr, int cc, size_t ff) -> double {
+      if (!first) return x64[ff];
+      const size_t k = L.at(rr, cc);
+      return upd ? (double)(e[k] + alpha * (cf.z[k] + beta * cf.pold[k])) : (double)e[k];
+    };
     const double xc = xv(r, c, f);
     if (first) x64[f] = xc;
     double ax = aP * xc;
@@ -1014,20 +1049,17 @@ __global__ __launch_bounds__(FT) void k_ir_resid(Grid0 G, Geo g, Lv<float> L, co
 
 // outer test: ||r64||^2 <= max(abs, rtol ||b||)^2 (||b||^2 = rr0 of the first
 // inner solve); one block
-// end_K > 0: first runs the inner solve's end-of-loop stop test (that of
-// k_pcg_end after end_K launched iterations), which this kernel then reads.
+// end_K > 0: first records the inner solve's iteration count after end_K
+// launched iterations (k_pcg_end's), which this kernel then reads.
 __global__ __launch_bounds__(FT) void k_ir_check(Grid0 G, PcgScal* outer, PcgScal* first, int o,
                                                  double abs_tol, double rtol, const double* part, IterState* st,
                                                  int end_K) {
   __shared__ double sh[FT / kWave + 1];
   __shared__ int shi;
   if (end_K > 0 && !block_read(&first->stop, &shi)) {
-    const double err = parts_sum<FT>(part + (size_t)S_ERR * G.NB, G.NB, sh);
-    if (tid() == 0) {
-      first->rr = err;
-      first->iters = end_K;
-      if (err <= first->tol2) first->stop = 1;
-    }
+    // the inner solve ran all end_K iterations (its last update is folded
+    // into k_ir_resid, which leaves no residual partials to test)
+    if (tid() == 0) first->iters = end_K;
   }
   if (block_read(&outer->stop, &shi)) return;
   const double rr = parts_sum<FT>(part + (size_t)S_IR * G.NB, G.NB, sh);
@@ -1223,7 +1255,7 @@ struct MGImpl {
   // keep / reset: see k_pcg_start; end = false leaves the end-of-loop stop
   // test to the caller (k_ir_check)
   void pcg(const double* rhs, double abs_tol, double rtol, int maxit, int sync_every, const int* outer_stop,
-           double* keep = nullptr, PcgScal* reset = nullptr, bool end = true) {
+           double* keep = nullptr, PcgScal* reset = nullptr, bool end = true, bool last = true) {
     hipStream_t s = ctx->stream;
     const double nP = (double)lv[0]->nx * (double)lv[0]->ny;
     // Without host polling the PCG runs in its single-reduction form: the
@@ -1271,7 +1303,7 @@ struct MGImpl {
       }
     }
     // the update of the last iteration (in place in its residual buffer)
-    if (fused && maxit > 0)
+    if (fused && maxit > 0 && last)
       k_pcg_cg_last<T><<<grid0(), blk0(), 0, s>>>(G0, view(0), cg_update(maxit - 1), rbuf(maxit - 1));
     if (end) k_pcg_end<<<1, blk0(), 0, s>>>(G0, maxit, ps, part.p);
   }
@@ -1329,11 +1361,18 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
     const bool poll = sync_every > 0 || max_outer > 4;
     const int endK = sync_every == 0 ? maxit : 0;
     for (int o = 0; o < max_outer; ++o) {
-      if (o == 0) m->pcg(r, 0.0, inner_rtol, maxit, sync_every, nullptr, b64, outer, endK == 0);
+      // the first sweep's last PCG update (fused path) rides on k_ir_resid
+      const bool fold = o == 0 && endK > 0 && maxit > 0;
+      if (o == 0) m->pcg(r, 0.0, inner_rtol, maxit, sync_every, nullptr, b64, outer, endK == 0, !fold);
       else m->pcg(r, 0.0, inner_rtol, maxit, sync_every, &outer->stop, nullptr, nullptr, endK == 0);
       if (o > 0) k_ir_add<<<m->lv[0]->grid(), dim3(MBX, MBY), 0, s>>>(L, m->xs.p, x, g_.P, &outer->stop);
+      CgFinal cfin{nullptr, nullptr, nullptr, nullptr, m->scal.p, -1};
+      if (fold) {
+        const XrFuse<float> u = m->cg_update(maxit - 1);
+        cfin = CgFinal{u.z, u.pold, u.rz, u.zw, m->scal.p, maxit - 1};
+      }
       k_ir_resid<<<m->grid0(), m->blk0(), 0, s>>>(m->G0, g_, L, dE, dN, x, b64, r, &outer->stop, m->part.p, m->xs.p,
-                                                o == 0 ? 1 : 0);
+                                                o == 0 ? 1 : 0, cfin);
       if (max_outer == 1 && !poll) {  // the only sweep: its check waits for finish()
         pending_ = PendingCheck{true, cfg.p_tol, cfg.p_rtol, endK};
         continue;
