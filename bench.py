@@ -176,7 +176,25 @@ def grads_batch_leg(pkg, n_fields=32, nx=512, ny=256, forward_steps=1500, rtol=1
                 all_converged=ok)
 
 
-def thermal_leg(pkg, reps=3, flow_steps=3000):
+def thermal_cpu_baseline(pkg, st, gamma, u, v, maxit=4000):
+    """The oracle's CPU restatement of the thermal solve on the same system:
+    assembly (thermal_solver.cpp:437-530) plus Jacobi-preconditioned
+    BiCGStab (OpenMP) to rtol 1e-6, at most maxit iterations.  AMGCL, the
+    reference's solver, is not available here; this is the stated stand-in."""
+    import __graft_entry__ as entry
+
+    orc = entry.load_oracle()
+    t0 = time.perf_counter()
+    o = orc.OracleThermal(pkg._types, st, gamma, u, v)
+    t_asm = time.perf_counter() - t0
+    o.solve(rtol=1e-6, maxit=maxit)
+    return {"seconds": t_asm + o.last_seconds, "assembly_seconds": t_asm, "iterations": abs(o.last_iters),
+            "converged": o.last_iters >= 0, "rel_residual": o.last_relres,
+            "cores": int(os.environ.get("OMP_NUM_THREADS", "0")) or None,
+            "kind": "port", "solver": "Jacobi-BiCGStab (oracle), rtol 1e-6, <= %d iterations" % maxit}
+
+
+def thermal_leg(pkg, reps=3, flow_steps=3000, cpu=True):
     """512 x 256 x 12 conjugate-heat-transfer solve (4 Cu base, 4 channel,
     4 Cu lid layers) to the reference's rtol 1e-6, FP64 and FP32-refined.
     The channel velocity is the cell-centred field of a 512 x 256 density-
@@ -213,6 +231,8 @@ def thermal_leg(pkg, reps=3, flow_steps=3000):
         if name == "fp64":
             out["tb_max"] = s.metrics(8.0)["tb_max"]
         s.close()
+    if cpu:
+        out["cpu_baseline"] = thermal_cpu_baseline(pkg, st, gamma, u, v)
     return out
 
 
@@ -301,7 +321,7 @@ def main():
                 "bytes_per_launch": kt["bytes_per_launch"]}
 
     grads = None if args.no_grads else grads_leg(pkg)
-    thermal = None if args.no_thermal else thermal_leg(pkg)
+    thermal = None if args.no_thermal else thermal_leg(pkg, cpu=not args.no_cpu)
     batch = None if args.no_batch else batch_leg(pkg)
     grads_batch = None if args.no_grads else grads_batch_leg(pkg)
     cpu = None if args.no_cpu else cpu_baseline(pkg, nx, ny, args.cpu_seconds)
