@@ -123,6 +123,46 @@ __device__ inline T block_max(T v, T* sh) {
   return r;
 }
 
+// Sum over the block, result in thread 0, without the trailing barrier of
+// block_sum: for the last use of sh before a barrier the caller places.
+template <int NT, class T>
+__device__ inline T block_sum_t0(T v, T* sh) {
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+  v = wave_sum(v);
+  if ((t & (kWave - 1)) == 0) sh[t / kWave] = v;
+  __syncthreads();
+  T r = T(0);
+  if (t == 0) {
+#pragma unroll
+    for (int k = 0; k < NT / kWave; ++k) r += sh[k];
+  }
+  return r;
+}
+// Two sums over the block in block_sum's order (wave sums, then the
+// wavefronts in order), delivered to every thread after one barrier.  sh2
+// holds 2 * NT / kWave entries and must not be written again before a later
+// barrier.  Equal, bit for bit, to parts_total / block_sum of each value.
+template <int NT>
+__device__ inline void block_sum2_all(double& a, double& b, double* sh2) {
+  constexpr int NW = NT / kWave;
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if ((t & (kWave - 1)) == 0) {
+    sh2[t / kWave] = a;
+    sh2[NW + t / kWave] = b;
+  }
+  __syncthreads();
+  double ra = 0.0, rb = 0.0;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    ra += sh2[k];
+    rb += sh2[NW + k];
+  }
+  a = ra;
+  b = rb;
+}
+
 // Fixed-order sum / max of n block partials written by an earlier kernel on
 // the same stream (the kernel boundary makes them visible; no in-kernel
 // fences).  Every thread of the block gets the result.  sh needs

@@ -329,6 +329,7 @@ template <class T>
 __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* stop, PcgTest test, XrFuse<T> xr) {
   __shared__ T sb[RN], se[RN], sn[RN], s0[RN], s1[RN];
   __shared__ double sh[FT / kWave + 1];
+  __shared__ double sh2[2 * (FT / kWave)];
   __shared__ int sflag;
   const int t = threadIdx.x + threadIdx.y * TX;
   const int stop0 = t == 0 ? *stop : 0;  // thread 0's view, published below
@@ -366,10 +367,9 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
       so = xr.sold[k0];
     }
     if (t == 0) sflag = stop0;
-    __syncthreads();
-    if (sflag) return;  // block-uniform
-    const double gam = parts_total<FT>(prz, sh);
-    const double del = parts_total<FT>(pzw, sh);
+    double gam = prz, del = pzw;
+    block_sum2_all<FT>(gam, del, sh2);  // its barrier also publishes sflag
+    if (sflag) return;                  // block-uniform
     const CgStep cs = cg_step(xr.ps, xr.it - 1, gam, del);
     if (fabs(cs.den) < kTiny20) {  // p.Ap < 1e-20 (pressure_solver.cpp:161)
       if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) { xr.ps->stop = 1; xr.ps->iters = xr.it - 1; }
@@ -400,7 +400,7 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
       L.b[k0] = rn;
       e = (double)rn * (double)rn;
     }
-    const double sblk = block_sum<FT>(e, sh);
+    const double sblk = block_sum_t0<FT>(e, sh);  // the barrier after this branch follows
     if (t == 0) xr.err[blockIdx.x + blockIdx.y * gridDim.x] = sblk;
   } else {
   const double perr = test.part ? parts_load<FT>(test.part, test.nb) : 0.0;
@@ -491,12 +491,11 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
       e = (double)sb[q] * (double)z;
       e2 = (double)z * (double)wz;
     }
-    __shared__ double sh2[FT / kWave + 1];
-    const double a = block_sum<FT>(e, sh2);
-    const double b2 = block_sum<FT>(e2, sh2);
+    __shared__ double sh2[2 * (FT / kWave)];
+    block_sum2_all<FT>(e, e2, sh2);
     if (t == 0) {
-      part[blockIdx.x + blockIdx.y * gridDim.x] = a;
-      part2[blockIdx.x + blockIdx.y * gridDim.x] = b2;
+      part[blockIdx.x + blockIdx.y * gridDim.x] = e;
+      part2[blockIdx.x + blockIdx.y * gridDim.x] = e2;
     }
     return;
   }
@@ -932,10 +931,10 @@ __global__ __launch_bounds__(FT) void k_pcg_cg_last(Grid0 G, Lv<T> L, XrFuse<T> 
   T xo = 0, rr = 0, zc = 0, po = 0, wo = 0, so = 0;
   if (in) { xo = xr.xs[k]; rr = rv[k]; zc = xr.z[k]; po = xr.pold[k]; wo = xr.w[k]; so = xr.sold[k]; }
   if (t == 0) sflag = stop0;
-  __syncthreads();
+  __shared__ double sh2[2 * (FT / kWave)];
+  double gam = prz, del = pzw;
+  block_sum2_all<FT>(gam, del, sh2);  // its barrier also publishes sflag
   if (sflag) return;
-  const double gam = parts_total<FT>(prz, sh);
-  const double del = parts_total<FT>(pzw, sh);
   const CgStep cs = cg_step(xr.ps, xr.it - 1, gam, del);
   if (fabs(cs.den) < kTiny20) {
     if (bid() == 0 && t == 0) { xr.ps->stop = 1; xr.ps->iters = xr.it - 1; }
@@ -1000,13 +999,12 @@ __global__ __launch_bounds__(FT) void k_ir_resid(Grid0 G, Geo g, Lv<float> L, co
   float alpha = 0.0f, beta = 0.0f;
   bool upd = false;
   if (first && cf.j >= 0) {
-    const double prz = parts_load<FT>(cf.rz, G.NB);
-    const double pzw = parts_load<FT>(cf.zw, G.NB);
+    double gam = parts_load<FT>(cf.rz, G.NB);
+    double del = parts_load<FT>(cf.zw, G.NB);
     if (tid() == 0) sfl = cf.ps->stop;
-    __syncthreads();
-    const bool stopped = sfl != 0;  // block-uniform
-    const double gam = parts_total<FT>(prz, sh);
-    const double del = parts_total<FT>(pzw, sh);
+    __shared__ double sh2[2 * (FT / kWave)];
+    block_sum2_all<FT>(gam, del, sh2);  // its barrier also publishes sfl
+    const bool stopped = sfl != 0;      // block-uniform
     if (!stopped) {
       const CgStep cs = cg_step(cf.ps, cf.j, gam, del);
       if (!(fabs(cs.den) < kTiny20)) {
