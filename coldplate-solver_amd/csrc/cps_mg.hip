@@ -456,6 +456,194 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
   }
 }
 
+// A region of one level in LDS: the TW x TH cells from (r0, c0) with a ring
+// of H cells; the arithmetic of Tile (diagonal, A x, damped sweeps) on it.
+template <class T, int TW, int TH, int H>
+struct Region {
+  static constexpr int W = TW + 2 * H, HH = TH + 2 * H, N = W * HH;
+  T* sb;
+  T* se;
+  T* sn;
+  int r0, c0, nx, ny;
+  __device__ int gr(int lr) const { return r0 - H + lr; }
+  __device__ int gc(int lc) const { return c0 - H + lc; }
+  __device__ bool inside(int r, int c) const { return r >= 0 && r < ny && c >= 0 && c < nx; }
+  __device__ T diag(int q, int r, int c) const {
+    T d = se[q] + (c > 0 ? se[q - 1] : (T)0) + sn[q] + (r > 0 ? sn[q - W] : (T)0);
+    return d < (T)kTiny20 ? (T)1 : d;
+  }
+  __device__ T ax(const T* x, int q, int r, int c, T d) const {
+    T a = d * x[q];
+    if (c + 1 < nx) a -= se[q] * x[q + 1];
+    if (c > 0) a -= se[q - 1] * x[q - 1];
+    if (r + 1 < ny) a -= sn[q] * x[q + W];
+    if (r > 0) a -= sn[q - W] * x[q - W];
+    return a;
+  }
+  // x0 = omega b / diag on [h, HH - h) x [h, W - h)
+  __device__ void first(T* dst, int h, int t) const {
+    for (int q = t; q < N; q += FT) {
+      const int lr = q / W, lc = q - (q / W) * W;
+      if (lr < h || lr >= HH - h || lc < h || lc >= W - h) continue;
+      const int r = gr(lr), c = gc(lc);
+      dst[q] = inside(r, c) ? (T)kOmega * sb[q] / diag(q, r, c) : (T)0;
+    }
+  }
+  __device__ void sweep(const T* src, T* dst, int h, int t) const {
+    for (int q = t; q < N; q += FT) {
+      const int lr = q / W, lc = q - (q / W) * W;
+      if (lr < h || lr >= HH - h || lc < h || lc >= W - h) continue;
+      const int r = gr(lr), c = gc(lc);
+      if (inside(r, c)) {
+        const T d = diag(q, r, c);
+        dst[q] = src[q] + (T)kOmega * (sb[q] - ax(src, q, r, c, d)) / d;
+      } else {
+        dst[q] = (T)0;
+      }
+    }
+  }
+  // residual b - A x1 on [h, HH - h) x [h, W - h) (0 outside the level)
+  __device__ void resid(const T* x1, T* dst, int h, int t) const {
+    for (int q = t; q < N; q += FT) {
+      const int lr = q / W, lc = q - (q / W) * W;
+      if (lr < h || lr >= HH - h || lc < h || lc >= W - h) continue;
+      const int r = gr(lr), c = gc(lc);
+      dst[q] = inside(r, c) ? sb[q] - ax(x1, q, r, c, diag(q, r, c)) : (T)0;
+    }
+  }
+};
+
+// Two grid levels of the down leg in one launch (A -> B -> C, no PCG
+// update or stop test: the mid levels).  The block owns a 32 x 8 tile of
+// level B and so the 64 x 16 footprint of level A below it.  To give B its
+// own tile's two sweeps and residual, B's right side is needed on the tile
+// and a 3-cell ring, i.e. A's restricted residuals over that 38 x 14 region,
+// which need A's sweeps on the footprint and a 9-cell ring: those are
+// recomputed here from A's right side and faces.  Every value is the same
+// expression, with the same operands, as in k_mg_down, so the results are
+// bit-identical to two k_mg_down launches.
+constexpr int PBX = TX / 2, PBY = TY / 2;    // B tile
+constexpr int PHA = 9, PHB = HL;            // rings on A and B
+using RegA = Region<float, 2 * PBX, 2 * PBY, PHA>;  // shape only (sizes below are type-free)
+constexpr int PAN = RegA::N, PAW = RegA::W;
+constexpr int PBW = PBX + 2 * PHB, PBH = PBY + 2 * PHB, PBN = PBW * PBH;
+template <class T>
+__host__ __device__ constexpr size_t pair_smem() {
+  return (size_t)(5 * PAN + 5 * PBN) * sizeof(T);
+}
+template <class T>
+__global__ __launch_bounds__(FT) void k_mg_down2(Lv<T> A, Lv<T> B, Lv<T> C, const int* stop) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  T* sbA = reinterpret_cast<T*>(smem_raw);
+  T* seA = sbA + PAN;
+  T* snA = seA + PAN;
+  T* s0A = snA + PAN;
+  T* s1A = s0A + PAN;
+  T* sbB = s1A + PAN;
+  T* seB = sbB + PBN;
+  T* snB = seB + PBN;
+  T* s0B = snB + PBN;
+  T* s1B = s0B + PBN;
+  __shared__ int sflag;
+  const int t = threadIdx.x + threadIdx.y * TX;
+  const int stop0 = t == 0 ? *stop : 0;
+  const int rB0 = (int)blockIdx.y * PBY, cB0 = (int)blockIdx.x * PBX;
+  const Region<T, 2 * PBX, 2 * PBY, PHA> ra{sbA, seA, snA, 2 * rB0, 2 * cB0, A.nx, A.ny};
+  const Region<T, PBX, PBY, PHB> rb{sbB, seB, snB, rB0, cB0, B.nx, B.ny};
+  {  // every global load first: A's b and faces on its region, B's faces on its region
+    constexpr int NA = (PAN + FT - 1) / FT, NB = (PBN + FT - 1) / FT;
+    T vb[NA], ve[NA], vn[NA], we[NB], wn[NB];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int q = t + i * FT;
+      const int lr = q / PAW, lc = q - (q / PAW) * PAW;
+      const int r = ra.gr(lr), c = ra.gc(lc);
+      vb[i] = ve[i] = vn[i] = (T)0;
+      if (q < PAN && ra.inside(r, c)) {
+        const size_t k = A.at(r, c);
+        vb[i] = A.b[k];
+        ve[i] = A.ex[k];
+        vn[i] = A.ey[k];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int q = t + i * FT;
+      const int lr = q / PBW, lc = q - (q / PBW) * PBW;
+      const int r = rb.gr(lr), c = rb.gc(lc);
+      we[i] = wn[i] = (T)0;
+      if (q < PBN && rb.inside(r, c)) {
+        const size_t k = B.at(r, c);
+        we[i] = B.ex[k];
+        wn[i] = B.ey[k];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int q = t + i * FT;
+      if (q < PAN) { sbA[q] = vb[i]; seA[q] = ve[i]; snA[q] = vn[i]; }
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int q = t + i * FT;
+      if (q < PBN) { seB[q] = we[i]; snB[q] = wn[i]; }
+    }
+  }
+  if (t == 0) sflag = stop0;
+  __syncthreads();
+  if (sflag) return;  // block-uniform
+  // level A: x0 on the 8-ring, x1 on the 7-ring, residual on the 6-ring
+  ra.first(s0A, 1, t);
+  __syncthreads();
+  ra.sweep(s0A, s1A, 2, t);
+  __syncthreads();
+  ra.resid(s1A, s0A, 3, t);
+  {  // the footprint's x1 (one cell per thread)
+    const int lr = PHA + t / (2 * PBX), lc = PHA + t % (2 * PBX), q = lr * PAW + lc;
+    const int r = ra.gr(lr), c = ra.gc(lc);
+    if (ra.inside(r, c)) A.x[A.at(r, c)] = s1A[q];
+  }
+  __syncthreads();
+  // B's right side on its region = sums of A's residuals over each 2 x 2
+  // aggregate; the tile's part is also B's right side in memory
+  for (int q = t; q < PBN; q += FT) {
+    const int lr = q / PBW, lc = q - (q / PBW) * PBW;
+    const int r = rb.gr(lr), c = rb.gc(lc);
+    T v = (T)0;
+    if (rb.inside(r, c)) {
+      const int qa = (3 + 2 * lr) * PAW + 3 + 2 * lc;
+      v = (s0A[qa] + s0A[qa + 1]) + (s0A[qa + PAW] + s0A[qa + PAW + 1]);
+      if (lr >= PHB && lr < PHB + PBY && lc >= PHB && lc < PHB + PBX) B.b[B.at(r, c)] = v;
+    }
+    sbB[q] = v;
+  }
+  __syncthreads();
+  // level B: k_mg_down's recipe on the 32 x 8 tile
+  rb.first(s0B, 1, t);
+  __syncthreads();
+  rb.sweep(s0B, s1B, 2, t);
+  __syncthreads();
+  if (t < PBX * PBY) {
+    const int lr = PHB + t / PBX, lc = PHB + t % PBX, q = lr * PBW + lc;
+    const int r = rb.gr(lr), c = rb.gc(lc);
+    if (rb.inside(r, c)) {
+      s0B[q] = sbB[q] - rb.ax(s1B, q, r, c, rb.diag(q, r, c));
+      B.x[B.at(r, c)] = s1B[q];
+    } else {
+      s0B[q] = (T)0;
+    }
+  }
+  __syncthreads();
+  if (t < (PBY / 2) * (PBX / 2)) {
+    const int cr = t / (PBX / 2), cc = t % (PBX / 2);
+    const int R = rB0 / 2 + cr, Cc = cB0 / 2 + cc;
+    if (R < C.ny && Cc < C.nx) {
+      const int q = (PHB + 2 * cr) * PBW + PHB + 2 * cc;
+      C.b[C.at(R, Cc)] = (s0B[q] + s0B[q + 1]) + (s0B[q + PBW] + s0B[q + PBW + 1]);
+    }
+  }
+}
+
 // up: x2 = x1 + P e (e = the coarse level's result in Cl.y), two Jacobi
 // sweeps, result in L.y; with part != nullptr also the partial b . z of the
 // block (the PCG's r . z on the fine level)
@@ -1110,6 +1298,7 @@ struct MGImpl {
   int nGrid = 0;
   CoarseDesc cd{};
   size_t coarseSmem = 0, coarseSetupSmem = 0;
+  bool pairOk = false;  // the two-level down kernel's LDS fits
   DevBuf<T> cf;  // coarse-solver faces and inverse diagonals
   DevBuf<T> r0, r1, xs, pa, pb, Ap;  // PCG vectors, level-0 layout (xs = solution; r0 / r1 residual by parity)
   DevBuf<T> sa, sb, wz;              // single-reduction form: s = A p by parity, w = A z
@@ -1171,6 +1360,7 @@ struct MGImpl {
     int lds = 0;
     CPS_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
     CPS_CHECK(coarseSmem <= (size_t)lds, CPS_ERR_STATE, "coarse multigrid levels exceed LDS");
+    pairOk = pair_smem<T>() <= (size_t)lds;
     cf.alloc((size_t)3 * off + (size_t)cd.dn * cd.dn);
     for (auto* b : {&lv[0]->x, &lv[0]->y, &lv[0]->ex, &lv[0]->ey, &r0, &r1, &xs, &pa, &pb, &Ap, &sa, &sb, &wz})
       CPS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), ctx->stream));
@@ -1229,6 +1419,12 @@ struct MGImpl {
     const PcgTest none{nullptr, 0, 0, nullptr, 0.0, 0.0};
     const PcgTest test{part.p + (size_t)(it == 0 ? S_RR0 : S_ERR) * G0.NB, G0.NB, it, scal.p, abs_tol_, rtol_};
     for (int l = 0; l < nGrid; ++l) {
+      if (l > 0 && l + 1 < nGrid && pairOk) {  // two mid levels in one launch
+        k_mg_down2<T><<<dim3(ceil_div(lv[l + 1]->nx, PBX), ceil_div(lv[l + 1]->ny, PBY)), dim3(TX, TY),
+                        pair_smem<T>(), s>>>(view(l), view(l + 1), view(l + 2), stop);
+        ++l;
+        continue;
+      }
       XrFuse<T> xr{};
       if (l == 0 && fused && it > 0) xr = cg_update(it - 1);
       const PcgTest dt = (l == 0 && !fused) ? test : none;
