@@ -707,6 +707,113 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
   }
 }
 
+// Two grid levels of the up leg in one launch (C -> B -> A, mid levels:
+// no PCG partials).  The block owns a 64 x 16 tile of level A and so the
+// 32 x 8 tile of level B above it.  A's two sweeps read x2 = x1 + P e on a
+// 2-cell ring, i.e. B's result on its tile and a 1-cell ring; that needs
+// B's first sweep on a 2-cell ring and B's x2 = x1 + P e (from C's result
+// in memory) on a 3-cell ring.  B's result is used from LDS only (no later
+// kernel reads a mid level's y except the next finer level's up leg, which
+// is the A half here).  Every value is the same expression, with the same
+// operands, as in k_mg_up, so A's result is bit-identical to two k_mg_up
+// launches.
+using UpA = Region<float, TX, TY, HL>;
+using UpB = Region<float, TX / 2, TY / 2, HL>;
+constexpr int UAN = UpA::N, UAW = UpA::W, UBN = UpB::N, UBW = UpB::W;
+static_assert(UAW == RW && UAN == RN, "A region = the k_mg_up tile");
+template <class T>
+__host__ __device__ constexpr size_t up_pair_smem() {
+  return (size_t)(5 * UAN + 5 * UBN) * sizeof(T);
+}
+template <class T>
+__global__ __launch_bounds__(FT) void k_mg_up2(Lv<T> A, Lv<T> B, Lv<T> C, const int* stop) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  T* sbA = reinterpret_cast<T*>(smem_raw);
+  T* seA = sbA + UAN;
+  T* snA = seA + UAN;
+  T* s0A = snA + UAN;
+  T* s1A = s0A + UAN;
+  T* sbB = s1A + UAN;
+  T* seB = sbB + UBN;
+  T* snB = seB + UBN;
+  T* s0B = snB + UBN;
+  T* s1B = s0B + UBN;
+  __shared__ int sflag;
+  const int t = threadIdx.x + threadIdx.y * TX;
+  const int stop0 = t == 0 ? *stop : 0;
+  const int rA0 = (int)blockIdx.y * TY, cA0 = (int)blockIdx.x * TX;
+  const int rB0 = rA0 / 2, cB0 = cA0 / 2;
+  const Region<T, TX, TY, HL> ra{sbA, seA, snA, rA0, cA0, A.nx, A.ny};
+  const Region<T, TX / 2, TY / 2, HL> rb{sbB, seB, snB, rB0, cB0, B.nx, B.ny};
+  {  // every global load first: A's b, faces and x1; B's b, faces and x2
+    constexpr int NA = (UAN + FT - 1) / FT, NB = (UBN + FT - 1) / FT;
+    T vb[NA], ve[NA], vn[NA], vx[NA], wb[NB], we[NB], wn[NB], wx[NB];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int q = t + i * FT;
+      const int lr = q / UAW, lc = q - (q / UAW) * UAW;
+      const int r = ra.gr(lr), c = ra.gc(lc);
+      vb[i] = ve[i] = vn[i] = vx[i] = (T)0;
+      if (q < UAN && ra.inside(r, c)) {
+        const size_t k = A.at(r, c);
+        vb[i] = A.b[k];
+        ve[i] = A.ex[k];
+        vn[i] = A.ey[k];
+        vx[i] = A.x[k];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int q = t + i * FT;
+      const int lr = q / UBW, lc = q - (q / UBW) * UBW;
+      const int r = rb.gr(lr), c = rb.gc(lc);
+      wb[i] = we[i] = wn[i] = wx[i] = (T)0;
+      if (q < UBN && rb.inside(r, c)) {
+        const size_t k = B.at(r, c);
+        wb[i] = B.b[k];
+        we[i] = B.ex[k];
+        wn[i] = B.ey[k];
+        wx[i] = B.x[k] + C.y[C.at(r >> 1, c >> 1)];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int q = t + i * FT;
+      if (q < UAN) { sbA[q] = vb[i]; seA[q] = ve[i]; snA[q] = vn[i]; s0A[q] = vx[i]; }
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int q = t + i * FT;
+      if (q < UBN) { sbB[q] = wb[i]; seB[q] = we[i]; snB[q] = wn[i]; s0B[q] = wx[i]; }
+    }
+  }
+  if (t == 0) sflag = stop0;
+  __syncthreads();
+  if (sflag) return;  // block-uniform
+  rb.sweep(s0B, s1B, 1, t);  // 2-cell ring
+  __syncthreads();
+  rb.sweep(s1B, s0B, 2, t);  // B's result on the 1-cell ring
+  __syncthreads();
+  // A's x2 = x1 + P e on the 2-cell ring (the 3-cell ring is never read)
+  for (int q = t; q < UAN; q += FT) {
+    const int lr = q / UAW, lc = q - (q / UAW) * UAW;
+    if (lr < 1 || lr >= UpA::HH - 1 || lc < 1 || lc >= UAW - 1) continue;
+    const int r = ra.gr(lr), c = ra.gc(lc);
+    if (ra.inside(r, c)) s0A[q] = s0A[q] + s0B[((r >> 1) - rB0 + HL) * UBW + (c >> 1) - cB0 + HL];
+  }
+  __syncthreads();
+  ra.sweep(s0A, s1A, 2, t);  // 1-cell ring
+  __syncthreads();
+  {
+    const int lr = HL + threadIdx.y, lc = HL + threadIdx.x, q = lr * UAW + lc;
+    const int r = ra.gr(lr), c = ra.gc(lc);
+    if (ra.inside(r, c)) {
+      const T d = ra.diag(q, r, c);
+      A.y[A.at(r, c)] = s1A[q] + (T)kOmega * (sbA[q] - ra.ax(s1A, q, r, c, d)) / d;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // All levels from `top` down inside one workgroup: the top level's rhs and
 // faces come from global memory, every smaller level is built and solved in
@@ -1360,7 +1467,7 @@ struct MGImpl {
     int lds = 0;
     CPS_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
     CPS_CHECK(coarseSmem <= (size_t)lds, CPS_ERR_STATE, "coarse multigrid levels exceed LDS");
-    pairOk = pair_smem<T>() <= (size_t)lds;
+    pairOk = pair_smem<T>() <= (size_t)lds && up_pair_smem<T>() <= (size_t)lds;
     cf.alloc((size_t)3 * off + (size_t)cd.dn * cd.dn);
     for (auto* b : {&lv[0]->x, &lv[0]->y, &lv[0]->ex, &lv[0]->ey, &r0, &r1, &xs, &pa, &pb, &Ap, &sa, &sb, &wz})
       CPS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), ctx->stream));
@@ -1435,11 +1542,17 @@ struct MGImpl {
         k_mg_down<T><<<tgrid(l), dim3(TX, TY), 0, s>>>(view(l), view(l + 1), stop, dt, xr);
     }
     k_mg_coarse<T><<<1, kCoarseThreads, coarseSmem, s>>>(view(nGrid), cd, cf.p, stop, fused ? test : none);
-    for (int l = nGrid - 1; l >= 0; --l)
+    for (int l = nGrid - 1; l >= 0; --l) {
+      if (l >= 2 && pairOk) {  // two mid levels in one launch
+        k_mg_up2<T><<<tgrid(l - 1), dim3(TX, TY), up_pair_smem<T>(), s>>>(view(l - 1), view(l), view(l + 1), stop);
+        --l;
+        continue;
+      }
       k_mg_up<T><<<tgrid(l), dim3(TX, TY), 0, s>>>(view(l), view(l + 1),
                                                   l == 0 ? part.p + (size_t)S_RZ * G0.NB : nullptr, stop,
                                                   l == 0 && fused ? wz.p : nullptr,
                                                   part.p + (size_t)S_ZW * G0.NB);
+    }
     CPS_HIP(hipGetLastError());
   }
 
