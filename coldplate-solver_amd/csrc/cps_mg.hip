@@ -824,11 +824,14 @@ struct CoarseDesc {
   int total;
   int wl;  // the last level (at most kDirectCells cells): solved with its inverse
   int dn;  // its cell count; the inverse sits at cf[3 total ..), column-major
+  unsigned long long mag[16];  // ceil(2^32 / nx): row = (q mag) >> 32 for a cell index q < 2^20
 };
 
-#define CELLS(l)                                              \
-  for (int r = ty; r < d.ny[l]; r += kCoarseThreads / 64)     \
-    for (int c = tx; c < d.nx[l]; c += 64)
+// the cells of level l, one per thread in row-major order (every lane of a
+// wave busy on narrow levels); r = q / nx by the reciprocal multiply
+#define CELLS(l)                                                                 \
+  for (int q_ = t, n_ = d.nx[l] * d.ny[l]; q_ < n_; q_ += kCoarseThreads)       \
+    if (const int r = (int)(((unsigned long long)q_ * d.mag[l]) >> 32), c = q_ - r * d.nx[l]; true)
 
 // Once per pressure solve: faces and inverse diagonals of every level of the
 // coarse solver, built from the top level's faces (half the sum of the
@@ -838,7 +841,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, C
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   T* se = reinterpret_cast<T*>(smem_raw);
   T* sn = se + d.total;
-  const int t = threadIdx.x, tx = t & 63, ty = t >> 6;
+  const int t = threadIdx.x;
   CELLS(0) {
     const size_t k = top.at(r, c);
     se[r * d.nx[0] + c] = top.ex[k];
@@ -968,7 +971,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
   T* sn = se + d.total;
   T* si = sn + d.total;  // 1 / diagonal
   T* sA = si + d.total;  // inverse of the last level's operator, column-major
-  const int t = threadIdx.x, tx = t & 63, ty = t >> 6;
+  const int t = threadIdx.x;
   {  // every global load of the thread is issued before the LDS stores
     constexpr int NQ = kCoarseTotalMax / kCoarseThreads;  // d.total <= kCoarseTotalMax
     constexpr int NB0 = kCoarseTopMax / kCoarseThreads;   // top level <= kCoarseTopMax cells
@@ -1002,14 +1005,14 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
   }
   __syncthreads();
   const T om = (T)kOmega;
-  // A x at cell k of level l, x given by the functor xv(k)
+  // A x at cell k = (r, c) of level l, x given by the functor xv(k, r, c)
   auto ax = [&](auto xv, int l, int k, int r, int c) -> T {
     const int nx = d.nx[l], ny = d.ny[l];
-    T a = xv(k) / si[k];
-    if (c + 1 < nx) a -= se[k] * xv(k + 1);
-    if (c > 0) a -= se[k - 1] * xv(k - 1);
-    if (r + 1 < ny) a -= sn[k] * xv(k + nx);
-    if (r > 0) a -= sn[k - nx] * xv(k - nx);
+    T a = xv(k, r, c) / si[k];
+    if (c + 1 < nx) a -= se[k] * xv(k + 1, r, c + 1);
+    if (c > 0) a -= se[k - 1] * xv(k - 1, r, c - 1);
+    if (r + 1 < ny) a -= sn[k] * xv(k + nx, r + 1, c);
+    if (r > 0) a -= sn[k - nx] * xv(k - nx, r - 1, c);
     return a;
   };
   const int wl = d.wl;
@@ -1017,13 +1020,13 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
   // residual into the next level's right side; cell (r, c) of level l
   auto down_sweep = [&](int l, int r, int c) {
     const int o = d.off[l], nx = d.nx[l];
-    auto x0 = [&](int k) -> T { return om * sb[k] * si[k]; };
+    auto x0 = [&](int k, int, int) -> T { return om * sb[k] * si[k]; };
     const int k = o + r * nx + c;
-    sy[k] = x0(k) + om * (sb[k] - ax(x0, l, k, r, c)) * si[k];
+    sy[k] = x0(k, r, c) + om * (sb[k] - ax(x0, l, k, r, c)) * si[k];
   };
   auto restrict_cell = [&](int l, int r, int c) {
     const int o = d.off[l], nx = d.nx[l], ny = d.ny[l], co = d.off[l + 1], cx = d.nx[l + 1];
-    auto x1 = [&](int k) -> T { return sy[k]; };
+    auto x1 = [&](int k, int, int) -> T { return sy[k]; };
     T acc = (T)0;
     for (int dr = 0; dr < 2; ++dr)
       for (int dc = 0; dc < 2; ++dc) {
@@ -1038,18 +1041,15 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
   // first up sweep: x2 = x1 + P e formed on the fly, into sx
   auto up_sweep1 = [&](int l, int r, int c) {
     const int o = d.off[l], nx = d.nx[l], co = d.off[l + 1], cx = d.nx[l + 1];
-    auto x2 = [&](int k) -> T {
-      const int q = k - o, rr = q / nx, cc = q - rr * nx;
-      return sy[k] + sy[co + (rr >> 1) * cx + (cc >> 1)];
-    };
+    auto x2 = [&](int k, int rr, int cc) -> T { return sy[k] + sy[co + (rr >> 1) * cx + (cc >> 1)]; };
     const int k = o + r * nx + c;
-    sx[k] = x2(k) + om * (sb[k] - ax(x2, l, k, r, c)) * si[k];
+    sx[k] = x2(k, r, c) + om * (sb[k] - ax(x2, l, k, r, c)) * si[k];
   };
   auto up_sweep2 = [&](int l, int r, int c) {
     const int o = d.off[l], nx = d.nx[l];
-    auto x3 = [&](int k) -> T { return sx[k]; };
+    auto x3 = [&](int k, int, int) -> T { return sx[k]; };
     const int k = o + r * nx + c;
-    sy[k] = x3(k) + om * (sb[k] - ax(x3, l, k, r, c)) * si[k];
+    sy[k] = x3(k, r, c) + om * (sb[k] - ax(x3, l, k, r, c)) * si[k];
   };
   for (int l = 0; l < wl; ++l) {
     CELLS(l) { down_sweep(l, r, c); }
@@ -1449,6 +1449,7 @@ struct MGImpl {
       cd.nx[n] = nx;
       cd.ny[n] = ny;
       cd.off[n] = off;
+      cd.mag[n] = ((1ull << 32) + (unsigned long long)nx - 1) / (unsigned long long)nx;
       off += nx * ny;
       ++n;
       if (nx * ny <= kDirectCells) break;
