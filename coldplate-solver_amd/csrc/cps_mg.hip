@@ -1005,14 +1005,22 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
   }
   __syncthreads();
   const T om = (T)kOmega;
-  // A x at cell k = (r, c) of level l, x given by the functor xv(k, r, c)
+  // A x at cell k = (r, c) of level l, x given by the functor xv(k, r, c).
+  // Branch-free: a missing neighbour is read at the cell itself and enters
+  // with a zero face, so every LDS read of the cell issues at once.
   auto ax = [&](auto xv, int l, int k, int r, int c) -> T {
     const int nx = d.nx[l], ny = d.ny[l];
+    const bool hE = c + 1 < nx, hW = c > 0, hN = r + 1 < ny, hS = r > 0;
+    const int kW = hW ? k - 1 : k, kS = hS ? k - nx : k;
+    const T fE = hE ? se[k] : (T)0, fW = hW ? se[kW] : (T)0;
+    const T fN = hN ? sn[k] : (T)0, fS = hS ? sn[kS] : (T)0;
+    const T vE = xv(hE ? k + 1 : k, r, hE ? c + 1 : c), vW = xv(kW, r, hW ? c - 1 : c);
+    const T vN = xv(hN ? k + nx : k, hN ? r + 1 : r, c), vS = xv(kS, hS ? r - 1 : r, c);
     T a = xv(k, r, c) / si[k];
-    if (c + 1 < nx) a -= se[k] * xv(k + 1, r, c + 1);
-    if (c > 0) a -= se[k - 1] * xv(k - 1, r, c - 1);
-    if (r + 1 < ny) a -= sn[k] * xv(k + nx, r + 1, c);
-    if (r > 0) a -= sn[k - nx] * xv(k - nx, r - 1, c);
+    a -= fE * vE;
+    a -= fW * vW;
+    a -= fN * vN;
+    a -= fS * vS;
     return a;
   };
   const int wl = d.wl;
@@ -1027,15 +1035,19 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
   auto restrict_cell = [&](int l, int r, int c) {
     const int o = d.off[l], nx = d.nx[l], ny = d.ny[l], co = d.off[l + 1], cx = d.nx[l + 1];
     auto x1 = [&](int k, int, int) -> T { return sy[k]; };
+    T v[4];
+    bool in[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {  // a fine cell past the level is evaluated at its clamp and left out
+      const int rr = 2 * r + (i >> 1), cc = 2 * c + (i & 1);
+      in[i] = rr < ny && cc < nx;
+      const int rc = min(rr, ny - 1), cl = min(cc, nx - 1), k = o + rc * nx + cl;
+      v[i] = sb[k] - ax(x1, l, k, rc, cl);
+    }
     T acc = (T)0;
-    for (int dr = 0; dr < 2; ++dr)
-      for (int dc = 0; dc < 2; ++dc) {
-        const int rr = 2 * r + dr, cc = 2 * c + dc;
-        if (rr < ny && cc < nx) {
-          const int k = o + rr * nx + cc;
-          acc += sb[k] - ax(x1, l, k, rr, cc);
-        }
-      }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (in[i]) acc += v[i];
     sb[co + r * cx + c] = acc;
   };
   // first up sweep: x2 = x1 + P e formed on the fly, into sx
