@@ -823,7 +823,7 @@ struct CoarseDesc {
   int nx[16], ny[16], off[16];  // compact LDS layout, pitch = nx
   int total;
   int wl;  // the last level (at most kDirectCells cells): solved with its inverse
-  int dn;  // its cell count; the inverse sits at cf[3 total ..), column-major
+  int dn;  // its cell count; the inverse sits at cf[4 total ..), column-major
   unsigned long long mag[16];  // ceil(2^32 / nx): row = (q mag) >> 32 for a cell index q < 2^20
 };
 
@@ -835,7 +835,7 @@ struct CoarseDesc {
 
 // Once per pressure solve: faces and inverse diagonals of every level of the
 // coarse solver, built from the top level's faces (half the sum of the
-// covered fine faces), stored as cf[0 .. 3 total) = east, north, 1/diag.
+// covered fine faces), stored as cf[0 .. 4 total) = east, north, 1/diag, diag.
 template <class T>
 __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, CoarseDesc d, T* cf) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
@@ -873,6 +873,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, C
       cf[k] = se[k];
       cf[d.total + k] = sn[k];
       cf[2 * d.total + k] = (T)1 / dd;
+      cf[3 * d.total + k] = dd;
     }
   }
   // Inverse of the last level's operator (the faces and floored diagonal the
@@ -930,7 +931,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, C
     if (own)
 #pragma unroll
       for (int i = 0; i < kDirectCells; ++i)
-        if (i < n) cf[3 * d.total + t * n + i] = (T)col[i];
+        if (i < n) cf[4 * d.total + t * n + i] = (T)col[i];
   }
 }
 
@@ -970,29 +971,32 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
   T* se = sb + d.total;
   T* sn = se + d.total;
   T* si = sn + d.total;  // 1 / diagonal
-  T* sA = si + d.total;  // inverse of the last level's operator, column-major
+  T* sd = si + d.total;  // diagonal
+  T* sA = sd + d.total;  // inverse of the last level's operator, column-major
   const int t = threadIdx.x;
   {  // every global load of the thread is issued before the LDS stores
     constexpr int NQ = kCoarseTotalMax / kCoarseThreads;  // d.total <= kCoarseTotalMax
     constexpr int NB0 = kCoarseTopMax / kCoarseThreads;   // top level <= kCoarseTopMax cells
-    T ve[NQ], vn[NQ], vi[NQ], vb[NB0];
+    T ve[NQ], vn[NQ], vi[NQ], vd[NQ], vb[NB0];
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
       const int q = t + i * kCoarseThreads;
       ve[i] = q < d.total ? cf[q] : (T)0;
       vn[i] = q < d.total ? cf[d.total + q] : (T)0;
       vi[i] = q < d.total ? cf[2 * d.total + q] : (T)0;
+      vd[i] = q < d.total ? cf[3 * d.total + q] : (T)0;
     }
     const int n0 = d.nx[0] * d.ny[0];
 #pragma unroll
     for (int i = 0; i < NB0; ++i) {
       const int q = t + i * kCoarseThreads;
-      vb[i] = q < n0 ? top.b[top.at(q / d.nx[0], q % d.nx[0])] : (T)0;
+      const int r = (int)(((unsigned long long)q * d.mag[0]) >> 32);
+      vb[i] = q < n0 ? top.b[top.at(r, q - r * d.nx[0])] : (T)0;
     }
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
       const int q = t + i * kCoarseThreads;
-      if (q < d.total) { se[q] = ve[i]; sn[q] = vn[i]; si[q] = vi[i]; }
+      if (q < d.total) { se[q] = ve[i]; sn[q] = vn[i]; si[q] = vi[i]; sd[q] = vd[i]; }
     }
 #pragma unroll
     for (int i = 0; i < NB0; ++i) {
@@ -1000,7 +1004,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
       if (q < n0) sb[q] = vb[i];
     }
     const int na = d.dn * d.dn;  // <= kDirectCells^2 = kCoarseThreads
-    const T va = t < na ? cf[3 * d.total + t] : (T)0;
+    const T va = t < na ? cf[4 * d.total + t] : (T)0;
     if (t < na) sA[t] = va;
   }
   __syncthreads();
@@ -1016,7 +1020,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
     const T fN = hN ? sn[k] : (T)0, fS = hS ? sn[kS] : (T)0;
     const T vE = xv(hE ? k + 1 : k, r, hE ? c + 1 : c), vW = xv(kW, r, hW ? c - 1 : c);
     const T vN = xv(hN ? k + nx : k, hN ? r + 1 : r, c), vS = xv(kS, hS ? r - 1 : r, c);
-    T a = xv(k, r, c) / si[k];
+    T a = sd[k] * xv(k, r, c);
     a -= fE * vE;
     a -= fW * vW;
     a -= fN * vN;
@@ -1475,13 +1479,13 @@ struct MGImpl {
     cd.dn = nx * ny;
     CPS_CHECK(off <= kCoarseTotalMax && lv.back()->nx * lv.back()->ny <= kCoarseTopMax, CPS_ERR_STATE,
               "coarse multigrid plan exceeds the coarse solver's staging");
-    coarseSmem = ((size_t)6 * off + (size_t)cd.dn * cd.dn) * sizeof(T);
+    coarseSmem = ((size_t)7 * off + (size_t)cd.dn * cd.dn) * sizeof(T);
     coarseSetupSmem = (size_t)2 * off * sizeof(T);
     int lds = 0;
     CPS_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
     CPS_CHECK(coarseSmem <= (size_t)lds, CPS_ERR_STATE, "coarse multigrid levels exceed LDS");
     pairOk = pair_smem<T>() <= (size_t)lds && up_pair_smem<T>() <= (size_t)lds;
-    cf.alloc((size_t)3 * off + (size_t)cd.dn * cd.dn);
+    cf.alloc((size_t)4 * off + (size_t)cd.dn * cd.dn);
     for (auto* b : {&lv[0]->x, &lv[0]->y, &lv[0]->ex, &lv[0]->ey, &r0, &r1, &xs, &pa, &pb, &Ap, &sa, &sb, &wz})
       CPS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), ctx->stream));
     G0.gx = ceil_div(lv[0]->nx, TX);
