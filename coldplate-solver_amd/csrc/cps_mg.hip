@@ -915,17 +915,19 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, C
       const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
       return __hiloint2double(hi, lo);
     };
-    for (int p = 0; p < n; ++p) {
-      double cp = 0.0;  // this column's row-p entry
+    // the pivot loop is unrolled so that every register index is static
 #pragma unroll
-      for (int i = 0; i < kDirectCells; ++i) cp = i == p ? col[i] : cp;
-      const double inv = 1.0 / lane(cp, p);
-      const double mp = (t == p ? 1.0 : cp) * inv;
+    for (int p = 0; p < kDirectCells; ++p) {
+      if (p < n) {  // wave-uniform
+        const double cp = col[p];  // this column's row-p entry
+        const double inv = 1.0 / lane(cp, p);
+        const double mp = (t == p ? 1.0 : cp) * inv;
 #pragma unroll
-      for (int i = 0; i < kDirectCells; ++i) {
-        const double f = lane(col[i], p);  // A[i][p], before lane p updates it
-        const double v = (t == p ? 0.0 : col[i]) - f * mp;
-        col[i] = i == p ? mp : v;
+        for (int i = 0; i < kDirectCells; ++i) {
+          const double f = lane(col[i], p);  // A[i][p], before lane p updates it
+          const double v = (t == p ? 0.0 : col[i]) - f * mp;
+          col[i] = i == p ? mp : v;
+        }
       }
     }
     if (own)
