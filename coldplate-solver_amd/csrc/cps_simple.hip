@@ -379,14 +379,22 @@ __global__ __launch_bounds__(NT) void k_jac_end(Geo g, int K, double tol2, IterS
 // ---------------------------------------------------------------------------
 // uStar / vStar = clamp(solution) (momentum_solver.cpp:335-351), their BCs
 // (iterations.cpp:169) and the change sums of iterations.cpp:499-533.
+// K >= 0: the sweep counts are settled here (K sweeps were launched and no
+// k_jac_end ran): a component still active after sweep K-1 has done K
+// sweeps whatever that sweep's stop test gives.
 template <class C, bool Delta>
 __global__ __launch_bounds__(NT) void k_finalize(Geo g, Fields f, MomSys<C> su, MomSys<C> sv, IterState* st,
-                                                 double* part) {
+                                                 double* part, int K) {
   __shared__ double sh[NT / kWave + 1];
   const int M = g.M, N = g.N;
   const double mv = g.maxVel;
   double su2 = 0.0, sv2 = 0.0, umx = 0.0, vmx = 0.0;
-  const int itU = st->jac_iters[0], itV = st->jac_iters[1];
+  int itU = st->jac_iters[0], itV = st->jac_iters[1];
+  if (K >= 0) {
+    if (st->jac_active[0]) itU = K;
+    if (st->jac_active[1]) itV = K;
+    if (block_id() == 0 && thread_id() == 0) { st->jac_iters[0] = itU; st->jac_iters[1] = itV; }
+  }
   FIELD_LOOP(g, i, j) {
     if (j > N) continue;
     const size_t k = at(g, i, j);
@@ -1064,6 +1072,7 @@ class SimpleSolver {
       // x, cS, cW, aP, cE, cN, b read + x' written, both components
       if (timer.which == KT_MOM_JACOBI)
         timer.bytes_per_launch = 8.0 * (mode_fp64 ? 8 : 4) * (double)(g.nU + g.nV);
+      if (k >= maxit && chunk == maxit) break;  // the counts are settled by k_finalize
       k_jac_end<<<1, block, 0, s>>>(g, k, tol2, st.p, part.p);
       CPS_HIP(hipGetLastError());
       if (k >= maxit) break;
@@ -1072,11 +1081,11 @@ class SimpleSolver {
       CPS_HIP(hipStreamSynchronize(s));
       if (!act[0] && !act[1]) break;
     }
-    if (maxit == 0) k_jac_end<<<1, block, 0, s>>>(g, 0, tol2, st.p, part.p);
+    const int K = maxit == 0 ? 0 : chunk == maxit ? maxit : -1;
     if (mode_fp64)
-      k_finalize<double, false><<<grid, block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p, part.p);
+      k_finalize<double, false><<<grid, block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p, part.p, K);
     else
-      k_finalize<float, true><<<grid, block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p, part.p);
+      k_finalize<float, true><<<grid, block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p, part.p, K);
     CPS_HIP(hipGetLastError());
   }
 
