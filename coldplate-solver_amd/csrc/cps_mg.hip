@@ -334,6 +334,7 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
   const int t = threadIdx.x + threadIdx.y * TX;
   const int stop0 = t == 0 ? *stop : 0;  // thread 0's view, published below
   Tile<T> tl{sb, se, sn, (int)blockIdx.y * TY, (int)blockIdx.x * TX, L.nx, L.ny};
+  bool x0_ready = false;  // the fused branch forms the first sweep with b
   if (xr.rold) {
     const double prz = parts_load<FT>(xr.rz, xr.nb);
     const double pzw = parts_load<FT>(xr.zw, xr.nb);
@@ -366,6 +367,14 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
       wo = xr.w[k0];
       so = xr.sold[k0];
     }
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {  // the faces first: the reduction's barrier publishes them
+      const int q = t + i * FT;
+      if (q < RN) {
+        se[q] = ve[i];
+        sn[q] = vn[i];
+      }
+    }
     if (t == 0) sflag = stop0;
     double gam = prz, del = pzw;
     block_sum2_all<FT>(gam, del, sh2);  // its barrier also publishes sflag
@@ -381,14 +390,19 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
     }
     const T alpha = (T)cs.alpha, beta = (T)cs.beta;
 #pragma unroll
-    for (int i = 0; i < NI; ++i) {
+    for (int i = 0; i < NI; ++i) {  // b, and the first sweep from zero on the halo-2 ring
       const int q = t + i * FT;
       if (q < RN) {
-        sb[q] = vr[i] - alpha * (vw[i] + beta * vs[i]);
-        se[q] = ve[i];
-        sn[q] = vn[i];
+        const T b = vr[i] - alpha * (vw[i] + beta * vs[i]);
+        sb[q] = b;
+        const int lr = q / RW, lc = q - (q / RW) * RW;
+        if (lr >= 1 && lr < RH - 1 && lc >= 1 && lc < RW - 1) {
+          const int r = tl.gr(lr), c = tl.gc(lc);
+          s0[q] = tl.inside(r, c) ? (T)kOmega * b / tl.diag(q, r, c) : (T)0;
+        }
       }
     }
+    x0_ready = true;
     __syncthreads();
     double e = 0.0;
     if (own) {
@@ -426,15 +440,17 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
     }
   }
   }
-  __syncthreads();
-  // first sweep from a zero guess on the halo-2 ring: x0 = omega b / diag
-  for (int q = t; q < RN; q += FT) {
-    const int lr = q / RW, lc = q - (q / RW) * RW;
-    if (lr < 1 || lr >= RH - 1 || lc < 1 || lc >= RW - 1) continue;
-    const int r = tl.gr(lr), c = tl.gc(lc);
-    s0[q] = tl.inside(r, c) ? (T)kOmega * sb[q] / tl.diag(q, r, c) : (T)0;
+  if (!x0_ready) {  // block-uniform
+    __syncthreads();
+    // first sweep from a zero guess on the halo-2 ring: x0 = omega b / diag
+    for (int q = t; q < RN; q += FT) {
+      const int lr = q / RW, lc = q - (q / RW) * RW;
+      if (lr < 1 || lr >= RH - 1 || lc < 1 || lc >= RW - 1) continue;
+      const int r = tl.gr(lr), c = tl.gc(lc);
+      s0[q] = tl.inside(r, c) ? (T)kOmega * sb[q] / tl.diag(q, r, c) : (T)0;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   tl.sweep(s0, s1, 2, t);  // x1 on the halo-1 ring
   __syncthreads();
   {  // residual of the tile cell, x1 to memory
