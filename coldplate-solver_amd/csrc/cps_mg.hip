@@ -1451,7 +1451,7 @@ struct MGImpl {
   KernelTimer* timer = nullptr;
 
   MGImpl(Ctx* c, const Geo& geo) : ctx(c), g(geo) {
-    coarseCells = sizeof(T) == 8 ? 1024 : 2048;
+    coarseCells = 512;
     // level 0: the field layout, cell (r,c) = (i-1, j-1)
     auto L0 = std::make_unique<Level>();
     L0->nx = g.N - 2;
