@@ -11,10 +11,11 @@
 // the rediscretised operator for smooth coefficients; it keeps the Brinkman
 // 1e8 coefficient contrast because faces are summed, not averaged).
 // Smoother: damped Jacobi (omega 0.8), 2 pre + 2 post sweeps, so the V-cycle
-// is symmetric and PCG-safe.  Levels down to 1024 (FP64) / 2048 (FP32)
-// cells run as grid kernels; the smaller ones run inside one workgroup in LDS
-// (k_mg_coarse) down to the first level of at most 32 cells, which is solved
-// exactly with its inverse (Gauss-Jordan in FP64 once per solve).
+// is symmetric and PCG-safe.  Levels of more than 512 cells run as grid
+// kernels (below the finest, two levels per launch where a pair remains);
+// the smaller ones run inside one workgroup in LDS (k_mg_coarse) down to the
+// first level of at most 32 cells, which is solved exactly with its inverse
+// (Gauss-Jordan in FP64 once per solve).
 //
 // Mixed precision: r64 = b - A x64 (FP64), FP32 MG-PCG on A e = r64, x64 += e
 // (FP64), repeated until ||r64|| meets the tolerance.
