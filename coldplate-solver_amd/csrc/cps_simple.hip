@@ -34,6 +34,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <type_traits>
 #include <vector>
 
 #include <chrono>
@@ -62,6 +63,7 @@ template <class C>
 struct MomSys {  // coefficients of one component, indexed like the field
   C *cS, *cW, *aP, *cE, *cN, *b;
   C *x0, *x1;  // Jacobi double buffer (x-form: velocity, delta-form: correction)
+  C* x2;       // third buffer of the paired sweeps (delta form), x_j in buffer j mod 3
 };
 
 struct PcgBufs {
@@ -376,30 +378,239 @@ __global__ __launch_bounds__(NT) void k_jac_end(Geo g, int K, double tol2, IterS
   }
 }
 
+// Two fused U+V Jacobi sweeps k and k+1 (jacobi_solver.cpp:44-67) in one
+// launch: the delta form without host polling.  Blocks own 64 x 16 tiles
+// from cell (1, 1), the first active one, so 1024 x 512 active cells are
+// exactly 512 blocks, one resident wave of two 1024-thread blocks per CU.
+// Sweep k runs on the tile and a one-cell ring, from x_k on a two-cell ring
+// in LDS, and its result stays in LDS for sweep k+1 on the tile.  Every cell
+// value is the same expression as in k_jacobi; the r^2 sums are the same
+// terms, summed per block of this grid.  x_j lives in buffer j mod 3: the launch reads x_k and
+// writes x_(k+1) and x_(k+2) to the two other buffers, so neighbouring
+// blocks' halos, and a result the stop rule may still pick, stay intact.
+// Stop rule first: the test of sweep k-2, then of sweep k-1 (the previous
+// launch's partials; slots by sweep index mod 4, so this launch's own
+// partials never overwrite ones its blocks are still reading).  two = 0
+// runs sweep k only (an odd sweep cap).
+constexpr int JR1W = BX + 2, JR1H = BY + 2, JR1N = JR1W * JR1H;  // sweep-k region
+constexpr int JR2W = BX + 4, JR2H = BY + 4, JR2N = JR2W * JR2H;  // x_k region
+constexpr int JRING = JR1N - BX * BY;                             // ring cells of sweep k
+static_assert(JRING == 2 * JR1W + 2 * BY && JRING <= NT, "one ring cell per thread");
+
+template <class C>
+__device__ __forceinline__ C* jac_buf(const MomSys<C>& s, int j) {
+  const int m = j % 3;
+  return m == 0 ? s.x0 : m == 1 ? s.x1 : s.x2;
+}
+
+// r = b - A x (column order S, W, P, E, N) and x + r / aP at LDS cell q of a
+// region of width W, as in k_jacobi
+template <class C>
+__device__ __forceinline__ C jac_cell(const C* X, int W, int q, const C* c, C& r) {
+  C Ax = (C)0;
+  Ax += c[0] * X[q - W];
+  Ax += c[1] * X[q - 1];
+  Ax += c[2] * X[q];
+  Ax += c[3] * X[q + 1];
+  Ax += c[4] * X[q + W];
+  r = c[5] - Ax;
+  return X[q] + r / c[2];
+}
+
+template <class C>
+__global__ __launch_bounds__(NT) void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> sv, int k, int two, double tol2,
+                                                IterState* st, double* part) {
+  __shared__ C xa[2][JR2N];  // x_k, U and V
+  __shared__ C xb[2][JR1N];  // x_(k+1)
+  __shared__ double sh[NT / kWave + 1];
+  __shared__ int sfl[2];
+  const int t = thread_id();
+  const int f0 = t == 0 ? st->jac_active[0] : 0, f1 = t == 0 ? st->jac_active[1] : 0;
+  const bool tst = k >= 2;
+  const double* p2 = part + (size_t)(SLOT_JAC4 + 2 * ((k + 2) & 3)) * g.NB;  // sweep k-2
+  const double* p1 = part + (size_t)(SLOT_JAC4 + 2 * ((k + 3) & 3)) * g.NB;  // sweep k-1
+  const int nb = (int)(gridDim.x * gridDim.y);  // partials per slot (slot stride g.NB >= nb)
+  const double qu2 = tst ? parts_load<NT>(p2, nb) : 0.0, qv2 = tst ? parts_load<NT>(p2 + g.NB, nb) : 0.0;
+  const double qu1 = tst ? parts_load<NT>(p1, nb) : 0.0, qv1 = tst ? parts_load<NT>(p1 + g.NB, nb) : 0.0;
+  if (t == 0) { sfl[0] = f0; sfl[1] = f1; }
+  __syncthreads();
+  int au = sfl[0], av = sfl[1];
+  if (tst) {  // jacobi_solver.cpp:63 on sweep k-2, and on sweep k-1 if that sweep counts
+    const bool b0 = block_id() == 0 && t == 0;
+    if (au) {
+      if (parts_total<NT>(qu2, sh) < tol2 * (double)g.nU) {
+        au = 0;
+        if (b0) { st->jac_active[0] = 0; st->jac_iters[0] = k - 1; }
+      } else if (parts_total<NT>(qu1, sh) < tol2 * (double)g.nU) {
+        au = 0;
+        if (b0) { st->jac_active[0] = 0; st->jac_iters[0] = k; }
+      }
+    }
+    if (av) {
+      if (parts_total<NT>(qv2, sh) < tol2 * (double)g.nV) {
+        av = 0;
+        if (b0) { st->jac_active[1] = 0; st->jac_iters[1] = k - 1; }
+      } else if (parts_total<NT>(qv1, sh) < tol2 * (double)g.nV) {
+        av = 0;
+        if (b0) { st->jac_active[1] = 0; st->jac_iters[1] = k; }
+      }
+    }
+  }
+  if (!au && !av) return;  // uniform over the grid
+  const int M = g.M, N = g.N, P = g.P;
+  const int i0 = 1 + (int)blockIdx.y * BY, j0 = 1 + (int)blockIdx.x * BX;
+  auto actU = [&](int i, int j) { return i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2; };
+  auto actV = [&](int i, int j) { return i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1; };
+  const C* xu = jac_buf(su, k);
+  const C* xv = jac_buf(sv, k);
+  // the tile cell and (t < JRING) one ring cell of sweep k, in region-1 coordinates
+  const int tr = (int)threadIdx.y + 1, tc = (int)threadIdx.x + 1;
+  int rr = 0, rc = 0;
+  if (t < JR1W) {
+    rc = t;
+  } else if (t < 2 * JR1W) {
+    rr = JR1H - 1;
+    rc = t - JR1W;
+  } else {
+    rr = 1 + ((t - 2 * JR1W) >> 1);
+    rc = ((t - 2 * JR1W) & 1) ? JR1W - 1 : 0;
+  }
+  const bool ring = t < JRING;
+  const int ti = i0 - 1 + tr, tj = j0 - 1 + tc, ri = i0 - 1 + rr, rj = j0 - 1 + rc;
+  const bool tu = au && actU(ti, tj), tv = av && actV(ti, tj);
+  const bool gu = ring && au && actU(ri, rj), gv = ring && av && actV(ri, rj);
+  // every global load first
+  constexpr int NX = (JR2N + NT - 1) / NT;
+  C vxu[NX], vxv[NX], cu[6], cv[6], du[6], dv[6];
+#pragma unroll
+  for (int i = 0; i < NX; ++i) {
+    const int q = t + i * NT;
+    const int lr = q / JR2W, lc = q - (q / JR2W) * JR2W;
+    const int gi = i0 - 2 + lr, gj = j0 - 2 + lc;
+    const bool in = q < JR2N && gi >= 0 && gi <= M && gj >= 0 && gj < P;
+    const size_t kk = in ? (size_t)gi * P + gj : 0;
+    vxu[i] = (in && au) ? xu[kk] : (C)0;
+    vxv[i] = (in && av) ? xv[kk] : (C)0;
+  }
+  auto coefs = [&](const MomSys<C>& s, bool on, int i, int j, C* c) {
+    const size_t kk = on ? (size_t)i * P + j : 0;
+    c[0] = on ? s.cS[kk] : (C)0;
+    c[1] = on ? s.cW[kk] : (C)0;
+    c[2] = on ? s.aP[kk] : (C)1;
+    c[3] = on ? s.cE[kk] : (C)0;
+    c[4] = on ? s.cN[kk] : (C)0;
+    c[5] = on ? s.b[kk] : (C)0;
+  };
+  coefs(su, tu, ti, tj, cu);
+  coefs(sv, tv, ti, tj, cv);
+  coefs(su, gu, ri, rj, du);
+  coefs(sv, gv, ri, rj, dv);
+#pragma unroll
+  for (int i = 0; i < NX; ++i) {
+    const int q = t + i * NT;
+    if (q < JR2N) {
+      xa[0][q] = vxu[i];
+      xa[1][q] = vxv[i];
+    }
+  }
+  __syncthreads();
+  // sweep k: x_(k+1) on the tile (to memory) and the ring (LDS only)
+  double ru = 0.0, rv = 0.0;
+  const int qt2 = (tr + 1) * JR2W + tc + 1, qt1 = tr * JR1W + tc;
+  const int qr2 = (rr + 1) * JR2W + rc + 1, qr1 = rr * JR1W + rc;
+  const size_t kt = (size_t)ti * P + tj;
+  {
+    C r;
+    if (tu) {
+      const C x = jac_cell(xa[0], JR2W, qt2, cu, r);
+      xb[0][qt1] = x;
+      jac_buf(su, k + 1)[kt] = x;
+      ru = (double)r * (double)r;
+    } else {
+      xb[0][qt1] = xa[0][qt2];
+    }
+    if (tv) {
+      const C x = jac_cell(xa[1], JR2W, qt2, cv, r);
+      xb[1][qt1] = x;
+      jac_buf(sv, k + 1)[kt] = x;
+      rv = (double)r * (double)r;
+    } else {
+      xb[1][qt1] = xa[1][qt2];
+    }
+    if (ring) {
+      xb[0][qr1] = gu ? jac_cell(xa[0], JR2W, qr2, du, r) : xa[0][qr2];
+      xb[1][qr1] = gv ? jac_cell(xa[1], JR2W, qr2, dv, r) : xa[1][qr2];
+    }
+  }
+  {
+    const double su2 = block_sum<NT>(ru, sh);
+    const double sv2 = block_sum<NT>(rv, sh);
+    if (t == 0) {
+      double* pout = part + (size_t)(SLOT_JAC4 + 2 * (k & 3)) * g.NB;
+      pout[block_id()] = su2;
+      pout[g.NB + block_id()] = sv2;
+    }
+  }
+  if (!two) return;
+  __syncthreads();
+  // sweep k+1 on the tile
+  ru = 0.0;
+  rv = 0.0;
+  {
+    C r;
+    if (tu) {
+      jac_buf(su, k + 2)[kt] = jac_cell(xb[0], JR1W, qt1, cu, r);
+      ru = (double)r * (double)r;
+    }
+    if (tv) {
+      jac_buf(sv, k + 2)[kt] = jac_cell(xb[1], JR1W, qt1, cv, r);
+      rv = (double)r * (double)r;
+    }
+  }
+  const double su2 = block_sum<NT>(ru, sh);
+  const double sv2 = block_sum<NT>(rv, sh);
+  if (t == 0) {
+    double* pout = part + (size_t)(SLOT_JAC4 + 2 * ((k + 1) & 3)) * g.NB;
+    pout[block_id()] = su2;
+    pout[g.NB + block_id()] = sv2;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // uStar / vStar = clamp(solution) (momentum_solver.cpp:335-351), their BCs
 // (iterations.cpp:169) and the change sums of iterations.cpp:499-533.
 // K >= 0: the sweep counts are settled here (K sweeps were launched and no
 // k_jac_end ran): a component still active after sweep K-1 has done K
-// sweeps whatever that sweep's stop test gives.
+// sweeps whatever that sweep's stop test gives.  pairs: the sweeps ran as
+// k_jacobi2 pairs (x_j in buffer j mod 3) on a grid of `pairs` blocks; if
+// the last launch was a pair, its second sweep counts only if the first
+// one's stop test failed.
 template <class C, bool Delta>
 __global__ __launch_bounds__(NT) void k_finalize(Geo g, Fields f, MomSys<C> su, MomSys<C> sv, IterState* st,
-                                                 double* part, int K) {
+                                                 double* part, int K, int pairs, double tol2) {
   __shared__ double sh[NT / kWave + 1];
   const int M = g.M, N = g.N;
   const double mv = g.maxVel;
   double su2 = 0.0, sv2 = 0.0, umx = 0.0, vmx = 0.0;
   int itU = st->jac_iters[0], itV = st->jac_iters[1];
   if (K >= 0) {
-    if (st->jac_active[0]) itU = K;
-    if (st->jac_active[1]) itV = K;
+    const int au = st->jac_active[0], av = st->jac_active[1];
+    if (pairs && K >= 2 && !(K & 1)) {  // jacobi_solver.cpp:63 on sweep K-2
+      const double* p2 = part + (size_t)(SLOT_JAC4 + 2 * ((K + 2) & 3)) * g.NB;
+      const double qu = au ? parts_load<NT>(p2, pairs) : 0.0, qv = av ? parts_load<NT>(p2 + g.NB, pairs) : 0.0;
+      if (au) itU = parts_total<NT>(qu, sh) < tol2 * (double)g.nU ? K - 1 : K;
+      if (av) itV = parts_total<NT>(qv, sh) < tol2 * (double)g.nV ? K - 1 : K;
+    } else {
+      if (au) itU = K;
+      if (av) itV = K;
+    }
     if (block_id() == 0 && thread_id() == 0) { st->jac_iters[0] = itU; st->jac_iters[1] = itV; }
   }
   FIELD_LOOP(g, i, j) {
     if (j > N) continue;
     const size_t k = at(g, i, j);
     if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) {
-      const C* x = (itU & 1) ? su.x1 : su.x0;
+      const C* x = pairs ? jac_buf(su, itU) : (itU & 1) ? su.x1 : su.x0;
       const double cur = f.u[k];
       const double nv = clampv(Delta ? cur + (double)x[k] : (double)x[k], mv);
       f.us[k] = nv;
@@ -409,7 +620,7 @@ __global__ __launch_bounds__(NT) void k_finalize(Geo g, Fields f, MomSys<C> su, 
       if (j == N - 2) f.us[k + 1] = nv;  // outlet copy u(i,N-1) = u(i,N-2)
     }
     if (i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) {
-      const C* x = (itV & 1) ? sv.x1 : sv.x0;
+      const C* x = pairs ? jac_buf(sv, itV) : (itV & 1) ? sv.x1 : sv.x0;
       const double cur = f.v[k];
       const double nv = clampv(Delta ? cur + (double)x[k] : (double)x[k], mv);
       f.vs[k] = nv;
@@ -840,6 +1051,7 @@ class SimpleSolver {
   DevBuf<double> u, v, p, us, vs, ps, dE, dN, gam, alp, inlet, inlet0;
   DevBuf<double> momD[16];  // FP64 coefficient arrays (x-form)
   DevBuf<float> momF[16];   // FP32 coefficient arrays (delta form)
+  DevBuf<float> momX2[2];   // third x buffer of the paired Jacobi sweeps (delta form)
   DevBuf<double> pcg[6];
   DevBuf<double> part;  // SLOT_COUNT x NB block partials
   DevBuf<IterState> st;
@@ -899,6 +1111,7 @@ class SimpleSolver {
       for (auto& b : momD) { b.alloc(S); CPS_HIP(hipMemsetAsync(b.p, 0, b.bytes(), ctx->stream)); }
     } else {
       for (auto& b : momF) { b.alloc(S); CPS_HIP(hipMemsetAsync(b.p, 0, b.bytes(), ctx->stream)); }
+      for (auto& b : momX2) { b.alloc(S); CPS_HIP(hipMemsetAsync(b.p, 0, b.bytes(), ctx->stream)); }
     }
     for (auto& b : pcg) { b.alloc(S); CPS_HIP(hipMemsetAsync(b.p, 0, b.bytes(), ctx->stream)); }
     // 64 x 16 tiles; rows are strided only on grids large enough to need
@@ -1024,7 +1237,9 @@ class SimpleSolver {
   template <class C>
   MomSys<C> sys(DevBuf<C>* b, int comp) {
     DevBuf<C>* q = b + 8 * comp;
-    return MomSys<C>{q[0].p, q[1].p, q[2].p, q[3].p, q[4].p, q[5].p, q[6].p, q[7].p};
+    C* x2 = nullptr;
+    if constexpr (std::is_same<C, float>::value) x2 = momX2[comp].p;
+    return MomSys<C>{q[0].p, q[1].p, q[2].p, q[3].p, q[4].p, q[5].p, q[6].p, q[7].p, x2};
   }
   PcgBufs pbufs() { return PcgBufs{pcg[0].p, pcg[1].p, pcg[2].p, pcg[3].p, pcg[4].p, pcg[5].p}; }
 
@@ -1056,7 +1271,19 @@ class SimpleSolver {
     else k_assemble<float, true><<<grid, block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p);
     CPS_HIP(hipGetLastError());
     int k = 0;
-    while (k < maxit) {
+    // two sweeps per launch on the unpolled delta path (the timed single
+    // sweeps keep their own kernel); the pair grid tiles the active cells
+    // from (1, 1) and its partials fit the slots of the field grid
+    const dim3 jgrid(ceil_div(g.N - 1, BX), ceil_div(g.M - 1, BY));
+    const bool pairs = !mode_fp64 && maxit > 0 && chunk == maxit && timer.which != KT_MOM_JACOBI &&
+                       jgrid.x > 0 && jgrid.y > 0 && (int)(jgrid.x * jgrid.y) <= g.NB;
+    if (pairs) {
+      for (; k < maxit; k += 2)
+        k_jacobi2<float><<<jgrid, block, 0, s>>>(g, sys(momF, 0), sys(momF, 1), k, k + 1 < maxit ? 1 : 0, tol2,
+                                                 st.p, part.p);
+      CPS_HIP(hipGetLastError());
+    }
+    while (!pairs && k < maxit) {
       const int k1 = std::min(maxit, k + chunk);
       for (; k < k1; ++k) {
         const bool timed = timer.active(KT_MOM_JACOBI);
@@ -1083,9 +1310,10 @@ class SimpleSolver {
     }
     const int K = maxit == 0 ? 0 : chunk == maxit ? maxit : -1;
     if (mode_fp64)
-      k_finalize<double, false><<<grid, block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p, part.p, K);
+      k_finalize<double, false><<<grid, block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p, part.p, K, 0, tol2);
     else
-      k_finalize<float, true><<<grid, block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p, part.p, K);
+      k_finalize<float, true><<<grid, block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p, part.p, K,
+                                                      pairs ? (int)(jgrid.x * jgrid.y) : 0, tol2);
     CPS_HIP(hipGetLastError());
   }
 

@@ -43,7 +43,8 @@ enum PartSlot {
   SLOT_RHS = 8,     // 4 slots: max |b|, sum b^2 (j <= N-1), r.z, ||b||^2 (DOFs)
   SLOT_PCG = 12,    // 3 slots: pAp, ||r||^2, r.z
   SLOT_SOR = 16,    // 4 slots: max |change| per colour, by iteration parity
-  SLOT_COUNT = 20
+  SLOT_JAC4 = 20,   // 8 slots: U/V sum r^2 of the paired sweeps, by sweep index mod 4
+  SLOT_COUNT = 28
 };
 
 // Geometry + physics scalars passed to kernels by value.

@@ -304,3 +304,38 @@ def test_sor_rejected_in_mixed_precision(pkg):
     cfg.precision = pkg._types.CPS_PREC_MIXED
     with pytest.raises(pkg._capi.CpsError):
         pkg.SIMPLE(cfg, gamma, inlet)
+
+
+@pytest.mark.parametrize("maxit,tol", [(1, 0.0), (2, 0.0), (3, 0.0), (7, 0.0), (8, 0.0), (32, 3e-5), (32, 1e-5),
+                                       (32, 3e-6), (32, 1e-6), (32, 3e-7), (9, 1e-6)])
+def test_paired_jacobi_sweeps_match_single(pkg, maxit, tol):
+    """Without host polling the FP32 momentum sweeps run two per launch
+    (x_j in buffer j mod 3, the stop rule of jacobi_solver.cpp:63 applied to
+    sweep k-2 and then k-1 at each launch, the last pair resolved when the
+    velocities are finalised).  Timing the single-sweep kernel keeps the
+    one-sweep-per-launch path; both must give the same sweep counts and the
+    same fields, bit for bit, for odd and even caps and for convergence on
+    either sweep of a pair."""
+    T = pkg._types
+    runs = []
+    for single in (False, True):
+        cfg, gamma, inlet = pkg.configs.density_field(nx=97, ny=41, seed=4, sigma=3.0)
+        cfg.pressure_solver, cfg.precision = T.CPS_PRESSURE_GMG, 32
+        cfg.max_momentum_iter, cfg.momentum_tol = maxit, tol
+        print(f"case maxit={maxit} tol={tol} single={single}", flush=True)
+        s = pkg.SIMPLE(cfg, gamma, inlet)
+        if single:
+            s.time_kernel("mom_jacobi")
+        iters = []
+        for _ in range(4):
+            st = s.step(1)
+            iters.append((st["u_iters"], st["v_iters"]))
+        runs.append((iters, {n: s.get(n) for n in ("u", "v", "p", "pStar")}))
+        s.close()
+    (ia, fa), (ib, fb) = runs
+    print("sweeps", ia, flush=True)
+    assert ia == ib
+    if tol == 0.0:
+        assert all(u == maxit and v == maxit for u, v in ia)
+    for n in fa:
+        assert np.array_equal(fa[n], fb[n]), n
