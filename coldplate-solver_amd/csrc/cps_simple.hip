@@ -249,9 +249,11 @@ __device__ inline void store_mom(const Geo& g, const MomCoef<IsU>& c, const doub
   }
 }
 
+// Launched on the active-cell grid (tiles from cell (1, 1), one cell per
+// thread, see active_grid): every U and V cell lies in it.
 template <class C, bool Delta>
 __global__ __launch_bounds__(NT) void k_assemble(Geo g, Fields f, MomSys<C> su, MomSys<C> sv, IterState* st) {
-  const int j = blockIdx.x * BX + threadIdx.x;
+  const int j = 1 + blockIdx.x * BX + threadIdx.x;
   const int M = g.M, N = g.N;
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
     // the Jacobi loop's state for this step
@@ -260,7 +262,8 @@ __global__ __launch_bounds__(NT) void k_assemble(Geo g, Fields f, MomSys<C> su, 
     st->jac_iters[0] = 0;
     st->jac_iters[1] = 0;
   }
-  for (int i = blockIdx.y * BY + threadIdx.y; i <= M; i += g.gy * BY) {
+  {
+    const int i = 1 + blockIdx.y * BY + threadIdx.y;
     if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) {
       MomCoef<true> c = mom_coef<true>(g, f, i, j);
       f.dE[at(g, i, j)] = g.hy / c.aP0;
@@ -1234,6 +1237,10 @@ class SimpleSolver {
   Fields fields() {
     return Fields{u.p, v.p, p.p, us.p, vs.p, ps.p, dE.p, dN.p, gam.p, alp.p, inlet.p};
   }
+  // 64 x 16 tiles over the active cells i = 1 .. M-1, j = 1 .. N-1 (one
+  // thread each): 512 blocks for 1024 x 512, one resident wave of two
+  // 1024-thread blocks per CU, where the field grid's 561 leave a tail
+  dim3 active_grid() const { return dim3(ceil_div(g.N - 1, BX), ceil_div(g.M - 1, BY)); }
   template <class C>
   MomSys<C> sys(DevBuf<C>* b, int comp) {
     DevBuf<C>* q = b + 8 * comp;
@@ -1267,14 +1274,14 @@ class SimpleSolver {
     const double tol2 = cfg.momentum_tol * cfg.momentum_tol;
     const int maxit = cfg.max_momentum_iter;
     const int chunk = maxit > kNoSyncCap ? 8 : std::max(1, maxit);
-    if (mode_fp64) k_assemble<double, false><<<grid, block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p);
-    else k_assemble<float, true><<<grid, block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p);
+    if (mode_fp64) k_assemble<double, false><<<active_grid(), block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p);
+    else k_assemble<float, true><<<active_grid(), block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p);
     CPS_HIP(hipGetLastError());
     int k = 0;
     // two sweeps per launch on the unpolled delta path (the timed single
     // sweeps keep their own kernel); the pair grid tiles the active cells
     // from (1, 1) and its partials fit the slots of the field grid
-    const dim3 jgrid(ceil_div(g.N - 1, BX), ceil_div(g.M - 1, BY));
+    const dim3 jgrid = active_grid();
     const bool pairs = !mode_fp64 && maxit > 0 && chunk == maxit && timer.which != KT_MOM_JACOBI &&
                        jgrid.x > 0 && jgrid.y > 0 && (int)(jgrid.x * jgrid.y) <= g.NB;
     if (pairs) {
