@@ -273,7 +273,8 @@ int cps_batch_gradient_dp(cps_batch* b, int field, double rtol, int max_iter, do
  * measures).  kernel: CPS_KT_*; -1 stops timing.  Selecting a kernel clears
  * the counters. */
 #define CPS_KT_MG_SMOOTH0 0   /* fine-level multigrid Jacobi smoother (pressure) */
-#define CPS_KT_MOM_JACOBI 1   /* fused U+V momentum Jacobi sweep */
+#define CPS_KT_MOM_JACOBI 1   /* fused U+V momentum Jacobi sweep; while selected, the FP32 path runs one sweep
+                                 per launch (otherwise two per launch without host polling, same results) */
 #define CPS_KT_PCG_STENCIL 2  /* fine-level PCG p-update + 5-point apply (classical form: caps above 32) */
 int cps_simple_time_kernel(cps_simple* s, int kernel);
 /* launches timed so far, their mean duration [us] and the algorithmic bytes
