@@ -853,16 +853,44 @@ struct CoarseDesc {
 // Once per pressure solve: faces and inverse diagonals of every level of the
 // coarse solver, built from the top level's faces (half the sum of the
 // covered fine faces), stored as cf[0 .. 4 total) = east, north, 1/diag, diag.
+// With par.nx > 0 the top level's faces are not in memory: they are built
+// here from its parent grid level par, with k_mg_coarsen's arithmetic (no
+// other kernel reads the top level's faces).
 template <class T>
-__global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, CoarseDesc d, T* cf) {
+__global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, Lv<T> par, CoarseDesc d, T* cf) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   T* se = reinterpret_cast<T*>(smem_raw);
   T* sn = se + d.total;
   const int t = threadIdx.x;
-  CELLS(0) {
-    const size_t k = top.at(r, c);
-    se[r * d.nx[0] + c] = top.ex[k];
-    sn[r * d.nx[0] + c] = top.ey[k];
+  if (par.nx > 0) {
+    T* pe = sn + d.total;  // the parent's faces, pitch par.nx
+    T* pn = pe + par.nx * par.ny;
+    const int fx = par.nx, fy = par.ny;
+    for (int q = t; q < fx * fy; q += kCoarseThreads) {
+      const int r = q / fx, c = q - r * fx;
+      const size_t k = par.at(r, c);
+      pe[q] = par.ex[k];
+      pn[q] = par.ey[k];
+    }
+    __syncthreads();
+    CELLS(0) {
+      const int col = min(2 * c + 1, fx - 1), row = 2 * r + 1;
+      T e = pe[2 * r * fx + col];
+      if (2 * r + 1 < fy) e += pe[(2 * r + 1) * fx + col];
+      T n = (T)0;
+      if (row <= fy - 1) {
+        n = pn[row * fx + 2 * c];
+        if (2 * c + 1 < fx) n += pn[row * fx + 2 * c + 1];
+      }
+      se[r * d.nx[0] + c] = (T)kCoarseScale * e;
+      sn[r * d.nx[0] + c] = (T)kCoarseScale * n;
+    }
+  } else {
+    CELLS(0) {
+      const size_t k = top.at(r, c);
+      se[r * d.nx[0] + c] = top.ex[k];
+      sn[r * d.nx[0] + c] = top.ey[k];
+    }
   }
   __syncthreads();
   for (int l = 1; l < d.nlev; ++l) {
@@ -1500,6 +1528,8 @@ struct MGImpl {
               "coarse multigrid plan exceeds the coarse solver's staging");
     coarseSmem = ((size_t)7 * off + (size_t)cd.dn * cd.dn) * sizeof(T);
     coarseSetupSmem = (size_t)2 * off * sizeof(T);
+    if (nGrid > std::min(nGrid, kHierLevels))  // the setup builds the top's faces from its parent
+      coarseSetupSmem += (size_t)2 * lv[nGrid - 1]->nx * lv[nGrid - 1]->ny * sizeof(T);
     int lds = 0;
     CPS_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
     CPS_CHECK(coarseSmem <= (size_t)lds, CPS_ERR_STATE, "coarse multigrid levels exceed LDS");
@@ -1546,9 +1576,14 @@ struct MGImpl {
     H.n = std::min((int)lv.size() - 1, kHierLevels);
     for (int l = 0; l <= H.n; ++l) H.l[l] = lv[l]->view();
     k_mg_hier<T><<<dim3(ceil_div(lv[0]->nx, HX), ceil_div(lv[0]->ny, HY)), HT, 0, s>>>(g, dE, dN, H);
-    for (size_t l = H.n + 1; l < lv.size(); ++l)
+    // deeper grid levels one launch each; the coarse top's faces, if not
+    // built above, are formed by the coarse setup from its parent
+    const bool topFromParent = nGrid > H.n;
+    for (int l = H.n + 1; l < nGrid; ++l)
       k_mg_coarsen<T><<<lv[l]->grid(), blk(), 0, s>>>(lv[l - 1]->view(), lv[l]->view());
-    k_mg_coarse_setup<T><<<1, kCoarseThreads, coarseSetupSmem, s>>>(view(nGrid), cd, cf.p);
+    const Lv<T> none{};
+    k_mg_coarse_setup<T><<<1, kCoarseThreads, coarseSetupSmem, s>>>(view(nGrid), topFromParent ? view(nGrid - 1) : none,
+                                                                    cd, cf.p);
     CPS_HIP(hipGetLastError());
   }
 
