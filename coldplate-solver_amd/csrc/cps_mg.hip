@@ -292,6 +292,16 @@ struct Tile {
   }
 };
 
+// XCD-aware tile order for the level kernels: the hardware hands block b to
+// XCD b mod 8, so the logical tile L = (b mod 8) (nb / 8) + b / 8 gives each
+// XCD a contiguous band of tiles, whose shared halo lines then meet in that
+// XCD's L2.  Block partials are stored at L, so every reduction keeps its
+// order (and its result).
+__device__ inline int xcd_tile() {
+  const int nb = (int)(gridDim.x * gridDim.y), b = (int)(blockIdx.x + blockIdx.y * gridDim.x);
+  return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+}
+
 // Stage b, the faces and (up kernels) x1 + P e of the tile + halo in LDS.
 // All global loads of a thread are issued before the first LDS store.
 template <class T>
@@ -334,7 +344,8 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
   __shared__ int sflag;
   const int t = threadIdx.x + threadIdx.y * TX;
   const int stop0 = t == 0 ? *stop : 0;  // thread 0's view, published below
-  Tile<T> tl{sb, se, sn, (int)blockIdx.y * TY, (int)blockIdx.x * TX, L.nx, L.ny};
+  const int lt = xcd_tile();
+  Tile<T> tl{sb, se, sn, lt / (int)gridDim.x * TY, lt % (int)gridDim.x * TX, L.nx, L.ny};
   bool x0_ready = false;  // the fused branch forms the first sweep with b
   if (xr.rold) {
     const double prz = parts_load<FT>(xr.rz, xr.nb);
@@ -416,7 +427,7 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
       e = (double)rn * (double)rn;
     }
     const double sblk = block_sum_t0<FT>(e, sh);  // the barrier after this branch follows
-    if (t == 0) xr.err[blockIdx.x + blockIdx.y * gridDim.x] = sblk;
+    if (t == 0) xr.err[lt] = sblk;
   } else {
   const double perr = test.part ? parts_load<FT>(test.part, test.nb) : 0.0;
   tile_load<T>(L, tl, t, nullptr, nullptr);
@@ -671,7 +682,8 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
   __shared__ int sflag;
   const int t = threadIdx.x + threadIdx.y * TX;
   const int stop0 = t == 0 ? *stop : 0;
-  Tile<T> tl{sb, se, sn, (int)blockIdx.y * TY, (int)blockIdx.x * TX, L.nx, L.ny};
+  const int lt = xcd_tile();
+  Tile<T> tl{sb, se, sn, lt / (int)gridDim.x * TY, lt % (int)gridDim.x * TX, L.nx, L.ny};
   tile_load<T>(L, tl, t, s0, &Cl);
   if (t == 0) sflag = stop0;
   __syncthreads();
@@ -699,8 +711,8 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
     __shared__ double sh2[2 * (FT / kWave)];
     block_sum2_all<FT>(e, e2, sh2);
     if (t == 0) {
-      part[blockIdx.x + blockIdx.y * gridDim.x] = e;
-      part2[blockIdx.x + blockIdx.y * gridDim.x] = e2;
+      part[lt] = e;
+      part2[lt] = e2;
     }
     return;
   }
@@ -720,7 +732,7 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
   if (part) {
     __shared__ double sh[FT / kWave + 1];
     const double s = block_sum<FT>(e, sh);
-    if (t == 0) part[blockIdx.x + blockIdx.y * gridDim.x] = s;
+    if (t == 0) part[lt] = s;
   }
 }
 
