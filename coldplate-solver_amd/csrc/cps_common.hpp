@@ -10,6 +10,17 @@
 
 namespace cps {
 
+// XCD-aware tile order for tiled kernels whose blocks read halos: the hardware hands block b to
+// XCD b mod 8, so the logical tile L = (b mod 8) (nb / 8) + b / 8 gives each
+// XCD a contiguous band of tiles, whose shared halo lines then meet in that
+// XCD's L2.  Block partials are stored at L, so every reduction keeps its
+// order (and its result).
+__device__ inline int xcd_tile() {
+  const int nb = (int)(gridDim.x * gridDim.y), b = (int)(blockIdx.x + blockIdx.y * gridDim.x);
+  return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+
 // ---------------------------------------------------------------------------
 // error plumbing: a failed call records its message (thread-local) and the
 // C entry point returns the status.

@@ -292,16 +292,6 @@ struct Tile {
   }
 };
 
-// XCD-aware tile order for the level kernels: the hardware hands block b to
-// XCD b mod 8, so the logical tile L = (b mod 8) (nb / 8) + b / 8 gives each
-// XCD a contiguous band of tiles, whose shared halo lines then meet in that
-// XCD's L2.  Block partials are stored at L, so every reduction keeps its
-// order (and its result).
-__device__ inline int xcd_tile() {
-  const int nb = (int)(gridDim.x * gridDim.y), b = (int)(blockIdx.x + blockIdx.y * gridDim.x);
-  return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
-}
-
 // Stage b, the faces and (up kernels) x1 + P e of the tile + halo in LDS.
 // All global loads of a thread are issued before the first LDS store.
 template <class T>
@@ -1161,11 +1151,12 @@ struct Grid0 {  // tile grid of the level-0 kernels (one TX x TY tile per block)
 };
 
 // one cell per thread: blocks of TX x TY threads over the tile grid
-#define L0_LOOP(G, L, r, c)                                    \
-  const int c = blockIdx.x * TX + threadIdx.x;                 \
-  for (int r = blockIdx.y * TY + threadIdx.y; r < (L).ny; r = (L).ny)
+// (tiles in XCD-aware order; bid() is the tile, the index of its partials)
+#define L0_LOOP(G, L, r, c)                                                  \
+  const int c = xcd_tile() % (int)gridDim.x * TX + (int)threadIdx.x;         \
+  for (int r = xcd_tile() / (int)gridDim.x * TY + (int)threadIdx.y; r < (L).ny; r = (L).ny)
 
-__device__ inline int bid() { return blockIdx.x + blockIdx.y * gridDim.x; }
+__device__ inline int bid() { return xcd_tile(); }
 __device__ inline int tid() { return threadIdx.x + threadIdx.y * TX; }
 
 // r = src (cast), x = 0, partial ||r||^2.  An outer refinement loop that has
@@ -1215,7 +1206,7 @@ __global__ __launch_bounds__(FT) void k_pcg_ap(Grid0 G, Lv<T> L, const T* z, con
   const int stop0 = t == 0 ? ps->stop : 0;
   const double prz = parts_load<FT>(part + (size_t)S_RZ * G.NB, G.NB);
   const double rzold = it > 0 ? ps->rzs[(it - 1) & 1] : 1.0;
-  const int c = blockIdx.x * TX + threadIdx.x, r = blockIdx.y * TY + threadIdx.y;
+  const int c = bid() % (int)gridDim.x * TX + (int)threadIdx.x, r = bid() / (int)gridDim.x * TY + (int)threadIdx.y;
   const bool in = r < L.ny && c < L.nx;
   const size_t k = in ? L.at(r, c) : 0;
   T zc = 0, zE = 0, zW = 0, zN = 0, zS = 0, pc = 0, pE = 0, pW = 0, pN = 0, pS = 0, fe = 0, fw = 0, fn = 0, fs = 0;
@@ -1260,7 +1251,7 @@ __global__ __launch_bounds__(FT) void k_pcg_xr(Grid0 G, Lv<T> L, T* xv, T* rv, c
   const int stop0 = t == 0 ? ps->stop : 0;
   const double ppap = parts_load<FT>(part + (size_t)S_PAP * G.NB, G.NB);
   const double rz = ps->rzs[it & 1];
-  const int c = blockIdx.x * TX + threadIdx.x, r = blockIdx.y * TY + threadIdx.y;
+  const int c = bid() % (int)gridDim.x * TX + (int)threadIdx.x, r = bid() / (int)gridDim.x * TY + (int)threadIdx.y;
   const bool in = r < L.ny && c < L.nx;
   const size_t k = in ? L.at(r, c) : 0;
   T x = 0, rr = 0, p = 0, ap = 0;
@@ -1295,7 +1286,7 @@ __global__ __launch_bounds__(FT) void k_pcg_cg_last(Grid0 G, Lv<T> L, XrFuse<T> 
   const int stop0 = t == 0 ? xr.ps->stop : 0;
   const double prz = parts_load<FT>(xr.rz, G.NB);
   const double pzw = parts_load<FT>(xr.zw, G.NB);
-  const int c = blockIdx.x * TX + threadIdx.x, r = blockIdx.y * TY + threadIdx.y;
+  const int c = bid() % (int)gridDim.x * TX + (int)threadIdx.x, r = bid() / (int)gridDim.x * TY + (int)threadIdx.y;
   const bool in = r < L.ny && c < L.nx;
   const size_t k = in ? L.at(r, c) : 0;
   T xo = 0, rr = 0, zc = 0, po = 0, wo = 0, so = 0;

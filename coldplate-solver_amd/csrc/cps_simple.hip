@@ -253,7 +253,8 @@ __device__ inline void store_mom(const Geo& g, const MomCoef<IsU>& c, const doub
 // thread, see active_grid): every U and V cell lies in it.
 template <class C, bool Delta>
 __global__ __launch_bounds__(NT) void k_assemble(Geo g, Fields f, MomSys<C> su, MomSys<C> sv, IterState* st) {
-  const int j = 1 + blockIdx.x * BX + threadIdx.x;
+  const int lt = xcd_tile();  // tiles in XCD-aware order (the SOU stencil reads two-cell halos)
+  const int j = 1 + lt % (int)gridDim.x * BX + (int)threadIdx.x;
   const int M = g.M, N = g.N;
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
     // the Jacobi loop's state for this step
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(NT) void k_assemble(Geo g, Fields f, MomSys<C> su, 
     st->jac_iters[1] = 0;
   }
   {
-    const int i = 1 + blockIdx.y * BY + threadIdx.y;
+    const int i = 1 + lt / (int)gridDim.x * BY + (int)threadIdx.y;
     if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) {
       MomCoef<true> c = mom_coef<true>(g, f, i, j);
       f.dE[at(g, i, j)] = g.hy / c.aP0;
@@ -461,7 +462,8 @@ __global__ __launch_bounds__(NT) void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> s
   }
   if (!au && !av) return;  // uniform over the grid
   const int M = g.M, N = g.N, P = g.P;
-  const int i0 = 1 + (int)blockIdx.y * BY, j0 = 1 + (int)blockIdx.x * BX;
+  const int lt = xcd_tile();
+  const int i0 = 1 + lt / (int)gridDim.x * BY, j0 = 1 + lt % (int)gridDim.x * BX;
   auto actU = [&](int i, int j) { return i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2; };
   auto actV = [&](int i, int j) { return i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1; };
   const C* xu = jac_buf(su, k);
@@ -550,8 +552,8 @@ __global__ __launch_bounds__(NT) void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> s
     const double sv2 = block_sum<NT>(rv, sh);
     if (t == 0) {
       double* pout = part + (size_t)(SLOT_JAC4 + 2 * (k & 3)) * g.NB;
-      pout[block_id()] = su2;
-      pout[g.NB + block_id()] = sv2;
+      pout[lt] = su2;
+      pout[g.NB + lt] = sv2;
     }
   }
   if (!two) return;
@@ -574,8 +576,8 @@ __global__ __launch_bounds__(NT) void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> s
   const double sv2 = block_sum<NT>(rv, sh);
   if (t == 0) {
     double* pout = part + (size_t)(SLOT_JAC4 + 2 * ((k + 1) & 3)) * g.NB;
-    pout[block_id()] = su2;
-    pout[g.NB + block_id()] = sv2;
+    pout[lt] = su2;
+    pout[g.NB + lt] = sv2;
   }
 }
 
