@@ -3,7 +3,7 @@
 All calls go through the C ABI (libcoldplate_hip.so via ctypes).  Tolerance:
 the FP64 device path follows the oracle's arithmetic step by step, so after a
 few SIMPLE iterations fields agree to round-off (rel. L2 <= 1e-10); at the
-converged state the bar is the north-star 1e-8.
+converged state the bar is rel. L2 <= 1e-8.
 """
 import numpy as np
 import pytest
