@@ -63,14 +63,27 @@ struct Op {
   }
 };
 
-// one thread per cell: (i, j) from the 64 x 4 block, k = blockIdx.z
-#define CELL_IJK(NX, NY)                                                            \
-  const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y; \
-  const int k = blockIdx.z;                                                         \
+// Tiles in XCD-aware order: the hardware hands block b (linear over x, y, z)
+// to XCD b mod 8; tile L = the blocks before b's XCD's band + b / 8 gives
+// every XCD a contiguous band of tiles, whose shared halo lines then meet in
+// that XCD's L2.  flat_bid() is that tile, so block partials keep their order.
+__device__ inline int flat_bid() {
+  const int nb = (int)(gridDim.x * gridDim.y * gridDim.z);
+  const int b = (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+  const int q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+__device__ inline int tile_x() { return flat_bid() % (int)gridDim.x; }
+__device__ inline int tile_y() { return flat_bid() / (int)gridDim.x % (int)gridDim.y; }
+__device__ inline int tile_z() { return flat_bid() / (int)(gridDim.x * gridDim.y); }
+
+// one thread per cell: (i, j) from the 64 x 4 tile, k = the tile's layer
+#define CELL_IJK(NX, NY)                                                                    \
+  const int i = tile_x() * TBX + (int)threadIdx.x, j = tile_y() * TBY + (int)threadIdx.y; \
+  const int k = tile_z();                                                                   \
   const bool inside = i < (NX) && j < (NY)
 
 __device__ inline int flat_tid() { return threadIdx.x + threadIdx.y * TBX; }
-__device__ inline int flat_bid() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
 __device__ inline int num_blocks() { return gridDim.x * gridDim.y * gridDim.z; }
 
 // block partials of two sums into part[bid] and part[nb + bid]
@@ -242,7 +255,7 @@ __global__ __launch_bounds__(TNT) void k_cast_vec(Op<To> A, const Ti* in, To* ou
 // is the sum of the fine diagonals plus the couplings inside the aggregate.
 template <class T>
 __global__ __launch_bounds__(TNT) void k_coarsen(Op<T> F, Op<T> C) {
-  const int I = blockIdx.x * TBX + threadIdx.x, J = blockIdx.y * TBY + threadIdx.y, k = blockIdx.z;
+  const int I = tile_x() * TBX + (int)threadIdx.x, J = tile_y() * TBY + (int)threadIdx.y, k = tile_z();
   if (I >= C.nx || J >= C.ny) return;
   T d = 0, w = 0, e = 0, s = 0, n = 0, b = 0, t = 0;
   for (int dj = 0; dj < 2; ++dj)
@@ -326,7 +339,7 @@ __global__ __launch_bounds__(TNT) void k_jac(Lvl<T> L, const T* src, T* dst, con
 // zc = t / den, zdi = 1 / den with den = d - b zc(k-1).
 template <class T>
 __global__ __launch_bounds__(TNT) void k_zfactor(Op<T> A, T* zc, T* zdi) {
-  const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+  const int i = tile_x() * TBX + (int)threadIdx.x, j = tile_y() * TBY + (int)threadIdx.y;
   if (i >= A.nx || j >= A.ny) return;
   double cp = 0.0;
   for (int k = 0; k < A.nz; ++k) {
@@ -394,7 +407,7 @@ __device__ inline void zjac_col(const Lvl<T>& L, const T* zc, const T* zdi, cons
 template <class T, int NZ>
 __global__ __launch_bounds__(TNT) void k_zjac(Lvl<T> L, const T* zc, const T* zdi, const T* src, T* dst,
                                              const T* cy, Op<T> C, int mode) {
-  const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+  const int i = tile_x() * TBX + (int)threadIdx.x, j = tile_y() * TBY + (int)threadIdx.y;
   if (i >= L.A.nx || j >= L.A.ny || L.stopped()) return;
   zjac_col<T, NZ>(L, zc, zdi, src, dst, cy, C, mode, i, j);
 }
@@ -412,7 +425,7 @@ __device__ inline void restrict_cell(const Lvl<T>& L, const T* x, const Op<T>& C
 }
 template <class T>
 __global__ __launch_bounds__(TNT) void k_restrict(Lvl<T> L, const T* x, Op<T> C, T* crhs) {
-  const int I = blockIdx.x * TBX + threadIdx.x, J = blockIdx.y * TBY + threadIdx.y, k = blockIdx.z;
+  const int I = tile_x() * TBX + (int)threadIdx.x, J = tile_y() * TBY + (int)threadIdx.y, k = tile_z();
   if (I >= C.nx || J >= C.ny || L.stopped()) return;
   restrict_cell(L, x, C, crhs, I, J, k);
 }
@@ -874,7 +887,7 @@ __global__ __launch_bounds__(TNT) void k_objgrad(Phys P, Op<double> A, const dou
 // R = A T - b.  Only conduction depends on k: the x-y harmonic faces, the z
 // series faces and the inlet Dirichlet term of fluid cells at i = 0.
 __global__ __launch_bounds__(TNT) void k_sens(Phys P, Op<double> A, const double* T, const double* lam, double* grad) {
-  const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+  const int i = tile_x() * TBX + (int)threadIdx.x, j = tile_y() * TBY + (int)threadIdx.y;
   if (i >= P.nx || j >= P.ny) return;
   const double dx = P.dx, dy = P.dy, Az = dx * dy;
   const size_t ks = (size_t)P.ny * A.pitch;
@@ -904,7 +917,7 @@ __global__ __launch_bounds__(TNT) void k_sens(Phys P, Op<double> A, const double
 // boundary side on the diagonal (:64-71); row = y (i of the script), column
 // = x (j).  k, P, sens, gamma are ny x nx, row pitch nx.
 __global__ __launch_bounds__(TNT) void k_surr_assemble(int nx, int ny, double dx2inv, const double* k, Op<double> A) {
-  const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+  const int i = tile_x() * TBX + (int)threadIdx.x, j = tile_y() * TBY + (int)threadIdx.y;
   if (i >= nx || j >= ny) return;
   const double kP = k[j * nx + i];
   auto coef = [&](double kN) { return 2 * kP * kN / (kP + kN) * dx2inv; };
@@ -925,7 +938,7 @@ __global__ __launch_bounds__(TNT) void k_surr_assemble(int nx, int ny, double dx
 __global__ __launch_bounds__(TNT) void k_surr_sens(int nx, int ny, double dx2inv, const double* k, Op<double> A,
                                                   const double* T, const double* lam, double* sens,
                                                   const double* gamma, double ks, double kf, double qk, double* dJ) {
-  const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+  const int i = tile_x() * TBX + (int)threadIdx.x, j = tile_y() * TBY + (int)threadIdx.y;
   if (i >= nx || j >= ny) return;
   const size_t q = A.at(i, j, 0);
   const double kP = k[j * nx + i], tP = T[q], lP = lam[q];
