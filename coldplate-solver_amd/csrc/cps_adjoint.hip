@@ -55,7 +55,6 @@ namespace adj {
 
 constexpr int K = 16;          // dual slots: distinct DOFs one equation reads
 constexpr int BS = 256;        // threads per block of the vector kernels
-constexpr double kHalfCellTol = (double)0.999f;
 constexpr double kVelTol = (double)1e-8f;
 constexpr double kAP0Floor = (double)1e-10f;
 
@@ -213,17 +212,17 @@ __device__ Dual mom_residual(const View& V, Tape& t, int i, int j, double* ap0) 
   Dual aN = pos(-Fn) + g.Dn;
   Dual aS = pos(Fs) + g.Dn;
   if (IsU) {
-    if (i == 1 && 0.5 * (G(0, j - 1) + G(0, j)) >= kHalfCellTol && fabs(Ur(0, j)) <= kVelTol &&
+    if (i == 1 && face_fluid(G(0, j - 1), G(0, j)) && fabs(Ur(0, j)) <= kVelTol &&
         fabs(Vr(0, j)) <= kVelTol && fabs(Vr(0, j + 1)) <= kVelTol)
       aS = aS + g.Dn;
-    if (i == M - 1 && 0.5 * (G(M - 2, j - 1) + G(M - 2, j)) >= kHalfCellTol && fabs(Ur(M, j)) <= kVelTol &&
+    if (i == M - 1 && face_fluid(G(M - 2, j - 1), G(M - 2, j)) && fabs(Ur(M, j)) <= kVelTol &&
         fabs(Vr(M - 1, j)) <= kVelTol && fabs(Vr(M - 1, j + 1)) <= kVelTol)
       aN = aN + g.Dn;
   } else {
-    if (j == 1 && 0.5 * (G(i - 1, 0) + G(i, 0)) >= kHalfCellTol && !(fabs(g.targetVel) > kVelTol) &&
+    if (j == 1 && face_fluid(G(i - 1, 0), G(i, 0)) && !(fabs(g.targetVel) > kVelTol) &&
         fabs(Vr(i, 0)) <= kVelTol && fabs(Ur(i, 0)) <= kVelTol && fabs(Ur(i + 1, 0)) <= kVelTol)
       aW = aW + g.De;
-    if (j == N - 1 && 0.5 * (G(i - 1, N - 2) + G(i, N - 2)) >= kHalfCellTol &&
+    if (j == N - 1 && face_fluid(G(i - 1, N - 2), G(i, N - 2)) &&
         !(fabs(V.p[at(g, i, N)]) <= kVelTol) && fabs(Vr(i, N)) <= kVelTol && fabs(Ur(i, N - 1)) <= kVelTol &&
         fabs(Ur(i + 1, N - 1)) <= kVelTol)
       aE = aE + g.De;
@@ -797,7 +796,7 @@ __global__ void k_objective(View V, double tol, int n, double* gvec, double* out
   const Geo& g = V.g;
   double cnt = 0.0, sum = 0.0;
   for (int i = 1 + threadIdx.x; i < g.M; i += BS)
-    if (V.gam[at(g, i - 1, 0)] >= 1.0 - tol) {
+    if (pure_fluid(V.gam[at(g, i - 1, 0)], tol)) {
       cnt += 1.0;
       sum += V.p[at(g, i, 1)];
     }
@@ -811,7 +810,7 @@ __global__ void k_objective(View V, double tol, int n, double* gvec, double* out
   __syncthreads();
   if (cnt > 0)
     for (int i = 1 + threadIdx.x; i < g.M; i += BS)
-      if (V.gam[at(g, i - 1, 0)] >= 1.0 - tol) gvec[2 * V.S + at(g, i, 1)] = 1.0 / cnt;
+      if (pure_fluid(V.gam[at(g, i - 1, 0)], tol)) gvec[2 * V.S + at(g, i, 1)] = 1.0 / cnt;
   if (threadIdx.x == 0) {
     out[0] = cnt > 0 ? sum / cnt : 0.0;
     out[1] = cnt;

@@ -20,6 +20,16 @@ __device__ inline int xcd_tile() {
   return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
 }
 
+// Mask decisions in the reference's float arithmetic on its float gamma
+// (a MatrixXf, SIMPLE.h:479-520), so the FP64 path takes the reference's
+// discrete choices bit for bit:
+//   face_fluid   gammaAtU/V >= 0.999f, the average formed in float
+//                (SIMPLE.h:740-778, boundaries.cpp:148-206);
+//   inlet_fluid  gamma(cellRow, 0) > 0.5f (boundaries.cpp:82-100);
+//   pure_fluid   !(gamma < 1.0f - tol) (postprocessing.cpp:155-160, 268).
+__host__ __device__ inline bool face_fluid(double a, double b) { return 0.5f * ((float)a + (float)b) >= 0.999f; }
+__host__ __device__ inline bool inlet_fluid(double g) { return (float)g > 0.5f; }
+__host__ __device__ inline bool pure_fluid(double g, double tol) { return !((float)g < 1.0f - (float)tol); }
 
 // ---------------------------------------------------------------------------
 // error plumbing: a failed call records its message (thread-local) and the

@@ -1543,6 +1543,11 @@ struct MGImpl {
     G0.gx = ceil_div(lv[0]->nx, TX);
     G0.gy = ceil_div(lv[0]->ny, TY);
     G0.NB = G0.gx * G0.gy;
+    // every fused PCG reduction reads its block partials two per thread
+    // (parts_load), so the fine level may have at most 2 * FT tiles
+    CPS_CHECK(G0.NB <= 2 * FT && G0.NB <= 2 * kCoarseThreads, CPS_ERR_ARG,
+              "grid too large for the multigrid pressure solver: at most 2048 tiles of 64 x 16 pressure DOFs "
+              "(e.g. 2048 x 1024 cells)");
     part.alloc((size_t)S_COUNT * G0.NB);
     CPS_HIP(hipMemsetAsync(part.p, 0, part.bytes(), ctx->stream));
     scal.alloc(2);

@@ -46,7 +46,6 @@ namespace cps {
 
 // float literals of the reference (SIMPLE.h / *.cpp), widened exactly
 constexpr double kTenthF = (double)0.1f;
-constexpr double kHalfCellTol = (double)0.999f;
 constexpr double kVelTol = (double)1e-8f;
 constexpr double kAP0Floor = (double)1e-10f;
 constexpr double kAlphaEps = (double)1e-10f;
@@ -151,17 +150,17 @@ __device__ inline MomCoef<IsU> mom_coef(const Geo& g, const Fields& f, int i, in
   double aS = Dn + fmax(0.0, Fs);
   // half-cell wall diffusion (boundaries.cpp:148-206)
   if (IsU) {
-    if (i == 1 && 0.5 * (G(0, j - 1) + G(0, j)) >= kHalfCellTol && fabs(U(0, j)) <= kVelTol &&
+    if (i == 1 && face_fluid(G(0, j - 1), G(0, j)) && fabs(U(0, j)) <= kVelTol &&
         fabs(V(0, j)) <= kVelTol && fabs(V(0, j + 1)) <= kVelTol)
       aS += Dn;
-    if (i == M - 1 && 0.5 * (G(M - 2, j - 1) + G(M - 2, j)) >= kHalfCellTol && fabs(U(M, j)) <= kVelTol &&
+    if (i == M - 1 && face_fluid(G(M - 2, j - 1), G(M - 2, j)) && fabs(U(M, j)) <= kVelTol &&
         fabs(V(M - 1, j)) <= kVelTol && fabs(V(M - 1, j + 1)) <= kVelTol)
       aN += Dn;
   } else {
-    if (j == 1 && 0.5 * (G(i - 1, 0) + G(i, 0)) >= kHalfCellTol && !(fabs(g.targetVel) > kVelTol) &&
+    if (j == 1 && face_fluid(G(i - 1, 0), G(i, 0)) && !(fabs(g.targetVel) > kVelTol) &&
         fabs(V(i, 0)) <= kVelTol && fabs(U(i, 0)) <= kVelTol && fabs(U(i + 1, 0)) <= kVelTol)
       aW += De;
-    if (j == N - 1 && 0.5 * (G(i - 1, N - 2) + G(i, N - 2)) >= kHalfCellTol &&
+    if (j == N - 1 && face_fluid(G(i - 1, N - 2), G(i, N - 2)) &&
         !(fabs(f.p[at(g, i, N)]) <= kVelTol) && fabs(V(i, N)) <= kVelTol && fabs(U(i, N - 1)) <= kVelTol &&
         fabs(U(i + 1, N - 1)) <= kVelTol)
       aE += De;
@@ -678,6 +677,10 @@ __device__ inline double mass_b(const Geo& g, const Fields& f, int i, int j) {
 
 // Mass residual (j = 1..N-1) + RHS and PCG start (x0 = 0, r = b,
 // z = r/diag, p_old = 0) for the DOFs j = 1..N-2.  pStar is cleared.
+// PcgStart: also the Jacobi-PCG start (diagonal, z = r / diag, cleared
+// search directions, r.z).  The SOR and multigrid solves read only r, so
+// their steps skip those stores and the pressure coefficients.
+template <bool PcgStart>
 __global__ __launch_bounds__(NT) void k_prhs(Geo g, Fields f, PcgBufs q, double* part) {
   __shared__ double sh[NT / kWave + 1];
   const int M = g.M, N = g.N;
@@ -692,15 +695,17 @@ __global__ __launch_bounds__(NT) void k_prhs(Geo g, Fields f, PcgBufs q, double*
       mx = fmax(mx, ab);
       ss += ab * ab;
       if (j <= N - 2) {
-        PCoef c = p_coef(g, f.dE, f.dN, i, j);
-        q.diag[k] = c.aP;
         q.r[k] = b;
-        const double z = (fabs(c.aP) > kTiny20) ? b / c.aP : b;
-        q.z[k] = z;
-        q.pa[k] = 0.0;
-        q.pb[k] = 0.0;
-        rz += b * z;
         bb += b * b;
+        if (PcgStart) {
+          PCoef c = p_coef(g, f.dE, f.dN, i, j);
+          q.diag[k] = c.aP;
+          const double z = (fabs(c.aP) > kTiny20) ? b / c.aP : b;
+          q.z[k] = z;
+          q.pa[k] = 0.0;
+          q.pb[k] = 0.0;
+          rz += b * z;
+        }
       }
     }
   }
@@ -1005,7 +1010,7 @@ __global__ void k_sample(Geo g, Fields f, double tol, SampleSet set, double* out
   double sA = 0, dA = 0, tw = 0, area = 0, flux = 0;
   for (int i = 1 + threadIdx.x; i < M; i += blockDim.x) {
     const int row = i - 1;
-    if (f.gam[at(g, row, c.gl)] < 1.0 - tol || f.gam[at(g, row, c.gr)] < 1.0 - tol) continue;
+    if (!pure_fluid(f.gam[at(g, row, c.gl)], tol) || !pure_fluid(f.gam[at(g, row, c.gr)], tol)) continue;
     const double fa = g.hy;
     area += fa;
     const int vTop = i - 1, vBot = min(i, M - 1);
@@ -1067,6 +1072,28 @@ class SimpleSolver {
   dim3 grid, block;
   int* hflag = nullptr;
 
+  // pressureSolverType 2 / 4 (AMGCL CG with AMG, CPU / CUDA) and 3 (sparse
+  // LDLT) have no build of their own: like the reference when it is built
+  // without those libraries (SIMPLE.cpp:147-186), they fall back to another
+  // backend.  Here that backend is the device multigrid PCG with the stop
+  // rule of the solver asked for: AMGCL's ||r|| <= pTol ||b|| within
+  // maxPressureIter iterations (pressure_solver.cpp:754-755,
+  // pressure_solver.cu:115-116), and for the direct solve a residual at
+  // FP64 round-off level (1e-12 ||b||, at least 100 iterations allowed).
+  static void resolve_pressure_solver(cps_simple_config& c) {
+    if (c.pressure_solver == CPS_PRESSURE_AMGCL || c.pressure_solver == CPS_PRESSURE_AMGCL_CUDA) {
+      c.p_rtol = c.p_tol;
+      c.p_tol = 0.0;
+      c.pressure_solver = CPS_PRESSURE_GMG;
+    } else if (c.pressure_solver == CPS_PRESSURE_LDLT) {
+      c.p_rtol = 1e-12;
+      c.p_tol = 0.0;
+      c.max_pressure_iter = std::max(c.max_pressure_iter, 100);
+      c.max_refine = std::max(c.max_refine, 8);
+      c.pressure_solver = CPS_PRESSURE_GMG;
+    }
+  }
+
   SimpleSolver(Ctx* c, const cps_simple_config& conf, const double* gamma, const double* inlet_u)
       : ctx(c), cfg(conf) {
     CPS_CHECK(cfg.nx >= 2 && cfg.ny >= 2, CPS_ERR_ARG, "nx, ny must be >= 2");
@@ -1074,9 +1101,9 @@ class SimpleSolver {
     CPS_CHECK(cfg.uv_alpha > 0 && cfg.p_alpha > 0, CPS_ERR_ARG, "relaxation factors must be > 0");
     CPS_CHECK(cfg.precision == CPS_PREC_FP64 || cfg.precision == CPS_PREC_MIXED, CPS_ERR_ARG,
               "precision must be 64 or 32");
-    CPS_CHECK(cfg.pressure_solver == CPS_PRESSURE_SOR || cfg.pressure_solver == CPS_PRESSURE_PCG ||
-                  cfg.pressure_solver == CPS_PRESSURE_GMG,
-              CPS_ERR_ARG, "pressure_solver must be 0 (SOR), 1 (PCG) or 5 (GMG)");
+    CPS_CHECK(cfg.pressure_solver >= CPS_PRESSURE_SOR && cfg.pressure_solver <= CPS_PRESSURE_GMG, CPS_ERR_ARG,
+              "pressure_solver must be 0 (SOR), 1 (PCG), 2 (AMGCL), 3 (LDLT), 4 (AMGCL-CUDA) or 5 (GMG)");
+    resolve_pressure_solver(cfg);
     CPS_CHECK(!(cfg.precision == CPS_PREC_MIXED && cfg.pressure_solver != CPS_PRESSURE_GMG), CPS_ERR_ARG,
               "mixed precision needs the GMG pressure solver (5)");
     CPS_CHECK(gamma != nullptr, CPS_ERR_ARG, "gamma is required");
@@ -1152,7 +1179,7 @@ class SimpleSolver {
       int cr = i <= 0 ? 0 : (i >= M ? std::max(0, M - 2) : i - 1);
       cr = std::max(0, std::min(cr, M - 1));
       const double speed = inlet_u ? inlet_u[i] : cfg.u_in;
-      hinlet[i] = hg[(size_t)cr * g.P] > (double)0.5f ? speed : 0.0;
+      hinlet[i] = inlet_fluid(hg[(size_t)cr * g.P]) ? speed : 0.0;
     }
     CPS_HIP(hipMemcpyAsync(inlet.p, hinlet.data(), (M + 1) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     CPS_HIP(hipMemcpyAsync(inlet0.p, hinlet.data(), (M + 1) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
@@ -1381,7 +1408,8 @@ class SimpleSolver {
     hipStream_t s = ctx->stream;
     const Fields f = fields();
     momentum();
-    k_prhs<<<grid, block, 0, s>>>(g, f, pbufs(), part.p);
+    if (cfg.pressure_solver == CPS_PRESSURE_PCG) k_prhs<true><<<grid, block, 0, s>>>(g, f, pbufs(), part.p);
+    else k_prhs<false><<<grid, block, 0, s>>>(g, f, pbufs(), part.p);
     // the Jacobi-PCG and SOR solves start from the statistics' r.z and
     // ||b||^2; the multigrid solve reads none of them, so there the
     // reductions wait until the host asks (stats())
@@ -1611,7 +1639,8 @@ class BatchSolver {
       : base(c) {
     CPS_CHECK(n >= 1 && cfgs && gammas, CPS_ERR_ARG, "batch needs n >= 1 fields with configs and gammas");
     for (int i = 0; i < n; ++i) {
-      const cps_simple_config& cf = cfgs[i];
+      cps_simple_config cf = cfgs[i];
+      SimpleSolver::resolve_pressure_solver(cf);
       // capture-safe caps: no host synchronisation inside one iteration
       CPS_CHECK(cf.max_momentum_iter <= SimpleSolver::kNoSyncCap && cf.max_pressure_iter <= SimpleSolver::kNoSyncCap &&
                     (cf.precision != CPS_PREC_MIXED || cf.max_refine <= 4),

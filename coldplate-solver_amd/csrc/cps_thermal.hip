@@ -14,11 +14,14 @@
 // Solver: right-preconditioned BiCGStab with a geometric multigrid
 // preconditioner.  Levels aggregate 2 x 2 cells in x-y and keep every layer;
 // coarse operators are Galerkin P0 products (sums of fine couplings), so the
-// upwind M-matrix structure survives coarsening; smoother damped Jacobi
-// (0.7), 2 + 2 sweeps; the coarsest level is one x-y column, a tridiagonal
-// system in z solved exactly in FP64.  Precision: FP64 throughout, or FP32
-// BiCGStab + multigrid inside FP64 iterative refinement (true residual
-// b - A T in FP64).  The stop rule is the reference's relative one,
+// upwind M-matrix structure survives coarsening; the smoother is z-line
+// Jacobi (each x-y column's tridiagonal z system solved by Thomas with
+// factors stored at setup, x-y neighbours lagged, damping 0.8); the levels
+// from the first with at most 256 columns down to one column run in one
+// workgroup, the last level solved exactly in z.  Precision: FP64
+// throughout, or the multigrid V-cycle in FP32 inside FP64 BiCGStab (the
+// Krylov vectors, the 7-point products and the true residual b - A T stay
+// FP64).  The stop rule is the reference's relative one,
 // ||b - A T|| <= rtol ||b|| (AMGCL convention, thermal_solver.cpp:549).
 //
 // The adjoint solve uses the transposed stencil, built on the device; the

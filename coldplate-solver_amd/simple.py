@@ -261,7 +261,10 @@ class Batch:
 
     def gradients_dp(self, fields=None, rtol=1e-10, max_iter=4000, workers=8):
         """Gradients of several fields at once: one host thread per field in
-        flight, each field's adjoint on its own stream."""
+        flight.  Each field's adjoint runs on the stream its field was given
+        (cps_batch_tune spreads the fields over the batch's streams), so
+        fields on different streams overlap and fields sharing a stream run
+        one after another."""
         from concurrent.futures import ThreadPoolExecutor
 
         fields = list(range(self.n)) if fields is None else list(fields)

@@ -77,7 +77,12 @@ int cps_momentum_jacobi_csr(cps_ctx* ctx, int n, const int* row_ptr, const int* 
  * ------------------------------------------------------------------------- */
 #define CPS_PRESSURE_SOR 0    /* pressureSolverType 0: red-black SOR, max-change stop */
 #define CPS_PRESSURE_PCG 1    /* pressureSolverType 1: Jacobi-PCG, absolute tol */
-#define CPS_PRESSURE_GMG 5    /* new type: geometric-multigrid PCG on the device */
+#define CPS_PRESSURE_AMGCL 2  /* pressureSolverType 2: AMGCL CG, stop ||r|| <= p_tol ||b||; no AMGCL here, so it
+                                 falls back (as SIMPLE.cpp:147-186 does) to the GMG PCG with that stop rule */
+#define CPS_PRESSURE_LDLT 3   /* pressureSolverType 3: direct LDLT; falls back to the GMG PCG solved to 1e-12 ||b|| */
+#define CPS_PRESSURE_AMGCL_CUDA 4 /* pressureSolverType 4 (the reference default, SIMPLE.h:145): as type 2 */
+#define CPS_PRESSURE_GMG 5    /* new type: geometric-multigrid PCG on the device; the grid's pressure DOFs may
+                                 span at most 2048 tiles of 64 x 16 (e.g. 2048 x 1024 cells) */
 
 #define CPS_PREC_FP64 64      /* every kernel in FP64 */
 #define CPS_PREC_MIXED 32     /* FP32 inner Jacobi / MG-PCG, FP64 state, residuals, refinement */

@@ -10,22 +10,32 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = None
+_LIBS = {}
 
 
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib(types):
-    """Load liboracle.so; `types` is the package's _types module (shared
-    struct mirrors).  Builds it first when only the sources are present."""
-    global _LIB
-    if _LIB is not None:
-        return _LIB
-    path = os.path.join(_HERE, "liboracle.so")
-    if not os.path.exists(path):
-        build()
+def build_mutant(k, out_dir):
+    """Builds the oracle with mistake k planted (ORACLE_MUTANT=k, see the
+    header of simple_oracle.hpp) into out_dir; returns the library path."""
+    path = os.path.join(out_dir, f"liboracle_m{k}.so")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-fopenmp", "-shared", f"-DORACLE_MUTANT={k}", "-o", path,
+                    os.path.join(_HERE, "oracle_capi.cpp")], check=True)
+    return path
+
+
+def lib(types, path=None):
+    """Load liboracle.so (or the library at `path`, e.g. a mutant); `types`
+    is the package's _types module (shared struct mirrors).  Builds the
+    default library first when only the sources are present."""
+    if path is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+    if path in _LIBS:
+        return _LIBS[path]
     L = C.CDLL(path)
     P = C.POINTER
     dp = P(C.c_double)
@@ -58,7 +68,7 @@ def lib(types):
     L.orc_thermal_csr_size.argtypes = [C.c_void_p, ip, ip]
     L.orc_thermal_csr_get.argtypes = [C.c_void_p, ip, ip, dp, dp]
     L.orc_thermal_solve.argtypes = [C.c_void_p, dp, C.c_double, C.c_int, dp, dp]
-    _LIB = L
+    _LIBS[path] = L
     return L
 
 
@@ -73,9 +83,9 @@ def _i(a):
 class OracleSimple:
     """The reference SIMPLE solver restated on the CPU (prec 64 or 32)."""
 
-    def __init__(self, types, cfg, gamma, inlet=None, prec=64):
+    def __init__(self, types, cfg, gamma, inlet=None, prec=64, lib_path=None):
         self.T = types
-        self.L = lib(types)
+        self.L = lib(types, lib_path)
         self.cfg = cfg.copy()
         g = np.ascontiguousarray(gamma, dtype=np.float64)
         assert g.shape == (cfg.ny, cfg.nx)

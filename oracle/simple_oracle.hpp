@@ -33,6 +33,16 @@
 //     gamma > 0.5 inlet mask still applies on top of it;
 //   * a configurable pure-fluid tolerance for the boundary-patch dP.
 //
+// Mask decisions (half-cell wall flags, the inlet mask, the pure-fluid rows
+// of the dP samples) are taken in float arithmetic in both instantiations,
+// as the reference writes them on its float gamma (SIMPLE.h:479-520,
+// 740-778; boundaries.cpp:82-100,148-206; postprocessing.cpp:155-160,268),
+// so the FP64 oracle makes the reference's discrete choices bit for bit.
+//
+// ORACLE_MUTANT (default 0) plants one known mistake in the momentum
+// assembly, the Brinkman field or the inlet mask; the CPU tests build each
+// mutant and check that the closed-form pins of those terms reject it.
+//
 // Layout: row-major, element (i,j) of an R x C array at [i*C + j]; the
 // reference's Eigen arrays are column-major but index semantics are the same.
 // ============================================================================
@@ -46,7 +56,14 @@
 
 #include "../include/coldplate_hip.h"
 
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+
 namespace oracle {
+
+// the planted mistake of a mutant build (0 = the faithful oracle)
+constexpr int kMutant = ORACLE_MUTANT;
 
 #ifndef ORACLE_OMP_MIN_LOOP_DEFINED
 #define ORACLE_OMP_MIN_LOOP_DEFINED
@@ -199,7 +216,8 @@ class Simple {
     for (int i = 0; i < M; ++i)
       for (int j = 0; j < N; ++j) {
         R g = gamma[(size_t)i * N + j];
-        alpha[(size_t)i * N + j] = brinkmanAlphaMax * (R(1.0f) - g) / (R(1.0f) + brinkmanQ * g + R(1e-10f));
+        const R q = kMutant == 10 ? R(1.0f) : brinkmanQ;  // mutant 10: q ignored
+        alpha[(size_t)i * N + j] = brinkmanAlphaMax * (R(1.0f) - g) / (R(1.0f) + q * g + R(1e-10f));
       }
 
     hasInletProfile = inlet != nullptr;
@@ -220,14 +238,17 @@ class Simple {
     if (i < 1 || i >= M - 1 || j < 1 || j >= N) return R(0);
     return R(0.5f) * (A(i - 1, j - 1) + A(i, j - 1));
   }
-  R gammaAtU(int i, int j) const {
-    if (i < 1 || i >= M || j < 1 || j >= N - 1) return R(1);
-    return R(0.5f) * (G(i - 1, j - 1) + G(i - 1, j));
+  // float, as the reference evaluates it: these feed mask decisions only
+  float gammaAtU(int i, int j) const {
+    if (i < 1 || i >= M || j < 1 || j >= N - 1) return 1.0f;
+    return 0.5f * ((float)G(i - 1, j - 1) + (float)G(i - 1, j));
   }
-  R gammaAtV(int i, int j) const {
-    if (i < 1 || i >= M - 1 || j < 1 || j >= N) return R(1);
-    return R(0.5f) * (G(i - 1, j - 1) + G(i, j - 1));
+  float gammaAtV(int i, int j) const {
+    if (i < 1 || i >= M - 1 || j < 1 || j >= N) return 1.0f;
+    return 0.5f * ((float)G(i - 1, j - 1) + (float)G(i, j - 1));
   }
+  // pure-fluid test of the dP samples (postprocessing.cpp:155-160, 268)
+  bool pureFluid(int i, int j) const { return !((float)G(i, j) < 1.0f - (float)dpFluidTol); }
 
   // ------------------------------------------------------------------------
   // boundaries.cpp:50-118
@@ -245,11 +266,12 @@ class Simple {
   }
   bool inletFluidFromURow(int uRow) const {  // boundaries.cpp:82-96
     int cellRow;
-    if (uRow <= 0) cellRow = 0;
+    if (uRow <= 0) cellRow = kMutant == 9 ? 1 : 0;  // mutant 9: wrong corner row
     else if (uRow >= M) cellRow = std::max(0, M - 2);
     else cellRow = uRow - 1;
     cellRow = std::max(0, std::min(cellRow, M - 1));
-    return G(cellRow, 0) > R(0.5f);
+    if (kMutant == 8) return (float)G(cellRow, 0) >= 0.5f;  // mutant 8: >= for >
+    return (float)G(cellRow, 0) > 0.5f;
   }
   R inletSpeed(int uRow) const {
     // uniform reference inlet uses the (possibly ramped) inletVelocity
@@ -270,21 +292,21 @@ class Simple {
   bool southU(int i, int j) {
     if (i - 1 != 0) return false;
     if (j < 1 || j > N - 2) return false;
-    if (gammaAtU(i, j) < R(0.999f)) return false;
+    if (gammaAtU(i, j) < 0.999f) return false;
     const R t = R(1e-8f);
     return std::abs(U(0, j)) <= t && std::abs(V(0, j)) <= t && std::abs(V(0, j + 1)) <= t;
   }
   bool northU(int i, int j) {
     if (i + 1 != M) return false;
     if (j < 1 || j > N - 2) return false;
-    if (gammaAtU(i, j) < R(0.999f)) return false;
+    if (gammaAtU(i, j) < 0.999f) return false;
     const R t = R(1e-8f);
     return std::abs(U(M, j)) <= t && std::abs(V(M - 1, j)) <= t && std::abs(V(M - 1, j + 1)) <= t;
   }
   bool westV(int i, int j) {
     if (j - 1 != 0) return false;
     if (i < 1 || i > M - 2) return false;
-    if (gammaAtV(i, j) < R(0.999f)) return false;
+    if (gammaAtV(i, j) < 0.999f) return false;
     const R t = R(1e-8f);
     if (std::abs(targetVel) > t) return false;
     return std::abs(V(i, 0)) <= t && std::abs(U(i, 0)) <= t && std::abs(U(i + 1, 0)) <= t;
@@ -292,7 +314,7 @@ class Simple {
   bool eastV(int i, int j) {
     if (j + 1 != N) return false;
     if (i < 1 || i > M - 2) return false;
-    if (gammaAtV(i, j) < R(0.999f)) return false;
+    if (gammaAtV(i, j) < 0.999f) return false;
     const R t = R(1e-8f);
     if (std::abs(P(i, N)) <= t) return false;
     return std::abs(V(i, N)) <= t && std::abs(U(i, N - 1)) <= t && std::abs(U(i + 1, N - 1)) <= t;
@@ -419,10 +441,11 @@ class Simple {
     const auto& map = IsU ? mapU : mapV;
     const R vol = hx * hy;
     const R maxVel = R(3.0f) * std::max(std::abs(inletVelocity), R(0.1f));
-    const R sinkCoeff = enable25D ? sinkMult25D * (R(12.0f) * eta / (Ht * Ht)) : R(0);
-    const R sinkDiag = sinkCoeff * vol;
+    // mutants 1 (6 mu), 4 (no vol), 2 (factor dropped)
+    const R sinkCoeff = enable25D ? sinkMult25D * (R(kMutant == 1 ? 6.0f : 12.0f) * eta / (Ht * Ht)) : R(0);
+    const R sinkDiag = kMutant == 4 ? sinkCoeff : sinkCoeff * vol;
     const R DeScale = hy / hx, DnScale = hx / hy;
-    const R convScale = enable25D ? convFactor25D : R(1.0f);
+    const R convScale = enable25D && kMutant != 2 ? convFactor25D : R(1.0f);
     const int n = csr.n;
 #pragma omp parallel for schedule(static) if(n >= kOmpMinLoop)
     for (int row = 0; row < n; ++row) {
@@ -455,26 +478,27 @@ class Simple {
         if (eastV(i, j)) aE += De;
       }
       R sumA = aE + aW + aN + aS;
-      R brinkmanDrag = alphaLocal * vol;
+      R brinkmanDrag = kMutant == 3 ? alphaLocal : alphaLocal * vol;  // mutant 3: no vol
       R aP0 = sumA + (Fe - Fw + Fn - Fs) + R(0) + sinkDiag + brinkmanDrag;
       aP0 = std::max(aP0, R(1e-10f));
       R aP = aP0 / uvAlpha;
       R Sp, velCur;
       if (IsU) {
-        Sp = (P(i, j) - P(i, j + 1)) * hy;
+        Sp = (P(i, j) - P(i, j + 1)) * (kMutant == 6 ? hx : hy);  // mutant 6: wrong face
         velCur = U(i, j);
       } else {
         Sp = (P(i, j) - P(i + 1, j)) * hx;
         velCur = V(i, j);
       }
-      Sp += (R(1.0f) - uvAlpha) / uvAlpha * aP0 * velCur;
+      if (kMutant == 5) Sp += (R(1.0f) - uvAlpha) * aP0 * velCur;  // mutant 5: relaxation
+      else Sp += (R(1.0f) - uvAlpha) / uvAlpha * aP0 * velCur;
       if (convectionScheme == 1) Sp += IsU ? souU(i, j, Fe, Fw, Fn, Fs) : souV(i, j, Fe, Fw, Fn, Fs);
-      if (IsU) dE[(size_t)i * N + j] = hy / aP0;
+      if (IsU) dE[(size_t)i * N + j] = (kMutant == 11 ? hx : hy) / aP0;  // mutant 11: wrong face
       else dN[(size_t)i * (N + 1) + j] = hx / aP0;
       csr.values[csr.diag[row]] = aP;
       auto upd = [&](int off, R coeff, R nb) {
         if (off >= 0) csr.values[off] = -coeff;
-        else Sp += coeff * nb;
+        else if (kMutant != 7) Sp += coeff * nb;  // mutant 7: boundary values not folded
       };
       if (IsU) {
         upd(csr.east[row], aE, U(i, j + 1));
@@ -786,7 +810,7 @@ class Simple {
     for (int i = 1; i < M; ++i) {
       const int cr = i - 1;
       if (cr < 0 || cr >= MCells) continue;
-      if (G(cr, adj) < R(1.0f) - dpFluidTol) continue;
+      if (!pureFluid(cr, adj)) continue;
       const R fa = hy;
       area += fa;
       const int vTop = std::max(0, std::min(i - 1, M - 1));
@@ -840,7 +864,7 @@ class Simple {
     R sA = 0, dA = 0, tw = 0, area = 0, flux = 0;
     for (int i = 1; i < M; ++i) {
       const int row = i - 1;
-      if (G(row, cl) < R(1.0f) - dpFluidTol || G(row, cr) < R(1.0f) - dpFluidTol) continue;
+      if (!pureFluid(row, cl) || !pureFluid(row, cr)) continue;
       const R fa = hy;
       area += fa;
       const R sp = w * P(i, cl + 1) + t * P(i, cr + 1);

@@ -41,9 +41,9 @@ def main():
     k = ramp(gamma)
     T = 30.0 + 50.0 * rng.uniform(size=(ny, nx))
     P = 1e3 * rng.uniform(size=(ny, nx))
-    os.makedirs(os.path.join(REPO, "_work"), exist_ok=True)
     cwd = os.getcwd()
-    with tempfile.TemporaryDirectory(dir=os.path.join(REPO, "_work")) as tmp:
+    # the script writes its outputs into the working directory
+    with tempfile.TemporaryDirectory() as tmp:
         os.chdir(tmp)
         try:
             sens_ext = mod.solve_thermal_adjoint(T_in=T.copy(), k_in=k.copy(), P_in=P.copy())

@@ -55,7 +55,8 @@ def test_gamma_padding_copies_last_row_and_column(pkg, orc, T):
 def test_face_averages_midpoint_and_out_of_range(pkg, orc, T):
     """Face values are midpoints of the two adjacent cells (exact for a
     field linear in the index); outside the DOF ranges alpha -> 0 and
-    gamma -> 1 (SIMPLE.h:642-686, 740-778)."""
+    gamma -> 1 (SIMPLE.h:642-686, 740-778).  gamma faces feed only mask
+    decisions and are formed in float, as the reference's float gamma is."""
     cfg = small_cfg(pkg, nx=10, ny=6)
     M, N = cfg.ny + 1, cfg.nx + 1
     jj = np.arange(cfg.nx) / 16.0
@@ -66,10 +67,10 @@ def test_face_averages_midpoint_and_out_of_range(pkg, orc, T):
     G = o.get("gamma")
     for i in range(1, M):
         for j in range(1, N - 1):  # U faces sit between cells (i-1, j-1) and (i-1, j)
-            assert o.face("gammaAtU", i, j) == 0.5 * (G[i - 1, j - 1] + G[i - 1, j])
+            assert o.face("gammaAtU", i, j) == float(np.float32(0.5) * (np.float32(G[i - 1, j - 1]) + np.float32(G[i - 1, j])))
     for i in range(1, M - 1):
         for j in range(1, N):  # V faces between cells (i-1, j-1) and (i, j-1)
-            assert o.face("gammaAtV", i, j) == 0.5 * (G[i - 1, j - 1] + G[i, j - 1])
+            assert o.face("gammaAtV", i, j) == float(np.float32(0.5) * (np.float32(G[i - 1, j - 1]) + np.float32(G[i, j - 1])))
     for (i, j) in ((0, 3), (M, 3), (2, 0), (2, N - 1)):
         assert o.face("alphaAtU", i, j) == 0.0 and o.face("gammaAtU", i, j) == 1.0
     for (i, j) in ((0, 3), (M - 1, 3), (2, 0), (2, N)):
