@@ -52,7 +52,7 @@ def lib():
         L.cps_destroy.argtypes = [vp]
         L.cps_destroy.restype = None
         L.cps_synchronize.argtypes = [vp]
-        for f in (L.cps_pressure_solve_csr, L.cps_momentum_jacobi_csr):
+        for f in (L.cps_pressure_solve_csr, L.cps_momentum_jacobi_csr, L.cps_pressure_solve_csr_noalloc):
             f.argtypes = [vp, C.c_int, ip, ip, fp, fp, fp, C.c_float, C.c_int, ip]
         L.cps_simple_config_default.argtypes = [P(T.SimpleConfig)]
         L.cps_simple_config_default.restype = None

@@ -27,6 +27,7 @@ int cps_create(int device, cps_ctx** out) {
     CPS_CHECK(n > 0, CPS_ERR_HIP, "no HIP device");
     CPS_CHECK(device >= 0 && device < n, CPS_ERR_ARG, "device index out of range");
     CPS_HIP(hipSetDevice(device));
+    cps::csr_scratch_init(device);
     auto* c = new cps_ctx;
     c->c.device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking);

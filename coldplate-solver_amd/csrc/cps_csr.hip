@@ -8,6 +8,7 @@
 //
 // Vector arithmetic is FP32 like the reference; dot products and residual
 // sums accumulate in FP64 (wave shuffle + fixed-order block partials).
+#include <mutex>
 #include <vector>
 
 #include "cps_simple.hpp"
@@ -245,6 +246,115 @@ static CsrWork& work(cps_ctx* ctx) {
 
 void csr_free(void* p) { delete static_cast<CsrWork*>(p); }
 
+// Device buffers of one Jacobi-PCG solve.
+struct PcgView {
+  int *rp, *ci;
+  float *val, *b, *x, *r, *z, *pa, *pb, *Ap, *diag;
+  double* part;
+  CsrState* st;
+};
+
+static void upload_view(hipStream_t s, const PcgView& w, int n, const int* rp, const int* ci, const float* val,
+                        const float* b, const float* x) {
+  const int nnz = rp[n];
+  CPS_HIP(hipMemcpyAsync(w.rp, rp, (n + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+  if (nnz > 0) {
+    CPS_HIP(hipMemcpyAsync(w.ci, ci, nnz * sizeof(int), hipMemcpyHostToDevice, s));
+    CPS_HIP(hipMemcpyAsync(w.val, val, nnz * sizeof(float), hipMemcpyHostToDevice, s));
+  }
+  CPS_HIP(hipMemcpyAsync(w.b, b, n * sizeof(float), hipMemcpyHostToDevice, s));
+  CPS_HIP(hipMemcpyAsync(w.x, x, n * sizeof(float), hipMemcpyHostToDevice, s));
+}
+
+// solvePCG_CSR (pressure_solver.cpp:121-184) on uploaded buffers: the start
+// kernel writes every vector and the state the loop reads, then iterations
+// run in chunks of 16 with one stop-flag read per chunk.
+static void pcg_run(hipStream_t s, const PcgView& w, int n, float tolerance, int max_iterations, float* solution,
+                    int* iterations) {
+  CsrDev A{n, w.rp, w.ci, w.val};
+  const int nb = blocks(n);
+  k_csr_pcg_init<<<nb, CT, 0, s>>>(A, w.b, w.x, w.diag, w.r, w.z, w.pa, w.pb, w.st, w.part);
+  CPS_HIP(hipGetLastError());
+  const int chunk = 16;
+  for (int k0 = 0; k0 < max_iterations; k0 += chunk) {
+    const int k1 = std::min(max_iterations, k0 + chunk);
+    for (int k = k0; k < k1; ++k) {
+      k_csr_pcg_a<<<nb, CT, 0, s>>>(A, w.z, w.pa, w.pb, w.Ap, k, tolerance, w.st, w.part);
+      k_csr_pcg_b<<<nb, CT, 0, s>>>(A, w.x, w.r, w.z, w.diag, w.pa, w.pb, w.Ap, k, w.st, w.part);
+    }
+    k_csr_pcg_end<<<1, CT, 0, s>>>(nb, k1, tolerance, w.st, w.part);
+    CPS_HIP(hipGetLastError());
+    if (k1 >= max_iterations) break;
+    int stop = 0;
+    CPS_HIP(hipMemcpyAsync(&stop, &w.st->stop, sizeof(int), hipMemcpyDeviceToHost, s));
+    CPS_HIP(hipStreamSynchronize(s));
+    if (stop) break;
+  }
+  CsrState h{};
+  CPS_HIP(hipMemcpyAsync(&h, w.st, sizeof(CsrState), hipMemcpyDeviceToHost, s));
+  CPS_HIP(hipMemcpyAsync(solution, w.x, n * sizeof(float), hipMemcpyDeviceToHost, s));
+  CPS_HIP(hipStreamSynchronize(s));
+  if (iterations) *iterations = h.iters;
+}
+
+// ---------------------------------------------------------------------------
+// Buffers of cps_pressure_solve_csr_noalloc: one file-scope device array,
+// the same on every call on a GPU (each GPU holds its own copy until the
+// process ends).  Layout, each piece 256-B aligned: row_ptr, col_idx,
+// values, then the vectors b, x, r, z, p (two), A p and the diagonal, the
+// block partials and the solver state.  Limits: 2^19 rows (the pressure
+// system of a 1024 x 512 grid has 523,776) and 5 * 2^19 nonzeros.
+constexpr int kNoAllocMaxRows = 1 << 19;
+constexpr long long kNoAllocMaxNnz = 5LL << 19;
+constexpr int kNoAllocMaxBlocks = 1024;  // blocks(n) never exceeds it
+constexpr size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kScrCi = al256((size_t)(kNoAllocMaxRows + 1) * sizeof(int));
+constexpr size_t kScrVal = kScrCi + al256((size_t)kNoAllocMaxNnz * sizeof(int));
+constexpr size_t kScrVec = kScrVal + al256((size_t)kNoAllocMaxNnz * sizeof(float));
+constexpr size_t kScrVecBytes = al256((size_t)kNoAllocMaxRows * sizeof(float));
+constexpr size_t kScrPart = kScrVec + 8 * kScrVecBytes;
+constexpr size_t kScrState = kScrPart + al256((size_t)C_COUNT * kNoAllocMaxBlocks * sizeof(double));
+constexpr size_t kScratchBytes = kScrState + al256(sizeof(CsrState));
+constexpr int kScratchDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on each GPU, looked up once per device by
+// cps_create (with that device current).  The host-side name `scratch` is a
+// stand-in, not a device address: only these pointers reach kernels and
+// copies.
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kScratchDevices] = {};
+
+void csr_scratch_init(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  CPS_CHECK(device >= 0 && device < kScratchDevices, CPS_ERR_ARG, "device index beyond the scratch table");
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  CPS_CHECK(e == hipSuccess && p != nullptr, CPS_ERR_HIP,
+            std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+}
+
+static PcgView scratch_view(int device) {
+  unsigned char* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = device >= 0 && device < kScratchDevices ? g_scratch_ptr[device] : nullptr;
+  }
+  CPS_CHECK(base != nullptr, CPS_ERR_STATE, "scratch of this device was not set up (cps_create)");
+  PcgView w;
+  w.rp = reinterpret_cast<int*>(base);
+  w.ci = reinterpret_cast<int*>(base + kScrCi);
+  w.val = reinterpret_cast<float*>(base + kScrVal);
+  float** slots[8] = {&w.b, &w.x, &w.r, &w.z, &w.pa, &w.pb, &w.Ap, &w.diag};
+  for (int k = 0; k < 8; ++k) *slots[k] = reinterpret_cast<float*>(base + kScrVec + (size_t)k * kScrVecBytes);
+  w.part = reinterpret_cast<double*>(base + kScrPart);
+  w.st = reinterpret_cast<CsrState*>(base + kScrState);
+  return w;
+}
+
 }  // namespace cps
 
 extern "C" {
@@ -263,31 +373,33 @@ int cps_pressure_solve_csr(cps_ctx* ctx, int n, const int* row_ptr, const int* c
     hipStream_t s = ctx->c.stream;
     CsrWork& w = work(ctx);
     upload(s, w, n, row_ptr, col_idx, values, rhs, solution);
-    CsrDev A{n, w.rp.p, w.ci.p, w.val.p};
-    const int nb = blocks(n);
-    k_csr_pcg_init<<<nb, CT, 0, s>>>(A, w.b.p, w.x.p, w.diag.p, w.r.p, w.z.p, w.pa.p, w.pb.p, w.st.p, w.part.p);
-    CPS_HIP(hipGetLastError());
-    const int chunk = 16;
-    for (int k0 = 0; k0 < max_iterations; k0 += chunk) {
-      const int k1 = std::min(max_iterations, k0 + chunk);
-      for (int k = k0; k < k1; ++k) {
-        k_csr_pcg_a<<<nb, CT, 0, s>>>(A, w.z.p, w.pa.p, w.pb.p, w.Ap.p, k, tolerance, w.st.p, w.part.p);
-        k_csr_pcg_b<<<nb, CT, 0, s>>>(A, w.x.p, w.r.p, w.z.p, w.diag.p, w.pa.p, w.pb.p, w.Ap.p, k, w.st.p,
-                                      w.part.p);
-      }
-      k_csr_pcg_end<<<1, CT, 0, s>>>(nb, k1, tolerance, w.st.p, w.part.p);
-      CPS_HIP(hipGetLastError());
-      if (k1 >= max_iterations) break;
-      int stop = 0;
-      CPS_HIP(hipMemcpyAsync(&stop, &w.st.p->stop, sizeof(int), hipMemcpyDeviceToHost, s));
-      CPS_HIP(hipStreamSynchronize(s));
-      if (stop) break;
-    }
-    CsrState h{};
-    CPS_HIP(hipMemcpyAsync(&h, w.st.p, sizeof(CsrState), hipMemcpyDeviceToHost, s));
-    CPS_HIP(hipMemcpyAsync(solution, w.x.p, n * sizeof(float), hipMemcpyDeviceToHost, s));
-    CPS_HIP(hipStreamSynchronize(s));
-    if (iterations) *iterations = h.iters;
+    const PcgView v{w.rp.p, w.ci.p, w.val.p, w.b.p, w.x.p, w.r.p, w.z.p, w.pa.p, w.pb.p, w.Ap.p, w.diag.p,
+                    w.part.p, w.st.p};
+    pcg_run(s, v, n, tolerance, max_iterations, solution, iterations);
+  });
+}
+
+int cps_pressure_solve_csr_noalloc(cps_ctx* ctx, int n, const int* row_ptr, const int* col_idx,
+                                   const float* values, const float* rhs, float* solution, float tolerance,
+                                   int max_iterations, int* iterations) {
+  using namespace cps;
+  return guard([&] {
+    CPS_CHECK(ctx && row_ptr && rhs && solution, CPS_ERR_ARG, "null argument");
+    CPS_CHECK(n >= 0 && max_iterations >= 0, CPS_ERR_ARG, "negative size");
+    if (iterations) *iterations = 0;
+    if (n == 0) return;  // pressure_solver.cpp:124
+    CPS_CHECK(n <= kNoAllocMaxRows, CPS_ERR_ARG,
+              "cps_pressure_solve_csr_noalloc: n exceeds its scratch limit of 524288 rows");
+    CPS_CHECK(row_ptr[0] == 0 && row_ptr[n] >= 0, CPS_ERR_ARG, "row_ptr must start at 0 and be non-decreasing");
+    CPS_CHECK((long long)row_ptr[n] <= kNoAllocMaxNnz, CPS_ERR_ARG,
+              "cps_pressure_solve_csr_noalloc: nnz exceeds its scratch limit of 2621440 entries");
+    CPS_CHECK(row_ptr[n] == 0 || (col_idx && values), CPS_ERR_ARG, "null CSR arrays");
+    validate_csr(n, row_ptr, col_idx);
+    CPS_HIP(hipSetDevice(ctx->c.device));
+    hipStream_t s = ctx->c.stream;
+    const PcgView v = scratch_view(ctx->c.device);
+    upload_view(s, v, n, row_ptr, col_idx, values, rhs, solution);
+    pcg_run(s, v, n, tolerance, max_iterations, solution, iterations);
   });
 }
 

@@ -143,6 +143,9 @@ void flow_adjoint_options(FlowAdjoint* a, int kind, int sweeps, double omega, in
 
 namespace cps {
 void csr_free(void* p);
+// looks up the device address of the allocation-free drop-in's scratch on
+// `device` (current) once per process (cps_csr.hip)
+void csr_scratch_init(int device);
 }
 
 struct cps_ctx {

@@ -306,6 +306,19 @@ def pressure_solve_csr(row_ptr, col_idx, values, rhs, solution, tolerance, max_i
     return it.value
 
 
+def pressure_solve_csr_noalloc(row_ptr, col_idx, values, rhs, solution, tolerance, max_iterations, device=0):
+    """pressure_solve_csr without device allocation (cps_pressure_solve_csr_noalloc:
+    static per-GPU buffers, n <= 524288, nnz <= 2621440; calls overlapping
+    on one GPU race).  Returns the iteration count."""
+    ctx = _capi.context(device)
+    n, rp, ci, val, b = _csr_args(row_ptr, col_idx, values, rhs, solution)
+    it = C.c_int()
+    _capi.check(ctx.L.cps_pressure_solve_csr_noalloc(ctx.h, n, _capi.iptr(rp), _capi.iptr(ci), _capi.fptr(val),
+                                                     _capi.fptr(b), _capi.fptr(solution), tolerance,
+                                                     max_iterations, C.byref(it)))
+    return it.value
+
+
 def momentum_jacobi_csr(row_ptr, col_idx, values, rhs, solution, tolerance, max_iterations, device=0):
     """Drop-in for amgcl_cuda::solveMomentumU/V (cuda_solver.hpp:36-57) with
     the stop rule of solvers::cpu::solveMomentumJacobi (jacobi_solver.cpp:

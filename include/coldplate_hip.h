@@ -70,6 +70,20 @@ int cps_pressure_solve_csr(cps_ctx* ctx, int n, const int* row_ptr, const int* c
 int cps_momentum_jacobi_csr(cps_ctx* ctx, int n, const int* row_ptr, const int* col_idx, const float* values,
                             const float* rhs, float* solution, float tolerance, int max_iterations,
                             int* iterations);
+/* cps_pressure_solve_csr without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * per-iteration loops such as the reference's own call site,
+ * pressure_solver.cpp:628, serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, semantics and results as cps_pressure_solve_csr; its
+ * buffers (the uploaded CSR, the PCG vectors, partials and state) live in one
+ * static device array per GPU (about 40 MB, held until the process ends),
+ * whose address cps_create looks up.  Limits: n <= 524288 rows and
+ * row_ptr[n] <= 2621440 nonzeros, else CPS_ERR_ARG.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (from several
+ * threads or contexts) race. */
+int cps_pressure_solve_csr_noalloc(cps_ctx* ctx, int n, const int* row_ptr, const int* col_idx,
+                                   const float* values, const float* rhs, float* solution, float tolerance,
+                                   int max_iterations, int* iterations);
 
 /* ---------------------------------------------------------------------------
  * Structured SIMPLE solver (device-resident fields).  Knobs carry the
