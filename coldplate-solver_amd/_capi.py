@@ -66,6 +66,8 @@ def lib():
         L.cps_simple_boundary_patch.argtypes = [vp, C.c_int, P(T.Patch)]
         L.cps_simple_plane.argtypes = [vp, C.c_double, P(T.Patch)]
         L.cps_simple_run_logged.argtypes = [vp, P(T.StepStats), P(T.IterLog), C.c_int, P(C.c_int)]
+        L.cps_simple_run_polled.argtypes = [vp, C.c_int, P(T.StepStats), P(T.IterLog), C.c_int, P(C.c_int),
+                                            P(C.c_int)]
         L.cps_simple_time_kernel.argtypes = [vp, C.c_int]
         L.cps_simple_kernel_timing.argtypes = [vp, ip, dp, dp]
         L.cps_simple_gradient_dp.argtypes = [vp, C.c_double, C.c_int, dp, dp, P(T.SolveStats)]

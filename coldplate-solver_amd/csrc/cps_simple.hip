@@ -1039,13 +1039,140 @@ __global__ void k_inlet_scale(int n, double s, const double* __restrict__ inlet0
 }
 
 // Averages of one sample from its k_sample sums (postprocessing.cpp:226-240).
-static void finish_sample(const double* h, cps_patch* out) {
+__host__ __device__ inline void finish_sample(const double* h, cps_patch* out) {
   out->area = h[0];
   out->mass_flux = h[1];
   out->avg_static = h[0] > 0 ? h[2] / h[0] : 0.0;
   out->avg_dynamic = h[0] > 0 ? h[3] / h[0] : 0.0;
   out->valid = h[0] > 0;
   out->avg_total = h[1] > 0 ? h[4] / h[1] : (h[0] > 0 ? out->avg_static + out->avg_dynamic : 0.0);
+}
+
+// The statistics the host reads after an iteration (iterations.cpp:628-658,
+// SIMPLE.cpp:421-439), from the step's IterState.
+__host__ __device__ inline void fill_stats(const IterState& h, int iteration, cps_step_stats* out) {
+  out->error = h.error;
+  out->resid_u = h.resid_u;
+  out->resid_v = h.resid_v;
+  out->resid_mass = h.resid_mass;
+  out->resid_u_max = h.umax;
+  out->resid_v_max = h.vmax;
+  out->resid_mass_max = h.mass_max;
+  out->u_iters = h.jac_iters[0];
+  out->v_iters = h.jac_iters[1];
+  out->p_iters = h.pcg_iters;
+  out->iteration = iteration;
+  cps_patch m[4];
+  for (int k = 0; k < 4; ++k) finish_sample(h.mon[k], &m[k]);
+  out->dp_core_total = m[0].avg_total - m[1].avg_total;
+  out->dp_core_static = m[0].avg_static - m[1].avg_static;
+  out->dp_full_total = m[2].avg_total - m[3].avg_total;
+  out->dp_full_static = m[2].avg_static - m[3].avg_static;
+  out->dp_converged = 0;
+}
+
+// Outer-loop state of runIterations kept on the device (SIMPLE.cpp:378-491).
+struct RunState {
+  int stopped;     // the loop condition failed after iteration stop_it - 1
+  int stop_it;     // iterations the loop runs
+  int conv;        // stopped on the settled dP window
+  int nonfinite;   // the residual was not finite
+  int idx, filled; // window ring position, window full
+  int sn, spos;    // slope-gate entries held, next position
+  cps_step_stats last;  // statistics of iteration stop_it - 1
+};
+struct RunParams {
+  int max_iterations, ramp_steps, use_dp, win, use_slope, slope_win, log_cap, pad_;
+  double epsilon, dp_percent, slope_max_delta;
+};
+
+__global__ void k_run_reset(RunState* rs) {
+  if (threadIdx.x == 0) {
+    RunState z{};
+    *rs = z;
+  }
+}
+
+// One thread, after iteration `it` (its k_sample and k_stats are done): the
+// log row, the slope-gate ring, the dP window and the loop condition of
+// SIMPLE.cpp:378, in the host loop's operation order (no contraction, sums in
+// the same order), so the stop iteration is the host loop's.  Nothing
+// changes once stopped.
+__global__ void k_run_monitor(int it, int iteration, RunParams rp, const IterState* st, RunState* rs, double* hist,
+                              double* slx, double* sly, cps_iter_log* log) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0 || rs->stopped) return;
+  cps_step_stats sst;
+  fill_stats(*st, iteration, &sst);
+  const double core = sst.dp_core_total;
+  bool stop = false, conv = false;
+  if (!isfinite(sst.error)) {
+    rs->nonfinite = 1;
+    stop = true;
+  } else {
+    if (log && it < rp.log_cap) {
+      cps_iter_log& r = log[it];
+      r.resid_mass = sst.resid_mass;
+      r.resid_u = sst.resid_u;
+      r.resid_v = sst.resid_v;
+      r.dp_core_total = sst.dp_core_total;
+      r.dp_full_total = sst.dp_full_total;
+      r.dp_core_static = sst.dp_core_static;
+      r.dp_full_static = sst.dp_full_static;
+      r.p_iters = sst.p_iters;
+      r.pad_ = 0;
+    }
+    if (rp.use_slope && rp.slope_win > 0) {
+      slx[rs->spos] = (double)it;
+      sly[rs->spos] = core;
+      rs->spos = (rs->spos + 1) % rp.slope_win;
+      if (rs->sn < rp.slope_win) rs->sn++;
+    }
+    if (rp.use_dp && it >= rp.ramp_steps) {
+      const int win = rp.win;
+      hist[rs->idx] = core;
+      rs->idx = (rs->idx + 1) % win;
+      if (rs->idx == 0) rs->filled = 1;
+      if (rs->filled) {
+        // SIMPLE.cpp:456-490
+        double sum = 0.0;
+        for (int k = 0; k < win; ++k) sum += hist[k];
+        const double mean = sum / win;
+        double var = 0.0;
+        for (int k = 0; k < win; ++k) var += (hist[k] - mean) * (hist[k] - mean);
+        const double den = fmax(fabs(mean), 1.0);
+        const double sdPct = sqrt(var / win) / den * 100.0;
+        const double lastPct = fabs(hist[(rs->idx - 1 + win) % win] - mean) / den * 100.0;
+        bool slopeOk = true;
+        if (rp.use_slope && rp.slope_win > 0 && rs->sn >= rp.slope_win) {
+          // least-squares slope over the ring in time order (SIMPLE.cpp:347-366)
+          const int n0 = rs->sn, first = (rs->spos - n0 + rp.slope_win) % rp.slope_win;
+          const double x0 = slx[first], n = (double)n0;
+          double sx = 0, sy = 0, sxx = 0, sxy = 0;
+          for (int k = 0; k < n0; ++k) {
+            const int q = (first + k) % rp.slope_win;
+            const double x = slx[q] - x0;
+            sx += x;
+            sy += sly[q];
+            sxx += x * x;
+            sxy += x * sly[q];
+          }
+          const double d = n * sxx - sx * sx;
+          const double sl = fabs(d) < (double)1e-12f ? 0.0 : (n * sxy - sx * sy) / d;
+          slopeOk = fabs(sl) <= rp.slope_max_delta / rp.slope_win;
+        }
+        conv = sdPct < rp.dp_percent && lastPct < rp.dp_percent && slopeOk;
+      }
+    }
+    stop = it + 1 >= rp.max_iterations || !(sst.error > rp.epsilon) || conv;
+  }
+  if (stop) {
+    sst.dp_converged = conv ? 1 : 0;
+    rs->conv = conv ? 1 : 0;
+    rs->last = sst;
+    rs->stop_it = it + 1;
+    rs->stopped = 1;
+  }
 }
 
 // ===========================================================================
@@ -1435,24 +1562,15 @@ class SimpleSolver {
     CPS_HIP(hipMemcpyAsync(&h, st.p, sizeof(IterState), hipMemcpyDeviceToHost, ctx->stream));
     CPS_HIP(hipStreamSynchronize(ctx->stream));
     if (!out) return;
-    out->error = h.error;
-    out->resid_u = h.resid_u;
-    out->resid_v = h.resid_v;
-    out->resid_mass = h.resid_mass;
-    out->resid_u_max = h.umax;
-    out->resid_v_max = h.vmax;
-    out->resid_mass_max = h.mass_max;
-    out->u_iters = h.jac_iters[0];
-    out->v_iters = h.jac_iters[1];
-    out->p_iters = h.pcg_iters;
-    out->iteration = iteration;
-    cps_patch m[4];
-    for (int k = 0; k < 4; ++k) finish_sample(h.mon[k], &m[k]);
-    out->dp_core_total = m[0].avg_total - m[1].avg_total;
-    out->dp_core_static = m[0].avg_static - m[1].avg_static;
-    out->dp_full_total = m[2].avg_total - m[3].avg_total;
-    out->dp_full_static = m[2].avg_static - m[3].avg_static;
-    out->dp_converged = 0;
+    fill_stats(h, iteration, out);
+  }
+  // the statistics kernels of stats() without the read-back
+  void stats_launch() {
+    if (cfg.pressure_solver == CPS_PRESSURE_GMG) {
+      k_stats<<<1, block, 0, ctx->stream>>>(g, st.p, part.p, 0);
+      CPS_HIP(hipGetLastError());
+      if (mg) mg->finish(st.p);
+    }
   }
 
   void step(int n, cps_step_stats* out) {
@@ -1465,86 +1583,121 @@ class SimpleSolver {
     CPS_CHECK(std::isfinite(e), CPS_ERR_DIVERGED, "SIMPLE residual is not finite");
   }
 
-  // SIMPLE.cpp:310-491 without logging, checkpoints and the pseudo-time
-  // controller.  The four drops are reduced on the device after every
-  // iteration and read back with the residuals; the window is host state.
-  int run(cps_step_stats* out, cps_iter_log* log = nullptr, int log_cap = 0) {
+  // SIMPLE.cpp:310-491 without console output, checkpoints and the
+  // pseudo-time controller, with the loop's services on the device: after
+  // every iteration k_run_monitor writes the log row, updates the dP window
+  // and the slope gate and evaluates the loop condition.  The host enqueues
+  // `poll` iterations, then reads the monitor (one synchronisation per
+  // chunk).  A chunk that ran past the stop iteration is undone: u, v, p,
+  // the inlet ramp and the iteration count are restored from the chunk's
+  // start and the iterations up to the stop replayed, which repeats them bit
+  // for bit (every kernel is deterministic).  Returns the iterations run.
+  DevBuf<double> snap[3];
+  DevBuf<RunState> runst;
+  DevBuf<double> hist, slx, sly;
+  DevBuf<cps_iter_log> dlog;
+  int host_syncs = 0;
+
+  static double ramp_of(const cps_simple_config& c, int it) {
+    return c.inlet_ramp && c.ramp_steps > 0 && it < c.ramp_steps ? double(it + 1) / double(c.ramp_steps) : 1.0;
+  }
+
+  int run(cps_step_stats* out, cps_iter_log* log = nullptr, int log_cap = 0, int poll = kDefaultPoll) {
+    CPS_CHECK(poll >= 1, CPS_ERR_ARG, "poll interval must be >= 1");
+    host_syncs = 0;
     apply_bc();
     cps_step_stats sst{};
     sst.error = 1.0;
+    if (!(cfg.max_iterations > 0 && sst.error > cfg.epsilon)) {  // the loop body never runs
+      if (out) *out = sst;
+      return 0;
+    }
+    const size_t nf[3] = {u.n, v.n, p.n};
+    for (int k = 0; k < 3; ++k)
+      if (snap[k].n != nf[k]) snap[k].alloc(nf[k]);
+    if (!runst.p) runst.alloc(1);
     const int win = std::max(1, cfg.dp_window);
-    std::vector<double> hist(win, 0.0);
-    int idx = 0;
-    bool filled = false, conv = false;
-    std::deque<std::pair<int, double>> slope;
-    int it = 0;
-    while (it < cfg.max_iterations && sst.error > cfg.epsilon && !conv) {
-      set_inlet_scale(cfg.inlet_ramp && cfg.ramp_steps > 0 && it < cfg.ramp_steps
-                          ? double(it + 1) / double(cfg.ramp_steps)
-                          : 1.0);
-      step_once();
-      sample();
-      stats(&sst);
-      CPS_CHECK(std::isfinite(sst.error), CPS_ERR_DIVERGED, "SIMPLE residual is not finite");
-      const double core = sst.dp_core_total;
-      if (log && it < log_cap) {
-        cps_iter_log& r = log[it];
-        r.resid_mass = sst.resid_mass;
-        r.resid_u = sst.resid_u;
-        r.resid_v = sst.resid_v;
-        r.dp_core_total = sst.dp_core_total;
-        r.dp_full_total = sst.dp_full_total;
-        r.dp_core_static = sst.dp_core_static;
-        r.dp_full_static = sst.dp_full_static;
-        r.p_iters = sst.p_iters;
-        r.pad_ = 0;
+    const int swin = cfg.use_dp_slope_gate ? std::max(0, cfg.dp_slope_window) : 0;
+    if ((int)hist.n < win) hist.alloc(win);
+    if ((int)slx.n < std::max(1, swin)) { slx.alloc(std::max(1, swin)); sly.alloc(std::max(1, swin)); }
+    const int lcap = log ? std::min(log_cap, cfg.max_iterations) : 0;
+    if ((int)dlog.n < lcap) dlog.alloc(lcap);
+    RunParams rp{};
+    rp.max_iterations = cfg.max_iterations;
+    rp.ramp_steps = cfg.ramp_steps;
+    rp.use_dp = cfg.use_dp_convergence;
+    rp.win = win;
+    rp.use_slope = cfg.use_dp_slope_gate;
+    rp.slope_win = swin;
+    rp.log_cap = lcap;
+    rp.epsilon = cfg.epsilon;
+    rp.dp_percent = cfg.dp_percent;
+    rp.slope_max_delta = cfg.dp_slope_max_delta;
+    hipStream_t s = ctx->stream;
+    k_run_reset<<<1, 64, 0, s>>>(runst.p);
+    CPS_HIP(hipGetLastError());
+    DevBuf<double>* fld[3] = {&u, &v, &p};
+    RunState h{};
+    int k0 = 0;
+    for (;;) {
+      // chunk start: what the next iterations read from earlier ones
+      for (int k = 0; k < 3; ++k)
+        CPS_HIP(hipMemcpyAsync(snap[k].p, fld[k]->p, fld[k]->bytes(), hipMemcpyDeviceToDevice, s));
+      const double scale0 = inlet_scale;
+      const int iter0 = iteration;
+      const int k1 = std::min(cfg.max_iterations, k0 + poll);
+      for (int it = k0; it < k1; ++it) {
+        set_inlet_scale(ramp_of(cfg, it));
+        step_once();
+        sample();
+        stats_launch();
+        k_run_monitor<<<1, 64, 0, s>>>(it, iteration, rp, st.p, runst.p, hist.p, slx.p, sly.p,
+                                       lcap > 0 ? dlog.p : nullptr);
+        CPS_HIP(hipGetLastError());
       }
-      if (cfg.use_dp_slope_gate && cfg.dp_slope_window > 0) {
-        slope.emplace_back(it, core);
-        if (slope.size() > (size_t)cfg.dp_slope_window) slope.pop_front();
+      CPS_HIP(hipMemcpyAsync(&h, runst.p, sizeof(RunState), hipMemcpyDeviceToHost, s));
+      CPS_HIP(hipStreamSynchronize(s));
+      host_syncs++;
+      if (h.stopped) {
+        if (h.stop_it < k1) {  // ran past the stop: restore the chunk start, replay to the stop
+          for (int k = 0; k < 3; ++k)
+            CPS_HIP(hipMemcpyAsync(fld[k]->p, snap[k].p, fld[k]->bytes(), hipMemcpyDeviceToDevice, s));
+          restore_inlet_scale(scale0);
+          iteration = iter0;
+          for (int it = k0; it < h.stop_it; ++it) {  // the same launches as the first pass, minus the monitor
+            set_inlet_scale(ramp_of(cfg, it));
+            step_once();
+            sample();
+            stats_launch();
+          }
+        }
+        break;
       }
-      if (cfg.use_dp_convergence && it >= cfg.ramp_steps) {
-        hist[idx] = core;
-        idx = (idx + 1) % win;
-        if (idx == 0) filled = true;
-        if (filled) conv = window_settled(hist, idx, slope);
-      }
-      ++it;
+      CPS_CHECK(k1 < cfg.max_iterations, CPS_ERR_STATE, "outer loop monitor missed the iteration cap");
+      k0 = k1;
     }
-    sst.dp_converged = conv;
+    CPS_CHECK(!h.nonfinite, CPS_ERR_DIVERGED, "SIMPLE residual is not finite");
+    if (lcap > 0) {
+      const int rows = std::min(lcap, h.stop_it);
+      CPS_HIP(hipMemcpyAsync(log, dlog.p, (size_t)rows * sizeof(cps_iter_log), hipMemcpyDeviceToHost, s));
+      CPS_HIP(hipStreamSynchronize(s));
+      host_syncs++;
+    }
+    sst = h.last;
+    sst.iteration = iteration;
     if (out) *out = sst;
-    return it;
+    return h.stop_it;
   }
+  static constexpr int kDefaultPoll = 32;
 
-  // SIMPLE.cpp:456-490: standard deviation of the window and distance of the
-  // newest entry to its mean, both in % of max(|mean|, 1); optional
-  // least-squares slope gate (SIMPLE.cpp:347-366).
-  bool window_settled(const std::vector<double>& hist, int idx, const std::deque<std::pair<int, double>>& slope) const {
-    const int win = (int)hist.size();
-    double sum = 0.0;
-    for (double x : hist) sum += x;
-    const double mean = sum / win;
-    double var = 0.0;
-    for (double x : hist) var += (x - mean) * (x - mean);
-    const double den = std::max(std::fabs(mean), 1.0);
-    const double sdPct = std::sqrt(var / win) / den * 100.0;
-    const double lastPct = std::fabs(hist[(idx - 1 + win) % win] - mean) / den * 100.0;
-    bool slopeOk = true;
-    if (cfg.use_dp_slope_gate && cfg.dp_slope_window > 0 && slope.size() >= (size_t)cfg.dp_slope_window) {
-      double sx = 0, sy = 0, sxx = 0, sxy = 0;
-      const double x0 = slope.front().first, n = (double)slope.size();
-      for (const auto& e : slope) {
-        const double x = e.first - x0;
-        sx += x;
-        sy += e.second;
-        sxx += x * x;
-        sxy += x * e.second;
-      }
-      const double d = n * sxx - sx * sx;
-      const double s = std::fabs(d) < (double)1e-12f ? 0.0 : (n * sxy - sx * sy) / d;
-      slopeOk = std::fabs(s) <= cfg.dp_slope_max_delta / cfg.dp_slope_window;
-    }
-    return sdPct < cfg.dp_percent && lastPct < cfg.dp_percent && slopeOk;
+  // sets the inlet ramp factor to sc without touching u (the fields already
+  // hold the boundary values of that moment)
+  void restore_inlet_scale(double sc) {
+    inlet_scale = sc;
+    g.maxVel = 3.0 * std::max(std::fabs(sc * cfg.u_in), kTenthF);
+    const int n = g.M + 1;
+    k_inlet_scale<<<ceil_div(n, 256), 256, 0, ctx->stream>>>(n, sc, inlet0.p, inlet.p);
+    CPS_HIP(hipGetLastError());
   }
 
   DevBuf<double>* field(int which) {
@@ -1845,6 +1998,18 @@ int cps_simple_run_logged(cps_simple* s, cps_step_stats* last, cps_iter_log* log
     CPS_CHECK(log_cap >= 0 && (log || log_cap == 0), CPS_ERR_ARG, "log_cap needs a log of that many rows");
     const int n = s->s->run(last, log, log_cap);
     if (n_iters) *n_iters = n;
+  });
+}
+
+int cps_simple_run_polled(cps_simple* s, int poll_every, cps_step_stats* last, cps_iter_log* log, int log_cap,
+                          int* n_iters, int* host_syncs) {
+  return guard([&] {
+    CPS_CHECK(s, CPS_ERR_ARG, "null solver");
+    CPS_CHECK(log_cap >= 0 && (log || log_cap == 0), CPS_ERR_ARG, "log_cap needs a log of that many rows");
+    CPS_CHECK(poll_every >= 1, CPS_ERR_ARG, "poll_every must be >= 1");
+    const int n = s->s->run(last, log, log_cap, poll_every);
+    if (n_iters) *n_iters = n;
+    if (host_syncs) *host_syncs = s->s->host_syncs;
   });
 }
 

@@ -71,16 +71,24 @@ class SIMPLE:
         self.last = st.as_dict()
         return self.last
 
-    def runIterationsLogged(self):
+    def runIterationsLogged(self, poll_every=None):
         """runIterations plus the per-iteration log rows (residuals and the
         four pressure drops, writeIterationLogs output.cpp:330-360) as a
-        dict of numpy arrays."""
+        dict of numpy arrays.  poll_every: iterations the host enqueues
+        between reads of the device's stop flag (default 32); the host
+        synchronisations of the run are in self.last_host_syncs."""
         st = T.StepStats()
         cap = max(int(self.cfg.max_iterations), 0)
         log = (T.IterLog * max(cap, 1))()
-        n = C.c_int()
-        _capi.check(self.L.cps_simple_run_logged(self.h, C.byref(st), log, cap, C.byref(n)))
+        n, syncs = C.c_int(), C.c_int()
+        if poll_every is None:
+            _capi.check(self.L.cps_simple_run_logged(self.h, C.byref(st), log, cap, C.byref(n)))
+            syncs.value = -1
+        else:
+            _capi.check(self.L.cps_simple_run_polled(self.h, int(poll_every), C.byref(st), log, cap, C.byref(n),
+                                                     C.byref(syncs)))
         self.last = st.as_dict()
+        self.last_host_syncs = syncs.value
         rows = log[: n.value]
         hist = {k: np.array([getattr(r, k) for r in rows]) for k, _ in T.IterLog._fields_ if k != "pad_"}
         return self.last, hist

@@ -203,6 +203,17 @@ typedef struct cps_iter_log {
 /* cps_simple_run that also fills log[k] for iterations k < log_cap (log may
  * be NULL); *n_iters (may be NULL) receives the iterations done. */
 int cps_simple_run_logged(cps_simple* s, cps_step_stats* last, cps_iter_log* log, int log_cap, int* n_iters);
+/* cps_simple_run_logged with the host's polling interval exposed.  The loop's
+ * services (the log rows, the dP window, the slope gate and the stop test)
+ * run on the device after every iteration; the host enqueues poll_every
+ * iterations between reads of the device's stop flag, so a run of k
+ * iterations synchronises ceil(k / poll_every) times (+1 to copy the log
+ * out), reported in *host_syncs (may be NULL).  A chunk that passes the stop
+ * iteration is rolled back to its start and replayed to the stop, which
+ * reproduces the per-iteration loop bit for bit.  cps_simple_run and
+ * cps_simple_run_logged poll every 32 iterations. */
+int cps_simple_run_polled(cps_simple* s, int poll_every, cps_step_stats* last, cps_iter_log* log, int log_cap,
+                          int* n_iters, int* host_syncs);
 int cps_simple_get_field(cps_simple* s, int field, double* out);
 int cps_simple_set_field(cps_simple* s, int field, const double* in);
 /* inlet = 1: x = 0 patch, 0: x = L patch. */

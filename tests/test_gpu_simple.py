@@ -90,6 +90,50 @@ def test_run_dp_window_and_inlet_ramp_match_oracle(pkg, orc):
     assert rel_l2(gpu.get("u"), ref.get("u")) < 1e-10
 
 
+@pytest.mark.parametrize("slope", [0, 1])
+def test_run_on_device_polled_matches_oracle(pkg, orc, slope):
+    """runIterations with its services on the device (log rows, dP window,
+    slope gate, stop test after every iteration) and the host reading the
+    stop flag every K iterations: for every K the run stops at the oracle's
+    iteration with its drops and fields, the log rows equal the oracle's
+    history, fields and rows are bit-identical across K (a chunk that passes
+    the stop is rolled back and replayed), and the host synchronises
+    ceil(iterations / K) times plus once for the log."""
+    cfg, gamma, inlet = pkg.configs.channel_200x50(nx=60, ny=20, max_pressure_iter=600, p_tol=1e-14)
+    for k, v in dict(use_dp_convergence=1, inlet_ramp=1, ramp_steps=20, dp_window=15, dp_percent=1.0,
+                     x_core_in=0.002, x_core_out=0.004, epsilon=1e-12, max_iterations=150,
+                     use_dp_slope_gate=slope, dp_slope_window=10, dp_slope_max_delta=5.0).items():
+        setattr(cfg, k, v)
+    ref = orc.OracleSimple(pkg._types, cfg, gamma, inlet, prec=64)
+    sr = ref.run(history_len=cfg.max_iterations)
+    assert sr["dp_converged"] == 1 and sr["iterations"] < cfg.max_iterations
+    n_ref = sr["iterations"]
+    runs = {}
+    for K in (1, 7, 32, 1000):
+        print(f"case polled run K={K} slope={slope}", flush=True)
+        gpu = pkg.SIMPLE(cfg, gamma, inlet)
+        sg, hist = gpu.runIterationsLogged(poll_every=K)
+        print("  iterations", sg["iteration"], "oracle", n_ref, "host syncs", gpu.last_host_syncs, flush=True)
+        assert sg["iteration"] == n_ref and sg["dp_converged"] == 1
+        assert gpu.last_host_syncs == -(-n_ref // K) + 1
+        for key in ("dp_core_total", "dp_full_total", "error"):
+            assert sg[key] == pytest.approx(sr[key], rel=1e-9), key
+        err = np.maximum(np.maximum(hist["resid_u"], hist["resid_v"]), hist["resid_mass"])
+        assert err == pytest.approx(sr["history"], rel=1e-9)
+        assert hist["dp_core_total"] == pytest.approx(sr["dp_history"], rel=1e-9)
+        f = {n: gpu.get(n) for n in ("u", "v", "p", "uStar")}
+        for n in ("u", "v", "p"):
+            assert rel_l2(f[n], ref.get(n)) < 1e-10, n
+        runs[K] = (f, hist)
+        gpu.close()
+    f1, h1 = runs[1]
+    for K, (f, h) in runs.items():
+        for n in f1:
+            assert np.array_equal(f[n], f1[n]), (K, n)
+        for n in h1:
+            assert np.array_equal(h[n], h1[n]), (K, n)
+
+
 def test_fou_uniform_inlet_short_run(pkg, orc):
     # p_tol far below the round-off of the solve, so the PCG stop iteration
     # cannot flip between the two reduction orders
