@@ -84,3 +84,40 @@ def test_batch_gradients_threaded_match_single_field(pkg):
         np.testing.assert_array_equal(grad, grad1)
         s.close()
     b.close()
+
+
+def test_batch_48_fields_at_real_sizes_replays_match_eager(pkg):
+    """The design sweep at its own sizes: 48 fields drawn from 256 x 128 ..
+    1024 x 512 (sigma U[4, 12], per-field seeds), FP32-inner / FP64-refined
+    with the headline knobs, 16 streams.  After two rounds of graph replays,
+    one field of each size equals the same field stepped eagerly on its own,
+    bit for bit, and every field stays finite."""
+    T = pkg._types
+    fl = []
+    for cfg, g, inlet in pkg.configs.batch_sweep(n_fields=48):
+        cfg.pressure_solver, cfg.precision = T.CPS_PRESSURE_GMG, T.CPS_PREC_MIXED
+        cfg.p_tol, cfg.p_rtol, cfg.max_pressure_iter, cfg.max_refine = 0.0, 1e-3, 6, 1
+        cfg.max_momentum_iter = 8
+        fl.append((cfg, g, inlet))
+    pick = {}
+    for k, (cfg, _, _) in enumerate(fl):
+        pick.setdefault((cfg.nx, cfg.ny), k)
+    assert len(pick) == 4
+    print("case batch 48 fields, sizes", sorted(pick), flush=True)
+    b = pkg.Batch(fl)
+    b.tune(streams=16, reps=2)
+    for rounds in (3, 7):
+        b.step(rounds)
+        b.synchronize()
+        for (nx, ny), k in sorted(pick.items()):
+            cfg, g, inlet = fl[k]
+            st = b.stats(k)
+            s = pkg.SIMPLE(cfg, g, inlet)
+            s.step(st["iteration"])
+            for name in ("u", "v", "p"):
+                np.testing.assert_array_equal(b.get(k, name), s.get(name), err_msg=f"{name} field {k} {nx}x{ny}")
+            assert b.pressure_drop(k) == s.full_static_drop()
+            s.close()
+    for k in range(b.n):
+        assert np.isfinite(b.stats(k)["error"]), k
+    b.close()
