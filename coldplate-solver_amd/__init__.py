@@ -12,6 +12,9 @@ from ._types import default_config, SimpleConfig, StepStats, Patch, field_shape 
 from . import _capi  # noqa: F401  (loads nothing until first use)
 from .simple import SIMPLE, Batch, pressure_solve_csr, pressure_solve_csr_noalloc, momentum_jacobi_csr, pressure_drop  # noqa: F401,E501
 from .thermal import ThermalSolver3D, thermal_objective, surrogate_adjoint  # noqa: F401
+from . import coupled  # noqa: F401
+from .coupled import CoupledObjective, coupled_f0, f0_smooth  # noqa: F401
 
 __all__ = ["_types", "configs", "exportfiles", "default_config", "SimpleConfig", "StepStats", "Patch", "field_shape",
-           "SIMPLE", "Batch", "pressure_solve_csr", "pressure_solve_csr_noalloc", "momentum_jacobi_csr", "pressure_drop", "ThermalSolver3D", "thermal_objective", "surrogate_adjoint"]
+           "SIMPLE", "Batch", "pressure_solve_csr", "pressure_solve_csr_noalloc", "momentum_jacobi_csr", "pressure_drop", "ThermalSolver3D", "thermal_objective", "surrogate_adjoint",
+           "coupled", "CoupledObjective", "coupled_f0", "f0_smooth"]

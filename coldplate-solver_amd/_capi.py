@@ -71,6 +71,7 @@ def lib():
         L.cps_simple_time_kernel.argtypes = [vp, C.c_int]
         L.cps_simple_kernel_timing.argtypes = [vp, ip, dp, dp]
         L.cps_simple_gradient_dp.argtypes = [vp, C.c_double, C.c_int, dp, dp, P(T.SolveStats)]
+        L.cps_simple_gradient_vjp.argtypes = [vp, dp, dp, dp, C.c_double, C.c_int, dp, P(T.SolveStats)]
         L.cps_simple_state_residual.argtypes = [vp, dp]
         L.cps_simple_jacobian_apply.argtypes = [vp, C.c_int, dp, dp]
         L.cps_simple_adjoint_options.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int]
@@ -97,6 +98,7 @@ def lib():
         L.cps_thermal_get_metrics.argtypes = [vp, C.c_double, P(T.ThermalMetrics)]
         L.cps_thermal_solve_adjoint.argtypes = [vp, dp, dp, P(T.SolveStats)]
         L.cps_thermal_gradient.argtypes = [vp, C.c_int, C.c_double, dp, dp, P(T.SolveStats)]
+        L.cps_thermal_sensitivities.argtypes = [vp, dp, dp, dp, dp]
         L.cps_thermal_get_operator.argtypes = [vp, dp, dp]
         _lib = L
         return L

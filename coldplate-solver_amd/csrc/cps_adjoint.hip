@@ -1174,6 +1174,63 @@ class FlowAdjoint {
     }
   }
 
+  // d J / d gamma of a caller's objective J(u, v, p) of the converged state:
+  // dJ/dw (host, public shapes, null = 0) is folded through the boundary
+  // rules the equations read the ghost values by (readU / readV / readP:
+  // prescribed and zero values drop out, copies add to the DOF they copy),
+  // then J^T lambda = dJ/dw and dJ/dgamma = -lambda^T dR/dgamma as for dP.
+  void run_vjp(const SimpleView& sv, const double* hu, const double* hv, const double* hp, double rtol,
+               int max_iter, double* hgrad, cps_solve_stats* stats) {
+    hipStream_t s = ctx->stream;
+    adj::View V{sv.g, sv.S, sv.u, sv.v, sv.p, sv.gam, sv.alp, sv.inlet, sv.dE, sv.dN};
+    build(V);
+    const int M = g.M, N = g.N;
+    const size_t P = (size_t)g.P;
+    std::vector<double> hg((size_t)n, 0.0);
+    if (hu)
+      for (int i = 0; i <= M; ++i)
+        for (int j = 0; j < N; ++j) {
+          const double x = hu[(size_t)i * N + j];
+          if (x == 0.0 || j == 0 || i == 0 || i == M) continue;  // inlet, walls
+          hg[(size_t)i * P + (j == N - 1 ? N - 2 : j)] += x;       // outlet copy
+        }
+    if (hv)
+      for (int i = 0; i < M; ++i)
+        for (int j = 0; j <= N; ++j) {
+          const double x = hv[(size_t)i * (N + 1) + j];
+          if (x == 0.0 || i == 0 || i == M - 1 || j == 0) continue;
+          hg[S + (size_t)i * P + (j == N ? N - 1 : j)] += x;
+        }
+    if (hp)
+      for (int i = 0; i <= M; ++i)
+        for (int j = 0; j <= N; ++j) {
+          const double x = hp[(size_t)i * (N + 1) + j];
+          if (x == 0.0 || j >= N - 1) continue;  // columns N-1, N are 0
+          const int ii = std::min(std::max(i, 1), M - 1), jj = j == 0 ? 1 : j;
+          hg[2 * S + (size_t)ii * P + jj] += x;
+        }
+    CPS_HIP(hipMemcpyAsync(gvec.p, hg.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+    cps_simple_config pcfg = sv.cfg;
+    pcfg.p_tol = 0.0;
+    pcfg.p_rtol = 1e-3;
+    pcfg.max_pressure_iter = schur_iters;
+    pcfg.max_refine = 1;
+    double relres = 0.0;
+    const int its = fgmres(V, rtol, max_iter, pcfg, &relres);
+    const double alphaMax = sv.cfg.eta / std::max(sv.cfg.brinkman_kmin, (double)1e-20f);
+    const int ncell = (g.M - 1) * (g.N - 1);
+    adj::k_grad<<<blocks(ncell), adj::BS, 0, s>>>(V, lam.p, alphaMax, sv.cfg.brinkman_q, (double)1e-10f, grad.p);
+    CPS_HIP(hipGetLastError());
+    if (hgrad) CPS_HIP(hipMemcpyAsync(hgrad, grad.p, grad.bytes(), hipMemcpyDeviceToHost, s));
+    CPS_HIP(hipStreamSynchronize(s));
+    if (stats) {
+      stats->iterations = its;
+      stats->refinements = 0;
+      stats->rel_residual = relres;
+      stats->converged = relres <= rtol ? 1 : 0;
+    }
+  }
+
   // y = J x or J^T x on host field arrays [u | v | p] in their public shapes
   void apply(const adj::View& V, bool transpose, const double* hx, double* hy) {
     hipStream_t s = ctx->stream;
@@ -1218,6 +1275,10 @@ void flow_adjoint_free(FlowAdjoint* a) { delete a; }
 void flow_adjoint_run(FlowAdjoint* a, const SimpleView& sv, double rtol, int max_iter, double* dp, double* grad,
                       cps_solve_stats* stats) {
   a->run(sv, rtol, max_iter, dp, grad, stats);
+}
+void flow_adjoint_vjp(FlowAdjoint* a, const SimpleView& sv, const double* du, const double* dv, const double* dp,
+                      double rtol, int max_iter, double* grad, cps_solve_stats* stats) {
+  a->run_vjp(sv, du, dv, dp, rtol, max_iter, grad, stats);
 }
 void flow_adjoint_apply(FlowAdjoint* a, const SimpleView& sv, int transpose, const double* x, double* y) {
   adj::View V{sv.g, sv.S, sv.u, sv.v, sv.p, sv.gam, sv.alp, sv.inlet, sv.dE, sv.dN};

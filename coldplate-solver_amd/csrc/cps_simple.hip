@@ -2064,6 +2064,15 @@ int cps_simple_gradient_dp(cps_simple* s, double rtol, int max_iter, double* dp,
   });
 }
 
+int cps_simple_gradient_vjp(cps_simple* s, const double* dJ_du, const double* dJ_dv, const double* dJ_dp,
+                            double rtol, int max_iter, double* dJ_dgamma, cps_solve_stats* stats) {
+  return guard([&] {
+    CPS_CHECK(s && dJ_dgamma, CPS_ERR_ARG, "null argument");
+    CPS_CHECK(rtol > 0 && max_iter >= 1, CPS_ERR_ARG, "rtol must be > 0 and max_iter >= 1");
+    cps::flow_adjoint_vjp(s->s->adj(), s->s->view(), dJ_du, dJ_dv, dJ_dp, rtol, max_iter, dJ_dgamma, stats);
+  });
+}
+
 int cps_simple_state_residual(cps_simple* s, double* out3) {
   return guard([&] {
     CPS_CHECK(s && out3, CPS_ERR_ARG, "null argument");

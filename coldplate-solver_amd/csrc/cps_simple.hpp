@@ -137,6 +137,8 @@ void flow_adjoint_run(FlowAdjoint* a, const SimpleView& sv, double rtol, int max
                       cps_solve_stats* stats);
 void flow_adjoint_residual(FlowAdjoint* a, const SimpleView& sv, double* out3);
 void flow_adjoint_apply(FlowAdjoint* a, const SimpleView& sv, int transpose, const double* x, double* y);
+void flow_adjoint_vjp(FlowAdjoint* a, const SimpleView& sv, const double* du, const double* dv, const double* dp,
+                      double rtol, int max_iter, double* grad, cps_solve_stats* stats);
 void flow_adjoint_options(FlowAdjoint* a, int kind, int sweeps, double omega, int schur_iters);
 
 }  // namespace cps

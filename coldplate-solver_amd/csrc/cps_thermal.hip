@@ -914,6 +914,72 @@ __global__ __launch_bounds__(TNT) void k_sens(Phys P, Op<double> A, const double
   grad[j * P.nx + i] = -dkdg(P, i, j) * acc;
 }
 
+// dJ/du_xy(i, j) and dJ/dv_xy(i, j) = - lambda^T dR/du (R = A T - b): the
+// channel velocity enters the upwind advection of thermal_solver.cpp:469-507
+// through the face fluxes F = rho cp (u_L + u_R) / 2 pois(k) A between two
+// fluid cells (L west / south of R, F > 0 from L to R), the inlet term of
+// fluid cells at i = 0 (F_bc = rho cp u_P A: inflow on b, outflow on the
+// diagonal) and the outflow terms at i = nx - 1 and j = ny - 1.  For a face,
+// lambda^T dR/dF = T_up (lambda_L - lambda_R) with T_up the upwind cell's
+// temperature (L's when F > 0).  One thread per column sums, over the
+// channel layers, the faces of its own velocity; each interior face is
+// evaluated by both of its columns.  The fluid masks (gamma > 0.01) are
+// piecewise constant in gamma and held fixed.
+__global__ __launch_bounds__(TNT) void k_sens_vel(Phys P, Op<double> A, const double* T, const double* lam,
+                                                  double* gu, double* gv) {
+  const int i = tile_x() * TBX + (int)threadIdx.x, j = tile_y() * TBY + (int)threadIdx.y;
+  if (i >= P.nx || j >= P.ny) return;
+  const double dx = P.dx, dy = P.dy, rc = P.rc, Tin = P.t_in;
+  double su = 0.0, sv = 0.0;
+  for (int k = 0; k < P.nz; ++k) {
+    if (!fluid(P, i, j, k)) continue;
+    const size_t q = A.at(i, j, k);
+    const double po = P.pois[k], dzk = P.dz[k], Ax = dy * dzk, Ay = dx * dzk;
+    const double tP = T[q], lP = lam[q];
+    const double uP = P.u[j * P.nx + i] * po, vP = P.v[j * P.nx + i] * po;
+    // face between cells L and R (both fluid), flux F: d J / d F
+    auto dJdF = [&](double F, double tL, double lL, double tR, double lR) {
+      const double tup = F > 0.0 ? tL : tR;
+      return -tup * (lL - lR);
+    };
+    if (i > 0) {
+      if (fluid(P, i - 1, j, k)) {
+        const size_t w = q - 1;
+        const double F = rc * 0.5 * (uP + P.u[j * P.nx + i - 1] * po) * Ax;
+        su += dJdF(F, T[w], lam[w], tP, lP) * rc * 0.5 * po * Ax;
+      }
+    } else {
+      const double Fbc = rc * uP * Ax;  // r_P -= Fbc T_in (Fbc > 0) or += -Fbc T_P
+      su += -lP * (Fbc > 0.0 ? -Tin : -tP) * rc * po * Ax;
+    }
+    if (i < P.nx - 1) {
+      if (fluid(P, i + 1, j, k)) {
+        const size_t e = q + 1;
+        const double F = rc * 0.5 * (uP + P.u[j * P.nx + i + 1] * po) * Ax;
+        su += dJdF(F, tP, lP, T[e], lam[e]) * rc * 0.5 * po * Ax;
+      }
+    } else if (uP > 0.0) {
+      su += -lP * tP * rc * po * Ax;
+    }
+    if (j > 0 && fluid(P, i, j - 1, k)) {
+      const size_t sq = q - A.pitch;
+      const double F = rc * 0.5 * (vP + P.v[(j - 1) * P.nx + i] * po) * Ay;
+      sv += dJdF(F, T[sq], lam[sq], tP, lP) * rc * 0.5 * po * Ay;
+    }
+    if (j < P.ny - 1) {
+      if (fluid(P, i, j + 1, k)) {
+        const size_t nq = q + A.pitch;
+        const double F = rc * 0.5 * (vP + P.v[(j + 1) * P.nx + i] * po) * Ay;
+        sv += dJdF(F, tP, lP, T[nq], lam[nq]) * rc * 0.5 * po * Ay;
+      }
+    } else if (vP > 0.0) {
+      sv += -lP * tP * rc * po * Ay;
+    }
+  }
+  if (gu) gu[j * P.nx + i] = su;
+  if (gv) gv[j * P.nx + i] = sv;
+}
+
 // ---------------------------------------------------------------------------
 // 2-D conduction surrogate of AdjointMethod/thermal_adjoint.py:8-159 (nz = 1
 // operator): harmonic-mean faces k_eff / dx^2 (:56-62) and 2 k / dx^2 per
@@ -1492,6 +1558,25 @@ class ThermalSolver {
     if (J) *J = Jv;
   }
 
+  // -lambda^T d(A T - b)/d(gamma, u_xy, v_xy) at the last solve's T for a
+  // caller's adjoint lambda (nz x ny x nx); outputs ny x nx, may be null
+  void sensitivities(const double* hlam, double* hg, double* hu, double* hv) {
+    CPS_CHECK(solved, CPS_ERR_STATE, "no temperature field: call cps_thermal_solve first");
+    hipStream_t st = ctx->stream;
+    const dim3 G = grid(), B(th::TBX, th::TBY);
+    upload(hlam, lam.p);
+    const size_t m2 = (size_t)nx * ny;
+    if (vsens.n != 2 * m2) vsens.alloc(2 * m2);
+    th::k_sens<<<dim3(G.x, G.y), B, 0, st>>>(phys(), op64(false), T.p, lam.p, dgrad.p);
+    th::k_sens_vel<<<dim3(G.x, G.y), B, 0, st>>>(phys(), op64(false), T.p, lam.p, vsens.p, vsens.p + m2);
+    CPS_HIP(hipGetLastError());
+    if (hg) CPS_HIP(hipMemcpyAsync(hg, dgrad.p, dgrad.bytes(), hipMemcpyDeviceToHost, st));
+    if (hu) CPS_HIP(hipMemcpyAsync(hu, vsens.p, m2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (hv) CPS_HIP(hipMemcpyAsync(hv, vsens.p + m2, m2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    CPS_HIP(hipStreamSynchronize(st));
+  }
+  DevBuf<double> vsens;
+
   void get_operator(double* hcoef, double* hb) {
     const size_t m = (size_t)nz * ny * nx;
     for (int c = 0; c < 7; ++c) download(A64[c].p, hcoef + c * m);
@@ -1693,6 +1778,14 @@ int cps_thermal_gradient(cps_thermal* t, int objective, double pnorm_exp, double
   return guard([&] {
     CPS_CHECK(t && dJ_dgamma, CPS_ERR_ARG, "null argument");
     t->s->gradient(objective, pnorm_exp, dJ_dgamma, J, stats);
+  });
+}
+
+int cps_thermal_sensitivities(cps_thermal* t, const double* lambda, double* dJ_dgamma, double* dJ_du,
+                              double* dJ_dv) {
+  return guard([&] {
+    CPS_CHECK(t && lambda, CPS_ERR_ARG, "null argument");
+    t->s->sensitivities(lambda, dJ_dgamma, dJ_du, dJ_dv);
   });
 }
 

@@ -133,6 +133,26 @@ class SIMPLE:
         self.last_adjoint = st.as_dict()
         return dp.value, grad
 
+    def gradient_vjp(self, dJ_du=None, dJ_dv=None, dJ_dp=None, rtol=1e-10, max_iter=400):
+        """d J / d gamma (ny, nx) of a caller's objective J(u, v, p) of the
+        converged state, given dJ/du, dJ/dv, dJ/dp in the field shapes (None
+        = 0), by the same discrete adjoint as gradient_dp."""
+        arrs = []
+        for name, a in (("u", dJ_du), ("v", dJ_dv), ("p", dJ_dp)):
+            if a is None:
+                arrs.append(None)
+                continue
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.shape != T.field_shape(name, self.cfg.nx, self.cfg.ny):
+                raise ValueError(f"dJ_d{name} must have shape {T.field_shape(name, self.cfg.nx, self.cfg.ny)}")
+            arrs.append(a)
+        grad = np.empty((self.cfg.ny, self.cfg.nx))
+        st = T.SolveStats()
+        ptr = [None if a is None else _capi.dptr(a) for a in arrs]
+        _capi.check(self.L.cps_simple_gradient_vjp(self.h, *ptr, rtol, max_iter, _capi.dptr(grad), C.byref(st)))
+        self.last_adjoint = st.as_dict()
+        return grad
+
     def adjoint_options(self, preconditioner="lsc_adi", line_sweeps=2, omega=0.8, schur_iters=4):
         """Preconditioner of the adjoint solve: "lsc_adi" (default: x- and
         y-line momentum solves with damped alternating sweeps, least-squares-

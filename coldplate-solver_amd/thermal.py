@@ -120,6 +120,15 @@ class ThermalSolver3D:
         self.last_adjoint = st.as_dict()
         return lam
 
+    def sensitivities(self, lam):
+        """-lambda^T d(A T - b)/d(gamma, u, v) at the last solve's T for an
+        adjoint lambda (nz, ny, nx): (dJ/dgamma, dJ/du, dJ/dv), each (ny, nx),
+        u and v being the cell-centred channel velocity."""
+        lam = _f64(lam, (self.nz, self.ny, self.nx), "lambda")
+        out = [np.empty((self.ny, self.nx)) for _ in range(3)]
+        _capi.check(self.L.cps_thermal_sensitivities(self.h, _capi.dptr(lam), *[_capi.dptr(a) for a in out]))
+        return tuple(out)
+
     def gradient(self, objective="t_avg_base", pnorm_exp=8.0):
         """(J, dJ/dgamma (ny, nx)) of the last solve."""
         grad = np.empty((self.ny, self.nx))

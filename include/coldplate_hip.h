@@ -238,6 +238,16 @@ int cps_simple_plane(cps_simple* s, double x, cps_patch* out);
  * point (cps_simple_state_residual). */
 int cps_simple_gradient_dp(cps_simple* s, double rtol, int max_iter, double* dp, double* dJ_dgamma,
                            cps_solve_stats* stats);
+/* The same adjoint for a caller's objective J(u, v, p) of the converged
+ * state (no reference counterpart; the coupled optimiser objective of
+ * mma_optimizer.py:477-487 reaches the flow through the thermal solve's
+ * velocity): dJ_du, dJ_dv, dJ_dp are dJ/d(field) in the field shapes above
+ * (NULL = 0).  Entries the boundary rules fix (inlet, walls, pinned pressure
+ * columns) drop out and ghost copies count for the DOF they copy
+ * (boundaries.cpp:50-146); dJ_dgamma (ny x nx) = -lambda^T dR/dgamma with
+ * J^T lambda = dJ/dw solved to rtol. */
+int cps_simple_gradient_vjp(cps_simple* s, const double* dJ_du, const double* dJ_dv, const double* dJ_dp,
+                            double rtol, int max_iter, double* dJ_dgamma, cps_solve_stats* stats);
 /* max |R| over the U-momentum, V-momentum and continuity equations at the
  * current fields (out3[3]). */
 int cps_simple_state_residual(cps_simple* s, double* out3);
@@ -389,6 +399,16 @@ int cps_thermal_gradient(cps_thermal* t, int objective, double pnorm_exp, double
 int cps_surrogate_adjoint(cps_ctx* ctx, int nx, int ny, const double* k, const double* P, const double* T_in,
                           double rtol, const double* gamma, double k_s, double k_f, double q_k, double* T_out,
                           double* lambda, double* sens, double* dJ_dgamma, cps_solve_stats* stats);
+/* -lambda^T d(A T - b) / d(gamma, u, v) at the last solve's T for a caller's
+ * adjoint lambda (nz x ny x nx, from cps_thermal_solve_adjoint with the
+ * caller's dJ/dT): the design sensitivity through k(gamma) as in
+ * cps_thermal_gradient, and the sensitivities to the cell-centred channel
+ * velocity (ny x nx each) through the upwind advection, inlet and outflow
+ * terms (thermal_solver.cpp:469-507); the gamma > 0.01 masks are held fixed.
+ * Outputs may be NULL.  No reference counterpart: it closes the chain rule
+ * from a thermal objective through the velocity to the flow adjoint. */
+int cps_thermal_sensitivities(cps_thermal* t, const double* lambda, double* dJ_dgamma, double* dJ_du,
+                              double* dJ_dv);
 /* Assembled operator as 7 stencil arrays [d, w, e, s, n, b, t] of nz*ny*nx
  * each (entry of A for the neighbour; 0 where none) and the right side b. */
 int cps_thermal_get_operator(cps_thermal* t, double* coef7, double* rhs);
