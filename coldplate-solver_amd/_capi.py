@@ -99,6 +99,8 @@ def lib():
         L.cps_thermal_solve_adjoint.argtypes = [vp, dp, dp, P(T.SolveStats)]
         L.cps_thermal_gradient.argtypes = [vp, C.c_int, C.c_double, dp, dp, P(T.SolveStats)]
         L.cps_thermal_sensitivities.argtypes = [vp, dp, dp, dp, dp]
+        L.cps_thermal_solve_csr.argtypes = [vp, C.c_int, C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int,
+                                            C.c_int, P(T.SolveStats)]
         L.cps_thermal_get_operator.argtypes = [vp, dp, dp]
         _lib = L
         return L

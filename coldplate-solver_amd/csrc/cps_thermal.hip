@@ -914,6 +914,37 @@ __global__ __launch_bounds__(TNT) void k_sens(Phys P, Op<double> A, const double
   grad[j * P.nx + i] = -dkdg(P, i, j) * acc;
 }
 
+// A CSR matrix of the thermal stencil's shape (row r = (k ny + j) nx + i,
+// thermal_solver.cpp:235) into the seven stencil arrays, one thread per row;
+// duplicate entries add up (SparseMatrix::from_coo sums them,
+// thermal_solver.cpp:93-131).  An entry that is not the cell itself or one
+// of its six grid neighbours sets *bad.
+__global__ __launch_bounds__(256) void k_csr_scatter(int n, const int* rp, const int* ci, const double* val,
+                                                    const double* b, Op<double> A, double* rhs, int* bad) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
+  const int i = r % nx, j = (r / nx) % ny, k = r / nxy;
+  double c[7] = {0, 0, 0, 0, 0, 0, 0};  // d, w, e, s, n, b, t
+  int wrong = 0;
+  for (int q = rp[r]; q < rp[r + 1]; ++q) {
+    const int dl = ci[q] - r;
+    const double v = val[q];
+    if (dl == 0) c[0] += v;
+    else if (dl == -1 && i > 0) c[1] += v;
+    else if (dl == 1 && i < nx - 1) c[2] += v;
+    else if (dl == -nx && j > 0) c[3] += v;
+    else if (dl == nx && j < ny - 1) c[4] += v;
+    else if (dl == -nxy && k > 0) c[5] += v;
+    else if (dl == nxy && k < nz - 1) c[6] += v;
+    else wrong = 1;
+  }
+  const size_t o = A.at(i, j, k);
+  A.d[o] = c[0]; A.w[o] = c[1]; A.e[o] = c[2]; A.s[o] = c[3]; A.n[o] = c[4]; A.b[o] = c[5]; A.t[o] = c[6];
+  rhs[o] = b[r];
+  if (wrong) *bad = 1;
+}
+
 // dJ/du_xy(i, j) and dJ/dv_xy(i, j) = - lambda^T dR/du (R = A T - b): the
 // channel velocity enters the upwind advection of thermal_solver.cpp:469-507
 // through the face fluxes F = rho cp (u_L + u_R) / 2 pois(k) A between two
@@ -1333,6 +1364,66 @@ class ThermalSolver {
     else
       k32.reset(new ThermalKrylov<float, double>(st, nx, ny, nz, false));
     update(hg, hu, hv, hq);
+  }
+
+  // Solver for an assembled matrix only (the AMGCL call of
+  // thermal_solver.cpp:534-552): the stencil arrays come from load_csr; no
+  // physical inputs, so no metrics or design gradients.
+  struct CsrOnly {};
+  ThermalSolver(Ctx* c, const cps_thermal_config& conf, CsrOnly) : ctx(c), cfg(conf) {
+    CPS_CHECK(cfg.nx >= 1 && cfg.ny >= 1 && cfg.nz >= 1 && cfg.nz <= th::kMaxNz, CPS_ERR_ARG,
+              "thermal grid needs nx, ny >= 1 and 1 <= nz <= 64");
+    CPS_CHECK(cfg.precision == CPS_PREC_FP64 || cfg.precision == CPS_PREC_MIXED, CPS_ERR_ARG,
+              "precision must be CPS_PREC_FP64 or CPS_PREC_MIXED");
+    CPS_CHECK(cfg.rtol > 0 && cfg.max_iter >= 1, CPS_ERR_ARG, "rtol must be > 0 and max_iter >= 1");
+    nx = cfg.nx;
+    ny = cfg.ny;
+    nz = cfg.nz;
+    pitch = ceil_div(nx, 64) * 64;
+    const size_t S = (size_t)nz * ny * pitch;
+    hipStream_t st = ctx->stream;
+    for (auto& a : A64) {
+      a.alloc(S);
+      CPS_HIP(hipMemsetAsync(a.p, 0, a.bytes(), st));
+    }
+    for (auto* b : {&rhs, &T}) {
+      b->alloc(S);
+      CPS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), st));
+    }
+    part.alloc((size_t)nblocks() * th::MS_COUNT);
+    sums.alloc(th::MS_COUNT + th::MX_COUNT);
+    if (cfg.precision == CPS_PREC_FP64)
+      k64.reset(new ThermalKrylov<double>(st, nx, ny, nz, true));
+    else
+      k32.reset(new ThermalKrylov<float, double>(st, nx, ny, nz, false));
+  }
+
+  void load_csr(const int* hrp, const int* hci, const double* hval, const double* hb) {
+    const int n = nx * ny * nz;
+    const int nnz = hrp[n];
+    hipStream_t st = ctx->stream;
+    DevBuf<int> rp, ci, bad;
+    DevBuf<double> val, b;
+    rp.alloc(n + 1);
+    ci.alloc(std::max(nnz, 1));
+    val.alloc(std::max(nnz, 1));
+    b.alloc(n);
+    bad.alloc(1);
+    CPS_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), st));
+    CPS_HIP(hipMemcpyAsync(rp.p, hrp, (n + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+    if (nnz > 0) {
+      CPS_HIP(hipMemcpyAsync(ci.p, hci, nnz * sizeof(int), hipMemcpyHostToDevice, st));
+      CPS_HIP(hipMemcpyAsync(val.p, hval, nnz * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    CPS_HIP(hipMemcpyAsync(b.p, hb, n * sizeof(double), hipMemcpyHostToDevice, st));
+    th::k_csr_scatter<<<ceil_div(n, 256), 256, 0, st>>>(n, rp.p, ci.p, val.p, b.p, op64(false), rhs.p, bad.p);
+    CPS_HIP(hipGetLastError());
+    int hbad = 0;
+    CPS_HIP(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    CPS_HIP(hipStreamSynchronize(st));
+    CPS_CHECK(!hbad, CPS_ERR_ARG, "matrix is not a 7-point stencil on the nx x ny x nz grid");
+    solved = false;
+    transposed_ready = false;
   }
 
   dim3 grid() const { return dim3(ceil_div(nx, th::TBX), ceil_div(ny, th::TBY), nz); }
@@ -1778,6 +1869,33 @@ int cps_thermal_gradient(cps_thermal* t, int objective, double pnorm_exp, double
   return guard([&] {
     CPS_CHECK(t && dJ_dgamma, CPS_ERR_ARG, "null argument");
     t->s->gradient(objective, pnorm_exp, dJ_dgamma, J, stats);
+  });
+}
+
+int cps_thermal_solve_csr(cps_ctx* ctx, int nx, int ny, int nz, const int* row_ptr, const int* col_idx,
+                          const double* values, const double* rhs, double* T, double rtol, int max_iter,
+                          int precision, cps_solve_stats* stats) {
+  return guard([&] {
+    CPS_CHECK(ctx && row_ptr && rhs && T, CPS_ERR_ARG, "null argument");
+    CPS_CHECK(nx >= 1 && ny >= 1 && nz >= 1 && (long long)nx * ny * nz < (1LL << 31), CPS_ERR_ARG, "bad grid size");
+    const int n = nx * ny * nz;
+    CPS_CHECK(row_ptr[0] == 0, CPS_ERR_ARG, "row_ptr must start at 0");
+    for (int r = 0; r < n; ++r) CPS_CHECK(row_ptr[r + 1] >= row_ptr[r], CPS_ERR_ARG, "row_ptr must be non-decreasing");
+    CPS_CHECK(row_ptr[n] == 0 || (col_idx && values), CPS_ERR_ARG, "null CSR arrays");
+    for (int q = 0; q < row_ptr[n]; ++q) CPS_CHECK(col_idx[q] >= 0 && col_idx[q] < n, CPS_ERR_ARG, "col_idx out of range");
+    CPS_HIP(hipSetDevice(ctx->c.device));
+    cps_thermal_config cfg;
+    cps_thermal_config_default(&cfg);
+    cfg.nx = nx;
+    cfg.ny = ny;
+    cfg.nz = nz;
+    cfg.rtol = rtol;
+    cfg.max_iter = max_iter;
+    cfg.precision = precision;
+    cps::ThermalSolver s(&ctx->c, cfg, cps::ThermalSolver::CsrOnly{});
+    s.load_csr(row_ptr, col_idx, values, rhs);
+    s.solve(T, stats);
+    s.download(s.T.p, T);
   });
 }
 

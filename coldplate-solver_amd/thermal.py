@@ -140,6 +140,28 @@ class ThermalSolver3D:
         return J.value, grad
 
 
+def solve_csr(nx, ny, nz, row_ptr, col_idx, values, rhs, T0, rtol=1e-6, max_iter=1000, precision=T.CPS_PREC_FP64,
+              device=0):
+    """Drop-in for the reference's AMGCL call (thermal_solver.cpp:534-552):
+    solves the caller's assembled 7-point matrix (CSR, rows numbered
+    (k ny + j) nx + i) from T0 to ||b - A T|| <= rtol ||b||.  Returns
+    (T (n,), (iterations, relative residual))."""
+    ctx = _capi.context(device)
+    n = nx * ny * nz
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    ci = np.ascontiguousarray(col_idx, dtype=np.int32)
+    val = np.ascontiguousarray(values, dtype=np.float64)
+    b = np.ascontiguousarray(rhs, dtype=np.float64)
+    x = np.array(T0, dtype=np.float64).reshape(-1).copy()
+    if rp.shape != (n + 1,) or b.shape != (n,) or x.shape != (n,) or ci.shape != val.shape:
+        raise ValueError("CSR arrays do not match the grid")
+    st = T.SolveStats()
+    _capi.check(ctx.L.cps_thermal_solve_csr(ctx.h, nx, ny, nz, _capi.iptr(rp), _capi.iptr(ci), _capi.dptr(val),
+                                            _capi.dptr(b), _capi.dptr(x), rtol, max_iter, precision, C.byref(st)))
+    d = st.as_dict()
+    return x, (d["iterations"], d["rel_residual"])
+
+
 def surrogate_adjoint(k, P=None, T=None, gamma=None, k_s=400.0, k_f=0.6, q_k=1.0, rtol=1e-12, device=0):
     """The 2-D conduction surrogate adjoint of AdjointMethod/thermal_adjoint.py
     (solve_thermal_adjoint) on the device: T (solved from P unless given),

@@ -399,6 +399,20 @@ int cps_thermal_gradient(cps_thermal* t, int objective, double pnorm_exp, double
 int cps_surrogate_adjoint(cps_ctx* ctx, int nx, int ny, const double* k, const double* P, const double* T_in,
                           double rtol, const double* gamma, double k_s, double k_f, double q_k, double* T_out,
                           double* lambda, double* sens, double* dJ_dgamma, cps_solve_stats* stats);
+/* Drop-in for the AMGCL call itself (thermal_solver.cpp:534-552:
+ * Solver solver(crs(n, n, ptr, col, val), prm); solver(b, T)): the caller's
+ * assembled matrix in CSR (n = nx ny nz rows, row r = (k ny + j) nx + i as
+ * thermal_solver.cpp:235 numbers them; duplicate entries add up) must have
+ * the 7-point pattern on that grid, else CPS_ERR_ARG.  T: initial guess in
+ * (the reference starts from T_in, :428), solution out; stop at
+ * ||b - A T|| <= rtol ||b|| or max_iter Krylov iterations (AMGCL's tol /
+ * maxiter, :547-548) with the device BiCGStab + geometric multigrid of
+ * cps_thermal_solve in CPS_PREC_FP64 or CPS_PREC_MIXED.  stats (may be NULL)
+ * take the iterations and the relative residual, the pair solver(b, T)
+ * returns. */
+int cps_thermal_solve_csr(cps_ctx* ctx, int nx, int ny, int nz, const int* row_ptr, const int* col_idx,
+                          const double* values, const double* rhs, double* T, double rtol, int max_iter,
+                          int precision, cps_solve_stats* stats);
 /* -lambda^T d(A T - b) / d(gamma, u, v) at the last solve's T for a caller's
  * adjoint lambda (nz x ny x nx, from cps_thermal_solve_adjoint with the
  * caller's dJ/dT): the design sensitivity through k(gamma) as in

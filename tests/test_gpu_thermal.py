@@ -182,3 +182,34 @@ def test_surrogate_adjoint_matches_reference_vectors(pkg):
                                atol=1e-12 * np.abs(z["ext_sens"] * dk).max())
     out = pkg.surrogate_adjoint(z["def_k"], P=z["def_P"], rtol=1e-13)
     np.testing.assert_allclose(out["sens"], z["def_sens"], rtol=1e-8, atol=1e-12 * np.abs(z["def_sens"]).max())
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("seed,nx,ny", [(5, 32, 16), (6, 45, 27)])
+def test_csr_dropin_solves_the_oracle_matrix(pkg, orc, precision, seed, nx, ny):
+    """cps_thermal_solve_csr on the oracle's assembled CSR (the
+    thermal_solver.cpp:534-552 AMGCL call): from T = T_in to rtol 1e-12 it
+    matches the sparse direct solve, and at the reference's rtol 1e-6 the
+    oracle's matrix confirms the residual; a matrix with an entry outside
+    the 7-point pattern is rejected."""
+    stack, gamma, u, v = synthetic_case(pkg, seed, nx, ny)
+    o = orc.OracleThermal(pkg._types, stack, gamma, u, v)
+    rp, ci, val, b = o.csr()
+    nz = len(stack["dz"])
+    T0 = np.full(nx * ny * nz, stack["t_in"])
+    print(f"case thermal csr nx={nx} ny={ny} precision={precision}", flush=True)
+    T, (its, rr) = pkg.thermal.solve_csr(nx, ny, nz, rp, ci, val, b, T0, rtol=1e-12, max_iter=3000,
+                                         precision=precision)
+    Td = o.direct().ravel()
+    rise = np.abs(Td - stack["t_in"]).max()
+    print(f"  iterations {its} rel residual {rr:.2e}", flush=True)
+    assert rr <= 1e-12 and o.rel_residual(T.reshape(nz, ny, nx)) < 2e-12
+    assert np.abs(T - Td).max() / rise < 1e-8
+    T6, (its6, rr6) = pkg.thermal.solve_csr(nx, ny, nz, rp, ci, val, b, T0, rtol=1e-6, max_iter=1000,
+                                            precision=precision)
+    assert rr6 <= 1e-6 and o.rel_residual(T6.reshape(nz, ny, nx)) < 1.0001e-6
+    bad_ci = ci.copy()
+    r0 = 3
+    bad_ci[rp[r0]] = (r0 + 2) % (nx * ny * nz)  # not a grid neighbour of row 3
+    with pytest.raises(pkg._capi.CpsError):
+        pkg.thermal.solve_csr(nx, ny, nz, rp, bad_ci, val, b, T0)
