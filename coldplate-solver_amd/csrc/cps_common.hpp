@@ -20,6 +20,14 @@ __device__ inline int xcd_tile() {
   return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
 }
 
+// Streaming store for results the next kernel reads (possibly on another
+// XCD): marked non-temporal, so the line is not kept dirty in this XCD's L2
+// for the end-of-kernel write-back.
+template <class T>
+__device__ __forceinline__ void st_nt(T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
 // Mask decisions in the reference's float arithmetic on its float gamma
 // (a MatrixXf, SIMPLE.h:479-520), so the FP64 path takes the reference's
 // discrete choices bit for bit:

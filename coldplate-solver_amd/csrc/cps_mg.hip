@@ -30,8 +30,23 @@ namespace cps {
 namespace {
 
 constexpr int MBX = 64, MBY = 4, MNT = MBX * MBY;
-constexpr double kOmega = 0.8;
-constexpr double kCoarseScale = 0.5;
+#ifndef CPS_MG_OMEGA1
+#define CPS_MG_OMEGA1 0.8
+#endif
+#ifndef CPS_MG_OMEGA2
+#define CPS_MG_OMEGA2 0.8
+#endif
+#ifndef CPS_MG_COARSE_SCALE
+#define CPS_MG_COARSE_SCALE 0.5
+#endif
+// weights of the two Jacobi sweeps of each smoothing pass (pre: the sweep
+// from zero, then the second; post: the sweep after the prolongation, then
+// the second); the sweeps are polynomials in D^-1 A, so the V-cycle stays
+// symmetric for any pair.  Coarse faces are this share of the covered fine
+// faces' sum (0.5: twice the Galerkin coarse correction).
+constexpr double kOmega1 = CPS_MG_OMEGA1;
+constexpr double kOmega2 = CPS_MG_OMEGA2;
+constexpr double kCoarseScale = CPS_MG_COARSE_SCALE;
 #ifndef CPS_COARSE_THREADS
 #define CPS_COARSE_THREADS 1024
 #endif
@@ -277,14 +292,14 @@ struct Tile {
     return a;
   }
   // one damped-Jacobi sweep src -> dst on the region ring [h, RH-h) x [h, RW-h)
-  __device__ void sweep(const T* src, T* dst, int h, int t) const {
+  __device__ void sweep(const T* src, T* dst, int h, int t, T om) const {
     for (int q = t; q < RN; q += FT) {
       const int lr = q / RW, lc = q - (q / RW) * RW;
       if (lr < h || lr >= RH - h || lc < h || lc >= RW - h) continue;
       const int r = gr(lr), c = gc(lc);
       if (inside(r, c)) {
         const T d = diag(q, r, c);
-        dst[q] = src[q] + (T)kOmega * (sb[q] - ax(src, q, r, c, d)) / d;
+        dst[q] = src[q] + om * (sb[q] - ax(src, q, r, c, d)) / d;
       } else {
         dst[q] = (T)0;
       }
@@ -400,7 +415,7 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
         const int lr = q / RW, lc = q - (q / RW) * RW;
         if (lr >= 1 && lr < RH - 1 && lc >= 1 && lc < RW - 1) {
           const int r = tl.gr(lr), c = tl.gc(lc);
-          s0[q] = tl.inside(r, c) ? (T)kOmega * b / tl.diag(q, r, c) : (T)0;
+          s0[q] = tl.inside(r, c) ? (T)kOmega1 * b / tl.diag(q, r, c) : (T)0;
         }
       }
     }
@@ -410,10 +425,10 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
     if (own) {
       const T rn = sb[q0];
       const T pn = zc + beta * po;
-      xr.xs[k0] = xo + alpha * pn;
-      xr.pnew[k0] = pn;
-      xr.snew[k0] = wo + beta * so;
-      L.b[k0] = rn;
+      st_nt(&xr.xs[k0], (T)(xo + alpha * pn));
+      st_nt(&xr.pnew[k0], pn);
+      st_nt(&xr.snew[k0], (T)(wo + beta * so));
+      st_nt(&L.b[k0], rn);
       e = (double)rn * (double)rn;
     }
     const double sblk = block_sum_t0<FT>(e, sh);  // the barrier after this branch follows
@@ -449,18 +464,18 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
       const int lr = q / RW, lc = q - (q / RW) * RW;
       if (lr < 1 || lr >= RH - 1 || lc < 1 || lc >= RW - 1) continue;
       const int r = tl.gr(lr), c = tl.gc(lc);
-      s0[q] = tl.inside(r, c) ? (T)kOmega * sb[q] / tl.diag(q, r, c) : (T)0;
+      s0[q] = tl.inside(r, c) ? (T)kOmega1 * sb[q] / tl.diag(q, r, c) : (T)0;
     }
     __syncthreads();
   }
-  tl.sweep(s0, s1, 2, t);  // x1 on the halo-1 ring
+  tl.sweep(s0, s1, 2, t, (T)kOmega2);  // x1 on the halo-1 ring
   __syncthreads();
   {  // residual of the tile cell, x1 to memory
     const int lr = HL + threadIdx.y, lc = HL + threadIdx.x, q = lr * RW + lc;
     const int r = tl.gr(lr), c = tl.gc(lc);
     if (tl.inside(r, c)) {
       s0[q] = sb[q] - tl.ax(s1, q, r, c, tl.diag(q, r, c));
-      L.x[L.at(r, c)] = s1[q];
+      st_nt(&L.x[L.at(r, c)], s1[q]);
     } else {
       s0[q] = (T)0;
     }
@@ -470,7 +485,7 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
   const int R = tl.r0 / 2 + cr, C = tl.c0 / 2 + cc;
   if (t < (TY / 2) * (TX / 2) && R < Cl.ny && C < Cl.nx) {
     const int q = (HL + 2 * cr) * RW + HL + 2 * cc;
-    Cl.b[Cl.at(R, C)] = (s0[q] + s0[q + 1]) + (s0[q + RW] + s0[q + RW + 1]);
+    st_nt(&Cl.b[Cl.at(R, C)], (T)((s0[q] + s0[q + 1]) + (s0[q + RW] + s0[q + RW + 1])));
   }
 }
 
@@ -504,17 +519,17 @@ struct Region {
       const int lr = q / W, lc = q - (q / W) * W;
       if (lr < h || lr >= HH - h || lc < h || lc >= W - h) continue;
       const int r = gr(lr), c = gc(lc);
-      dst[q] = inside(r, c) ? (T)kOmega * sb[q] / diag(q, r, c) : (T)0;
+      dst[q] = inside(r, c) ? (T)kOmega1 * sb[q] / diag(q, r, c) : (T)0;
     }
   }
-  __device__ void sweep(const T* src, T* dst, int h, int t) const {
+  __device__ void sweep(const T* src, T* dst, int h, int t, T om) const {
     for (int q = t; q < N; q += FT) {
       const int lr = q / W, lc = q - (q / W) * W;
       if (lr < h || lr >= HH - h || lc < h || lc >= W - h) continue;
       const int r = gr(lr), c = gc(lc);
       if (inside(r, c)) {
         const T d = diag(q, r, c);
-        dst[q] = src[q] + (T)kOmega * (sb[q] - ax(src, q, r, c, d)) / d;
+        dst[q] = src[q] + om * (sb[q] - ax(src, q, r, c, d)) / d;
       } else {
         dst[q] = (T)0;
       }
@@ -613,7 +628,7 @@ __global__ __launch_bounds__(FT) void k_mg_down2(Lv<T> A, Lv<T> B, Lv<T> C, cons
   // level A: x0 on the 8-ring, x1 on the 7-ring, residual on the 6-ring
   ra.first(s0A, 1, t);
   __syncthreads();
-  ra.sweep(s0A, s1A, 2, t);
+  ra.sweep(s0A, s1A, 2, t, (T)kOmega2);
   __syncthreads();
   ra.resid(s1A, s0A, 3, t);
   {  // the footprint's x1 (one cell per thread)
@@ -639,7 +654,7 @@ __global__ __launch_bounds__(FT) void k_mg_down2(Lv<T> A, Lv<T> B, Lv<T> C, cons
   // level B: k_mg_down's recipe on the 32 x 8 tile
   rb.first(s0B, 1, t);
   __syncthreads();
-  rb.sweep(s0B, s1B, 2, t);
+  rb.sweep(s0B, s1B, 2, t, (T)kOmega2);
   __syncthreads();
   if (t < PBX * PBY) {
     const int lr = PHB + t / PBX, lc = PHB + t % PBX, q = lr * PBW + lc;
@@ -682,9 +697,9 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
     // single-reduction PCG: z on the tile and its halo-1 ring (the same
     // arithmetic as the owner blocks'), w = A z on the tile, partials r.z
     // and z.w
-    tl.sweep(s0, s1, 1, t);  // halo-2 ring
+    tl.sweep(s0, s1, 1, t, (T)kOmega1);  // halo-2 ring
     __syncthreads();
-    tl.sweep(s1, s0, 2, t);  // z on the halo-1 ring
+    tl.sweep(s1, s0, 2, t, (T)kOmega2);  // z on the halo-1 ring
     __syncthreads();
     double e = 0.0, e2 = 0.0;
     const int lr = HL + threadIdx.y, lc = HL + threadIdx.x, q = lr * RW + lc;
@@ -693,8 +708,8 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
       const T z = s0[q];
       const T wz = tl.ax(s0, q, r, c, tl.diag(q, r, c));
       const size_t k = L.at(r, c);
-      L.y[k] = z;
-      w[k] = wz;
+      st_nt(&L.y[k], z);
+      st_nt(&w[k], wz);
       e = (double)sb[q] * (double)z;
       e2 = (double)z * (double)wz;
     }
@@ -706,7 +721,7 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
     }
     return;
   }
-  tl.sweep(s0, s1, 2, t);  // halo-1 ring
+  tl.sweep(s0, s1, 2, t, (T)kOmega1);  // halo-1 ring
   __syncthreads();
   double e = 0.0;
   {
@@ -714,8 +729,8 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
     const int r = tl.gr(lr), c = tl.gc(lc);
     if (tl.inside(r, c)) {
       const T d = tl.diag(q, r, c);
-      const T z = s1[q] + (T)kOmega * (sb[q] - tl.ax(s1, q, r, c, d)) / d;
-      L.y[L.at(r, c)] = z;
+      const T z = s1[q] + (T)kOmega2 * (sb[q] - tl.ax(s1, q, r, c, d)) / d;
+      st_nt(&L.y[L.at(r, c)], z);
       e += (double)sb[q] * (double)z;
     }
   }
@@ -809,9 +824,9 @@ __global__ __launch_bounds__(FT) void k_mg_up2(Lv<T> A, Lv<T> B, Lv<T> C, const 
   if (t == 0) sflag = stop0;
   __syncthreads();
   if (sflag) return;  // block-uniform
-  rb.sweep(s0B, s1B, 1, t);  // 2-cell ring
+  rb.sweep(s0B, s1B, 1, t, (T)kOmega1);  // 2-cell ring
   __syncthreads();
-  rb.sweep(s1B, s0B, 2, t);  // B's result on the 1-cell ring
+  rb.sweep(s1B, s0B, 2, t, (T)kOmega2);  // B's result on the 1-cell ring
   __syncthreads();
   // A's x2 = x1 + P e on the 2-cell ring (the 3-cell ring is never read)
   for (int q = t; q < UAN; q += FT) {
@@ -821,14 +836,14 @@ __global__ __launch_bounds__(FT) void k_mg_up2(Lv<T> A, Lv<T> B, Lv<T> C, const 
     if (ra.inside(r, c)) s0A[q] = s0A[q] + s0B[((r >> 1) - rB0 + HL) * UBW + (c >> 1) - cB0 + HL];
   }
   __syncthreads();
-  ra.sweep(s0A, s1A, 2, t);  // 1-cell ring
+  ra.sweep(s0A, s1A, 2, t, (T)kOmega1);  // 1-cell ring
   __syncthreads();
   {
     const int lr = HL + threadIdx.y, lc = HL + threadIdx.x, q = lr * UAW + lc;
     const int r = ra.gr(lr), c = ra.gc(lc);
     if (ra.inside(r, c)) {
       const T d = ra.diag(q, r, c);
-      A.y[A.at(r, c)] = s1A[q] + (T)kOmega * (sbA[q] - ra.ax(s1A, q, r, c, d)) / d;
+      A.y[A.at(r, c)] = s1A[q] + (T)kOmega2 * (sbA[q] - ra.ax(s1A, q, r, c, d)) / d;
     }
   }
 }
@@ -1057,7 +1072,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
     if (t < na) sA[t] = va;
   }
   __syncthreads();
-  const T om = (T)kOmega;
+  const T om1 = (T)kOmega1, om2 = (T)kOmega2;
   // A x at cell k = (r, c) of level l, x given by the functor xv(k, r, c).
   // Branch-free: a missing neighbour is read at the cell itself and enters
   // with a zero face, so every LDS read of the cell issues at once.
@@ -1081,9 +1096,9 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
   // residual into the next level's right side; cell (r, c) of level l
   auto down_sweep = [&](int l, int r, int c) {
     const int o = d.off[l], nx = d.nx[l];
-    auto x0 = [&](int k, int, int) -> T { return om * sb[k] * si[k]; };
+    auto x0 = [&](int k, int, int) -> T { return om1 * sb[k] * si[k]; };
     const int k = o + r * nx + c;
-    sy[k] = x0(k, r, c) + om * (sb[k] - ax(x0, l, k, r, c)) * si[k];
+    sy[k] = x0(k, r, c) + om2 * (sb[k] - ax(x0, l, k, r, c)) * si[k];
   };
   auto restrict_cell = [&](int l, int r, int c) {
     const int o = d.off[l], nx = d.nx[l], ny = d.ny[l], co = d.off[l + 1], cx = d.nx[l + 1];
@@ -1108,13 +1123,13 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
     const int o = d.off[l], nx = d.nx[l], co = d.off[l + 1], cx = d.nx[l + 1];
     auto x2 = [&](int k, int rr, int cc) -> T { return sy[k] + sy[co + (rr >> 1) * cx + (cc >> 1)]; };
     const int k = o + r * nx + c;
-    sx[k] = x2(k, r, c) + om * (sb[k] - ax(x2, l, k, r, c)) * si[k];
+    sx[k] = x2(k, r, c) + om1 * (sb[k] - ax(x2, l, k, r, c)) * si[k];
   };
   auto up_sweep2 = [&](int l, int r, int c) {
     const int o = d.off[l], nx = d.nx[l];
     auto x3 = [&](int k, int, int) -> T { return sx[k]; };
     const int k = o + r * nx + c;
-    sy[k] = x3(k, r, c) + om * (sb[k] - ax(x3, l, k, r, c)) * si[k];
+    sy[k] = x3(k, r, c) + om2 * (sb[k] - ax(x3, l, k, r, c)) * si[k];
   };
   for (int l = 0; l < wl; ++l) {
     CELLS(l) { down_sweep(l, r, c); }
