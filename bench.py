@@ -186,12 +186,13 @@ def grads_leg(pkg, nx=512, ny=256, forward_steps=4000, reps=3, rtol=1e-8):
     return out
 
 
-def batch_leg(pkg, steps=500, n_fields=48):
+def batch_leg(pkg, steps=500, n_fields=48, plan=None):
     """48 density fields, sizes drawn from {256x128, 512x256, 768x384,
     1024x512}, sigma ~ U[4, 12], per-field seeds, the headline knobs
     (FP32-inner GMG-PCG, p_rtol 1e-3, <= 6 PCG iterations, <= 8 Jacobi
-    sweeps): one hipGraph launch per stream and batch iteration, stream count
-    picked by the autotuner, `steps` iterations timed."""
+    sweeps): plan (class plan, one launch per kernel covering a size class,
+    or per-field chains on several streams) and graph unroll picked by the
+    autotuner, `steps` iterations timed."""
     T = pkg._types
     fl = []
     for cfg, gamma, inlet in pkg.configs.batch_sweep(n_fields=n_fields):
@@ -200,7 +201,14 @@ def batch_leg(pkg, steps=500, n_fields=48):
         cfg.max_momentum_iter = 8
         fl.append((cfg, gamma, inlet))
     b = pkg.Batch(fl)
-    ns, us_tune = b.tune(streams=0, reps=3)
+    if plan:  # "class:U" or "per_field:S" instead of the tuner's pick
+        kind, n = plan.split(":")
+        b.plan(kind, unroll=int(n)) if kind == "class" else b.plan(kind, streams=int(n))
+        b.step(int(n) if kind == "class" else 1)  # warm-up
+    else:
+        b.tune(streams=0, reps=3)
+    ns = b.streams
+    plan = b.plan_info()
     b.synchronize()
     t0 = time.perf_counter()
     b.step(steps)
@@ -209,14 +217,16 @@ def batch_leg(pkg, steps=500, n_fields=48):
     cells = sum(c.nx * c.ny for c, _, _ in fl)
     finite = all(math.isfinite(b.stats(k)["error"]) for k in range(b.n))
     b.close()
-    return dict(fields=n_fields, steps=steps, seconds=dt, streams=ns, batch_it_s=steps / dt,
+    return dict(fields=n_fields, steps=steps, seconds=dt, streams=ns, plan=plan["plan"], unroll=plan["unroll"],
+                size_classes=plan["classes"], batch_it_s=steps / dt,
                 field_it_s=n_fields * steps / dt, cell_updates_per_s=cells * steps / dt, total_cells=cells,
                 all_finite=finite)
 
 
 def grads_batch_leg(pkg, n_fields=32, nx=512, ny=256, forward_steps=1500, rtol=1e-8, workers=8):
     """32 seeded 512 x 256 density fields brought near their fixed points as
-    one batch (per-stream graphs), then all 32 d(dP)/d(gamma) computed at
+    one batch (one size class: each kernel launch covers the 32 fields,
+    graphs of 10 iterations), then all 32 d(dP)/d(gamma) computed at
     once: one host thread per field in flight, each adjoint on its field's
     stream.  Timed: the 32 gradients."""
     T = pkg._types
@@ -228,7 +238,7 @@ def grads_batch_leg(pkg, n_fields=32, nx=512, ny=256, forward_steps=1500, rtol=1
         cfg.momentum_tol, cfg.max_momentum_iter = 1e-13, 30
         fl.append((cfg, gamma, inlet))
     b = pkg.Batch(fl)
-    b.tune(streams=8, reps=2)
+    b.plan("class", unroll=10)
     t0 = time.perf_counter()
     b.step(forward_steps)
     b.synchronize()
@@ -344,6 +354,9 @@ def main():
     ap.add_argument("--no-grads", action="store_true", help="skip the d(dP)/d(gamma) leg")
     ap.add_argument("--no-thermal", action="store_true", help="skip the 3-D thermal leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-field batch leg")
+    ap.add_argument("--batch-plan", default=None,
+                    help="batch leg plan instead of the autotuner's: class:UNROLL or per_field:STREAMS")
+    ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--copy-calibration", action="store_true", help="only copy 1 GiB on the device (4x) and exit")
     ap.add_argument("--leg", choices=["grads", "grads_batch", "thermal", "batch", "time_to_dp", "parity"],
                     help="run only this leg and print its results (for profiling)")
@@ -375,7 +388,8 @@ def main():
 
     pkg = entry.load_package()
     if args.leg:
-        legs = {"grads": grads_leg, "grads_batch": grads_batch_leg, "thermal": thermal_leg, "batch": batch_leg,
+        legs = {"grads": grads_leg, "grads_batch": grads_batch_leg, "thermal": thermal_leg,
+                "batch": lambda p: batch_leg(p, steps=args.batch_steps, plan=args.batch_plan),
                 "time_to_dp": time_to_dp_leg,
                 "parity": lambda p: oracle_parity(p, args.nx, args.ny, args.parity_iters)}
         res = legs[args.leg](pkg)
@@ -399,7 +413,7 @@ def main():
 
     grads = None if args.no_grads else grads_leg(pkg)
     thermal = None if args.no_thermal else thermal_leg(pkg, cpu=not args.no_cpu)
-    batch = None if args.no_batch else batch_leg(pkg)
+    batch = None if args.no_batch else batch_leg(pkg, steps=args.batch_steps, plan=args.batch_plan)
     grads_batch = None if args.no_grads else grads_batch_leg(pkg)
     tdp = None if args.no_fp64 else time_to_dp_leg(pkg, nx, ny)
     cpu = None if args.no_cpu else cpu_baseline(pkg, nx, ny, args.cpu_seconds)
@@ -456,7 +470,8 @@ def main():
         },
         "batch": None if batch is None else {
             "workload": "48 density fields, sizes from {256x128, 512x256, 768x384, 1024x512}, sigma U[4,12], "
-                        "headline knobs; one hipGraph launch per stream per batch iteration",
+                        "headline knobs; autotuned plan (class plan: one launch per kernel over all fields "
+                        "of a grid size, one hipGraph per size class)",
             **{k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in batch.items()},
         },
         "thermal": None if thermal is None else {

@@ -17,6 +17,8 @@ CPS_PRESSURE_AMGCL = 2        # falls back to GMG-PCG, stop ||r|| <= p_tol ||b||
 CPS_PRESSURE_LDLT = 3         # falls back to GMG-PCG solved to 1e-12 ||b||
 CPS_PRESSURE_AMGCL_CUDA = 4   # the reference default; as type 2
 CPS_PRESSURE_GMG = 5
+CPS_BATCH_CLASS = 0
+CPS_BATCH_PER_FIELD = 1
 
 CPS_ADJ_PRECOND_SIMPLE = 0
 CPS_ADJ_PRECOND_LSC_LINE = 1

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <cstdio>
 #include <string>
 
@@ -79,11 +80,40 @@ int guard(F&& fn) {
 }
 
 // ---------------------------------------------------------------------------
+// Class-batched launches: the fields of one size class keep their buffers
+// in one allocation, field k's copy of every buffer sitting k * zs bytes
+// after field 0's.  A kernel launched with gridDim.z = fields gets field 0's
+// pointers and zs, and block (x, y, z) shifts them to field z at entry
+// (nullptr, a "not used" marker, stays nullptr).  gridDim.z = 1 shifts by 0.
+__device__ __forceinline__ size_t zoff(size_t zs) { return (size_t)blockIdx.z * zs; }
+template <class T>
+__device__ __forceinline__ T* zsh(T* p, size_t off) {
+  return p ? reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) + off) : p;
+}
+
+// Bump allocator over one region: while a thread has an arena installed
+// (ArenaScope), DevBuf::alloc takes its memory from it instead of hipMalloc,
+// so two solvers built the same way under arenas at base and base + zs hold
+// every buffer zs bytes apart.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, off = 0;
+};
+inline thread_local Arena* t_arena = nullptr;
+struct ArenaScope {
+  Arena* prev;
+  explicit ArenaScope(Arena* a) : prev(t_arena) { t_arena = a; }
+  ~ArenaScope() { t_arena = prev; }
+  ArenaScope(const ArenaScope&) = delete;
+  ArenaScope& operator=(const ArenaScope&) = delete;
+};
+
 // device buffer (RAII)
 template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  bool owned = false;  // from hipMalloc (else from an arena, freed with it)
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
@@ -91,13 +121,24 @@ struct DevBuf {
   void alloc(size_t count) {
     release();
     if (count == 0) return;
-    CPS_HIP(hipMalloc(&p, count * sizeof(T)));
+    if (Arena* a = t_arena) {
+      const size_t off = (a->off + 255) & ~(size_t)255;
+      if (off + count * sizeof(T) > a->cap)
+        throw Error{CPS_ERR_STATE, "batch arena exhausted (" + std::to_string(a->cap) + " bytes)"};
+      p = reinterpret_cast<T*>(a->base + off);
+      a->off = off + count * sizeof(T);
+      owned = false;
+    } else {
+      CPS_HIP(hipMalloc(&p, count * sizeof(T)));
+      owned = true;
+    }
     n = count;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     n = 0;
+    owned = false;
   }
   size_t bytes() const { return n * sizeof(T); }
 };

@@ -64,7 +64,15 @@ template <class T>
 struct Lv {  // device view of one level; cell (r,c) at off + r*pitch + c
   int nx, ny, pitch, off;
   T *x, *y, *b, *ex, *ey;
+  size_t zs;  // byte stride between the fields of a class-batched launch
   __device__ size_t at(int r, int c) const { return (size_t)off + (size_t)r * pitch + c; }
+  __device__ void zshift(size_t o) {
+    x = zsh(x, o);
+    y = zsh(y, o);
+    b = zsh(b, o);
+    ex = zsh(ex, o);
+    ey = zsh(ey, o);
+  }
 };
 
 // Face coefficients around cell k: ex(k) is the east face (the Dirichlet
@@ -131,6 +139,8 @@ struct HierLevels {
 };
 template <class T>
 __global__ __launch_bounds__(HT) void k_mg_hier(Geo g, const double* dE, const double* dN, HierLevels<T> H) {
+  const size_t zo_ = zoff(g.zs);
+  dE = zsh(dE, zo_); dN = zsh(dN, zo_);  // the levels' face pointers are shifted where they are used
   // even levels in the level-0-sized buffers, odd levels in the quarter-sized ones
   __shared__ T sEa[HY * HX], sNa[HY * HX], sEb[(HY / 2) * (HX / 2)], sNb[(HY / 2) * (HX / 2)];
   const int t = threadIdx.x;
@@ -147,8 +157,8 @@ __global__ __launch_bounds__(HT) void k_mg_hier(Geo g, const double* dE, const d
       const double aE = g.rho * dE[f] * g.hy;
       const double aN = (i < M - 1) ? g.rho * dN[f] * g.hx : 0.0;
       const size_t k = L.at(r, c);
-      L.ex[k] = (T)aE;
-      L.ey[k] = (T)aN;
+      zsh(L.ex, zo_)[k] = (T)aE;
+      zsh(L.ey, zo_)[k] = (T)aN;
       sEa[q] = (T)aE;
       sNa[q] = (T)aN;
     }
@@ -182,8 +192,8 @@ __global__ __launch_bounds__(HT) void k_mg_hier(Geo g, const double* dE, const d
       e = (T)kCoarseScale * e;
       n = (T)kCoarseScale * n;
       const size_t k = Cl.at(R, C);
-      Cl.ex[k] = e;
-      Cl.ey[k] = n;
+      zsh(Cl.ex, zo_)[k] = e;
+      zsh(Cl.ey, zo_)[k] = n;
       ce[lr * tw + lc] = e;
       cn[lr * tw + lc] = n;
     }
@@ -192,6 +202,8 @@ __global__ __launch_bounds__(HT) void k_mg_hier(Geo g, const double* dE, const d
 
 template <class T>
 __global__ __launch_bounds__(MNT) void k_mg_coarsen(Lv<T> F, Lv<T> Cl) {
+  const size_t zo_ = zoff(F.zs);
+  F.zshift(zo_); Cl.zshift(zo_);
   const int C = blockIdx.x * MBX + threadIdx.x, R = blockIdx.y * MBY + threadIdx.y;
   if (R >= Cl.ny || C >= Cl.nx) return;
   const size_t k = Cl.at(R, C);
@@ -238,6 +250,10 @@ struct PcgTest {
   int nb, it;
   PcgScal* ps;
   double abs_tol, rtol;  // it == 0: the target max(abs_tol, rtol ||r0||) is set here
+  __device__ void zshift(size_t o) {
+    part = zsh(part, o);
+    ps = zsh(ps, o);
+  }
 };
 
 // PCG update of the previous iteration j = it - 1 carried by the fine-level
@@ -262,6 +278,20 @@ struct XrFuse {
   double* err;       // ||r_new||^2 partials
   int nb, it;        // it: this V-cycle; the update is iteration it - 1's
   PcgScal* ps;
+  __device__ void zshift(size_t o) {
+    rold = zsh(rold, o);
+    w = zsh(w, o);
+    sold = zsh(sold, o);
+    snew = zsh(snew, o);
+    z = zsh(z, o);
+    pold = zsh(pold, o);
+    pnew = zsh(pnew, o);
+    xs = zsh(xs, o);
+    rz = zsh(rz, o);
+    zw = zsh(zw, o);
+    err = zsh(err, o);
+    ps = zsh(ps, o);
+  }
 };
 
 // One thread per tile cell: blocks of TX x TY = 64 x 16 threads.
@@ -343,6 +373,8 @@ __device__ inline void tile_load(const Lv<T>& L, const Tile<T>& tl, int t, T* sx
 // the residuals b - A x1 over each 2x2 aggregate
 template <class T>
 __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* stop, PcgTest test, XrFuse<T> xr) {
+  const size_t zo_ = zoff(L.zs);
+  L.zshift(zo_); Cl.zshift(zo_); stop = zsh(stop, zo_); test.zshift(zo_); xr.zshift(zo_);
   __shared__ T sb[RN], se[RN], sn[RN], s0[RN], s1[RN];
   __shared__ double sh[FT / kWave + 1];
   __shared__ double sh2[2 * (FT / kWave)];
@@ -566,6 +598,8 @@ __host__ __device__ constexpr size_t pair_smem() {
 }
 template <class T>
 __global__ __launch_bounds__(FT) void k_mg_down2(Lv<T> A, Lv<T> B, Lv<T> C, const int* stop) {
+  const size_t zo_ = zoff(A.zs);
+  A.zshift(zo_); B.zshift(zo_); C.zshift(zo_); stop = zsh(stop, zo_);
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   T* sbA = reinterpret_cast<T*>(smem_raw);
   T* seA = sbA + PAN;
@@ -683,6 +717,8 @@ __global__ __launch_bounds__(FT) void k_mg_down2(Lv<T> A, Lv<T> B, Lv<T> C, cons
 template <class T>
 __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, const int* stop, T* w,
                                              double* part2) {
+  const size_t zo_ = zoff(L.zs);
+  L.zshift(zo_); Cl.zshift(zo_); part = zsh(part, zo_); stop = zsh(stop, zo_); w = zsh(w, zo_); part2 = zsh(part2, zo_);
   __shared__ T sb[RN], se[RN], sn[RN], s0[RN], s1[RN];
   __shared__ int sflag;
   const int t = threadIdx.x + threadIdx.y * TX;
@@ -761,6 +797,8 @@ __host__ __device__ constexpr size_t up_pair_smem() {
 }
 template <class T>
 __global__ __launch_bounds__(FT) void k_mg_up2(Lv<T> A, Lv<T> B, Lv<T> C, const int* stop) {
+  const size_t zo_ = zoff(A.zs);
+  A.zshift(zo_); B.zshift(zo_); C.zshift(zo_); stop = zsh(stop, zo_);
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   T* sbA = reinterpret_cast<T*>(smem_raw);
   T* seA = sbA + UAN;
@@ -875,6 +913,8 @@ struct CoarseDesc {
 // other kernel reads the top level's faces).
 template <class T>
 __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, Lv<T> par, CoarseDesc d, T* cf) {
+  const size_t zo_ = zoff(top.zs);
+  top.zshift(zo_); par.zshift(zo_); cf = zsh(cf, zo_);
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   T* se = reinterpret_cast<T*>(smem_raw);
   T* sn = se + d.total;
@@ -1009,6 +1049,8 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, L
 template <class T>
 __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseDesc d, const T* cf, const int* stop,
                                                               PcgTest test) {
+  const size_t zo_ = zoff(top.zs);
+  top.zshift(zo_); cf = zsh(cf, zo_); stop = zsh(stop, zo_); test.zshift(zo_);
   if (*stop) return;
   if (test.part) {  // the PCG stop test when the fine down kernel carries the update
     __shared__ double shr[kCoarseThreads / kWave + 1];
@@ -1163,6 +1205,7 @@ enum MgSlot { S_RR0 = 0, S_RZ = 1, S_PAP = 2, S_ERR = 3, S_IR = 4, S_ZW = 5, S_C
 
 struct Grid0 {  // tile grid of the level-0 kernels (one TX x TY tile per block)
   int gx, gy, NB;
+  size_t zs;  // byte stride between the fields of a class-batched launch
 };
 
 // one cell per thread: blocks of TX x TY threads over the tile grid
@@ -1181,6 +1224,8 @@ __device__ inline int tid() { return threadIdx.x + threadIdx.y * TX; }
 template <class T, class S>
 __global__ __launch_bounds__(FT) void k_pcg_start(Grid0 G, Lv<T> L, const S* src, T* xv, T* rv, PcgScal* ps,
                                                   const int* outer_stop, double* part, S* keep, PcgScal* reset) {
+  const size_t zo_ = zoff(G.zs);
+  L.zshift(zo_); src = zsh(src, zo_); xv = zsh(xv, zo_); rv = zsh(rv, zo_); ps = zsh(ps, zo_); outer_stop = zsh(outer_stop, zo_); part = zsh(part, zo_); keep = zsh(keep, zo_); reset = zsh(reset, zo_);
   __shared__ double sh[FT / kWave + 1];
   if (outer_stop && *outer_stop) {
     if (bid() == 0 && tid() == 0) { ps->stop = 1; ps->iters = 0; }
@@ -1215,6 +1260,8 @@ __global__ __launch_bounds__(FT) void k_pcg_start(Grid0 G, Lv<T> L, const S* src
 template <class T>
 __global__ __launch_bounds__(FT) void k_pcg_ap(Grid0 G, Lv<T> L, const T* z, const T* po, T* pn, T* Ap, int it,
                                                PcgScal* ps, double* part) {
+  const size_t zo_ = zoff(G.zs);
+  L.zshift(zo_); z = zsh(z, zo_); po = zsh(po, zo_); pn = zsh(pn, zo_); Ap = zsh(Ap, zo_); ps = zsh(ps, zo_); part = zsh(part, zo_);
   __shared__ double sh[FT / kWave + 1];
   __shared__ int sflag;
   const int t = tid();
@@ -1260,6 +1307,8 @@ __global__ __launch_bounds__(FT) void k_pcg_ap(Grid0 G, Lv<T> L, const T* z, con
 template <class T>
 __global__ __launch_bounds__(FT) void k_pcg_xr(Grid0 G, Lv<T> L, T* xv, T* rv, const T* pn, const T* Ap, int it,
                                                PcgScal* ps, double* part) {
+  const size_t zo_ = zoff(G.zs);
+  L.zshift(zo_); xv = zsh(xv, zo_); rv = zsh(rv, zo_); pn = zsh(pn, zo_); Ap = zsh(Ap, zo_); ps = zsh(ps, zo_); part = zsh(part, zo_);
   __shared__ double sh[FT / kWave + 1];
   __shared__ int sflag;
   const int t = tid();
@@ -1295,6 +1344,8 @@ __global__ __launch_bounds__(FT) void k_pcg_xr(Grid0 G, Lv<T> L, T* xv, T* rv, c
 // alpha p, r -= alpha s in place in r_j's buffer), partial ||r||^2
 template <class T>
 __global__ __launch_bounds__(FT) void k_pcg_cg_last(Grid0 G, Lv<T> L, XrFuse<T> xr, T* rv) {
+  const size_t zo_ = zoff(G.zs);
+  L.zshift(zo_); xr.zshift(zo_); rv = zsh(rv, zo_);
   __shared__ double sh[FT / kWave + 1];
   __shared__ int sflag;
   const int t = tid();
@@ -1330,6 +1381,8 @@ __global__ __launch_bounds__(FT) void k_pcg_cg_last(Grid0 G, Lv<T> L, XrFuse<T> 
 
 // after K launched iterations: stop test of iteration K-1 (one block)
 __global__ __launch_bounds__(FT) void k_pcg_end(Grid0 G, int K, PcgScal* ps, const double* part) {
+  const size_t zo_ = zoff(G.zs);
+  ps = zsh(ps, zo_); part = zsh(part, zo_);
   __shared__ double sh[FT / kWave + 1];
   __shared__ int shi;
   if (block_read(&ps->stop, &shi)) return;
@@ -1344,6 +1397,8 @@ __global__ __launch_bounds__(FT) void k_pcg_end(Grid0 G, int K, PcgScal* ps, con
 // FP64 refinement: stage 0: x64 += e; stage 1: r64 = b64 - A x64 with the
 // FP64 fine operator from dE, dN and the partial ||r64||^2.
 __global__ __launch_bounds__(MNT) void k_ir_add(Lv<float> L, const float* e, double* x64, int P, const int* stop) {
+  const size_t zo_ = zoff(L.zs);
+  L.zshift(zo_); e = zsh(e, zo_); x64 = zsh(x64, zo_); stop = zsh(stop, zo_);
   if (*stop) return;
   const int c = blockIdx.x * MBX + threadIdx.x, r = blockIdx.y * MBY + threadIdx.y;
   if (r >= L.ny || c >= L.nx) return;
@@ -1364,10 +1419,19 @@ struct CgFinal {
   const double* zw;
   const PcgScal* ps;
   int j;  // -1: no update
+  __device__ void zshift(size_t o) {
+    z = zsh(z, o);
+    pold = zsh(pold, o);
+    rz = zsh(rz, o);
+    zw = zsh(zw, o);
+    ps = zsh(ps, o);
+  }
 };
 __global__ __launch_bounds__(FT) void k_ir_resid(Grid0 G, Geo g, Lv<float> L, const double* dE, const double* dN,
                                                  double* x64, const double* b64, double* r64, const int* stop,
                                                  double* part, const float* e, int first, CgFinal cf) {
+  const size_t zo_ = zoff(G.zs);
+  L.zshift(zo_); dE = zsh(dE, zo_); dN = zsh(dN, zo_); x64 = zsh(x64, zo_); b64 = zsh(b64, zo_); r64 = zsh(r64, zo_); stop = zsh(stop, zo_); part = zsh(part, zo_); e = zsh(e, zo_); cf.zshift(zo_);
   __shared__ double sh[FT / kWave + 1];
   __shared__ int sfl;
   if (*stop) return;
@@ -1428,6 +1492,8 @@ __global__ __launch_bounds__(FT) void k_ir_resid(Grid0 G, Geo g, Lv<float> L, co
 __global__ __launch_bounds__(FT) void k_ir_check(Grid0 G, PcgScal* outer, PcgScal* first, int o,
                                                  double abs_tol, double rtol, const double* part, IterState* st,
                                                  int end_K) {
+  const size_t zo_ = zoff(G.zs);
+  outer = zsh(outer, zo_); first = zsh(first, zo_); part = zsh(part, zo_); st = zsh(st, zo_);
   __shared__ double sh[FT / kWave + 1];
   __shared__ int shi;
   if (end_K > 0 && !block_read(&first->stop, &shi)) {
@@ -1454,7 +1520,9 @@ __global__ __launch_bounds__(FT) void k_ir_check(Grid0 G, PcgScal* outer, PcgSca
   }
 }
 
-__global__ void k_copy_state(IterState* st, const PcgScal* inner, const PcgScal* outer, int mixed) {
+__global__ void k_copy_state(IterState* st, const PcgScal* inner, const PcgScal* outer, int mixed, size_t zs) {
+  const size_t zo_ = zoff(zs);
+  st = zsh(st, zo_); inner = zsh(inner, zo_); outer = zsh(outer, zo_);
   if (threadIdx.x == 0) {
     st->pcg_iters = inner->iters;
     st->err = mixed ? outer->rr : inner->rr;
@@ -1464,6 +1532,8 @@ __global__ void k_copy_state(IterState* st, const PcgScal* inner, const PcgScal*
 
 template <class T>
 __global__ __launch_bounds__(MNT) void k_copy_out(Lv<T> L, const T* src, double* x64, int P) {
+  const size_t zo_ = zoff(L.zs);
+  L.zshift(zo_); src = zsh(src, zo_); x64 = zsh(x64, zo_);
   const int c = blockIdx.x * MBX + threadIdx.x, r = blockIdx.y * MBY + threadIdx.y;
   if (r >= L.ny || c >= L.nx) return;
   x64[(size_t)(r + 1) * P + c + 1] = (double)src[L.at(r, c)];
@@ -1558,6 +1628,7 @@ struct MGImpl {
     G0.gx = ceil_div(lv[0]->nx, TX);
     G0.gy = ceil_div(lv[0]->ny, TY);
     G0.NB = G0.gx * G0.gy;
+    G0.zs = ctx->zs;
     // every fused PCG reduction reads its block partials two per thread
     // (parts_load), so the fine level may have at most 2 * FT tiles
     CPS_CHECK(G0.NB <= 2 * FT && G0.NB <= 2 * kCoarseThreads, CPS_ERR_ARG,
@@ -1576,6 +1647,7 @@ struct MGImpl {
   Lv<T> view(int l) {
     Lv<T> v = lv[l]->view();
     if (l == 0) v.b = b0cur;
+    v.zs = ctx->zs;
     return v;
   }
   T* rbuf(int it) { return (it & 1) ? r1.p : r0.p; }
@@ -1598,14 +1670,14 @@ struct MGImpl {
     HierLevels<T> H{};
     H.n = std::min((int)lv.size() - 1, kHierLevels);
     for (int l = 0; l <= H.n; ++l) H.l[l] = lv[l]->view();
-    k_mg_hier<T><<<dim3(ceil_div(lv[0]->nx, HX), ceil_div(lv[0]->ny, HY)), HT, 0, s>>>(g, dE, dN, H);
+    k_mg_hier<T><<<ctx->zg(dim3(ceil_div(lv[0]->nx, HX), ceil_div(lv[0]->ny, HY))), HT, 0, s>>>(g, dE, dN, H);
     // deeper grid levels one launch each; the coarse top's faces, if not
     // built above, are formed by the coarse setup from its parent
     const bool topFromParent = nGrid > H.n;
     for (int l = H.n + 1; l < nGrid; ++l)
-      k_mg_coarsen<T><<<lv[l]->grid(), blk(), 0, s>>>(lv[l - 1]->view(), lv[l]->view());
+      k_mg_coarsen<T><<<ctx->zg(lv[l]->grid()), blk(), 0, s>>>(view(l - 1), view(l));
     const Lv<T> none{};
-    k_mg_coarse_setup<T><<<1, kCoarseThreads, coarseSetupSmem, s>>>(view(nGrid), topFromParent ? view(nGrid - 1) : none,
+    k_mg_coarse_setup<T><<<ctx->zg(1), kCoarseThreads, coarseSetupSmem, s>>>(view(nGrid), topFromParent ? view(nGrid - 1) : none,
                                                                     cd, cf.p);
     CPS_HIP(hipGetLastError());
   }
@@ -1621,7 +1693,7 @@ struct MGImpl {
     const PcgTest test{part.p + (size_t)(it == 0 ? S_RR0 : S_ERR) * G0.NB, G0.NB, it, scal.p, abs_tol_, rtol_};
     for (int l = 0; l < nGrid; ++l) {
       if (l > 0 && l + 1 < nGrid && pairOk) {  // two mid levels in one launch
-        k_mg_down2<T><<<dim3(ceil_div(lv[l + 1]->nx, PBX), ceil_div(lv[l + 1]->ny, PBY)), dim3(TX, TY),
+        k_mg_down2<T><<<ctx->zg(dim3(ceil_div(lv[l + 1]->nx, PBX), ceil_div(lv[l + 1]->ny, PBY))), dim3(TX, TY),
                         pair_smem<T>(), s>>>(view(l), view(l + 1), view(l + 2), stop);
         ++l;
         continue;
@@ -1631,18 +1703,18 @@ struct MGImpl {
       const PcgTest dt = (l == 0 && !fused) ? test : none;
       // the timed launches are the fused ones when fusing (the byte count is theirs)
       if (l == 0 && (!fused || it > 0) && timer->active(KT_MG_SMOOTH0))
-        timer->ext_launch(k_mg_down<T>, tgrid(l), dim3(TX, TY), 0, s, view(l), view(l + 1), stop, dt, xr);
+        timer->ext_launch(k_mg_down<T>, ctx->zg(tgrid(l)), dim3(TX, TY), 0, s, view(l), view(l + 1), stop, dt, xr);
       else
-        k_mg_down<T><<<tgrid(l), dim3(TX, TY), 0, s>>>(view(l), view(l + 1), stop, dt, xr);
+        k_mg_down<T><<<ctx->zg(tgrid(l)), dim3(TX, TY), 0, s>>>(view(l), view(l + 1), stop, dt, xr);
     }
-    k_mg_coarse<T><<<1, kCoarseThreads, coarseSmem, s>>>(view(nGrid), cd, cf.p, stop, fused ? test : none);
+    k_mg_coarse<T><<<ctx->zg(1), kCoarseThreads, coarseSmem, s>>>(view(nGrid), cd, cf.p, stop, fused ? test : none);
     for (int l = nGrid - 1; l >= 0; --l) {
       if (l >= 2 && pairOk) {  // two mid levels in one launch
-        k_mg_up2<T><<<tgrid(l - 1), dim3(TX, TY), up_pair_smem<T>(), s>>>(view(l - 1), view(l), view(l + 1), stop);
+        k_mg_up2<T><<<ctx->zg(tgrid(l - 1)), dim3(TX, TY), up_pair_smem<T>(), s>>>(view(l - 1), view(l), view(l + 1), stop);
         --l;
         continue;
       }
-      k_mg_up<T><<<tgrid(l), dim3(TX, TY), 0, s>>>(view(l), view(l + 1),
+      k_mg_up<T><<<ctx->zg(tgrid(l)), dim3(TX, TY), 0, s>>>(view(l), view(l + 1),
                                                   l == 0 ? part.p + (size_t)S_RZ * G0.NB : nullptr, stop,
                                                   l == 0 && fused ? wz.p : nullptr,
                                                   part.p + (size_t)S_ZW * G0.NB);
@@ -1677,7 +1749,7 @@ struct MGImpl {
     abs_tol_ = abs_tol;
     rtol_ = rtol;
     b0cur = r0.p;
-    k_pcg_start<T, double><<<grid0(), blk0(), 0, s>>>(G0, L, rhs, xs.p, r0.p, ps, outer_stop, part.p, keep, reset);
+    k_pcg_start<T, double><<<ctx->zg(grid0()), blk0(), 0, s>>>(G0, L, rhs, xs.p, r0.p, ps, outer_stop, part.p, keep, reset);
     // p_old of iteration 0 is pa: zero since creation, finite afterwards, and
     // multiplied by beta = 0
     for (int it = 0; it < maxit; ++it) {
@@ -1688,15 +1760,15 @@ struct MGImpl {
         T* po = (it & 1) ? pb.p : pa.p;
         T* pn = pnew(it);
         if (timer->active(KT_PCG_STENCIL))
-          timer->ext_launch(k_pcg_ap<T>, grid0(), blk0(), 0, s, G0, Lc, (const T*)lv[0]->y.p, (const T*)po, pn,
+          timer->ext_launch(k_pcg_ap<T>, ctx->zg(grid0()), blk0(), 0, s, G0, Lc, (const T*)lv[0]->y.p, (const T*)po, pn,
                             Ap.p, it, ps, part.p);
         else
-          k_pcg_ap<T><<<grid0(), blk0(), 0, s>>>(G0, Lc, lv[0]->y.p, po, pn, Ap.p, it, ps, part.p);
-        k_pcg_xr<T><<<grid0(), blk0(), 0, s>>>(G0, Lc, xs.p, r0.p, pn, Ap.p, it, ps, part.p);
+          k_pcg_ap<T><<<ctx->zg(grid0()), blk0(), 0, s>>>(G0, Lc, lv[0]->y.p, po, pn, Ap.p, it, ps, part.p);
+        k_pcg_xr<T><<<ctx->zg(grid0()), blk0(), 0, s>>>(G0, Lc, xs.p, r0.p, pn, Ap.p, it, ps, part.p);
       }
       CPS_HIP(hipGetLastError());
       if (sync_every > 0 && (it + 1) % sync_every == 0 && it + 1 < maxit) {
-        k_pcg_end<<<1, blk0(), 0, s>>>(G0, it + 1, ps, part.p);
+        k_pcg_end<<<ctx->zg(1), blk0(), 0, s>>>(G0, it + 1, ps, part.p);
         int stop = 0;
         CPS_HIP(hipMemcpyAsync(&stop, &ps->stop, sizeof(int), hipMemcpyDeviceToHost, s));
         CPS_HIP(hipStreamSynchronize(s));
@@ -1705,8 +1777,8 @@ struct MGImpl {
     }
     // the update of the last iteration (in place in its residual buffer)
     if (fused && maxit > 0 && last)
-      k_pcg_cg_last<T><<<grid0(), blk0(), 0, s>>>(G0, view(0), cg_update(maxit - 1), rbuf(maxit - 1));
-    if (end) k_pcg_end<<<1, blk0(), 0, s>>>(G0, maxit, ps, part.p);
+      k_pcg_cg_last<T><<<ctx->zg(grid0()), blk0(), 0, s>>>(G0, view(0), cg_update(maxit - 1), rbuf(maxit - 1));
+    if (end) k_pcg_end<<<ctx->zg(1), blk0(), 0, s>>>(G0, maxit, ps, part.p);
   }
 };
 
@@ -1735,8 +1807,8 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
     m->timer = timer;
     if (!same_operator || !m->built) m->setup(dE, dN);
     m->pcg(r, cfg.p_tol, cfg.p_rtol, maxit, sync_every, nullptr);
-    k_copy_out<double><<<m->lv[0]->grid(), dim3(MBX, MBY), 0, s>>>(m->view(0), m->xs.p, x, g_.P);
-    k_copy_state<<<1, 64, 0, s>>>(st, m->scal.p, m->scal.p + 1, 0);
+    k_copy_out<double><<<ctx_->zg(m->lv[0]->grid()), dim3(MBX, MBY), 0, s>>>(m->view(0), m->xs.p, x, g_.P);
+    k_copy_state<<<ctx_->zg(1), 64, 0, s>>>(st, m->scal.p, m->scal.p + 1, 0, ctx_->zs);
   } else {
     if (!impl32_) impl32_ = new MGImpl<float>(ctx_, g_);
     auto* m = static_cast<MGImpl<float>*>(impl32_);
@@ -1766,19 +1838,19 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
       const bool fold = o == 0 && endK > 0 && maxit > 0;
       if (o == 0) m->pcg(r, 0.0, inner_rtol, maxit, sync_every, nullptr, b64, outer, endK == 0, !fold);
       else m->pcg(r, 0.0, inner_rtol, maxit, sync_every, &outer->stop, nullptr, nullptr, endK == 0);
-      if (o > 0) k_ir_add<<<m->lv[0]->grid(), dim3(MBX, MBY), 0, s>>>(L, m->xs.p, x, g_.P, &outer->stop);
+      if (o > 0) k_ir_add<<<ctx_->zg(m->lv[0]->grid()), dim3(MBX, MBY), 0, s>>>(L, m->xs.p, x, g_.P, &outer->stop);
       CgFinal cfin{nullptr, nullptr, nullptr, nullptr, m->scal.p, -1};
       if (fold) {
         const XrFuse<float> u = m->cg_update(maxit - 1);
         cfin = CgFinal{u.z, u.pold, u.rz, u.zw, m->scal.p, maxit - 1};
       }
-      k_ir_resid<<<m->grid0(), m->blk0(), 0, s>>>(m->G0, g_, L, dE, dN, x, b64, r, &outer->stop, m->part.p, m->xs.p,
+      k_ir_resid<<<ctx_->zg(m->grid0()), m->blk0(), 0, s>>>(m->G0, g_, L, dE, dN, x, b64, r, &outer->stop, m->part.p, m->xs.p,
                                                 o == 0 ? 1 : 0, cfin);
       if (max_outer == 1 && !poll) {  // the only sweep: its check waits for finish()
         pending_ = PendingCheck{true, cfg.p_tol, cfg.p_rtol, endK};
         continue;
       }
-      k_ir_check<<<1, m->blk0(), 0, s>>>(m->G0, outer, inner, o, cfg.p_tol, cfg.p_rtol, m->part.p, st, endK);
+      k_ir_check<<<ctx_->zg(1), m->blk0(), 0, s>>>(m->G0, outer, inner, o, cfg.p_tol, cfg.p_rtol, m->part.p, st, endK);
       CPS_HIP(hipGetLastError());
       if (poll && o + 1 < max_outer) {
         int stop = 0;
@@ -1788,6 +1860,7 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
       }
     }
   }
+  armed_ = pending_;
   CPS_HIP(hipGetLastError());
 }
 

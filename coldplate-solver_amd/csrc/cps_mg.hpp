@@ -26,6 +26,11 @@ class PressureMG {
   // pressure statistics in IterState) until the host reads the statistics:
   // no later kernel of the step reads them.  Launches it if one is pending.
   void finish(IterState* st);
+  // A replayed graph of captured steps runs none of the host code that
+  // leaves the check pending: after replays, rearm() restores the pending
+  // check as the captured solve (`captured`, possibly another field's solver
+  // of the same size and settings) left it.
+  void rearm(const PressureMG& captured) { pending_ = captured.armed_; }
 
  private:
   Ctx* ctx_;
@@ -34,7 +39,7 @@ class PressureMG {
     bool on = false;
     double abs_tol = 0.0, rtol = 0.0;
     int endK = 0;
-  } pending_;
+  } pending_, armed_;
   void* impl64_ = nullptr;  // MGImpl<double>
   void* impl32_ = nullptr;  // MGImpl<float>
   void* b64_ = nullptr;     // FP64 rhs kept across refinement sweeps

@@ -56,6 +56,19 @@ constexpr int BX = 64, BY = 16, NT = BX * BY;  // 1024-thread blocks, one row pe
 
 struct Fields {
   double *u, *v, *p, *us, *vs, *ps, *dE, *dN, *gam, *alp, *inlet;
+  __device__ void zshift(size_t o) {
+    u = zsh(u, o);
+    v = zsh(v, o);
+    p = zsh(p, o);
+    us = zsh(us, o);
+    vs = zsh(vs, o);
+    ps = zsh(ps, o);
+    dE = zsh(dE, o);
+    dN = zsh(dN, o);
+    gam = zsh(gam, o);
+    alp = zsh(alp, o);
+    inlet = zsh(inlet, o);
+  }
 };
 
 template <class C>
@@ -63,10 +76,29 @@ struct MomSys {  // coefficients of one component, indexed like the field
   C *cS, *cW, *aP, *cE, *cN, *b;
   C *x0, *x1;  // Jacobi double buffer (x-form: velocity, delta-form: correction)
   C* x2;       // third buffer of the paired sweeps (delta form), x_j in buffer j mod 3
+  __device__ void zshift(size_t o) {
+    cS = zsh(cS, o);
+    cW = zsh(cW, o);
+    aP = zsh(aP, o);
+    cE = zsh(cE, o);
+    cN = zsh(cN, o);
+    b = zsh(b, o);
+    x0 = zsh(x0, o);
+    x1 = zsh(x1, o);
+    x2 = zsh(x2, o);
+  }
 };
 
 struct PcgBufs {
   double *r, *z, *pa, *pb, *Ap, *diag;
+  __device__ void zshift(size_t o) {
+    r = zsh(r, o);
+    z = zsh(z, o);
+    pa = zsh(pa, o);
+    pb = zsh(pb, o);
+    Ap = zsh(Ap, o);
+    diag = zsh(diag, o);
+  }
 };
 
 __device__ __forceinline__ double clampv(double x, double m) { return fmin(fmax(x, -m), m); }
@@ -251,7 +283,9 @@ __device__ inline void store_mom(const Geo& g, const MomCoef<IsU>& c, const doub
 // Launched on the active-cell grid (tiles from cell (1, 1), one cell per
 // thread, see active_grid): every U and V cell lies in it.
 template <class C, bool Delta>
-__global__ __launch_bounds__(NT) void k_assemble(Geo g, Fields f, MomSys<C> su, MomSys<C> sv, IterState* st) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_assemble(Geo g, Fields f, MomSys<C> su, MomSys<C> sv, IterState* st) {
+  const size_t zo_ = zoff(g.zs);
+  f.zshift(zo_); su.zshift(zo_); sv.zshift(zo_); st = zsh(st, zo_);
   const int lt = xcd_tile();  // tiles in XCD-aware order (the SOU stencil reads two-cell halos)
   const int j = 1 + lt % (int)gridDim.x * BX + (int)threadIdx.x;
   const int M = g.M, N = g.N;
@@ -316,6 +350,8 @@ __device__ inline void jac_detect(const Geo& g, int k, double tol2, IterState* s
 template <class C>
 __global__ __launch_bounds__(NT) void k_jacobi(Geo g, MomSys<C> su, MomSys<C> sv, int sweep, double tol2,
                                                IterState* st, double* part) {
+  const size_t zo_ = zoff(g.zs);
+  su.zshift(zo_); sv.zshift(zo_); st = zsh(st, zo_); part = zsh(part, zo_);
   __shared__ double sh[NT / kWave + 1];
   __shared__ int sfl[2];
   const int t = thread_id();
@@ -371,6 +407,8 @@ __global__ __launch_bounds__(NT) void k_jacobi(Geo g, MomSys<C> su, MomSys<C> sv
 // After K launched sweeps: stop test of sweep K-1 and sweep counts of the
 // components still iterating (one block).
 __global__ __launch_bounds__(NT) void k_jac_end(Geo g, int K, double tol2, IterState* st, const double* part) {
+  const size_t zo_ = zoff(g.zs);
+  st = zsh(st, zo_); part = zsh(part, zo_);
   __shared__ double sh[NT / kWave + 1];
   __shared__ int shi;
   int au = block_read(&st->jac_active[0], &shi), av = block_read(&st->jac_active[1], &shi);
@@ -403,7 +441,10 @@ static_assert(JRING == 2 * JR1W + 2 * BY && JRING <= NT, "one ring cell per thre
 template <class C>
 __device__ __forceinline__ C* jac_buf(const MomSys<C>& s, int j) {
   const int m = j % 3;
-  return m == 0 ? s.x0 : m == 1 ? s.x1 : s.x2;
+  C* const b0 = s.x0;  // a select of values (not of member addresses), so the
+  C* const b1 = s.x1;  // struct can stay in registers
+  C* const b2 = s.x2;
+  return m == 0 ? b0 : m == 1 ? b1 : b2;
 }
 
 // r = b - A x (column order S, W, P, E, N) and x + r / aP at LDS cell q of a
@@ -423,6 +464,8 @@ __device__ __forceinline__ C jac_cell(const C* X, int W, int q, const C* c, C& r
 template <class C>
 __global__ __launch_bounds__(NT) void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> sv, int k, int two, double tol2,
                                                 IterState* st, double* part) {
+  const size_t zo_ = zoff(g.zs);
+  su.zshift(zo_); sv.zshift(zo_); st = zsh(st, zo_); part = zsh(part, zo_);
   __shared__ C xa[2][JR2N];  // x_k, U and V
   __shared__ C xb[2][JR1N];  // x_(k+1)
   __shared__ double sh[NT / kWave + 1];
@@ -592,6 +635,8 @@ __global__ __launch_bounds__(NT) void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> s
 template <class C, bool Delta>
 __global__ __launch_bounds__(NT) void k_finalize(Geo g, Fields f, MomSys<C> su, MomSys<C> sv, IterState* st,
                                                  double* part, int K, int pairs, double tol2) {
+  const size_t zo_ = zoff(g.zs);
+  f.zshift(zo_); su.zshift(zo_); sv.zshift(zo_); st = zsh(st, zo_); part = zsh(part, zo_);
   __shared__ double sh[NT / kWave + 1];
   const int M = g.M, N = g.N;
   const double mv = g.maxVel;
@@ -682,6 +727,8 @@ __device__ inline double mass_b(const Geo& g, const Fields& f, int i, int j) {
 // their steps skip those stores and the pressure coefficients.
 template <bool PcgStart>
 __global__ __launch_bounds__(NT) void k_prhs(Geo g, Fields f, PcgBufs q, double* part) {
+  const size_t zo_ = zoff(g.zs);
+  f.zshift(zo_); q.zshift(zo_); part = zsh(part, zo_);
   __shared__ double sh[NT / kWave + 1];
   const int M = g.M, N = g.N;
   double mx = 0, ss = 0, rz = 0, bb = 0;
@@ -727,6 +774,8 @@ __global__ __launch_bounds__(NT) void k_prhs(Geo g, Fields f, PcgBufs q, double*
 // p[t], p[t + NT], ..., then the wavefronts in order), so the results are
 // the same.
 __global__ __launch_bounds__(NT) void k_stats(Geo g, IterState* st, const double* part, int reset_pcg) {
+  const size_t zo_ = zoff(g.zs);
+  st = zsh(st, zo_); part = zsh(part, zo_);
   constexpr int NW = NT / kWave;
   __shared__ double sh[8][NW];
   const int NB = g.NB, t = thread_id();
@@ -788,6 +837,8 @@ __global__ __launch_bounds__(NT) void k_stats(Geo g, IterState* st, const double
 // p = z + beta p_old fused into Ap = A p (CSR sorted order S, W, P, E, N).
 __global__ __launch_bounds__(NT) void k_pcg_a(Geo g, const double* dE, const double* dN, PcgBufs q, int it,
                                              double tol, IterState* st, double* part) {
+  const size_t zo_ = zoff(g.zs);
+  dE = zsh(dE, zo_); dN = zsh(dN, zo_); q.zshift(zo_); st = zsh(st, zo_); part = zsh(part, zo_);
   __shared__ double sh[NT / kWave + 1];
   __shared__ int shi;
   if (block_read(&st->pcg_stop, &shi)) return;
@@ -831,6 +882,8 @@ __global__ __launch_bounds__(NT) void k_pcg_a(Geo g, const double* dE, const dou
 // PCG part B (pressure_solver.cpp:158-170): |pAp| test, alpha = rz / pAp,
 // x += alpha p, r -= alpha Ap, z = r / diag, partial ||r||^2 and r.z.
 __global__ __launch_bounds__(NT) void k_pcg_b(Geo g, double* x, PcgBufs q, int it, IterState* st, double* part) {
+  const size_t zo_ = zoff(g.zs);
+  x = zsh(x, zo_); q.zshift(zo_); st = zsh(st, zo_); part = zsh(part, zo_);
   __shared__ double sh[NT / kWave + 1];
   __shared__ int shi;
   if (block_read(&st->pcg_stop, &shi)) return;
@@ -865,6 +918,8 @@ __global__ __launch_bounds__(NT) void k_pcg_b(Geo g, double* x, PcgBufs q, int i
 
 // After K launched PCG iterations: stop test of iteration K-1 (one block).
 __global__ __launch_bounds__(NT) void k_pcg_end(Geo g, int K, double tol, IterState* st, const double* part) {
+  const size_t zo_ = zoff(g.zs);
+  st = zsh(st, zo_); part = zsh(part, zo_);
   __shared__ double sh[NT / kWave + 1];
   __shared__ int shi;
   if (block_read(&st->pcg_stop, &shi)) return;
@@ -888,6 +943,8 @@ __global__ __launch_bounds__(NT) void k_pcg_end(Geo g, int K, double tol, IterSt
 // residual k_prhs left in q.r; pStar starts at 0 (cleared by k_prhs).
 __global__ __launch_bounds__(NT) void k_sor(Geo g, Fields f, const double* b, int it, int colour, double omega,
                                            double tol, IterState* st, double* part) {
+  const size_t zo_ = zoff(g.zs);
+  f.zshift(zo_); b = zsh(b, zo_); st = zsh(st, zo_); part = zsh(part, zo_);
   __shared__ double sh[NT / kWave + 1];
   __shared__ int shi;
   if (block_read(&st->pcg_stop, &shi)) return;
@@ -935,6 +992,8 @@ __global__ __launch_bounds__(NT) void k_sor(Geo g, Fields f, const double* b, in
 
 // After K launched SOR iterations: stop test of iteration K-1 (one block).
 __global__ __launch_bounds__(NT) void k_sor_end(Geo g, int K, double tol, IterState* st, const double* part) {
+  const size_t zo_ = zoff(g.zs);
+  st = zsh(st, zo_); part = zsh(part, zo_);
   __shared__ double sh[NT / kWave + 1];
   __shared__ int shi;
   if (block_read(&st->pcg_stop, &shi)) return;
@@ -950,6 +1009,8 @@ __global__ __launch_bounds__(NT) void k_sor_end(Geo g, int K, double tol, IterSt
 // Corrections (iterations.cpp:556-600) with the BCs of u, v, p fused: the
 // thread that owns an interior value also writes the ghost copied from it.
 __global__ __launch_bounds__(NT) void k_correct(Geo g, Fields f) {
+  const size_t zo_ = zoff(g.zs);
+  f.zshift(zo_);
   const int j = blockIdx.x * BX + threadIdx.x;
   const int M = g.M, N = g.N, P = g.P;
   if (j > N) return;
@@ -1258,6 +1319,7 @@ class SimpleSolver {
     g.uvAlpha = cfg.uv_alpha;
     g.pAlpha = cfg.p_alpha;
     g.sou = cfg.convection_scheme == 1;
+    g.zs = ctx->zs;
     S = (size_t)(M + 1) * g.P;
 
     for (auto* b : {&u, &v, &p, &us, &vs, &ps, &dE, &dN, &gam, &alp}) b->alloc(S);
@@ -1430,8 +1492,8 @@ class SimpleSolver {
     const double tol2 = cfg.momentum_tol * cfg.momentum_tol;
     const int maxit = cfg.max_momentum_iter;
     const int chunk = maxit > kNoSyncCap ? 8 : std::max(1, maxit);
-    if (mode_fp64) k_assemble<double, false><<<active_grid(), block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p);
-    else k_assemble<float, true><<<active_grid(), block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p);
+    if (mode_fp64) k_assemble<double, false><<<ctx->zg(active_grid()), block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p);
+    else k_assemble<float, true><<<ctx->zg(active_grid()), block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p);
     CPS_HIP(hipGetLastError());
     int k = 0;
     // two sweeps per launch on the unpolled delta path (the timed single
@@ -1442,7 +1504,7 @@ class SimpleSolver {
                        jgrid.x > 0 && jgrid.y > 0 && (int)(jgrid.x * jgrid.y) <= g.NB;
     if (pairs) {
       for (; k < maxit; k += 2)
-        k_jacobi2<float><<<jgrid, block, 0, s>>>(g, sys(momF, 0), sys(momF, 1), k, k + 1 < maxit ? 1 : 0, tol2,
+        k_jacobi2<float><<<ctx->zg(jgrid), block, 0, s>>>(g, sys(momF, 0), sys(momF, 1), k, k + 1 < maxit ? 1 : 0, tol2,
                                                  st.p, part.p);
       CPS_HIP(hipGetLastError());
     }
@@ -1451,19 +1513,19 @@ class SimpleSolver {
       for (; k < k1; ++k) {
         const bool timed = timer.active(KT_MOM_JACOBI);
         if (mode_fp64 && timed)
-          timer.ext_launch(k_jacobi<double>, grid, block, 0, s, g, sys(momD, 0), sys(momD, 1), k, tol2, st.p, part.p);
+          timer.ext_launch(k_jacobi<double>, ctx->zg(grid), block, 0, s, g, sys(momD, 0), sys(momD, 1), k, tol2, st.p, part.p);
         else if (mode_fp64)
-          k_jacobi<double><<<grid, block, 0, s>>>(g, sys(momD, 0), sys(momD, 1), k, tol2, st.p, part.p);
+          k_jacobi<double><<<ctx->zg(grid), block, 0, s>>>(g, sys(momD, 0), sys(momD, 1), k, tol2, st.p, part.p);
         else if (timed)
-          timer.ext_launch(k_jacobi<float>, grid, block, 0, s, g, sys(momF, 0), sys(momF, 1), k, tol2, st.p, part.p);
+          timer.ext_launch(k_jacobi<float>, ctx->zg(grid), block, 0, s, g, sys(momF, 0), sys(momF, 1), k, tol2, st.p, part.p);
         else
-          k_jacobi<float><<<grid, block, 0, s>>>(g, sys(momF, 0), sys(momF, 1), k, tol2, st.p, part.p);
+          k_jacobi<float><<<ctx->zg(grid), block, 0, s>>>(g, sys(momF, 0), sys(momF, 1), k, tol2, st.p, part.p);
       }
       // x, cS, cW, aP, cE, cN, b read + x' written, both components
       if (timer.which == KT_MOM_JACOBI)
         timer.bytes_per_launch = 8.0 * (mode_fp64 ? 8 : 4) * (double)(g.nU + g.nV);
       if (k >= maxit && chunk == maxit) break;  // the counts are settled by k_finalize
-      k_jac_end<<<1, block, 0, s>>>(g, k, tol2, st.p, part.p);
+      k_jac_end<<<ctx->zg(1), block, 0, s>>>(g, k, tol2, st.p, part.p);
       CPS_HIP(hipGetLastError());
       if (k >= maxit) break;
       int act[2];
@@ -1473,9 +1535,9 @@ class SimpleSolver {
     }
     const int K = maxit == 0 ? 0 : chunk == maxit ? maxit : -1;
     if (mode_fp64)
-      k_finalize<double, false><<<grid, block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p, part.p, K, 0, tol2);
+      k_finalize<double, false><<<ctx->zg(grid), block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p, part.p, K, 0, tol2);
     else
-      k_finalize<float, true><<<grid, block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p, part.p, K,
+      k_finalize<float, true><<<ctx->zg(grid), block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p, part.p, K,
                                                       pairs ? (int)(jgrid.x * jgrid.y) : 0, tol2);
     CPS_HIP(hipGetLastError());
   }
@@ -1492,13 +1554,13 @@ class SimpleSolver {
       const int k1 = std::min(maxit, k + chunk);
       for (; k < k1; ++k) {
         if (timer.active(KT_PCG_STENCIL))
-          timer.ext_launch(k_pcg_a, grid, block, 0, s, g, (const double*)dE.p, (const double*)dN.p, q, k, cfg.p_tol,
+          timer.ext_launch(k_pcg_a, ctx->zg(grid), block, 0, s, g, (const double*)dE.p, (const double*)dN.p, q, k, cfg.p_tol,
                            st.p, part.p);
         else
-          k_pcg_a<<<grid, block, 0, s>>>(g, dE.p, dN.p, q, k, cfg.p_tol, st.p, part.p);
-        k_pcg_b<<<grid, block, 0, s>>>(g, ps.p, q, k, st.p, part.p);
+          k_pcg_a<<<ctx->zg(grid), block, 0, s>>>(g, dE.p, dN.p, q, k, cfg.p_tol, st.p, part.p);
+        k_pcg_b<<<ctx->zg(grid), block, 0, s>>>(g, ps.p, q, k, st.p, part.p);
       }
-      k_pcg_end<<<1, block, 0, s>>>(g, k, cfg.p_tol, st.p, part.p);
+      k_pcg_end<<<ctx->zg(1), block, 0, s>>>(g, k, cfg.p_tol, st.p, part.p);
       CPS_HIP(hipGetLastError());
       if (k >= maxit) break;
       if (read_int(&st.p->pcg_stop)) break;
@@ -1520,8 +1582,8 @@ class SimpleSolver {
       const int k1 = std::min(maxit, k + chunk);
       for (; k < k1; ++k)
         for (int colour = 0; colour < 2; ++colour)
-          k_sor<<<grid, block, 0, s>>>(g, f, q.r, k, colour, omega, cfg.p_tol, st.p, part.p);
-      k_sor_end<<<1, block, 0, s>>>(g, k, cfg.p_tol, st.p, part.p);
+          k_sor<<<ctx->zg(grid), block, 0, s>>>(g, f, q.r, k, colour, omega, cfg.p_tol, st.p, part.p);
+      k_sor_end<<<ctx->zg(1), block, 0, s>>>(g, k, cfg.p_tol, st.p, part.p);
       CPS_HIP(hipGetLastError());
       if (k >= maxit) break;
       if (read_int(&st.p->pcg_stop)) break;
@@ -1535,17 +1597,17 @@ class SimpleSolver {
     hipStream_t s = ctx->stream;
     const Fields f = fields();
     momentum();
-    if (cfg.pressure_solver == CPS_PRESSURE_PCG) k_prhs<true><<<grid, block, 0, s>>>(g, f, pbufs(), part.p);
-    else k_prhs<false><<<grid, block, 0, s>>>(g, f, pbufs(), part.p);
+    if (cfg.pressure_solver == CPS_PRESSURE_PCG) k_prhs<true><<<ctx->zg(grid), block, 0, s>>>(g, f, pbufs(), part.p);
+    else k_prhs<false><<<ctx->zg(grid), block, 0, s>>>(g, f, pbufs(), part.p);
     // the Jacobi-PCG and SOR solves start from the statistics' r.z and
     // ||b||^2; the multigrid solve reads none of them, so there the
     // reductions wait until the host asks (stats())
-    if (cfg.pressure_solver != CPS_PRESSURE_GMG) k_stats<<<1, block, 0, s>>>(g, st.p, part.p, 1);
+    if (cfg.pressure_solver != CPS_PRESSURE_GMG) k_stats<<<ctx->zg(1), block, 0, s>>>(g, st.p, part.p, 1);
     CPS_HIP(hipGetLastError());
     if (cfg.pressure_solver == CPS_PRESSURE_PCG) pressure_pcg();
     else if (cfg.pressure_solver == CPS_PRESSURE_SOR) pressure_sor();
     else pressure_mg();
-    k_correct<<<grid, block, 0, s>>>(g, f);
+    k_correct<<<ctx->zg(grid), block, 0, s>>>(g, f);
     CPS_HIP(hipGetLastError());
     iteration++;
   }
@@ -1768,31 +1830,74 @@ void SimpleSolver::pressure_mg() {
 }
 
 // ===========================================================================
-// Batch of fields: every field keeps its own solver state.  Fields are
-// assigned largest-first to the least loaded of `nstreams` streams, and one
-// SIMPLE iteration of each stream's fields is captured once into a hipGraph
-// of its own; a batch iteration launches one graph per stream, so the
-// fields' small kernels run concurrently on the otherwise idle CUs (the
-// parallel branches of a single graph are executed one after another by
-// this runtime, which is why the graph is split per stream).
+// Batch of fields.  Fields whose grids and step settings agree form a size
+// class.  A class keeps its fields' buffers in one allocation, field k's at
+// k * stride: each solver of the class is built (and takes its first step,
+// which creates the multigrid state) under an Arena at its own slot, so all
+// of them lay their buffers out alike.  One SIMPLE iteration of the whole
+// class is then its first field's step launched with gridDim.z = fields in
+// the class: every kernel of the chain steps all of the class's fields at
+// once, so the small multigrid levels and the one-workgroup kernels carry
+// the whole class instead of one field.  A class's iteration (or `unroll`
+// of them) is captured into one hipGraph, and the classes' graphs run on
+// streams of their own (this runtime executes the parallel branches of a
+// single graph one after another, so concurrency comes from streams).  The
+// per-field plan, each field's own chain with the fields spread over up to
+// 32 streams, stays as the tuner's alternative.  Either way a field's
+// results are those of stepping it alone, bit for bit: the same kernels do
+// the same arithmetic on the same data.
 class BatchSolver {
  public:
+  enum Plan { kClass = 0, kPerField = 1 };
+  struct Group {
+    std::vector<int> members;  // field indices; members[0] launches the class
+    size_t stride = 0;         // bytes between consecutive members' buffers
+  };
+  struct Graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    hipStream_t s = nullptr;
+  };
+  struct Streams {
+    std::vector<hipStream_t> v;
+    ~Streams() {
+      for (auto s : v) (void)hipStreamDestroy(s);
+    }
+  };
+  static constexpr int kMaxStreams = 32;
+  // arena bytes per field: 56 padded FP64 fields bound what a solver and its
+  // multigrid allocate (at most ~47 in the FP64 form), plus the small
+  // buffers (partials, coarse levels, scalars)
+  static constexpr size_t kArenaFields = 56, kArenaExtra = size_t(8) << 20, kArenaAlign = size_t(2) << 20;
+
   Ctx* base;
+  Streams streams;  // declared first: destroyed after the solvers
+  std::vector<std::unique_ptr<DevBuf<char>>> arenas;
   std::vector<std::unique_ptr<Ctx>> fctx;
   std::vector<std::unique_ptr<SimpleSolver>> f;
-  std::vector<hipStream_t> streams;
-  std::vector<hipGraph_t> graphs;
-  std::vector<hipGraphExec_t> execs;
-  int nstreams = 0;
+  std::vector<Group> groups;
+  std::vector<int> leader;       // field -> the field whose captured step covers it
+  std::vector<Graph> one, many;  // graphs of one iteration / of `unroll` iterations
+  int plan = kClass, nstreams = 0, unroll = 1;
   double us_per_step = 0.0;
-  static constexpr int kMaxStreams = 32;
+
+  // the settings one step reads (the outer loop's and the samples' are per field)
+  static bool same_step(const cps_simple_config& a, const cps_simple_config& b) {
+    return a.nx == b.nx && a.ny == b.ny && a.hx == b.hx && a.hy == b.hy && a.rho == b.rho && a.eta == b.eta &&
+           a.u_in == b.u_in && a.convection_scheme == b.convection_scheme && a.enable_25d == b.enable_25d &&
+           a.ht == b.ht && a.conv_factor_25d == b.conv_factor_25d && a.sink_mult_25d == b.sink_mult_25d &&
+           a.brinkman_kmin == b.brinkman_kmin && a.brinkman_q == b.brinkman_q && a.uv_alpha == b.uv_alpha &&
+           a.p_alpha == b.p_alpha && a.momentum_tol == b.momentum_tol && a.max_momentum_iter == b.max_momentum_iter &&
+           a.p_tol == b.p_tol && a.max_pressure_iter == b.max_pressure_iter && a.pressure_solver == b.pressure_solver &&
+           a.precision == b.precision && a.p_rtol == b.p_rtol && a.max_refine == b.max_refine;
+  }
 
   BatchSolver(Ctx* c, int n, const cps_simple_config* cfgs, const double* const* gammas,
               const double* const* inlets)
       : base(c) {
     CPS_CHECK(n >= 1 && cfgs && gammas, CPS_ERR_ARG, "batch needs n >= 1 fields with configs and gammas");
-    for (int i = 0; i < n; ++i) {
-      cps_simple_config cf = cfgs[i];
+    std::vector<cps_simple_config> rc(cfgs, cfgs + n);
+    for (auto& cf : rc) {
       SimpleSolver::resolve_pressure_solver(cf);
       // capture-safe caps: no host synchronisation inside one iteration
       CPS_CHECK(cf.max_momentum_iter <= SimpleSolver::kNoSyncCap && cf.max_pressure_iter <= SimpleSolver::kNoSyncCap &&
@@ -1802,43 +1907,97 @@ class BatchSolver {
     for (int k = 0; k < kMaxStreams; ++k) {
       hipStream_t st;
       CPS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));  // default priority
-      streams.push_back(st);
+      streams.v.push_back(st);
     }
     for (int i = 0; i < n; ++i) {
-      fctx.emplace_back(new Ctx(*c));
-      fctx.back()->stream = streams[0];
-      f.emplace_back(new SimpleSolver(fctx.back().get(), cfgs[i], gammas[i], inlets ? inlets[i] : nullptr));
+      size_t gi = 0;
+      while (gi < groups.size() && !same_step(rc[groups[gi].members[0]], rc[i])) ++gi;
+      if (gi == groups.size()) groups.emplace_back();
+      groups[gi].members.push_back(i);
     }
-    // one eager iteration per field: creates the lazily built multigrid
-    // state (allocations and clears stay out of the captured graphs)
-    for (auto& s : f) {
-      s->apply_bc();
-      s->step_once();
+    fctx.resize(n);
+    f.resize(n);
+    leader.assign(n, 0);
+    for (size_t gi = 0; gi < groups.size(); ++gi) build_group(gi, cfgs, gammas, inlets);
+    sync_all();
+  }
+
+  // The class's solvers, each under its own arena slot, each taking one
+  // eager iteration (which creates the lazily built multigrid state:
+  // allocations and clears stay out of the captured graphs).
+  void build_group(size_t gi, const cps_simple_config* cfgs, const double* const* gammas,
+                   const double* const* inlets) {
+    Group& G = groups[gi];
+    const cps_simple_config& c0 = cfgs[G.members[0]];
+    const size_t P = ((size_t)c0.nx + 2 + 63) / 64 * 64, S = ((size_t)c0.ny + 2) * P;
+    G.stride = (kArenaFields * S * sizeof(double) + kArenaExtra + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+    arenas.emplace_back(new DevBuf<char>);
+    arenas.back()->alloc(G.stride * G.members.size());
+    char* base_p = arenas.back()->p;
+    size_t used = 0;
+    for (size_t k = 0; k < G.members.size(); ++k) {
+      const int i = G.members[k];
+      Arena a{base_p + k * G.stride, G.stride, 0};
+      ArenaScope scope(&a);
+      fctx[i].reset(new Ctx(*base));
+      fctx[i]->stream = streams.v[i % kMaxStreams];
+      fctx[i]->zs = G.stride;
+      fctx[i]->nz = 1;
+      f[i].reset(new SimpleSolver(fctx[i].get(), cfgs[i], gammas[i], inlets ? inlets[i] : nullptr));
+      f[i]->apply_bc();
+      f[i]->step_once();
+      if (k == 0) used = a.off;
+      CPS_CHECK(a.off == used, CPS_ERR_STATE, "fields of one size class laid out their buffers differently");
+      leader[i] = G.members[0];
     }
-    CPS_HIP(hipStreamSynchronize(streams[0]));
   }
 
   ~BatchSolver() {
     (void)sync_all_nothrow();
     release_graphs();
     f.clear();
-    for (auto st : streams) (void)hipStreamDestroy(st);
+    arenas.clear();
   }
 
   bool sync_all_nothrow() {
     bool ok = true;
-    for (auto st : streams) ok = hipStreamSynchronize(st) == hipSuccess && ok;
+    for (auto st : streams.v) ok = hipStreamSynchronize(st) == hipSuccess && ok;
     return ok;
   }
   void sync_all() {
-    for (auto st : streams) CPS_HIP(hipStreamSynchronize(st));
+    for (auto st : streams.v) CPS_HIP(hipStreamSynchronize(st));
   }
 
   void release_graphs() {
-    for (auto e : execs) (void)hipGraphExecDestroy(e);
-    for (auto g : graphs) (void)hipGraphDestroy(g);
-    execs.clear();
-    graphs.clear();
+    for (auto* v : {&one, &many})
+      for (auto& G : *v) {
+        if (G.e) (void)hipGraphExecDestroy(G.e);
+        if (G.g) (void)hipGraphDestroy(G.g);
+      }
+    one.clear();
+    many.clear();
+  }
+
+  // `steps` iterations of the listed fields on stream s as one graph
+  Graph capture_steps(const std::vector<int>& fields, hipStream_t s, int steps) {
+    Graph G;
+    G.s = s;
+    CPS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int r = 0; r < steps; ++r)
+        for (int i : fields) {
+          f[i]->step_once();
+          f[i]->iteration--;  // capture does not run an iteration
+        }
+    } catch (...) {
+      hipGraph_t dead = nullptr;
+      (void)hipStreamEndCapture(s, &dead);
+      if (dead) (void)hipGraphDestroy(dead);
+      throw;
+    }
+    CPS_HIP(hipStreamEndCapture(s, &G.g));
+    CPS_HIP(hipGraphInstantiate(&G.e, G.g, nullptr, nullptr, 0));
+    return G;
   }
 
   // greedy longest-processing-time assignment (cost ~ cells)
@@ -1856,59 +2015,100 @@ class BatchSolver {
     return owner;
   }
 
-  void capture(int ns) {
+  // Captures plan p: kClass, one graph per size class (of `un` iterations,
+  // plus one of a single iteration for remainders), class k on stream k mod
+  // 32; kPerField, the fields spread over ns streams, one graph per stream.
+  void capture(int p, int ns, int un) {
+    CPS_CHECK(p == kClass || p == kPerField, CPS_ERR_ARG, "plan must be 0 (class) or 1 (per field)");
+    CPS_CHECK(un >= 1 && un <= 64, CPS_ERR_ARG, "unroll must be in 1..64");
     sync_all();
     release_graphs();
-    ns = std::max(1, std::min(ns, std::min((int)f.size(), kMaxStreams)));
-    const std::vector<int> owner = assign(ns);
-    for (size_t i = 0; i < f.size(); ++i) fctx[i]->stream = streams[owner[i]];
-    for (int k = 0; k < ns; ++k) {
-      hipGraph_t g = nullptr;
-      hipGraphExec_t e = nullptr;
-      CPS_HIP(hipStreamBeginCapture(streams[k], hipStreamCaptureModeThreadLocal));
-      for (size_t i = 0; i < f.size(); ++i)
-        if (owner[i] == k) {
-          f[i]->step_once();
-          f[i]->iteration--;  // capture does not run an iteration
+    const int n = (int)f.size();
+    if (p == kClass) {
+      for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const Group& G = groups[gi];
+        hipStream_t s = streams.v[gi % kMaxStreams];
+        const int L = G.members[0];
+        fctx[L]->stream = s;  // members' data are written by the leader's launches
+        for (int i : G.members) leader[i] = L;
+        fctx[L]->nz = (int)G.members.size();
+        try {
+          one.push_back(capture_steps({L}, s, 1));
+          if (un > 1) many.push_back(capture_steps({L}, s, un));
+        } catch (...) {
+          fctx[L]->nz = 1;
+          throw;
         }
-      CPS_HIP(hipStreamEndCapture(streams[k], &g));
-      graphs.push_back(g);
-      CPS_HIP(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
-      execs.push_back(e);
+        fctx[L]->nz = 1;
+      }
+      nstreams = std::min((int)groups.size(), kMaxStreams);
+    } else {
+      ns = std::max(1, std::min(ns, std::min(n, kMaxStreams)));
+      const std::vector<int> owner = assign(ns);
+      for (int i = 0; i < n; ++i) {
+        fctx[i]->stream = streams.v[owner[i]];
+        leader[i] = i;
+      }
+      for (int k = 0; k < ns; ++k) {
+        std::vector<int> mine;
+        for (int i = 0; i < n; ++i)
+          if (owner[i] == k) mine.push_back(i);
+        one.push_back(capture_steps(mine, streams.v[k], 1));
+        if (un > 1) many.push_back(capture_steps(mine, streams.v[k], un));
+      }
+      nstreams = ns;
     }
-    nstreams = ns;
+    plan = p;
+    unroll = un;
   }
 
   void launch(int n) {
-    CPS_CHECK(!execs.empty(), CPS_ERR_STATE, "batch graphs not built: call cps_batch_tune first");
-    for (int it = 0; it < n; ++it)
-      for (int k = 0; k < nstreams; ++k) CPS_HIP(hipGraphLaunch(execs[k], streams[k]));
+    CPS_CHECK(!one.empty(), CPS_ERR_STATE, "batch graphs not built: call cps_batch_tune or cps_batch_plan first");
+    int it = 0;
+    if (!many.empty())
+      for (; it + unroll <= n; it += unroll)
+        for (auto& G : many) CPS_HIP(hipGraphLaunch(G.e, G.s));
+    for (; it < n; ++it)
+      for (auto& G : one) CPS_HIP(hipGraphLaunch(G.e, G.s));
     for (auto& s : f) s->iteration += n;
+    if (n > 0)
+      for (size_t i = 0; i < f.size(); ++i)
+        if (f[i]->mg) f[i]->mg->rearm(*f[leader[i]]->mg);
   }
 
-  // times `reps` batch iterations per candidate stream count, keeps the best
+  // Times `reps` batch iterations per candidate plan and keeps the fastest.
+  // fixed > 0: the per-field plan on that many streams only (unroll 1);
+  // fixed == 0: the class plan with unroll 1, 2, 4, 8 and the per-field
+  // plan on 1 .. 32 streams.  Every candidate gives the same results.
   void tune(int fixed, int reps) {
-    std::vector<int> cand;
-    if (fixed > 0) cand.push_back(fixed);
-    else
+    struct Cand {
+      int p, ns, un;
+    };
+    std::vector<Cand> cand;
+    if (fixed > 0) cand.push_back({kPerField, fixed, 1});
+    else {
+      for (int un : {1, 2, 4, 8}) cand.push_back({kClass, 0, un});
       for (int ns : {1, 2, 4, 8, 16, 24, 32})
-        if (ns <= (int)f.size()) cand.push_back(ns);
-    int best = cand[0];
+        if (ns <= (int)f.size()) cand.push_back({kPerField, ns, 1});
+    }
+    size_t best = 0;
     double best_us = 1e300;
-    for (int ns : cand) {
-      capture(ns);
-      launch(1);  // warm-up
+    for (size_t k = 0; k < cand.size(); ++k) {
+      const Cand& c = cand[k];
+      capture(c.p, c.ns, c.un);
+      const int r = std::max(reps, c.un);
+      launch(c.un);  // warm-up
       sync_all();
       const auto t0 = std::chrono::steady_clock::now();
-      launch(reps);
+      launch(r);
       sync_all();
-      const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() / reps;
+      const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() / r;
       if (us < best_us) {
         best_us = us;
-        best = ns;
+        best = k;
       }
     }
-    if (best != nstreams) capture(best);
+    if (best + 1 != cand.size()) capture(cand[best].p, cand[best].ns, cand[best].un);
     us_per_step = best_us;
   }
 };
@@ -2149,6 +2349,25 @@ int cps_batch_tune(cps_batch* b, int streams, int reps, int* chosen, double* us_
   });
 }
 
+int cps_batch_plan(cps_batch* b, int plan, int streams, int unroll) {
+  return guard([&] {
+    CPS_CHECK(b && streams >= 0, CPS_ERR_ARG, "bad arguments");
+    b->b->capture(plan, streams, unroll);
+    b->b->us_per_step = 0.0;
+  });
+}
+
+int cps_batch_plan_info(cps_batch* b, int* plan, int* streams, int* unroll, int* classes, double* us_per_step) {
+  return guard([&] {
+    CPS_CHECK(b, CPS_ERR_ARG, "null batch");
+    if (plan) *plan = b->b->plan;
+    if (streams) *streams = b->b->nstreams;
+    if (unroll) *unroll = b->b->unroll;
+    if (classes) *classes = (int)b->b->groups.size();
+    if (us_per_step) *us_per_step = b->b->us_per_step;
+  });
+}
+
 int cps_batch_step(cps_batch* b, int n_steps) {
   return guard([&] {
     CPS_CHECK(b && n_steps >= 0, CPS_ERR_ARG, "bad arguments");
@@ -2185,6 +2404,8 @@ int cps_batch_gradient_dp(cps_batch* b, int field, double rtol, int max_iter, do
     CPS_CHECK(b && field >= 0 && field < (int)b->b->f.size(), CPS_ERR_ARG, "bad field index");
     CPS_CHECK(rtol > 0 && max_iter >= 1, CPS_ERR_ARG, "rtol must be > 0 and max_iter >= 1");
     cps::SimpleSolver& s = *b->b->f[field];
+    // the field's data are written on its class leader's stream (class plan)
+    CPS_HIP(hipStreamSynchronize(b->b->f[b->b->leader[field]]->ctx->stream));
     CPS_HIP(hipStreamSynchronize(s.ctx->stream));
     s.apply_bc();
     cps::flow_adjoint_run(s.adj(), s.view(), rtol, max_iter, dp, dJ_dgamma, stats);

@@ -15,6 +15,11 @@ struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int* pinned_i = nullptr;  // small pinned host scratch for flag reads
+  // class-batched stepping: launches cover nz fields whose buffers sit zs
+  // bytes apart (nz = 1: this solver's own field only)
+  int nz = 1;
+  size_t zs = 0;
+  dim3 zg(dim3 g) const { return dim3(g.x, g.y, (unsigned)nz); }
 };
 
 // Scalars shared by kernels of one SIMPLE iteration (device memory).
@@ -57,6 +62,7 @@ struct Geo {
   double uvAlpha, pAlpha;
   double targetVel;
   int sou;
+  size_t zs;  // byte stride between the fields of a class-batched launch
 };
 
 class SimpleSolver;

@@ -209,11 +209,16 @@ class SIMPLE:
 
 
 class Batch:
-    """Many density fields stepped together: fields are spread over several
-    streams (count picked by `tune`) and one SIMPLE iteration of each
-    stream's fields is one hipGraph launch, the streams running concurrently.  `fields` is a list of
-    (cfg, gamma, inlet) tuples; iteration caps must allow graph capture
+    """Many density fields stepped together.  Fields with the same grid and
+    step settings form a size class; under the class plan (the default pick
+    of `tune`) one launch per kernel of the SIMPLE chain steps every field of
+    a class, one class iteration (or `unroll` of them) is one hipGraph and
+    the classes' graphs run on streams of their own.  The per-field plan
+    spreads the fields' own chains over several streams.  `fields` is a list
+    of (cfg, gamma, inlet) tuples; iteration caps must allow graph capture
     (momentum and pressure iterations <= 32)."""
+
+    PLANS = {"class": T.CPS_BATCH_CLASS, "per_field": T.CPS_BATCH_PER_FIELD}
 
     def __init__(self, fields, device=0):
         self.ctx = _capi.context(device)
@@ -251,14 +256,65 @@ class Batch:
         except Exception:
             pass
 
-    def tune(self, streams=0, reps=3):
-        """Captures the batch iteration; streams = 0 times 1, 2, 4, 8, 16
-        streams (reps launches each, which advance the fields) and keeps the
-        fastest."""
+    def signature(self):
+        """Key of the tuner cache: the size classes (grid, count) and the
+        step settings that set the kernels' work."""
+        cls = {}
+        for c in self.cfgs:
+            k = (c.nx, c.ny, c.precision, c.pressure_solver, c.max_momentum_iter, c.max_pressure_iter, c.max_refine,
+                 c.convection_scheme)
+            cls[k] = cls.get(k, 0) + 1
+        return ";".join(f"{nx}x{ny}:{m} prec{pr} p{ps} mom{mi} pit{pi} ref{rf} conv{cv}"
+                        for (nx, ny, pr, ps, mi, pi, rf, cv), m in sorted(cls.items()))
+
+    def tune(self, streams=0, reps=3, cache=None):
+        """Captures the batch iteration.  streams > 0: the per-field plan on
+        that many streams.  streams = 0: the autotuner times the class plan
+        (unroll 1, 2, 4, 8) and the per-field plan (1 .. 32 streams), reps
+        launches each (which advance the fields), and keeps the fastest.
+        cache: path of a JSON file of earlier picks keyed by signature(): a
+        hit is captured without timing, a miss is tuned and stored.
+        Returns (streams used, microseconds per batch iteration)."""
+        import json
+        import os
+
+        key = self.signature()
+        if streams == 0 and cache is not None and os.path.exists(cache):
+            with open(cache) as fh:
+                entry = json.load(fh).get("entries", {}).get(key)
+            if entry is not None:
+                self.plan(entry["plan"], entry.get("streams", 0), entry.get("unroll", 1))
+                self.us_per_step = entry.get("us_per_step")
+                return self.streams, self.us_per_step
         ch, us = C.c_int(), C.c_double()
         _capi.check(self.L.cps_batch_tune(self.h, streams, reps, C.byref(ch), C.byref(us)))
         self.streams, self.us_per_step = ch.value, us.value
+        if streams == 0 and cache is not None:
+            info = self.plan_info()
+            data = {"version": 1, "entries": {}}
+            if os.path.exists(cache):
+                with open(cache) as fh:
+                    data = json.load(fh)
+            data.setdefault("entries", {})[key] = dict(plan=info["plan"], streams=info["streams"],
+                                                       unroll=info["unroll"], us_per_step=info["us_per_step"])
+            tmp = cache + ".tmp"
+            with open(tmp, "w") as fh:
+                json.dump(data, fh, indent=1, sort_keys=True)
+            os.replace(tmp, cache)
         return self.streams, self.us_per_step
+
+    def plan(self, plan="class", streams=0, unroll=1):
+        """Captures a given plan ("class" or "per_field") without timing."""
+        p = self.PLANS[plan] if isinstance(plan, str) else int(plan)
+        _capi.check(self.L.cps_batch_plan(self.h, p, streams, unroll))
+        self.streams = self.plan_info()["streams"]
+
+    def plan_info(self):
+        p, ns, un, ncls, us = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        _capi.check(self.L.cps_batch_plan_info(self.h, C.byref(p), C.byref(ns), C.byref(un), C.byref(ncls),
+                                               C.byref(us)))
+        name = {v: k for k, v in self.PLANS.items()}[p.value]
+        return dict(plan=name, streams=ns.value, unroll=un.value, classes=ncls.value, us_per_step=us.value)
 
     def step(self, n=1):
         _capi.check(self.L.cps_batch_step(self.h, n))
@@ -290,9 +346,9 @@ class Batch:
     def gradients_dp(self, fields=None, rtol=1e-10, max_iter=4000, workers=8):
         """Gradients of several fields at once: one host thread per field in
         flight.  Each field's adjoint runs on the stream its field was given
-        (cps_batch_tune spreads the fields over the batch's streams), so
-        fields on different streams overlap and fields sharing a stream run
-        one after another."""
+        (field k on stream k mod 32 under the class plan, its owner stream
+        under the per-field plan), so fields on different streams overlap and
+        fields sharing a stream run one after another."""
         from concurrent.futures import ThreadPoolExecutor
 
         fields = list(range(self.n)) if fields is None else list(fields)

@@ -1,9 +1,13 @@
-"""Batched fields through one captured hipGraph per batch iteration.
+"""Batched fields through captured hipGraphs.
 
 Every field of a batch must equal the same field stepped eagerly on its own
-(identical kernels in identical order per field, so bit for bit), after
-each of several graph replays separated by eager work, for every stream
-count the autotuner may pick.
+(identical kernels doing identical arithmetic per field, so bit for bit),
+after each of several graph replays separated by eager work, under every
+plan the autotuner may pick: the class plan (one launch per kernel covering
+all fields of a size class, gridDim.z = fields, buffers at a fixed stride),
+with and without unrolled graphs, and the per-field plan on 1 or 3 streams.
+The batch mixes sizes and holds several fields of the same size, so the
+class plan launches over 1, 2 and 3 fields at once.
 """
 import numpy as np
 import pytest
@@ -11,10 +15,13 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def fields(pkg, precision):
+SIZES = [(40, 20), (64, 33), (97, 41), (64, 33), (64, 33), (40, 20)]
+
+
+def fields(pkg, precision, sizes=SIZES):
     T = pkg._types
     out = []
-    for k, (nx, ny) in enumerate([(40, 20), (64, 33), (97, 41)]):
+    for k, (nx, ny) in enumerate(sizes):
         cfg, g, inlet = pkg.configs.density_field(nx=nx, ny=ny, seed=10 + k, sigma=3.0)
         cfg.pressure_solver = T.CPS_PRESSURE_GMG
         cfg.precision = precision
@@ -25,12 +32,15 @@ def fields(pkg, precision):
 
 
 @pytest.mark.parametrize("precision", [32, 64])
-@pytest.mark.parametrize("streams", [1, 3])
-def test_batch_replays_match_eager_fields(pkg, precision, streams):
+@pytest.mark.parametrize("plan", [("class", 0, 1), ("class", 0, 3), ("per_field", 1, 1), ("per_field", 3, 1)])
+def test_batch_replays_match_eager_fields(pkg, precision, plan):
     fl = fields(pkg, precision)
+    print("case batch replays", precision, plan, flush=True)
     b = pkg.Batch(fl)
-    b.tune(streams=streams, reps=2)
-    assert b.streams == streams
+    b.plan(*plan)
+    info = b.plan_info()
+    assert info["plan"] == plan[0] and info["unroll"] == plan[2] and info["classes"] == 3
+    assert b.streams == (3 if plan[0] == "class" else plan[1])
     for replay in range(3):
         b.step(1 + replay)
         b.synchronize()
@@ -45,16 +55,33 @@ def test_batch_replays_match_eager_fields(pkg, precision, streams):
     b.close()
 
 
-def test_batch_autotune_and_mixed_sizes(pkg):
+def test_batch_autotune_and_json_cache(pkg, tmp_path):
+    """The tuner times every plan and keeps one; its pick is stored under the
+    batch's signature and a second batch of the same shape captures it from
+    the cache without timing, and steps to the same fields."""
+    import json
+
     fl = fields(pkg, 32)
-    b = pkg.Batch(fl + fl[:1])
-    ns, us = b.tune(streams=0, reps=2)
-    assert ns in (1, 2, 4) and us > 0
-    b.step(5)
-    b.synchronize()
+    cache = str(tmp_path / "batch_tune.json")
+    b = pkg.Batch(fl)
+    ns, us = b.tune(streams=0, reps=2, cache=cache)
+    info = b.plan_info()
+    print("tuned", info, flush=True)
+    assert ns >= 1 and us > 0 and info["classes"] == 3
+    entry = json.load(open(cache))["entries"][b.signature()]
+    assert entry["plan"] == info["plan"] and entry["unroll"] == info["unroll"]
+    b2 = pkg.Batch(fl)
+    ns2, us2 = b2.tune(streams=0, cache=cache)
+    assert b2.plan_info()["plan"] == info["plan"] and b2.plan_info()["unroll"] == info["unroll"]
+    assert us2 == pytest.approx(entry["us_per_step"])
+    it = b.stats(0)["iteration"]
+    b2.step(it - b2.stats(0)["iteration"])
+    b2.synchronize()
     for k in range(b.n):
         assert np.isfinite(b.stats(k)["error"])
+        np.testing.assert_array_equal(b.get(k, "p"), b2.get(k, "p"), err_msg=f"field {k}")
     b.close()
+    b2.close()
 
 
 def test_batch_rejects_uncapturable_caps(pkg):
@@ -66,10 +93,11 @@ def test_batch_rejects_uncapturable_caps(pkg):
 
 def test_batch_gradients_threaded_match_single_field(pkg):
     """Gradients of several fields computed concurrently (host threads,
-    per-field streams) equal the single-field adjoint at the same state."""
+    per-field streams) after class-plan stepping equal the single-field
+    adjoint at the same state."""
     fl = fields(pkg, 64)
     b = pkg.Batch(fl)
-    b.tune(streams=2, reps=2)
+    b.plan("class", unroll=4)
     b.step(300)
     b.synchronize()
     threaded = b.gradients_dp(rtol=1e-10, workers=3)
@@ -89,7 +117,8 @@ def test_batch_gradients_threaded_match_single_field(pkg):
 def test_batch_48_fields_at_real_sizes_replays_match_eager(pkg):
     """The design sweep at its own sizes: 48 fields drawn from 256 x 128 ..
     1024 x 512 (sigma U[4, 12], per-field seeds), FP32-inner / FP64-refined
-    with the headline knobs, 16 streams.  After two rounds of graph replays,
+    with the headline knobs, class plan (four classes of about 12 fields,
+    graphs of 4 iterations).  After two rounds of graph replays,
     one field of each size equals the same field stepped eagerly on its own,
     bit for bit, and every field stays finite."""
     T = pkg._types
@@ -105,7 +134,8 @@ def test_batch_48_fields_at_real_sizes_replays_match_eager(pkg):
     assert len(pick) == 4
     print("case batch 48 fields, sizes", sorted(pick), flush=True)
     b = pkg.Batch(fl)
-    b.tune(streams=16, reps=2)
+    b.plan("class", unroll=4)
+    assert b.plan_info()["classes"] == 4
     for rounds in (3, 7):
         b.step(rounds)
         b.synchronize()

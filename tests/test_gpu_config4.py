@@ -88,9 +88,10 @@ def test_sou_512x256_gradient_matches_central_differences_on_64_cells(pkg):
 
 
 def test_32_batched_gradients_match_single_field_gradients(pkg):
-    """32 seeded 512 x 256 fields stepped as one batch (per-stream graphs),
-    their 32 gradients computed concurrently, each equal to the gradient of a
-    standalone solver set to that field's state (same kernels: bit for bit)."""
+    """32 seeded 512 x 256 fields stepped as one batch (one size class: each
+    kernel launch covers all 32 fields), their 32 gradients computed
+    concurrently, each equal to the gradient of a standalone solver set to
+    that field's state (same kernels: bit for bit)."""
     fl = []
     for k in range(32):
         cfg, gamma, inlet = pkg.configs.density_field(nx=512, ny=256, seed=k)
@@ -100,7 +101,8 @@ def test_32_batched_gradients_match_single_field_gradients(pkg):
         fl.append((cfg, gamma, inlet))
     print("case 32 batched gradients 512x256", flush=True)
     b = pkg.Batch(fl)
-    b.tune(streams=8, reps=2)
+    b.plan("class", unroll=10)
+    assert b.plan_info()["classes"] == 1
     b.step(300)
     b.synchronize()
     res = b.gradients_dp(rtol=1e-8, workers=8)
