@@ -201,10 +201,13 @@ def batch_leg(pkg, steps=500, n_fields=48, plan=None):
         cfg.max_momentum_iter = 8
         fl.append((cfg, gamma, inlet))
     b = pkg.Batch(fl)
-    if plan:  # "class:U" or "per_field:S" instead of the tuner's pick
-        kind, n = plan.split(":")
-        b.plan(kind, unroll=int(n)) if kind == "class" else b.plan(kind, streams=int(n))
-        b.step(int(n) if kind == "class" else 1)  # warm-up
+    if plan:  # "class:UNROLL[:CHAINS]" or "per_field:STREAMS" instead of the tuner's pick
+        kind, *n = plan.split(":")
+        if kind == "class":
+            b.plan(kind, streams=int(n[1]) if len(n) > 1 else 1, unroll=int(n[0]))
+        else:
+            b.plan(kind, streams=int(n[0]))
+        b.step(int(n[0]) if kind == "class" else 1)  # warm-up
     else:
         b.tune(streams=0, reps=3)
     ns = b.streams
@@ -217,7 +220,8 @@ def batch_leg(pkg, steps=500, n_fields=48, plan=None):
     cells = sum(c.nx * c.ny for c, _, _ in fl)
     finite = all(math.isfinite(b.stats(k)["error"]) for k in range(b.n))
     b.close()
-    return dict(fields=n_fields, steps=steps, seconds=dt, streams=ns, plan=plan["plan"], unroll=plan["unroll"],
+    return dict(fields=n_fields, steps=steps, seconds=dt, streams=ns, plan=plan["plan"], plan_param=plan["param"],
+                unroll=plan["unroll"],
                 size_classes=plan["classes"], batch_it_s=steps / dt,
                 field_it_s=n_fields * steps / dt, cell_updates_per_s=cells * steps / dt, total_cells=cells,
                 all_finite=finite)
@@ -355,7 +359,7 @@ def main():
     ap.add_argument("--no-thermal", action="store_true", help="skip the 3-D thermal leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-field batch leg")
     ap.add_argument("--batch-plan", default=None,
-                    help="batch leg plan instead of the autotuner's: class:UNROLL or per_field:STREAMS")
+                    help="batch leg plan instead of the autotuner's: class:UNROLL[:CHAINS] or per_field:STREAMS")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--copy-calibration", action="store_true", help="only copy 1 GiB on the device (4x) and exit")
     ap.add_argument("--leg", choices=["grads", "grads_batch", "thermal", "batch", "time_to_dp", "parity"],

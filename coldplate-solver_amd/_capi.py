@@ -80,7 +80,7 @@ def lib():
         L.cps_batch_destroy.restype = None
         L.cps_batch_tune.argtypes = [vp, C.c_int, C.c_int, ip, dp]
         L.cps_batch_plan.argtypes = [vp, C.c_int, C.c_int, C.c_int]
-        L.cps_batch_plan_info.argtypes = [vp, ip, ip, ip, ip, dp]
+        L.cps_batch_plan_info.argtypes = [vp, ip, ip, ip, ip, ip, dp]
         L.cps_batch_step.argtypes = [vp, C.c_int]
         L.cps_batch_synchronize.argtypes = [vp]
         L.cps_batch_field_stats.argtypes = [vp, C.c_int, P(T.StepStats)]

@@ -233,6 +233,31 @@ __device__ inline void block_sum2_all(double& a, double& b, double* sh2) {
   b = rb;
 }
 
+// block_sum2_all's two sums (same order, same bits) in thread 0 only, with
+// no trailing barrier.
+template <int NT>
+__device__ inline void block_sum2_t0(double& a, double& b, double* sh2) {
+  constexpr int NW = NT / kWave;
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if ((t & (kWave - 1)) == 0) {
+    sh2[t / kWave] = a;
+    sh2[NW + t / kWave] = b;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double ra = 0.0, rb = 0.0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      ra += sh2[k];
+      rb += sh2[NW + k];
+    }
+    a = ra;
+    b = rb;
+  }
+}
+
 // Fixed-order sum / max of n block partials written by an earlier kernel on
 // the same stream (the kernel boundary makes them visible; no in-kernel
 // fences).  Every thread of the block gets the result.  sh needs

@@ -337,6 +337,71 @@ struct Tile {
   }
 };
 
+// The region positions a thread visits in every pass over the tile
+// region (q = t + i FT, i < TNP), located once per kernel: the ring of each
+// (distance to the region border, -1 past it) and whether its cell lies in
+// the level.  With the inverse diagonals staged once (si), a sweep over the
+// ring >= h cells (h >= 1) takes no index arithmetic, no bounds tests and no
+// division.  Cells outside the level hold zero faces, zero b and zero x, so
+// their terms in a neighbour's stencil vanish exactly and the east / north
+// faces of the last column / row meet a zero x: the sums equal the guarded
+// ones of Tile.
+constexpr int TNP = (RN + FT - 1) / FT;
+template <class T>
+struct TileCells {
+  const Tile<T>* tl;
+  T* si;
+  int ring[TNP];
+  unsigned in;  // bit i: position i lies in the level
+  __device__ TileCells(const Tile<T>* tile, T* inv, int t) : tl(tile), si(inv), in(0u) {
+#pragma unroll
+    for (int i = 0; i < TNP; ++i) {
+      const int q = t + i * FT;
+      const int lr = q / RW, lc = q - (q / RW) * RW;
+      ring[i] = q < RN ? min(min(lr, RH - 1 - lr), min(lc, RW - 1 - lc)) : -1;
+      if (q < RN && tile->inside(tile->gr(lr), tile->gc(lc))) in |= 1u << i;
+    }
+  }
+  // the diagonal at a ring >= 1 position: the faces in Tile::diag's order
+  __device__ T diag(int q) const {
+    const T d = tl->se[q] + tl->se[q - 1] + tl->sn[q] + tl->sn[q - RW];
+    return d < (T)kTiny20 ? (T)1 : d;
+  }
+  // off-diagonal part of A x, negated: sum of face x neighbour
+  __device__ T nb(const T* x, int q) const {
+    return tl->se[q] * x[q + 1] + tl->se[q - 1] * x[q - 1] + tl->sn[q] * x[q + RW] + tl->sn[q - RW] * x[q - RW];
+  }
+  __device__ T ax(const T* x, int q, T d) const { return d * x[q] - nb(x, q); }
+  // si on the ring >= 1 positions (the faces must be visible to the block)
+  __device__ void stage_inv(int t) const {
+#pragma unroll
+    for (int i = 0; i < TNP; ++i)
+      if (ring[i] >= 1) {
+        const int q = t + i * FT;
+        si[q] = (T)1 / diag(q);
+      }
+  }
+  // x0 = om b / diag on ring >= h from a zero guess, 0 outside the level
+  __device__ void first(T* dst, int h, int t, T om) const {
+#pragma unroll
+    for (int i = 0; i < TNP; ++i)
+      if (ring[i] >= h) {
+        const int q = t + i * FT;
+        dst[q] = (in >> i & 1u) ? om * tl->sb[q] * si[q] : (T)0;
+      }
+  }
+  // one damped-Jacobi sweep src -> dst on ring >= h:
+  // x' = (1 - om) x + om (b + sum face x neighbour) / diag
+  __device__ void sweep(const T* src, T* dst, int h, int t, T om) const {
+#pragma unroll
+    for (int i = 0; i < TNP; ++i)
+      if (ring[i] >= h) {
+        const int q = t + i * FT;
+        dst[q] = (in >> i & 1u) ? ((T)1 - om) * src[q] + om * (tl->sb[q] + nb(src, q)) * si[q] : (T)0;
+      }
+  }
+};
+
 // Stage b, the faces and (up kernels) x1 + P e of the tile + halo in LDS.
 // All global loads of a thread are issued before the first LDS store.
 template <class T>
@@ -375,7 +440,7 @@ template <class T>
 __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* stop, PcgTest test, XrFuse<T> xr) {
   const size_t zo_ = zoff(L.zs);
   L.zshift(zo_); Cl.zshift(zo_); stop = zsh(stop, zo_); test.zshift(zo_); xr.zshift(zo_);
-  __shared__ T sb[RN], se[RN], sn[RN], s0[RN], s1[RN];
+  __shared__ T sb[RN], se[RN], sn[RN], s0[RN], s1[RN], si[RN];
   __shared__ double sh[FT / kWave + 1];
   __shared__ double sh2[2 * (FT / kWave)];
   __shared__ int sflag;
@@ -383,6 +448,7 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
   const int stop0 = t == 0 ? *stop : 0;  // thread 0's view, published below
   const int lt = xcd_tile();
   Tile<T> tl{sb, se, sn, lt / (int)gridDim.x * TY, lt % (int)gridDim.x * TX, L.nx, L.ny};
+  const TileCells<T> tc(&tl, si, t);
   bool x0_ready = false;  // the fused branch forms the first sweep with b
   if (xr.rold) {
     const double prz = parts_load<FT>(xr.rz, xr.nb);
@@ -438,16 +504,17 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
       xr.ps->alphas[(xr.it - 1) & 1] = cs.alpha;
     }
     const T alpha = (T)cs.alpha, beta = (T)cs.beta;
+    static_assert(NI == TNP, "region passes");
 #pragma unroll
-    for (int i = 0; i < NI; ++i) {  // b, and the first sweep from zero on the halo-2 ring
+    for (int i = 0; i < NI; ++i) {  // b, si and the first sweep from zero on the halo-2 ring
       const int q = t + i * FT;
       if (q < RN) {
         const T b = vr[i] - alpha * (vw[i] + beta * vs[i]);
         sb[q] = b;
-        const int lr = q / RW, lc = q - (q / RW) * RW;
-        if (lr >= 1 && lr < RH - 1 && lc >= 1 && lc < RW - 1) {
-          const int r = tl.gr(lr), c = tl.gc(lc);
-          s0[q] = tl.inside(r, c) ? (T)kOmega1 * b / tl.diag(q, r, c) : (T)0;
+        if (tc.ring[i] >= 1) {
+          const T inv = (T)1 / tc.diag(q);
+          si[q] = inv;
+          s0[q] = (tc.in >> i & 1u) ? (T)kOmega1 * b * inv : (T)0;
         }
       }
     }
@@ -491,22 +558,24 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
   }
   if (!x0_ready) {  // block-uniform
     __syncthreads();
-    // first sweep from a zero guess on the halo-2 ring: x0 = omega b / diag
-    for (int q = t; q < RN; q += FT) {
-      const int lr = q / RW, lc = q - (q / RW) * RW;
-      if (lr < 1 || lr >= RH - 1 || lc < 1 || lc >= RW - 1) continue;
-      const int r = tl.gr(lr), c = tl.gc(lc);
-      s0[q] = tl.inside(r, c) ? (T)kOmega1 * sb[q] / tl.diag(q, r, c) : (T)0;
-    }
+    // si and the first sweep from a zero guess on the halo-2 ring: x0 = omega b / diag
+#pragma unroll
+    for (int i = 0; i < TNP; ++i)
+      if (tc.ring[i] >= 1) {
+        const int q = t + i * FT;
+        const T inv = (T)1 / tc.diag(q);
+        si[q] = inv;
+        s0[q] = (tc.in >> i & 1u) ? (T)kOmega1 * sb[q] * inv : (T)0;
+      }
     __syncthreads();
   }
-  tl.sweep(s0, s1, 2, t, (T)kOmega2);  // x1 on the halo-1 ring
+  tc.sweep(s0, s1, 2, t, (T)kOmega2);  // x1 on the halo-1 ring
   __syncthreads();
   {  // residual of the tile cell, x1 to memory
     const int lr = HL + threadIdx.y, lc = HL + threadIdx.x, q = lr * RW + lc;
     const int r = tl.gr(lr), c = tl.gc(lc);
     if (tl.inside(r, c)) {
-      s0[q] = sb[q] - tl.ax(s1, q, r, c, tl.diag(q, r, c));
+      s0[q] = sb[q] - tc.ax(s1, q, tc.diag(q));
       st_nt(&L.x[L.at(r, c)], s1[q]);
     } else {
       s0[q] = (T)0;
@@ -719,30 +788,33 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
                                              double* part2) {
   const size_t zo_ = zoff(L.zs);
   L.zshift(zo_); Cl.zshift(zo_); part = zsh(part, zo_); stop = zsh(stop, zo_); w = zsh(w, zo_); part2 = zsh(part2, zo_);
-  __shared__ T sb[RN], se[RN], sn[RN], s0[RN], s1[RN];
+  __shared__ T sb[RN], se[RN], sn[RN], s0[RN], s1[RN], si[RN];
   __shared__ int sflag;
   const int t = threadIdx.x + threadIdx.y * TX;
   const int stop0 = t == 0 ? *stop : 0;
   const int lt = xcd_tile();
   Tile<T> tl{sb, se, sn, lt / (int)gridDim.x * TY, lt % (int)gridDim.x * TX, L.nx, L.ny};
   tile_load<T>(L, tl, t, s0, &Cl);
+  const TileCells<T> tc(&tl, si, t);
   if (t == 0) sflag = stop0;
   __syncthreads();
   if (sflag) return;
+  tc.stage_inv(t);
+  __syncthreads();
   if (w) {
     // single-reduction PCG: z on the tile and its halo-1 ring (the same
     // arithmetic as the owner blocks'), w = A z on the tile, partials r.z
     // and z.w
-    tl.sweep(s0, s1, 1, t, (T)kOmega1);  // halo-2 ring
+    tc.sweep(s0, s1, 1, t, (T)kOmega1);  // halo-2 ring
     __syncthreads();
-    tl.sweep(s1, s0, 2, t, (T)kOmega2);  // z on the halo-1 ring
+    tc.sweep(s1, s0, 2, t, (T)kOmega2);  // z on the halo-1 ring
     __syncthreads();
     double e = 0.0, e2 = 0.0;
     const int lr = HL + threadIdx.y, lc = HL + threadIdx.x, q = lr * RW + lc;
     const int r = tl.gr(lr), c = tl.gc(lc);
     if (tl.inside(r, c)) {
       const T z = s0[q];
-      const T wz = tl.ax(s0, q, r, c, tl.diag(q, r, c));
+      const T wz = tc.ax(s0, q, tc.diag(q));
       const size_t k = L.at(r, c);
       st_nt(&L.y[k], z);
       st_nt(&w[k], wz);
@@ -750,22 +822,21 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
       e2 = (double)z * (double)wz;
     }
     __shared__ double sh2[2 * (FT / kWave)];
-    block_sum2_all<FT>(e, e2, sh2);
+    block_sum2_t0<FT>(e, e2, sh2);
     if (t == 0) {
       part[lt] = e;
       part2[lt] = e2;
     }
     return;
   }
-  tl.sweep(s0, s1, 2, t, (T)kOmega1);  // halo-1 ring
+  tc.sweep(s0, s1, 2, t, (T)kOmega1);  // halo-1 ring
   __syncthreads();
   double e = 0.0;
   {
     const int lr = HL + threadIdx.y, lc = HL + threadIdx.x, q = lr * RW + lc;
     const int r = tl.gr(lr), c = tl.gc(lc);
     if (tl.inside(r, c)) {
-      const T d = tl.diag(q, r, c);
-      const T z = s1[q] + (T)kOmega2 * (sb[q] - tl.ax(s1, q, r, c, d)) / d;
+      const T z = ((T)1 - (T)kOmega2) * s1[q] + (T)kOmega2 * (sb[q] + tc.nb(s1, q)) * si[q];
       st_nt(&L.y[L.at(r, c)], z);
       e += (double)sb[q] * (double)z;
     }

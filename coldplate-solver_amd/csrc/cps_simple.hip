@@ -1878,7 +1878,7 @@ class BatchSolver {
   std::vector<Group> groups;
   std::vector<int> leader;       // field -> the field whose captured step covers it
   std::vector<Graph> one, many;  // graphs of one iteration / of `unroll` iterations
-  int plan = kClass, nstreams = 0, unroll = 1;
+  int plan = kClass, plan_ns = 1, nstreams = 0, unroll = 1;  // plan_ns: chains per class / streams
   double us_per_step = 0.0;
 
   // the settings one step reads (the outer loop's and the samples' are per field)
@@ -2015,9 +2015,10 @@ class BatchSolver {
     return owner;
   }
 
-  // Captures plan p: kClass, one graph per size class (of `un` iterations,
-  // plus one of a single iteration for remainders), class k on stream k mod
-  // 32; kPerField, the fields spread over ns streams, one graph per stream.
+  // Captures plan p: kClass, each size class split into ns (>= 1) chains of
+  // consecutive fields, one graph per chain (of `un` iterations, plus one of
+  // a single iteration for remainders), chain k on stream k mod 32;
+  // kPerField, the fields spread over ns streams, one graph per stream.
   void capture(int p, int ns, int un) {
     CPS_CHECK(p == kClass || p == kPerField, CPS_ERR_ARG, "plan must be 0 (class) or 1 (per field)");
     CPS_CHECK(un >= 1 && un <= 64, CPS_ERR_ARG, "unroll must be in 1..64");
@@ -2025,25 +2026,35 @@ class BatchSolver {
     release_graphs();
     const int n = (int)f.size();
     if (p == kClass) {
+      // each class in `ns` chains of consecutive members (slots at the
+      // class stride, so a chain's first member launches for the chain)
+      const int split = std::max(1, ns);
+      plan_ns = split;
+      int chain = 0;
       for (size_t gi = 0; gi < groups.size(); ++gi) {
         const Group& G = groups[gi];
-        hipStream_t s = streams.v[gi % kMaxStreams];
-        const int L = G.members[0];
-        fctx[L]->stream = s;  // members' data are written by the leader's launches
-        for (int i : G.members) leader[i] = L;
-        fctx[L]->nz = (int)G.members.size();
-        try {
-          one.push_back(capture_steps({L}, s, 1));
-          if (un > 1) many.push_back(capture_steps({L}, s, un));
-        } catch (...) {
+        const int m = (int)G.members.size(), nc = std::min(split, m);
+        for (int c = 0; c < nc; ++c, ++chain) {
+          const int k0 = c * m / nc, k1 = (c + 1) * m / nc;
+          hipStream_t s = streams.v[chain % kMaxStreams];
+          const int L = G.members[k0];
+          fctx[L]->stream = s;  // the chain's data are written by its first member's launches
+          for (int k = k0; k < k1; ++k) leader[G.members[k]] = L;
+          fctx[L]->nz = k1 - k0;
+          try {
+            one.push_back(capture_steps({L}, s, 1));
+            if (un > 1) many.push_back(capture_steps({L}, s, un));
+          } catch (...) {
+            fctx[L]->nz = 1;
+            throw;
+          }
           fctx[L]->nz = 1;
-          throw;
         }
-        fctx[L]->nz = 1;
       }
-      nstreams = std::min((int)groups.size(), kMaxStreams);
+      nstreams = std::min(chain, kMaxStreams);
     } else {
       ns = std::max(1, std::min(ns, std::min(n, kMaxStreams)));
+      plan_ns = ns;
       const std::vector<int> owner = assign(ns);
       for (int i = 0; i < n; ++i) {
         fctx[i]->stream = streams.v[owner[i]];
@@ -2078,8 +2089,9 @@ class BatchSolver {
 
   // Times `reps` batch iterations per candidate plan and keeps the fastest.
   // fixed > 0: the per-field plan on that many streams only (unroll 1);
-  // fixed == 0: the class plan with unroll 1, 2, 4, 8 and the per-field
-  // plan on 1 .. 32 streams.  Every candidate gives the same results.
+  // fixed == 0: the class plan with 1 .. 4 chains per class and unroll 1,
+  // 4, 8, and the per-field plan on 1 .. 32 streams.  Every candidate gives
+  // the same results.
   void tune(int fixed, int reps) {
     struct Cand {
       int p, ns, un;
@@ -2087,7 +2099,8 @@ class BatchSolver {
     std::vector<Cand> cand;
     if (fixed > 0) cand.push_back({kPerField, fixed, 1});
     else {
-      for (int un : {1, 2, 4, 8}) cand.push_back({kClass, 0, un});
+      for (int split : {1, 2, 3, 4})
+        for (int un : {1, 4, 8}) cand.push_back({kClass, split, un});
       for (int ns : {1, 2, 4, 8, 16, 24, 32})
         if (ns <= (int)f.size()) cand.push_back({kPerField, ns, 1});
     }
@@ -2357,10 +2370,12 @@ int cps_batch_plan(cps_batch* b, int plan, int streams, int unroll) {
   });
 }
 
-int cps_batch_plan_info(cps_batch* b, int* plan, int* streams, int* unroll, int* classes, double* us_per_step) {
+int cps_batch_plan_info(cps_batch* b, int* plan, int* param, int* streams, int* unroll, int* classes,
+                        double* us_per_step) {
   return guard([&] {
     CPS_CHECK(b, CPS_ERR_ARG, "null batch");
     if (plan) *plan = b->b->plan;
+    if (param) *param = b->b->plan_ns;
     if (streams) *streams = b->b->nstreams;
     if (unroll) *unroll = b->b->unroll;
     if (classes) *classes = (int)b->b->groups.size();

@@ -270,7 +270,8 @@ class Batch:
     def tune(self, streams=0, reps=3, cache=None):
         """Captures the batch iteration.  streams > 0: the per-field plan on
         that many streams.  streams = 0: the autotuner times the class plan
-        (unroll 1, 2, 4, 8) and the per-field plan (1 .. 32 streams), reps
+        (1 .. 4 chains per class, unroll 1, 4, 8) and the per-field plan
+        (1 .. 32 streams), reps
         launches each (which advance the fields), and keeps the fastest.
         cache: path of a JSON file of earlier picks keyed by signature(): a
         hit is captured without timing, a miss is tuned and stored.
@@ -283,7 +284,7 @@ class Batch:
             with open(cache) as fh:
                 entry = json.load(fh).get("entries", {}).get(key)
             if entry is not None:
-                self.plan(entry["plan"], entry.get("streams", 0), entry.get("unroll", 1))
+                self.plan(entry["plan"], entry.get("param", 0), entry.get("unroll", 1))
                 self.us_per_step = entry.get("us_per_step")
                 return self.streams, self.us_per_step
         ch, us = C.c_int(), C.c_double()
@@ -295,7 +296,7 @@ class Batch:
             if os.path.exists(cache):
                 with open(cache) as fh:
                     data = json.load(fh)
-            data.setdefault("entries", {})[key] = dict(plan=info["plan"], streams=info["streams"],
+            data.setdefault("entries", {})[key] = dict(plan=info["plan"], param=info["param"], streams=info["streams"],
                                                        unroll=info["unroll"], us_per_step=info["us_per_step"])
             tmp = cache + ".tmp"
             with open(tmp, "w") as fh:
@@ -304,17 +305,21 @@ class Batch:
         return self.streams, self.us_per_step
 
     def plan(self, plan="class", streams=0, unroll=1):
-        """Captures a given plan ("class" or "per_field") without timing."""
+        """Captures a given plan without timing: "class" (streams = chains
+        per size class, 0 = one) or "per_field" (streams = streams)."""
         p = self.PLANS[plan] if isinstance(plan, str) else int(plan)
         _capi.check(self.L.cps_batch_plan(self.h, p, streams, unroll))
         self.streams = self.plan_info()["streams"]
 
     def plan_info(self):
-        p, ns, un, ncls, us = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_double()
-        _capi.check(self.L.cps_batch_plan_info(self.h, C.byref(p), C.byref(ns), C.byref(un), C.byref(ncls),
-                                               C.byref(us)))
+        """plan, param (chains per class or streams), streams (used by the
+        graphs), unroll, classes (size classes), us_per_step (tuner's time)."""
+        p, pa, ns, un, ncls, us = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        _capi.check(self.L.cps_batch_plan_info(self.h, C.byref(p), C.byref(pa), C.byref(ns), C.byref(un),
+                                               C.byref(ncls), C.byref(us)))
         name = {v: k for k, v in self.PLANS.items()}[p.value]
-        return dict(plan=name, streams=ns.value, unroll=un.value, classes=ncls.value, us_per_step=us.value)
+        return dict(plan=name, param=pa.value, streams=ns.value, unroll=un.value, classes=ncls.value,
+                    us_per_step=us.value)
 
     def step(self, n=1):
         _capi.check(self.L.cps_batch_step(self.h, n))

@@ -281,10 +281,11 @@ int cps_simple_adjoint_options(cps_simple* s, int preconditioner, int line_sweep
  * state.  Fields with the same grid and step settings form a size class,
  * whose buffers share one allocation at a fixed stride; two plans step them:
  *   CPS_BATCH_CLASS      one launch per kernel of the SIMPLE chain covers
- *                        every field of a class (gridDim.z = fields); one
- *                        class iteration (or `unroll` of them) is one
- *                        hipGraph, the classes' graphs on streams of their
- *                        own;
+ *                        every field of a chain of consecutive fields of a
+ *                        class (gridDim.z = fields; a class is split into
+ *                        `streams` chains); one chain iteration (or `unroll`
+ *                        of them) is one hipGraph, the chains' graphs on
+ *                        streams of their own;
  *   CPS_BATCH_PER_FIELD  each field's own chain; fields spread over
  *                        `streams` streams, one graph per stream.
  * A field's results are the same, bit for bit, under either plan and equal
@@ -301,17 +302,22 @@ int cps_batch_create(cps_ctx* ctx, int n_fields, const cps_simple_config* cfgs, 
                      const double* const* inlets, cps_batch** out);
 void cps_batch_destroy(cps_batch* b);
 /* Captures the batch iteration.  streams > 0: the per-field plan on that
- * many streams; 0: the autotuner times the class plan with unroll 1, 2, 4,
- * 8 and the per-field plan on 1 .. 32 streams (max(reps, unroll) launches
- * each; those launches advance the fields) and keeps the fastest.  chosen:
- * the streams the chosen plan uses (one per class for the class plan). */
+ * many streams; 0: the autotuner times the class plan with 1 .. 4 chains
+ * per class and unroll 1, 4, 8 and the per-field plan on 1 .. 32 streams
+ * (max(reps, unroll) launches each; those launches advance the fields) and
+ * keeps the fastest.  chosen: the streams the chosen plan uses (one per
+ * chain for the class plan). */
 int cps_batch_tune(cps_batch* b, int streams, int reps, int* chosen, double* us_per_step);
-/* Captures the given plan without timing (a cached tuner choice):
- * plan CPS_BATCH_*, streams (per-field plan) >= 1, unroll in 1..64. */
+/* Captures the given plan without timing (a cached tuner choice): plan
+ * CPS_BATCH_*; streams: chains per class (class plan; 0 = 1) or streams
+ * (per-field plan); unroll in 1..64. */
 int cps_batch_plan(cps_batch* b, int plan, int streams, int unroll);
-/* The captured plan, its streams and unroll, the number of size classes
- * and the tuner's time per batch iteration (0 when not timed). */
-int cps_batch_plan_info(cps_batch* b, int* plan, int* streams, int* unroll, int* classes, double* us_per_step);
+/* The captured plan and its parameter (the `streams` argument of
+ * cps_batch_plan: chains per class or streams), the streams its graphs use,
+ * its unroll, the number of size classes and the tuner's time per batch
+ * iteration (0 when not timed). */
+int cps_batch_plan_info(cps_batch* b, int* plan, int* param, int* streams, int* unroll, int* classes,
+                        double* us_per_step);
 /* n_steps graph launches (asynchronous). */
 int cps_batch_step(cps_batch* b, int n_steps);
 int cps_batch_synchronize(cps_batch* b);

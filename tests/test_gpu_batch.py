@@ -32,7 +32,8 @@ def fields(pkg, precision, sizes=SIZES):
 
 
 @pytest.mark.parametrize("precision", [32, 64])
-@pytest.mark.parametrize("plan", [("class", 0, 1), ("class", 0, 3), ("per_field", 1, 1), ("per_field", 3, 1)])
+@pytest.mark.parametrize("plan", [("class", 0, 1), ("class", 0, 3), ("class", 2, 4), ("per_field", 1, 1),
+                                  ("per_field", 3, 1)])
 def test_batch_replays_match_eager_fields(pkg, precision, plan):
     fl = fields(pkg, precision)
     print("case batch replays", precision, plan, flush=True)
@@ -40,7 +41,9 @@ def test_batch_replays_match_eager_fields(pkg, precision, plan):
     b.plan(*plan)
     info = b.plan_info()
     assert info["plan"] == plan[0] and info["unroll"] == plan[2] and info["classes"] == 3
-    assert b.streams == (3 if plan[0] == "class" else plan[1])
+    # class plan: classes of 2, 3 and 1 fields, each in min(chains, fields) chains
+    chains = sum(min(max(plan[1], 1), m) for m in (2, 3, 1))
+    assert b.streams == (chains if plan[0] == "class" else plan[1])
     for replay in range(3):
         b.step(1 + replay)
         b.synchronize()
@@ -69,10 +72,12 @@ def test_batch_autotune_and_json_cache(pkg, tmp_path):
     print("tuned", info, flush=True)
     assert ns >= 1 and us > 0 and info["classes"] == 3
     entry = json.load(open(cache))["entries"][b.signature()]
-    assert entry["plan"] == info["plan"] and entry["unroll"] == info["unroll"]
+    assert entry["plan"] == info["plan"] and entry["unroll"] == info["unroll"] and entry["param"] == info["param"]
     b2 = pkg.Batch(fl)
     ns2, us2 = b2.tune(streams=0, cache=cache)
-    assert b2.plan_info()["plan"] == info["plan"] and b2.plan_info()["unroll"] == info["unroll"]
+    info2 = b2.plan_info()
+    assert (info2["plan"], info2["param"], info2["unroll"], info2["streams"]) == \
+        (info["plan"], info["param"], info["unroll"], info["streams"])
     assert us2 == pytest.approx(entry["us_per_step"])
     it = b.stats(0)["iteration"]
     b2.step(it - b2.stats(0)["iteration"])
