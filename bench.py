@@ -26,7 +26,11 @@ of mixed sizes stepped by per-stream hipGraph launches, and the
 
 --copy-calibration runs only a device-to-device copy of 1 GiB (4 times),
 the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters
-to bytes; --leg NAME runs one leg alone (for profiling it).
+to bytes; --leg NAME runs one leg alone (for profiling it);
+--profile-table OUT.md digests rocprofv3 CSV output (a --kernel-trace
+directory and --pmc FETCH_SIZE / WRITE_SIZE directories of the same command
+and of --copy-calibration) into a per-kernel table of durations, HBM bytes
+and GB/s.
 
 usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20]
 """
@@ -343,6 +347,82 @@ def cpu_baseline(pkg, nx, ny, seconds):
                        f"<= 8 Jacobi sweeps (time_to_dp compares time to a converged dP)")
 
 
+def _short_kernel(name):
+    import re
+
+    m = re.search(r"(k_[A-Za-z0-9_]+(<[^>]*>)?)", name)
+    return m.group(1) if m else name.split("(")[0][:48]
+
+
+def _csv_rows(d, suffix):
+    import csv
+    import glob
+
+    for f in sorted(glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True)):
+        with open(f) as fh:
+            yield from csv.DictReader(fh)
+
+
+def profile_table(out, trace, fetch, write, copy_fetch, copy_write, title, skip=0.0):
+    """Per kernel and grid: dispatches, mean duration (kernel trace, end -
+    start; dispatches after the first `skip` fraction of the run), HBM
+    bytes read / written per dispatch (median FETCH_SIZE / WRITE_SIZE in
+    KiB, scaled by the factor that maps the same counter of a 1 GiB device
+    copy to 1 GiB) and the HBM GB/s they give."""
+    import statistics as st
+
+    def grid_of(r):
+        return int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+
+    rows = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"]), _short_kernel(r["Kernel_Name"]), grid_of(r),
+             f'{r["Grid_Size_X"]}x{r["Grid_Size_Y"]}x{r["Grid_Size_Z"]}') for r in _csv_rows(trace, "kernel_trace.csv")]
+    rows.sort()
+    rows = rows[int(len(rows) * skip):]
+    dur = {}
+    for t0, t1, k, g, gs in rows:
+        dur.setdefault((k, g, gs), []).append((t1 - t0) / 1e3)
+
+    def counters(d, name):
+        acc = {}
+        for r in _csv_rows(d, "counter_collection.csv"):
+            if r["Counter_Name"] != name:
+                continue
+            g = int(r["Grid_Size"]) if r.get("Grid_Size") else grid_of(r)
+            acc.setdefault((_short_kernel(r["Kernel_Name"]), g), []).append(float(r["Counter_Value"]))
+        return {k: st.median(v) for k, v in acc.items()}
+
+    def scale(d, name):
+        c = counters(d, name)
+        big = max(c.values())  # the 1 GiB copy is the largest dispatch of the calibration run
+        return (1 << 30) / (big * 1024.0)
+
+    F, W = counters(fetch, "FETCH_SIZE"), counters(write, "WRITE_SIZE")
+    fs, ws = scale(copy_fetch, "FETCH_SIZE"), scale(copy_write, "WRITE_SIZE")
+    tot = sum(sum(v) for v in dur.values())
+    lines = [f"## {title}", "",
+             f"HBM bytes: FETCH_SIZE x {fs:.3f}, WRITE_SIZE x {ws:.3f} (KiB -> bytes of a 1 GiB copy); "
+             f"GB/s = (read + written) / mean duration; fraction of the 8 TB/s peak.", "",
+             "| kernel | grid (threads) | dispatches | mean us | total ms | % | read MB | written MB | HBM GB/s | of 8 TB/s |",
+             "|---|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for (k, g, gs), v in sorted(dur.items(), key=lambda kv: -sum(kv[1])):
+        if not k.startswith("k_"):
+            continue
+        mean = sum(v) / len(v)
+        rd = F.get((k, g))
+        wr = W.get((k, g))
+        if rd is None or wr is None:
+            cells = "| - | - | - | - |"
+        else:
+            rb, wb = rd * 1024 * fs, wr * 1024 * ws
+            gbs = (rb + wb) / (mean * 1e-6) / 1e9
+            cells = f"| {rb / 1e6:.2f} | {wb / 1e6:.2f} | {gbs:.0f} | {gbs / 8000:.2f} |"
+        lines.append(f"| {k} | {gs} | {len(v)} | {mean:.2f} | {sum(v) / 1e3:.3f} | {100 * sum(v) / tot:.1f} " + cells)
+    text = "\n".join(lines) + "\n"
+    with open(out, "w") as fh:
+        fh.write(text)
+    print(text, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -358,13 +438,24 @@ def main():
     ap.add_argument("--no-grads", action="store_true", help="skip the d(dP)/d(gamma) leg")
     ap.add_argument("--no-thermal", action="store_true", help="skip the 3-D thermal leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-field batch leg")
+    ap.add_argument("--no-time-to-dp", action="store_true", help="skip the time-to-converged-dP leg")
     ap.add_argument("--batch-plan", default=None,
                     help="batch leg plan instead of the autotuner's: class:UNROLL[:CHAINS] or per_field:STREAMS")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--copy-calibration", action="store_true", help="only copy 1 GiB on the device (4x) and exit")
+    ap.add_argument("--profile-table", metavar="OUT_MD", default=None,
+                    help="digest rocprofv3 CSVs (--trace, --fetch, --write, --copy-fetch, --copy-write dirs) and exit")
+    for name in ("trace", "fetch", "write", "copy-fetch", "copy-write"):
+        ap.add_argument("--" + name, default=None)
+    ap.add_argument("--title", default="per-kernel HBM traffic")
+    ap.add_argument("--skip", type=float, default=0.0, help="profile table: fraction of early dispatches left out")
     ap.add_argument("--leg", choices=["grads", "grads_batch", "thermal", "batch", "time_to_dp", "parity"],
                     help="run only this leg and print its results (for profiling)")
     args = ap.parse_args()
+    if args.profile_table:
+        profile_table(args.profile_table, args.trace, args.fetch, args.write, args.copy_fetch, args.copy_write,
+                      args.title, args.skip)
+        return
     if args.copy_calibration:
         import torch
 
@@ -419,7 +510,7 @@ def main():
     thermal = None if args.no_thermal else thermal_leg(pkg, cpu=not args.no_cpu)
     batch = None if args.no_batch else batch_leg(pkg, steps=args.batch_steps, plan=args.batch_plan)
     grads_batch = None if args.no_grads else grads_batch_leg(pkg)
-    tdp = None if args.no_fp64 else time_to_dp_leg(pkg, nx, ny)
+    tdp = None if args.no_fp64 or args.no_time_to_dp else time_to_dp_leg(pkg, nx, ny)
     cpu = None if args.no_cpu else cpu_baseline(pkg, nx, ny, args.cpu_seconds)
     parity = None if args.no_cpu else oracle_parity(pkg, nx, ny, args.parity_iters)
 

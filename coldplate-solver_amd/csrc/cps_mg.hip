@@ -340,9 +340,9 @@ struct Tile {
 // The region positions a thread visits in every pass over the tile
 // region (q = t + i FT, i < TNP), located once per kernel: the ring of each
 // (distance to the region border, -1 past it) and whether its cell lies in
-// the level.  With the inverse diagonals staged once (si), a sweep over the
-// ring >= h cells (h >= 1) takes no index arithmetic, no bounds tests and no
-// division.  Cells outside the level hold zero faces, zero b and zero x, so
+// the level.  With the inverse diagonals staged once (si, by the first
+// sweep of the kernel), a sweep over the ring >= h cells (h >= 1) takes no
+// index arithmetic, no bounds tests and no division.  Cells outside the level hold zero faces, zero b and zero x, so
 // their terms in a neighbour's stencil vanish exactly and the east / north
 // faces of the last column / row meet a zero x: the sums equal the guarded
 // ones of Tile.
@@ -372,32 +372,24 @@ struct TileCells {
     return tl->se[q] * x[q + 1] + tl->se[q - 1] * x[q - 1] + tl->sn[q] * x[q + RW] + tl->sn[q - RW] * x[q - RW];
   }
   __device__ T ax(const T* x, int q, T d) const { return d * x[q] - nb(x, q); }
-  // si on the ring >= 1 positions (the faces must be visible to the block)
-  __device__ void stage_inv(int t) const {
-#pragma unroll
-    for (int i = 0; i < TNP; ++i)
-      if (ring[i] >= 1) {
-        const int q = t + i * FT;
-        si[q] = (T)1 / diag(q);
-      }
-  }
-  // x0 = om b / diag on ring >= h from a zero guess, 0 outside the level
-  __device__ void first(T* dst, int h, int t, T om) const {
-#pragma unroll
-    for (int i = 0; i < TNP; ++i)
-      if (ring[i] >= h) {
-        const int q = t + i * FT;
-        dst[q] = (in >> i & 1u) ? om * tl->sb[q] * si[q] : (T)0;
-      }
-  }
   // one damped-Jacobi sweep src -> dst on ring >= h:
-  // x' = (1 - om) x + om (b + sum face x neighbour) / diag
+  // x' = (1 - om) x + om (b + sum face x neighbour) / diag.  Stage: the
+  // sweep also stages si on its positions (a thread reads si only at its own
+  // positions until the block's next barrier).
+  template <bool Stage = false>
   __device__ void sweep(const T* src, T* dst, int h, int t, T om) const {
 #pragma unroll
     for (int i = 0; i < TNP; ++i)
       if (ring[i] >= h) {
         const int q = t + i * FT;
-        dst[q] = (in >> i & 1u) ? ((T)1 - om) * src[q] + om * (tl->sb[q] + nb(src, q)) * si[q] : (T)0;
+        T inv;
+        if (Stage) {
+          inv = (T)1 / diag(q);
+          si[q] = inv;
+        } else {
+          inv = si[q];
+        }
+        dst[q] = (in >> i & 1u) ? ((T)1 - om) * src[q] + om * (tl->sb[q] + nb(src, q)) * inv : (T)0;
       }
   }
 };
@@ -799,13 +791,11 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
   if (t == 0) sflag = stop0;
   __syncthreads();
   if (sflag) return;
-  tc.stage_inv(t);
-  __syncthreads();
   if (w) {
     // single-reduction PCG: z on the tile and its halo-1 ring (the same
     // arithmetic as the owner blocks'), w = A z on the tile, partials r.z
     // and z.w
-    tc.sweep(s0, s1, 1, t, (T)kOmega1);  // halo-2 ring
+    tc.template sweep<true>(s0, s1, 1, t, (T)kOmega1);  // halo-2 ring
     __syncthreads();
     tc.sweep(s1, s0, 2, t, (T)kOmega2);  // z on the halo-1 ring
     __syncthreads();
@@ -829,7 +819,7 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
     }
     return;
   }
-  tc.sweep(s0, s1, 2, t, (T)kOmega1);  // halo-1 ring
+  tc.template sweep<true>(s0, s1, 2, t, (T)kOmega1);  // halo-1 ring
   __syncthreads();
   double e = 0.0;
   {

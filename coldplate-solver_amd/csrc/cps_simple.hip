@@ -478,9 +478,64 @@ __global__ __launch_bounds__(NT) void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> s
   const int nb = (int)(gridDim.x * gridDim.y);  // partials per slot (slot stride g.NB >= nb)
   const double qu2 = tst ? parts_load<NT>(p2, nb) : 0.0, qv2 = tst ? parts_load<NT>(p2 + g.NB, nb) : 0.0;
   const double qu1 = tst ? parts_load<NT>(p1, nb) : 0.0, qv1 = tst ? parts_load<NT>(p1 + g.NB, nb) : 0.0;
+  // the flags as thread 0 found them, uniform over the block (a block that
+  // reads a flag block 0 has already cleared takes the same decision: every
+  // block runs the same test on the same partials)
   if (t == 0) { sfl[0] = f0; sfl[1] = f1; }
   __syncthreads();
-  int au = sfl[0], av = sfl[1];
+  const int au0 = sfl[0], av0 = sfl[1];
+  if (!au0 && !av0) return;  // block-uniform
+  const int M = g.M, N = g.N, P = g.P;
+  const int lt = xcd_tile();
+  const int i0 = 1 + lt / (int)gridDim.x * BY, j0 = 1 + lt % (int)gridDim.x * BX;
+  auto actU = [&](int i, int j) { return i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2; };
+  auto actV = [&](int i, int j) { return i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1; };
+  const C* xu = jac_buf(su, k);
+  const C* xv = jac_buf(sv, k);
+  // the tile cell and (t < JRING) one ring cell of sweep k, in region-1 coordinates
+  const int tr = (int)threadIdx.y + 1, tc = (int)threadIdx.x + 1;
+  int rr = 0, rc = 0;
+  if (t < JR1W) {
+    rc = t;
+  } else if (t < 2 * JR1W) {
+    rr = JR1H - 1;
+    rc = t - JR1W;
+  } else {
+    rr = 1 + ((t - 2 * JR1W) >> 1);
+    rc = ((t - 2 * JR1W) & 1) ? JR1W - 1 : 0;
+  }
+  const bool ring = t < JRING;
+  const int ti = i0 - 1 + tr, tj = j0 - 1 + tc, ri = i0 - 1 + rr, rj = j0 - 1 + rc;
+  // every global load first, for the components active at entry; the stop
+  // tests below may retire a component (its loaded values then go unused)
+  bool tu = au0 && actU(ti, tj), tv = av0 && actV(ti, tj);
+  bool gu = ring && au0 && actU(ri, rj), gv = ring && av0 && actV(ri, rj);
+  constexpr int NX = (JR2N + NT - 1) / NT;
+  C vxu[NX], vxv[NX], cu[6], cv[6], du[6], dv[6];
+#pragma unroll
+  for (int i = 0; i < NX; ++i) {
+    const int q = t + i * NT;
+    const int lr = q / JR2W, lc = q - (q / JR2W) * JR2W;
+    const int gi = i0 - 2 + lr, gj = j0 - 2 + lc;
+    const bool in = q < JR2N && gi >= 0 && gi <= M && gj >= 0 && gj < P;
+    const size_t kk = in ? (size_t)gi * P + gj : 0;
+    vxu[i] = (in && au0) ? xu[kk] : (C)0;
+    vxv[i] = (in && av0) ? xv[kk] : (C)0;
+  }
+  auto coefs = [&](const MomSys<C>& s, bool on, int i, int j, C* c) {
+    const size_t kk = on ? (size_t)i * P + j : 0;
+    c[0] = on ? s.cS[kk] : (C)0;
+    c[1] = on ? s.cW[kk] : (C)0;
+    c[2] = on ? s.aP[kk] : (C)1;
+    c[3] = on ? s.cE[kk] : (C)0;
+    c[4] = on ? s.cN[kk] : (C)0;
+    c[5] = on ? s.b[kk] : (C)0;
+  };
+  coefs(su, tu, ti, tj, cu);
+  coefs(sv, tv, ti, tj, cv);
+  coefs(su, gu, ri, rj, du);
+  coefs(sv, gv, ri, rj, dv);
+  int au = au0, av = av0;
   if (tst) {  // jacobi_solver.cpp:63 on sweep k-2, and on sweep k-1 if that sweep counts
     const bool b0 = block_id() == 0 && t == 0;
     if (au) {
@@ -503,61 +558,16 @@ __global__ __launch_bounds__(NT) void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> s
     }
   }
   if (!au && !av) return;  // uniform over the grid
-  const int M = g.M, N = g.N, P = g.P;
-  const int lt = xcd_tile();
-  const int i0 = 1 + lt / (int)gridDim.x * BY, j0 = 1 + lt % (int)gridDim.x * BX;
-  auto actU = [&](int i, int j) { return i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2; };
-  auto actV = [&](int i, int j) { return i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1; };
-  const C* xu = jac_buf(su, k);
-  const C* xv = jac_buf(sv, k);
-  // the tile cell and (t < JRING) one ring cell of sweep k, in region-1 coordinates
-  const int tr = (int)threadIdx.y + 1, tc = (int)threadIdx.x + 1;
-  int rr = 0, rc = 0;
-  if (t < JR1W) {
-    rc = t;
-  } else if (t < 2 * JR1W) {
-    rr = JR1H - 1;
-    rc = t - JR1W;
-  } else {
-    rr = 1 + ((t - 2 * JR1W) >> 1);
-    rc = ((t - 2 * JR1W) & 1) ? JR1W - 1 : 0;
-  }
-  const bool ring = t < JRING;
-  const int ti = i0 - 1 + tr, tj = j0 - 1 + tc, ri = i0 - 1 + rr, rj = j0 - 1 + rc;
-  const bool tu = au && actU(ti, tj), tv = av && actV(ti, tj);
-  const bool gu = ring && au && actU(ri, rj), gv = ring && av && actV(ri, rj);
-  // every global load first
-  constexpr int NX = (JR2N + NT - 1) / NT;
-  C vxu[NX], vxv[NX], cu[6], cv[6], du[6], dv[6];
-#pragma unroll
-  for (int i = 0; i < NX; ++i) {
-    const int q = t + i * NT;
-    const int lr = q / JR2W, lc = q - (q / JR2W) * JR2W;
-    const int gi = i0 - 2 + lr, gj = j0 - 2 + lc;
-    const bool in = q < JR2N && gi >= 0 && gi <= M && gj >= 0 && gj < P;
-    const size_t kk = in ? (size_t)gi * P + gj : 0;
-    vxu[i] = (in && au) ? xu[kk] : (C)0;
-    vxv[i] = (in && av) ? xv[kk] : (C)0;
-  }
-  auto coefs = [&](const MomSys<C>& s, bool on, int i, int j, C* c) {
-    const size_t kk = on ? (size_t)i * P + j : 0;
-    c[0] = on ? s.cS[kk] : (C)0;
-    c[1] = on ? s.cW[kk] : (C)0;
-    c[2] = on ? s.aP[kk] : (C)1;
-    c[3] = on ? s.cE[kk] : (C)0;
-    c[4] = on ? s.cN[kk] : (C)0;
-    c[5] = on ? s.b[kk] : (C)0;
-  };
-  coefs(su, tu, ti, tj, cu);
-  coefs(sv, tv, ti, tj, cv);
-  coefs(su, gu, ri, rj, du);
-  coefs(sv, gv, ri, rj, dv);
+  tu = tu && au;
+  tv = tv && av;
+  gu = gu && au;
+  gv = gv && av;
 #pragma unroll
   for (int i = 0; i < NX; ++i) {
     const int q = t + i * NT;
     if (q < JR2N) {
-      xa[0][q] = vxu[i];
-      xa[1][q] = vxv[i];
+      xa[0][q] = au ? vxu[i] : (C)0;
+      xa[1][q] = av ? vxv[i] : (C)0;
     }
   }
   __syncthreads();
