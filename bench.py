@@ -366,7 +366,7 @@ def _csv_rows(d, suffix):
 def profile_table(out, trace, fetch, write, copy_fetch, copy_write, title, skip=0.0):
     """Per kernel and grid: dispatches, mean duration (kernel trace, end -
     start; dispatches after the first `skip` fraction of the run), HBM
-    bytes read / written per dispatch (median FETCH_SIZE / WRITE_SIZE in
+    bytes read / written per dispatch (mean FETCH_SIZE / WRITE_SIZE in
     KiB, scaled by the factor that maps the same counter of a 1 GiB device
     copy to 1 GiB) and the HBM GB/s they give."""
     import statistics as st
@@ -389,7 +389,7 @@ def profile_table(out, trace, fetch, write, copy_fetch, copy_write, title, skip=
                 continue
             g = int(r["Grid_Size"]) if r.get("Grid_Size") else grid_of(r)
             acc.setdefault((_short_kernel(r["Kernel_Name"]), g), []).append(float(r["Counter_Value"]))
-        return {k: st.median(v) for k, v in acc.items()}
+        return {k: sum(v) / len(v) for k, v in acc.items()}
 
     def scale(d, name):
         c = counters(d, name)
