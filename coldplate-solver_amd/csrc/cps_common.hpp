@@ -297,6 +297,17 @@ __device__ inline double parts_load(const double* p, int n) {
   if (t + NT < n) a += p[t + NT];
   return a;
 }
+// parts_load for n <= K NT partials: thread t holds p[t] + p[t + NT] + ...
+// + p[t + (K-1) NT] in that order (K = 2: parts_load).
+template <int NT, int K>
+__device__ inline double parts_load_k(const double* p, int n) {
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+  double a = t < n ? p[t] : 0.0;
+#pragma unroll
+  for (int k = 1; k < K; ++k)
+    if (t + k * NT < n) a += p[t + k * NT];
+  return a;
+}
 template <int NT>
 __device__ inline double parts_total(double a, double* sh) {
   const int t = threadIdx.x + threadIdx.y * blockDim.x;

@@ -47,6 +47,12 @@ constexpr int MBX = 64, MBY = 4, MNT = MBX * MBY;
 constexpr double kOmega1 = CPS_MG_OMEGA1;
 constexpr double kOmega2 = CPS_MG_OMEGA2;
 constexpr double kCoarseScale = CPS_MG_COARSE_SCALE;
+#ifndef CPS_MG_UP_ROWS
+#define CPS_MG_UP_ROWS 2
+#endif
+#ifndef CPS_MG_DOWN_ROWS
+#define CPS_MG_DOWN_ROWS 2
+#endif
 #ifndef CPS_COARSE_THREADS
 #define CPS_COARSE_THREADS 1024
 #endif
@@ -294,9 +300,18 @@ struct XrFuse {
   }
 };
 
-// One thread per tile cell: blocks of TX x TY = 64 x 16 threads.
+// One thread per tile cell: blocks of TX x TY = 64 x 16 threads (the fine
+// down and up kernels in FP32: TX x TY / 2 threads, two rows each).
 constexpr int TX = 64, TY = 16, HL = 3, RW = TX + 2 * HL, RH = TY + 2 * HL, RN = RW * RH;
 constexpr int FT = TX * TY;
+template <class T>
+constexpr int kUpRows = CPS_MG_UP_ROWS;
+template <>
+constexpr int kUpRows<double> = 1;  // LDS already holds FP64 blocks to two per CU
+template <class T>
+constexpr int kDownRows = CPS_MG_DOWN_ROWS;
+template <>
+constexpr int kDownRows<double> = 1;
 static_assert(TX == MBX && (TY / 2) * (TX / 2) <= FT, "tile shape");
 
 template <class T>
@@ -338,25 +353,25 @@ struct Tile {
 };
 
 // The region positions a thread visits in every pass over the tile
-// region (q = t + i FT, i < TNP), located once per kernel: the ring of each
+// region (q = t + i NTH, i < NP, NTH threads), located once per kernel: the ring of each
 // (distance to the region border, -1 past it) and whether its cell lies in
 // the level.  With the inverse diagonals staged once (si, by the first
 // sweep of the kernel), a sweep over the ring >= h cells (h >= 1) takes no
-// index arithmetic, no bounds tests and no division.  Cells outside the level hold zero faces, zero b and zero x, so
-// their terms in a neighbour's stencil vanish exactly and the east / north
-// faces of the last column / row meet a zero x: the sums equal the guarded
-// ones of Tile.
-constexpr int TNP = (RN + FT - 1) / FT;
-template <class T>
+// index arithmetic, no bounds tests and no division.  Cells outside the
+// level hold zero faces, zero b and zero x, so their terms in a
+// neighbour's stencil vanish exactly and the east / north faces of the last
+// column / row meet a zero x: the sums equal the guarded ones of Tile.
+template <class T, int NTH = FT>
 struct TileCells {
+  static constexpr int NP = (RN + NTH - 1) / NTH;
   const Tile<T>* tl;
   T* si;
-  int ring[TNP];
+  int ring[NP];
   unsigned in;  // bit i: position i lies in the level
   __device__ TileCells(const Tile<T>* tile, T* inv, int t) : tl(tile), si(inv), in(0u) {
 #pragma unroll
-    for (int i = 0; i < TNP; ++i) {
-      const int q = t + i * FT;
+    for (int i = 0; i < NP; ++i) {
+      const int q = t + i * NTH;
       const int lr = q / RW, lc = q - (q / RW) * RW;
       ring[i] = q < RN ? min(min(lr, RH - 1 - lr), min(lc, RW - 1 - lc)) : -1;
       if (q < RN && tile->inside(tile->gr(lr), tile->gc(lc))) in |= 1u << i;
@@ -379,9 +394,9 @@ struct TileCells {
   template <bool Stage = false>
   __device__ void sweep(const T* src, T* dst, int h, int t, T om) const {
 #pragma unroll
-    for (int i = 0; i < TNP; ++i)
+    for (int i = 0; i < NP; ++i)
       if (ring[i] >= h) {
-        const int q = t + i * FT;
+        const int q = t + i * NTH;
         T inv;
         if (Stage) {
           inv = (T)1 / diag(q);
@@ -396,13 +411,13 @@ struct TileCells {
 
 // Stage b, the faces and (up kernels) x1 + P e of the tile + halo in LDS.
 // All global loads of a thread are issued before the first LDS store.
-template <class T>
+template <class T, int NTH = FT>
 __device__ inline void tile_load(const Lv<T>& L, const Tile<T>& tl, int t, T* sx, const Lv<T>* Cl) {
-  constexpr int NI = (RN + FT - 1) / FT;
+  constexpr int NI = (RN + NTH - 1) / NTH;
   T vb[NI], ve[NI], vn[NI], vx[NI];
 #pragma unroll
   for (int i = 0; i < NI; ++i) {
-    const int q = t + i * FT;
+    const int q = t + i * NTH;
     const int lr = q / RW, lc = q - (q / RW) * RW;
     const int r = tl.gr(lr), c = tl.gc(lc);
     vb[i] = ve[i] = vn[i] = vx[i] = (T)0;
@@ -416,7 +431,7 @@ __device__ inline void tile_load(const Lv<T>& L, const Tile<T>& tl, int t, T* sx
   }
 #pragma unroll
   for (int i = 0; i < NI; ++i) {
-    const int q = t + i * FT;
+    const int q = t + i * NTH;
     if (q < RN) {
       tl.sb[q] = vb[i];
       tl.se[q] = ve[i];
@@ -427,29 +442,32 @@ __device__ inline void tile_load(const Lv<T>& L, const Tile<T>& tl, int t, T* sx
 }
 
 // down: x1 = two Jacobi sweeps from zero (stored in L.x), coarse b = sum of
-// the residuals b - A x1 over each 2x2 aggregate
-template <class T>
-__global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* stop, PcgTest test, XrFuse<T> xr) {
+// the residuals b - A x1 over each 2x2 aggregate.  RT tile rows per thread
+// (blocks of TX x TY / RT threads), as k_mg_up.
+template <class T, int RT>
+__global__ __launch_bounds__(FT / RT) __attribute__((amdgpu_waves_per_eu(8)))
+void k_mg_down(Lv<T> L, Lv<T> Cl, const int* stop, PcgTest test, XrFuse<T> xr) {
+  constexpr int NTH = FT / RT, TYR = TY / RT, KP = 2 * FT / NTH;  // KP: partials per thread (<= 2 FT tiles)
   const size_t zo_ = zoff(L.zs);
   L.zshift(zo_); Cl.zshift(zo_); stop = zsh(stop, zo_); test.zshift(zo_); xr.zshift(zo_);
   __shared__ T sb[RN], se[RN], sn[RN], s0[RN], s1[RN], si[RN];
-  __shared__ double sh[FT / kWave + 1];
-  __shared__ double sh2[2 * (FT / kWave)];
+  __shared__ double sh[NTH / kWave + 1];
+  __shared__ double sh2[2 * (NTH / kWave)];
   __shared__ int sflag;
   const int t = threadIdx.x + threadIdx.y * TX;
   const int stop0 = t == 0 ? *stop : 0;  // thread 0's view, published below
   const int lt = xcd_tile();
   Tile<T> tl{sb, se, sn, lt / (int)gridDim.x * TY, lt % (int)gridDim.x * TX, L.nx, L.ny};
-  const TileCells<T> tc(&tl, si, t);
+  const TileCells<T, NTH> tc(&tl, si, t);
+  constexpr int NI = TileCells<T, NTH>::NP;
   bool x0_ready = false;  // the fused branch forms the first sweep with b
   if (xr.rold) {
-    const double prz = parts_load<FT>(xr.rz, xr.nb);
-    const double pzw = parts_load<FT>(xr.zw, xr.nb);
-    constexpr int NI = (RN + FT - 1) / FT;
+    const double prz = parts_load_k<NTH, KP>(xr.rz, xr.nb);
+    const double pzw = parts_load_k<NTH, KP>(xr.zw, xr.nb);
     T vr[NI], vw[NI], vs[NI], ve[NI], vn[NI];
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
-      const int q = t + i * FT;
+      const int q = t + i * NTH;
       const int lr = q / RW, lc = q - (q / RW) * RW;
       const int r = tl.gr(lr), c = tl.gc(lc);
       vr[i] = vw[i] = vs[i] = ve[i] = vn[i] = (T)0;
@@ -462,21 +480,29 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
         vn[i] = L.ey[k];
       }
     }
-    const int lr0 = HL + threadIdx.y, lc0 = HL + threadIdx.x, q0 = lr0 * RW + lc0;
-    const int r0 = tl.gr(lr0), c0 = tl.gc(lc0);
-    const bool own = tl.inside(r0, c0);
-    const size_t k0 = own ? L.at(r0, c0) : 0;
-    T xo = (T)0, zc = (T)0, po = (T)0, wo = (T)0, so = (T)0;
-    if (own) {
-      xo = xr.xs[k0];
-      zc = xr.z[k0];
-      po = xr.pold[k0];
-      wo = xr.w[k0];
-      so = xr.sold[k0];
+    bool own[RT];
+    size_t k0[RT];
+    int q0[RT];
+    T xo[RT], zc[RT], po[RT], wo[RT], so[RT];
+#pragma unroll
+    for (int o = 0; o < RT; ++o) {
+      const int lr0 = HL + threadIdx.y + o * TYR, lc0 = HL + threadIdx.x;
+      q0[o] = lr0 * RW + lc0;
+      const int r0 = tl.gr(lr0), c0 = tl.gc(lc0);
+      own[o] = tl.inside(r0, c0);
+      k0[o] = own[o] ? L.at(r0, c0) : 0;
+      xo[o] = zc[o] = po[o] = wo[o] = so[o] = (T)0;
+      if (own[o]) {
+        xo[o] = xr.xs[k0[o]];
+        zc[o] = xr.z[k0[o]];
+        po[o] = xr.pold[k0[o]];
+        wo[o] = xr.w[k0[o]];
+        so[o] = xr.sold[k0[o]];
+      }
     }
 #pragma unroll
     for (int i = 0; i < NI; ++i) {  // the faces first: the reduction's barrier publishes them
-      const int q = t + i * FT;
+      const int q = t + i * NTH;
       if (q < RN) {
         se[q] = ve[i];
         sn[q] = vn[i];
@@ -484,8 +510,8 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
     }
     if (t == 0) sflag = stop0;
     double gam = prz, del = pzw;
-    block_sum2_all<FT>(gam, del, sh2);  // its barrier also publishes sflag
-    if (sflag) return;                  // block-uniform
+    block_sum2_all<NTH>(gam, del, sh2);  // its barrier also publishes sflag
+    if (sflag) return;                   // block-uniform
     const CgStep cs = cg_step(xr.ps, xr.it - 1, gam, del);
     if (fabs(cs.den) < kTiny20) {  // p.Ap < 1e-20 (pressure_solver.cpp:161)
       if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) { xr.ps->stop = 1; xr.ps->iters = xr.it - 1; }
@@ -496,10 +522,9 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
       xr.ps->alphas[(xr.it - 1) & 1] = cs.alpha;
     }
     const T alpha = (T)cs.alpha, beta = (T)cs.beta;
-    static_assert(NI == TNP, "region passes");
 #pragma unroll
     for (int i = 0; i < NI; ++i) {  // b, si and the first sweep from zero on the halo-2 ring
-      const int q = t + i * FT;
+      const int q = t + i * NTH;
       if (q < RN) {
         const T b = vr[i] - alpha * (vw[i] + beta * vs[i]);
         sb[q] = b;
@@ -513,25 +538,27 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
     x0_ready = true;
     __syncthreads();
     double e = 0.0;
-    if (own) {
-      const T rn = sb[q0];
-      const T pn = zc + beta * po;
-      st_nt(&xr.xs[k0], (T)(xo + alpha * pn));
-      st_nt(&xr.pnew[k0], pn);
-      st_nt(&xr.snew[k0], (T)(wo + beta * so));
-      st_nt(&L.b[k0], rn);
-      e = (double)rn * (double)rn;
-    }
-    const double sblk = block_sum_t0<FT>(e, sh);  // the barrier after this branch follows
+#pragma unroll
+    for (int o = 0; o < RT; ++o)
+      if (own[o]) {
+        const T rn = sb[q0[o]];
+        const T pn = zc[o] + beta * po[o];
+        st_nt(&xr.xs[k0[o]], (T)(xo[o] + alpha * pn));
+        st_nt(&xr.pnew[k0[o]], pn);
+        st_nt(&xr.snew[k0[o]], (T)(wo[o] + beta * so[o]));
+        st_nt(&L.b[k0[o]], rn);
+        e += (double)rn * (double)rn;
+      }
+    const double sblk = block_sum_t0<NTH>(e, sh);  // the barrier after this branch follows
     if (t == 0) xr.err[lt] = sblk;
   } else {
-  const double perr = test.part ? parts_load<FT>(test.part, test.nb) : 0.0;
-  tile_load<T>(L, tl, t, nullptr, nullptr);
+  const double perr = test.part ? parts_load_k<NTH, KP>(test.part, test.nb) : 0.0;
+  tile_load<T, NTH>(L, tl, t, nullptr, nullptr);
   if (t == 0) sflag = stop0;
   __syncthreads();
   if (sflag) return;  // block-uniform
   if (test.part) {    // fine level: stop test on ||r|| of the previous PCG update
-    const double err = parts_total<FT>(perr, sh);
+    const double err = parts_total<NTH>(perr, sh);
     const bool b0 = blockIdx.x == 0 && blockIdx.y == 0 && t == 0;
     double tol2;
     if (test.it == 0) {  // err = ||r0||^2
@@ -552,9 +579,9 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
     __syncthreads();
     // si and the first sweep from a zero guess on the halo-2 ring: x0 = omega b / diag
 #pragma unroll
-    for (int i = 0; i < TNP; ++i)
+    for (int i = 0; i < NI; ++i)
       if (tc.ring[i] >= 1) {
-        const int q = t + i * FT;
+        const int q = t + i * NTH;
         const T inv = (T)1 / tc.diag(q);
         si[q] = inv;
         s0[q] = (tc.in >> i & 1u) ? (T)kOmega1 * sb[q] * inv : (T)0;
@@ -563,8 +590,9 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
   }
   tc.sweep(s0, s1, 2, t, (T)kOmega2);  // x1 on the halo-1 ring
   __syncthreads();
-  {  // residual of the tile cell, x1 to memory
-    const int lr = HL + threadIdx.y, lc = HL + threadIdx.x, q = lr * RW + lc;
+#pragma unroll
+  for (int o = 0; o < RT; ++o) {  // residual of the tile cells, x1 to memory
+    const int lr = HL + threadIdx.y + o * TYR, lc = HL + threadIdx.x, q = lr * RW + lc;
     const int r = tl.gr(lr), c = tl.gc(lc);
     if (tl.inside(r, c)) {
       s0[q] = sb[q] - tc.ax(s1, q, tc.diag(q));
@@ -574,6 +602,7 @@ __global__ __launch_bounds__(FT) void k_mg_down(Lv<T> L, Lv<T> Cl, const int* st
     }
   }
   __syncthreads();
+  static_assert((TY / 2) * (TX / 2) <= NTH, "one coarse cell per thread");
   const int cr = t / (TX / 2), cc = t - (t / (TX / 2)) * (TX / 2);
   const int R = tl.r0 / 2 + cr, C = tl.c0 / 2 + cc;
   if (t < (TY / 2) * (TX / 2) && R < Cl.ny && C < Cl.nx) {
@@ -775,9 +804,12 @@ __global__ __launch_bounds__(FT) void k_mg_down2(Lv<T> A, Lv<T> B, Lv<T> C, cons
 // up: x2 = x1 + P e (e = the coarse level's result in Cl.y), two Jacobi
 // sweeps, result in L.y; with part != nullptr also the partial b . z of the
 // block (the PCG's r . z on the fine level)
-template <class T>
-__global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, const int* stop, T* w,
-                                             double* part2) {
+// RT tile rows per thread: blocks of TX x TY / RT threads, so more blocks
+// share a CU and one block's load phase overlaps another's sweeps.
+template <class T, int RT>
+__global__ __launch_bounds__(FT / RT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, const int* stop, T* w,
+                                                  double* part2) {
+  constexpr int NTH = FT / RT, TYR = TY / RT;
   const size_t zo_ = zoff(L.zs);
   L.zshift(zo_); Cl.zshift(zo_); part = zsh(part, zo_); stop = zsh(stop, zo_); w = zsh(w, zo_); part2 = zsh(part2, zo_);
   __shared__ T sb[RN], se[RN], sn[RN], s0[RN], s1[RN], si[RN];
@@ -786,8 +818,8 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
   const int stop0 = t == 0 ? *stop : 0;
   const int lt = xcd_tile();
   Tile<T> tl{sb, se, sn, lt / (int)gridDim.x * TY, lt % (int)gridDim.x * TX, L.nx, L.ny};
-  tile_load<T>(L, tl, t, s0, &Cl);
-  const TileCells<T> tc(&tl, si, t);
+  tile_load<T, NTH>(L, tl, t, s0, &Cl);
+  const TileCells<T, NTH> tc(&tl, si, t);
   if (t == 0) sflag = stop0;
   __syncthreads();
   if (sflag) return;
@@ -800,19 +832,22 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
     tc.sweep(s1, s0, 2, t, (T)kOmega2);  // z on the halo-1 ring
     __syncthreads();
     double e = 0.0, e2 = 0.0;
-    const int lr = HL + threadIdx.y, lc = HL + threadIdx.x, q = lr * RW + lc;
-    const int r = tl.gr(lr), c = tl.gc(lc);
-    if (tl.inside(r, c)) {
-      const T z = s0[q];
-      const T wz = tc.ax(s0, q, tc.diag(q));
-      const size_t k = L.at(r, c);
-      st_nt(&L.y[k], z);
-      st_nt(&w[k], wz);
-      e = (double)sb[q] * (double)z;
-      e2 = (double)z * (double)wz;
+#pragma unroll
+    for (int o = 0; o < RT; ++o) {
+      const int lr = HL + threadIdx.y + o * TYR, lc = HL + threadIdx.x, q = lr * RW + lc;
+      const int r = tl.gr(lr), c = tl.gc(lc);
+      if (tl.inside(r, c)) {
+        const T z = s0[q];
+        const T wz = tc.ax(s0, q, tc.diag(q));
+        const size_t k = L.at(r, c);
+        st_nt(&L.y[k], z);
+        st_nt(&w[k], wz);
+        e += (double)sb[q] * (double)z;
+        e2 += (double)z * (double)wz;
+      }
     }
-    __shared__ double sh2[2 * (FT / kWave)];
-    block_sum2_t0<FT>(e, e2, sh2);
+    __shared__ double sh2[2 * (NTH / kWave)];
+    block_sum2_t0<NTH>(e, e2, sh2);
     if (t == 0) {
       part[lt] = e;
       part2[lt] = e2;
@@ -822,8 +857,9 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
   tc.template sweep<true>(s0, s1, 2, t, (T)kOmega1);  // halo-1 ring
   __syncthreads();
   double e = 0.0;
-  {
-    const int lr = HL + threadIdx.y, lc = HL + threadIdx.x, q = lr * RW + lc;
+#pragma unroll
+  for (int o = 0; o < RT; ++o) {
+    const int lr = HL + threadIdx.y + o * TYR, lc = HL + threadIdx.x, q = lr * RW + lc;
     const int r = tl.gr(lr), c = tl.gc(lc);
     if (tl.inside(r, c)) {
       const T z = ((T)1 - (T)kOmega2) * s1[q] + (T)kOmega2 * (sb[q] + tc.nb(s1, q)) * si[q];
@@ -832,8 +868,8 @@ __global__ __launch_bounds__(FT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* part, c
     }
   }
   if (part) {
-    __shared__ double sh[FT / kWave + 1];
-    const double s = block_sum<FT>(e, sh);
+    __shared__ double sh[NTH / kWave + 1];
+    const double s = block_sum<NTH>(e, sh);
     if (t == 0) part[lt] = s;
   }
 }
@@ -1764,9 +1800,11 @@ struct MGImpl {
       const PcgTest dt = (l == 0 && !fused) ? test : none;
       // the timed launches are the fused ones when fusing (the byte count is theirs)
       if (l == 0 && (!fused || it > 0) && timer->active(KT_MG_SMOOTH0))
-        timer->ext_launch(k_mg_down<T>, ctx->zg(tgrid(l)), dim3(TX, TY), 0, s, view(l), view(l + 1), stop, dt, xr);
+        timer->ext_launch(k_mg_down<T, kDownRows<T>>, ctx->zg(tgrid(l)), dim3(TX, TY / kDownRows<T>), 0, s, view(l),
+                          view(l + 1), stop, dt, xr);
       else
-        k_mg_down<T><<<ctx->zg(tgrid(l)), dim3(TX, TY), 0, s>>>(view(l), view(l + 1), stop, dt, xr);
+        k_mg_down<T, kDownRows<T>><<<ctx->zg(tgrid(l)), dim3(TX, TY / kDownRows<T>), 0, s>>>(view(l), view(l + 1),
+                                                                                           stop, dt, xr);
     }
     k_mg_coarse<T><<<ctx->zg(1), kCoarseThreads, coarseSmem, s>>>(view(nGrid), cd, cf.p, stop, fused ? test : none);
     for (int l = nGrid - 1; l >= 0; --l) {
@@ -1775,7 +1813,7 @@ struct MGImpl {
         --l;
         continue;
       }
-      k_mg_up<T><<<ctx->zg(tgrid(l)), dim3(TX, TY), 0, s>>>(view(l), view(l + 1),
+      k_mg_up<T, kUpRows<T>><<<ctx->zg(tgrid(l)), dim3(TX, TY / kUpRows<T>), 0, s>>>(view(l), view(l + 1),
                                                   l == 0 ? part.p + (size_t)S_RZ * G0.NB : nullptr, stop,
                                                   l == 0 && fused ? wz.p : nullptr,
                                                   part.p + (size_t)S_ZW * G0.NB);
