@@ -437,6 +437,12 @@ constexpr int JR1W = BX + 2, JR1H = BY + 2, JR1N = JR1W * JR1H;  // sweep-k regi
 constexpr int JR2W = BX + 4, JR2H = BY + 4, JR2N = JR2W * JR2H;  // x_k region
 constexpr int JRING = JR1N - BX * BY;                             // ring cells of sweep k
 static_assert(JRING == 2 * JR1W + 2 * BY && JRING <= NT, "one ring cell per thread");
+#ifndef CPS_JAC_ROWS
+#define CPS_JAC_ROWS 2
+#endif
+// tile rows per thread of the paired sweeps (blocks of BX x BY / kJacRows
+// threads: more blocks per CU, one block's loads overlapping another's sweeps)
+constexpr int kJacRows = CPS_JAC_ROWS;
 
 template <class C>
 __device__ __forceinline__ C* jac_buf(const MomSys<C>& s, int j) {
@@ -461,14 +467,16 @@ __device__ __forceinline__ C jac_cell(const C* X, int W, int q, const C* c, C& r
   return X[q] + r / c[2];
 }
 
-template <class C>
-__global__ __launch_bounds__(NT) void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> sv, int k, int two, double tol2,
-                                                IterState* st, double* part) {
+template <class C, int RT>
+__global__ __launch_bounds__(NT / RT)
+void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> sv, int k, int two, double tol2, IterState* st, double* part) {
+  constexpr int NTH = NT / RT, BYR = BY / RT, KP = 2 * NT / NTH;  // KP: partials per thread (<= 2048 tiles)
+  static_assert(JRING <= NTH, "one ring cell per thread");
   const size_t zo_ = zoff(g.zs);
   su.zshift(zo_); sv.zshift(zo_); st = zsh(st, zo_); part = zsh(part, zo_);
   __shared__ C xa[2][JR2N];  // x_k, U and V
   __shared__ C xb[2][JR1N];  // x_(k+1)
-  __shared__ double sh[NT / kWave + 1];
+  __shared__ double sh[NTH / kWave + 1];
   __shared__ int sfl[2];
   const int t = thread_id();
   const int f0 = t == 0 ? st->jac_active[0] : 0, f1 = t == 0 ? st->jac_active[1] : 0;
@@ -476,8 +484,8 @@ __global__ __launch_bounds__(NT) void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> s
   const double* p2 = part + (size_t)(SLOT_JAC4 + 2 * ((k + 2) & 3)) * g.NB;  // sweep k-2
   const double* p1 = part + (size_t)(SLOT_JAC4 + 2 * ((k + 3) & 3)) * g.NB;  // sweep k-1
   const int nb = (int)(gridDim.x * gridDim.y);  // partials per slot (slot stride g.NB >= nb)
-  const double qu2 = tst ? parts_load<NT>(p2, nb) : 0.0, qv2 = tst ? parts_load<NT>(p2 + g.NB, nb) : 0.0;
-  const double qu1 = tst ? parts_load<NT>(p1, nb) : 0.0, qv1 = tst ? parts_load<NT>(p1 + g.NB, nb) : 0.0;
+  const double qu2 = tst ? parts_load_k<NTH, KP>(p2, nb) : 0.0, qv2 = tst ? parts_load_k<NTH, KP>(p2 + g.NB, nb) : 0.0;
+  const double qu1 = tst ? parts_load_k<NTH, KP>(p1, nb) : 0.0, qv1 = tst ? parts_load_k<NTH, KP>(p1 + g.NB, nb) : 0.0;
   // the flags as thread 0 found them, uniform over the block (a block that
   // reads a flag block 0 has already cleared takes the same decision: every
   // block runs the same test on the same partials)
@@ -492,8 +500,8 @@ __global__ __launch_bounds__(NT) void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> s
   auto actV = [&](int i, int j) { return i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1; };
   const C* xu = jac_buf(su, k);
   const C* xv = jac_buf(sv, k);
-  // the tile cell and (t < JRING) one ring cell of sweep k, in region-1 coordinates
-  const int tr = (int)threadIdx.y + 1, tc = (int)threadIdx.x + 1;
+  // the RT tile cells and (t < JRING) one ring cell of sweep k, in region-1 coordinates
+  const int tc = (int)threadIdx.x + 1;
   int rr = 0, rc = 0;
   if (t < JR1W) {
     rc = t;
@@ -505,16 +513,23 @@ __global__ __launch_bounds__(NT) void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> s
     rc = ((t - 2 * JR1W) & 1) ? JR1W - 1 : 0;
   }
   const bool ring = t < JRING;
-  const int ti = i0 - 1 + tr, tj = j0 - 1 + tc, ri = i0 - 1 + rr, rj = j0 - 1 + rc;
+  const int tj = j0 - 1 + tc, ri = i0 - 1 + rr, rj = j0 - 1 + rc;
   // every global load first, for the components active at entry; the stop
   // tests below may retire a component (its loaded values then go unused)
-  bool tu = au0 && actU(ti, tj), tv = av0 && actV(ti, tj);
+  int ti[RT];
+  bool tu[RT], tv[RT];
+#pragma unroll
+  for (int o = 0; o < RT; ++o) {
+    ti[o] = i0 - 1 + (int)threadIdx.y + 1 + o * BYR;
+    tu[o] = au0 && actU(ti[o], tj);
+    tv[o] = av0 && actV(ti[o], tj);
+  }
   bool gu = ring && au0 && actU(ri, rj), gv = ring && av0 && actV(ri, rj);
-  constexpr int NX = (JR2N + NT - 1) / NT;
-  C vxu[NX], vxv[NX], cu[6], cv[6], du[6], dv[6];
+  constexpr int NX = (JR2N + NTH - 1) / NTH;
+  C vxu[NX], vxv[NX], cu[RT][6], cv[RT][6], du[6], dv[6];
 #pragma unroll
   for (int i = 0; i < NX; ++i) {
-    const int q = t + i * NT;
+    const int q = t + i * NTH;
     const int lr = q / JR2W, lc = q - (q / JR2W) * JR2W;
     const int gi = i0 - 2 + lr, gj = j0 - 2 + lc;
     const bool in = q < JR2N && gi >= 0 && gi <= M && gj >= 0 && gj < P;
@@ -531,40 +546,52 @@ __global__ __launch_bounds__(NT) void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> s
     c[4] = on ? s.cN[kk] : (C)0;
     c[5] = on ? s.b[kk] : (C)0;
   };
-  coefs(su, tu, ti, tj, cu);
-  coefs(sv, tv, ti, tj, cv);
-  coefs(su, gu, ri, rj, du);
-  coefs(sv, gv, ri, rj, dv);
+#pragma unroll
+  for (int o = 0; o < RT; ++o) {
+    coefs(su, tu[o], ti[o], tj, cu[o]);
+    coefs(sv, tv[o], ti[o], tj, cv[o]);
+  }
+  if (RT == 1) {  // with more tile rows per thread the ring's loads wait for the tests (registers)
+    coefs(su, gu, ri, rj, du);
+    coefs(sv, gv, ri, rj, dv);
+  }
   int au = au0, av = av0;
   if (tst) {  // jacobi_solver.cpp:63 on sweep k-2, and on sweep k-1 if that sweep counts
     const bool b0 = block_id() == 0 && t == 0;
     if (au) {
-      if (parts_total<NT>(qu2, sh) < tol2 * (double)g.nU) {
+      if (parts_total<NTH>(qu2, sh) < tol2 * (double)g.nU) {
         au = 0;
         if (b0) { st->jac_active[0] = 0; st->jac_iters[0] = k - 1; }
-      } else if (parts_total<NT>(qu1, sh) < tol2 * (double)g.nU) {
+      } else if (parts_total<NTH>(qu1, sh) < tol2 * (double)g.nU) {
         au = 0;
         if (b0) { st->jac_active[0] = 0; st->jac_iters[0] = k; }
       }
     }
     if (av) {
-      if (parts_total<NT>(qv2, sh) < tol2 * (double)g.nV) {
+      if (parts_total<NTH>(qv2, sh) < tol2 * (double)g.nV) {
         av = 0;
         if (b0) { st->jac_active[1] = 0; st->jac_iters[1] = k - 1; }
-      } else if (parts_total<NT>(qv1, sh) < tol2 * (double)g.nV) {
+      } else if (parts_total<NTH>(qv1, sh) < tol2 * (double)g.nV) {
         av = 0;
         if (b0) { st->jac_active[1] = 0; st->jac_iters[1] = k; }
       }
     }
   }
   if (!au && !av) return;  // uniform over the grid
-  tu = tu && au;
-  tv = tv && av;
+#pragma unroll
+  for (int o = 0; o < RT; ++o) {
+    tu[o] = tu[o] && au;
+    tv[o] = tv[o] && av;
+  }
   gu = gu && au;
   gv = gv && av;
+  if (RT > 1) {
+    coefs(su, gu, ri, rj, du);
+    coefs(sv, gv, ri, rj, dv);
+  }
 #pragma unroll
   for (int i = 0; i < NX; ++i) {
-    const int q = t + i * NT;
+    const int q = t + i * NTH;
     if (q < JR2N) {
       xa[0][q] = au ? vxu[i] : (C)0;
       xa[1][q] = av ? vxv[i] : (C)0;
@@ -573,35 +600,38 @@ __global__ __launch_bounds__(NT) void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> s
   __syncthreads();
   // sweep k: x_(k+1) on the tile (to memory) and the ring (LDS only)
   double ru = 0.0, rv = 0.0;
-  const int qt2 = (tr + 1) * JR2W + tc + 1, qt1 = tr * JR1W + tc;
   const int qr2 = (rr + 1) * JR2W + rc + 1, qr1 = rr * JR1W + rc;
-  const size_t kt = (size_t)ti * P + tj;
-  {
+#pragma unroll
+  for (int o = 0; o < RT; ++o) {
+    const int tr = (int)threadIdx.y + 1 + o * BYR;
+    const int qt2 = (tr + 1) * JR2W + tc + 1, qt1 = tr * JR1W + tc;
+    const size_t kt = (size_t)ti[o] * P + tj;
     C r;
-    if (tu) {
-      const C x = jac_cell(xa[0], JR2W, qt2, cu, r);
+    if (tu[o]) {
+      const C x = jac_cell(xa[0], JR2W, qt2, cu[o], r);
       xb[0][qt1] = x;
       jac_buf(su, k + 1)[kt] = x;
-      ru = (double)r * (double)r;
+      ru += (double)r * (double)r;
     } else {
       xb[0][qt1] = xa[0][qt2];
     }
-    if (tv) {
-      const C x = jac_cell(xa[1], JR2W, qt2, cv, r);
+    if (tv[o]) {
+      const C x = jac_cell(xa[1], JR2W, qt2, cv[o], r);
       xb[1][qt1] = x;
       jac_buf(sv, k + 1)[kt] = x;
-      rv = (double)r * (double)r;
+      rv += (double)r * (double)r;
     } else {
       xb[1][qt1] = xa[1][qt2];
     }
-    if (ring) {
-      xb[0][qr1] = gu ? jac_cell(xa[0], JR2W, qr2, du, r) : xa[0][qr2];
-      xb[1][qr1] = gv ? jac_cell(xa[1], JR2W, qr2, dv, r) : xa[1][qr2];
-    }
+  }
+  if (ring) {
+    C r;
+    xb[0][qr1] = gu ? jac_cell(xa[0], JR2W, qr2, du, r) : xa[0][qr2];
+    xb[1][qr1] = gv ? jac_cell(xa[1], JR2W, qr2, dv, r) : xa[1][qr2];
   }
   {
-    const double su2 = block_sum<NT>(ru, sh);
-    const double sv2 = block_sum<NT>(rv, sh);
+    const double su2 = block_sum<NTH>(ru, sh);
+    const double sv2 = block_sum<NTH>(rv, sh);
     if (t == 0) {
       double* pout = part + (size_t)(SLOT_JAC4 + 2 * (k & 3)) * g.NB;
       pout[lt] = su2;
@@ -613,19 +643,23 @@ __global__ __launch_bounds__(NT) void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> s
   // sweep k+1 on the tile
   ru = 0.0;
   rv = 0.0;
-  {
+#pragma unroll
+  for (int o = 0; o < RT; ++o) {
+    const int tr = (int)threadIdx.y + 1 + o * BYR;
+    const int qt1 = tr * JR1W + tc;
+    const size_t kt = (size_t)ti[o] * P + tj;
     C r;
-    if (tu) {
-      jac_buf(su, k + 2)[kt] = jac_cell(xb[0], JR1W, qt1, cu, r);
-      ru = (double)r * (double)r;
+    if (tu[o]) {
+      jac_buf(su, k + 2)[kt] = jac_cell(xb[0], JR1W, qt1, cu[o], r);
+      ru += (double)r * (double)r;
     }
-    if (tv) {
-      jac_buf(sv, k + 2)[kt] = jac_cell(xb[1], JR1W, qt1, cv, r);
-      rv = (double)r * (double)r;
+    if (tv[o]) {
+      jac_buf(sv, k + 2)[kt] = jac_cell(xb[1], JR1W, qt1, cv[o], r);
+      rv += (double)r * (double)r;
     }
   }
-  const double su2 = block_sum<NT>(ru, sh);
-  const double sv2 = block_sum<NT>(rv, sh);
+  const double su2 = block_sum<NTH>(ru, sh);
+  const double sv2 = block_sum<NTH>(rv, sh);
   if (t == 0) {
     double* pout = part + (size_t)(SLOT_JAC4 + 2 * ((k + 1) & 3)) * g.NB;
     pout[lt] = su2;
@@ -1514,8 +1548,8 @@ class SimpleSolver {
                        jgrid.x > 0 && jgrid.y > 0 && (int)(jgrid.x * jgrid.y) <= g.NB;
     if (pairs) {
       for (; k < maxit; k += 2)
-        k_jacobi2<float><<<ctx->zg(jgrid), block, 0, s>>>(g, sys(momF, 0), sys(momF, 1), k, k + 1 < maxit ? 1 : 0, tol2,
-                                                 st.p, part.p);
+        k_jacobi2<float, kJacRows><<<ctx->zg(jgrid), dim3(BX, BY / kJacRows), 0, s>>>(
+            g, sys(momF, 0), sys(momF, 1), k, k + 1 < maxit ? 1 : 0, tol2, st.p, part.p);
       CPS_HIP(hipGetLastError());
     }
     while (!pairs && k < maxit) {
