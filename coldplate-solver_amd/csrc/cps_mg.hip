@@ -10,8 +10,11 @@
 // sum of the fine faces it covers (half the Galerkin P0 operator, which is
 // the rediscretised operator for smooth coefficients; it keeps the Brinkman
 // 1e8 coefficient contrast because faces are summed, not averaged).
-// Smoother: damped Jacobi (omega 0.8), 2 pre + 2 post sweeps, so the V-cycle
-// is symmetric and PCG-safe.  Levels of more than 512 cells run as grid
+// The last coarse column's east (outlet) face carries an extra factor 1.25 per
+// level (kDirichletScale below).
+// Smoother: two Jacobi sweeps with weights 0.6 and 1.1 (a degree-2
+// polynomial in D^-1 A damping the upper part of its spectrum), 2 pre + 2
+// post, so the V-cycle is symmetric and PCG-safe.  Levels of more than 512 cells run as grid
 // kernels (below the finest, two levels per launch where a pair remains);
 // the smaller ones run inside one workgroup in LDS (k_mg_coarse) down to the
 // first level of at most 32 cells, which is solved exactly with its inverse
@@ -31,13 +34,16 @@ namespace {
 
 constexpr int MBX = 64, MBY = 4, MNT = MBX * MBY;
 #ifndef CPS_MG_OMEGA1
-#define CPS_MG_OMEGA1 0.8
+#define CPS_MG_OMEGA1 0.6
 #endif
 #ifndef CPS_MG_OMEGA2
-#define CPS_MG_OMEGA2 0.8
+#define CPS_MG_OMEGA2 1.1
 #endif
 #ifndef CPS_MG_COARSE_SCALE
 #define CPS_MG_COARSE_SCALE 0.5
+#endif
+#ifndef CPS_MG_DIRICHLET_SCALE
+#define CPS_MG_DIRICHLET_SCALE 1.25
 #endif
 // weights of the two Jacobi sweeps of each smoothing pass (pre: the sweep
 // from zero, then the second; post: the sweep after the prolongation, then
@@ -47,6 +53,18 @@ constexpr int MBX = 64, MBY = 4, MNT = MBX * MBY;
 constexpr double kOmega1 = CPS_MG_OMEGA1;
 constexpr double kOmega2 = CPS_MG_OMEGA2;
 constexpr double kCoarseScale = CPS_MG_COARSE_SCALE;
+// The east face of a level's last column couples it to the pinned outlet
+// column p'(., N-1) = 0.  On a coarse level that column's centre lies half a
+// coarse cell or less from the pinned one, while its face sum keeps the fine
+// face's size, so kCoarseScale alone under-weights the outlet and the
+// smoothest error mode (the only one the outlet pins) is over-corrected.
+// The last coarse column's east face carries this extra factor.
+constexpr double kDirichletScale = CPS_MG_DIRICHLET_SCALE;
+// the factor of a coarse east face whose covered fine column is fcol of a
+// finer level fnx wide (the last coarse column covers the last fine one)
+__host__ __device__ constexpr double coarse_e_scale(int fcol, int fnx) {
+  return fcol >= fnx - 1 ? kCoarseScale * kDirichletScale : kCoarseScale;
+}
 #ifndef CPS_MG_UP_ROWS
 #define CPS_MG_UP_ROWS 2
 #endif
@@ -119,7 +137,7 @@ __device__ inline T coarse_ex(const Lv<T>& F, int R, int C) {
   const int col = min(2 * C + 1, F.nx - 1);
   T s = F.ex[F.at(2 * R, col)];
   if (2 * R + 1 < F.ny) s += F.ex[F.at(2 * R + 1, col)];
-  return (T)kCoarseScale * s;
+  return (T)coarse_e_scale(col, F.nx) * s;
 }
 template <class T>
 __device__ inline T coarse_ey(const Lv<T>& F, int R, int C) {
@@ -195,7 +213,7 @@ __global__ __launch_bounds__(HT) void k_mg_hier(Geo g, const double* dE, const d
         n = fn[(2 * lr + 1) * fw + 2 * lc];
         if (2 * C + 1 < F.nx) n += fn[(2 * lr + 1) * fw + 2 * lc + 1];
       }
-      e = (T)kCoarseScale * e;
+      e = (T)coarse_e_scale(min(2 * C + 1, F.nx - 1), F.nx) * e;
       n = (T)kCoarseScale * n;
       const size_t k = Cl.at(R, C);
       zsh(Cl.ex, zo_)[k] = e;
@@ -284,6 +302,7 @@ struct XrFuse {
   double* err;       // ||r_new||^2 partials
   int nb, it;        // it: this V-cycle; the update is iteration it - 1's
   PcgScal* ps;
+  int* ran;  // nullptr, or set to 1 by a launch that passes its stop tests (launch timing; not shifted)
   __device__ void zshift(size_t o) {
     rold = zsh(rold, o);
     w = zsh(w, o);
@@ -454,6 +473,10 @@ void k_mg_down(Lv<T> L, Lv<T> Cl, const int* stop, PcgTest test, XrFuse<T> xr) {
   __shared__ double sh[NTH / kWave + 1];
   __shared__ double sh2[2 * (NTH / kWave)];
   __shared__ int sflag;
+  // a fused launch enqueued after its PCG stopped exits before loading its
+  // tile (uniform scalar test); the test below (after the loads) remains for
+  // the unfused path, which stops on its own test
+  if (xr.rold && *stop) return;
   const int t = threadIdx.x + threadIdx.y * TX;
   const int stop0 = t == 0 ? *stop : 0;  // thread 0's view, published below
   const int lt = xcd_tile();
@@ -518,6 +541,7 @@ void k_mg_down(Lv<T> L, Lv<T> Cl, const int* stop, PcgTest test, XrFuse<T> xr) {
       return;
     }
     if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) {
+      if (xr.ran) *xr.ran = 1;
       xr.ps->rzs[(xr.it - 1) & 1] = gam;
       xr.ps->alphas[(xr.it - 1) & 1] = cs.alpha;
     }
@@ -574,6 +598,7 @@ void k_mg_down(Lv<T> L, Lv<T> Cl, const int* stop, PcgTest test, XrFuse<T> xr) {
       return;
     }
   }
+  if (xr.ran && blockIdx.x == 0 && blockIdx.y == 0 && t == 0) *xr.ran = 1;
   }
   if (!x0_ready) {  // block-uniform
     __syncthreads();
@@ -703,6 +728,7 @@ __global__ __launch_bounds__(FT) void k_mg_down2(Lv<T> A, Lv<T> B, Lv<T> C, cons
   T* s1B = s0B + PBN;
   __shared__ int sflag;
   const int t = threadIdx.x + threadIdx.y * TX;
+  if (*stop) return;  // uniform: nothing to load once the PCG has stopped
   const int stop0 = t == 0 ? *stop : 0;
   const int rB0 = (int)blockIdx.y * PBY, cB0 = (int)blockIdx.x * PBX;
   const Region<T, 2 * PBX, 2 * PBY, PHA> ra{sbA, seA, snA, 2 * rB0, 2 * cB0, A.nx, A.ny};
@@ -815,6 +841,7 @@ __global__ __launch_bounds__(FT / RT) void k_mg_up(Lv<T> L, Lv<T> Cl, double* pa
   __shared__ T sb[RN], se[RN], sn[RN], s0[RN], s1[RN], si[RN];
   __shared__ int sflag;
   const int t = threadIdx.x + threadIdx.y * TX;
+  if (*stop) return;  // uniform: nothing to load once the PCG has stopped
   const int stop0 = t == 0 ? *stop : 0;
   const int lt = xcd_tile();
   Tile<T> tl{sb, se, sn, lt / (int)gridDim.x * TY, lt % (int)gridDim.x * TX, L.nx, L.ny};
@@ -909,6 +936,7 @@ __global__ __launch_bounds__(FT) void k_mg_up2(Lv<T> A, Lv<T> B, Lv<T> C, const 
   T* s1B = s0B + UBN;
   __shared__ int sflag;
   const int t = threadIdx.x + threadIdx.y * TX;
+  if (*stop) return;  // uniform: nothing to load once the PCG has stopped
   const int stop0 = t == 0 ? *stop : 0;
   const int rA0 = (int)blockIdx.y * TY, cA0 = (int)blockIdx.x * TX;
   const int rB0 = rA0 / 2, cB0 = cA0 / 2;
@@ -1036,7 +1064,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, L
         n = pn[row * fx + 2 * c];
         if (2 * c + 1 < fx) n += pn[row * fx + 2 * c + 1];
       }
-      se[r * d.nx[0] + c] = (T)kCoarseScale * e;
+      se[r * d.nx[0] + c] = (T)coarse_e_scale(col, fx) * e;
       sn[r * d.nx[0] + c] = (T)kCoarseScale * n;
     }
   } else {
@@ -1058,7 +1086,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, L
         n = sn[fo + row * fx + 2 * c];
         if (2 * c + 1 < fx) n += sn[fo + row * fx + 2 * c + 1];
       }
-      se[co + r * cx + c] = (T)kCoarseScale * e;
+      se[co + r * cx + c] = (T)coarse_e_scale(col, fx) * e;
       sn[co + r * cx + c] = (T)kCoarseScale * n;
     }
     __syncthreads();
@@ -1799,10 +1827,11 @@ struct MGImpl {
       if (l == 0 && fused && it > 0) xr = cg_update(it - 1);
       const PcgTest dt = (l == 0 && !fused) ? test : none;
       // the timed launches are the fused ones when fusing (the byte count is theirs)
-      if (l == 0 && (!fused || it > 0) && timer->active(KT_MG_SMOOTH0))
+      if (l == 0 && (!fused || it > 0) && timer->active(KT_MG_SMOOTH0)) {
+        xr.ran = timer->ran_slot();
         timer->ext_launch(k_mg_down<T, kDownRows<T>>, ctx->zg(tgrid(l)), dim3(TX, TY / kDownRows<T>), 0, s, view(l),
                           view(l + 1), stop, dt, xr);
-      else
+      } else
         k_mg_down<T, kDownRows<T>><<<ctx->zg(tgrid(l)), dim3(TX, TY / kDownRows<T>), 0, s>>>(view(l), view(l + 1),
                                                                                            stop, dt, xr);
     }

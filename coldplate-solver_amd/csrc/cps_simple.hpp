@@ -79,20 +79,33 @@ enum TimedKernel {
 // Times the launches of one kernel: the selected kernel is dispatched with
 // hipExtLaunchKernel, whose start / stop events are written by the dispatch
 // itself (kernel execution only, like a profiler's kernel trace); drain()
-// sums the pairs.
+// sums the pairs.  A kernel that can exit early on a device stop flag
+// (KT_MG_SMOOTH0: the fine down kernel once its PCG has converged) sets its
+// sample's `ran` slot when it passes its stop tests, and only those samples
+// count, so the average is the duration of launches that did the work.
 struct KernelTimer {
   int which = KT_NONE;
   hipStream_t stream = nullptr;
   std::vector<hipEvent_t> ev;
   size_t used = 0;
-  size_t launches = 0;
+  size_t launches = 0;  // counted samples (that did the work)
+  size_t skipped = 0;   // samples that exited on the stop flag
   double total_ms = 0.0;
   double bytes_per_launch = 0.0;
   // one launch in kSample of the selected kernel carries the events, so the
   // timing barely perturbs the run it measures
   static constexpr size_t kSample = 8;
+  // ran flags of the samples between two drains (device, zeroed by reset and drain)
+  static constexpr size_t kRanSlots = 4096;
+  int* ran_dev = nullptr;
+  std::vector<int> ran_host;
   size_t seen = 0;
-  bool active(int id) { return which == id && (seen++ % kSample) == 0; }
+  bool tracks_ran() const { return which == KT_MG_SMOOTH0; }
+  bool active(int id) {
+    if (which != id || (seen++ % kSample) != 0) return false;
+    return !tracks_ran() || (ran_dev && used / 2 < kRanSlots);
+  }
+  int* ran_slot() { return tracks_ran() && ran_dev ? ran_dev + used / 2 : nullptr; }
   hipEvent_t next_event() {
     if (used == ev.size()) {
       hipEvent_t e;
@@ -105,14 +118,29 @@ struct KernelTimer {
   void ext_launch(F kernel, dim3 grid, dim3 block, unsigned shm, hipStream_t s, Args... args) {
     hipEvent_t a = next_event(), b = next_event();
     hipExtLaunchKernelGGL(kernel, grid, block, shm, s, a, b, 0, args...);
-    launches++;
   }
-  // adds the elapsed time of the recorded (begin, end) pairs to total_ms
+  // adds the elapsed time of the recorded (begin, end) pairs that did the
+  // work to total_ms
   void drain() {
+    const size_t pairs = used / 2;
     if (used >= 2) (void)hipEventSynchronize(ev[used - 1]);
-    for (size_t k = 0; k + 1 < used; k += 2) {
+    const bool rt = tracks_ran() && ran_dev && pairs > 0;
+    if (rt) {
+      ran_host.assign(pairs, 0);
+      if (hipMemcpy(ran_host.data(), ran_dev, pairs * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        ran_host.assign(pairs, 1);
+      (void)hipMemset(ran_dev, 0, pairs * sizeof(int));
+    }
+    for (size_t k = 0; k < pairs; ++k) {
+      if (rt && !ran_host[k]) {
+        ++skipped;
+        continue;
+      }
       float ms = 0.f;
-      if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) == hipSuccess) total_ms += ms;
+      if (hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]) == hipSuccess) {
+        total_ms += ms;
+        ++launches;
+      }
     }
     used = 0;
   }
@@ -120,11 +148,15 @@ struct KernelTimer {
     drain();
     which = id;
     launches = 0;
+    skipped = 0;
     seen = 0;
     total_ms = 0.0;
+    if (tracks_ran() && !ran_dev && hipMalloc((void**)&ran_dev, kRanSlots * sizeof(int)) != hipSuccess) ran_dev = nullptr;
+    if (ran_dev) (void)hipMemset(ran_dev, 0, kRanSlots * sizeof(int));
   }
   ~KernelTimer() {
     for (auto e : ev) (void)hipEventDestroy(e);
+    if (ran_dev) (void)hipFree(ran_dev);
   }
 };
 

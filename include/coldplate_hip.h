@@ -341,7 +341,10 @@ int cps_batch_gradient_dp(cps_batch* b, int field, double rtol, int max_iter, do
 #define CPS_KT_PCG_STENCIL 2  /* fine-level PCG p-update + 5-point apply (classical form: caps above 32) */
 int cps_simple_time_kernel(cps_simple* s, int kernel);
 /* launches timed so far, their mean duration [us] and the algorithmic bytes
- * one launch moves (every operand read once, every result written once). */
+ * one launch moves (every operand read once, every result written once).
+ * For CPS_KT_MG_SMOOTH0 only launches that did the work count: a fine down
+ * kernel enqueued after its PCG converged exits on the device stop flag and
+ * is left out of both figures. */
 int cps_simple_kernel_timing(cps_simple* s, int* launches, double* avg_us, double* bytes_per_launch);
 
 /* ---------------------------------------------------------------------------

@@ -11,7 +11,7 @@ source of truth for the one mode nothing else pins):
                    (400 + 0.6) / 2, a 100 W block source), T solved by it.
 
 The reference module saves a plot into the working directory; it runs in a
-scratch directory under _work/.
+temporary directory (tempfile) that is removed afterwards.
 
 usage: python tests/golden/make_surrogate_golden.py
 """
