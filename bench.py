@@ -363,12 +363,37 @@ def _csv_rows(d, suffix):
             yield from csv.DictReader(fh)
 
 
+def split_early(values, ratio=2.0):
+    """Dispatches enqueued after a device stop flag was set (a converged
+    PCG or Jacobi solve) return at once; their durations and counters sit in
+    a separate cluster well below those of the dispatches that did the work.
+    Returns (values of the upper cluster, number left out): the split is the
+    largest jump between consecutive sorted values when it exceeds `ratio`
+    and the upper cluster's mean is over twice the lower one's."""
+    v = sorted(values)
+    if len(v) < 2:
+        return v, 0
+    best, cut = 0.0, None
+    for i in range(1, len(v)):
+        lo = max(v[i - 1], 1e-12)
+        if v[i] / lo > best:
+            best, cut = v[i] / lo, i
+    if cut is None or best <= ratio:
+        return v, 0
+    low, high = v[:cut], v[cut:]
+    if sum(high) / len(high) <= 2.0 * max(sum(low) / len(low), 1e-12):
+        return v, 0
+    return high, len(low)
+
+
 def profile_table(out, trace, fetch, write, copy_fetch, copy_write, title, skip=0.0):
     """Per kernel and grid: dispatches, mean duration (kernel trace, end -
     start; dispatches after the first `skip` fraction of the run), HBM
     bytes read / written per dispatch (mean FETCH_SIZE / WRITE_SIZE in
     KiB, scaled by the factor that maps the same counter of a 1 GiB device
-    copy to 1 GiB) and the HBM GB/s they give."""
+    copy to 1 GiB) and the HBM GB/s they give.  Dispatches that exited early
+    on a device stop flag (split_early) are counted apart and left out of
+    the means."""
     import statistics as st
 
     def grid_of(r):
@@ -389,7 +414,11 @@ def profile_table(out, trace, fetch, write, copy_fetch, copy_write, title, skip=
                 continue
             g = int(r["Grid_Size"]) if r.get("Grid_Size") else grid_of(r)
             acc.setdefault((_short_kernel(r["Kernel_Name"]), g), []).append(float(r["Counter_Value"]))
-        return {k: sum(v) / len(v) for k, v in acc.items()}
+        out = {}
+        for k, v in acc.items():
+            hi, _ = split_early(v)
+            out[k] = sum(hi) / len(hi)
+        return out
 
     def scale(d, name):
         c = counters(d, name)
@@ -402,12 +431,14 @@ def profile_table(out, trace, fetch, write, copy_fetch, copy_write, title, skip=
     lines = [f"## {title}", "",
              f"HBM bytes: FETCH_SIZE x {fs:.3f}, WRITE_SIZE x {ws:.3f} (KiB -> bytes of a 1 GiB copy); "
              f"GB/s = (read + written) / mean duration; fraction of the 8 TB/s peak.", "",
-             "| kernel | grid (threads) | dispatches | mean us | total ms | % | read MB | written MB | HBM GB/s | of 8 TB/s |",
-             "|---|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
+             "Early exits: dispatches that returned at once on a device stop flag (not in the means).", "",
+             "| kernel | grid (threads) | dispatches | early exits | mean us | total ms | % | read MB | written MB | HBM GB/s | of 8 TB/s |",
+             "|---|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
     for (k, g, gs), v in sorted(dur.items(), key=lambda kv: -sum(kv[1])):
         if not k.startswith("k_"):
             continue
-        mean = sum(v) / len(v)
+        work, nearly = split_early(v)
+        mean = sum(work) / len(work)
         rd = F.get((k, g))
         wr = W.get((k, g))
         if rd is None or wr is None:
@@ -416,7 +447,8 @@ def profile_table(out, trace, fetch, write, copy_fetch, copy_write, title, skip=
             rb, wb = rd * 1024 * fs, wr * 1024 * ws
             gbs = (rb + wb) / (mean * 1e-6) / 1e9
             cells = f"| {rb / 1e6:.2f} | {wb / 1e6:.2f} | {gbs:.0f} | {gbs / 8000:.2f} |"
-        lines.append(f"| {k} | {gs} | {len(v)} | {mean:.2f} | {sum(v) / 1e3:.3f} | {100 * sum(v) / tot:.1f} " + cells)
+        lines.append(f"| {k} | {gs} | {len(v)} | {nearly} | {mean:.2f} | {sum(v) / 1e3:.3f} | {100 * sum(v) / tot:.1f} "
+                     + cells)
     text = "\n".join(lines) + "\n"
     with open(out, "w") as fh:
         fh.write(text)

@@ -24,6 +24,7 @@
 // (FP64), repeated until ||r64|| meets the tolerance.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "cps_mg.hpp"
@@ -919,11 +920,12 @@ template <class T>
 __host__ __device__ constexpr size_t up_pair_smem() {
   return (size_t)(5 * UAN + 5 * UBN) * sizeof(T);
 }
+// The body of k_mg_up2 over LDS at smem_raw (up_pair_smem bytes), pointers
+// already shifted to the block's field.  topy: C's result in LDS (row pitch
+// topw), or nullptr to read it from C.y.
 template <class T>
-__global__ __launch_bounds__(FT) void k_mg_up2(Lv<T> A, Lv<T> B, Lv<T> C, const int* stop) {
-  const size_t zo_ = zoff(A.zs);
-  A.zshift(zo_); B.zshift(zo_); C.zshift(zo_); stop = zsh(stop, zo_);
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+__device__ __forceinline__ void up2_body(const Lv<T>& A, const Lv<T>& B, const Lv<T>& C, const int* stop,
+                                         unsigned char* smem_raw, const T* topy, int topw) {
   T* sbA = reinterpret_cast<T*>(smem_raw);
   T* seA = sbA + UAN;
   T* snA = seA + UAN;
@@ -970,7 +972,7 @@ __global__ __launch_bounds__(FT) void k_mg_up2(Lv<T> A, Lv<T> B, Lv<T> C, const 
         wb[i] = B.b[k];
         we[i] = B.ex[k];
         wn[i] = B.ey[k];
-        wx[i] = B.x[k] + C.y[C.at(r >> 1, c >> 1)];
+        wx[i] = B.x[k] + (topy ? topy[(r >> 1) * topw + (c >> 1)] : C.y[C.at(r >> 1, c >> 1)]);
       }
     }
 #pragma unroll
@@ -1009,6 +1011,13 @@ __global__ __launch_bounds__(FT) void k_mg_up2(Lv<T> A, Lv<T> B, Lv<T> C, const 
       A.y[A.at(r, c)] = s1A[q] + (T)kOmega2 * (sbA[q] - ra.ax(s1A, q, r, c, d)) / d;
     }
   }
+}
+template <class T>
+__global__ __launch_bounds__(FT) void k_mg_up2(Lv<T> A, Lv<T> B, Lv<T> C, const int* stop) {
+  const size_t zo_ = zoff(A.zs);
+  A.zshift(zo_); B.zshift(zo_); C.zshift(zo_); stop = zsh(stop, zo_);
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  up2_body(A, B, C, stop, smem_raw, (const T*)nullptr, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1171,16 +1180,18 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, L
 // workgroup barriers.  The last level (at most kDirectCells cells) is one
 // product with its inverse, from the setup kernel.  The top level's
 // correction goes to top.y.
+// The body of k_mg_coarse for kCoarseThreads threads (t = linear thread
+// index) over LDS at smem_raw (coarse_smem bytes), pointers already shifted
+// to the block's field.  Returns false when the PCG stop test ends the solve;
+// otherwise the top level's correction is in LDS (coarse_topy) and, with
+// write_top, in top.y.  lead: the thread that records the PCG state.
 template <class T>
-__global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseDesc d, const T* cf, const int* stop,
-                                                              PcgTest test) {
-  const size_t zo_ = zoff(top.zs);
-  top.zshift(zo_); cf = zsh(cf, zo_); stop = zsh(stop, zo_); test.zshift(zo_);
-  if (*stop) return;
+__device__ __forceinline__ bool coarse_body(const Lv<T>& top, const CoarseDesc& d, const T* cf, const PcgTest& test,
+                                            unsigned char* smem_raw, int t, bool lead, bool write_top) {
   if (test.part) {  // the PCG stop test when the fine down kernel carries the update
     __shared__ double shr[kCoarseThreads / kWave + 1];
     const double err = parts_total<kCoarseThreads>(parts_load<kCoarseThreads>(test.part, test.nb), shr);
-    const bool b0 = threadIdx.x == 0;
+    const bool b0 = lead;
     double tol2;
     if (test.it == 0) {
       const double tgt = fmax(test.abs_tol, test.rtol * sqrt(err));
@@ -1192,10 +1203,9 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
     if (b0) { test.ps->rr = err; test.ps->iters = test.it; }
     if (err <= tol2) {
       if (b0) test.ps->stop = 1;
-      return;
+      return false;
     }
   }
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   T* sx = reinterpret_cast<T*>(smem_raw);
   T* sy = sx + d.total;
   T* sb = sy + d.total;
@@ -1204,7 +1214,6 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
   T* si = sn + d.total;  // 1 / diagonal
   T* sd = si + d.total;  // diagonal
   T* sA = sd + d.total;  // inverse of the last level's operator, column-major
-  const int t = threadIdx.x;
   {  // every global load of the thread is issued before the LDS stores
     constexpr int NQ = kCoarseTotalMax / kCoarseThreads;  // d.total <= kCoarseTotalMax
     constexpr int NB0 = kCoarseTopMax / kCoarseThreads;   // top level <= kCoarseTopMax cells
@@ -1318,7 +1327,50 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseD
     CELLS(l) { up_sweep2(l, r, c); }
     __syncthreads();
   }
-  CELLS(0) { top.y[top.at(r, c)] = sy[r * d.nx[0] + c]; }
+  if (write_top) {
+    CELLS(0) { top.y[top.at(r, c)] = sy[r * d.nx[0] + c]; }
+  }
+  return true;
+}
+template <class T>
+__global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(Lv<T> top, CoarseDesc d, const T* cf, const int* stop,
+                                                              PcgTest test) {
+  const size_t zo_ = zoff(top.zs);
+  top.zshift(zo_); cf = zsh(cf, zo_); stop = zsh(stop, zo_); test.zshift(zo_);
+  if (*stop) return;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  coarse_body(top, d, cf, test, smem_raw, (int)threadIdx.x, threadIdx.x == 0, true);
+}
+// LDS of coarse_body: seven arrays over the coarse levels and the inverse
+// of the last level; coarse_topy: the top level's correction within it
+template <class T>
+__host__ __device__ inline size_t coarse_smem(const CoarseDesc& d) {
+  return ((size_t)7 * d.total + (size_t)d.dn * d.dn) * sizeof(T);
+}
+template <class T>
+__device__ inline const T* coarse_topy(const unsigned char* smem_raw, const CoarseDesc& d) {
+  return reinterpret_cast<const T*>(smem_raw) + d.total;
+}
+// The coarse solve and the first up leg (two grid levels A, B whose coarser
+// neighbour C is the coarse solver's top) in one launch: every block of the
+// up-pair grid runs the whole coarse V-cycle in its own LDS (the same
+// operations in the same order, so every block holds the same correction,
+// bit for bit) and then its up-pair tile, reading C's correction from LDS.
+// One launch fewer per V-cycle; block (0, 0) records the PCG state.
+static_assert(FT == kCoarseThreads, "the coarse solve and the up pair share one block shape");
+template <class T>
+__global__ __launch_bounds__(FT) void k_mg_coarse_up2(Lv<T> A, Lv<T> B, Lv<T> top, CoarseDesc d, const T* cf,
+                                                      const int* stop, PcgTest test) {
+  const size_t zo_ = zoff(A.zs);
+  A.zshift(zo_); B.zshift(zo_); top.zshift(zo_); cf = zsh(cf, zo_); stop = zsh(stop, zo_); test.zshift(zo_);
+  if (*stop) return;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int t = threadIdx.x + threadIdx.y * TX;
+  const bool first = blockIdx.x == 0 && blockIdx.y == 0;
+  if (!coarse_body(top, d, cf, test, smem_raw, t, first && t == 0, false)) return;
+  __syncthreads();
+  const size_t cs = (coarse_smem<T>(d) + 15) & ~(size_t)15;
+  up2_body(A, B, top, stop, smem_raw + cs, coarse_topy<T>(smem_raw, d), d.nx[0]);
 }
 #undef CELLS
 
@@ -1680,8 +1732,9 @@ struct MGImpl {
   std::vector<std::unique_ptr<Level>> lv;  // grid-kernel levels 0 .. nGrid-1, lv[nGrid] = coarse top
   int nGrid = 0;
   CoarseDesc cd{};
-  size_t coarseSmem = 0, coarseSetupSmem = 0;
-  bool pairOk = false;  // the two-level down kernel's LDS fits
+  size_t coarseSmem = 0, coarseSetupSmem = 0, coarseUpSmem = 0;
+  bool pairOk = false;      // the two-level down kernel's LDS fits
+  bool coarseUpOk = false;  // the coarse solve + first up pair kernel's LDS fits
   DevBuf<T> cf;  // coarse-solver faces and inverse diagonals
   DevBuf<T> r0, r1, xs, pa, pb, Ap;  // PCG vectors, level-0 layout (xs = solution; r0 / r1 residual by parity)
   DevBuf<T> sa, sb, wz;              // single-reduction form: s = A p by parity, w = A z
@@ -1747,6 +1800,11 @@ struct MGImpl {
     CPS_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
     CPS_CHECK(coarseSmem <= (size_t)lds, CPS_ERR_STATE, "coarse multigrid levels exceed LDS");
     pairOk = pair_smem<T>() <= (size_t)lds && up_pair_smem<T>() <= (size_t)lds;
+    coarseUpSmem = ((coarseSmem + 15) & ~(size_t)15) + up_pair_smem<T>();
+    // CPS_MG_SEPARATE_COARSE=1 keeps the coarse solve and the first up pair
+    // in two launches (a validation switch: the results are bit-identical)
+    const char* sep = std::getenv("CPS_MG_SEPARATE_COARSE");
+    coarseUpOk = coarseUpSmem <= (size_t)lds && !(sep && sep[0] == '1');
     cf.alloc((size_t)4 * off + (size_t)cd.dn * cd.dn);
     for (auto* b : {&lv[0]->x, &lv[0]->y, &lv[0]->ex, &lv[0]->ey, &r0, &r1, &xs, &pa, &pb, &Ap, &sa, &sb, &wz})
       CPS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), ctx->stream));
@@ -1835,8 +1893,15 @@ struct MGImpl {
         k_mg_down<T, kDownRows<T>><<<ctx->zg(tgrid(l)), dim3(TX, TY / kDownRows<T>), 0, s>>>(view(l), view(l + 1),
                                                                                            stop, dt, xr);
     }
-    k_mg_coarse<T><<<ctx->zg(1), kCoarseThreads, coarseSmem, s>>>(view(nGrid), cd, cf.p, stop, fused ? test : none);
-    for (int l = nGrid - 1; l >= 0; --l) {
+    int l = nGrid - 1;
+    if (l >= 2 && pairOk && coarseUpOk) {  // the coarse solve and the first up pair in one launch
+      k_mg_coarse_up2<T><<<ctx->zg(tgrid(l - 1)), dim3(TX, TY), coarseUpSmem, s>>>(view(l - 1), view(l), view(nGrid), cd,
+                                                                                  cf.p, stop, fused ? test : none);
+      l -= 2;
+    } else {
+      k_mg_coarse<T><<<ctx->zg(1), kCoarseThreads, coarseSmem, s>>>(view(nGrid), cd, cf.p, stop, fused ? test : none);
+    }
+    for (; l >= 0; --l) {
       if (l >= 2 && pairOk) {  // two mid levels in one launch
         k_mg_up2<T><<<ctx->zg(tgrid(l - 1)), dim3(TX, TY), up_pair_smem<T>(), s>>>(view(l - 1), view(l), view(l + 1), stop);
         --l;

@@ -383,3 +383,28 @@ def test_paired_jacobi_sweeps_match_single(pkg, maxit, tol):
         assert all(u == maxit and v == maxit for u, v in ia)
     for n in fa:
         assert np.array_equal(fa[n], fb[n]), n
+
+
+@pytest.mark.parametrize("nx,ny", [(256, 128), (300, 150), (1024, 512)])
+@pytest.mark.parametrize("precision", [64, 32])
+def test_coarse_solve_fused_with_up_pair_is_bit_identical(pkg, nx, ny, precision, monkeypatch):
+    """Where the first up leg is a two-level kernel, the coarse solve runs
+    inside it (every block repeats the coarse V-cycle in its own LDS); with
+    CPS_MG_SEPARATE_COARSE=1 the two stay separate launches.  The fields,
+    the pressure drop and the iteration counts of 12 SIMPLE steps agree bit
+    for bit."""
+    import bench
+    T = pkg._types
+    out = []
+    for sep in ("1", "0"):
+        monkeypatch.setenv("CPS_MG_SEPARATE_COARSE", sep)
+        cfg, gamma, inlet = bench.perf_config(pkg, nx, ny, precision)
+        print(f"case {nx}x{ny} precision={precision} separate={sep}", flush=True)
+        s = pkg.SIMPLE(cfg, gamma, inlet)
+        its = [s.step(1)["p_iters"] for _ in range(12)]
+        out.append((its, {n: s.get(n) for n in ("u", "v", "p")}, s.full_static_drop()))
+        s.close()
+    assert out[0][0] == out[1][0]
+    for n in ("u", "v", "p"):
+        assert np.array_equal(out[0][1][n], out[1][1][n]), n
+    assert out[0][2] == out[1][2]
