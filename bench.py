@@ -365,23 +365,28 @@ def _csv_rows(d, suffix):
 
 def split_early(values, ratio=2.0):
     """Dispatches enqueued after a device stop flag was set (a converged
-    PCG or Jacobi solve) return at once; their durations and counters sit in
-    a separate cluster well below those of the dispatches that did the work.
-    Returns (values of the upper cluster, number left out): the split is the
-    largest jump between consecutive sorted values when it exceeds `ratio`
-    and the upper cluster's mean is over twice the lower one's."""
+    PCG or Jacobi solve) return at once; their durations and counters form a
+    cluster well below those of the dispatches that did the work.  Two-means
+    clustering of log(value): returns (values of the upper cluster, number
+    left out) when the cluster means differ by more than `ratio` and each
+    cluster holds at least a tenth of the values, else (all values, 0)."""
     v = sorted(values)
     if len(v) < 2:
         return v, 0
-    best, cut = 0.0, None
-    for i in range(1, len(v)):
-        lo = max(v[i - 1], 1e-12)
-        if v[i] / lo > best:
-            best, cut = v[i] / lo, i
-    if cut is None or best <= ratio:
-        return v, 0
+    lg = [math.log(max(x, 1e-12)) for x in v]
+    lo, hi = lg[0], lg[-1]
+    cut = None
+    for _ in range(50):
+        mid = 0.5 * (lo + hi)
+        c = next((i for i, x in enumerate(lg) if x > mid), len(lg))
+        if c == 0 or c == len(lg):
+            return v, 0
+        nlo, nhi = sum(lg[:c]) / c, sum(lg[c:]) / (len(lg) - c)
+        if cut == c:
+            break
+        cut, lo, hi = c, nlo, nhi
     low, high = v[:cut], v[cut:]
-    if sum(high) / len(high) <= 2.0 * max(sum(low) / len(low), 1e-12):
+    if (sum(high) / len(high)) <= ratio * (sum(low) / len(low)) or min(len(low), len(high)) < 0.1 * len(v):
         return v, 0
     return high, len(low)
 

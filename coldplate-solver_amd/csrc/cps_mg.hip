@@ -324,6 +324,30 @@ struct XrFuse {
 // down and up kernels in FP32: TX x TY / 2 threads, two rows each).
 constexpr int TX = 64, TY = 16, HL = 3, RW = TX + 2 * HL, RH = TY + 2 * HL, RN = RW * RH;
 constexpr int FT = TX * TY;
+
+// The PCG stop test of a block of FT threads: perr is this thread's share of
+// the ||r||^2 partials (parts_load<FT>), summed in fixed order, so every
+// block of a launch takes the same decision.  Returns true when the solve
+// has converged; `lead` then has recorded the state and set the stop flag.
+// At it == 0 the partials hold ||r0||^2 and the target is set here.
+__device__ inline bool pcg_converged(const PcgTest& test, double perr, bool lead, double* shr) {
+  const double err = parts_total<FT>(perr, shr);
+  double tol2;
+  if (test.it == 0) {
+    const double tgt = fmax(test.abs_tol, test.rtol * sqrt(err));
+    tol2 = tgt * tgt;
+    if (lead) { test.ps->rr0 = err; test.ps->tol2 = tol2; }
+  } else {
+    tol2 = test.ps->tol2;
+  }
+  if (lead) { test.ps->rr = err; test.ps->iters = test.it; }
+  if (err <= tol2) {
+    if (lead) test.ps->stop = 1;
+    return true;
+  }
+  return false;
+}
+
 template <class T>
 constexpr int kUpRows = CPS_MG_UP_ROWS;
 template <>
@@ -712,10 +736,14 @@ template <class T>
 __host__ __device__ constexpr size_t pair_smem() {
   return (size_t)(5 * PAN + 5 * PBN) * sizeof(T);
 }
+// test.part != nullptr: this launch carries the PCG stop test (the first
+// kernel after a fused fine down kernel), run while its tile loads are in
+// flight; once converged no block smooths, and the later kernels of the
+// V-cycle exit on the stop flag.
 template <class T>
-__global__ __launch_bounds__(FT) void k_mg_down2(Lv<T> A, Lv<T> B, Lv<T> C, const int* stop) {
+__global__ __launch_bounds__(FT) void k_mg_down2(Lv<T> A, Lv<T> B, Lv<T> C, const int* stop, PcgTest test) {
   const size_t zo_ = zoff(A.zs);
-  A.zshift(zo_); B.zshift(zo_); C.zshift(zo_); stop = zsh(stop, zo_);
+  A.zshift(zo_); B.zshift(zo_); C.zshift(zo_); stop = zsh(stop, zo_); test.zshift(zo_);
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   T* sbA = reinterpret_cast<T*>(smem_raw);
   T* seA = sbA + PAN;
@@ -731,6 +759,7 @@ __global__ __launch_bounds__(FT) void k_mg_down2(Lv<T> A, Lv<T> B, Lv<T> C, cons
   const int t = threadIdx.x + threadIdx.y * TX;
   if (*stop) return;  // uniform: nothing to load once the PCG has stopped
   const int stop0 = t == 0 ? *stop : 0;
+  const double perr = test.part ? parts_load<FT>(test.part, test.nb) : 0.0;
   const int rB0 = (int)blockIdx.y * PBY, cB0 = (int)blockIdx.x * PBX;
   const Region<T, 2 * PBX, 2 * PBY, PHA> ra{sbA, seA, snA, 2 * rB0, 2 * cB0, A.nx, A.ny};
   const Region<T, PBX, PBY, PHB> rb{sbB, seB, snB, rB0, cB0, B.nx, B.ny};
@@ -761,6 +790,10 @@ __global__ __launch_bounds__(FT) void k_mg_down2(Lv<T> A, Lv<T> B, Lv<T> C, cons
         we[i] = B.ex[k];
         wn[i] = B.ey[k];
       }
+    }
+    if (test.part) {  // block-uniform decision while the loads are in flight
+      __shared__ double shr[FT / kWave + 1];
+      if (pcg_converged(test, perr, blockIdx.x == 0 && blockIdx.y == 0 && t == 0, shr)) return;
     }
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
@@ -1113,63 +1146,45 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, L
     }
   }
   // Inverse of the last level's operator (the faces and floored diagonal the
-  // V-cycle applies) by Gauss-Jordan in FP64 in wave 0: lane t holds column
-  // t in registers, pivot entries travel by shuffles.  The operator is a
-  // symmetric M-matrix, so no pivoting is needed.
+  // V-cycle applies) by Gauss-Jordan in FP64 over the whole block: thread t
+  // < n^2 holds entry (i, j) = (t / n, t mod n) in a register; each pivot's
+  // row and column travel through LDS (double-buffered, one barrier per
+  // pivot).  The operator is a symmetric M-matrix, so no pivoting is needed.
+  // Entry (i, j) is built from cell j's faces and updated as
+  // a_ij <- (j == p ? 0 : a_ij) - a_ip (j == p ? 1 : a_pj) / a_pp, row p <- (j == p ? 1 : a_pj) / a_pp.
+  __shared__ double gj_row[2][kDirectCells], gj_col[2][kDirectCells];
   __syncthreads();
-  if (t < 64) {
+  {
     const int l = d.wl, o = d.off[l], nx = d.nx[l], ny = d.ny[l], n = d.dn;
-    double col[kDirectCells];
-    const int r = t / max(nx, 1), c = t - r * nx, k = o + t;
-    const bool own = t < n;
-    double dg = 1.0, ee = 0.0, ww = 0.0, nn = 0.0, ss = 0.0;
-    if (own) {
+    const bool mine = t < n * n;
+    const int gi = mine ? t / n : 0, gj = mine ? t - gi * n : 0;
+    double a = 0.0;
+    if (mine) {
+      const int r = gj / max(nx, 1), c = gj - r * nx, k = o + gj;
       const T dT = se[k] + (c > 0 ? se[k - 1] : (T)0) + sn[k] + (r > 0 ? sn[k - nx] : (T)0);
-      dg = dT < (T)kTiny20 ? 1.0 : (double)dT;
-      ee = c + 1 < nx ? -(double)se[k] : 0.0;
-      ww = c > 0 ? -(double)se[k - 1] : 0.0;
-      nn = r + 1 < ny ? -(double)sn[k] : 0.0;
-      ss = r > 0 ? -(double)sn[k - nx] : 0.0;
+      const double dg = dT < (T)kTiny20 ? 1.0 : (double)dT;
+      const double ee = c + 1 < nx ? -(double)se[k] : 0.0;
+      const double ww = c > 0 ? -(double)se[k - 1] : 0.0;
+      const double nn = r + 1 < ny ? -(double)sn[k] : 0.0;
+      const double ss = r > 0 ? -(double)sn[k - nx] : 0.0;
+      if (gi == gj) a = dg;
+      else if (gi == gj + 1) a = ee;
+      else if (gi == gj - 1) a = ww;
+      else if (gi == gj + nx) a = nn;
+      else if (gi == gj - nx) a = ss;
     }
-    // column t of a symmetric matrix = row t
-#pragma unroll
-    for (int i = 0; i < kDirectCells; ++i) {
-      double v = 0.0;
-      if (own) {
-        if (i == t) v = dg;
-        else if (i == t + 1) v = ee;
-        else if (i == t - 1) v = ww;
-        else if (i == t + nx) v = nn;
-        else if (i == t - nx) v = ss;
-      }
-      col[i] = v;
-    }
-    // p is uniform across the wave, so lane p's values are read into scalar
-    // registers (v_readlane) instead of going through LDS permutes
-    auto lane = [](double v, int src) {
-      const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
-      const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
-      return __hiloint2double(hi, lo);
-    };
-    // the pivot loop is unrolled so that every register index is static
-#pragma unroll
-    for (int p = 0; p < kDirectCells; ++p) {
-      if (p < n) {  // wave-uniform
-        const double cp = col[p];  // this column's row-p entry
-        const double inv = 1.0 / lane(cp, p);
-        const double mp = (t == p ? 1.0 : cp) * inv;
-#pragma unroll
-        for (int i = 0; i < kDirectCells; ++i) {
-          const double f = lane(col[i], p);  // A[i][p], before lane p updates it
-          const double v = (t == p ? 0.0 : col[i]) - f * mp;
-          col[i] = i == p ? mp : v;
-        }
+    for (int p = 0; p < n; ++p) {
+      const int b = p & 1;
+      if (mine && gi == p) gj_row[b][gj] = a;
+      if (mine && gj == p) gj_col[b][gi] = a;
+      __syncthreads();
+      if (mine) {
+        const double inv = 1.0 / gj_row[b][p];
+        const double mp = (gj == p ? 1.0 : gj_row[b][gj]) * inv;
+        a = gi == p ? mp : (gj == p ? 0.0 : a) - gj_col[b][gi] * mp;
       }
     }
-    if (own)
-#pragma unroll
-      for (int i = 0; i < kDirectCells; ++i)
-        if (i < n) cf[4 * d.total + t * n + i] = (T)col[i];
+    if (mine) cf[4 * d.total + gj * n + gi] = (T)a;
   }
 }
 
@@ -1190,21 +1205,7 @@ __device__ __forceinline__ bool coarse_body(const Lv<T>& top, const CoarseDesc& 
                                             unsigned char* smem_raw, int t, bool lead, bool write_top) {
   if (test.part) {  // the PCG stop test when the fine down kernel carries the update
     __shared__ double shr[kCoarseThreads / kWave + 1];
-    const double err = parts_total<kCoarseThreads>(parts_load<kCoarseThreads>(test.part, test.nb), shr);
-    const bool b0 = lead;
-    double tol2;
-    if (test.it == 0) {
-      const double tgt = fmax(test.abs_tol, test.rtol * sqrt(err));
-      tol2 = tgt * tgt;
-      if (b0) { test.ps->rr0 = err; test.ps->tol2 = tol2; }
-    } else {
-      tol2 = test.ps->tol2;
-    }
-    if (b0) { test.ps->rr = err; test.ps->iters = test.it; }
-    if (err <= tol2) {
-      if (b0) test.ps->stop = 1;
-      return false;
-    }
+    if (pcg_converged(test, parts_load<kCoarseThreads>(test.part, test.nb), lead, shr)) return false;
   }
   T* sx = reinterpret_cast<T*>(smem_raw);
   T* sy = sx + d.total;
@@ -1874,10 +1875,14 @@ struct MGImpl {
     hipStream_t s = ctx->stream;
     const PcgTest none{nullptr, 0, 0, nullptr, 0.0, 0.0};
     const PcgTest test{part.p + (size_t)(it == 0 ? S_RR0 : S_ERR) * G0.NB, G0.NB, it, scal.p, abs_tol_, rtol_};
+    // fused: the stop test rides on the first kernel after the fine down
+    // kernel, the first level pair if there is one, else the coarse solve
+    const bool pairTest = fused && pairOk && nGrid >= 3;
     for (int l = 0; l < nGrid; ++l) {
       if (l > 0 && l + 1 < nGrid && pairOk) {  // two mid levels in one launch
         k_mg_down2<T><<<ctx->zg(dim3(ceil_div(lv[l + 1]->nx, PBX), ceil_div(lv[l + 1]->ny, PBY))), dim3(TX, TY),
-                        pair_smem<T>(), s>>>(view(l), view(l + 1), view(l + 2), stop);
+                        pair_smem<T>(), s>>>(view(l), view(l + 1), view(l + 2), stop,
+                                             (l == 1 && pairTest) ? test : none);
         ++l;
         continue;
       }
@@ -1896,10 +1901,11 @@ struct MGImpl {
     int l = nGrid - 1;
     if (l >= 2 && pairOk && coarseUpOk) {  // the coarse solve and the first up pair in one launch
       k_mg_coarse_up2<T><<<ctx->zg(tgrid(l - 1)), dim3(TX, TY), coarseUpSmem, s>>>(view(l - 1), view(l), view(nGrid), cd,
-                                                                                  cf.p, stop, fused ? test : none);
+                                                                                  cf.p, stop, (fused && !pairTest) ? test : none);
       l -= 2;
     } else {
-      k_mg_coarse<T><<<ctx->zg(1), kCoarseThreads, coarseSmem, s>>>(view(nGrid), cd, cf.p, stop, fused ? test : none);
+      k_mg_coarse<T><<<ctx->zg(1), kCoarseThreads, coarseSmem, s>>>(view(nGrid), cd, cf.p, stop,
+                                                                   (fused && !pairTest) ? test : none);
     }
     for (; l >= 0; --l) {
       if (l >= 2 && pairOk) {  // two mid levels in one launch

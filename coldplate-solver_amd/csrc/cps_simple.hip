@@ -354,6 +354,10 @@ __global__ __launch_bounds__(NT) void k_jacobi(Geo g, MomSys<C> su, MomSys<C> sv
   su.zshift(zo_); sv.zshift(zo_); st = zsh(st, zo_); part = zsh(part, zo_);
   __shared__ double sh[NT / kWave + 1];
   __shared__ int sfl[2];
+  // both components stopped in an earlier launch: nothing to load (uniform
+  // scalar test; a flag cleared by block 0 in this launch means the same
+  // decision in every block)
+  if (!st->jac_active[0] && !st->jac_active[1]) return;
   const int t = thread_id();
   // flags (thread 0) and the partials of sweep-1 are loaded together
   const int f0 = t == 0 ? st->jac_active[0] : 0, f1 = t == 0 ? st->jac_active[1] : 0;
@@ -478,6 +482,7 @@ void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> sv, int k, int two, double tol2, I
   __shared__ C xb[2][JR1N];  // x_(k+1)
   __shared__ double sh[NTH / kWave + 1];
   __shared__ int sfl[2];
+  if (!st->jac_active[0] && !st->jac_active[1]) return;  // as in k_jacobi
   const int t = thread_id();
   const int f0 = t == 0 ? st->jac_active[0] : 0, f1 = t == 0 ? st->jac_active[1] : 0;
   const bool tst = k >= 2;
