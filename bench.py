@@ -391,7 +391,29 @@ def split_early(values, ratio=2.0):
     return high, len(low)
 
 
-def profile_table(out, trace, fetch, write, copy_fetch, copy_write, title, skip=0.0):
+def fused_fine_down(trace, kernel="k_mg_down<float, 2>", marker="k_pcg_start<float"):
+    """Durations [us] of the roofline kernel's fused launches in a kernel
+    trace: dispatches of `kernel` that do not directly follow a `marker`
+    dispatch (the first V-cycle of each pressure solve runs the unfused
+    form), early exits left out (split_early).  Returns (durations, number
+    of early exits, number of unfused first launches)."""
+    rows = sorted(((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), _short_kernel(r["Kernel_Name"]))
+                   for r in _csv_rows(trace, "kernel_trace.csv")))
+    fused, first, after = [], 0, False
+    for t0, t1, k in rows:
+        if k.startswith(marker):
+            after = True
+        elif k == kernel:
+            if after:
+                first += 1
+            else:
+                fused.append((t1 - t0) / 1e3)
+            after = False
+    work, nearly = split_early(fused)
+    return work, nearly, first
+
+
+def profile_table(out, trace, fetch, write, copy_fetch, copy_write, title, skip=0.0, roofline_bytes=0.0):
     """Per kernel and grid: dispatches, mean duration (kernel trace, end -
     start; dispatches after the first `skip` fraction of the run), HBM
     bytes read / written per dispatch (mean FETCH_SIZE / WRITE_SIZE in
@@ -454,6 +476,15 @@ def profile_table(out, trace, fetch, write, copy_fetch, copy_write, title, skip=
             cells = f"| {rb / 1e6:.2f} | {wb / 1e6:.2f} | {gbs:.0f} | {gbs / 8000:.2f} |"
         lines.append(f"| {k} | {gs} | {len(v)} | {nearly} | {mean:.2f} | {sum(v) / 1e3:.3f} | {100 * sum(v) / tot:.1f} "
                      + cells)
+    if roofline_bytes > 0:
+        work, nearly, first = fused_fine_down(trace)
+        if work:
+            mean = sum(work) / len(work)
+            gbs = roofline_bytes / (mean * 1e-6) / 1e9
+            lines += ["", f"Roofline kernel (fused fine-level k_mg_down<float, 2>, FP32-inner form): {len(work)} dispatches "
+                      f"that did the work ({nearly} early exits and {first} unfused first-V-cycle launches left out), "
+                      f"mean {mean:.2f} us; {roofline_bytes / 1e6:.2f} MB algorithmic per launch -> {gbs:.0f} GB/s, "
+                      f"{gbs / 8000:.3f} of 8 TB/s."]
     text = "\n".join(lines) + "\n"
     with open(out, "w") as fh:
         fh.write(text)
@@ -485,13 +516,15 @@ def main():
     for name in ("trace", "fetch", "write", "copy-fetch", "copy-write"):
         ap.add_argument("--" + name, default=None)
     ap.add_argument("--title", default="per-kernel HBM traffic")
+    ap.add_argument("--roofline-bytes", type=float, default=0.0,
+                    help="profile table: algorithmic bytes per fused fine down launch (adds the roofline line)")
     ap.add_argument("--skip", type=float, default=0.0, help="profile table: fraction of early dispatches left out")
     ap.add_argument("--leg", choices=["grads", "grads_batch", "thermal", "batch", "time_to_dp", "parity"],
                     help="run only this leg and print its results (for profiling)")
     args = ap.parse_args()
     if args.profile_table:
         profile_table(args.profile_table, args.trace, args.fetch, args.write, args.copy_fetch, args.copy_write,
-                      args.title, args.skip)
+                      args.title, args.skip, args.roofline_bytes)
         return
     if args.copy_calibration:
         import torch
