@@ -190,17 +190,19 @@ def grads_leg(pkg, nx=512, ny=256, forward_steps=4000, reps=3, rtol=1e-8):
     return out
 
 
-def batch_leg(pkg, steps=500, n_fields=48, plan=None):
+def batch_leg(pkg, steps=500, n_fields=48, plan=None, precision=None):
     """48 density fields, sizes drawn from {256x128, 512x256, 768x384,
     1024x512}, sigma ~ U[4, 12], per-field seeds, the headline knobs
     (FP32-inner GMG-PCG, p_rtol 1e-3, <= 6 PCG iterations, <= 8 Jacobi
     sweeps): plan (class plan, one launch per kernel covering a size class,
     or per-field chains on several streams) and graph unroll picked by the
-    autotuner, `steps` iterations timed."""
+    autotuner, `steps` iterations timed.  precision: CPS_PREC_MIXED
+    (default) or CPS_PREC_FP64 (the same algorithm in plain FP64)."""
     T = pkg._types
+    precision = T.CPS_PREC_MIXED if precision is None else precision
     fl = []
     for cfg, gamma, inlet in pkg.configs.batch_sweep(n_fields=n_fields):
-        cfg.pressure_solver, cfg.precision = T.CPS_PRESSURE_GMG, T.CPS_PREC_MIXED
+        cfg.pressure_solver, cfg.precision = T.CPS_PRESSURE_GMG, precision
         cfg.p_tol, cfg.p_rtol, cfg.max_pressure_iter, cfg.max_refine = 0.0, 1e-3, 6, 1
         cfg.max_momentum_iter = 8
         fl.append((cfg, gamma, inlet))
@@ -579,6 +581,14 @@ def main():
     grads = None if args.no_grads else grads_leg(pkg)
     thermal = None if args.no_thermal else thermal_leg(pkg, cpu=not args.no_cpu)
     batch = None if args.no_batch else batch_leg(pkg, steps=args.batch_steps, plan=args.batch_plan)
+    if batch is not None:
+        # the same fields in plain FP64 under the same plan: the precision
+        # ratio where the kernels carry whole size classes
+        same = (f"class:{batch['unroll']}:{batch['plan_param']}" if batch["plan"] == "class"
+                else f"per_field:{batch['streams']}")
+        b64 = batch_leg(pkg, steps=args.batch_steps, plan=same, precision=pkg._types.CPS_PREC_FP64)
+        batch["fp64_cell_updates_per_s"] = b64["cell_updates_per_s"]
+        batch["speedup_vs_fp64"] = batch["cell_updates_per_s"] / b64["cell_updates_per_s"]
     grads_batch = None if args.no_grads else grads_batch_leg(pkg)
     tdp = None if args.no_fp64 or args.no_time_to_dp else time_to_dp_leg(pkg, nx, ny)
     cpu = None if args.no_cpu else cpu_baseline(pkg, nx, ny, args.cpu_seconds)
@@ -636,7 +646,7 @@ def main():
         "batch": None if batch is None else {
             "workload": "48 density fields, sizes from {256x128, 512x256, 768x384, 1024x512}, sigma U[4,12], "
                         "headline knobs; autotuned plan (class plan: one launch per kernel over all fields "
-                        "of a grid size, one hipGraph per size class)",
+                        "of a grid size, one hipGraph per size class); fp64_*: the same fields and plan in plain FP64",
             **{k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in batch.items()},
         },
         "thermal": None if thermal is None else {
