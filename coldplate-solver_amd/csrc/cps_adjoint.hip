@@ -858,14 +858,15 @@ class FlowAdjoint {
   DevBuf<int> col, colT, cnt, cur, flag, iout, bad;
   DevBuf<double> val, valT, res, dinv, gvec, lam, r, Vb, Zb, w, t1, t2, t3, t4, ta, lt, y, rp, xp, dEa, dNa, part,
       sums, grad, obj, bandx, bandy, dj;
-  // preconditioner: CPS_ADJ_PRECOND_SIMPLE (default), _LSC_LINE or _LSC_ADI
+  // preconditioner: CPS_ADJ_PRECOND_LSC_ADI (default), _LSC_LINE or _SIMPLE
   int kind = CPS_ADJ_PRECOND_LSC_ADI;
   int sweeps = 2;       // line sweeps per momentum-block solve after the first
   double omega = 0.8;   // damping of the sweeps
   // multigrid-PCG iterations per Schur-complement solve (rtol 1e-3): FGMRES
-  // is flexible and its iteration count is the same from 4 to 30 on 512 x
-  // 256 fields, so the inner solves stay short
-  int schur_iters = 4;
+  // is flexible and, with the multigrid's outlet-scaled coarse levels, its
+  // iteration count on 512 x 256 fields is the same from 2 to 30 (34 at 1),
+  // so the inner solves stay short
+  int schur_iters = 2;
   DevBuf<IterState> st;
   std::unique_ptr<PressureMG> mg;
   KernelTimer timer;

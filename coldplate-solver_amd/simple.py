@@ -153,7 +153,7 @@ class SIMPLE:
         self.last_adjoint = st.as_dict()
         return grad
 
-    def adjoint_options(self, preconditioner="lsc_adi", line_sweeps=2, omega=0.8, schur_iters=4):
+    def adjoint_options(self, preconditioner="lsc_adi", line_sweeps=2, omega=0.8, schur_iters=2):
         """Preconditioner of the adjoint solve: "lsc_adi" (default: x- and
         y-line momentum solves with damped alternating sweeps, least-squares-
         commutator Schur complement), "lsc_line" (x-lines only) or "simple"

@@ -265,7 +265,7 @@ int cps_simple_jacobian_apply(cps_simple* s, int transpose, const double* x, dou
  *                             complement by the least-squares commutator;
  *   CPS_ADJ_PRECOND_LSC_ADI   as LSC_LINE with each sweep an x-line pass then
  *                             a y-line pass (default: line_sweeps 2,
- *                             omega 0.8, schur_iters 4).
+ *                             omega 0.8, schur_iters 2).
  * line_sweeps in [0, 64], omega (sweep damping) in (0, 1]; schur_iters
  * (multigrid-PCG iterations per Schur-complement solve, rtol 1e-3) in
  * [1, 1000]. */
