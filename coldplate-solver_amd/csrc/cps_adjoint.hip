@@ -382,11 +382,11 @@ __global__ __launch_bounds__(BS) void k_spmv_rows(int n, const int* col, const d
   y[r] = a;
 }
 
-// y = J^T x
-__global__ __launch_bounds__(BS) void k_spmv(int n, int WT, const int* cnt, const int* colT, const double* valT,
-                                            const double* x, double* y) {
+// y = J^T x on rows [0, rows) (n: the slot stride, all rows)
+__global__ __launch_bounds__(BS) void k_spmv(int n, int rows, int WT, const int* cnt, const int* colT,
+                                            const double* valT, const double* x, double* y) {
   const int c = blockIdx.x * BS + threadIdx.x;
-  if (c >= n) return;
+  if (c >= rows) return;
   const int m = min(cnt[c], WT);
   double a = 0.0;
   for (int s = 0; s < m; ++s) a += valT[(size_t)s * n + c] * x[colT[(size_t)s * n + c]];
@@ -412,12 +412,12 @@ __global__ __launch_bounds__(BS) void k_mdot(int n, const double* Vb, int nv, co
   }
 }
 
-__global__ __launch_bounds__(BS) void k_psum(const double* part, int nb, int nslots, double* out) {
+// out[s] = sum of slot s's nb partials in fixed order, one block per slot
+__global__ __launch_bounds__(BS) void k_psum(const double* part, int nb, double* out) {
   __shared__ double sh[BS / kWave + 1];
-  for (int s = 0; s < nslots; ++s) {
-    const double a = parts_sum<BS>(part + (size_t)s * nb, nb, sh);
-    if (threadIdx.x == 0) out[s] = a;
-  }
+  const int s = blockIdx.x;
+  const double a = parts_sum<BS>(part + (size_t)s * nb, nb, sh);
+  if (threadIdx.x == 0) out[s] = a;
 }
 
 // w = a0 * w + sum_j c_j V_j
@@ -652,13 +652,16 @@ __device__ inline Aff aff_shfl(const Aff& a, int off) {
 // shuffle scan composes the maps of the lanes before it (after it for the
 // backward substitution), and each lane then replays its segment from the
 // state entering it.  The forward sweep goes to tmp; every element is written
-// and read back by the same lane.  4 lines per 256-thread block.
+// and read back by the same lane.  4 lines per 256-thread block; y-line
+// elements lie a row pitch apart, so the y-line blocks run in XCD-aware
+// order (xcd_band): the four blocks of 16 adjacent columns, which share
+// every row's cache line, run on one XCD and meet in its L2.
 constexpr int kSeg = 8;
 template <bool Y>
 __global__ __launch_bounds__(256) void k_line_solve(View V, const double* band, const double* f, const double* Ax,
                                                    double omega, double* x, double* tmp) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int l = blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+  const int l = (Y ? xcd_band() : (int)blockIdx.x) * (256 / kWave) + threadIdx.x / kWave;
   if (l >= n_lines<Y>(V.g)) return;  // uniform per wavefront
   size_t base;
   int len, step;
@@ -954,7 +957,13 @@ class FlowAdjoint {
   }
 
   void spmv(const double* x, double* out) {
-    adj::k_spmv<<<blocks(n), adj::BS, 0, ctx->stream>>>(n, WT, cnt.p, colT.p, valT.p, x, out);
+    adj::k_spmv<<<blocks(n), adj::BS, 0, ctx->stream>>>(n, n, WT, cnt.p, colT.p, valT.p, x, out);
+  }
+  // the momentum rows (u and v blocks, [0, 2 S)) of J^T x only: what a line
+  // pass reads (its x is zero on the pressure block, its lines lie in u, v)
+  void spmv_mom(const double* x, double* out) {
+    const int rows = (int)(2 * S);
+    adj::k_spmv<<<blocks(rows), adj::BS, 0, ctx->stream>>>(n, rows, WT, cnt.p, colT.p, valT.p, x, out);
   }
 
   int nlines(bool y) const { return y ? (g.N - 2) + (g.N - 1) : (g.M - 1) + (g.M - 2); }
@@ -964,7 +973,7 @@ class FlowAdjoint {
     hipStream_t s = ctx->stream;
     const double* Ax = nullptr;
     if (!first) {
-      spmv(x, ta.p);
+      spmv_mom(x, ta.p);
       Ax = ta.p;
     }
     const int lpb = 256 / kWave;  // lines per block (one wavefront each)
@@ -990,7 +999,10 @@ class FlowAdjoint {
   // on) is set up by the first solve after each build only
   bool schur_ready = false;
   void schur_solve(const cps_simple_config& pcfg) {
-    mg->solve(dEa.p, dNa.p, rp.p, xp.p, st.p, pcfg, true, &timer, schur_ready);
+    // FP32 multigrid inside one FP64 refinement sweep: the Schur solve is
+    // a preconditioner of flexible GMRES, so its precision sets only the
+    // iteration count (unchanged at 512 x 256), not the gradient's accuracy
+    mg->solve(dEa.p, dNa.p, rp.p, xp.p, st.p, pcfg, false, &timer, schur_ready);
     schur_ready = true;
   }
 
@@ -1041,7 +1053,7 @@ class FlowAdjoint {
   void mdot_dev(const double* Vbase, int nv, const double* x, double* out) {
     hipStream_t s = ctx->stream;
     adj::k_mdot<<<nb, adj::BS, 0, s>>>(n, Vbase, nv, x, part.p);
-    adj::k_psum<<<1, adj::BS, 0, s>>>(part.p, nb, nv, out);
+    adj::k_psum<<<nv, adj::BS, 0, s>>>(part.p, nb, out);
   }
   void mdot(const double* Vbase, int nv, const double* x, double* h) {
     hipStream_t s = ctx->stream;

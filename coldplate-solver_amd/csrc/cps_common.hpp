@@ -21,6 +21,15 @@ __device__ inline int xcd_tile() {
   return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
 }
 
+// The same XCD-aware order for any block count of a 1-D grid: XCD x (which
+// receives blocks x, x + 8, ...) takes the contiguous band that starts after
+// the bands of XCDs 0 .. x-1 (a bijection of the block indices).
+__device__ inline int xcd_band() {
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
 // Streaming store for results the next kernel reads (possibly on another
 // XCD): marked non-temporal, so the line is not kept dirty in this XCD's L2
 // for the end-of-kernel write-back.
