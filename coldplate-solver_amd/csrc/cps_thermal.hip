@@ -510,6 +510,106 @@ __global__ __launch_bounds__(TNT) void k_th_coarse(ThCoarse<T> H) {
   }
 }
 
+// k_th_coarse with each z-line sweep in two phases over LDS, for levels
+// whose cells fit (5 reals per cell of the top level): first every (column,
+// layer) cell at once, one thread each, forms its right side r_k = rhs - (w
+// x_w + e x_e + s x_s + n x_n) and stages it with x0, the lower coupling and
+// the stored Thomas factors (all its global loads in flight together); then
+// one thread per column runs the z recurrence from LDS and stores x0 +
+// omega (x - x0).  The same expressions as zjac_col in the same order, so
+// the results are bit-identical; the sweep waits for one round of loads
+// instead of one per layer.
+constexpr int kThCoarseThreads = 1024;
+template <class T, int NZ>
+__device__ inline void zjac_level(const Lvl<T>& L, const T* zc, const T* zdi, const T* src, T* dst, const T* cy,
+                                  const Op<T>& C, int mode, T* sm, int t) {
+  const Op<T>& A = L.A;
+  const int nz = A.nz, ncol = A.nx * A.ny, ncell = ncol * nz;
+  T* sR = sm;
+  T* sX0 = sR + ncell;
+  T* sLo = sX0 + ncell;
+  T* sZd = sLo + ncell;
+  T* sZc = sZd + ncell;
+  for (int c = t; c < ncell; c += kThCoarseThreads) {
+    const int k = c / ncol, col = c - k * ncol, i = col % A.nx, j = col / A.nx;
+    const bool hw = i > 0, he = i + 1 < A.nx, hs = j > 0, hn = j + 1 < A.ny;
+    const size_t q = A.at(i, j, k);
+    T xc = (T)0, xw = (T)0, xe = (T)0, xs_ = (T)0, xn = (T)0;
+    if (mode != 0) {
+      xc = src[q];
+      xw = hw ? src[q - 1] : (T)0;
+      xe = he ? src[q + 1] : (T)0;
+      xs_ = hs ? src[q - A.pitch] : (T)0;
+      xn = hn ? src[q + A.pitch] : (T)0;
+      if (mode == 2) {
+        const int I2 = i >> 1, J2 = j >> 1;
+        xc += cy[C.at(I2, J2, k)];
+        if (hw) xw += cy[C.at((i - 1) >> 1, J2, k)];
+        if (he) xe += cy[C.at((i + 1) >> 1, J2, k)];
+        if (hs) xs_ += cy[C.at(I2, (j - 1) >> 1, k)];
+        if (hn) xn += cy[C.at(I2, (j + 1) >> 1, k)];
+      }
+    }
+    sR[c] = L.rhs[q] - (A.w[q] * xw + A.e[q] * xe + A.s[q] * xs_ + A.n[q] * xn);
+    sX0[c] = xc;
+    sLo[c] = k > 0 ? A.b[q] : (T)0;
+    sZd[c] = zdi[q];
+    sZc[c] = zc[q];
+  }
+  __syncthreads();
+  for (int col = t; col < ncol; col += kThCoarseThreads) {
+    const int i = col % A.nx, j = col / A.nx;
+    T g[NZ];
+    T gp = (T)0;
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      if (k < nz) {
+        const int c = k * ncol + col;
+        gp = (sR[c] - sLo[c] * gp) * sZd[c];
+      }
+      g[k] = gp;
+    }
+    T xk = (T)0;
+#pragma unroll
+    for (int k = NZ - 1; k >= 0; --k) {
+      if (k < nz) {
+        const int c = k * ncol + col;
+        xk = k == nz - 1 ? g[k] : g[k] - sZc[c] * xk;
+        dst[A.at(i, j, k)] = sX0[c] + (T)kOmZ * (xk - sX0[c]);
+      }
+    }
+  }
+  __syncthreads();
+}
+template <class T, int NZ>
+__global__ __launch_bounds__(kThCoarseThreads) void k_th_coarse_lds(ThCoarse<T> H) {
+  if (H.L[0].stopped()) return;  // uniform
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  const int t = threadIdx.x;
+  const Op<T> none{};
+  const int last = H.n - 1;
+  for (int l = 0; l < last; ++l) {
+    const Lvl<T>& L = H.L[l];
+    zjac_level<T, NZ>(L, H.zc[l], H.zdi[l], nullptr, L.x, nullptr, none, 0, sm, t);
+    zjac_level<T, NZ>(L, H.zc[l], H.zdi[l], L.x, L.y, nullptr, none, 1, sm, t);
+    const Op<T>& C = H.L[l + 1].A;
+    const int ncc = C.nx * C.ny;
+    for (int q = t; q < ncc * C.nz; q += kThCoarseThreads) {
+      const int k = q / ncc, rest = q - k * ncc;
+      restrict_cell(L, L.y, C, H.rhs[l + 1], rest % C.nx, rest / C.nx, k);
+    }
+    __syncthreads();
+  }
+  if (t == 0) coarsest_col(H.L[last]);
+  __syncthreads();
+  for (int l = last - 1; l >= 0; --l) {
+    const Lvl<T>& L = H.L[l];
+    zjac_level<T, NZ>(L, H.zc[l], H.zdi[l], L.y, L.x, H.L[l + 1].y, H.L[l + 1].A, 2, sm, t);
+    zjac_level<T, NZ>(L, H.zc[l], H.zdi[l], L.x, L.y, nullptr, none, 1, sm, t);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Krylov vector kernels.  Reductions leave fixed-order block partials that
 // k_sum adds up in one block.
@@ -1087,9 +1187,16 @@ struct ThermalKrylov {
   DevBuf<th::BiState> bs;
   const int* stop_flag = nullptr;  // skip flag of the level kernels during a solve
   int nb0 = 0;
+  size_t coarseLdsMax = 0;  // dynamic LDS the coarse kernel's staged form may use
 
   // borrow_level0: the finest operator's arrays belong to the caller (set_level0)
   ThermalKrylov(hipStream_t stream, int nx, int ny, int nz, bool borrow_level0) : st(stream) {
+    {
+      int dev = 0, lds = 0;
+      CPS_HIP(hipGetDevice(&dev));
+      CPS_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+      coarseLdsMax = lds > 4096 ? (size_t)lds - 4096 : 0;  // a margin for the kernel's static LDS
+    }
     int cx = nx, cy = ny;
     for (int l = 0;; ++l) {
       auto L = std::make_unique<Level>();
@@ -1168,6 +1275,20 @@ struct ThermalKrylov {
   }
   void coarse(const th::ThCoarse<T>& H) {
     const int nz = lv[0]->nz;
+    // the LDS form when the top level's five staged reals per cell fit
+    const size_t lds = (size_t)5 * H.L[0].A.nx * H.L[0].A.ny * nz * sizeof(T);
+    if (lds <= coarseLdsMax) {
+#define CPS_THCL(NZ) th::k_th_coarse_lds<T, NZ><<<1, th::kThCoarseThreads, lds, st>>>(H)
+      if (nz <= 4) CPS_THCL(4);
+      else if (nz <= 8) CPS_THCL(8);
+      else if (nz <= 12) CPS_THCL(12);
+      else if (nz <= 16) CPS_THCL(16);
+      else if (nz <= 24) CPS_THCL(24);
+      else if (nz <= 32) CPS_THCL(32);
+      else CPS_THCL(64);
+#undef CPS_THCL
+      return;
+    }
 #define CPS_THC(NZ) th::k_th_coarse<T, NZ><<<1, th::TNT, 0, st>>>(H)
     if (nz <= 4) CPS_THC(4);
     else if (nz <= 8) CPS_THC(8);
