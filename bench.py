@@ -148,7 +148,7 @@ def oracle_parity(pkg, nx, ny, iters):
     return {"iterations": iters, "rel_l2": {k: float(f"{v:.3g}") for k, v in out.items()}}
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_mg_down_1024x512_mixed_traffic.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "kernels_simple_1024x512.md")
 
 
 def measured_traffic():
@@ -393,29 +393,44 @@ def split_early(values, ratio=2.0):
     return high, len(low)
 
 
-def fused_fine_down(trace, kernel="k_mg_down<float, 2>", marker="k_pcg_start<float"):
-    """Durations [us] of the roofline kernel's fused launches in a kernel
-    trace: dispatches of `kernel` that do not directly follow a `marker`
-    dispatch (the first V-cycle of each pressure solve runs the unfused
-    form), early exits left out (split_early).  Returns (durations, number
-    of early exits, number of unfused first launches)."""
-    rows = sorted(((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), _short_kernel(r["Kernel_Name"]))
-                   for r in _csv_rows(trace, "kernel_trace.csv")))
+def fused_fine_down_values(rows, kernel="k_mg_down<float, 2>", marker="k_pcg_start<float"):
+    """rows: (order key, short kernel name, value) of every dispatch.  The
+    values of the roofline kernel's fused launches: dispatches of `kernel`
+    that do not directly follow a `marker` dispatch (the first V-cycle of
+    each pressure solve runs the unfused form), early exits left out
+    (split_early).  Returns (values, number of early exits, number of
+    unfused first launches)."""
     fused, first, after = [], 0, False
-    for t0, t1, k in rows:
+    for _, k, v in sorted(rows):
         if k.startswith(marker):
             after = True
         elif k == kernel:
             if after:
                 first += 1
             else:
-                fused.append((t1 - t0) / 1e3)
+                fused.append(v)
             after = False
     work, nearly = split_early(fused)
     return work, nearly, first
 
 
-def profile_table(out, trace, fetch, write, copy_fetch, copy_write, title, skip=0.0, roofline_bytes=0.0):
+def fused_fine_down(trace):
+    """Durations [us] of the roofline kernel's fused launches in a kernel trace."""
+    return fused_fine_down_values([(int(r["Start_Timestamp"]), _short_kernel(r["Kernel_Name"]),
+                                    (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+                                   for r in _csv_rows(trace, "kernel_trace.csv")])
+
+
+def fused_fine_down_counter(d, name):
+    """Mean counter value [KiB] over the roofline kernel's fused launches of a --pmc run."""
+    work, _, _ = fused_fine_down_values([(int(r["Dispatch_Id"]), _short_kernel(r["Kernel_Name"]),
+                                          float(r["Counter_Value"]))
+                                         for r in _csv_rows(d, "counter_collection.csv") if r["Counter_Name"] == name])
+    return sum(work) / len(work) if work else None
+
+
+def profile_table(out, trace, fetch, write, copy_fetch, copy_write, title, skip=0.0, roofline_bytes=0.0,
+                  from_kernel=None):
     """Per kernel and grid: dispatches, mean duration (kernel trace, end -
     start; dispatches after the first `skip` fraction of the run), HBM
     bytes read / written per dispatch (mean FETCH_SIZE / WRITE_SIZE in
@@ -432,6 +447,9 @@ def profile_table(out, trace, fetch, write, copy_fetch, copy_write, title, skip=
              f'{r["Grid_Size_X"]}x{r["Grid_Size_Y"]}x{r["Grid_Size_Z"]}') for r in _csv_rows(trace, "kernel_trace.csv")]
     rows.sort()
     rows = rows[int(len(rows) * skip):]
+    if from_kernel:  # the window from the first dispatch of this kernel on
+        first = next((q for q, r in enumerate(rows) if r[2].startswith(from_kernel)), 0)
+        rows = rows[first:]
     dur = {}
     for t0, t1, k, g, gs in rows:
         dur.setdefault((k, g, gs), []).append((t1 - t0) / 1e3)
@@ -487,6 +505,13 @@ def profile_table(out, trace, fetch, write, copy_fetch, copy_write, title, skip=
                       f"that did the work ({nearly} early exits and {first} unfused first-V-cycle launches left out), "
                       f"mean {mean:.2f} us; {roofline_bytes / 1e6:.2f} MB algorithmic per launch -> {gbs:.0f} GB/s, "
                       f"{gbs / 8000:.3f} of 8 TB/s."]
+        fr, wr = fused_fine_down_counter(fetch, "FETCH_SIZE"), fused_fine_down_counter(write, "WRITE_SIZE")
+        if fr is not None and wr is not None:
+            rb, wb = fr * 1024 * fs, wr * 1024 * ws
+            lines += ["", f"Its measured HBM traffic (the same fused launches of the counter runs): {rb / 1e6:.2f} MB read + "
+                      f"{wb / 1e6:.2f} MB written = {(rb + wb) / 1e6:.2f} MB per launch, against "
+                      f"{roofline_bytes / 1e6:.2f} MB algorithmic.", "",
+                      f"traffic_bytes_per_launch: {rb + wb:.0f}"]
     text = "\n".join(lines) + "\n"
     with open(out, "w") as fh:
         fh.write(text)
@@ -518,6 +543,8 @@ def main():
     for name in ("trace", "fetch", "write", "copy-fetch", "copy-write"):
         ap.add_argument("--" + name, default=None)
     ap.add_argument("--title", default="per-kernel HBM traffic")
+    ap.add_argument("--from-kernel", default=None,
+                    help="profile table: durations from the first dispatch of this kernel on (counters: all)")
     ap.add_argument("--roofline-bytes", type=float, default=0.0,
                     help="profile table: algorithmic bytes per fused fine down launch (adds the roofline line)")
     ap.add_argument("--skip", type=float, default=0.0, help="profile table: fraction of early dispatches left out")
@@ -526,7 +553,7 @@ def main():
     args = ap.parse_args()
     if args.profile_table:
         profile_table(args.profile_table, args.trace, args.fetch, args.write, args.copy_fetch, args.copy_write,
-                      args.title, args.skip, args.roofline_bytes)
+                      args.title, args.skip, args.roofline_bytes, args.from_kernel)
         return
     if args.copy_calibration:
         import torch
