@@ -496,8 +496,33 @@ void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> sv, int k, int two, double tol2, I
   // block runs the same test on the same partials)
   if (t == 0) { sfl[0] = f0; sfl[1] = f1; }
   __syncthreads();
-  const int au0 = sfl[0], av0 = sfl[1];
+  int au0 = sfl[0], av0 = sfl[1];
   if (!au0 && !av0) return;  // block-uniform
+  // jacobi_solver.cpp:63 on sweep k-2, and on sweep k-1 if that sweep counts,
+  // before any tile load: once the solve has converged (in most SIMPLE steps
+  // after the first pair) the launch loads nothing
+  if (tst) {
+    const bool b0 = block_id() == 0 && t == 0;
+    if (au0) {
+      if (parts_total<NTH>(qu2, sh) < tol2 * (double)g.nU) {
+        au0 = 0;
+        if (b0) { st->jac_active[0] = 0; st->jac_iters[0] = k - 1; }
+      } else if (parts_total<NTH>(qu1, sh) < tol2 * (double)g.nU) {
+        au0 = 0;
+        if (b0) { st->jac_active[0] = 0; st->jac_iters[0] = k; }
+      }
+    }
+    if (av0) {
+      if (parts_total<NTH>(qv2, sh) < tol2 * (double)g.nV) {
+        av0 = 0;
+        if (b0) { st->jac_active[1] = 0; st->jac_iters[1] = k - 1; }
+      } else if (parts_total<NTH>(qv1, sh) < tol2 * (double)g.nV) {
+        av0 = 0;
+        if (b0) { st->jac_active[1] = 0; st->jac_iters[1] = k; }
+      }
+    }
+    if (!au0 && !av0) return;  // uniform over the grid
+  }
   const int M = g.M, N = g.N, P = g.P;
   const int lt = xcd_tile();
   const int i0 = 1 + lt / (int)gridDim.x * BY, j0 = 1 + lt % (int)gridDim.x * BX;
@@ -519,8 +544,7 @@ void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> sv, int k, int two, double tol2, I
   }
   const bool ring = t < JRING;
   const int tj = j0 - 1 + tc, ri = i0 - 1 + rr, rj = j0 - 1 + rc;
-  // every global load first, for the components active at entry; the stop
-  // tests below may retire a component (its loaded values then go unused)
+  // every global load first, for the components still active after the tests
   int ti[RT];
   bool tu[RT], tv[RT];
 #pragma unroll
@@ -560,28 +584,7 @@ void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> sv, int k, int two, double tol2, I
     coefs(su, gu, ri, rj, du);
     coefs(sv, gv, ri, rj, dv);
   }
-  int au = au0, av = av0;
-  if (tst) {  // jacobi_solver.cpp:63 on sweep k-2, and on sweep k-1 if that sweep counts
-    const bool b0 = block_id() == 0 && t == 0;
-    if (au) {
-      if (parts_total<NTH>(qu2, sh) < tol2 * (double)g.nU) {
-        au = 0;
-        if (b0) { st->jac_active[0] = 0; st->jac_iters[0] = k - 1; }
-      } else if (parts_total<NTH>(qu1, sh) < tol2 * (double)g.nU) {
-        au = 0;
-        if (b0) { st->jac_active[0] = 0; st->jac_iters[0] = k; }
-      }
-    }
-    if (av) {
-      if (parts_total<NTH>(qv2, sh) < tol2 * (double)g.nV) {
-        av = 0;
-        if (b0) { st->jac_active[1] = 0; st->jac_iters[1] = k - 1; }
-      } else if (parts_total<NTH>(qv1, sh) < tol2 * (double)g.nV) {
-        av = 0;
-        if (b0) { st->jac_active[1] = 0; st->jac_iters[1] = k; }
-      }
-    }
-  }
+  const int au = au0, av = av0;
   if (!au && !av) return;  // uniform over the grid
 #pragma unroll
   for (int o = 0; o < RT; ++o) {
