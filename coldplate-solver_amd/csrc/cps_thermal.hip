@@ -415,6 +415,75 @@ __global__ __launch_bounds__(TNT) void k_zjac(Lvl<T> L, const T* zc, const T* zd
   zjac_col<T, NZ>(L, zc, zdi, src, dst, cy, C, mode, i, j);
 }
 
+// k_zjac in two phases over LDS for a strip of 64 columns of one row: every
+// (column, layer) cell of the strip at once forms its right side r_k = rhs -
+// (w x_w + e x_e + s x_s + n x_n) and stages it with x0, the lower coupling and
+// the stored Thomas factors, all its global loads in flight together; then
+// one thread per column runs the z recurrence from LDS.  The same
+// expressions as zjac_col in the same order (bit-identical); a thread waits
+// for one round of loads instead of one per group of layers.
+constexpr int kStripW = 64;
+template <class T, int NZ>
+__global__ __launch_bounds__(TNT) void k_zjac_strip(Lvl<T> L, const T* zc, const T* zdi, const T* src, T* dst,
+                                                   const T* cy, Op<T> C, int mode) {
+  if (L.stopped()) return;  // uniform
+  __shared__ T sR[kStripW * NZ], sX0[kStripW * NZ], sLo[kStripW * NZ], sZd[kStripW * NZ], sZc[kStripW * NZ];
+  const Op<T>& A = L.A;
+  const int nz = A.nz, t = flat_tid();
+  const int fb = flat_bid(), i0 = fb % (int)gridDim.x * kStripW, j = fb / (int)gridDim.x;
+  const bool hs = j > 0, hn = j + 1 < A.ny;
+  const int J2 = j >> 1;
+  for (int c = t; c < kStripW * nz; c += TNT) {
+    const int k = c / kStripW, i = i0 + (c - k * kStripW);
+    if (i >= A.nx) continue;
+    const bool hw = i > 0, he = i + 1 < A.nx;
+    const size_t q = A.at(i, j, k);
+    T xc = (T)0, xw = (T)0, xe = (T)0, xs_ = (T)0, xn = (T)0;
+    if (mode != 0) {
+      xc = src[q];
+      xw = hw ? src[q - 1] : (T)0;
+      xe = he ? src[q + 1] : (T)0;
+      xs_ = hs ? src[q - A.pitch] : (T)0;
+      xn = hn ? src[q + A.pitch] : (T)0;
+      if (mode == 2) {
+        const int I2 = i >> 1;
+        xc += cy[C.at(I2, J2, k)];
+        if (hw) xw += cy[C.at((i - 1) >> 1, J2, k)];
+        if (he) xe += cy[C.at((i + 1) >> 1, J2, k)];
+        if (hs) xs_ += cy[C.at(I2, (j - 1) >> 1, k)];
+        if (hn) xn += cy[C.at(I2, (j + 1) >> 1, k)];
+      }
+    }
+    sR[c] = L.rhs[q] - (A.w[q] * xw + A.e[q] * xe + A.s[q] * xs_ + A.n[q] * xn);
+    sX0[c] = xc;
+    sLo[c] = k > 0 ? A.b[q] : (T)0;
+    sZd[c] = zdi[q];
+    sZc[c] = zc[q];
+  }
+  __syncthreads();
+  const int i = i0 + t;
+  if (t >= kStripW || i >= A.nx) return;
+  T g[NZ];
+  T gp = (T)0;
+#pragma unroll
+  for (int k = 0; k < NZ; ++k) {
+    if (k < nz) {
+      const int c = k * kStripW + t;
+      gp = (sR[c] - sLo[c] * gp) * sZd[c];
+    }
+    g[k] = gp;
+  }
+  T xk = (T)0;
+#pragma unroll
+  for (int k = NZ - 1; k >= 0; --k) {
+    if (k < nz) {
+      const int c = k * kStripW + t;
+      xk = k == nz - 1 ? g[k] : g[k] - sZc[c] * xk;
+      dst[A.at(i, j, k)] = sX0[c] + (T)kOmZ * (xk - sX0[c]);
+    }
+  }
+}
+
 // coarse right side = sum of the fine residuals over the aggregate
 template <class T>
 __device__ inline void restrict_cell(const Lvl<T>& L, const T* x, const Op<T>& C, T* crhs, int I, int J, int k) {
@@ -1167,6 +1236,12 @@ __global__ __launch_bounds__(TNT) void k_surr_sens(int nx, int ny, double dx2inv
 // BiCGStab with vectors in precision TK, preconditioned by a multigrid
 // hierarchy in precision T (TK = double, T = float: FP64 Krylov, FP32
 // V-cycles; the casts sit at the preconditioner's boundary)
+#ifndef CPS_ZJAC_STRIP_FROM
+#define CPS_ZJAC_STRIP_FROM 0
+#endif
+// the first level whose z-line sweeps run in the strip form (all of them:
+// measured faster than one thread per column on every level, the finest too)
+constexpr int kZjacStripFrom = CPS_ZJAC_STRIP_FROM;
 template <class T, class TK = T>
 struct ThermalKrylov {
   static constexpr bool kSplit = !std::is_same<T, TK>::value;
@@ -1263,6 +1338,16 @@ struct ThermalKrylov {
     const dim3 G = col_grid(l), B(th::TBX, th::TBY);
     const int nz = lv[l]->nz;
     const T *zc = lv[l]->zc.p, *zdi = lv[l]->zdi.p;
+    if (l >= kZjacStripFrom && nz <= 16) {  // the two-phase strip form
+      const dim3 GS(ceil_div(lv[l]->nx, th::kStripW), lv[l]->ny);
+#define CPS_ZJS(NZ) th::k_zjac_strip<T, NZ><<<GS, th::TNT, 0, st>>>(L, zc, zdi, src, dst, cy, C, mode)
+      if (nz <= 4) CPS_ZJS(4);
+      else if (nz <= 8) CPS_ZJS(8);
+      else if (nz <= 12) CPS_ZJS(12);
+      else CPS_ZJS(16);
+#undef CPS_ZJS
+      return;
+    }
 #define CPS_ZJAC(NZ) th::k_zjac<T, NZ><<<G, B, 0, st>>>(L, zc, zdi, src, dst, cy, C, mode)
     if (nz <= 4) CPS_ZJAC(4);
     else if (nz <= 8) CPS_ZJAC(8);
