@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 #include <cstdio>
 #include <string>
 
@@ -95,9 +96,13 @@ int guard(F&& fn) {
 // pointers and zs, and block (x, y, z) shifts them to field z at entry
 // (nullptr, a "not used" marker, stays nullptr).  gridDim.z = 1 shifts by 0.
 __device__ __forceinline__ size_t zoff(size_t zs) { return (size_t)blockIdx.z * zs; }
+// The shift is a byte offset on the pointer itself (no round trip through an
+// integer), so the compiler keeps the global address space it inferred for
+// the kernel argument and emits global, not flat, memory instructions.
 template <class T>
 __device__ __forceinline__ T* zsh(T* p, size_t off) {
-  return p ? reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) + off) : p;
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return p ? reinterpret_cast<T*>(reinterpret_cast<B*>(p) + off) : p;
 }
 
 // Bump allocator over one region: while a thread has an arena installed
@@ -299,23 +304,29 @@ __device__ inline double parts_max(const double* p, int n, double* sh) {
 // loads of this thread's partials (n <= 2 * NT) early; parts_total (after
 // any other loads have been issued) reduces them.  Same fixed order as
 // parts_sum: thread t holds p[t] + p[t + NT].
-template <int NT>
-__device__ inline double parts_load(const double* p, int n) {
-  const int t = threadIdx.x + threadIdx.y * blockDim.x;
-  double a = t < n ? p[t] : 0.0;
-  if (t + NT < n) a += p[t + NT];
-  return a;
-}
-// parts_load for n <= K NT partials: thread t holds p[t] + p[t + NT] + ...
-// + p[t + (K-1) NT] in that order (K = 2: parts_load).
+// parts_load for n <= K NT partials (n >= 1): thread t holds p[t] +
+// p[t + NT] + ... + p[t + (K-1) NT] in that order (K = 2: parts_load).
+// Every load is issued unconditionally at an index clamped into [0, n) and
+// the out-of-range ones are dropped by a select, so the K loads are in
+// flight together (a load under a branch is waited for inside the branch).
 template <int NT, int K>
 __device__ inline double parts_load_k(const double* p, int n) {
   const int t = threadIdx.x + threadIdx.y * blockDim.x;
-  double a = t < n ? p[t] : 0.0;
+  if (n <= NT) {  // block-uniform: one partial per thread at most
+    const double v0 = p[min(t, n - 1)];
+    return t < n ? v0 : 0.0;
+  }
+  double v[K];
 #pragma unroll
-  for (int k = 1; k < K; ++k)
-    if (t + k * NT < n) a += p[t + k * NT];
+  for (int k = 0; k < K; ++k) v[k] = p[min(t + k * NT, n - 1)];
+  double a = t < n ? v[0] : 0.0;
+#pragma unroll
+  for (int k = 1; k < K; ++k) a = t + k * NT < n ? a + v[k] : a;
   return a;
+}
+template <int NT>
+__device__ inline double parts_load(const double* p, int n) {
+  return parts_load_k<NT, 2>(p, n);
 }
 template <int NT>
 __device__ inline double parts_total(double a, double* sh) {
