@@ -28,8 +28,11 @@
 #include <vector>
 
 #include "cps_mg.hpp"
+#include "cps_mg_coarse.hpp"
 
 namespace cps {
+
+using namespace mgc;
 
 namespace {
 
@@ -40,65 +43,23 @@ constexpr int MBX = 64, MBY = 4, MNT = MBX * MBY;
 #ifndef CPS_MG_OMEGA2
 #define CPS_MG_OMEGA2 1.1
 #endif
-#ifndef CPS_MG_COARSE_SCALE
-#define CPS_MG_COARSE_SCALE 0.5
-#endif
-#ifndef CPS_MG_DIRICHLET_SCALE
-#define CPS_MG_DIRICHLET_SCALE 1.25
-#endif
 // weights of the two Jacobi sweeps of each smoothing pass (pre: the sweep
 // from zero, then the second; post: the sweep after the prolongation, then
 // the second); the sweeps are polynomials in D^-1 A, so the V-cycle stays
-// symmetric for any pair.  Coarse faces are this share of the covered fine
-// faces' sum (0.5: twice the Galerkin coarse correction).
+// symmetric for any pair.  The coarse faces' share of the covered fine
+// faces and the outlet factor are kCoarseScale / kDirichletScale
+// (cps_mg_coarse.hpp).
 constexpr double kOmega1 = CPS_MG_OMEGA1;
 constexpr double kOmega2 = CPS_MG_OMEGA2;
-constexpr double kCoarseScale = CPS_MG_COARSE_SCALE;
-// The east face of a level's last column couples it to the pinned outlet
-// column p'(., N-1) = 0.  On a coarse level that column's centre lies half a
-// coarse cell or less from the pinned one, while its face sum keeps the fine
-// face's size, so kCoarseScale alone under-weights the outlet and the
-// smoothest error mode (the only one the outlet pins) is over-corrected.
-// The last coarse column's east face carries this extra factor.
-constexpr double kDirichletScale = CPS_MG_DIRICHLET_SCALE;
-// the factor of a coarse east face whose covered fine column is fcol of a
-// finer level fnx wide (the last coarse column covers the last fine one)
-__host__ __device__ constexpr double coarse_e_scale(int fcol, int fnx) {
-  return fcol >= fnx - 1 ? kCoarseScale * kDirichletScale : kCoarseScale;
-}
 #ifndef CPS_MG_UP_ROWS
 #define CPS_MG_UP_ROWS 2
 #endif
 #ifndef CPS_MG_DOWN_ROWS
 #define CPS_MG_DOWN_ROWS 2
 #endif
-#ifndef CPS_COARSE_THREADS
-#define CPS_COARSE_THREADS 1024
-#endif
-constexpr int kCoarseThreads = CPS_COARSE_THREADS;
 // staging limits of the coarse solver: all its levels together, and its top
 constexpr int kCoarseTotalMax = 3072, kCoarseTopMax = 2048;
 static_assert(kCoarseTotalMax % kCoarseThreads == 0 && kCoarseTopMax % kCoarseThreads == 0, "coarse staging");
-// the coarse hierarchy ends at its first level of at most kDirectCells cells,
-// which is solved with its inverse (one column per lane of a wavefront)
-constexpr int kDirectCells = 32;
-static_assert(kDirectCells * kDirectCells <= kCoarseThreads && kDirectCells <= 64, "direct level staging");
-constexpr double kTiny20 = (double)1e-20f;
-
-template <class T>
-struct Lv {  // device view of one level; cell (r,c) at off + r*pitch + c
-  int nx, ny, pitch, off;
-  T *x, *y, *b, *ex, *ey;
-  size_t zs;  // byte stride between the fields of a class-batched launch
-  __device__ size_t at(int r, int c) const { return (size_t)off + (size_t)r * pitch + c; }
-  __device__ void zshift(size_t o) {
-    x = zsh(x, o);
-    y = zsh(y, o);
-    b = zsh(b, o);
-    ex = zsh(ex, o);
-    ey = zsh(ey, o);
-  }
-};
 
 // Face coefficients around cell k: ex(k) is the east face (the Dirichlet
 // outlet face in the last column), ey(k) the north face (0 in the last row).
@@ -1054,138 +1015,20 @@ __global__ __launch_bounds__(FT) void k_mg_up2(Lv<T> A, Lv<T> B, Lv<T> C, const 
 }
 
 // ---------------------------------------------------------------------------
-// All levels from `top` down inside one workgroup: the top level's rhs and
-// faces come from global memory, every smaller level is built and solved in
-// LDS; the top level's correction is written to top.y.
-struct CoarseDesc {
-  int nlev;
-  int nx[16], ny[16], off[16];  // compact LDS layout, pitch = nx
-  int total;
-  int wl;  // the last level (at most kDirectCells cells): solved with its inverse
-  int dn;  // its cell count; the inverse sits at cf[4 total ..), column-major
-  unsigned long long mag[16];  // ceil(2^32 / nx): row = (q mag) >> 32 for a cell index q < 2^20
-};
-
 // the cells of level l, one per thread in row-major order (every lane of a
 // wave busy on narrow levels); r = q / nx by the reciprocal multiply
 #define CELLS(l)                                                                 \
   for (int q_ = t, n_ = d.nx[l] * d.ny[l]; q_ < n_; q_ += kCoarseThreads)       \
     if (const int r = (int)(((unsigned long long)q_ * d.mag[l]) >> 32), c = q_ - r * d.nx[l]; true)
 
-// Once per pressure solve: faces and inverse diagonals of every level of the
-// coarse solver, built from the top level's faces (half the sum of the
-// covered fine faces), stored as cf[0 .. 4 total) = east, north, 1/diag, diag.
-// With par.nx > 0 the top level's faces are not in memory: they are built
-// here from its parent grid level par, with k_mg_coarsen's arithmetic (no
-// other kernel reads the top level's faces).
+// Once per pressure solve: the coarse solver's faces, inverse diagonals and
+// last-level inverse (coarse_setup_body) as a launch of its own.
 template <class T>
 __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse_setup(Lv<T> top, Lv<T> par, CoarseDesc d, T* cf) {
   const size_t zo_ = zoff(top.zs);
   top.zshift(zo_); par.zshift(zo_); cf = zsh(cf, zo_);
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-  T* se = reinterpret_cast<T*>(smem_raw);
-  T* sn = se + d.total;
-  const int t = threadIdx.x;
-  if (par.nx > 0) {
-    T* pe = sn + d.total;  // the parent's faces, pitch par.nx
-    T* pn = pe + par.nx * par.ny;
-    const int fx = par.nx, fy = par.ny;
-    for (int q = t; q < fx * fy; q += kCoarseThreads) {
-      const int r = q / fx, c = q - r * fx;
-      const size_t k = par.at(r, c);
-      pe[q] = par.ex[k];
-      pn[q] = par.ey[k];
-    }
-    __syncthreads();
-    CELLS(0) {
-      const int col = min(2 * c + 1, fx - 1), row = 2 * r + 1;
-      T e = pe[2 * r * fx + col];
-      if (2 * r + 1 < fy) e += pe[(2 * r + 1) * fx + col];
-      T n = (T)0;
-      if (row <= fy - 1) {
-        n = pn[row * fx + 2 * c];
-        if (2 * c + 1 < fx) n += pn[row * fx + 2 * c + 1];
-      }
-      se[r * d.nx[0] + c] = (T)coarse_e_scale(col, fx) * e;
-      sn[r * d.nx[0] + c] = (T)kCoarseScale * n;
-    }
-  } else {
-    CELLS(0) {
-      const size_t k = top.at(r, c);
-      se[r * d.nx[0] + c] = top.ex[k];
-      sn[r * d.nx[0] + c] = top.ey[k];
-    }
-  }
-  __syncthreads();
-  for (int l = 1; l < d.nlev; ++l) {
-    const int fx = d.nx[l - 1], fy = d.ny[l - 1], fo = d.off[l - 1], co = d.off[l], cx = d.nx[l];
-    CELLS(l) {
-      const int col = min(2 * c + 1, fx - 1), row = 2 * r + 1;
-      T e = se[fo + 2 * r * fx + col];
-      if (2 * r + 1 < fy) e += se[fo + (2 * r + 1) * fx + col];
-      T n = (T)0;
-      if (row <= fy - 1) {
-        n = sn[fo + row * fx + 2 * c];
-        if (2 * c + 1 < fx) n += sn[fo + row * fx + 2 * c + 1];
-      }
-      se[co + r * cx + c] = (T)coarse_e_scale(col, fx) * e;
-      sn[co + r * cx + c] = (T)kCoarseScale * n;
-    }
-    __syncthreads();
-  }
-  for (int l = 0; l < d.nlev; ++l) {
-    const int o = d.off[l], nx = d.nx[l];
-    CELLS(l) {
-      const int k = o + r * nx + c;
-      T dd = se[k] + (c > 0 ? se[k - 1] : (T)0) + sn[k] + (r > 0 ? sn[k - nx] : (T)0);
-      dd = dd < (T)kTiny20 ? (T)1 : dd;
-      cf[k] = se[k];
-      cf[d.total + k] = sn[k];
-      cf[2 * d.total + k] = (T)1 / dd;
-      cf[3 * d.total + k] = dd;
-    }
-  }
-  // Inverse of the last level's operator (the faces and floored diagonal the
-  // V-cycle applies) by Gauss-Jordan in FP64 over the whole block: thread t
-  // < n^2 holds entry (i, j) = (t / n, t mod n) in a register; each pivot's
-  // row and column travel through LDS (double-buffered, one barrier per
-  // pivot).  The operator is a symmetric M-matrix, so no pivoting is needed.
-  // Entry (i, j) is built from cell j's faces and updated as
-  // a_ij <- (j == p ? 0 : a_ij) - a_ip (j == p ? 1 : a_pj) / a_pp, row p <- (j == p ? 1 : a_pj) / a_pp.
-  __shared__ double gj_row[2][kDirectCells], gj_col[2][kDirectCells];
-  __syncthreads();
-  {
-    const int l = d.wl, o = d.off[l], nx = d.nx[l], ny = d.ny[l], n = d.dn;
-    const bool mine = t < n * n;
-    const int gi = mine ? t / n : 0, gj = mine ? t - gi * n : 0;
-    double a = 0.0;
-    if (mine) {
-      const int r = gj / max(nx, 1), c = gj - r * nx, k = o + gj;
-      const T dT = se[k] + (c > 0 ? se[k - 1] : (T)0) + sn[k] + (r > 0 ? sn[k - nx] : (T)0);
-      const double dg = dT < (T)kTiny20 ? 1.0 : (double)dT;
-      const double ee = c + 1 < nx ? -(double)se[k] : 0.0;
-      const double ww = c > 0 ? -(double)se[k - 1] : 0.0;
-      const double nn = r + 1 < ny ? -(double)sn[k] : 0.0;
-      const double ss = r > 0 ? -(double)sn[k - nx] : 0.0;
-      if (gi == gj) a = dg;
-      else if (gi == gj + 1) a = ee;
-      else if (gi == gj - 1) a = ww;
-      else if (gi == gj + nx) a = nn;
-      else if (gi == gj - nx) a = ss;
-    }
-    for (int p = 0; p < n; ++p) {
-      const int b = p & 1;
-      if (mine && gi == p) gj_row[b][gj] = a;
-      if (mine && gj == p) gj_col[b][gi] = a;
-      __syncthreads();
-      if (mine) {
-        const double inv = 1.0 / gj_row[b][p];
-        const double mp = (gj == p ? 1.0 : gj_row[b][gj]) * inv;
-        a = gi == p ? mp : (gj == p ? 0.0 : a) - gj_col[b][gi] * mp;
-      }
-    }
-    if (mine) cf[4 * d.total + gj * n + gi] = (T)a;
-  }
+  coarse_setup_body(top, par, d, cf, smem_raw, (int)threadIdx.x);
 }
 
 // All levels from `top` down inside one workgroup, in LDS: per level one
@@ -1845,10 +1688,12 @@ struct MGImpl {
                      part.p + (size_t)S_ERR * G0.NB, G0.NB, j + 1, scal.p};
   }
 
-  bool built = false;  // a hierarchy has been set up
-  void setup(const double* dE, const double* dN) {
+  bool built = false;     // a hierarchy has been set up
+  bool prebuilt = false;  // the coming solve's hierarchy was built ahead (hier() + coarse_job())
+  // the grid levels' faces: level 0 and the next kHierLevels levels in one
+  // launch, any deeper levels one launch each
+  void hier(const double* dE, const double* dN) {
     hipStream_t s = ctx->stream;
-    built = true;
     // level 0 and the next kHierLevels levels in one launch, any deeper
     // levels one launch each
     HierLevels<T> H{};
@@ -1856,13 +1701,21 @@ struct MGImpl {
     for (int l = 0; l <= H.n; ++l) H.l[l] = lv[l]->view();
     k_mg_hier<T><<<ctx->zg(dim3(ceil_div(lv[0]->nx, HX), ceil_div(lv[0]->ny, HY))), HT, 0, s>>>(g, dE, dN, H);
     // deeper grid levels one launch each; the coarse top's faces, if not
-    // built above, are formed by the coarse setup from its parent
-    const bool topFromParent = nGrid > H.n;
+    // built above, are formed by the coarse set-up from its parent
     for (int l = H.n + 1; l < nGrid; ++l)
       k_mg_coarsen<T><<<ctx->zg(lv[l]->grid()), blk(), 0, s>>>(view(l - 1), view(l));
-    const Lv<T> none{};
-    k_mg_coarse_setup<T><<<ctx->zg(1), kCoarseThreads, coarseSetupSmem, s>>>(view(nGrid), topFromParent ? view(nGrid - 1) : none,
-                                                                    cd, cf.p);
+    CPS_HIP(hipGetLastError());
+  }
+  // the coarse solver's set-up (after hier()), for a kernel that carries it
+  CoarseJob<T> coarse_job() {
+    const bool topFromParent = nGrid > std::min((int)lv.size() - 1, kHierLevels);
+    return CoarseJob<T>{view(nGrid), topFromParent ? view(nGrid - 1) : Lv<T>{}, cd, cf.p, 1};
+  }
+  void setup(const double* dE, const double* dN) {
+    built = true;
+    hier(dE, dN);
+    const CoarseJob<T> j = coarse_job();
+    k_mg_coarse_setup<T><<<ctx->zg(1), kCoarseThreads, coarseSetupSmem, ctx->stream>>>(j.top, j.par, j.d, j.cf);
     CPS_HIP(hipGetLastError());
   }
 
@@ -1983,6 +1836,40 @@ struct MGImpl {
 
 PressureMG::PressureMG(Ctx* ctx, const Geo& g) : ctx_(ctx), g_(g) {}
 
+template <class T>
+static MGImpl<T>* impl_of(void*& p, Ctx* ctx, const Geo& g) {
+  if (!p) p = new MGImpl<T>(ctx, g);
+  return static_cast<MGImpl<T>*>(p);
+}
+
+void PressureMG::prebuild(const double* dE, const double* dN, bool fp64) {
+  if (fp64) {
+    auto* m = impl_of<double>(impl64_, ctx_, g_);
+    m->hier(dE, dN);
+    m->built = m->prebuilt = true;
+  } else {
+    auto* m = impl_of<float>(impl32_, ctx_, g_);
+    m->hier(dE, dN);
+    m->built = m->prebuilt = true;
+  }
+}
+
+bool PressureMG::coarse_job(mgc::CoarseJob<double>* j, size_t* smem) {
+  auto* m = static_cast<MGImpl<double>*>(impl64_);
+  if (!m || !m->prebuilt) return false;
+  *j = m->coarse_job();
+  *smem = m->coarseSetupSmem;
+  return true;
+}
+
+bool PressureMG::coarse_job(mgc::CoarseJob<float>* j, size_t* smem) {
+  auto* m = static_cast<MGImpl<float>*>(impl32_);
+  if (!m || !m->prebuilt) return false;
+  *j = m->coarse_job();
+  *smem = m->coarseSetupSmem;
+  return true;
+}
+
 PressureMG::~PressureMG() {
   delete static_cast<MGImpl<double>*>(impl64_);
   delete static_cast<MGImpl<float>*>(impl32_);
@@ -2001,16 +1888,15 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
   const int maxit = std::max(0, cfg.max_pressure_iter);
   const int sync_every = maxit > kNoSyncCap ? 4 : 0;
   if (fp64) {
-    if (!impl64_) impl64_ = new MGImpl<double>(ctx_, g_);
-    auto* m = static_cast<MGImpl<double>*>(impl64_);
+    auto* m = impl_of<double>(impl64_, ctx_, g_);
     m->timer = timer;
-    if (!same_operator || !m->built) m->setup(dE, dN);
+    if (m->prebuilt) m->prebuilt = false;
+    else if (!same_operator || !m->built) m->setup(dE, dN);
     m->pcg(r, cfg.p_tol, cfg.p_rtol, maxit, sync_every, nullptr);
     k_copy_out<double><<<ctx_->zg(m->lv[0]->grid()), dim3(MBX, MBY), 0, s>>>(m->view(0), m->xs.p, x, g_.P);
     k_copy_state<<<ctx_->zg(1), 64, 0, s>>>(st, m->scal.p, m->scal.p + 1, 0, ctx_->zs);
   } else {
-    if (!impl32_) impl32_ = new MGImpl<float>(ctx_, g_);
-    auto* m = static_cast<MGImpl<float>*>(impl32_);
+    auto* m = impl_of<float>(impl32_, ctx_, g_);
     const size_t S = (size_t)(g_.M + 1) * g_.P;
     if (!b64_) {
       auto* b = new DevBuf<double>;
@@ -2019,7 +1905,8 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
     }
     double* b64 = static_cast<DevBuf<double>*>(b64_)->p;
     m->timer = timer;
-    if (!same_operator || !m->built) m->setup(dE, dN);
+    if (m->prebuilt) m->prebuilt = false;
+    else if (!same_operator || !m->built) m->setup(dE, dN);
     // The first sweep's PCG start also keeps the FP64 right side in b64 and
     // clears the refinement state; its x64 update (x is zero on entry) is
     // folded into the residual kernel; without host polling the inner

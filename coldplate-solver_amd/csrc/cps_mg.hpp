@@ -6,6 +6,7 @@
 #include <memory>
 #include <vector>
 
+#include "cps_mg_coarse.hpp"
 #include "cps_simple.hpp"
 
 namespace cps {
@@ -22,6 +23,15 @@ class PressureMG {
   // built then is reused.
   void solve(const double* dE, const double* dN, double* r, double* x, IterState* st, const cps_simple_config& cfg,
              bool fp64, KernelTimer* timer, bool same_operator = false);
+  // Builds the coming solve's hierarchy ahead of it, once dE and dN are
+  // final: prebuild() launches the grid levels' faces, and coarse_job()
+  // hands the coarse solver's set-up to a kernel the caller launches next
+  // (one extra block beside that kernel's own work).  The next solve in that
+  // precision then builds nothing.  coarse_job() returns false when no
+  // prebuilt hierarchy is pending.
+  void prebuild(const double* dE, const double* dN, bool fp64);
+  bool coarse_job(mgc::CoarseJob<double>* j, size_t* smem);
+  bool coarse_job(mgc::CoarseJob<float>* j, size_t* smem);
   // Mixed solves with one refinement sweep leave their closing check (the
   // pressure statistics in IterState) until the host reads the statistics:
   // no later kernel of the step reads them.  Launches it if one is pending.

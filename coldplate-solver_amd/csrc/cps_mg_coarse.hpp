@@ -1,0 +1,205 @@
+// cps_mg_coarse.hpp -- device pieces of the pressure multigrid that the
+// SIMPLE step kernels share with the multigrid kernels: the level view, the
+// coarse solver's descriptor and its once-per-solve set-up.  The set-up is
+// one workgroup's work; it runs either as its own launch
+// (k_mg_coarse_setup) or as one extra block of the step's momentum
+// finalize kernel, which reads nothing the set-up writes and so carries it
+// beside its own tiles.
+#pragma once
+
+#include "cps_common.hpp"
+
+namespace cps {
+namespace mgc {
+
+#ifndef CPS_MG_COARSE_SCALE
+#define CPS_MG_COARSE_SCALE 0.5
+#endif
+#ifndef CPS_MG_DIRICHLET_SCALE
+#define CPS_MG_DIRICHLET_SCALE 1.25
+#endif
+// Coarse faces are this share of the covered fine faces' sum (0.5: twice
+// the Galerkin coarse correction).
+constexpr double kCoarseScale = CPS_MG_COARSE_SCALE;
+// The east face of a level's last column couples it to the pinned outlet
+// column p'(., N-1) = 0.  On a coarse level that column's centre lies half a
+// coarse cell or less from the pinned one, while its face sum keeps the fine
+// face's size, so kCoarseScale alone under-weights the outlet and the
+// smoothest error mode (the only one the outlet pins) is over-corrected.
+// The last coarse column's east face carries this extra factor.
+constexpr double kDirichletScale = CPS_MG_DIRICHLET_SCALE;
+// the factor of a coarse east face whose covered fine column is fcol of a
+// finer level fnx wide (the last coarse column covers the last fine one)
+__host__ __device__ constexpr double coarse_e_scale(int fcol, int fnx) {
+  return fcol >= fnx - 1 ? kCoarseScale * kDirichletScale : kCoarseScale;
+}
+#ifndef CPS_COARSE_THREADS
+#define CPS_COARSE_THREADS 1024
+#endif
+constexpr int kCoarseThreads = CPS_COARSE_THREADS;
+// the coarse hierarchy ends at its first level of at most kDirectCells cells,
+// which is solved with its inverse (one column per lane of a wavefront)
+constexpr int kDirectCells = 32;
+static_assert(kDirectCells * kDirectCells <= kCoarseThreads && kDirectCells <= 64, "direct level staging");
+constexpr double kTiny20 = (double)1e-20f;
+
+template <class T>
+struct Lv {  // device view of one level; cell (r,c) at off + r*pitch + c
+  int nx, ny, pitch, off;
+  T *x, *y, *b, *ex, *ey;
+  size_t zs;  // byte stride between the fields of a class-batched launch
+  __device__ size_t at(int r, int c) const { return (size_t)off + (size_t)r * pitch + c; }
+  __device__ void zshift(size_t o) {
+    x = zsh(x, o);
+    y = zsh(y, o);
+    b = zsh(b, o);
+    ex = zsh(ex, o);
+    ey = zsh(ey, o);
+  }
+};
+
+// All levels from `top` down inside one workgroup: the top level's rhs and
+// faces come from global memory, every smaller level is built and solved in
+// LDS; the top level's correction is written to top.y.
+struct CoarseDesc {
+  int nlev;
+  int nx[16], ny[16], off[16];  // compact LDS layout, pitch = nx
+  int total;
+  int wl;  // the last level (at most kDirectCells cells): solved with its inverse
+  int dn;  // its cell count; the inverse sits at cf[4 total ..), column-major
+  unsigned long long mag[16];  // ceil(2^32 / nx): row = (q mag) >> 32 for a cell index q < 2^20
+};
+
+// the cells of level l, one per thread in row-major order (every lane of a
+// wave busy on narrow levels); r = q / nx by the reciprocal multiply
+#define MGC_CELLS(l)                                                             \
+  for (int q_ = t, n_ = d.nx[l] * d.ny[l]; q_ < n_; q_ += kCoarseThreads)       \
+    if (const int r = (int)(((unsigned long long)q_ * d.mag[l]) >> 32), c = q_ - r * d.nx[l]; true)
+
+// Once per pressure solve: faces and inverse diagonals of every level of the
+// coarse solver, built from the top level's faces (half the sum of the
+// covered fine faces), stored as cf[0 .. 4 total) = east, north, 1/diag, diag.
+// With par.nx > 0 the top level's faces are not in memory: they are built
+// here from its parent grid level par, with k_mg_coarsen's arithmetic (no
+// other kernel reads the top level's faces).  Run by kCoarseThreads threads
+// (t = linear thread index) of one block, pointers already shifted to the
+// block's field; smem_raw holds coarse_setup_smem() bytes.
+template <class T>
+__device__ __forceinline__ void coarse_setup_body(const Lv<T>& top, const Lv<T>& par, const CoarseDesc& d, T* cf,
+                                                  unsigned char* smem_raw, int t) {
+  T* se = reinterpret_cast<T*>(smem_raw);
+  T* sn = se + d.total;
+  if (par.nx > 0) {
+    T* pe = sn + d.total;  // the parent's faces, pitch par.nx
+    T* pn = pe + par.nx * par.ny;
+    const int fx = par.nx, fy = par.ny;
+    for (int q = t; q < fx * fy; q += kCoarseThreads) {
+      const int r = q / fx, c = q - r * fx;
+      const size_t k = par.at(r, c);
+      pe[q] = par.ex[k];
+      pn[q] = par.ey[k];
+    }
+    __syncthreads();
+    MGC_CELLS(0) {
+      const int col = min(2 * c + 1, fx - 1), row = 2 * r + 1;
+      T e = pe[2 * r * fx + col];
+      if (2 * r + 1 < fy) e += pe[(2 * r + 1) * fx + col];
+      T n = (T)0;
+      if (row <= fy - 1) {
+        n = pn[row * fx + 2 * c];
+        if (2 * c + 1 < fx) n += pn[row * fx + 2 * c + 1];
+      }
+      se[r * d.nx[0] + c] = (T)coarse_e_scale(col, fx) * e;
+      sn[r * d.nx[0] + c] = (T)kCoarseScale * n;
+    }
+  } else {
+    MGC_CELLS(0) {
+      const size_t k = top.at(r, c);
+      se[r * d.nx[0] + c] = top.ex[k];
+      sn[r * d.nx[0] + c] = top.ey[k];
+    }
+  }
+  __syncthreads();
+  for (int l = 1; l < d.nlev; ++l) {
+    const int fx = d.nx[l - 1], fy = d.ny[l - 1], fo = d.off[l - 1], co = d.off[l], cx = d.nx[l];
+    MGC_CELLS(l) {
+      const int col = min(2 * c + 1, fx - 1), row = 2 * r + 1;
+      T e = se[fo + 2 * r * fx + col];
+      if (2 * r + 1 < fy) e += se[fo + (2 * r + 1) * fx + col];
+      T n = (T)0;
+      if (row <= fy - 1) {
+        n = sn[fo + row * fx + 2 * c];
+        if (2 * c + 1 < fx) n += sn[fo + row * fx + 2 * c + 1];
+      }
+      se[co + r * cx + c] = (T)coarse_e_scale(col, fx) * e;
+      sn[co + r * cx + c] = (T)kCoarseScale * n;
+    }
+    __syncthreads();
+  }
+  for (int l = 0; l < d.nlev; ++l) {
+    const int o = d.off[l], nx = d.nx[l];
+    MGC_CELLS(l) {
+      const int k = o + r * nx + c;
+      T dd = se[k] + (c > 0 ? se[k - 1] : (T)0) + sn[k] + (r > 0 ? sn[k - nx] : (T)0);
+      dd = dd < (T)kTiny20 ? (T)1 : dd;
+      cf[k] = se[k];
+      cf[d.total + k] = sn[k];
+      cf[2 * d.total + k] = (T)1 / dd;
+      cf[3 * d.total + k] = dd;
+    }
+  }
+  // Inverse of the last level's operator (the faces and floored diagonal the
+  // V-cycle applies) by Gauss-Jordan in FP64 over the whole block: thread t
+  // < n^2 holds entry (i, j) = (t / n, t mod n) in a register; each pivot's
+  // row and column travel through LDS (double-buffered, one barrier per
+  // pivot).  The operator is a symmetric M-matrix, so no pivoting is needed.
+  // Entry (i, j) is built from cell j's faces and updated as
+  // a_ij <- (j == p ? 0 : a_ij) - a_ip (j == p ? 1 : a_pj) / a_pp, row p <- (j == p ? 1 : a_pj) / a_pp.
+  __shared__ double gj_row[2][kDirectCells], gj_col[2][kDirectCells];
+  __syncthreads();
+  {
+    const int l = d.wl, o = d.off[l], nx = d.nx[l], ny = d.ny[l], n = d.dn;
+    const bool mine = t < n * n;
+    const int gi = mine ? t / n : 0, gj = mine ? t - gi * n : 0;
+    double a = 0.0;
+    if (mine) {
+      const int r = gj / max(nx, 1), c = gj - r * nx, k = o + gj;
+      const T dT = se[k] + (c > 0 ? se[k - 1] : (T)0) + sn[k] + (r > 0 ? sn[k - nx] : (T)0);
+      const double dg = dT < (T)kTiny20 ? 1.0 : (double)dT;
+      const double ee = c + 1 < nx ? -(double)se[k] : 0.0;
+      const double ww = c > 0 ? -(double)se[k - 1] : 0.0;
+      const double nn = r + 1 < ny ? -(double)sn[k] : 0.0;
+      const double ss = r > 0 ? -(double)sn[k - nx] : 0.0;
+      if (gi == gj) a = dg;
+      else if (gi == gj + 1) a = ee;
+      else if (gi == gj - 1) a = ww;
+      else if (gi == gj + nx) a = nn;
+      else if (gi == gj - nx) a = ss;
+    }
+    for (int p = 0; p < n; ++p) {
+      const int b = p & 1;
+      if (mine && gi == p) gj_row[b][gj] = a;
+      if (mine && gj == p) gj_col[b][gi] = a;
+      __syncthreads();
+      if (mine) {
+        const double inv = 1.0 / gj_row[b][p];
+        const double mp = (gj == p ? 1.0 : gj_row[b][gj]) * inv;
+        a = gi == p ? mp : (gj == p ? 0.0 : a) - gj_col[b][gi] * mp;
+      }
+    }
+    if (mine) cf[4 * d.total + gj * n + gi] = (T)a;
+  }
+}
+
+// The coarse set-up of the coming pressure solve, handed to a kernel that
+// carries it as one extra block (on = 0: none).
+template <class T>
+struct CoarseJob {
+  Lv<T> top, par;  // par.nx == 0: the top's faces are in memory
+  CoarseDesc d;
+  T* cf;
+  int on;
+};
+
+}  // namespace mgc
+}  // namespace cps

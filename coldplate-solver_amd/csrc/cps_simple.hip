@@ -684,12 +684,33 @@ void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> sv, int k, int two, double tol2, I
 // k_jacobi2 pairs (x_j in buffer j mod 3) on a grid of `pairs` blocks; if
 // the last launch was a pair, its second sweep counts only if the first
 // one's stop test failed.
+//
+// cj.on: the grid has one extra column of blocks (blockIdx.x == g.gx), and
+// block (g.gx, 0) runs the coarse set-up of the coming multigrid pressure
+// solve (faces, inverse diagonals and last-level inverse, from the grid
+// levels' faces that dE, dN gave) beside the finalize tiles, which touch
+// none of its data; the column's other blocks return at once.  The tiles'
+// block index stays bx + by gx, so their partials are those of the plain
+// grid.
 template <class C, bool Delta>
 __global__ __launch_bounds__(NT) void k_finalize(Geo g, Fields f, MomSys<C> su, MomSys<C> sv, IterState* st,
-                                                 double* part, int K, int pairs, double tol2) {
+                                                 double* part, int K, int pairs, double tol2, mgc::CoarseJob<C> cj) {
   const size_t zo_ = zoff(g.zs);
+  if ((int)blockIdx.x >= g.gx) {  // block-uniform
+    if (cj.on && blockIdx.y == 0) {
+      // shifted copies of the two level views only: the descriptor stays in
+      // the argument segment (its arrays are indexed by level)
+      mgc::Lv<C> top = cj.top, par = cj.par;
+      top.zshift(zo_);
+      par.zshift(zo_);
+      extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+      mgc::coarse_setup_body<C>(top, par, cj.d, zsh(cj.cf, zo_), smem_raw, thread_id());
+    }
+    return;
+  }
   f.zshift(zo_); su.zshift(zo_); sv.zshift(zo_); st = zsh(st, zo_); part = zsh(part, zo_);
   __shared__ double sh[NT / kWave + 1];
+  const int bid = (int)blockIdx.x + (int)blockIdx.y * g.gx;
   const int M = g.M, N = g.N;
   const double mv = g.maxVel;
   double su2 = 0.0, sv2 = 0.0, umx = 0.0, vmx = 0.0;
@@ -705,7 +726,7 @@ __global__ __launch_bounds__(NT) void k_finalize(Geo g, Fields f, MomSys<C> su, 
       if (au) itU = K;
       if (av) itV = K;
     }
-    if (block_id() == 0 && thread_id() == 0) { st->jac_iters[0] = itU; st->jac_iters[1] = itV; }
+    if (bid == 0 && thread_id() == 0) { st->jac_iters[0] = itU; st->jac_iters[1] = itV; }
   }
   FIELD_LOOP(g, i, j) {
     if (j > N) continue;
@@ -739,7 +760,7 @@ __global__ __launch_bounds__(NT) void k_finalize(Geo g, Fields f, MomSys<C> su, 
   const double a = block_sum<NT>(su2, sh), b = block_sum<NT>(sv2, sh);
   const double c = block_max<NT>(umx, sh), d = block_max<NT>(vmx, sh);
   if (thread_id() == 0) {
-    double* p = part + (size_t)SLOT_FIN * g.NB + block_id();
+    double* p = part + (size_t)SLOT_FIN * g.NB + bid;
     p[0] = a;
     p[g.NB] = b;
     p[2 * g.NB] = c;
@@ -1547,6 +1568,9 @@ class SimpleSolver {
     if (mode_fp64) k_assemble<double, false><<<ctx->zg(active_grid()), block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p);
     else k_assemble<float, true><<<ctx->zg(active_grid()), block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p);
     CPS_HIP(hipGetLastError());
+    // dE, dN are final: the multigrid pressure solve's hierarchy is built
+    // now (grid levels here, the coarse set-up inside k_finalize)
+    if (mg && cfg.pressure_solver == CPS_PRESSURE_GMG) mg->prebuild(dE.p, dN.p, mode_fp64 != 0);
     int k = 0;
     // two sweeps per launch on the unpolled delta path (the timed single
     // sweeps keep their own kernel); the pair grid tiles the active cells
@@ -1586,11 +1610,23 @@ class SimpleSolver {
       if (!act[0] && !act[1]) break;
     }
     const int K = maxit == 0 ? 0 : chunk == maxit ? maxit : -1;
-    if (mode_fp64)
-      k_finalize<double, false><<<ctx->zg(grid), block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p, part.p, K, 0, tol2);
-    else
-      k_finalize<float, true><<<ctx->zg(grid), block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p, part.p, K,
-                                                      pairs ? (int)(jgrid.x * jgrid.y) : 0, tol2);
+    // the multigrid's coarse set-up rides on the finalize launch as one
+    // extra column of blocks (the grid levels' faces were built after the
+    // assembly)
+    size_t smem = 0;
+    if (mode_fp64) {
+      mgc::CoarseJob<double> cj{};
+      const bool job = mg && mg->coarse_job(&cj, &smem);
+      const dim3 fg(grid.x + (job ? 1 : 0), grid.y);
+      k_finalize<double, false><<<ctx->zg(fg), block, smem, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p, part.p, K, 0,
+                                                                 tol2, cj);
+    } else {
+      mgc::CoarseJob<float> cj{};
+      const bool job = mg && mg->coarse_job(&cj, &smem);
+      const dim3 fg(grid.x + (job ? 1 : 0), grid.y);
+      k_finalize<float, true><<<ctx->zg(fg), block, smem, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p, part.p, K,
+                                                               pairs ? (int)(jgrid.x * jgrid.y) : 0, tol2, cj);
+    }
     CPS_HIP(hipGetLastError());
   }
 
