@@ -17,9 +17,9 @@ namespace cps {
 // XCD a contiguous band of tiles, whose shared halo lines then meet in that
 // XCD's L2.  Block partials are stored at L, so every reduction keeps its
 // order (and its result).
+__device__ inline int xcd_tile_of(int b, int nb) { return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3); }
 __device__ inline int xcd_tile() {
-  const int nb = (int)(gridDim.x * gridDim.y), b = (int)(blockIdx.x + blockIdx.y * gridDim.x);
-  return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+  return xcd_tile_of((int)(blockIdx.x + blockIdx.y * gridDim.x), (int)(gridDim.x * gridDim.y));
 }
 
 // The same XCD-aware order for any block count of a 1-D grid: XCD x (which

@@ -110,80 +110,15 @@ __device__ inline T coarse_ey(const Lv<T>& F, int R, int C) {
   return (T)kCoarseScale * s;
 }
 
-// Level 0 and up to kHierLevels coarser levels in one launch: coarse faces
-// are half the sums of the faces they cover (coarse_ex / coarse_ey), a purely
-// local 2x2 aggregation, so a block that owns a HX x HY level-0 tile owns the
-// nested tiles of every coarser level.  Level 0: aE = rho dE hy, aN = rho dN
-// hx (0 on the last row), the pressure operator's faces
-// (pressure_solver.cpp:589-594); coarser levels: the arithmetic of
-// k_mg_coarsen.
-constexpr int kHierLevels = 4, HX = 128, HY = 32, HT = 1024;
-template <class T>
-struct HierLevels {
-  Lv<T> l[kHierLevels + 1];
-  int n;  // coarser levels built here (<= kHierLevels)
-};
+// Level 0 and up to kHierLevels coarser levels in one launch (hier_body):
+// HX x HY level-0 tiles, one per block.
+constexpr int HX = 128, HY = 32, HT = 1024;
 template <class T>
 __global__ __launch_bounds__(HT) void k_mg_hier(Geo g, const double* dE, const double* dN, HierLevels<T> H) {
   const size_t zo_ = zoff(g.zs);
-  dE = zsh(dE, zo_); dN = zsh(dN, zo_);  // the levels' face pointers are shifted where they are used
-  // even levels in the level-0-sized buffers, odd levels in the quarter-sized ones
   __shared__ T sEa[HY * HX], sNa[HY * HX], sEb[(HY / 2) * (HX / 2)], sNb[(HY / 2) * (HX / 2)];
-  const int t = threadIdx.x;
-  const int M = g.M;
-  int cx0 = blockIdx.x * HX, cy0 = blockIdx.y * HY;  // tile origin at the current level
-  int tw = HX, th = HY;                              // tile size at the current level
-  {
-    const Lv<T>& L = H.l[0];
-    for (int q = t; q < HX * HY; q += HT) {
-      const int lr = q / HX, lc = q - lr * HX, r = cy0 + lr, c = cx0 + lc;
-      if (r >= L.ny || c >= L.nx) continue;
-      const int i = r + 1, j = c + 1;
-      const size_t f = (size_t)i * g.P + j;
-      const double aE = g.rho * dE[f] * g.hy;
-      const double aN = (i < M - 1) ? g.rho * dN[f] * g.hx : 0.0;
-      const size_t k = L.at(r, c);
-      zsh(L.ex, zo_)[k] = (T)aE;
-      zsh(L.ey, zo_)[k] = (T)aN;
-      sEa[q] = (T)aE;
-      sNa[q] = (T)aN;
-    }
-  }
-  for (int l = 1; l <= H.n; ++l) {
-    __syncthreads();
-    const Lv<T>& F = H.l[l - 1];
-    const Lv<T>& Cl = H.l[l];
-    const T* fe = (l & 1) ? sEa : sEb;  // the finer level's faces
-    const T* fn = (l & 1) ? sNa : sNb;
-    T* ce = (l & 1) ? sEb : sEa;
-    T* cn = (l & 1) ? sNb : sNa;
-    const int fw = tw;  // row length of the finer tile in LDS
-    cx0 >>= 1;
-    cy0 >>= 1;
-    tw >>= 1;
-    th >>= 1;
-    for (int q = t; q < tw * th; q += HT) {
-      const int lr = q / tw, lc = q - lr * tw, R = cy0 + lr, C = cx0 + lc;
-      if (R >= Cl.ny || C >= Cl.nx) continue;
-      // fine local (2 lr + dr, 2 lc + dc); the east column clamps to the
-      // last fine column, the second row exists when 2R + 1 < F.ny
-      const int lcol = 2 * C + 1 <= F.nx - 1 ? 2 * lc + 1 : 2 * lc;
-      T e = fe[(2 * lr) * fw + lcol];
-      if (2 * R + 1 < F.ny) e += fe[(2 * lr + 1) * fw + lcol];
-      T n = (T)0;
-      if (2 * R + 1 <= F.ny - 1) {
-        n = fn[(2 * lr + 1) * fw + 2 * lc];
-        if (2 * C + 1 < F.nx) n += fn[(2 * lr + 1) * fw + 2 * lc + 1];
-      }
-      e = (T)coarse_e_scale(min(2 * C + 1, F.nx - 1), F.nx) * e;
-      n = (T)kCoarseScale * n;
-      const size_t k = Cl.at(R, C);
-      zsh(Cl.ex, zo_)[k] = e;
-      zsh(Cl.ey, zo_)[k] = n;
-      ce[lr * tw + lc] = e;
-      cn[lr * tw + lc] = n;
-    }
-  }
+  hier_body<T, HT, HX, HY>(g, zsh(dE, zo_), zsh(dN, zo_), H, (int)blockIdx.x, (int)blockIdx.y, (int)threadIdx.x, zo_,
+                           sEa, sNa, sEb, sNb);
 }
 
 template <class T>
@@ -1706,6 +1641,20 @@ struct MGImpl {
       k_mg_coarsen<T><<<ctx->zg(lv[l]->grid()), blk(), 0, s>>>(view(l - 1), view(l));
     CPS_HIP(hipGetLastError());
   }
+  // the hier() launch's work for a kernel that carries it as extra blocks
+  // (hier_body on HJX x HJY tiles); none where deeper grid levels need
+  // their own launches after it
+  bool hier_job(HierJob<T>* j) {
+    const int n = std::min((int)lv.size() - 1, kHierLevels);
+    if (n + 1 < nGrid) return false;
+    *j = HierJob<T>{};
+    j->H.n = n;
+    for (int l = 0; l <= n; ++l) j->H.l[l] = lv[l]->view();
+    j->hgx = ceil_div(lv[0]->nx, HJX);
+    j->hgy = ceil_div(lv[0]->ny, HJY);
+    j->on = 1;
+    return true;
+  }
   // the coarse solver's set-up (after hier()), for a kernel that carries it
   CoarseJob<T> coarse_job() {
     const bool topFromParent = nGrid > std::min((int)lv.size() - 1, kHierLevels);
@@ -1842,14 +1791,20 @@ static MGImpl<T>* impl_of(void*& p, Ctx* ctx, const Geo& g) {
   return static_cast<MGImpl<T>*>(p);
 }
 
-void PressureMG::prebuild(const double* dE, const double* dN, bool fp64) {
+void PressureMG::prebuild(const double* dE, const double* dN, bool fp64, mgc::HierJob<float>* carry) {
+  if (carry) carry->on = 0;
   if (fp64) {
     auto* m = impl_of<double>(impl64_, ctx_, g_);
     m->hier(dE, dN);
     m->built = m->prebuilt = true;
   } else {
     auto* m = impl_of<float>(impl32_, ctx_, g_);
-    m->hier(dE, dN);
+    if (carry && m->hier_job(carry)) {
+      carry->dE = dE;
+      carry->dN = dN;
+    } else {
+      m->hier(dE, dN);
+    }
     m->built = m->prebuilt = true;
   }
 }

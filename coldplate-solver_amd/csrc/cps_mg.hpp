@@ -29,7 +29,10 @@ class PressureMG {
   // (one extra block beside that kernel's own work).  The next solve in that
   // precision then builds nothing.  coarse_job() returns false when no
   // prebuilt hierarchy is pending.
-  void prebuild(const double* dE, const double* dN, bool fp64);
+  // carry != nullptr (FP32-inner form): where it can, the grid levels' build
+  // is not launched but returned in *carry (on = 1) for the caller's next
+  // kernel to run as extra blocks.
+  void prebuild(const double* dE, const double* dN, bool fp64, mgc::HierJob<float>* carry = nullptr);
   bool coarse_job(mgc::CoarseJob<double>* j, size_t* smem);
   bool coarse_job(mgc::CoarseJob<float>* j, size_t* smem);
   // Mixed solves with one refinement sweep leave their closing check (the

@@ -7,7 +7,7 @@
 // beside its own tiles.
 #pragma once
 
-#include "cps_common.hpp"
+#include "cps_simple.hpp"
 
 namespace cps {
 namespace mgc {
@@ -190,6 +190,97 @@ __device__ __forceinline__ void coarse_setup_body(const Lv<T>& top, const Lv<T>&
     if (mine) cf[4 * d.total + gj * n + gi] = (T)a;
   }
 }
+
+// Level 0 and up to kHierLevels coarser levels of the grid hierarchy from dE,
+// dN: coarse faces are half the sums of the faces they cover, a purely local
+// 2x2 aggregation, so a block that owns a TW x TH level-0 tile (TW, TH
+// multiples of 2^kHierLevels) owns the nested tiles of every coarser level,
+// and the results do not depend on the tile shape.  Level 0: aE = rho dE hy,
+// aN = rho dN hx (0 on the last row), the pressure operator's faces
+// (pressure_solver.cpp:589-594); coarser levels: the arithmetic of
+// k_mg_coarsen.  NTH threads (t = linear index) of block (bx, by); dE, dN
+// already shifted to the block's field, the levels' face pointers shifted by
+// zo here; LDS: sEa, sNa hold TW x TH, sEb, sNb a quarter of that (even levels
+// in the first pair, odd levels in the second).
+constexpr int kHierLevels = 4;
+template <class T>
+struct HierLevels {
+  Lv<T> l[kHierLevels + 1];
+  int n;  // coarser levels built here (<= kHierLevels)
+};
+template <class T, int NTH, int TW, int TH>
+__device__ __forceinline__ void hier_body(const Geo& g, const double* dE, const double* dN, const HierLevels<T>& H,
+                                          int bx, int by, int t, size_t zo, T* sEa, T* sNa, T* sEb, T* sNb) {
+  static_assert(TW % (1 << kHierLevels) == 0 && TH % (1 << kHierLevels) == 0, "nested tiles");
+  const int M = g.M;
+  int cx0 = bx * TW, cy0 = by * TH;  // tile origin at the current level
+  int tw = TW, th = TH;              // tile size at the current level
+  {
+    const Lv<T>& L = H.l[0];
+    for (int q = t; q < TW * TH; q += NTH) {
+      const int lr = q / TW, lc = q - lr * TW, r = cy0 + lr, c = cx0 + lc;
+      if (r >= L.ny || c >= L.nx) continue;
+      const int i = r + 1, j = c + 1;
+      const size_t f = (size_t)i * g.P + j;
+      const double aE = g.rho * dE[f] * g.hy;
+      const double aN = (i < M - 1) ? g.rho * dN[f] * g.hx : 0.0;
+      const size_t k = L.at(r, c);
+      zsh(L.ex, zo)[k] = (T)aE;
+      zsh(L.ey, zo)[k] = (T)aN;
+      sEa[q] = (T)aE;
+      sNa[q] = (T)aN;
+    }
+  }
+  for (int l = 1; l <= H.n; ++l) {
+    __syncthreads();
+    const Lv<T>& F = H.l[l - 1];
+    const Lv<T>& Cl = H.l[l];
+    const T* fe = (l & 1) ? sEa : sEb;  // the finer level's faces
+    const T* fn = (l & 1) ? sNa : sNb;
+    T* ce = (l & 1) ? sEb : sEa;
+    T* cn = (l & 1) ? sNb : sNa;
+    const int fw = tw;  // row length of the finer tile in LDS
+    cx0 >>= 1;
+    cy0 >>= 1;
+    tw >>= 1;
+    th >>= 1;
+    for (int q = t; q < tw * th; q += NTH) {
+      const int lr = q / tw, lc = q - lr * tw, R = cy0 + lr, C = cx0 + lc;
+      if (R >= Cl.ny || C >= Cl.nx) continue;
+      // fine local (2 lr + dr, 2 lc + dc); the east column clamps to the
+      // last fine column, the second row exists when 2R + 1 < F.ny
+      const int lcol = 2 * C + 1 <= F.nx - 1 ? 2 * lc + 1 : 2 * lc;
+      T e = fe[(2 * lr) * fw + lcol];
+      if (2 * R + 1 < F.ny) e += fe[(2 * lr + 1) * fw + lcol];
+      T n = (T)0;
+      if (2 * R + 1 <= F.ny - 1) {
+        n = fn[(2 * lr + 1) * fw + 2 * lc];
+        if (2 * C + 1 < F.nx) n += fn[(2 * lr + 1) * fw + 2 * lc + 1];
+      }
+      e = (T)coarse_e_scale(min(2 * C + 1, F.nx - 1), F.nx) * e;
+      n = (T)kCoarseScale * n;
+      const size_t k = Cl.at(R, C);
+      zsh(Cl.ex, zo)[k] = e;
+      zsh(Cl.ey, zo)[k] = n;
+      ce[lr * tw + lc] = e;
+      cn[lr * tw + lc] = n;
+    }
+  }
+}
+
+// The hierarchy build handed to the first paired momentum Jacobi launch:
+// HJX x HJY level-0 tiles on blocks of 512 threads, in extra columns of
+// its grid (on = 0: none).
+constexpr int HJX = 64, HJY = 32, HJT = 512;
+template <class T>
+struct HierJob {
+  HierLevels<T> H;
+  const double* dE;
+  const double* dN;
+  int gxm;       // the launch's own grid width; blocks at x >= gxm are hierarchy blocks
+  int hgx, hgy;  // hierarchy tiles
+  int on;
+};
 
 // The coarse set-up of the coming pressure solve, handed to a kernel that
 // carries it as one extra block (on = 0: none).
