@@ -137,12 +137,6 @@ __global__ __launch_bounds__(MNT) void k_mg_coarsen(Lv<T> F, Lv<T> Cl) {
 // stages b and the faces of the tile plus a halo of HL cells in LDS, so the
 // two Jacobi sweeps, the residual and the restriction (down) or the
 // prolongation and the two sweeps (up) of one level run in one launch.
-struct PcgScal {
-  double rzs[2], rr0, rr, tol2;
-  int stop, iters, pad_[2];
-  double alphas[2];  // step lengths of the single-reduction form, by iteration parity
-};
-
 // Single-reduction (Chronopoulos-Gear) PCG step of iteration j from
 // gamma = r_j . z_j and delta = z_j . A z_j: beta_j = gamma / gamma_(j-1),
 // alpha_j = gamma / (delta - beta_j gamma / alpha_(j-1)); the denominator
@@ -386,12 +380,12 @@ __device__ inline void tile_load(const Lv<T>& L, const Tile<T>& tl, int t, T* sx
 // (blocks of TX x TY / RT threads), as k_mg_up.
 template <class T, int RT>
 __global__ __launch_bounds__(FT / RT) __attribute__((amdgpu_waves_per_eu(8)))
-void k_mg_down(Lv<T> L, Lv<T> Cl, const int* stop, PcgTest test, XrFuse<T> xr) {
+void k_mg_down(Lv<T> L, Lv<T> Cl, const int* stop, PcgTest test, XrFuse<T> xr, double* rr0) {
   constexpr int NTH = FT / RT, TYR = TY / RT, KP = 2 * FT / NTH;  // KP: partials per thread (<= 2 FT tiles)
   const size_t zo_ = zoff(L.zs);
-  L.zshift(zo_); Cl.zshift(zo_); stop = zsh(stop, zo_); test.zshift(zo_); xr.zshift(zo_);
+  L.zshift(zo_); Cl.zshift(zo_); stop = zsh(stop, zo_); test.zshift(zo_); xr.zshift(zo_); rr0 = zsh(rr0, zo_);
   __shared__ T sb[RN], se[RN], sn[RN], s0[RN], s1[RN], si[RN];
-  __shared__ double sh[NTH / kWave + 1];
+  __shared__ double sh[TY > NTH / kWave + 1 ? TY : NTH / kWave + 1];
   __shared__ double sh2[2 * (NTH / kWave)];
   __shared__ int sflag;
   // a fused launch enqueued after its PCG stopped exits before loading its
@@ -520,6 +514,31 @@ void k_mg_down(Lv<T> L, Lv<T> Cl, const int* stop, PcgTest test, XrFuse<T> xr) {
     }
   }
   if (xr.ran && blockIdx.x == 0 && blockIdx.y == 0 && t == 0) *xr.ran = 1;
+  if (rr0) {
+    // the first V-cycle of a solve whose start rode on the right side's
+    // kernel: the ||r0||^2 partial of the tile, b = r0 being in LDS, in the
+    // start kernel's order (wave sums of the tile rows, one row a wave, then
+    // the rows in order), so the stop test sees the same sums
+    double er[RT];
+#pragma unroll
+    for (int o = 0; o < RT; ++o) {
+      const int lr = HL + threadIdx.y + o * TYR, lc = HL + threadIdx.x;
+      const double v = tl.inside(tl.gr(lr), tl.gc(lc)) ? (double)sb[lr * RW + lc] : 0.0;
+      er[o] = wave_sum(v * v);
+    }
+    static_assert(TX == kWave, "one tile row per wavefront");
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int o = 0; o < RT; ++o) sh[threadIdx.y + o * TYR] = er[o];
+    }
+    __syncthreads();
+    if (t == 0) {
+      double sblk = 0.0;
+#pragma unroll
+      for (int w = 0; w < TY; ++w) sblk += sh[w];
+      rr0[lt] = sblk;
+    }
+  }
   }
   if (!x0_ready) {  // block-uniform
     __syncthreads();
@@ -1690,15 +1709,18 @@ struct MGImpl {
       }
       XrFuse<T> xr{};
       if (l == 0 && fused && it > 0) xr = cg_update(it - 1);
+      // a start that rode on the right side's kernel leaves ||r0||^2 to the
+      // first fine down kernel (the stop test rides on a later kernel)
+      double* const rr0 = (l == 0 && it == 0 && fused && started) ? part.p + (size_t)S_RR0 * G0.NB : nullptr;
       const PcgTest dt = (l == 0 && !fused) ? test : none;
       // the timed launches are the fused ones when fusing (the byte count is theirs)
       if (l == 0 && (!fused || it > 0) && timer->active(KT_MG_SMOOTH0)) {
         xr.ran = timer->ran_slot();
         timer->ext_launch(k_mg_down<T, kDownRows<T>>, ctx->zg(tgrid(l)), dim3(TX, TY / kDownRows<T>), 0, s, view(l),
-                          view(l + 1), stop, dt, xr);
+                          view(l + 1), stop, dt, xr, rr0);
       } else
         k_mg_down<T, kDownRows<T>><<<ctx->zg(tgrid(l)), dim3(TX, TY / kDownRows<T>), 0, s>>>(view(l), view(l + 1),
-                                                                                           stop, dt, xr);
+                                                                                           stop, dt, xr, rr0);
     }
     int l = nGrid - 1;
     if (l >= 2 && pairOk && coarseUpOk) {  // the coarse solve and the first up pair in one launch
@@ -1728,6 +1750,7 @@ struct MGImpl {
   // flag every sync_every iterations.
   // keep / reset: see k_pcg_start; end = false leaves the end-of-loop stop
   // test to the caller (k_ir_check)
+  bool started = false;  // the coming solve's start rode on the right side's kernel (start_job)
   void pcg(const double* rhs, double abs_tol, double rtol, int maxit, int sync_every, const int* outer_stop,
            double* keep = nullptr, PcgScal* reset = nullptr, bool end = true, bool last = true) {
     hipStream_t s = ctx->stream;
@@ -1750,7 +1773,12 @@ struct MGImpl {
     abs_tol_ = abs_tol;
     rtol_ = rtol;
     b0cur = r0.p;
-    k_pcg_start<T, double><<<ctx->zg(grid0()), blk0(), 0, s>>>(G0, L, rhs, xs.p, r0.p, ps, outer_stop, part.p, keep, reset);
+    // started: r0, x, keep and the flags are set; the first fine down
+    // kernel forms the ||r0||^2 partials (fused path only)
+    CPS_CHECK(!started || (fused && maxit > 0 && !outer_stop), CPS_ERR_STATE,
+              "a solve started by the right side's kernel must run without host polling");
+    if (!started)
+      k_pcg_start<T, double><<<ctx->zg(grid0()), blk0(), 0, s>>>(G0, L, rhs, xs.p, r0.p, ps, outer_stop, part.p, keep, reset);
     // p_old of iteration 0 is pa: zero since creation, finite afterwards, and
     // multiplied by beta = 0
     for (int it = 0; it < maxit; ++it) {
@@ -1780,6 +1808,7 @@ struct MGImpl {
     if (fused && maxit > 0 && last)
       k_pcg_cg_last<T><<<ctx->zg(grid0()), blk0(), 0, s>>>(G0, view(0), cg_update(maxit - 1), rbuf(maxit - 1));
     if (end) k_pcg_end<<<ctx->zg(1), blk0(), 0, s>>>(G0, maxit, ps, part.p);
+    started = false;
   }
 };
 
@@ -1789,6 +1818,41 @@ template <class T>
 static MGImpl<T>* impl_of(void*& p, Ctx* ctx, const Geo& g) {
   if (!p) p = new MGImpl<T>(ctx, g);
   return static_cast<MGImpl<T>*>(p);
+}
+
+// Host polling happens only when the iteration caps are large (reference-
+// style runs); small caps launch a fixed kernel chain with device-side
+// termination (no host synchronisation, capturable in a graph).
+static constexpr int kNoSyncCap = 32;
+
+// The start of the coming solve for the right side's kernel: only where
+// the PCG runs without host polling (its first fine down kernel then forms
+// ||r0||^2), and for the first refinement sweep of the mixed form.
+void PressureMG::start_job(const cps_simple_config& cfg, mgc::StartJob<double>* j) {
+  *j = mgc::StartJob<double>{};
+  const int maxit = cfg.max_pressure_iter;
+  if (maxit <= 0 || maxit > kNoSyncCap) return;
+  auto* m = impl_of<double>(impl64_, ctx_, g_);
+  *j = mgc::StartJob<double>{m->r0.p, m->xs.p, nullptr, m->scal.p, nullptr, 1};
+  m->started = true;
+}
+
+void PressureMG::start_job(const cps_simple_config& cfg, mgc::StartJob<float>* j) {
+  *j = mgc::StartJob<float>{};
+  const int maxit = cfg.max_pressure_iter;
+  if (maxit <= 0 || maxit > kNoSyncCap) return;
+  auto* m = impl_of<float>(impl32_, ctx_, g_);
+  *j = mgc::StartJob<float>{m->r0.p, m->xs.p, b64(), m->scal.p, m->scal.p + 1, 1};
+  m->started = true;
+}
+
+double* PressureMG::b64() {
+  if (!b64_) {
+    auto* b = new DevBuf<double>;
+    b->alloc((size_t)(g_.M + 1) * g_.P);
+    b64_ = b;
+  }
+  return static_cast<DevBuf<double>*>(b64_)->p;
 }
 
 void PressureMG::prebuild(const double* dE, const double* dN, bool fp64, mgc::HierJob<float>* carry) {
@@ -1831,10 +1895,6 @@ PressureMG::~PressureMG() {
   delete static_cast<DevBuf<double>*>(b64_);
 }
 
-// Host polling happens only when the iteration caps are large (reference-
-// style runs); small caps launch a fixed kernel chain with device-side
-// termination (no host synchronisation, capturable in a graph).
-static constexpr int kNoSyncCap = 32;
 
 void PressureMG::solve(const double* dE, const double* dN, double* r, double* x, IterState* st,
                        const cps_simple_config& cfg, bool fp64, KernelTimer* timer, bool same_operator) {
@@ -1852,13 +1912,7 @@ void PressureMG::solve(const double* dE, const double* dN, double* r, double* x,
     k_copy_state<<<ctx_->zg(1), 64, 0, s>>>(st, m->scal.p, m->scal.p + 1, 0, ctx_->zs);
   } else {
     auto* m = impl_of<float>(impl32_, ctx_, g_);
-    const size_t S = (size_t)(g_.M + 1) * g_.P;
-    if (!b64_) {
-      auto* b = new DevBuf<double>;
-      b->alloc(S);
-      b64_ = b;
-    }
-    double* b64 = static_cast<DevBuf<double>*>(b64_)->p;
+    double* b64 = this->b64();
     m->timer = timer;
     if (m->prebuilt) m->prebuilt = false;
     else if (!same_operator || !m->built) m->setup(dE, dN);

@@ -35,6 +35,12 @@ class PressureMG {
   void prebuild(const double* dE, const double* dN, bool fp64, mgc::HierJob<float>* carry = nullptr);
   bool coarse_job(mgc::CoarseJob<double>* j, size_t* smem);
   bool coarse_job(mgc::CoarseJob<float>* j, size_t* smem);
+  // The start of the coming solve (r = b, x = 0, flags cleared, FP64 b
+  // kept), for the kernel that forms b; j->on = 0 where the solve must start
+  // itself (host polling).  The next solve in that precision then skips its
+  // start kernel.
+  void start_job(const cps_simple_config& cfg, mgc::StartJob<double>* j);
+  void start_job(const cps_simple_config& cfg, mgc::StartJob<float>* j);
   // Mixed solves with one refinement sweep leave their closing check (the
   // pressure statistics in IterState) until the host reads the statistics:
   // no later kernel of the step reads them.  Launches it if one is pending.
@@ -56,6 +62,7 @@ class PressureMG {
   void* impl64_ = nullptr;  // MGImpl<double>
   void* impl32_ = nullptr;  // MGImpl<float>
   void* b64_ = nullptr;     // FP64 rhs kept across refinement sweeps
+  double* b64();            // allocated on first use
 };
 
 }  // namespace cps

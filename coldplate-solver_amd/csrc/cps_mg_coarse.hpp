@@ -282,6 +282,28 @@ struct HierJob {
   int on;
 };
 
+// State of one multigrid PCG solve (inner) or of its refinement loop (outer).
+struct PcgScal {
+  double rzs[2], rr0, rr, tol2;
+  int stop, iters, pad_[2];
+  double alphas[2];  // step lengths of the single-reduction form, by iteration parity
+};
+
+// The start of the coming multigrid PCG solve (r = b in the inner type, x =
+// 0, the FP64 right side kept, the stop flags cleared), handed to the kernel
+// that forms b (k_prhs, which then leaves its FP64 b nowhere else); the
+// first V-cycle's fine down kernel adds the ||r0||^2 partials (on = 0: the
+// solve starts itself).
+template <class T>
+struct StartJob {
+  T* r;            // level-0 residual (field layout)
+  T* x;            // level-0 solution
+  double* keep;    // FP64 right side kept (nullptr: none)
+  PcgScal* ps;     // inner solve state
+  PcgScal* reset;  // outer refinement state (nullptr: none)
+  int on;
+};
+
 // The coarse set-up of the coming pressure solve, handed to a kernel that
 // carries it as one extra block (on = 0: none).
 template <class T>

@@ -824,12 +824,31 @@ __device__ inline double mass_b(const Geo& g, const Fields& f, int i, int j) {
 // PcgStart: also the Jacobi-PCG start (diagonal, z = r / diag, cleared
 // search directions, r.z).  The SOR and multigrid solves read only r, so
 // their steps skip those stores and the pressure coefficients.
-template <bool PcgStart>
-__global__ __launch_bounds__(NT) void k_prhs(Geo g, Fields f, PcgBufs q, double* part) {
+//
+// sj.on (multigrid solves without host polling): the right side goes
+// straight into the solve's start state instead of q.r: sj.r = (C) b,
+// sj.x = 0, sj.keep = b (FP64), and block 0 clears the solve's stop flags,
+// the work of the solve's own start kernel.  The level-0 layout of the
+// multigrid is the field layout, so DOF (i, j) is element k of both.
+template <bool PcgStart, class C>
+__global__ __launch_bounds__(NT) void k_prhs(Geo g, Fields f, PcgBufs q, double* part, mgc::StartJob<C> sj) {
   const size_t zo_ = zoff(g.zs);
   f.zshift(zo_); q.zshift(zo_); part = zsh(part, zo_);
   __shared__ double sh[NT / kWave + 1];
   const int M = g.M, N = g.N;
+  C* const sr = zsh(sj.r, zo_);
+  C* const sx = zsh(sj.x, zo_);
+  double* const skeep = zsh(sj.keep, zo_);
+  if (sj.on && block_id() == 0 && thread_id() == 0) {
+    mgc::PcgScal* ps = zsh(sj.ps, zo_);
+    ps->stop = 0;
+    ps->iters = 0;
+    if (sj.reset) {
+      mgc::PcgScal* rs = zsh(sj.reset, zo_);
+      rs->stop = 0;
+      rs->iters = 0;
+    }
+  }
   double mx = 0, ss = 0, rz = 0, bb = 0;
   FIELD_LOOP(g, i, j) {
     if (j > N) continue;
@@ -841,7 +860,13 @@ __global__ __launch_bounds__(NT) void k_prhs(Geo g, Fields f, PcgBufs q, double*
       mx = fmax(mx, ab);
       ss += ab * ab;
       if (j <= N - 2) {
-        q.r[k] = b;
+        if (sj.on) {
+          if (skeep) skeep[k] = b;
+          sr[k] = (C)b;
+          sx[k] = (C)0;
+        } else {
+          q.r[k] = b;
+        }
         bb += b * b;
         if (PcgStart) {
           PCoef c = p_coef(g, f.dE, f.dN, i, j);
@@ -1722,8 +1747,18 @@ class SimpleSolver {
     hipStream_t s = ctx->stream;
     const Fields f = fields();
     momentum();
-    if (cfg.pressure_solver == CPS_PRESSURE_PCG) k_prhs<true><<<ctx->zg(grid), block, 0, s>>>(g, f, pbufs(), part.p);
-    else k_prhs<false><<<ctx->zg(grid), block, 0, s>>>(g, f, pbufs(), part.p);
+    // the multigrid solve's start rides on the right side's kernel
+    if (cfg.pressure_solver == CPS_PRESSURE_PCG) {
+      k_prhs<true, double><<<ctx->zg(grid), block, 0, s>>>(g, f, pbufs(), part.p, mgc::StartJob<double>{});
+    } else if (mode_fp64) {
+      mgc::StartJob<double> sj{};
+      if (mg) mg->start_job(cfg, &sj);
+      k_prhs<false, double><<<ctx->zg(grid), block, 0, s>>>(g, f, pbufs(), part.p, sj);
+    } else {
+      mgc::StartJob<float> sj{};
+      if (mg) mg->start_job(cfg, &sj);
+      k_prhs<false, float><<<ctx->zg(grid), block, 0, s>>>(g, f, pbufs(), part.p, sj);
+    }
     // the Jacobi-PCG and SOR solves start from the statistics' r.z and
     // ||b||^2; the multigrid solve reads none of them, so there the
     // reductions wait until the host asks (stats())
