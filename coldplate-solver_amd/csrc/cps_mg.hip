@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 #include "cps_mg.hpp"
@@ -360,7 +361,7 @@ __device__ inline void tile_load(const Lv<T>& L, const Tile<T>& tl, int t, T* sx
       vb[i] = L.b[k];
       ve[i] = L.ex[k];
       vn[i] = L.ey[k];
-      if (sx) vx[i] = L.x[k] + Cl->y[Cl->at(r >> 1, c >> 1)];
+      if (sx) vx[i] = Cl ? L.x[k] + Cl->y[Cl->at(r >> 1, c >> 1)] : L.x[k];
     }
   }
 #pragma unroll
@@ -609,8 +610,9 @@ struct Region {
       dst[q] = inside(r, c) ? (T)kOmega1 * sb[q] / diag(q, r, c) : (T)0;
     }
   }
+  template <int NTR = FT>  // NTR threads share the region
   __device__ void sweep(const T* src, T* dst, int h, int t, T om) const {
-    for (int q = t; q < N; q += FT) {
+    for (int q = t; q < N; q += NTR) {
       const int lr = q / W, lc = q - (q / W) * W;
       if (lr < h || lr >= HH - h || lc < h || lc >= W - h) continue;
       const int r = gr(lr), c = gc(lc);
@@ -966,6 +968,178 @@ __global__ __launch_bounds__(FT) void k_mg_up2(Lv<T> A, Lv<T> B, Lv<T> C, const 
   A.zshift(zo_); B.zshift(zo_); C.zshift(zo_); stop = zsh(stop, zo_);
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   up2_body(A, B, C, stop, smem_raw, (const T*)nullptr, 0);
+}
+
+// The up legs of the last two mid levels and of the fine level in one launch
+// (L3 -> L2 -> L1 -> L0: the k_mg_up2 launch over L2, L1 and the fine
+// k_mg_up launch merged).  The block owns a 64 x 16 tile of level 0.  Its
+// prolongation reads L1's result on the tile's 32 x 8 footprint and a 2-cell
+// ring, which needs L1's two sweeps on a 4-cell ring (a 40 x 16 region from
+// L1's x1 and L2's result there), which needs L2's result on the 16 x 4
+// footprint and a 2-cell ring, i.e. L2's sweeps on a 4-cell ring (24 x 12,
+// from L2's x1 and L3's result in memory).  L2 and L1 use k_mg_up2's
+// arithmetic (Region), level 0 k_mg_up's, so every value, and the fine
+// level's result, partials and w, are bit-identical to the two launches;
+// no mid level's result goes to memory (only the next finer up leg reads
+// it).
+constexpr int U3H = 4;  // ring of the L1 and L2 regions
+using Up3B = Region<float, TX / 2, TY / 2, U3H>;
+using Up3C = Region<float, TX / 4, TY / 4, U3H>;
+constexpr int U3BN = Up3B::N, U3BW = Up3B::W, U3CN = Up3C::N, U3CW = Up3C::W;
+template <class T>
+__host__ __device__ constexpr size_t up3_smem() {
+  return (size_t)(6 * RN + 5 * U3BN + 5 * U3CN) * sizeof(T);
+}
+template <class T, int RT>
+__global__ __launch_bounds__(FT / RT) void k_mg_up3(Lv<T> L, Lv<T> B, Lv<T> C, Lv<T> D, double* part,
+                                                   const int* stop, T* w, double* part2) {
+  constexpr int NTH = FT / RT, TYR = TY / RT;
+  const size_t zo_ = zoff(L.zs);
+  L.zshift(zo_); B.zshift(zo_); C.zshift(zo_); D.zshift(zo_); part = zsh(part, zo_); stop = zsh(stop, zo_);
+  w = zsh(w, zo_); part2 = zsh(part2, zo_);
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  T* sb = reinterpret_cast<T*>(smem_raw);
+  T* se = sb + RN;
+  T* sn = se + RN;
+  T* s0 = sn + RN;
+  T* s1 = s0 + RN;
+  T* si = s1 + RN;
+  T* sbB = si + RN;
+  T* seB = sbB + U3BN;
+  T* snB = seB + U3BN;
+  T* s0B = snB + U3BN;
+  T* s1B = s0B + U3BN;
+  T* sbC = s1B + U3BN;
+  T* seC = sbC + U3CN;
+  T* snC = seC + U3CN;
+  T* s0C = snC + U3CN;
+  T* s1C = s0C + U3CN;
+  __shared__ int sflag;
+  const int t = threadIdx.x + threadIdx.y * TX;
+  if (*stop) return;  // uniform: nothing to load once the PCG has stopped
+  const int stop0 = t == 0 ? *stop : 0;
+  const int lt = xcd_tile();
+  Tile<T> tl{sb, se, sn, lt / (int)gridDim.x * TY, lt % (int)gridDim.x * TX, L.nx, L.ny};
+  const int rB0 = tl.r0 / 2, cB0 = tl.c0 / 2, rC0 = tl.r0 / 4, cC0 = tl.c0 / 4;
+  const Region<T, TX / 2, TY / 2, U3H> rb{sbB, seB, snB, rB0, cB0, B.nx, B.ny};
+  const Region<T, TX / 4, TY / 4, U3H> rc{sbC, seC, snC, rC0, cC0, C.nx, C.ny};
+  {  // every global load first: L2's b, faces and x1 + P (L3's result); L1's
+     // b, faces and x1; level 0's b, faces and x1 (its P e comes from LDS)
+    constexpr int NC = (U3CN + NTH - 1) / NTH, NB = (U3BN + NTH - 1) / NTH;
+    T cb[NC], ce[NC], cn[NC], cx[NC], wb[NB], we[NB], wn[NB], wx[NB];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      const int q = t + i * NTH;
+      const int lr = q / U3CW, lc = q - (q / U3CW) * U3CW;
+      const int r = rc.gr(lr), c = rc.gc(lc);
+      cb[i] = ce[i] = cn[i] = cx[i] = (T)0;
+      if (q < U3CN && rc.inside(r, c)) {
+        const size_t k = C.at(r, c);
+        cb[i] = C.b[k];
+        ce[i] = C.ex[k];
+        cn[i] = C.ey[k];
+        cx[i] = C.x[k] + D.y[D.at(r >> 1, c >> 1)];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int q = t + i * NTH;
+      const int lr = q / U3BW, lc = q - (q / U3BW) * U3BW;
+      const int r = rb.gr(lr), c = rb.gc(lc);
+      wb[i] = we[i] = wn[i] = wx[i] = (T)0;
+      if (q < U3BN && rb.inside(r, c)) {
+        const size_t k = B.at(r, c);
+        wb[i] = B.b[k];
+        we[i] = B.ex[k];
+        wn[i] = B.ey[k];
+        wx[i] = B.x[k];
+      }
+    }
+    tile_load<T, NTH>(L, tl, t, s0, nullptr);  // s0 = x1 (P e added below)
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      const int q = t + i * NTH;
+      if (q < U3CN) { sbC[q] = cb[i]; seC[q] = ce[i]; snC[q] = cn[i]; s0C[q] = cx[i]; }
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int q = t + i * NTH;
+      if (q < U3BN) { sbB[q] = wb[i]; seB[q] = we[i]; snB[q] = wn[i]; s0B[q] = wx[i]; }
+    }
+  }
+  if (t == 0) sflag = stop0;
+  __syncthreads();
+  if (sflag) return;  // block-uniform
+  // L2: two sweeps, the result on the 2-cell ring
+  rc.template sweep<NTH>(s0C, s1C, 1, t, (T)kOmega1);
+  __syncthreads();
+  rc.template sweep<NTH>(s1C, s0C, 2, t, (T)kOmega2);
+  __syncthreads();
+  // L1: x2 = x1 + P e on the region, two sweeps, the result on the 2-cell ring
+  for (int q = t; q < U3BN; q += NTH) {
+    const int lr = q / U3BW, lc = q - (q / U3BW) * U3BW;
+    const int r = rb.gr(lr), c = rb.gc(lc);
+    if (rb.inside(r, c)) s0B[q] = s0B[q] + s0C[((r >> 1) - rC0 + U3H) * U3CW + (c >> 1) - cC0 + U3H];
+  }
+  __syncthreads();
+  rb.template sweep<NTH>(s0B, s1B, 1, t, (T)kOmega1);
+  __syncthreads();
+  rb.template sweep<NTH>(s1B, s0B, 2, t, (T)kOmega2);
+  __syncthreads();
+  // level 0: x2 = x1 + P e on the tile region (cells in the level)
+  for (int q = t; q < RN; q += NTH) {
+    const int lr = q / RW, lc = q - (q / RW) * RW;
+    const int r = tl.gr(lr), c = tl.gc(lc);
+    if (tl.inside(r, c)) s0[q] = s0[q] + s0B[((r >> 1) - rB0 + U3H) * U3BW + (c >> 1) - cB0 + U3H];
+  }
+  const TileCells<T, NTH> tc(&tl, si, t);
+  __syncthreads();
+  if (w) {  // as k_mg_up's single-reduction branch
+    tc.template sweep<true>(s0, s1, 1, t, (T)kOmega1);  // halo-2 ring
+    __syncthreads();
+    tc.sweep(s1, s0, 2, t, (T)kOmega2);  // z on the halo-1 ring
+    __syncthreads();
+    double e = 0.0, e2 = 0.0;
+#pragma unroll
+    for (int o = 0; o < RT; ++o) {
+      const int lr = HL + threadIdx.y + o * TYR, lc = HL + threadIdx.x, q = lr * RW + lc;
+      const int r = tl.gr(lr), c = tl.gc(lc);
+      if (tl.inside(r, c)) {
+        const T z = s0[q];
+        const T wz = tc.ax(s0, q, tc.diag(q));
+        const size_t k = L.at(r, c);
+        st_nt(&L.y[k], z);
+        st_nt(&w[k], wz);
+        e += (double)sb[q] * (double)z;
+        e2 += (double)z * (double)wz;
+      }
+    }
+    __shared__ double sh2[2 * (NTH / kWave)];
+    block_sum2_t0<NTH>(e, e2, sh2);
+    if (t == 0) {
+      part[lt] = e;
+      part2[lt] = e2;
+    }
+    return;
+  }
+  tc.template sweep<true>(s0, s1, 2, t, (T)kOmega1);  // halo-1 ring
+  __syncthreads();
+  double e = 0.0;
+#pragma unroll
+  for (int o = 0; o < RT; ++o) {
+    const int lr = HL + threadIdx.y + o * TYR, lc = HL + threadIdx.x, q = lr * RW + lc;
+    const int r = tl.gr(lr), c = tl.gc(lc);
+    if (tl.inside(r, c)) {
+      const T z = ((T)1 - (T)kOmega2) * s1[q] + (T)kOmega2 * (sb[q] + tc.nb(s1, q)) * si[q];
+      st_nt(&L.y[L.at(r, c)], z);
+      e += (double)sb[q] * (double)z;
+    }
+  }
+  if (part) {
+    __shared__ double sh[NTH / kWave + 1];
+    const double s = block_sum<NTH>(e, sh);
+    if (t == 0) part[lt] = s;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1533,6 +1707,7 @@ struct MGImpl {
   size_t coarseSmem = 0, coarseSetupSmem = 0, coarseUpSmem = 0;
   bool pairOk = false;      // the two-level down kernel's LDS fits
   bool coarseUpOk = false;  // the coarse solve + first up pair kernel's LDS fits
+  bool up3Ok = false;       // the last mid pair's and the fine level's up legs in one launch (FP32)
   DevBuf<T> cf;  // coarse-solver faces and inverse diagonals
   DevBuf<T> r0, r1, xs, pa, pb, Ap;  // PCG vectors, level-0 layout (xs = solution; r0 / r1 residual by parity)
   DevBuf<T> sa, sb, wz;              // single-reduction form: s = A p by parity, w = A z
@@ -1603,6 +1778,11 @@ struct MGImpl {
     // in two launches (a validation switch: the results are bit-identical)
     const char* sep = std::getenv("CPS_MG_SEPARATE_COARSE");
     coarseUpOk = coarseUpSmem <= (size_t)lds && !(sep && sep[0] == '1');
+    // CPS_MG_SEPARATE_UP3=1 keeps them in two launches (a validation switch:
+    // the results are bit-identical).  FP64 keeps two launches: its merged
+    // kernel's LDS would leave one block per CU.
+    const char* sep3 = std::getenv("CPS_MG_SEPARATE_UP3");
+    up3Ok = std::is_same<T, float>::value && pairOk && up3_smem<T>() <= (size_t)lds && !(sep3 && sep3[0] == '1');
     cf.alloc((size_t)4 * off + (size_t)cd.dn * cd.dn);
     for (auto* b : {&lv[0]->x, &lv[0]->y, &lv[0]->ex, &lv[0]->ey, &r0, &r1, &xs, &pa, &pb, &Ap, &sa, &sb, &wz})
       CPS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), ctx->stream));
@@ -1732,6 +1912,12 @@ struct MGImpl {
                                                                    (fused && !pairTest) ? test : none);
     }
     for (; l >= 0; --l) {
+      if (l == 2 && up3Ok) {  // the last mid pair and the fine level in one launch
+        k_mg_up3<T, kUpRows<T>><<<ctx->zg(tgrid(0)), dim3(TX, TY / kUpRows<T>), up3_smem<T>(), s>>>(
+            view(0), view(1), view(2), view(3), part.p + (size_t)S_RZ * G0.NB, stop, fused ? wz.p : nullptr,
+            part.p + (size_t)S_ZW * G0.NB);
+        break;
+      }
       if (l >= 2 && pairOk) {  // two mid levels in one launch
         k_mg_up2<T><<<ctx->zg(tgrid(l - 1)), dim3(TX, TY), up_pair_smem<T>(), s>>>(view(l - 1), view(l), view(l + 1), stop);
         --l;

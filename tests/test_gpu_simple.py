@@ -408,3 +408,26 @@ def test_coarse_solve_fused_with_up_pair_is_bit_identical(pkg, nx, ny, precision
     for n in ("u", "v", "p"):
         assert np.array_equal(out[0][1][n], out[1][1][n]), n
     assert out[0][2] == out[1][2]
+
+
+@pytest.mark.parametrize("nx,ny", [(256, 128), (300, 150), (1024, 512)])
+def test_fine_up_leg_fused_with_last_mid_pair_is_bit_identical(pkg, nx, ny, monkeypatch):
+    """In the FP32-inner form the up legs of the last two mid levels and of
+    the fine level run as one launch (the mid levels recomputed on the fine
+    tile's footprint in LDS); with CPS_MG_SEPARATE_UP3=1 they stay two
+    launches.  The fields, the pressure drop and the iteration counts of 12
+    SIMPLE steps agree bit for bit."""
+    import bench
+    out = []
+    for sep in ("1", "0"):
+        monkeypatch.setenv("CPS_MG_SEPARATE_UP3", sep)
+        cfg, gamma, inlet = bench.perf_config(pkg, nx, ny, 32)
+        print(f"case {nx}x{ny} separate_up3={sep}", flush=True)
+        s = pkg.SIMPLE(cfg, gamma, inlet)
+        its = [s.step(1)["p_iters"] for _ in range(12)]
+        out.append((its, {n: s.get(n) for n in ("u", "v", "p")}, s.full_static_drop()))
+        s.close()
+    assert out[0][0] == out[1][0]
+    for n in ("u", "v", "p"):
+        assert np.array_equal(out[0][1][n], out[1][1][n]), n
+    assert out[0][2] == out[1][2]
