@@ -1840,17 +1840,15 @@ struct MGImpl {
       k_mg_coarsen<T><<<ctx->zg(lv[l]->grid()), blk(), 0, s>>>(view(l - 1), view(l));
     CPS_HIP(hipGetLastError());
   }
-  // the hier() launch's work for a kernel that carries it as extra blocks
-  // (hier_body on HJX x HJY tiles); none where deeper grid levels need
-  // their own launches after it
+  // the hier() launch's work for the assembly kernel, which forms dE, dN
+  // on the level-0 tiles; none where deeper grid levels need their own
+  // launches after it
   bool hier_job(HierJob<T>* j) {
     const int n = std::min((int)lv.size() - 1, kHierLevels);
     if (n + 1 < nGrid) return false;
     *j = HierJob<T>{};
     j->H.n = n;
     for (int l = 0; l <= n; ++l) j->H.l[l] = lv[l]->view();
-    j->hgx = ceil_div(lv[0]->nx, HJX);
-    j->hgy = ceil_div(lv[0]->ny, HJY);
     j->on = 1;
     return true;
   }
@@ -2041,20 +2039,28 @@ double* PressureMG::b64() {
   return static_cast<DevBuf<double>*>(b64_)->p;
 }
 
-void PressureMG::prebuild(const double* dE, const double* dN, bool fp64, mgc::HierJob<float>* carry) {
-  if (carry) carry->on = 0;
+bool PressureMG::prebuild(mgc::HierJob<double>* j) {
+  auto* m = impl_of<double>(impl64_, ctx_, g_);
+  if (!m->hier_job(j)) return false;
+  m->built = m->prebuilt = true;
+  return true;
+}
+
+bool PressureMG::prebuild(mgc::HierJob<float>* j) {
+  auto* m = impl_of<float>(impl32_, ctx_, g_);
+  if (!m->hier_job(j)) return false;
+  m->built = m->prebuilt = true;
+  return true;
+}
+
+void PressureMG::prebuild_launch(const double* dE, const double* dN, bool fp64) {
   if (fp64) {
     auto* m = impl_of<double>(impl64_, ctx_, g_);
     m->hier(dE, dN);
     m->built = m->prebuilt = true;
   } else {
     auto* m = impl_of<float>(impl32_, ctx_, g_);
-    if (carry && m->hier_job(carry)) {
-      carry->dE = dE;
-      carry->dN = dN;
-    } else {
-      m->hier(dE, dN);
-    }
+    m->hier(dE, dN);
     m->built = m->prebuilt = true;
   }
 }

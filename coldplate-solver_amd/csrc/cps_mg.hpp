@@ -24,15 +24,16 @@ class PressureMG {
   void solve(const double* dE, const double* dN, double* r, double* x, IterState* st, const cps_simple_config& cfg,
              bool fp64, KernelTimer* timer, bool same_operator = false);
   // Builds the coming solve's hierarchy ahead of it, once dE and dN are
-  // final: prebuild() launches the grid levels' faces, and coarse_job()
-  // hands the coarse solver's set-up to a kernel the caller launches next
-  // (one extra block beside that kernel's own work).  The next solve in that
-  // precision then builds nothing.  coarse_job() returns false when no
+  // final.  prebuild(j) hands the grid levels' faces to the kernel that
+  // forms dE, dN (the assembly, on the level-0 tiles; false where deeper
+  // levels need launches of their own), prebuild_launch() launches them;
+  // coarse_job() then hands the coarse solver's set-up to a later kernel
+  // (one extra block beside that kernel's own work).  The next solve in
+  // that precision builds nothing.  coarse_job() returns false when no
   // prebuilt hierarchy is pending.
-  // carry != nullptr (FP32-inner form): where it can, the grid levels' build
-  // is not launched but returned in *carry (on = 1) for the caller's next
-  // kernel to run as extra blocks.
-  void prebuild(const double* dE, const double* dN, bool fp64, mgc::HierJob<float>* carry = nullptr);
+  bool prebuild(mgc::HierJob<double>* j);
+  bool prebuild(mgc::HierJob<float>* j);
+  void prebuild_launch(const double* dE, const double* dN, bool fp64);
   bool coarse_job(mgc::CoarseJob<double>* j, size_t* smem);
   bool coarse_job(mgc::CoarseJob<float>* j, size_t* smem);
   // The start of the coming solve (r = b, x = 0, flags cleared, FP64 b

@@ -208,29 +208,14 @@ struct HierLevels {
   Lv<T> l[kHierLevels + 1];
   int n;  // coarser levels built here (<= kHierLevels)
 };
+// The coarser levels from the tile's level-0 faces in sEa, sNa (cells in the
+// level only, row pitch TW), the part of hier_body after level 0.
 template <class T, int NTH, int TW, int TH>
-__device__ __forceinline__ void hier_body(const Geo& g, const double* dE, const double* dN, const HierLevels<T>& H,
-                                          int bx, int by, int t, size_t zo, T* sEa, T* sNa, T* sEb, T* sNb) {
+__device__ __forceinline__ void hier_coarsen(const HierLevels<T>& H, int bx, int by, int t, size_t zo, T* sEa, T* sNa,
+                                             T* sEb, T* sNb) {
   static_assert(TW % (1 << kHierLevels) == 0 && TH % (1 << kHierLevels) == 0, "nested tiles");
-  const int M = g.M;
   int cx0 = bx * TW, cy0 = by * TH;  // tile origin at the current level
   int tw = TW, th = TH;              // tile size at the current level
-  {
-    const Lv<T>& L = H.l[0];
-    for (int q = t; q < TW * TH; q += NTH) {
-      const int lr = q / TW, lc = q - lr * TW, r = cy0 + lr, c = cx0 + lc;
-      if (r >= L.ny || c >= L.nx) continue;
-      const int i = r + 1, j = c + 1;
-      const size_t f = (size_t)i * g.P + j;
-      const double aE = g.rho * dE[f] * g.hy;
-      const double aN = (i < M - 1) ? g.rho * dN[f] * g.hx : 0.0;
-      const size_t k = L.at(r, c);
-      zsh(L.ex, zo)[k] = (T)aE;
-      zsh(L.ey, zo)[k] = (T)aN;
-      sEa[q] = (T)aE;
-      sNa[q] = (T)aN;
-    }
-  }
   for (int l = 1; l <= H.n; ++l) {
     __syncthreads();
     const Lv<T>& F = H.l[l - 1];
@@ -267,18 +252,34 @@ __device__ __forceinline__ void hier_body(const Geo& g, const double* dE, const 
     }
   }
 }
+template <class T, int NTH, int TW, int TH>
+__device__ __forceinline__ void hier_body(const Geo& g, const double* dE, const double* dN, const HierLevels<T>& H,
+                                          int bx, int by, int t, size_t zo, T* sEa, T* sNa, T* sEb, T* sNb) {
+  const int M = g.M;
+  const int cx0 = bx * TW, cy0 = by * TH;
+  {
+    const Lv<T>& L = H.l[0];
+    for (int q = t; q < TW * TH; q += NTH) {
+      const int lr = q / TW, lc = q - lr * TW, r = cy0 + lr, c = cx0 + lc;
+      if (r >= L.ny || c >= L.nx) continue;
+      const int i = r + 1, j = c + 1;
+      const size_t f = (size_t)i * g.P + j;
+      const double aE = g.rho * dE[f] * g.hy;
+      const double aN = (i < M - 1) ? g.rho * dN[f] * g.hx : 0.0;
+      const size_t k = L.at(r, c);
+      zsh(L.ex, zo)[k] = (T)aE;
+      zsh(L.ey, zo)[k] = (T)aN;
+      sEa[q] = (T)aE;
+      sNa[q] = (T)aN;
+    }
+  }
+  hier_coarsen<T, NTH, TW, TH>(H, bx, by, t, zo, sEa, sNa, sEb, sNb);
+}
 
-// The hierarchy build handed to the first paired momentum Jacobi launch:
-// HJX x HJY level-0 tiles on blocks of 512 threads, in extra columns of
-// its grid (on = 0: none).
-constexpr int HJX = 64, HJY = 32, HJT = 512;
+// The hierarchy build handed to the kernel that forms dE, dN (on = 0: none).
 template <class T>
 struct HierJob {
   HierLevels<T> H;
-  const double* dE;
-  const double* dN;
-  int gxm;       // the launch's own grid width; blocks at x >= gxm are hierarchy blocks
-  int hgx, hgy;  // hierarchy tiles
   int on;
 };
 

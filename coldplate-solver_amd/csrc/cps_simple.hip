@@ -282,8 +282,15 @@ __device__ inline void store_mom(const Geo& g, const MomCoef<IsU>& c, const doub
 
 // Launched on the active-cell grid (tiles from cell (1, 1), one cell per
 // thread, see active_grid): every U and V cell lies in it.
+//
+// hj.on: the block's tile is also a 64 x 16 tile of the pressure multigrid's
+// level 0 (DOF (i, j) = level-0 cell (i-1, j-1)), whose faces come from the
+// dE, dN just formed here, so the block also builds its tile of level 0 and
+// the nested tiles of the next hj.H.n levels (hier_body's arithmetic; the
+// 2x2 aggregation is local to 16-aligned tiles).
 template <class C, bool Delta>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_assemble(Geo g, Fields f, MomSys<C> su, MomSys<C> sv, IterState* st) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_assemble(Geo g, Fields f, MomSys<C> su, MomSys<C> sv, IterState* st,
+                                                                                         mgc::HierJob<C> hj) {
   const size_t zo_ = zoff(g.zs);
   f.zshift(zo_); su.zshift(zo_); sv.zshift(zo_); st = zsh(st, zo_);
   const int lt = xcd_tile();  // tiles in XCD-aware order (the SOU stencil reads two-cell halos)
@@ -296,18 +303,36 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
     st->jac_iters[0] = 0;
     st->jac_iters[1] = 0;
   }
-  {
-    const int i = 1 + lt / (int)gridDim.x * BY + (int)threadIdx.y;
-    if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) {
-      MomCoef<true> c = mom_coef<true>(g, f, i, j);
-      f.dE[at(g, i, j)] = g.hy / c.aP0;
-      store_mom<true, C, Delta>(g, c, f.u, su, i, j);
+  const int i = 1 + lt / (int)gridDim.x * BY + (int)threadIdx.y;
+  double dEv = 0.0, dNv = 0.0;
+  if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) {
+    MomCoef<true> c = mom_coef<true>(g, f, i, j);
+    dEv = g.hy / c.aP0;
+    f.dE[at(g, i, j)] = dEv;
+    store_mom<true, C, Delta>(g, c, f.u, su, i, j);
+  }
+  if (i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) {
+    MomCoef<false> c = mom_coef<false>(g, f, i, j);
+    dNv = g.hx / c.aP0;
+    f.dN[at(g, i, j)] = dNv;
+    store_mom<false, C, Delta>(g, c, f.v, sv, i, j);
+  }
+  if (hj.on) {  // kernel-uniform
+    constexpr int QN = (BX / 2) * (BY / 2);
+    __shared__ C sEa[NT], sNa[NT], sEb[QN], sNb[QN];
+    const int t = (int)threadIdx.x + (int)threadIdx.y * BX;
+    const mgc::Lv<C>& L = hj.H.l[0];
+    const int r = i - 1, c = j - 1;
+    if (r < L.ny && c < L.nx) {  // level 0: aE = rho dE hy, aN = rho dN hx (0 on the last row)
+      const double aE = g.rho * dEv * g.hy;
+      const double aN = (i < M - 1) ? g.rho * dNv * g.hx : 0.0;
+      const size_t k = L.at(r, c);
+      zsh(L.ex, zo_)[k] = (C)aE;
+      zsh(L.ey, zo_)[k] = (C)aN;
+      sEa[t] = (C)aE;
+      sNa[t] = (C)aN;
     }
-    if (i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) {
-      MomCoef<false> c = mom_coef<false>(g, f, i, j);
-      f.dN[at(g, i, j)] = g.hx / c.aP0;
-      store_mom<false, C, Delta>(g, c, f.v, sv, i, j);
-    }
+    mgc::hier_coarsen<C, NT, BX, BY>(hj.H, lt % (int)gridDim.x, lt / (int)gridDim.x, t, zo_, sEa, sNa, sEb, sNb);
   }
 }
 
@@ -471,50 +496,24 @@ __device__ __forceinline__ C jac_cell(const C* X, int W, int q, const C* c, C& r
   return X[q] + r / c[2];
 }
 
-//
-// hj.on: the grid has extra columns of blocks (blockIdx.x >= hj.gxm) that
-// build the pressure multigrid's grid levels from dE, dN (hier_body on
-// HJX x HJY tiles) beside the sweeps, which touch none of that data; the
-// sweep tiles keep the block index bx + by gxm, so their tile order and
-// partials are those of the plain grid.
 template <class C, int RT>
 __global__ __launch_bounds__(NT / RT)
-void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> sv, int k, int two, double tol2, IterState* st, double* part,
-               mgc::HierJob<C> hj) {
+void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> sv, int k, int two, double tol2, IterState* st, double* part) {
   constexpr int NTH = NT / RT, BYR = BY / RT, KP = 2 * NT / NTH;  // KP: partials per thread (<= 2048 tiles)
   static_assert(JRING <= NTH, "one ring cell per thread");
   const size_t zo_ = zoff(g.zs);
-  // LDS: the sweeps' x_k (U and V) and x_(k+1), or a hierarchy block's faces
-  constexpr size_t kJacLds = sizeof(C) * 2 * (JR2N + JR1N);
-  constexpr size_t kHierLds = sizeof(C) * (2 * mgc::HJX * mgc::HJY + 2 * (mgc::HJX / 2) * (mgc::HJY / 2));
-  __shared__ __attribute__((aligned(16))) unsigned char jsm[kJacLds > kHierLds ? kJacLds : kHierLds];
-  const int gxm = hj.on ? hj.gxm : (int)gridDim.x;
-  if ((int)blockIdx.x >= gxm) {  // block-uniform
-    static_assert(mgc::HJT == NTH, "hierarchy blocks have the sweeps' block size");
-    const int sb = (int)blockIdx.x - gxm + ((int)gridDim.x - gxm) * (int)blockIdx.y;
-    if (sb < hj.hgx * hj.hgy) {
-      C* sEa = reinterpret_cast<C*>(jsm);
-      C* sNa = sEa + mgc::HJX * mgc::HJY;
-      C* sEb = sNa + mgc::HJX * mgc::HJY;
-      C* sNb = sEb + (mgc::HJX / 2) * (mgc::HJY / 2);
-      mgc::hier_body<C, NTH, mgc::HJX, mgc::HJY>(g, zsh(hj.dE, zo_), zsh(hj.dN, zo_), hj.H, sb % hj.hgx, sb / hj.hgx,
-                                                  thread_id(), zo_, sEa, sNa, sEb, sNb);
-    }
-    return;
-  }
   su.zshift(zo_); sv.zshift(zo_); st = zsh(st, zo_); part = zsh(part, zo_);
-  C (*xa)[JR2N] = reinterpret_cast<C (*)[JR2N]>(jsm);                      // x_k, U and V
-  C (*xb)[JR1N] = reinterpret_cast<C (*)[JR1N]>(jsm + sizeof(C) * 2 * JR2N);  // x_(k+1)
+  __shared__ C xa[2][JR2N];  // x_k, U and V
+  __shared__ C xb[2][JR1N];  // x_(k+1)
   __shared__ double sh[NTH / kWave + 1];
   __shared__ int sfl[2];
   if (!st->jac_active[0] && !st->jac_active[1]) return;  // as in k_jacobi
   const int t = thread_id();
-  const int bk = (int)blockIdx.x + (int)blockIdx.y * gxm;  // the block's index in the sweep grid
   const int f0 = t == 0 ? st->jac_active[0] : 0, f1 = t == 0 ? st->jac_active[1] : 0;
   const bool tst = k >= 2;
   const double* p2 = part + (size_t)(SLOT_JAC4 + 2 * ((k + 2) & 3)) * g.NB;  // sweep k-2
   const double* p1 = part + (size_t)(SLOT_JAC4 + 2 * ((k + 3) & 3)) * g.NB;  // sweep k-1
-  const int nb = gxm * (int)gridDim.y;  // partials per slot (slot stride g.NB >= nb)
+  const int nb = (int)(gridDim.x * gridDim.y);  // partials per slot (slot stride g.NB >= nb)
   const double qu2 = tst ? parts_load_k<NTH, KP>(p2, nb) : 0.0, qv2 = tst ? parts_load_k<NTH, KP>(p2 + g.NB, nb) : 0.0;
   const double qu1 = tst ? parts_load_k<NTH, KP>(p1, nb) : 0.0, qv1 = tst ? parts_load_k<NTH, KP>(p1 + g.NB, nb) : 0.0;
   // the flags as thread 0 found them, uniform over the block (a block that
@@ -528,7 +527,7 @@ void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> sv, int k, int two, double tol2, I
   // before any tile load: once the solve has converged (in most SIMPLE steps
   // after the first pair) the launch loads nothing
   if (tst) {
-    const bool b0 = bk == 0 && t == 0;
+    const bool b0 = block_id() == 0 && t == 0;
     if (au0) {
       if (parts_total<NTH>(qu2, sh) < tol2 * (double)g.nU) {
         au0 = 0;
@@ -550,8 +549,8 @@ void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> sv, int k, int two, double tol2, I
     if (!au0 && !av0) return;  // uniform over the grid
   }
   const int M = g.M, N = g.N, P = g.P;
-  const int lt = xcd_tile_of(bk, nb);
-  const int i0 = 1 + lt / gxm * BY, j0 = 1 + lt % gxm * BX;
+  const int lt = xcd_tile();
+  const int i0 = 1 + lt / (int)gridDim.x * BY, j0 = 1 + lt % (int)gridDim.x * BX;
   auto actU = [&](int i, int j) { return i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2; };
   auto actV = [&](int i, int j) { return i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1; };
   const C* xu = jac_buf(su, k);
@@ -1616,8 +1615,19 @@ class SimpleSolver {
     const double tol2 = cfg.momentum_tol * cfg.momentum_tol;
     const int maxit = cfg.max_momentum_iter;
     const int chunk = maxit > kNoSyncCap ? 8 : std::max(1, maxit);
-    if (mode_fp64) k_assemble<double, false><<<ctx->zg(active_grid()), block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p);
-    else k_assemble<float, true><<<ctx->zg(active_grid()), block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p);
+    // the multigrid pressure solve's hierarchy is built as soon as dE, dN
+    // are final: its grid levels by the assembly's own blocks (else one
+    // launch after it), the coarse set-up by k_finalize
+    const bool mgpre = mg && cfg.pressure_solver == CPS_PRESSURE_GMG;
+    if (mode_fp64) {
+      mgc::HierJob<double> hj{};
+      if (mgpre && !mg->prebuild(&hj)) mg->prebuild_launch(dE.p, dN.p, true);
+      k_assemble<double, false><<<ctx->zg(active_grid()), block, 0, s>>>(g, f, sys(momD, 0), sys(momD, 1), st.p, hj);
+    } else {
+      mgc::HierJob<float> hj{};
+      if (mgpre && !mg->prebuild(&hj)) mg->prebuild_launch(dE.p, dN.p, false);
+      k_assemble<float, true><<<ctx->zg(active_grid()), block, 0, s>>>(g, f, sys(momF, 0), sys(momF, 1), st.p, hj);
+    }
     CPS_HIP(hipGetLastError());
     int k = 0;
     // two sweeps per launch on the unpolled delta path (the timed single
@@ -1626,24 +1636,10 @@ class SimpleSolver {
     const dim3 jgrid = active_grid();
     const bool pairs = !mode_fp64 && maxit > 0 && chunk == maxit && timer.which != KT_MOM_JACOBI &&
                        jgrid.x > 0 && jgrid.y > 0 && (int)(jgrid.x * jgrid.y) <= g.NB;
-    // dE, dN are final: the multigrid pressure solve's hierarchy is built
-    // now.  Its grid levels ride on the first paired sweep launch as extra
-    // columns of blocks (else one launch here), the coarse set-up on
-    // k_finalize.
-    mgc::HierJob<float> hj{};
-    if (mg && cfg.pressure_solver == CPS_PRESSURE_GMG) mg->prebuild(dE.p, dN.p, mode_fp64 != 0, pairs ? &hj : nullptr);
     if (pairs) {
-      for (; k < maxit; k += 2) {
-        dim3 jg = jgrid;
-        mgc::HierJob<float> hk{};
-        if (k == 0 && hj.on) {
-          hk = hj;
-          hk.gxm = (int)jgrid.x;
-          jg.x += ceil_div(hj.hgx * hj.hgy, (int)jgrid.y);
-        }
-        k_jacobi2<float, kJacRows><<<ctx->zg(jg), dim3(BX, BY / kJacRows), 0, s>>>(
-            g, sys(momF, 0), sys(momF, 1), k, k + 1 < maxit ? 1 : 0, tol2, st.p, part.p, hk);
-      }
+      for (; k < maxit; k += 2)
+        k_jacobi2<float, kJacRows><<<ctx->zg(jgrid), dim3(BX, BY / kJacRows), 0, s>>>(
+            g, sys(momF, 0), sys(momF, 1), k, k + 1 < maxit ? 1 : 0, tol2, st.p, part.p);
       CPS_HIP(hipGetLastError());
     }
     while (!pairs && k < maxit) {
