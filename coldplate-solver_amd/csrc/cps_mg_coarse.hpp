@@ -151,10 +151,12 @@ __device__ __forceinline__ void coarse_setup_body(const Lv<T>& top, const Lv<T>&
   // Inverse of the last level's operator (the faces and floored diagonal the
   // V-cycle applies) by Gauss-Jordan in FP64 over the whole block: thread t
   // < n^2 holds entry (i, j) = (t / n, t mod n) in a register; each pivot's
-  // row and column travel through LDS (double-buffered, one barrier per
-  // pivot).  The operator is a symmetric M-matrix, so no pivoting is needed.
-  // Entry (i, j) is built from cell j's faces and updated as
-  // a_ij <- (j == p ? 0 : a_ij) - a_ip (j == p ? 1 : a_pj) / a_pp, row p <- (j == p ? 1 : a_pj) / a_pp.
+  // scaled row and its column travel through LDS (double-buffered, one
+  // barrier per pivot).  The operator is a symmetric M-matrix, so no
+  // pivoting is needed.  Entry (i, j) is built from cell j's faces and
+  // updated as m_j = (j == p ? 1 : a_pj) / a_pp (formed by row p's threads,
+  // which share a wavefront and take a_pp by a shuffle), row p <- m_j,
+  // a_ij <- (j == p ? 0 : a_ij) - a_ip m_j.
   __shared__ double gj_row[2][kDirectCells], gj_col[2][kDirectCells];
   __syncthreads();
   {
@@ -176,15 +178,27 @@ __device__ __forceinline__ void coarse_setup_body(const Lv<T>& top, const Lv<T>&
       else if (gi == gj + nx) a = nn;
       else if (gi == gj - nx) a = ss;
     }
-    for (int p = 0; p < n; ++p) {
-      const int b = p & 1;
-      if (mine && gi == p) gj_row[b][gj] = a;
-      if (mine && gj == p) gj_col[b][gi] = a;
-      __syncthreads();
-      if (mine) {
-        const double inv = 1.0 / gj_row[b][p];
-        const double mp = (gj == p ? 1.0 : gj_row[b][gj]) * inv;
-        a = gi == p ? mp : (gj == p ? 0.0 : a) - gj_col[b][gi] * mp;
+    if (kWave % n == 0) {  // block-uniform: every row of the matrix lies in one wavefront
+      for (int p = 0; p < n; ++p) {
+        const int b = p & 1;
+        // a_pp from thread (p, p) of the same wavefront (every lane of the
+        // wavefront takes part in the shuffle)
+        const double app = __shfl(a, (p * n + p) & (kWave - 1));
+        if (mine && gi == p) gj_row[b][gj] = (gj == p ? 1.0 : a) * (1.0 / app);
+        if (mine && gj == p) gj_col[b][gi] = a;
+        __syncthreads();
+        if (mine) a = gi == p ? gj_row[b][gj] : (gj == p ? 0.0 : a) - gj_col[b][gi] * gj_row[b][gj];
+      }
+    } else {  // the same values, a_pp through LDS
+      for (int p = 0; p < n; ++p) {
+        const int b = p & 1;
+        if (mine && gi == p) gj_row[b][gj] = a;
+        if (mine && gj == p) gj_col[b][gi] = a;
+        __syncthreads();
+        if (mine) {
+          const double mp = (gj == p ? 1.0 : gj_row[b][gj]) * (1.0 / gj_row[b][p]);
+          a = gi == p ? mp : (gj == p ? 0.0 : a) - gj_col[b][gi] * mp;
+        }
       }
     }
     if (mine) cf[4 * d.total + gj * n + gi] = (T)a;
