@@ -1780,9 +1780,13 @@ struct MGImpl {
     coarseUpOk = coarseUpSmem <= (size_t)lds && !(sep && sep[0] == '1');
     // CPS_MG_SEPARATE_UP3=1 keeps them in two launches (a validation switch:
     // the results are bit-identical).  FP64 keeps two launches: its merged
-    // kernel's LDS would leave one block per CU.
+    // kernel's LDS would leave one block per CU.  So do the fields of a
+    // batch (ctx->zs != 0): there the GPU is busy with many fields' blocks,
+    // and the merged kernel's recomputed mid-level rings cost more than the
+    // launch they save (48-field sweep 4.48e9 against 4.31e9 cell-updates/s).
     const char* sep3 = std::getenv("CPS_MG_SEPARATE_UP3");
-    up3Ok = std::is_same<T, float>::value && pairOk && up3_smem<T>() <= (size_t)lds && !(sep3 && sep3[0] == '1');
+    up3Ok = std::is_same<T, float>::value && ctx->zs == 0 && pairOk && up3_smem<T>() <= (size_t)lds &&
+            !(sep3 && sep3[0] == '1');
     cf.alloc((size_t)4 * off + (size_t)cd.dn * cd.dn);
     for (auto* b : {&lv[0]->x, &lv[0]->y, &lv[0]->ex, &lv[0]->ey, &r0, &r1, &xs, &pa, &pb, &Ap, &sa, &sb, &wz})
       CPS_HIP(hipMemsetAsync(b->p, 0, b->bytes(), ctx->stream));

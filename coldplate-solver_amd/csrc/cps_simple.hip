@@ -2260,25 +2260,31 @@ class BatchSolver {
       for (int ns : {1, 2, 4, 8, 16, 24, 32})
         if (ns <= (int)f.size()) cand.push_back({kPerField, ns, 1});
     }
-    size_t best = 0;
-    double best_us = 1e300;
-    for (size_t k = 0; k < cand.size(); ++k) {
-      const Cand& c = cand[k];
-      capture(c.p, c.ns, c.un);
-      const int r = std::max(reps, c.un);
-      launch(c.un);  // warm-up
-      sync_all();
-      const auto t0 = std::chrono::steady_clock::now();
-      launch(r);
-      sync_all();
-      const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() / r;
-      if (us < best_us) {
-        best_us = us;
-        best = k;
+    // The fields keep iterating while the candidates are timed, and their
+    // steps get cheaper as they converge (fewer live multigrid cycles), so
+    // a single pass favours the candidates timed last.  Two passes, the
+    // second in reverse order, cancel that drift to first order: each
+    // candidate's score is the sum of its two timings.
+    std::vector<double> us(cand.size(), 0.0);
+    const size_t nc = cand.size();
+    for (size_t pass = 0; pass < (nc > 1 ? 2u : 1u); ++pass)
+      for (size_t kk = 0; kk < nc; ++kk) {
+        const size_t k = pass == 0 ? kk : nc - 1 - kk;
+        const Cand& c = cand[k];
+        capture(c.p, c.ns, c.un);
+        const int r = std::max(reps, c.un);
+        launch(c.un);  // warm-up
+        sync_all();
+        const auto t0 = std::chrono::steady_clock::now();
+        launch(r);
+        sync_all();
+        us[k] += std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() / r;
       }
-    }
-    if (best + 1 != cand.size()) capture(cand[best].p, cand[best].ns, cand[best].un);
-    us_per_step = best_us;
+    size_t best = 0;
+    for (size_t k = 1; k < nc; ++k)
+      if (us[k] < us[best]) best = k;
+    capture(cand[best].p, cand[best].ns, cand[best].un);
+    us_per_step = us[best] / (nc > 1 ? 2.0 : 1.0);
   }
 };
 
