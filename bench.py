@@ -393,16 +393,17 @@ def split_early(values, ratio=2.0):
     return high, len(low)
 
 
-def fused_fine_down_values(rows, kernel="k_mg_down<float, 2>", marker="k_pcg_start<float"):
+def fused_fine_down_values(rows, kernel="k_mg_down<float, 2>", markers=("k_pcg_start<float", "k_prhs<false, float>")):
     """rows: (order key, short kernel name, value) of every dispatch.  The
     values of the roofline kernel's fused launches: dispatches of `kernel`
-    that do not directly follow a `marker` dispatch (the first V-cycle of
-    each pressure solve runs the unfused form), early exits left out
-    (split_early).  Returns (values, number of early exits, number of
-    unfused first launches)."""
+    that do not directly follow a dispatch of one of `markers` (the kernel
+    that starts a pressure solve: its own start kernel, or the right-side
+    kernel that carries the start; the first V-cycle of each solve runs the
+    unfused form), early exits left out (split_early).  Returns (values,
+    number of early exits, number of unfused first launches)."""
     fused, first, after = [], 0, False
     for _, k, v in sorted(rows):
-        if k.startswith(marker):
+        if k.startswith(markers):
             after = True
         elif k == kernel:
             if after:

@@ -247,6 +247,30 @@ __device__ inline void block_sum2_all(double& a, double& b, double* sh2) {
   b = rb;
 }
 
+// M sums over the block in block_sum's order (wave sums, then the
+// wavefronts in order), delivered to every thread after one barrier.  sh
+// holds M * NT / kWave entries and must not be written again before a later
+// barrier.  Equal, bit for bit, to parts_total / block_sum of each value.
+template <int NT, int M>
+__device__ inline void block_sumn_all(double (&v)[M], double* sh) {
+  constexpr int NW = NT / kWave;
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+#pragma unroll
+  for (int m = 0; m < M; ++m) v[m] = wave_sum(v[m]);
+  if ((t & (kWave - 1)) == 0) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) sh[m * NW + t / kWave] = v[m];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    double r = 0.0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) r += sh[m * NW + k];
+    v[m] = r;
+  }
+}
+
 // block_sum2_all's two sums (same order, same bits) in thread 0 only, with
 // no trailing barrier.
 template <int NT>

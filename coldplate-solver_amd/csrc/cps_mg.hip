@@ -222,7 +222,9 @@ constexpr int FT = TX * TY;
 // has converged; `lead` then has recorded the state and set the stop flag.
 // At it == 0 the partials hold ||r0||^2 and the target is set here.
 __device__ inline bool pcg_converged(const PcgTest& test, double perr, bool lead, double* shr) {
-  const double err = parts_total<FT>(perr, shr);
+  double e1[1] = {perr};
+  block_sumn_all<FT, 1>(e1, shr);  // parts_total's sum, one barrier; shr is not written again
+  const double err = e1[0];
   double tol2;
   if (test.it == 0) {
     const double tgt = fmax(test.abs_tol, test.rtol * sqrt(err));

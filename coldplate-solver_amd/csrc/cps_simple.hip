@@ -528,20 +528,24 @@ void k_jacobi2(Geo g, MomSys<C> su, MomSys<C> sv, int k, int two, double tol2, I
   // after the first pair) the launch loads nothing
   if (tst) {
     const bool b0 = block_id() == 0 && t == 0;
+    // the four sums in one reduction (one barrier), each equal to parts_total's
+    __shared__ double sh4[4 * (NTH / kWave)];
+    double q4[4] = {qu2, qu1, qv2, qv1};
+    block_sumn_all<NTH, 4>(q4, sh4);
     if (au0) {
-      if (parts_total<NTH>(qu2, sh) < tol2 * (double)g.nU) {
+      if (q4[0] < tol2 * (double)g.nU) {
         au0 = 0;
         if (b0) { st->jac_active[0] = 0; st->jac_iters[0] = k - 1; }
-      } else if (parts_total<NTH>(qu1, sh) < tol2 * (double)g.nU) {
+      } else if (q4[1] < tol2 * (double)g.nU) {
         au0 = 0;
         if (b0) { st->jac_active[0] = 0; st->jac_iters[0] = k; }
       }
     }
     if (av0) {
-      if (parts_total<NTH>(qv2, sh) < tol2 * (double)g.nV) {
+      if (q4[2] < tol2 * (double)g.nV) {
         av0 = 0;
         if (b0) { st->jac_active[1] = 0; st->jac_iters[1] = k - 1; }
-      } else if (parts_total<NTH>(qv1, sh) < tol2 * (double)g.nV) {
+      } else if (q4[3] < tol2 * (double)g.nV) {
         av0 = 0;
         if (b0) { st->jac_active[1] = 0; st->jac_iters[1] = k; }
       }
@@ -744,9 +748,11 @@ __global__ __launch_bounds__(NT) void k_finalize(Geo g, Fields f, MomSys<C> su, 
     const int au = st->jac_active[0], av = st->jac_active[1];
     if (pairs && K >= 2 && !(K & 1)) {  // jacobi_solver.cpp:63 on sweep K-2
       const double* p2 = part + (size_t)(SLOT_JAC4 + 2 * ((K + 2) & 3)) * g.NB;
-      const double qu = au ? parts_load<NT>(p2, pairs) : 0.0, qv = av ? parts_load<NT>(p2 + g.NB, pairs) : 0.0;
-      if (au) itU = parts_total<NT>(qu, sh) < tol2 * (double)g.nU ? K - 1 : K;
-      if (av) itV = parts_total<NT>(qv, sh) < tol2 * (double)g.nV ? K - 1 : K;
+      double q2[2] = {au ? parts_load<NT>(p2, pairs) : 0.0, av ? parts_load<NT>(p2 + g.NB, pairs) : 0.0};
+      __shared__ double sh2[2 * (NT / kWave)];  // not sh: block_sum below writes it with no barrier between
+      block_sumn_all<NT, 2>(q2, sh2);
+      if (au) itU = q2[0] < tol2 * (double)g.nU ? K - 1 : K;
+      if (av) itV = q2[1] < tol2 * (double)g.nV ? K - 1 : K;
     } else {
       if (au) itU = K;
       if (av) itV = K;

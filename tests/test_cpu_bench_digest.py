@@ -37,6 +37,10 @@ def test_fused_launches_skip_the_first_after_the_marker():
             t += 1
     work, nearly, first = bench.fused_fine_down_values(seq)
     assert first == 3 and nearly == 6 and sorted(work) == [11.0] * 3 + [12.0] * 3
+    # a solve whose start rides on the right-side kernel: that kernel marks the first launch
+    seq = [(t, "k_prhs<false, float>" if k == M else k, v) for t, k, v in seq]
+    work, nearly, first = bench.fused_fine_down_values(seq)
+    assert first == 3 and nearly == 6 and sorted(work) == [11.0] * 3 + [12.0] * 3
 
 
 def _write(path, header, rows):
