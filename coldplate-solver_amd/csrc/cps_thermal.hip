@@ -33,6 +33,7 @@
 #include <cstring>
 #include <memory>
 #include <type_traits>
+#include <cstdlib>
 #include <vector>
 
 #include "cps_simple.hpp"
@@ -523,6 +524,32 @@ __device__ inline void coarsest_col(const Lvl<T>& L) {
     L.y[L.A.at(0, 0, k)] = (T)xk;
   }
 }
+// coarsest_col for columns of at most NZ layers: the same expressions with
+// the loops unrolled, so the factors stay in registers (the kMaxNz arrays
+// above live in scratch memory)
+template <class T, int NZ>
+__device__ inline void coarsest_col_nz(const Lvl<T>& L) {
+  const int nz = L.A.nz;
+  double c[NZ], g[NZ];
+#pragma unroll
+  for (int k = 0; k < NZ; ++k) {
+    if (k < nz) {
+      const size_t q = L.A.at(0, 0, k);
+      const double lo = k > 0 ? (double)L.A.b[q] : 0.0, up = k + 1 < nz ? (double)L.A.t[q] : 0.0;
+      const double den = (double)L.A.d[q] - (k > 0 ? lo * c[k - 1] : 0.0);
+      c[k] = up / den;
+      g[k] = ((double)L.rhs[q] - (k > 0 ? lo * g[k - 1] : 0.0)) / den;
+    }
+  }
+  double xk = 0.0;
+#pragma unroll
+  for (int k = NZ - 1; k >= 0; --k) {
+    if (k < nz) {
+      xk = k == nz - 1 ? g[k] : g[k] - c[k] * xk;
+      L.y[L.A.at(0, 0, k)] = (T)xk;
+    }
+  }
+}
 template <class T>
 __global__ void k_coarsest(Lvl<T> L) {
   if (threadIdx.x != 0 || L.stopped()) return;
@@ -566,7 +593,7 @@ __global__ __launch_bounds__(TNT) void k_th_coarse(ThCoarse<T> H) {
     }
     __syncthreads();
   }
-  if (t == 0) coarsest_col(H.L[last]);
+  if (t == 0) coarsest_col_nz<T, NZ>(H.L[last]);
   __syncthreads();
   for (int l = last - 1; l >= 0; --l) {
     const Lvl<T>& L = H.L[l];
@@ -670,12 +697,196 @@ __global__ __launch_bounds__(kThCoarseThreads) void k_th_coarse_lds(ThCoarse<T> 
     }
     __syncthreads();
   }
-  if (t == 0) coarsest_col(H.L[last]);
+  if (t == 0) coarsest_col_nz<T, NZ>(H.L[last]);
   __syncthreads();
   for (int l = last - 1; l >= 0; --l) {
     const Lvl<T>& L = H.L[l];
     zjac_level<T, NZ>(L, H.zc[l], H.zdi[l], L.y, L.x, H.L[l + 1].y, H.L[l + 1].A, 2, sm, t);
     zjac_level<T, NZ>(L, H.zc[l], H.zdi[l], L.x, L.y, nullptr, none, 1, sm, t);
+  }
+}
+
+// k_th_coarse_lds with every coarse level resident in LDS for the whole
+// kernel (FP32: the seven stencil arrays, the two Thomas factors, the right
+// side and both iterates of every level, loaded once; the top level's
+// result is the only store).  Each phase then waits on LDS instead of a
+// global round trip.  The same expressions as zjac_level, restrict_cell and
+// coarsest_col in the same order, so the results are bit-identical.
+constexpr int kResArrays = 12;  // d w e s n b t zdi zc rhs x y
+template <class T>
+__host__ __device__ inline size_t th_coarse_res_smem(int ncell_total, int ncell_top) {
+  return ((size_t)kResArrays * ncell_total + 2 * (size_t)ncell_top) * sizeof(T);
+}
+template <class T, int NZ>
+__global__ __launch_bounds__(kThCoarseThreads) void k_th_coarse_res(ThCoarse<T> H) {
+  if (H.L[0].stopped()) return;  // uniform
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  __shared__ int soff[kThCoarseMaxLv + 1];
+  const int t = threadIdx.x, last = H.n - 1;
+  if (t == 0) {
+    int o = 0;
+    for (int l = 0; l < H.n; ++l) {
+      soff[l] = o;
+      o += kResArrays * H.L[l].A.nx * H.L[l].A.ny * H.L[l].A.nz;
+    }
+    soff[H.n] = o;
+  }
+  __syncthreads();
+  // array a of level l (cells (k ny + j) nx + i)
+  auto arr = [&](int l, int a) -> T* {
+    return sm + soff[l] + a * ((soff[l + 1] - soff[l]) / kResArrays);
+  };
+  T* sR = sm + soff[H.n];
+  T* sX0 = sR + H.L[0].A.nx * H.L[0].A.ny * H.L[0].A.nz;
+  enum { D, W, E, S, N, B, TT, ZD, ZC, RHS, X, Y };
+  // stage every level's operator and factors, and the top level's right side
+  for (int l = 0; l < H.n; ++l) {
+    const Op<T>& A = H.L[l].A;
+    const int ncol = A.nx * A.ny, ncell = ncol * A.nz;
+    for (int c = t; c < ncell; c += kThCoarseThreads) {
+      const int k = c / ncol, col = c - k * ncol, i = col % A.nx, j = col / A.nx;
+      const size_t q = A.at(i, j, k);
+      arr(l, D)[c] = A.d[q];
+      arr(l, W)[c] = A.w[q];
+      arr(l, E)[c] = A.e[q];
+      arr(l, S)[c] = A.s[q];
+      arr(l, N)[c] = A.n[q];
+      arr(l, B)[c] = A.b[q];
+      arr(l, TT)[c] = A.t[q];
+      arr(l, ZD)[c] = H.zdi[l][q];
+      arr(l, ZC)[c] = H.zc[l][q];
+      if (l == 0) arr(l, RHS)[c] = H.L[0].rhs[q];
+    }
+  }
+  __syncthreads();
+  // one z-line Jacobi sweep of level l, src -> dst (zjac_level's two phases)
+  auto zjac = [&](int l, int src, int dst, int mode) {
+    const Op<T>& A = H.L[l].A;
+    const int nx = A.nx, ny = A.ny, nz = A.nz, ncol = nx * ny, ncell = ncol * nz;
+    const T* xs = arr(l, src);
+    const T* cy = mode == 2 ? arr(l + 1, Y) : nullptr;
+    const int cnx = mode == 2 ? H.L[l + 1].A.nx : 0, cny = mode == 2 ? H.L[l + 1].A.ny : 0;
+    auto cat = [&](int I, int J, int k) { return (k * cny + J) * cnx + I; };
+    for (int c = t; c < ncell; c += kThCoarseThreads) {
+      const int k = c / ncol, col = c - k * ncol, i = col % nx, j = col / nx;
+      const bool hw = i > 0, he = i + 1 < nx, hs = j > 0, hn = j + 1 < ny;
+      T xc = (T)0, xw = (T)0, xe = (T)0, xs_ = (T)0, xn = (T)0;
+      if (mode != 0) {
+        xc = xs[c];
+        xw = hw ? xs[c - 1] : (T)0;
+        xe = he ? xs[c + 1] : (T)0;
+        xs_ = hs ? xs[c - nx] : (T)0;
+        xn = hn ? xs[c + nx] : (T)0;
+        if (mode == 2) {
+          const int I2 = i >> 1, J2 = j >> 1;
+          xc += cy[cat(I2, J2, k)];
+          if (hw) xw += cy[cat((i - 1) >> 1, J2, k)];
+          if (he) xe += cy[cat((i + 1) >> 1, J2, k)];
+          if (hs) xs_ += cy[cat(I2, (j - 1) >> 1, k)];
+          if (hn) xn += cy[cat(I2, (j + 1) >> 1, k)];
+        }
+      }
+      sR[c] = arr(l, RHS)[c] - (arr(l, W)[c] * xw + arr(l, E)[c] * xe + arr(l, S)[c] * xs_ + arr(l, N)[c] * xn);
+      sX0[c] = xc;
+    }
+    __syncthreads();
+    const T* lo = arr(l, B);
+    const T* zdi = arr(l, ZD);
+    const T* zc = arr(l, ZC);
+    T* xd = arr(l, dst);
+    for (int col = t; col < ncol; col += kThCoarseThreads) {
+      T g[NZ];
+      T gp = (T)0;
+#pragma unroll
+      for (int k = 0; k < NZ; ++k) {
+        if (k < nz) {
+          const int c = k * ncol + col;
+          gp = (sR[c] - (k > 0 ? lo[c] : (T)0) * gp) * zdi[c];
+        }
+        g[k] = gp;
+      }
+      T xk = (T)0;
+#pragma unroll
+      for (int k = NZ - 1; k >= 0; --k) {
+        if (k < nz) {
+          const int c = k * ncol + col;
+          xk = k == nz - 1 ? g[k] : g[k] - zc[c] * xk;
+          xd[c] = sX0[c] + (T)kOmZ * (xk - sX0[c]);
+        }
+      }
+    }
+    __syncthreads();
+  };
+  for (int l = 0; l < last; ++l) {
+    zjac(l, X, X, 0);
+    zjac(l, X, Y, 1);
+    // coarse right side = sum of the fine residuals rhs - A y over the aggregate
+    const Op<T>& F = H.L[l].A;
+    const Op<T>& C = H.L[l + 1].A;
+    const int fnx = F.nx, fny = F.ny, fnz = F.nz, fks = fnx * fny;
+    const T *fd = arr(l, D), *fw = arr(l, W), *fe = arr(l, E), *fs = arr(l, S), *fn = arr(l, N), *fb = arr(l, B),
+            *ft = arr(l, TT), *fr = arr(l, RHS), *y = arr(l, Y);
+    T* cr = arr(l + 1, RHS);
+    const int ncc = C.nx * C.ny;
+    for (int q = t; q < ncc * C.nz; q += kThCoarseThreads) {
+      const int k = q / ncc, rest = q - k * ncc, I = rest % C.nx, J = rest / C.nx;
+      T acc = 0;
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) {
+          const int i = 2 * I + di, j = 2 * J + dj;
+          if (i < fnx && j < fny) {
+            const int c = (k * fny + j) * fnx + i;
+            T a = fd[c] * y[c];
+            if (i > 0) a += fw[c] * y[c - 1];
+            if (i + 1 < fnx) a += fe[c] * y[c + 1];
+            if (j > 0) a += fs[c] * y[c - fnx];
+            if (j + 1 < fny) a += fn[c] * y[c + fnx];
+            if (k > 0) a += fb[c] * y[c - fks];
+            if (k + 1 < fnz) a += ft[c] * y[c + fks];
+            acc += fr[c] - a;
+          }
+        }
+      cr[(k * C.ny + J) * C.nx + I] = acc;
+    }
+    __syncthreads();
+  }
+  if (t == 0) {  // the 1 x 1 column: Thomas in FP64 (coarsest_col)
+    const int nz = H.L[last].A.nz;
+    const T *d = arr(last, D), *lb = arr(last, B), *lt = arr(last, TT), *r = arr(last, RHS);
+    T* y = arr(last, Y);
+    double c[NZ], g[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      if (k < nz) {
+        const double lo = k > 0 ? (double)lb[k] : 0.0, up = k + 1 < nz ? (double)lt[k] : 0.0;
+        const double den = (double)d[k] - (k > 0 ? lo * c[k - 1] : 0.0);
+        c[k] = up / den;
+        g[k] = ((double)r[k] - (k > 0 ? lo * g[k - 1] : 0.0)) / den;
+      }
+    }
+    double xk = 0.0;
+#pragma unroll
+    for (int k = NZ - 1; k >= 0; --k) {
+      if (k < nz) {
+        xk = k == nz - 1 ? g[k] : g[k] - c[k] * xk;
+        y[k] = (T)xk;
+      }
+    }
+  }
+  __syncthreads();
+  for (int l = last - 1; l >= 0; --l) {
+    zjac(l, Y, X, 2);
+    zjac(l, X, Y, 1);
+  }
+  {  // the top level's result
+    const Op<T>& A = H.L[0].A;
+    const int ncol = A.nx * A.ny, ncell = ncol * A.nz;
+    const T* y = arr(0, Y);
+    for (int c = t; c < ncell; c += kThCoarseThreads) {
+      const int k = c / ncol, col = c - k * ncol, i = col % A.nx, j = col / A.nx;
+      H.L[0].y[A.at(i, j, k)] = y[c];
+    }
   }
 }
 
@@ -1358,8 +1569,24 @@ struct ThermalKrylov {
     else CPS_ZJAC(64);
 #undef CPS_ZJAC
   }
-  void coarse(const th::ThCoarse<T>& H) {
+  void coarse(const th::ThCoarse<T>& H, int lc) {
     const int nz = lv[0]->nz;
+    if constexpr (std::is_same<T, float>::value) {
+      // FP32: every coarse level resident in LDS when it fits
+      int total = 0;
+      for (int q = 0; q < H.n; ++q) total += lv[lc + q]->nx * lv[lc + q]->ny * lv[lc + q]->nz;
+      const size_t res = th::th_coarse_res_smem<T>(total, lv[lc]->nx * lv[lc]->ny * lv[lc]->nz);
+      const char* off = std::getenv("CPS_THERMAL_COARSE_GLOBAL");  // validation switch: the global-memory form
+      if (res <= coarseLdsMax && nz <= 16 && !(off && off[0] == '1')) {
+#define CPS_THCR(NZ) th::k_th_coarse_res<T, NZ><<<1, th::kThCoarseThreads, res, st>>>(H)
+        if (nz <= 4) CPS_THCR(4);
+        else if (nz <= 8) CPS_THCR(8);
+        else if (nz <= 12) CPS_THCR(12);
+        else CPS_THCR(16);
+#undef CPS_THCR
+        return;
+      }
+    }
     // the LDS form when the top level's five staged reals per cell fit
     const size_t lds = (size_t)5 * H.L[0].A.nx * H.L[0].A.ny * nz * sizeof(T);
     if (lds <= coarseLdsMax) {
@@ -1432,7 +1659,7 @@ struct ThermalKrylov {
         H.zdi[q] = lv[lc + q]->zdi.p;
         H.rhs[q] = q == 0 ? nullptr : lv[lc + q]->rhs.p;
       }
-      coarse(H);
+      coarse(H, lc);
     } else {
       th::k_coarsest<T><<<1, 64, 0, st>>>(lvl(last));
     }

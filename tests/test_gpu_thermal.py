@@ -71,6 +71,26 @@ def test_solve_matches_direct(pkg, orc, precision, seed, nx, ny):
     s.close()
 
 
+@pytest.mark.parametrize("seed,nx,ny,layers", [(4, 45, 27, (2, 3, 1)), (7, 512, 256, (4, 4, 4))])
+def test_resident_coarse_kernel_is_bit_identical(pkg, seed, nx, ny, layers, monkeypatch):
+    """The FP32 V-cycle's one-workgroup coarse end runs with every coarse
+    level resident in LDS; CPS_THERMAL_COARSE_GLOBAL=1 keeps the levels in
+    global memory.  The mixed solve's temperature field and its iteration
+    and refinement counts agree bit for bit."""
+    stack, gamma, u, v = synthetic_case(pkg, seed, nx, ny, n_base=layers[0], n_channel=layers[1], n_lid=layers[2],
+                                        h=2e-5)
+    out = []
+    for glob in ("1", "0"):
+        monkeypatch.setenv("CPS_THERMAL_COARSE_GLOBAL", glob)
+        print(f"case thermal coarse {nx}x{ny} layers={layers} global={glob}", flush=True)
+        s = pkg.ThermalSolver3D(stack, gamma, u, v, precision=32)
+        st = s.solve()
+        out.append((st["iterations"], st["refinements"], s.T))
+        s.close()
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+    assert np.array_equal(out[0][2], out[1][2])
+
+
 def test_reference_tolerance_full_stack(pkg, orc):
     """512 x 256 x 12 stack (4 Cu base, 4 channel, 4 Cu lid) at the
     reference stop rule rtol = 1e-6: the oracle's matrix confirms the
