@@ -655,14 +655,20 @@ __device__ inline void zjac_level(const Lvl<T>& L, const T* zc, const T* zdi, co
   __syncthreads();
   for (int col = t; col < ncol; col += kThCoarseThreads) {
     const int i = col % A.nx, j = col / A.nx;
-    T g[NZ];
+    // FP32: the column's LDS operands in registers first, so the back
+    // substitution's stores do not hold the loads of the next layer (FP64
+    // reads them in the loop: the copies would not fit the registers)
+    constexpr bool kPre = sizeof(T) == 4;
+    T g[NZ], cz[kPre ? NZ : 1], x0[kPre ? NZ : 1];
     T gp = (T)0;
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
-      if (k < nz) {
-        const int c = k * ncol + col;
-        gp = (sR[c] - sLo[c] * gp) * sZd[c];
+      const int c = k * ncol + col;
+      if (kPre) {
+        cz[kPre ? k : 0] = k < nz ? sZc[c] : (T)0;
+        x0[kPre ? k : 0] = k < nz ? sX0[c] : (T)0;
       }
+      if (k < nz) gp = (sR[c] - sLo[c] * gp) * sZd[c];
       g[k] = gp;
     }
     T xk = (T)0;
@@ -670,8 +676,9 @@ __device__ inline void zjac_level(const Lvl<T>& L, const T* zc, const T* zdi, co
     for (int k = NZ - 1; k >= 0; --k) {
       if (k < nz) {
         const int c = k * ncol + col;
-        xk = k == nz - 1 ? g[k] : g[k] - sZc[c] * xk;
-        dst[A.at(i, j, k)] = sX0[c] + (T)kOmZ * (xk - sX0[c]);
+        const T zck = kPre ? cz[kPre ? k : 0] : sZc[c], x0k = kPre ? x0[kPre ? k : 0] : sX0[c];
+        xk = k == nz - 1 ? g[k] : g[k] - zck * xk;
+        dst[A.at(i, j, k)] = x0k + (T)kOmZ * (xk - x0k);
       }
     }
   }
@@ -796,23 +803,24 @@ __global__ __launch_bounds__(kThCoarseThreads) void k_th_coarse_res(ThCoarse<T> 
     const T* zc = arr(l, ZC);
     T* xd = arr(l, dst);
     for (int col = t; col < ncol; col += kThCoarseThreads) {
-      T g[NZ];
+      // every LDS operand of the column in registers first, so the stores
+      // of the back substitution do not hold the loads of the next layer
+      T g[NZ], cz[NZ], x0[NZ];
       T gp = (T)0;
 #pragma unroll
       for (int k = 0; k < NZ; ++k) {
-        if (k < nz) {
-          const int c = k * ncol + col;
-          gp = (sR[c] - (k > 0 ? lo[c] : (T)0) * gp) * zdi[c];
-        }
+        const int c = k * ncol + col;
+        cz[k] = k < nz ? zc[c] : (T)0;
+        x0[k] = k < nz ? sX0[c] : (T)0;
+        if (k < nz) gp = (sR[c] - (k > 0 ? lo[c] : (T)0) * gp) * zdi[c];
         g[k] = gp;
       }
       T xk = (T)0;
 #pragma unroll
       for (int k = NZ - 1; k >= 0; --k) {
         if (k < nz) {
-          const int c = k * ncol + col;
-          xk = k == nz - 1 ? g[k] : g[k] - zc[c] * xk;
-          xd[c] = sX0[c] + (T)kOmZ * (xk - sX0[c]);
+          xk = k == nz - 1 ? g[k] : g[k] - cz[k] * xk;
+          xd[k * ncol + col] = x0[k] + (T)kOmZ * (xk - x0[k]);
         }
       }
     }
