@@ -2254,6 +2254,7 @@ class BatchSolver {
   // fixed == 0: the class plan with 1 .. 4 chains per class and unroll 1,
   // 4, 8, and the per-field plan on 1 .. 32 streams.  Every candidate gives
   // the same results.
+  static constexpr int kTuneWarmup = 128;
   void tune(int fixed, int reps) {
     struct Cand {
       int p, ns, un;
@@ -2273,12 +2274,29 @@ class BatchSolver {
     // candidate's score is the sum of its two timings.
     std::vector<double> us(cand.size(), 0.0);
     const size_t nc = cand.size();
+    // The drift is fastest in a field's first iterations from rest (the
+    // multigrid needs 7-8 cycles per step there, 2-3 later), where the
+    // two passes no longer cancel it: the candidates at both ends of the
+    // order were penalised.  Fields that start there first take
+    // kTuneWarmup iterations under the first candidate.
+    if (nc > 1) {
+      int young = 0;
+      for (auto& s : f) young = std::max(young, kTuneWarmup - s->iteration);
+      if (young > 0) {
+        capture(cand[0].p, cand[0].ns, cand[0].un);
+        launch(young);
+        sync_all();
+      }
+    }
     for (size_t pass = 0; pass < (nc > 1 ? 2u : 1u); ++pass)
       for (size_t kk = 0; kk < nc; ++kk) {
         const size_t k = pass == 0 ? kk : nc - 1 - kk;
         const Cand& c = cand[k];
         capture(c.p, c.ns, c.un);
-        const int r = std::max(reps, c.un);
+        // reps graph replays, at least 16 iterations: the steps of a
+        // converging field vary in cost (1-5 live multigrid cycles), so
+        // shorter samples picked plans 10 % slower than the best
+        const int r = std::max(reps * c.un, 16);
         launch(c.un);  // warm-up
         sync_all();
         const auto t0 = std::chrono::steady_clock::now();

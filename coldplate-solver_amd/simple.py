@@ -271,8 +271,10 @@ class Batch:
         """Captures the batch iteration.  streams > 0: the per-field plan on
         that many streams.  streams = 0: the autotuner times the class plan
         (1 .. 4 chains per class, unroll 1, 4, 8) and the per-field plan
-        (1 .. 32 streams), reps
-        launches each (which advance the fields), and keeps the fastest.
+        (1 .. 32 streams), max(reps x unroll, 16) iterations each in two
+        passes, the second in reverse order, after fields younger than 128
+        iterations have reached it (all of which advance the fields), and
+        keeps the fastest.
         cache: path of a JSON file of earlier picks keyed by signature(): a
         hit is captured without timing, a miss is tuned and stored.
         Returns (streams used, microseconds per batch iteration)."""

@@ -304,8 +304,9 @@ void cps_batch_destroy(cps_batch* b);
 /* Captures the batch iteration.  streams > 0: the per-field plan on that
  * many streams; 0: the autotuner times the class plan with 1 .. 4 chains
  * per class and unroll 1, 4, 8 and the per-field plan on 1 .. 32 streams
- * (max(reps, unroll) launches each; those launches advance the fields) and
- * keeps the fastest.  chosen: the streams the chosen plan uses (one per
+ * (max(reps x unroll, 16) iterations each, timed in two passes, the second
+ * in reverse order, after fields younger than 128 iterations have reached
+ * it; those iterations advance the fields) and keeps the fastest.  chosen: the streams the chosen plan uses (one per
  * chain for the class plan). */
 int cps_batch_tune(cps_batch* b, int streams, int reps, int* chosen, double* us_per_step);
 /* Captures the given plan without timing (a cached tuner choice): plan
