@@ -875,9 +875,56 @@ __host__ __device__ constexpr size_t up_pair_smem() {
 // The body of k_mg_up2 over LDS at smem_raw (up_pair_smem bytes), pointers
 // already shifted to the block's field.  topy: C's result in LDS (row pitch
 // topw), or nullptr to read it from C.y.
+// A's and B's global inputs of the up pair in registers (b, faces and x of
+// the tile and its rings; B's x with C's correction when withC).
 template <class T>
+struct Up2In {
+  static constexpr int NA = (UAN + FT - 1) / FT, NB = (UBN + FT - 1) / FT;
+  T vb[NA], ve[NA], vn[NA], vx[NA], wb[NB], we[NB], wn[NB], wx[NB];
+};
+template <class T>
+__device__ __forceinline__ void up2_fetch(const Lv<T>& A, const Lv<T>& B, const Lv<T>& C, bool withC, int t,
+                                          Up2In<T>& in) {
+  constexpr int NA = Up2In<T>::NA, NB = Up2In<T>::NB;
+  const int rA0 = (int)blockIdx.y * TY, cA0 = (int)blockIdx.x * TX;
+  const Region<T, TX, TY, HL> ra{nullptr, nullptr, nullptr, rA0, cA0, A.nx, A.ny};
+  const Region<T, TX / 2, TY / 2, HL> rb{nullptr, nullptr, nullptr, rA0 / 2, cA0 / 2, B.nx, B.ny};
+#pragma unroll
+  for (int i = 0; i < NA; ++i) {
+    const int q = t + i * FT;
+    const int lr = q / UAW, lc = q - (q / UAW) * UAW;
+    const int r = ra.gr(lr), c = ra.gc(lc);
+    in.vb[i] = in.ve[i] = in.vn[i] = in.vx[i] = (T)0;
+    if (q < UAN && ra.inside(r, c)) {
+      const size_t k = A.at(r, c);
+      in.vb[i] = A.b[k];
+      in.ve[i] = A.ex[k];
+      in.vn[i] = A.ey[k];
+      in.vx[i] = A.x[k];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int q = t + i * FT;
+    const int lr = q / UBW, lc = q - (q / UBW) * UBW;
+    const int r = rb.gr(lr), c = rb.gc(lc);
+    in.wb[i] = in.we[i] = in.wn[i] = in.wx[i] = (T)0;
+    if (q < UBN && rb.inside(r, c)) {
+      const size_t k = B.at(r, c);
+      in.wb[i] = B.b[k];
+      in.we[i] = B.ex[k];
+      in.wn[i] = B.ey[k];
+      in.wx[i] = withC ? B.x[k] + C.y[C.at(r >> 1, c >> 1)] : B.x[k];
+    }
+  }
+}
+// The body of k_mg_up2 over LDS at smem_raw (up_pair_smem bytes), pointers
+// already shifted to the block's field.  topy: C's result in LDS (row pitch
+// topw), or nullptr to read it from C.y.  Pre: `in` was filled by up2_fetch
+// (without C) and the stop flag tested by the caller; else both happen here.
+template <class T, bool Pre>
 __device__ __forceinline__ void up2_body(const Lv<T>& A, const Lv<T>& B, const Lv<T>& C, const int* stop,
-                                         unsigned char* smem_raw, const T* topy, int topw) {
+                                         unsigned char* smem_raw, const T* topy, int topw, Up2In<T>& in) {
   T* sbA = reinterpret_cast<T*>(smem_raw);
   T* seA = sbA + UAN;
   T* snA = seA + UAN;
@@ -890,52 +937,35 @@ __device__ __forceinline__ void up2_body(const Lv<T>& A, const Lv<T>& B, const L
   T* s1B = s0B + UBN;
   __shared__ int sflag;
   const int t = threadIdx.x + threadIdx.y * TX;
-  if (*stop) return;  // uniform: nothing to load once the PCG has stopped
-  const int stop0 = t == 0 ? *stop : 0;
+  int stop0 = 0;
+  if (!Pre) {
+    if (*stop) return;  // uniform: nothing to load once the PCG has stopped
+    stop0 = t == 0 ? *stop : 0;
+    up2_fetch(A, B, C, topy == nullptr, t, in);
+  }
   const int rA0 = (int)blockIdx.y * TY, cA0 = (int)blockIdx.x * TX;
   const int rB0 = rA0 / 2, cB0 = cA0 / 2;
   const Region<T, TX, TY, HL> ra{sbA, seA, snA, rA0, cA0, A.nx, A.ny};
   const Region<T, TX / 2, TY / 2, HL> rb{sbB, seB, snB, rB0, cB0, B.nx, B.ny};
-  {  // every global load first: A's b, faces and x1; B's b, faces and x2
-    constexpr int NA = (UAN + FT - 1) / FT, NB = (UBN + FT - 1) / FT;
-    T vb[NA], ve[NA], vn[NA], vx[NA], wb[NB], we[NB], wn[NB], wx[NB];
+  {  // A's b, faces and x1; B's b, faces and x2 = B's x + C's correction
+    constexpr int NA = Up2In<T>::NA, NB = Up2In<T>::NB;
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
       const int q = t + i * FT;
-      const int lr = q / UAW, lc = q - (q / UAW) * UAW;
-      const int r = ra.gr(lr), c = ra.gc(lc);
-      vb[i] = ve[i] = vn[i] = vx[i] = (T)0;
-      if (q < UAN && ra.inside(r, c)) {
-        const size_t k = A.at(r, c);
-        vb[i] = A.b[k];
-        ve[i] = A.ex[k];
-        vn[i] = A.ey[k];
-        vx[i] = A.x[k];
-      }
+      if (q < UAN) { sbA[q] = in.vb[i]; seA[q] = in.ve[i]; snA[q] = in.vn[i]; s0A[q] = in.vx[i]; }
     }
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
       const int q = t + i * FT;
-      const int lr = q / UBW, lc = q - (q / UBW) * UBW;
-      const int r = rb.gr(lr), c = rb.gc(lc);
-      wb[i] = we[i] = wn[i] = wx[i] = (T)0;
-      if (q < UBN && rb.inside(r, c)) {
-        const size_t k = B.at(r, c);
-        wb[i] = B.b[k];
-        we[i] = B.ex[k];
-        wn[i] = B.ey[k];
-        wx[i] = B.x[k] + (topy ? topy[(r >> 1) * topw + (c >> 1)] : C.y[C.at(r >> 1, c >> 1)]);
+      if (q < UBN) {
+        T x = in.wx[i];
+        if (topy) {
+          const int lr = q / UBW, lc = q - (q / UBW) * UBW;
+          const int r = rb.gr(lr), c = rb.gc(lc);
+          if (rb.inside(r, c)) x = x + topy[(r >> 1) * topw + (c >> 1)];
+        }
+        sbB[q] = in.wb[i]; seB[q] = in.we[i]; snB[q] = in.wn[i]; s0B[q] = x;
       }
-    }
-#pragma unroll
-    for (int i = 0; i < NA; ++i) {
-      const int q = t + i * FT;
-      if (q < UAN) { sbA[q] = vb[i]; seA[q] = ve[i]; snA[q] = vn[i]; s0A[q] = vx[i]; }
-    }
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      const int q = t + i * FT;
-      if (q < UBN) { sbB[q] = wb[i]; seB[q] = we[i]; snB[q] = wn[i]; s0B[q] = wx[i]; }
     }
   }
   if (t == 0) sflag = stop0;
@@ -969,7 +999,8 @@ __global__ __launch_bounds__(FT) void k_mg_up2(Lv<T> A, Lv<T> B, Lv<T> C, const 
   const size_t zo_ = zoff(A.zs);
   A.zshift(zo_); B.zshift(zo_); C.zshift(zo_); stop = zsh(stop, zo_);
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-  up2_body(A, B, C, stop, smem_raw, (const T*)nullptr, 0);
+  Up2In<T> in;
+  up2_body<T, false>(A, B, C, stop, smem_raw, (const T*)nullptr, 0, in);
 }
 
 // The up legs of the last two mid levels and of the fine level in one launch
@@ -1341,10 +1372,12 @@ __global__ __launch_bounds__(FT) void k_mg_coarse_up2(Lv<T> A, Lv<T> B, Lv<T> to
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int t = threadIdx.x + threadIdx.y * TX;
   const bool first = blockIdx.x == 0 && blockIdx.y == 0;
+  Up2In<T> in;  // the up pair's loads, in flight during the coarse V-cycle
+  up2_fetch(A, B, top, false, t, in);
   if (!coarse_body(top, d, cf, test, smem_raw, t, first && t == 0, false)) return;
   __syncthreads();
   const size_t cs = (coarse_smem<T>(d) + 15) & ~(size_t)15;
-  up2_body(A, B, top, stop, smem_raw + cs, coarse_topy<T>(smem_raw, d), d.nx[0]);
+  up2_body<T, true>(A, B, top, stop, smem_raw + cs, coarse_topy<T>(smem_raw, d), d.nx[0], in);
 }
 #undef CELLS
 
