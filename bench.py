@@ -190,14 +190,18 @@ def grads_leg(pkg, nx=512, ny=256, forward_steps=4000, reps=3, rtol=1e-8):
     return out
 
 
-def batch_leg(pkg, steps=500, n_fields=48, plan=None, precision=None):
+def batch_leg(pkg, steps=500, n_fields=48, plan=None, precision=None, start_at=0):
     """48 density fields, sizes drawn from {256x128, 512x256, 768x384,
     1024x512}, sigma ~ U[4, 12], per-field seeds, the headline knobs
     (FP32-inner GMG-PCG, p_rtol 1e-3, <= 6 PCG iterations, <= 8 Jacobi
     sweeps): plan (class plan, one launch per kernel covering a size class,
     or per-field chains on several streams) and graph unroll picked by the
     autotuner, `steps` iterations timed.  precision: CPS_PREC_MIXED
-    (default) or CPS_PREC_FP64 (the same algorithm in plain FP64)."""
+    (default) or CPS_PREC_FP64 (the same algorithm in plain FP64).
+    start_at: with a given plan, the fields are first stepped to at least
+    this iteration (a step costs less as the fields converge, so two legs
+    compare only from the same start; the tuner's own timing leaves the
+    fields past their 128th iteration)."""
     T = pkg._types
     precision = T.CPS_PREC_MIXED if precision is None else precision
     fl = []
@@ -213,12 +217,14 @@ def batch_leg(pkg, steps=500, n_fields=48, plan=None, precision=None):
             b.plan(kind, streams=int(n[1]) if len(n) > 1 else 1, unroll=int(n[0]))
         else:
             b.plan(kind, streams=int(n[0]))
-        b.step(int(n[0]) if kind == "class" else 1)  # warm-up
+        u = int(n[0]) if kind == "class" else 1
+        b.step(u * max(1, -(-start_at // u)))  # warm-up
     else:
         b.tune(streams=0, reps=3)
     ns = b.streams
     plan = b.plan_info()
     b.synchronize()
+    start = b.stats(0)["iteration"]
     t0 = time.perf_counter()
     b.step(steps)
     b.synchronize()
@@ -227,7 +233,7 @@ def batch_leg(pkg, steps=500, n_fields=48, plan=None, precision=None):
     finite = all(math.isfinite(b.stats(k)["error"]) for k in range(b.n))
     b.close()
     return dict(fields=n_fields, steps=steps, seconds=dt, streams=ns, plan=plan["plan"], plan_param=plan["param"],
-                unroll=plan["unroll"],
+                unroll=plan["unroll"], start_iteration=start,
                 size_classes=plan["classes"], batch_it_s=steps / dt,
                 field_it_s=n_fields * steps / dt, cell_updates_per_s=cells * steps / dt, total_cells=cells,
                 all_finite=finite)
@@ -614,7 +620,9 @@ def main():
         # ratio where the kernels carry whole size classes
         same = (f"class:{batch['unroll']}:{batch['plan_param']}" if batch["plan"] == "class"
                 else f"per_field:{batch['streams']}")
-        b64 = batch_leg(pkg, steps=args.batch_steps, plan=same, precision=pkg._types.CPS_PREC_FP64)
+        b64 = batch_leg(pkg, steps=args.batch_steps, plan=same, precision=pkg._types.CPS_PREC_FP64,
+                        start_at=batch["start_iteration"])
+        batch["fp64_start_iteration"] = b64["start_iteration"]
         batch["fp64_cell_updates_per_s"] = b64["cell_updates_per_s"]
         batch["speedup_vs_fp64"] = batch["cell_updates_per_s"] / b64["cell_updates_per_s"]
     grads_batch = None if args.no_grads else grads_batch_leg(pkg)
@@ -674,7 +682,8 @@ def main():
         "batch": None if batch is None else {
             "workload": "48 density fields, sizes from {256x128, 512x256, 768x384, 1024x512}, sigma U[4,12], "
                         "headline knobs; autotuned plan (class plan: one launch per kernel over all fields "
-                        "of a grid size, one hipGraph per size class); fp64_*: the same fields and plan in plain FP64",
+                        "of a grid size, one hipGraph per size class); fp64_*: the same fields and plan in plain FP64, "
+                        "timed from the same field iteration (start_iteration)",
             **{k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in batch.items()},
         },
         "thermal": None if thermal is None else {
